@@ -1,0 +1,169 @@
+"""ctypes front end of the two CPU checkers of the Krylov path.
+
+TEST INFRASTRUCTURE ONLY -- imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, never by the product library.
+
+* :class:`Oracle` -- the C restatement ``liboracle.so`` (krylov_oracle.c),
+  which follows Process/CG.f90, BiCG.f90, CGS.f90, Prec1.f90, Resid.f90
+  operation by operation; it runs wherever gcc built it, the GPU box
+  included.
+* :class:`Reference` -- the reference's own Fortran solvers compiled from
+  their sources into ``_ref/libtflows_ref.so`` by ``oracle/Makefile``.
+
+Both expose the reference calling convention: ``cg(N, NB, NONZ, A, Acol,
+Arow, Adia, Ab, x, r1, prec, niter, tol)`` updates ``x`` and ``r1`` in place
+and returns ``(niter, IniRes, FinRes)``.  ``IniRes`` is returned as ``None``
+when the solver did not write it (the early exit on ||b|| < tol,
+Process/CG.f90:69-72).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_ORACLE = os.path.join(HERE, "liboracle.so")
+LIB_REF = os.path.join(HERE, "_ref", "libtflows_ref.so")
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_SENTINEL = -1.2345e300
+
+
+def build(with_reference: bool = True) -> None:
+    """Compile the checkers (``make -C oracle``)."""
+    target = "all" if with_reference else "liboracle.so"
+    subprocess.run(["make", "-s", "-C", HERE, target], check=True)
+
+
+class _Solvers:
+    _prefix = ""
+
+    def __init__(self, path: str):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (run make -C oracle)")
+        self.lib = C.CDLL(path)
+        ip, dp = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+        for name in ("cg", "bicg", "cgs"):
+            f = getattr(self.lib, self._prefix + name + "_")
+            f.restype = None
+            f.argtypes = [ip, ip, ip, _f64p, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p,
+                          ip, ip, dp, dp, dp]
+        f = getattr(self.lib, self._prefix + "prec1_")
+        f.restype = None
+        f.argtypes = [ip, ip, _f64p, _i32p, _i32p, _i32p, _f64p, ip]
+        f = getattr(self.lib, self._prefix + "resid_")
+        f.restype = None
+        f.argtypes = [ip, ip, ip, _f64p, _i32p, _i32p, _f64p, _f64p, _f64p]
+
+    def _setup(self, n: int, nb: int) -> None:
+        pass
+
+    def solve(self, method: str, N, NB, NONZ, A, Acol, Arow, Adia, Ab, x, r1, prec, niter, tol):
+        """Call ``method`` in {'cg','bicg','cgs'}; x (length NB+N+1, Fortran
+        x(-NB:N)) and r1 (length N) are updated in place."""
+        self._setup(N, NB)
+        n_, nb_, nz_ = C.c_int32(N), C.c_int32(NB), C.c_int32(NONZ)
+        pr_, it_ = C.c_int32(prec), C.c_int32(niter)
+        tol_ = C.c_double(tol)
+        ini, fin = C.c_double(_SENTINEL), C.c_double(0.0)
+        Ab = Ab if Ab is not None and len(Ab) else np.zeros(1)
+        assert x.dtype == np.float64 and len(x) == NB + N + 1
+        getattr(self.lib, self._prefix + method + "_")(
+            C.byref(n_), C.byref(nb_), C.byref(nz_), A, Acol, Arow, Adia, Ab, x, r1,
+            C.byref(pr_), C.byref(it_), C.byref(tol_), C.byref(ini), C.byref(fin))
+        return it_.value, (None if ini.value == _SENTINEL else ini.value), fin.value
+
+    def cg(self, *a):
+        return self.solve("cg", *a)
+
+    def bicg(self, *a):
+        return self.solve("bicg", *a)
+
+    def cgs(self, *a):
+        return self.solve("cgs", *a)
+
+    def prec1(self, N, NONZ, A, Acol, Arow, Adia, prec):
+        D = np.zeros(N)
+        getattr(self.lib, self._prefix + "prec1_")(
+            C.byref(C.c_int32(N)), C.byref(C.c_int32(NONZ)), A, Acol, Arow, Adia, D,
+            C.byref(C.c_int32(prec)))
+        return D
+
+    def resid(self, N, NB, NONZ, A, Acol, Arow, Ab, x, r1):
+        Ab = Ab if Ab is not None and len(Ab) else np.zeros(1)
+        getattr(self.lib, self._prefix + "resid_")(
+            C.byref(C.c_int32(N)), C.byref(C.c_int32(NB)), C.byref(C.c_int32(NONZ)),
+            A, Acol, Arow, Ab, x, r1)
+
+    # convenience: run a LinearSystem-like object; returns a result dict
+    def run(self, method: str, m, b, x0=None, prec=1, niter=2000, tol=0.0, nb=0):
+        x = np.zeros(nb + m.n + 1)
+        if x0 is not None:
+            x[nb + 1:] = x0
+        r1 = np.array(b, dtype=np.float64, copy=True)
+        it, ini, fin = self.solve(method, m.n, nb, m.nonz, m.aval, m.acol, m.arow, m.adia,
+                                  np.zeros(max(nb, 1)), x, r1, prec, niter, tol)
+        return dict(x=x[nb + 1:].copy(), r=r1, niter=it, ini_res=ini, fin_res=fin)
+
+
+class Oracle(_Solvers):
+    """C restatement (oracle/krylov_oracle.c)."""
+    _prefix = "oracle_"
+
+    def __init__(self, path: str = LIB_ORACLE):
+        super().__init__(path)
+        self.lib.oracle_get_error.restype = C.c_double
+        self.lib.oracle_spmv.restype = None
+        self.lib.oracle_spmv.argtypes = [C.c_int, _f64p, _i32p, _i32p, _f64p, _f64p]
+        self.lib.oracle_prec_apply.restype = None
+        self.lib.oracle_prec_apply.argtypes = [C.c_int, _f64p, _i32p, _i32p, _i32p, _f64p,
+                                               C.c_int, _f64p, _f64p]
+
+    def reset_state(self):
+        self.lib.oracle_reset_state()
+
+    def get_error(self) -> float:
+        return self.lib.oracle_get_error()
+
+    def spmv(self, m, p):
+        q = np.zeros(m.n)
+        self.lib.oracle_spmv(m.n, m.aval, m.acol, m.arow, np.ascontiguousarray(p, np.float64), q)
+        return q
+
+    def prec_apply(self, m, D, prec, u):
+        z = np.zeros(m.n)
+        self.lib.oracle_prec_apply(m.n, m.aval, m.acol, m.arow, m.adia, D, prec,
+                                   np.ascontiguousarray(u, np.float64), z)
+        return z
+
+
+class Reference(_Solvers):
+    """The reference Fortran solvers, compiled from their own sources."""
+    _prefix = ""
+
+    def __init__(self, path: str = LIB_REF):
+        super().__init__(path)
+        self.lib.ref_setup_.restype = None
+        self.lib.ref_setup_.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        self._shape = None
+
+    def _setup(self, n, nb):
+        if self._shape != (n, nb):
+            self.lib.ref_setup_(C.byref(C.c_int32(n)), C.byref(C.c_int32(nb)))
+            self._shape = (n, nb)
+
+    def set_error(self, e: float):
+        self.lib.ref_set_error_(C.byref(C.c_double(e)))
+
+    def get_error(self) -> float:
+        e = C.c_double(0.0)
+        self.lib.ref_get_error_(C.byref(e))
+        return e.value
+
+
+def available_reference() -> bool:
+    return os.path.exists(LIB_REF)

@@ -1,0 +1,6 @@
+"""MI355X-native T-FlowS Krylov linear-solver path (CG / BiCG / CGS + Prec1).
+
+The directory name carries hyphens, so it is loaded as the package
+``tflows_amd`` by :func:`load` in ``__graft_entry__`` and ``tests/conftest``.
+"""
+from . import meshgen  # noqa: F401

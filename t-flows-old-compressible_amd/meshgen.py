@@ -1,0 +1,534 @@
+"""Synthetic finite-volume systems in the T-FlowS matrix layout.
+
+The T-FlowS solvers act on a CSR matrix whose pattern is built once by
+``TopolM`` (Process/TopolM.f90:35-94) and whose values are filled by the
+equation assemblers.  The layout, which every function here reproduces, is:
+
+* ``Acol(N+1)`` -- the row pointer (the reference names it "col"),
+  1-based, ``Acol(1) = 1`` and ``Acol(N+1) = nnz + 1``
+  (TopolM.f90:53-58);
+* ``Arow(NONZ)`` -- the column indices, 1-based, sorted ascending inside
+  each row (TopolM.f90:61-74, ISORT), with ``NONZ = nnz + 1``: one
+  trailing slot that is allocated and left zero (TopolM.f90:49-51);
+* ``Aval(NONZ)`` -- FP64 values, trailing slot zero;
+* ``Adia(N)`` -- 1-based position of the diagonal entry of each row
+  (TopolM.f90:75-77).
+
+The pattern is structurally symmetric because every interior face inserts
+both (c1,c2) and (c2,c1) (TopolM.f90:65-66).
+
+Matrix recipes follow the reference assemblers:
+
+* pressure correction: off-diagonal ``-A12`` and diagonal ``+A12`` per
+  interior face with ``A12 = SMDPN = |S|^2/(S.d)`` for unit ``V/Asave``
+  (Process/CalcPS.f90:77-94); a pinned cell gets one face-sized Dirichlet
+  term on its diagonal the way OUTFLOW/PRESSURE faces do
+  (CalcPS.f90:146-147, 159-160);
+* momentum / scalar transport: first-order upwind ``A12 = -min(F,0)``,
+  ``A21 = max(F,0)``, diffusion ``D = Gamma*|S|/d``, "m1" placement
+  (Process/NewUVW.f90:356-367) and the time term ``V/dt``
+  (NewUVW.f90:447-449).
+
+Wall stretching follows the hyperbolic-tangent law of the mesh generator,
+case 3 (both walls), Generate/Linija.f90:114-117; natural cell numbering
+(i fastest, then j, then k) follows Generate/Calc1.f90:398-402.
+
+Everything is vectorised numpy, so a 256^3 box (16.8M rows) builds in well
+under a minute.  The generator is deterministic for a given seed.
+"""
+from __future__ import annotations
+
+import hashlib
+from dataclasses import dataclass, field
+
+import numpy as np
+
+GENERATOR_VERSION = 1
+
+# slot order of the 7-point stencil inside a row before any renumbering;
+# the offsets are ascending, so rows come out sorted in natural order.
+_SLOTS = ("-z", "-y", "-x", "self", "+x", "+y", "+z")
+
+
+# --------------------------------------------------------------------------
+# geometry
+# --------------------------------------------------------------------------
+def tanh_nodes(n: int, length: float, ratio: float, origin: float = 0.0) -> np.ndarray:
+    """Node coordinates clustered at both ends (Generate/Linija.f90:114-117).
+
+    ``y(i) = y0 + 0.5*(1 + tanh(xi*atanh(pr))/pr)*L`` with
+    ``xi = -1 + 2i/n``.  ``pr`` in (0,1) is found by bisection so that the
+    largest ratio of adjacent cell sizes equals ``ratio``; ``ratio <= 1``
+    gives uniform spacing.
+    """
+    i = np.arange(n + 1, dtype=np.float64)
+    xi = -1.0 + 2.0 * i / n
+
+    def nodes(pr: float) -> np.ndarray:
+        return origin + 0.5 * (1.0 + np.tanh(xi * np.arctanh(pr)) / pr) * length
+
+    def max_ratio(pr: float) -> float:
+        d = np.diff(nodes(pr))
+        return float(np.max(np.maximum(d[1:] / d[:-1], d[:-1] / d[1:])))
+
+    if ratio <= 1.0 or n < 3:
+        return origin + length * i / n
+    lo, hi = 1e-9, 1.0 - 1e-15
+    if max_ratio(hi) < ratio:
+        return nodes(hi)
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if max_ratio(mid) < ratio:
+            lo = mid
+        else:
+            hi = mid
+    return nodes(lo)
+
+
+def geometric_wall_nodes(n: int, length: float, ratio: float, max_aspect: float,
+                         ref_spacing: float) -> np.ndarray:
+    """Nodes clustered at both walls with a geometric growth ``ratio``.
+
+    The first cell is ``ref_spacing / max_aspect`` thick (so that a wall cell
+    reaches the requested aspect ratio against a transverse spacing of
+    ``ref_spacing``); cells grow by ``ratio`` towards the centre until they
+    reach uniform spacing that fills the rest of the span.
+    """
+    h0 = ref_spacing / max_aspect
+    half = n // 2
+    sizes = h0 * ratio ** np.arange(half)
+    # cap the growth so both halves fit exactly into the length
+    cap_lo, cap_hi = h0, length
+    for _ in range(200):
+        cap = 0.5 * (cap_lo + cap_hi)
+        s = np.minimum(sizes, cap)
+        tot = 2.0 * s.sum() + (cap if n % 2 else 0.0)
+        if tot > length:
+            cap_hi = cap
+        else:
+            cap_lo = cap
+    s = np.minimum(sizes, cap_lo)
+    mid = [cap_lo] if n % 2 else []
+    cells = np.concatenate([s, mid, s[::-1]])
+    cells *= length / cells.sum()
+    return np.concatenate([[0.0], np.cumsum(cells)])
+
+
+@dataclass
+class BoxMesh:
+    """Tensor-product hex box, cells numbered naturally (i fastest)."""
+    xn: np.ndarray
+    yn: np.ndarray
+    zn: np.ndarray
+
+    @property
+    def shape(self):
+        return (len(self.xn) - 1, len(self.yn) - 1, len(self.zn) - 1)
+
+    @property
+    def n_cells(self) -> int:
+        nx, ny, nz = self.shape
+        return nx * ny * nz
+
+    def centres(self):
+        return tuple(0.5 * (a[1:] + a[:-1]) for a in (self.xn, self.yn, self.zn))
+
+    def spacings(self):
+        return tuple(np.diff(a) for a in (self.xn, self.yn, self.zn))
+
+
+def channel_mesh(nx: int, ny: int, nz: int, ratio: float = 1.1) -> BoxMesh:
+    """2*pi x 2 x pi channel, tanh wall stretching in y."""
+    return BoxMesh(np.linspace(0.0, 2.0 * np.pi, nx + 1),
+                   tanh_nodes(ny, 2.0, ratio),
+                   np.linspace(0.0, np.pi, nz + 1))
+
+
+def unit_cube_mesh(n: int) -> BoxMesh:
+    g = np.linspace(0.0, 1.0, n + 1)
+    return BoxMesh(g, g.copy(), g.copy())
+
+
+# --------------------------------------------------------------------------
+# the T-FlowS CSR container
+# --------------------------------------------------------------------------
+@dataclass
+class TFlowSMatrix:
+    """A CSR matrix in the T-FlowS layout (1-based, NONZ = nnz + 1)."""
+    n: int
+    nonz: int
+    acol: np.ndarray          # int32 (N+1)  row pointer, 1-based
+    arow: np.ndarray          # int32 (NONZ) column index, 1-based, last slot 0
+    adia: np.ndarray          # int32 (N)    diagonal position, 1-based
+    aval: np.ndarray          # f64  (NONZ)  values, last slot 0
+    mirror: np.ndarray | None = None   # int64 (nnz) 0-based position of (j,i) for (i,j)
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nnz(self) -> int:
+        return self.nonz - 1
+
+    def row_of_entry(self) -> np.ndarray:
+        """0-based row index of every stored entry (length nnz)."""
+        return np.repeat(np.arange(self.n, dtype=np.int64),
+                         np.diff(self.acol.astype(np.int64)))
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        nnz = self.nnz
+        return sp.csr_matrix((self.aval[:nnz], self.arow[:nnz].astype(np.int64) - 1,
+                              self.acol.astype(np.int64) - 1), shape=(self.n, self.n))
+
+    def with_values(self, aval: np.ndarray) -> "TFlowSMatrix":
+        return TFlowSMatrix(self.n, self.nonz, self.acol, self.arow, self.adia,
+                            np.ascontiguousarray(aval, dtype=np.float64),
+                            self.mirror, dict(self.meta))
+
+    def transpose_values(self) -> np.ndarray:
+        """Values of A^T on the same (structurally symmetric) pattern."""
+        out = np.zeros_like(self.aval)
+        out[: self.nnz] = self.aval[: self.nnz][self.mirror]
+        return out
+
+    def digest(self) -> str:
+        h = hashlib.sha256()
+        for a in (self.acol, self.arow, self.adia, self.aval):
+            h.update(np.ascontiguousarray(a).tobytes())
+        return h.hexdigest()
+
+
+def vector_digest(*arrays) -> str:
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def _assemble(shape, off_lo, off_hi, diag, perm=None) -> TFlowSMatrix:
+    """Build the T-FlowS CSR from per-direction coefficients.
+
+    ``off_lo[d][c]`` is the entry A(c, c - stride_d) and ``off_hi[d][c]`` is
+    A(c, c + stride_d) for d = x, y, z, indexed by natural 0-based cell.
+    Entries pointing outside the box are dropped.  With ``perm`` the cell
+    with natural index c becomes row ``perm[c]`` and every row's columns are
+    re-sorted ascending, as TopolM's ISORT does for any numbering.
+    """
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    c = np.arange(n, dtype=np.int64)
+    i = c % nx
+    j = (c // nx) % ny
+    k = c // (nx * ny)
+    sx, sy, sz = 1, nx, nx * ny
+    cols = np.stack([c - sz, c - sy, c - sx, c, c + sx, c + sy, c + sz], axis=1)
+    valid = np.stack([k > 0, j > 0, i > 0, np.ones(n, bool),
+                      i < nx - 1, j < ny - 1, k < nz - 1], axis=1)
+    vals = np.stack([off_lo[2], off_lo[1], off_lo[0], diag,
+                     off_hi[0], off_hi[1], off_hi[2]], axis=1).astype(np.float64)
+    cols = np.where(valid, cols, 0)
+
+    if perm is not None:
+        perm = np.asarray(perm, dtype=np.int64)
+        newcols = np.where(valid, perm[cols], np.iinfo(np.int64).max)
+        order = np.argsort(newcols, axis=1, kind="stable")
+        newcols = np.take_along_axis(newcols, order, axis=1)
+        vals = np.take_along_axis(vals, order, axis=1)
+        valid = np.take_along_axis(valid, order, axis=1)
+        inv = np.empty(n, dtype=np.int64)
+        inv[perm] = c
+        cols = np.where(valid, newcols, 0)[inv]
+        vals = vals[inv]
+        valid = valid[inv]
+
+    counts = valid.sum(axis=1)
+    nnz = int(counts.sum())
+    acol = np.empty(n + 1, dtype=np.int32)
+    acol[0] = 1
+    np.cumsum(counts, out=acol[1:])
+    acol[1:] += 1
+    arow = np.zeros(nnz + 1, dtype=np.int32)
+    aval = np.zeros(nnz + 1, dtype=np.float64)
+    arow[:nnz] = (cols[valid] + 1).astype(np.int32)
+    aval[:nnz] = vals[valid]
+    rows = np.repeat(np.arange(n, dtype=np.int64), counts)
+    is_diag = (arow[:nnz].astype(np.int64) - 1) == rows
+    adia = np.empty(n, dtype=np.int32)
+    adia[rows[is_diag]] = (np.nonzero(is_diag)[0] + 1).astype(np.int32)
+    m = TFlowSMatrix(n, nnz + 1, acol, arow, adia, aval)
+    m.mirror = mirror_positions(m)
+    return m
+
+
+def mirror_positions(m: TFlowSMatrix) -> np.ndarray:
+    """0-based position of entry (j,i) for every stored entry (i,j).
+
+    This is the role of SidAij(1,s)/SidAij(2,s) in the reference
+    (TopolM.f90:81-94), extended to the diagonal (which maps to itself).
+    Requires a structurally symmetric pattern.
+    """
+    nnz = m.nnz
+    rows = m.row_of_entry()
+    cols = m.arow[:nnz].astype(np.int64) - 1
+    key = rows * m.n + cols           # sorted ascending already (CSR order)
+    tkey = cols * m.n + rows
+    pos = np.searchsorted(key, tkey)
+    if not np.array_equal(key[np.minimum(pos, nnz - 1)], tkey):
+        raise ValueError("pattern is not structurally symmetric")
+    return pos.astype(np.int64)
+
+
+def random_permutation(n: int, seed: int) -> np.ndarray:
+    return np.random.default_rng(seed).permutation(n).astype(np.int64)
+
+
+# --------------------------------------------------------------------------
+# face geometry of a tensor-product box
+# --------------------------------------------------------------------------
+def _face_coeffs(mesh: BoxMesh):
+    """|S|/d for interior faces in each direction, per low-side cell.
+
+    Returns three arrays of shape (n_cells,) holding the coefficient of the
+    face between cell c and c + stride_d (zero where that face does not
+    exist), plus the per-cell face areas used for boundary terms.
+    """
+    nx, ny, nz = mesh.shape
+    dx, dy, dz = mesh.spacings()
+    xc, yc, zc = mesh.centres()
+    DX, DY, DZ = np.meshgrid(dx, dy, dz, indexing="ij")
+    # transpose to (k, j, i) order then flatten so that i runs fastest
+    def flat(a):
+        return np.ascontiguousarray(a.transpose(2, 1, 0)).ravel()
+
+    ax = np.zeros((nx, ny, nz))
+    ay = np.zeros((nx, ny, nz))
+    az = np.zeros((nx, ny, nz))
+    if nx > 1:
+        ax[:-1] = (DY * DZ)[:-1] / np.diff(xc)[:, None, None]
+    if ny > 1:
+        ay[:, :-1] = (DX * DZ)[:, :-1] / np.diff(yc)[None, :, None]
+    if nz > 1:
+        az[:, :, :-1] = (DX * DY)[:, :, :-1] / np.diff(zc)[None, None, :]
+    vol = DX * DY * DZ
+    return (flat(ax), flat(ay), flat(az)), flat(DY * DZ), flat(DX), flat(vol)
+
+
+def _shift_down(a: np.ndarray, stride: int) -> np.ndarray:
+    """out[c] = a[c - stride] (zero where c < stride)."""
+    out = np.zeros_like(a)
+    out[stride:] = a[:-stride] if stride < len(a) else 0
+    return out
+
+
+# --------------------------------------------------------------------------
+# system recipes
+# --------------------------------------------------------------------------
+@dataclass
+class LinearSystem:
+    matrix: TFlowSMatrix
+    b: np.ndarray
+    x0: np.ndarray
+    tol: float
+    prec: int
+    niter: int
+    meta: dict = field(default_factory=dict)
+
+
+def pressure_matrix(mesh: BoxMesh, pin_cell: int = 1, perm=None) -> TFlowSMatrix:
+    """Pressure-correction matrix (Process/CalcPS.f90:77-94, 146-147).
+
+    For an orthogonal hex face SMDPN = |S|^2/(S.d) = A_f/d_f.  Each interior
+    face contributes ``-A12`` to both off-diagonal slots and ``+A12`` to both
+    diagonals; ``pin_cell`` (1-based natural index, 0 to disable) gets one
+    boundary-face term A/(dx/2) on its diagonal to remove the null space.
+    """
+    nx, ny, nz = mesh.shape
+    (ax, ay, az), area_x, dxc, _ = _face_coeffs(mesh)
+    strides = (1, nx, nx * ny)
+    hi = [ax, ay, az]
+    lo = [_shift_down(a, s) for a, s in zip(hi, strides)]
+    diag = np.zeros(mesh.n_cells)
+    for a in hi:
+        diag += a
+    for a in lo:
+        diag += a
+    if pin_cell:
+        c = pin_cell - 1
+        diag[c] += area_x[c] / (0.5 * dxc[c])
+    m = _assemble(mesh.shape, [-a for a in lo], [-a for a in hi], diag, perm)
+    m.meta.update(kind="pressure", shape=mesh.shape, pin_cell=pin_cell)
+    return m
+
+
+def divergence_free_fluxes(mesh: BoxMesh, seed: int, amplitude: float = 1.0, modes: int = 4):
+    """Face volume fluxes F = curl(psi) of a seeded edge vector potential.
+
+    psi vanishes on boundary edges, so every boundary-face flux is zero and
+    the discrete divergence of every cell is exactly zero (up to rounding).
+    Returns (Fx, Fy, Fz) indexed by the low-side cell of each interior face
+    (natural numbering), scaled so that max |F|/area = ``amplitude``.
+    """
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = mesh.shape
+    xn, yn, zn = mesh.xn, mesh.yn, mesh.zn
+    xc, yc, zc = mesh.centres()
+    L = (xn[-1] - xn[0], yn[-1] - yn[0], zn[-1] - zn[0])
+
+    def smooth_field(X, Y, Z):
+        f = np.zeros(np.broadcast(X, Y, Z).shape)
+        for _ in range(modes):
+            kx, ky, kz = rng.integers(1, 4, size=3)
+            ph = rng.uniform(0, 2 * np.pi, size=3)
+            amp = rng.uniform(-1, 1)
+            f += amp * (np.sin(np.pi * kx * X / L[0] + ph[0]) *
+                        np.sin(np.pi * ky * Y / L[1] + ph[1]) *
+                        np.sin(np.pi * kz * Z / L[2] + ph[2]))
+        # window that vanishes on the box boundary
+        w = (np.sin(np.pi * X / L[0]) * np.sin(np.pi * Y / L[1]) * np.sin(np.pi * Z / L[2]))
+        return f * w
+
+    # psi_x lives on x-edges: (xc[i], yn[j], zn[k]); psi_y on (xn, yc, zn); psi_z on (xn, yn, zc)
+    psx = smooth_field(*np.meshgrid(xc, yn, zn, indexing="ij"))   # (nx, ny+1, nz+1)
+    psy = smooth_field(*np.meshgrid(xn, yc, zn, indexing="ij"))   # (nx+1, ny, nz+1)
+    psz = smooth_field(*np.meshgrid(xn, yn, zc, indexing="ij"))   # (nx+1, ny+1, nz)
+    # circulation along edges (psi * edge length) -> fluxes through faces
+    dx, dy, dz = mesh.spacings()
+    cx = psx * dx[:, None, None]
+    cy = psy * dy[None, :, None]
+    cz = psz * dz[None, None, :]
+    # flux through x-face at node plane xn[i] (shape (nx+1, ny, nz))
+    Fx = (cz[:, 1:, :] - cz[:, :-1, :]) - (cy[:, :, 1:] - cy[:, :, :-1])
+    Fy = (cx[:, :, 1:] - cx[:, :, :-1]) - (cz[1:, :, :] - cz[:-1, :, :])
+    Fz = (cy[1:, :, :] - cy[:-1, :, :]) - (cx[:, 1:, :] - cx[:, :-1, :])
+    # velocity scale: |F| / face area
+    DY, DZ = np.meshgrid(dy, dz, indexing="ij")
+    DX2, DZ2 = np.meshgrid(dx, dz, indexing="ij")
+    DX3, DY3 = np.meshgrid(dx, dy, indexing="ij")
+    umax = max(np.abs(Fx / DY[None]).max(), np.abs(Fy / DX2[:, None, :]).max(),
+               np.abs(Fz / DX3[:, :, None]).max(), 1e-300)
+    s = amplitude / umax
+    Fx, Fy, Fz = Fx * s, Fy * s, Fz * s
+
+    def flat(a):
+        return np.ascontiguousarray(a.transpose(2, 1, 0)).ravel()
+
+    # interior faces indexed by low-side cell: x-face between i and i+1 sits at xn[i+1]
+    fx = np.zeros((nx, ny, nz)); fx[:-1] = Fx[1:-1]
+    fy = np.zeros((nx, ny, nz)); fy[:, :-1] = Fy[:, 1:-1]
+    fz = np.zeros((nx, ny, nz)); fz[:, :, :-1] = Fz[:, :, 1:-1]
+    return flat(fx), flat(fy), flat(fz), amplitude
+
+
+def transport_matrix(mesh: BoxMesh, gamma: float, flux_seed: int, cfl: float = 0.5,
+                     perm=None, amplitude: float = 1.0):
+    """Upwind convection-diffusion matrix with a time term.
+
+    Follows the "m1" placement of Process/NewUVW.f90:356-367 for every
+    interior face s = (c1, c2), c1 the low-side cell and F the flux c1->c2:
+      A12 = -min(F,0), A21 = max(F,0), D12 = D21 = Gamma*|S|/d
+      diag(c1) += A21 + D21;   A(c2,c1) -= A21 + D21
+      diag(c2) += A12 + D12;   A(c1,c2) -= A12 + D12
+    then diag += V/dt (NewUVW.f90:447-449) with dt = cfl*h_min/max|u|.
+    """
+    nx, ny, nz = mesh.shape
+    (ax, ay, az), _, _, vol = _face_coeffs(mesh)
+    fx, fy, fz, umax = divergence_free_fluxes(mesh, flux_seed, amplitude)
+    strides = (1, nx, nx * ny)
+    hi_vals, lo_vals = [], []
+    diag = np.zeros(mesh.n_cells)
+    for a, F, s in zip((ax, ay, az), (fx, fy, fz), strides):
+        exists = a > 0
+        F = np.where(exists, F, 0.0)
+        D = gamma * a
+        A12 = -np.minimum(F, 0.0)
+        A21 = np.maximum(F, 0.0)
+        diag += A21 + D                       # diag(c1)
+        diag += _shift_down(A12 + D, s)       # diag(c2)
+        hi_vals.append(-(A12 + D))            # A(c1, c2) stored in row c1
+        lo_vals.append(_shift_down(-(A21 + D), s))  # A(c2, c1) stored in row c2
+    hmin = min(float(np.min(d)) for d in mesh.spacings())
+    dt = cfl * hmin / umax
+    diag += vol / dt
+    m = _assemble(mesh.shape, lo_vals, hi_vals, diag, perm)
+    m.meta.update(kind="transport", shape=mesh.shape, gamma=gamma, dt=dt)
+    return m
+
+
+def seeded_rhs(n: int, seed: int, subtract_mean: bool = True, perm=None) -> np.ndarray:
+    b = np.random.default_rng(seed).uniform(-1.0, 1.0, size=n)
+    if subtract_mean:
+        b = b - b.mean()
+    if perm is not None:
+        out = np.empty_like(b)
+        out[np.asarray(perm)] = b
+        b = out
+    return np.ascontiguousarray(b)
+
+
+def pressure_system(n: int | tuple, seed: int = 1, ratio: float = 1.1, prec: int = 1,
+                    niter: int = 2000, rel_tol: float = 1e-8, perm_seed: int | None = None
+                    ) -> LinearSystem:
+    """Channel pressure-correction system (32^3 / 128^3 / ... recipe).
+
+    2*pi x 2 x pi box, tanh wall stretching in y with adjacent-cell ratio
+    ``ratio``, cell 1 pinned, b ~ U(-1,1) from ``seed`` with its mean
+    subtracted, x0 = 0 and the absolute tolerance ``rel_tol*||b||_2``.
+    """
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+    mesh = channel_mesh(nx, ny, nz, ratio)
+    perm = random_permutation(mesh.n_cells, perm_seed) if perm_seed is not None else None
+    m = pressure_matrix(mesh, pin_cell=1, perm=perm)
+    b = seeded_rhs(m.n, seed, True, perm)
+    tol = rel_tol * float(np.sqrt(np.dot(b, b)))
+    return LinearSystem(m, b, np.zeros(m.n), tol, prec, niter,
+                        dict(recipe="pressure", shape=(nx, ny, nz), seed=seed, ratio=ratio,
+                             rel_tol=rel_tol, perm_seed=perm_seed,
+                             generator_version=GENERATOR_VERSION))
+
+
+def momentum_system(n: int | tuple, seed: int = 2, gamma: float = 1e-3, prec: int = 1,
+                    niter: int = 2000, rel_tol: float = 1e-8, renumber: bool = True,
+                    flux_seed: int | None = None) -> LinearSystem:
+    """Nonsymmetric momentum-type system on a unit cube.
+
+    Upwind convection from a seeded divergence-free velocity of amplitude 1,
+    diffusion ``gamma``, time term for CFL 0.5, random cell renumbering from
+    ``seed``; b ~ U(-1,1).
+    """
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+    mesh = BoxMesh(np.linspace(0, 1, nx + 1), np.linspace(0, 1, ny + 1), np.linspace(0, 1, nz + 1))
+    perm = random_permutation(mesh.n_cells, seed) if renumber else None
+    m = transport_matrix(mesh, gamma, flux_seed if flux_seed is not None else 1000 + seed,
+                         perm=perm)
+    b = seeded_rhs(m.n, seed, False, perm)
+    tol = rel_tol * float(np.sqrt(np.dot(b, b)))
+    return LinearSystem(m, b, np.zeros(m.n), tol, prec, niter,
+                        dict(recipe="momentum", shape=(nx, ny, nz), seed=seed, gamma=gamma,
+                             rel_tol=rel_tol, renumber=renumber,
+                             generator_version=GENERATOR_VERSION))
+
+
+def from_scipy(A, meta=None) -> TFlowSMatrix:
+    """Wrap a square scipy sparse matrix (structurally symmetric pattern with a
+    stored diagonal) in the T-FlowS layout, columns sorted per row."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    n = A.shape[0]
+    nnz = A.nnz
+    acol = (A.indptr + 1).astype(np.int32)
+    arow = np.zeros(nnz + 1, np.int32)
+    arow[:nnz] = A.indices + 1
+    aval = np.zeros(nnz + 1)
+    aval[:nnz] = A.data
+    rows = np.repeat(np.arange(n), np.diff(A.indptr))
+    is_d = A.indices == rows
+    if is_d.sum() != n:
+        raise ValueError("every row needs a stored diagonal entry")
+    adia = np.empty(n, np.int32)
+    adia[rows[is_d]] = np.nonzero(is_d)[0] + 1
+    m = TFlowSMatrix(n, nnz + 1, acol, arow, adia, aval, meta=dict(meta or {}))
+    try:
+        m.mirror = mirror_positions(m)
+    except ValueError:
+        m.mirror = None
+    return m
