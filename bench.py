@@ -1,0 +1,201 @@
+"""Benchmark of the Krylov hot path on one MI355X.
+
+Workload (one "step" = one complete solve to 1e-8 relative residual, inputs
+already resident in HBM): the 128^3 channel pressure-correction system
+(2,097,152 cells, 7-point stencil, 14,581,760 stored entries; 2*pi x 2 x pi
+box, tanh wall stretching ratio 1.1, cell 1 pinned, b ~ U(-1,1) with its mean
+removed, x0 = 0), solved with
+
+  * plain FP64 CG + diagonal preconditioning (the reference algorithm), and
+  * FP32-inner CG inside FP64 iterative refinement (when built),
+
+both to ||b - A x||_2 <= 1e-8 ||b||_2, checked in FP64 after the solve.
+
+Printed: one JSON line.  ``value`` is the time to 1e-8 of the headline
+solver (mixed precision when available, else FP64) in ms; the FP64 time,
+the speed-up and the SpMV bandwidth are extra fields.  ``roofline`` is for
+the SpMV kernel: algorithmic bytes per launch (12*nnz + 4*(N+1) + 16*N for
+FP64) over its mean launch time measured with HIP events on the stream it is
+launched on.  ``cpu_baseline`` times the reference solver (oracle/_ref) or
+its C restatement on one host core over a bounded number of iterations and
+scales to the full iteration count.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def spmv_bytes(n, nnz, word):
+    return (word + 4) * nnz + 4 * (n + 1) + 2 * word * n
+
+
+def time_spmv(torch, ds, n, reps=50, fp32=False):
+    """Mean duration of one SpMV launch, from HIP events on the launch stream."""
+    dt = torch.float32 if fp32 else torch.float64
+    x = torch.randn(n, dtype=dt, device="cuda")
+    y = torch.empty(n, dtype=dt, device="cuda")
+    f = ds.spmv32 if fp32 else ds.spmv
+    for _ in range(5):
+        f(x, y)
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        f(x, y)
+    e1.record(s)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps * 1e-3  # seconds per launch
+
+
+def cpu_baseline(sysm, b, tol, full_iters, budget_s=20.0):
+    """Reference Fortran CG (or its C restatement) on one core, bounded sample."""
+    from oracle import oracle as O
+    kind = "reference" if O.available_reference() else "port"
+    solver = O.Reference() if kind == "reference" else O.Oracle()
+    m = sysm
+    # calibrate on a few iterations, then run a sample of ~budget_s
+    t0 = time.perf_counter()
+    solver.run("cg", m, b, prec=1, niter=3, tol=tol)
+    per_it = (time.perf_counter() - t0) / 3.0
+    k = int(max(5, min(full_iters, budget_s / max(per_it, 1e-6))))
+    t0 = time.perf_counter()
+    res = solver.run("cg", m, b, prec=1, niter=k, tol=tol)
+    el = time.perf_counter() - t0
+    it_done = res["niter"] if res["fin_res"] < tol else k
+    per_it = el / it_done
+    est_ms = per_it * full_iters * 1e3
+    return dict(value=round(est_ms, 1), unit="ms", cores=1, kind=kind,
+                sample=f"{it_done} of {full_iters} FP64 Jacobi-CG iterations on the same 128^3 system, "
+                       f"1 thread, {per_it*1e3:.1f} ms/iteration, scaled to the full solve",
+                ms_per_iteration=round(per_it * 1e3, 3))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128, help="cells per direction")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note "
+              "(DESIGN.md), not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__ as ge
+    pkg = ge.load()
+    torch.cuda.set_device(0)
+
+    n = args.n
+    log(f"generating {n}^3 pressure system")
+    s = pkg.meshgen.pressure_system(n, seed=1, prec=1)
+    m = s.matrix
+    N, nnz = m.n, m.nnz
+    log(f"N={N} nnz={nnz}")
+    ds = pkg.krylov.DeviceSystem(m)
+    b_dev = torch.from_numpy(s.b).cuda()
+    x = torch.zeros(N, dtype=torch.float64, device="cuda")
+    r = torch.empty_like(b_dev)
+    A_sp = m.to_scipy()
+    bnorm = float(np.linalg.norm(s.b))
+
+    def run_fp64():
+        x.zero_()
+        r.copy_(b_dev)
+        return ds.solve("cg", 1, x, r, 20000, s.tol)
+
+    def true_rel_res():
+        xh = x.cpu().numpy()
+        return float(np.linalg.norm(s.b - A_sp @ xh) / bnorm)
+
+    have_mixed = hasattr(ds, "solve_mixed") and ds.lib is not None and hasattr(ds.lib, "tflows_solve_mixed")
+
+    def run_mixed():
+        x.zero_()
+        return ds.solve_mixed("cg", 1, x, b_dev, 20000, s.tol)
+
+    results = {}
+    for name, fn in (("fp64", run_fp64),) + ((("mixed", run_mixed),) if have_mixed else ()):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        times, its = [], None
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            its = out[0]
+        rr = true_rel_res()
+        results[name] = dict(ms=float(np.mean(times)) * 1e3, ms_min=float(np.min(times)) * 1e3, iters=its,
+                             true_rel_res=rr)
+        log(f"{name}: {results[name]}")
+        assert rr <= 1.0001e-8 * (1 + 1e-6) or name == "fp64" and rr <= 2e-8, f"{name} missed the tolerance"
+
+    # SpMV bandwidth (dominant kernel), timed on the stream it runs on
+    t64 = time_spmv(torch, ds, N, fp32=False)
+    b64 = spmv_bytes(N, nnz, 8)
+    t32 = time_spmv(torch, ds, N, fp32=True)
+    b32 = spmv_bytes(N, nnz, 4)
+    head = "mixed" if "mixed" in results else "fp64"
+    spmv_t, spmv_b = (t32, b32) if head == "mixed" else (t64, b64)
+    ach = spmv_b / spmv_t / 1e9
+    out = {
+        "metric": "time-to-1e-8 rel. residual, 128^3 Poisson: mixed-prec vs FP64 CG; SpMV HBM GB/s",
+        "value": round(results[head]["ms"], 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(results[head]["ms"], 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if head == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"pressure-correction Poisson {n}^3 channel (N={N}, nnz={nnz}), "
+                               f"Jacobi-preconditioned CG to 1e-8 relative residual",
+                   "solver": head, "cells": N, "nnz": nnz},
+        "fp64_cg_ms": round(results["fp64"]["ms"], 3),
+        "fp64_cg_iters": results["fp64"]["iters"],
+        "fp64_true_rel_res": results["fp64"]["true_rel_res"],
+        "mixed_ms": round(results["mixed"]["ms"], 3) if "mixed" in results else None,
+        "mixed_iters": results["mixed"]["iters"] if "mixed" in results else None,
+        "mixed_true_rel_res": results["mixed"]["true_rel_res"] if "mixed" in results else None,
+        "speedup_mixed_vs_fp64": (round(results["fp64"]["ms"] / results["mixed"]["ms"], 3)
+                                  if "mixed" in results else None),
+        "spmv_fp64_GBs": round(b64 / t64 / 1e9, 1),
+        "spmv_fp32_GBs": round(b32 / t32 / 1e9, 1),
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_spmv<float>" if head == "mixed" else "k_spmv<double>",
+                     "bytes_per_launch": spmv_b, "us_per_launch": round(spmv_t * 1e6, 2)},
+    }
+    if not args.no_cpu:
+        log("timing the CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(m, s.b, s.tol, results["fp64"]["iters"], args.cpu_budget)
+    ds.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

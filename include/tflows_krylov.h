@@ -1,0 +1,127 @@
+/*
+ * tflows_krylov.h -- C ABI of the MI355X (gfx950) Krylov solver library
+ * libtflows_krylov.so.
+ *
+ * Two groups of entry points:
+ *
+ * 1. Drop-in replacements of the reference's external Fortran subroutines.
+ *    They keep the Fortran calling convention exactly (every argument by
+ *    reference, lowercase name with a trailing underscore, no hidden
+ *    arguments), take HOST arrays in the reference layout and return when
+ *    the results are back in host memory, so a Fortran host links them in
+ *    place of the reference objects and its call sites compile unchanged.
+ *    Array bases follow the Fortran declarations: x is x(-NB:N) and Ab is
+ *    Ab(-NB:-1), so x(i) is x[NB + i]; A, Acol, Arow, Adia, r1 are 1-based
+ *    and the stored indices are 1-based.  The single-domain build is
+ *    assumed (NPro = 0): Ab and x(-NB:-1) are never read or written.
+ *
+ * 2. Device-resident entry points (tflows_*): the matrix pattern and values
+ *    and the x / r vectors live in HBM, the caller passes device pointers
+ *    and a HIP stream; used by batching, mixed precision, the adjoint and
+ *    the benchmarks.  They return 0 on success and a negative code on
+ *    failure; tflows_last_error() gives the message.
+ *
+ * All FP arithmetic of the drop-ins is IEEE FP64; indices are int32.
+ */
+#ifndef TFLOWS_KRYLOV_H
+#define TFLOWS_KRYLOV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * 1. Drop-in Fortran-ABI entry points
+ * ---------------------------------------------------------------------- */
+
+/* Preconditioned CG.  Replaces SUBROUTINE cg, Process/CG.f90:2-3.
+ * r1 enters as the right-hand side b and leaves as the final residual;
+ * x enters as the initial guess and leaves as the solution; niter enters as
+ * the cap and leaves as the iteration count (cap+1 if not converged);
+ * the stopping test is ||r||_2 < tol (absolute).  prec: 0 none, 1 diagonal,
+ * 2 diagonal incomplete Cholesky.  On the early exit ||b|| < tol, IniRes is
+ * not written and FinRes receives the residual left by the previous solve,
+ * as the reference's module variable `error` does. */
+void cg_(const int32_t* N, const int32_t* NB, const int32_t* NONZ, const double* A, const int32_t* Acol,
+         const int32_t* Arow, const int32_t* Adia, const double* Ab, double* x, double* r1, const int32_t* prec,
+         int32_t* niter, const double* tol, double* IniRes, double* FinRes);
+
+/* "BiCG" with the reference's semantics: the shadow system is multiplied by
+ * A (not A^T) and starts from r2 = r1.  Replaces SUBROUTINE bicg,
+ * Process/BiCG.f90:2-3.  Arguments as cg_. */
+void bicg_(const int32_t* N, const int32_t* NB, const int32_t* NONZ, const double* A, const int32_t* Acol,
+           const int32_t* Arow, const int32_t* Adia, const double* Ab, double* x, double* r1,
+           const int32_t* prec, int32_t* niter, const double* tol, double* IniRes, double* FinRes);
+
+/* Preconditioned CGS with the fixed shadow r~ = r0.  Replaces SUBROUTINE
+ * CGS, Process/CGS.f90:2-3 (Fortran names are case-insensitive: cgs_).
+ * Arguments as cg_. */
+void cgs_(const int32_t* N, const int32_t* NB, const int32_t* NONZ, const double* A, const int32_t* Acol,
+          const int32_t* Arow, const int32_t* Adia, const double* Ab, double* x, double* r1, const int32_t* prec,
+          int32_t* niter, const double* tol, double* IniRes, double* FinRes);
+
+/* Preconditioner set-up D(1:N).  Replaces SUBROUTINE Prec1,
+ * Process/Prec1.f90:2: prec 1 -> D = diag(A); 2 -> DIC pivots
+ * D(i) = 1/(a_ii - sum_{k<i} D(k) a_ik^2); otherwise D = 1. */
+void prec1_(const int32_t* N, const int32_t* NONZ, const double* A, const int32_t* Acol, const int32_t* Arow,
+            const int32_t* Adia, double* D, const int32_t* prec);
+
+/* r1 <- r1 - A x.  Replaces SUBROUTINE Resid, Process/Resid.f90:2. */
+void resid_(const int32_t* N, const int32_t* NB, const int32_t* NONZ, const double* A, const int32_t* Acol,
+            const int32_t* Arow, const double* Ab, const double* x, double* r1);
+
+/* ------------------------------------------------------------------------
+ * 2. Device-resident API (no counterpart in the reference)
+ * ---------------------------------------------------------------------- */
+
+enum { TFLOWS_CG = 0, TFLOWS_BICG = 1, TFLOWS_CGS = 2 };
+enum { TFLOWS_MEM_HOST = 0, TFLOWS_MEM_DEVICE = 1 };
+
+typedef struct tflows_system_s* tflows_system;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* tflows_last_error(void);
+
+/* Library version string. */
+const char* tflows_version(void);
+
+/* Create a system for an N x N matrix in the T-FlowS CSR layout
+ * (Acol = 1-based row pointer of N+1, Arow = 1-based column index of NONZ,
+ * Adia = 1-based diagonal position, NONZ = nnz + 1), from host or device
+ * arrays according to `mem`.  The pattern is copied; values are set with
+ * tflows_system_set_values.  `stream` is a hipStream_t (NULL = default). */
+int tflows_system_create(int32_t n, int32_t nonz, const int32_t* acol, const int32_t* arow, const int32_t* adia,
+                         int32_t mem, void* stream, tflows_system* out);
+int tflows_system_destroy(tflows_system s);
+
+/* Copy the NONZ values A(1:NONZ) (host or device). */
+int tflows_system_set_values(tflows_system s, const double* aval, int32_t mem, void* stream);
+
+/* Solve on the device.  x (N doubles, x(1:N)) and r1 (N doubles, in: b,
+ * out: final residual) are device pointers; *niter is cap in / count out;
+ * *ini_res is written only when the reference would write IniRes;
+ * *fin_res follows the reference's FinRes.  Synchronous with respect to
+ * the host on return. */
+int tflows_solve(tflows_system s, int32_t method, int32_t prec, double* x, double* r1, int32_t* niter,
+                 double tol, double* ini_res, double* fin_res, void* stream);
+
+/* y = A x on the device, FP64 (x, y: N doubles). */
+int tflows_spmv(tflows_system s, const double* x, double* y, void* stream);
+/* y = A x with the FP32 copy of the values (x, y: N floats). */
+int tflows_spmv32(tflows_system s, const float* x, float* y, void* stream);
+/* Preconditioner set-up on the device: D (N doubles, device). */
+int tflows_prec1(tflows_system s, int32_t prec, double* d, void* stream);
+/* r <- r - A x on the device (Resid form, FP64). */
+int tflows_resid(tflows_system s, const double* x, double* r, void* stream);
+
+/* Number of DIC levels of the lower-triangular dependency graph
+ * (0 before the first prec = 2 use); for reporting. */
+int32_t tflows_system_levels(tflows_system s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TFLOWS_KRYLOV_H */

@@ -45,6 +45,7 @@ static double g_error = 0.0;
 
 void oracle_reset_state(void) { g_error = 0.0; }
 double oracle_get_error(void) { return g_error; }
+void oracle_set_error(double e) { g_error = e; }
 
 /* 1-based accessors */
 #define AV(j) A[(j) - 1]

@@ -129,6 +129,9 @@ class Oracle(_Solvers):
     def get_error(self) -> float:
         return self.lib.oracle_get_error()
 
+    def set_error(self, e: float):
+        self.lib.oracle_set_error(C.c_double(e))
+
     def spmv(self, m, p):
         q = np.zeros(m.n)
         self.lib.oracle_spmv(m.n, m.aval, m.acol, m.arow, np.ascontiguousarray(p, np.float64), q)

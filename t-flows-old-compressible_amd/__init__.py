@@ -4,3 +4,4 @@ The directory name carries hyphens, so it is loaded as the package
 ``tflows_amd`` by :func:`load` in ``__graft_entry__`` and ``tests/conftest``.
 """
 from . import meshgen  # noqa: F401
+from . import krylov  # noqa: F401,E402

@@ -1,0 +1,198 @@
+// capi.hip -- the extern "C" boundary of libtflows_krylov.so
+// (declared in include/tflows_krylov.h).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "../../include/tflows_krylov.h"
+#include "system.h"
+
+using tflows::System;
+
+struct tflows_system_s {
+    std::unique_ptr<System> sys;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+// The reference keeps `error` in a module (Library/sol_mod.f90:11) shared by
+// cg, bicg and cgs; its value survives between calls and is what FinRes
+// receives on the ||b|| < tol early exit.  One process-wide copy here.
+double g_sol_error = 0.0;
+std::mutex g_mutex;
+
+int fail(const std::string& msg) {
+    g_last_error = msg;
+    return -1;
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e.what());
+    }
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+[[noreturn]] void die(const char* where) {
+    std::fprintf(stderr, "libtflows_krylov: %s failed: %s\n", where, g_last_error.c_str());
+    std::fflush(stderr);
+    std::abort();
+}
+
+// Host-array drop-in: upload pattern, values, x(1:N) and b, solve, download.
+// The device copy is rebuilt on every call (the caller may change any array
+// between calls); only allocations are reused while N and NONZ stay equal.
+struct HostCache {
+    std::unique_ptr<System> sys;
+    int n = -1, nonz = -1;
+    tflows::DevArray<double> x, r;
+};
+HostCache g_cache;
+
+void drop_in(int method, const int32_t* N, const int32_t* NB, const int32_t* NONZ, const double* A,
+             const int32_t* Acol, const int32_t* Arow, const int32_t* Adia, double* x, double* r1,
+             const int32_t* prec, int32_t* niter, const double* tol, double* IniRes, double* FinRes) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    const int n = *N, nb = *NB, nonz = *NONZ;
+    hipStream_t s = nullptr;
+    g_cache.sys.reset();
+    g_cache.sys.reset(new System(n, nonz, Acol, Arow, Adia, false, s));
+    g_cache.n = n;
+    g_cache.nonz = nonz;
+    System& S = *g_cache.sys;
+    S.set_values(A, false, s);
+    g_cache.x.ensure(n);
+    g_cache.r.ensure(n);
+    TF_CHECK(hipMemcpyAsync(g_cache.x.p, x + nb + 1, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    TF_CHECK(hipMemcpyAsync(g_cache.r.p, r1, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    tflows::SolveResult res = S.solve(method, *prec, g_cache.x.p, g_cache.r.p, *niter, *tol, g_sol_error, s);
+    TF_CHECK(hipMemcpyAsync(x + nb + 1, g_cache.x.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    TF_CHECK(hipMemcpyAsync(r1, g_cache.r.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    TF_CHECK(hipStreamSynchronize(s));
+    *niter = res.niter;
+    if (res.ini_written) *IniRes = res.ini_res;
+    *FinRes = res.fin_res;
+    g_sol_error = res.fin_res;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tflows_last_error(void) { return g_last_error.c_str(); }
+const char* tflows_version(void) { return "tflows-krylov 0.1 gfx950"; }
+
+#define TF_DROPIN(name, method)                                                                                 \
+    void name(const int32_t* N, const int32_t* NB, const int32_t* NONZ, const double* A, const int32_t* Acol,   \
+              const int32_t* Arow, const int32_t* Adia, const double* Ab, double* x, double* r1,               \
+              const int32_t* prec, int32_t* niter, const double* tol, double* IniRes, double* FinRes) {        \
+        (void)Ab;                                                                                               \
+        if (guarded([&] { drop_in(method, N, NB, NONZ, A, Acol, Arow, Adia, x, r1, prec, niter, tol, IniRes,  \
+                                  FinRes); }))                                                                  \
+            die(#name);                                                                                         \
+    }
+
+TF_DROPIN(cg_, 0)
+TF_DROPIN(bicg_, 1)
+TF_DROPIN(cgs_, 2)
+
+void prec1_(const int32_t* N, const int32_t* NONZ, const double* A, const int32_t* Acol, const int32_t* Arow,
+            const int32_t* Adia, double* D, const int32_t* prec) {
+    if (guarded([&] {
+            hipStream_t s = nullptr;
+            System S(*N, *NONZ, Acol, Arow, Adia, false, s);
+            S.set_values(A, false, s);
+            tflows::DevArray<double> d;
+            d.ensure(*N);
+            S.prec1(*prec, d.p, s);
+            TF_CHECK(hipMemcpy(D, d.p, sizeof(double) * (*N), hipMemcpyDeviceToHost));
+        }))
+        die("prec1_");
+}
+
+void resid_(const int32_t* N, const int32_t* NB, const int32_t* NONZ, const double* A, const int32_t* Acol,
+            const int32_t* Arow, const double* Ab, const double* x, double* r1) {
+    (void)Ab;
+    if (guarded([&] {
+            hipStream_t s = nullptr;
+            const int n = *N, nb = *NB;
+            // Resid has no Adia argument; the pattern is validated with the
+            // diagonal found by scanning each row.
+            std::unique_ptr<int32_t[]> adia(new int32_t[n]);
+            for (int i = 1; i <= n; ++i) {
+                adia[i - 1] = Acol[i - 1];
+                for (int j = Acol[i - 1]; j < Acol[i]; ++j)
+                    if (Arow[j - 1] == i) adia[i - 1] = j;
+            }
+            System S(n, *NONZ, Acol, Arow, adia.get(), false, s);
+            S.set_values(A, false, s);
+            tflows::DevArray<double> dx, dr;
+            dx.ensure(n);
+            dr.ensure(n);
+            TF_CHECK(hipMemcpy(dx.p, x + nb + 1, sizeof(double) * n, hipMemcpyHostToDevice));
+            TF_CHECK(hipMemcpy(dr.p, r1, sizeof(double) * n, hipMemcpyHostToDevice));
+            S.resid(dx.p, dr.p, s);
+            TF_CHECK(hipMemcpy(r1, dr.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        }))
+        die("resid_");
+}
+
+int tflows_system_create(int32_t n, int32_t nonz, const int32_t* acol, const int32_t* arow, const int32_t* adia,
+                         int32_t mem, void* stream, tflows_system* out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("out is NULL");
+        auto h = std::make_unique<tflows_system_s>();
+        h->sys.reset(new System(n, nonz, acol, arow, adia, mem == TFLOWS_MEM_DEVICE, as_stream(stream)));
+        *out = h.release();
+    });
+}
+
+int tflows_system_destroy(tflows_system s) {
+    return guarded([&] { delete s; });
+}
+
+int tflows_system_set_values(tflows_system s, const double* aval, int32_t mem, void* stream) {
+    return guarded([&] { s->sys->set_values(aval, mem == TFLOWS_MEM_DEVICE, as_stream(stream)); });
+}
+
+int tflows_solve(tflows_system s, int32_t method, int32_t prec, double* x, double* r1, int32_t* niter,
+                 double tol, double* ini_res, double* fin_res, void* stream) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        tflows::SolveResult r = s->sys->solve(method, prec, x, r1, *niter, tol, g_sol_error, as_stream(stream));
+        *niter = r.niter;
+        if (r.ini_written && ini_res) *ini_res = r.ini_res;
+        if (fin_res) *fin_res = r.fin_res;
+        g_sol_error = r.fin_res;
+    });
+}
+
+int tflows_spmv(tflows_system s, const double* x, double* y, void* stream) {
+    return guarded([&] { s->sys->spmv(x, y, as_stream(stream)); });
+}
+int tflows_spmv32(tflows_system s, const float* x, float* y, void* stream) {
+    return guarded([&] { s->sys->spmv32(x, y, as_stream(stream)); });
+}
+int tflows_prec1(tflows_system s, int32_t prec, double* d, void* stream) {
+    return guarded([&] { s->sys->prec1(prec, d, as_stream(stream)); });
+}
+int tflows_resid(tflows_system s, const double* x, double* r, void* stream) {
+    return guarded([&] { s->sys->resid(x, r, as_stream(stream)); });
+}
+int32_t tflows_system_levels(tflows_system s) {
+    int32_t v = 0;
+    if (guarded([&] { v = s->sys->levels().nfwd; })) return -1;
+    return v;
+}
+
+}  // extern "C"

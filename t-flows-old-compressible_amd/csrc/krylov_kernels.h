@@ -1,0 +1,89 @@
+// krylov_kernels.h -- host-side declarations of the kernel launchers
+// (definitions and explicit instantiations for double and float live in
+// krylov_kernels.hip).
+#pragma once
+#include "kernels.h"
+
+namespace tflows {
+
+enum { RHO_NONE_ = 0, RHO_JAC_ = 1, RHO_BICG_ = 2, RHO_RR_ = 3 };
+
+// Partial-sum buffers of the four reduction sites of a solve; each holds
+// 2 values x `grid` blocks.  Producer and consumer kernels of a site are
+// launched with the same grid.
+template <typename T>
+struct Sites {
+    T* A;
+    T* B;
+    T* C;
+    T* D;
+    int grid;
+    static Sites carve(T* base, int grid) { return Sites{base, base + 2 * grid, base + 4 * grid, base + 6 * grid, grid}; }
+    static size_t elems(int grid) { return (size_t)8 * grid; }
+};
+
+template <typename T>
+void launch_setup(const Csr<T>& A, int prec, bool write_d, T* D, const T* b, const Sites<T>& S, hipStream_t s);
+template <typename T>
+void launch_dic_setup_level(const Csr<T>& A, const int* rows, int count, T* D, hipStream_t s);
+template <typename T>
+void launch_dic_fwd_level(const Csr<T>& A, const int* rows, int count, const T* D, const T* u, T* z,
+                          const SolverState* st, hipStream_t s);
+template <typename T>
+void launch_dic_bwd_level(const Csr<T>& A, const int* rows, int count, const T* D, T* z,
+                          const SolverState* st, hipStream_t s);
+template <typename T>
+void launch_resid_init(int mode, const Csr<T>& A, const T* x, T* r, T* r2, const T* D, const Sites<T>& S,
+                       SolverState* st, HostMirror* hm, hipStream_t s);
+template <typename T>
+void launch_check(int k, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
+template <typename T>
+void launch_rho(int n, const T* a, const T* b, const Sites<T>& S, const SolverState* st, hipStream_t s);
+template <typename T>
+void launch_cg_p(bool jac, int k, int n, const T* r, const T* D, const T* z, T* p, const Sites<T>& S,
+                 SolverState* st, HostMirror* hm, hipStream_t s);
+template <typename T>
+void launch_cg_spmv(const Csr<T>& A, const T* p, T* q, const Sites<T>& S, const SolverState* st, hipStream_t s);
+template <typename T>
+void launch_cg_update(bool jac, int k, int n, T* x, T* r, const T* p, const T* q, const T* D, const Sites<T>& S,
+                      const SolverState* st, hipStream_t s);
+template <typename T>
+void launch_bicg_p(bool jac, int k, int n, const T* r1, const T* r2, const T* D, const T* z1, const T* z2, T* p1,
+                   T* p2, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
+template <typename T>
+void launch_bicg_spmv(const Csr<T>& A, const T* p1, const T* p2, T* q1, T* q2, const Sites<T>& S,
+                      const SolverState* st, hipStream_t s);
+template <typename T>
+void launch_bicg_update(bool jac, int k, int n, T* x, T* r1, T* r2, const T* p1, const T* q1, const T* q2,
+                        const T* D, const Sites<T>& S, const SolverState* st, hipStream_t s);
+template <typename T>
+void launch_cgs_u(bool jac, int k, int n, const T* r1, const T* q1, T* u1, T* u2, const T* D, T* p2,
+                  const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
+template <typename T>
+void launch_cgs_spmv_v(const Csr<T>& A, const T* p2, T* v2, const T* r2, const Sites<T>& S, const SolverState* st,
+                       hipStream_t s);
+template <typename T>
+void launch_cgs_q(bool jac, int k, int n, const T* u1, const T* v2, T* q1, const T* D, T* p1, const Sites<T>& S,
+                  const SolverState* st, hipStream_t s);
+template <typename T>
+void launch_cgs_spmv_r(int k, const Csr<T>& A, const T* p1, T* x, T* r1, const T* r2, const Sites<T>& S,
+                       const SolverState* st, hipStream_t s);
+template <typename T>
+void launch_spmv(const Csr<T>& A, const T* x, T* y, int grid, hipStream_t s);
+template <typename T>
+void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s);
+void launch_rebase(int count, const int* in, int* out, hipStream_t s);
+void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t s);
+
+// Number of blocks of the row-streaming kernels for n rows: one block per
+// 256-row tile up to 8 blocks per CU (2048 on MI355X), rounded to a
+// multiple of 8 so that the XCD-aware block mapping is a bijection.
+inline int stream_grid(int n, int max_blocks = 2048) {
+    int tiles = (n + kTileRows - 1) / kTileRows;
+    int g = tiles < max_blocks ? tiles : max_blocks;
+    if (g < 1) g = 1;
+    if (g >= kNumXcd) g = (g + kNumXcd - 1) / kNumXcd * kNumXcd;
+    return g;
+}
+
+}  // namespace tflows

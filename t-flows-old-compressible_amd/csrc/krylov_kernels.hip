@@ -1,0 +1,753 @@
+// krylov_kernels.hip -- the gfx950 kernels of the CG / BiCG / CGS path.
+//
+// One Krylov iteration is a short, fixed sequence of fused kernels.  Every
+// scalar the algorithm needs between them (rho, alfa, beta, ||r||) is
+// derived on the device from per-block partial sums ("reduction sites"):
+// the producing kernel stores one partial per block, the consuming kernel
+// sums them in a fixed order in every block.  There is no host round trip,
+// no atomic and no fence inside an iteration.  Each kernel first checks
+// SolverState::done and returns at once after convergence, so the host can
+// run ahead by whole chunks of iterations.
+//
+// Reduction sites (each 2 values x G blocks):
+//   A  ||b||^2                       k_setup          -> k_resid_init
+//   B  ||r||^2, next rho             k_resid_init / *_update / k_cgs_spmv_r
+//                                                     -> stopping test of the next iteration
+//   C  rho = (r,z) for DIC forms     k_rho            -> k_cg_p / k_bicg_p
+//   D  alfa denominator              *_spmv*          -> *_update / k_cgs_q / k_cgs_spmv_r
+//
+// Reference loops restated (single domain, Sequential build):
+//   Prec1  Process/Prec1.f90:45-66          -> k_setup, k_dic_setup_level
+//   Resid  Process/Resid.f90:29-34          -> k_resid_init, k_resid
+//   CG     Process/CG.f90:62-262            -> k_cg_p, k_cg_spmv, k_cg_update
+//   BiCG   Process/BiCG.f90:50-254          -> k_bicg_p, k_bicg_spmv, k_bicg_update
+//   CGS    Process/CGS.f90:62-337           -> k_cgs_u, k_cgs_spmv_v, k_cgs_q, k_cgs_spmv_r
+//   DIC    Process/CG.f90:135-156           -> k_dic_fwd_level, k_dic_bwd_level
+#include "krylov_kernels.h"
+
+namespace tflows {
+
+// ---------------------------------------------------------------------------
+// small functors for stream_tile
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Gather1 {
+    const T* __restrict__ x;
+    __device__ void operator()(int c, T (&g)[1]) const { g[0] = x[c]; }
+};
+template <typename T>
+struct Gather2 {
+    const T* __restrict__ x;
+    const T* __restrict__ y;
+    __device__ void operator()(int c, T (&g)[2]) const { g[0] = x[c]; g[1] = y[c]; }
+};
+template <typename T>
+struct InitZero {
+    __device__ T operator()(int, int) const { return T(0); }
+};
+template <typename T>
+struct InitVec {
+    const T* __restrict__ b;
+    __device__ T operator()(int r, int) const { return b[r]; }
+};
+
+#define TFLOWS_STREAM_SMEM(T, NRHS)                  \
+    __shared__ int s_rp[kTileRows + 1];               \
+    __shared__ T s_prod[(NRHS) * LdsCap<T>::value]
+
+// Rows [rb, re) of this block for the element-wise kernels (the same tile
+// split as the stream kernels, so a block touches the rows it streamed).
+#define TFLOWS_BLOCK_ROWS(n)                                  \
+    const int ntiles_ = ((n) + kTileRows - 1) / kTileRows;    \
+    int t0_, t1_;                                             \
+    block_tiles(ntiles_, t0_, t1_);                           \
+    const int rb = t0_ * kTileRows, re = min(t1_ * kTileRows, (n))
+
+__device__ __forceinline__ bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
+// z_i of the Jacobi / identity preconditioner: a true division by D
+// (Process/CG.f90:124-127; prec = 0 uses D = 1, and r/1 == r exactly).
+template <typename T>
+__device__ __forceinline__ T jac(T r, T d) { return r / d; }
+
+// ---------------------------------------------------------------------------
+// Stopping test at the start of iteration k, from residual site B written
+// by the previous kernel chain: for k == 1 it is the initial-residual test
+// (CG.f90:93-108), for k > 1 the test at the end of iteration k-1
+// (CG.f90:248-255), and k == cap + 1 closes the DO loop with niter = cap + 1
+// (CG.f90:259-262).  Every block takes the same decision; the leader
+// records it.  Returns true when the solve is over.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ bool check_stop(int k, const T* __restrict__ partB, SolverState* st, HostMirror* hm,
+                                           T (&tot)[2]) {
+    site_total<T, 2>(partB, tot);
+    const double err = sqrt((double)tot[0]);
+    const double tol = st->tol;
+    const int cap = st->cap;
+    if (leader()) {
+        st->error = err;
+        if (k == 1) {
+            st->ini_res = err;
+            st->ini_written = 1;
+        }
+    }
+    if (err < tol) {
+        if (leader()) {
+            st->done = 1;
+            st->niter_out = k - 1;
+            publish(hm, 1, k - 1);
+        }
+        return true;
+    }
+    if (k > cap) {
+        if (leader()) {
+            st->done = 1;
+            st->niter_out = (cap > 0 ? cap : 0) + 1;
+            publish(hm, 1, k - 1);
+        }
+        return true;
+    }
+    if (leader()) publish(hm, 0, k - 1);
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// set-up: D = diag(A) (prec 1) or 1 (prec 0), and ||b||^2 -> site A
+// (Prec1.f90:45-48, 63-66; CG.f90:62-67).  With prec 2 the DIC pivots are
+// computed before this kernel by k_dic_setup_level and write_d is false.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_setup(Csr<T> A, int prec, bool write_d, T* __restrict__ D,
+                                                  const T* __restrict__ b, T* __restrict__ partA) {
+    TFLOWS_BLOCK_ROWS(A.n);
+    T v[1] = {T(0)};
+    for (int i = rb + threadIdx.x; i < re; i += kBlock) {
+        if (write_d) D[i] = (prec == 1) ? A.val[A.dia[i]] : T(1);
+        const T bi = b[i];
+        v[0] += bi * bi;
+    }
+    site_write<T, 1>(v, partA);
+}
+
+// ---------------------------------------------------------------------------
+// DIC pivots for one level of rows (Prec1.f90:51-59):
+//   D(i) = 1 / (a_ii - sum_{lower j} D(k) * a_ij * a_ij)
+// Rows of a level depend only on rows of earlier levels.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dic_setup_level(Csr<T> A, const int* __restrict__ rows, int count,
+                                                            T* __restrict__ D) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    const int i = rows[t];
+    const int d = A.dia[i];
+    T sum1 = A.val[d];
+    for (int j = A.rowptr[i]; j < d; ++j) {
+        const T a = A.val[j];
+        sum1 = sum1 - D[A.col[j]] * a * a;
+    }
+    D[i] = T(1) / sum1;
+}
+
+// Forward sweep of the DIC apply for one level (CG.f90:135-142):
+//   z(i) = (u(i) - sum_{lower} a_ij z(k)) * D(i)
+// `u` and `z` may alias (each row reads its own u before writing its z).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dic_fwd_level(Csr<T> A, const int* __restrict__ rows, int count,
+                                                          const T* __restrict__ D, const T* u, T* z,
+                                                          const SolverState* st) {
+    if (st && st->done) return;
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    const int i = rows[t];
+    const int d = A.dia[i];
+    T sum1 = u[i];
+    for (int j = A.rowptr[i]; j < d; ++j) sum1 = sum1 - A.val[j] * z[A.col[j]];
+    z[i] = sum1 * D[i];
+}
+
+// Backward sweep for one level, fused with the scaling pass of
+// CG.f90:144-146 (each row scales its own forward value just before use,
+// which performs the same two roundings as the separate pass):
+//   z(i) = (z(i)/(D(i)+TINY) - sum_{upper} a_ij z(k)) * D(i)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dic_bwd_level(Csr<T> A, const int* __restrict__ rows, int count,
+                                                          const T* __restrict__ D, T* z,
+                                                          const SolverState* st) {
+    if (st && st->done) return;
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    const int i = rows[t];
+    const int d = A.dia[i];
+    const T di = D[i];
+    T sum1 = z[i] / (di + T(kTiny));
+    const int e = A.rowptr[i + 1];
+    for (int j = d + 1; j < e; ++j) sum1 = sum1 - A.val[j] * z[A.col[j]];
+    z[i] = sum1 * di;
+}
+
+// ---------------------------------------------------------------------------
+// Early exit on ||b|| < tol (CG.f90:69-72) from site A, then the initial
+// residual r = b - A x in place (Resid.f90:29-34: subtraction in column
+// order starting from b) with ||r||^2 and the first rho -> site B:
+//   RHO_JAC : rho = sum r*(r/D)          (CG with prec 0/1)
+//   RHO_BICG: rho = sum (r/D)*r2, r2 = r (BiCG with prec 0/1)
+//   RHO_RR  : rho = sum r*r2,     r2 = r (CGS, any prec)
+//   RHO_NONE: rho comes later from the DIC z (CG/BiCG with prec 2)
+// r2 (if not null) receives a copy of r (BiCG.f90:93-95, CGS.f90:84-86).
+// ---------------------------------------------------------------------------
+enum RhoMode { RHO_NONE = 0, RHO_JAC = 1, RHO_BICG = 2, RHO_RR = 3 };
+
+template <typename T, int MODE>
+struct ResidEpi {
+    T* __restrict__ r;
+    T* __restrict__ r2;
+    const T* __restrict__ D;
+    T acc[2];
+    __device__ void operator()(int row, T (&y)[1]) {
+        const T ri = y[0];
+        r[row] = ri;
+        if (r2) r2[row] = ri;
+        acc[0] += ri * ri;
+        if (MODE == RHO_JAC) acc[1] += ri * jac(ri, D[row]);
+        if (MODE == RHO_BICG) acc[1] += jac(ri, D[row]) * ri;
+        if (MODE == RHO_RR) acc[1] += ri * ri;
+    }
+};
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_resid_init(Csr<T> A, const T* __restrict__ x, T* __restrict__ r,
+                                                       T* __restrict__ r2, const T* __restrict__ D,
+                                                       const T* __restrict__ partA, T* __restrict__ partB,
+                                                       SolverState* st, HostMirror* hm) {
+    TFLOWS_STREAM_SMEM(T, 1);
+    T bb[1];
+    site_total<T, 1>(partA, bb);
+    const double bn = sqrt((double)bb[0]);
+    if (bn < st->tol) {
+        if (leader()) {
+            st->bnrm2 = bn;
+            st->done = 1;
+            st->niter_out = 0;
+            publish(hm, 1, 0);
+        }
+        return;
+    }
+    if (leader()) st->bnrm2 = bn;
+    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    Gather1<T> g{x};
+    InitVec<T> init{r};
+    ResidEpi<T, MODE> epi{r, r2, D, {T(0), T(0)}};
+    for (int t = t0; t < t1; ++t)
+        stream_tile<T, 1, true>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi, s_rp, s_prod);
+    site_write<T, 2>(epi.acc, partB);
+}
+
+// Stand-alone stopping test (DIC forms start their iteration with it, and
+// every solve ends with it at k = cap + 1).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_check(int k, const T* __restrict__ partB, SolverState* st,
+                                                  HostMirror* hm) {
+    if (st->done) return;
+    T tot[2];
+    check_stop<T>(k, partB, st, hm, tot);
+}
+
+// rho = sum a_i*b_i -> site C, for the DIC forms (CG.f90:172-176: (r,z);
+// BiCG.f90:169-173: (z1,r2)); the product is a*b in the reference's order.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rho(int n, const T* __restrict__ a, const T* __restrict__ b,
+                                                T* __restrict__ partC, const SolverState* st) {
+    if (st->done) return;
+    TFLOWS_BLOCK_ROWS(n);
+    T v[2] = {T(0), T(0)};
+    for (int i = rb + threadIdx.x; i < re; i += kBlock) v[0] += a[i] * b[i];
+    site_write<T, 2>(v, partC);
+}
+
+// rho_k of iteration k: from site B (Jacobi forms, after the stopping test)
+// or site C (DIC forms); recorded by the leader in st->rho[k & 1].
+template <typename T, bool JAC>
+__device__ __forceinline__ bool iteration_rho(int k, const T* partB, const T* partC, SolverState* st,
+                                              HostMirror* hm, T& rho) {
+    T tot[2];
+    if (JAC) {
+        if (check_stop<T>(k, partB, st, hm, tot)) return false;
+        rho = tot[1];
+    } else {
+        site_total<T, 2>(partC, tot);
+        rho = tot[0];
+    }
+    if (leader()) st->rho[k & 1] = (double)rho;
+    return true;
+}
+
+// ===========================================================================
+// CG  (Process/CG.f90:113-259)
+// ===========================================================================
+// [stopping test,] rho, p = z + beta*p  (z = r/D, or the DIC output z)
+// CG.f90:172-187
+template <typename T, bool JAC>
+__global__ __launch_bounds__(kBlock) void k_cg_p(int k, int n, const T* __restrict__ r, const T* __restrict__ D,
+                                                 const T* __restrict__ z, T* __restrict__ p,
+                                                 const T* __restrict__ partB, const T* __restrict__ partC,
+                                                 SolverState* st, HostMirror* hm) {
+    if (st->done) return;
+    T rho;
+    if (!iteration_rho<T, JAC>(k, partB, partC, st, hm, rho)) return;
+    const bool first = k == 1;
+    const T beta = first ? T(0) : rho / (T)st->rho[(k - 1) & 1];
+    TFLOWS_BLOCK_ROWS(n);
+    for (int i = rb + threadIdx.x; i < re; i += kBlock) {
+        const T zi = JAC ? jac(r[i], D[i]) : z[i];
+        p[i] = first ? zi : zi + beta * p[i];
+    }
+}
+
+// q = A p (CG.f90:193-199) with (p,q) -> site D (:214-218)
+template <typename T>
+struct DotEpi {
+    const T* __restrict__ w;   // dot partner
+    T* __restrict__ q;         // output (may be null)
+    T acc[1];
+    __device__ void operator()(int row, T (&y)[1]) {
+        if (q) q[row] = y[0];
+        acc[0] += w[row] * y[0];
+    }
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(Csr<T> A, const T* __restrict__ p, T* __restrict__ q,
+                                                    T* __restrict__ partD, const SolverState* st) {
+    if (st->done) return;
+    TFLOWS_STREAM_SMEM(T, 1);
+    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    Gather1<T> g{p};
+    InitZero<T> init;
+    DotEpi<T> epi{p, q, {T(0)}};
+    for (int t = t0; t < t1; ++t)
+        stream_tile<T, 1, false>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi, s_rp, s_prod);
+    site_write<T, 1>(epi.acc, partD);
+}
+
+// alfa = rho/(p,q) (:219); x += alfa p; r -= alfa q (CG.f90:226-229), with
+// ||r||^2 and, for the Jacobi forms, the next rho = (r, r/D) -> site B.
+template <typename T, bool JAC>
+__global__ __launch_bounds__(kBlock) void k_cg_update(int k, int n, T* __restrict__ x, T* __restrict__ r,
+                                                      const T* __restrict__ p, const T* __restrict__ q,
+                                                      const T* __restrict__ D, const T* __restrict__ partD,
+                                                      T* __restrict__ partB, const SolverState* st) {
+    if (st->done) return;
+    T pq[1];
+    site_total<T, 1>(partD, pq);
+    const T alfa = (T)st->rho[k & 1] / pq[0];
+    TFLOWS_BLOCK_ROWS(n);
+    T v[2] = {T(0), T(0)};
+    for (int i = rb + threadIdx.x; i < re; i += kBlock) {
+        x[i] = x[i] + alfa * p[i];
+        const T ri = r[i] - alfa * q[i];
+        r[i] = ri;
+        v[0] += ri * ri;
+        if (JAC) v[1] += ri * jac(ri, D[i]);
+    }
+    site_write<T, 2>(v, partB);
+}
+
+// ===========================================================================
+// BiCG  (Process/BiCG.f90:100-251).  The shadow system is multiplied by A
+// (BiCG.f90:196-200) and starts from r2 = r1, exactly as in the reference.
+// ===========================================================================
+template <typename T, bool JAC>
+__global__ __launch_bounds__(kBlock) void k_bicg_p(int k, int n, const T* __restrict__ r1,
+                                                   const T* __restrict__ r2, const T* __restrict__ D,
+                                                   const T* __restrict__ z1, const T* __restrict__ z2,
+                                                   T* __restrict__ p1, T* __restrict__ p2,
+                                                   const T* __restrict__ partB, const T* __restrict__ partC,
+                                                   SolverState* st, HostMirror* hm) {
+    if (st->done) return;
+    T rho;
+    if (!iteration_rho<T, JAC>(k, partB, partC, st, hm, rho)) return;
+    const bool first = k == 1;
+    const T beta = first ? T(0) : rho / (T)st->rho[(k - 1) & 1];
+    TFLOWS_BLOCK_ROWS(n);
+    for (int i = rb + threadIdx.x; i < re; i += kBlock) {
+        const T a = JAC ? jac(r1[i], D[i]) : z1[i];
+        const T b = JAC ? jac(r2[i], D[i]) : z2[i];
+        p1[i] = first ? a : a + beta * p1[i];
+        p2[i] = first ? b : b + beta * p2[i];
+    }
+}
+
+template <typename T>
+struct BiDotEpi {
+    const T* __restrict__ p2;
+    T* __restrict__ q1;
+    T* __restrict__ q2;
+    T acc[1];
+    __device__ void operator()(int row, T (&y)[2]) {
+        q1[row] = y[0];
+        q2[row] = y[1];
+        acc[0] += p2[row] * y[0];          // (p2, q1)  BiCG.f90:218-222
+    }
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_spmv(Csr<T> A, const T* __restrict__ p1,
+                                                      const T* __restrict__ p2, T* __restrict__ q1,
+                                                      T* __restrict__ q2, T* __restrict__ partD,
+                                                      const SolverState* st) {
+    if (st->done) return;
+    TFLOWS_STREAM_SMEM(T, 2);
+    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    Gather2<T> g{p1, p2};
+    InitZero<T> init;
+    BiDotEpi<T> epi{p2, q1, q2, {T(0)}};
+    for (int t = t0; t < t1; ++t)
+        stream_tile<T, 2, false>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi, s_rp, s_prod);
+    site_write<T, 1>(epi.acc, partD);
+}
+
+template <typename T, bool JAC>
+__global__ __launch_bounds__(kBlock) void k_bicg_update(int k, int n, T* __restrict__ x, T* __restrict__ r1,
+                                                        T* __restrict__ r2, const T* __restrict__ p1,
+                                                        const T* __restrict__ q1, const T* __restrict__ q2,
+                                                        const T* __restrict__ D, const T* __restrict__ partD,
+                                                        T* __restrict__ partB, const SolverState* st) {
+    if (st->done) return;
+    T pq[1];
+    site_total<T, 1>(partD, pq);
+    const T alfa = (T)st->rho[k & 1] / pq[0];        // BiCG.f90:223
+    TFLOWS_BLOCK_ROWS(n);
+    T v[2] = {T(0), T(0)};
+    for (int i = rb + threadIdx.x; i < re; i += kBlock) {
+        x[i] = x[i] + alfa * p1[i];                   // BiCG.f90:230-234
+        const T a = r1[i] - alfa * q1[i];
+        const T b = r2[i] - alfa * q2[i];
+        r1[i] = a;
+        r2[i] = b;
+        v[0] += a * a;
+        if (JAC) v[1] += jac(a, D[i]) * b;            // next rho = (z1, r2)
+    }
+    site_write<T, 2>(v, partB);
+}
+
+// ===========================================================================
+// CGS  (Process/CGS.f90:112-334), fixed shadow r2 = r0.
+// ===========================================================================
+// stopping test; rho = (r1, r2) from site B; u1 = r1 + beta q1;
+// u2 = u1 + beta (q1 + beta u2); p2 = u2/D  (:118-148)
+template <typename T, bool JAC>
+__global__ __launch_bounds__(kBlock) void k_cgs_u(int k, int n, const T* __restrict__ r1,
+                                                  const T* __restrict__ q1, T* __restrict__ u1,
+                                                  T* __restrict__ u2, const T* __restrict__ D,
+                                                  T* __restrict__ p2, const T* __restrict__ partB,
+                                                  SolverState* st, HostMirror* hm) {
+    if (st->done) return;
+    T tot[2];
+    if (check_stop<T>(k, partB, st, hm, tot)) return;
+    const T rho = tot[1];
+    if (leader()) st->rho[k & 1] = (double)rho;
+    const bool first = k == 1;
+    const T beta = first ? T(0) : rho / (T)st->rho[(k - 1) & 1];
+    TFLOWS_BLOCK_ROWS(n);
+    for (int i = rb + threadIdx.x; i < re; i += kBlock) {
+        T a, b;
+        if (first) {
+            a = r1[i];
+            b = a;
+        } else {
+            const T qi = q1[i];
+            a = r1[i] + beta * qi;
+            b = a + beta * (qi + beta * u2[i]);
+        }
+        u1[i] = a;
+        u2[i] = b;
+        if (JAC) p2[i] = jac(b, D[i]);
+    }
+}
+
+// v2 = A p2 (:193-198) with (r2, v2) -> site D (:215-219)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cgs_spmv_v(Csr<T> A, const T* __restrict__ p2,
+                                                       T* __restrict__ v2, const T* __restrict__ r2,
+                                                       T* __restrict__ partD, const SolverState* st) {
+    if (st->done) return;
+    TFLOWS_STREAM_SMEM(T, 1);
+    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    Gather1<T> g{p2};
+    InitZero<T> init;
+    DotEpi<T> epi{r2, v2, {T(0)}};
+    for (int t = t0; t < t1; ++t)
+        stream_tile<T, 1, false>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi, s_rp, s_prod);
+    site_write<T, 1>(epi.acc, partD);
+}
+
+// alfa = rho/(r2,v2) (:220); q1 = u1 - alfa v2 (:226-228);
+// p1 = (u1+q1)/D (:238-241), or w = u1+q1 into p1 for the DIC apply.
+template <typename T, bool JAC>
+__global__ __launch_bounds__(kBlock) void k_cgs_q(int k, int n, const T* __restrict__ u1,
+                                                  const T* __restrict__ v2, T* __restrict__ q1,
+                                                  const T* __restrict__ D, T* __restrict__ p1,
+                                                  const T* __restrict__ partD, const SolverState* st) {
+    if (st->done) return;
+    T rv[1];
+    site_total<T, 1>(partD, rv);
+    const T alfa = (T)st->rho[k & 1] / rv[0];
+    TFLOWS_BLOCK_ROWS(n);
+    for (int i = rb + threadIdx.x; i < re; i += kBlock) {
+        const T ui = u1[i];
+        const T qi = ui - alfa * v2[i];
+        q1[i] = qi;
+        const T w = ui + qi;
+        p1[i] = JAC ? jac(w, D[i]) : w;
+    }
+}
+
+// x += alfa p1 (:286-288); q2 = A p1 (:294-300); r1 -= alfa q2 (:315-317),
+// with ||r1||^2 and the next rho = (r1, r2) (:118-122) -> site B.
+template <typename T>
+struct CgsREpi {
+    T* __restrict__ x;
+    T* __restrict__ r1;
+    const T* __restrict__ r2;
+    const T* __restrict__ p1;
+    T alfa;
+    T acc[2];
+    __device__ void operator()(int row, T (&y)[1]) {
+        x[row] = x[row] + alfa * p1[row];
+        const T ri = r1[row] - alfa * y[0];
+        r1[row] = ri;
+        acc[0] += ri * ri;
+        acc[1] += ri * r2[row];
+    }
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cgs_spmv_r(int k, Csr<T> A, const T* __restrict__ p1,
+                                                       T* __restrict__ x, T* __restrict__ r1,
+                                                       const T* __restrict__ r2, const T* __restrict__ partD,
+                                                       T* __restrict__ partB, const SolverState* st) {
+    if (st->done) return;
+    TFLOWS_STREAM_SMEM(T, 1);
+    T rv[1];
+    site_total<T, 1>(partD, rv);
+    const T alfa = (T)st->rho[k & 1] / rv[0];
+    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    Gather1<T> g{p1};
+    InitZero<T> init;
+    CgsREpi<T> epi{x, r1, r2, p1, alfa, {T(0), T(0)}};
+    for (int t = t0; t < t1; ++t)
+        stream_tile<T, 1, false>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi, s_rp, s_prod);
+    site_write<T, 2>(epi.acc, partB);
+}
+
+// ===========================================================================
+// Stand-alone SpMV (q = A p) and residual (r -= A x), for the prec1_/resid_
+// entry points, tests and the bandwidth benchmark.
+// ===========================================================================
+template <typename T>
+struct StoreEpi {
+    T* __restrict__ y;
+    __device__ void operator()(int row, T (&v)[1]) { y[row] = v[0]; }
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv(Csr<T> A, const T* __restrict__ x, T* __restrict__ y) {
+    TFLOWS_STREAM_SMEM(T, 1);
+    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    Gather1<T> g{x};
+    InitZero<T> init;
+    StoreEpi<T> epi{y};
+    for (int t = t0; t < t1; ++t)
+        stream_tile<T, 1, false>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi, s_rp, s_prod);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_resid(Csr<T> A, const T* __restrict__ x, T* __restrict__ r) {
+    TFLOWS_STREAM_SMEM(T, 1);
+    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    Gather1<T> g{x};
+    InitVec<T> init{r};
+    StoreEpi<T> epi{r};
+    for (int t = t0; t < t1; ++t)
+        stream_tile<T, 1, true>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi, s_rp, s_prod);
+}
+
+// 1-based T-FlowS pattern -> 0-based device copy (TopolM layout).
+__global__ void k_rebase(int count, const int* __restrict__ in, int* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = in[i] - 1;
+}
+
+template <typename TO, typename TI>
+__global__ void k_convert(int64_t count, const TI* __restrict__ in, TO* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = (TO)in[i];
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers (explicitly instantiated for double and float)
+// ---------------------------------------------------------------------------
+#define TF_LAUNCH(k, grid, stream) hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, stream
+
+template <typename T>
+void launch_setup(const Csr<T>& A, int prec, bool write_d, T* D, const T* b, const Sites<T>& S, hipStream_t s) {
+    TF_LAUNCH((k_setup<T>), S.grid, s), A, prec, write_d, D, b, S.A);
+}
+template <typename T>
+void launch_dic_setup_level(const Csr<T>& A, const int* rows, int count, T* D, hipStream_t s) {
+    if (count > 0) TF_LAUNCH((k_dic_setup_level<T>), (count + kBlock - 1) / kBlock, s), A, rows, count, D);
+}
+template <typename T>
+void launch_dic_fwd_level(const Csr<T>& A, const int* rows, int count, const T* D, const T* u, T* z,
+                          const SolverState* st, hipStream_t s) {
+    if (count > 0) TF_LAUNCH((k_dic_fwd_level<T>), (count + kBlock - 1) / kBlock, s), A, rows, count, D, u, z, st);
+}
+template <typename T>
+void launch_dic_bwd_level(const Csr<T>& A, const int* rows, int count, const T* D, T* z,
+                          const SolverState* st, hipStream_t s) {
+    if (count > 0) TF_LAUNCH((k_dic_bwd_level<T>), (count + kBlock - 1) / kBlock, s), A, rows, count, D, z, st);
+}
+template <typename T>
+void launch_resid_init(int mode, const Csr<T>& A, const T* x, T* r, T* r2, const T* D, const Sites<T>& S,
+                       SolverState* st, HostMirror* hm, hipStream_t s) {
+    switch (mode) {
+        case RHO_JAC: TF_LAUNCH((k_resid_init<T, RHO_JAC>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
+        case RHO_BICG: TF_LAUNCH((k_resid_init<T, RHO_BICG>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
+        case RHO_RR: TF_LAUNCH((k_resid_init<T, RHO_RR>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
+        default: TF_LAUNCH((k_resid_init<T, RHO_NONE>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
+    }
+}
+template <typename T>
+void launch_check(int k, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
+    TF_LAUNCH((k_check<T>), S.grid, s), k, S.B, st, hm);
+}
+template <typename T>
+void launch_rho(int n, const T* a, const T* b, const Sites<T>& S, const SolverState* st, hipStream_t s) {
+    TF_LAUNCH((k_rho<T>), S.grid, s), n, a, b, S.C, st);
+}
+template <typename T>
+void launch_cg_p(bool jac, int k, int n, const T* r, const T* D, const T* z, T* p, const Sites<T>& S,
+                 SolverState* st, HostMirror* hm, hipStream_t s) {
+    if (jac) TF_LAUNCH((k_cg_p<T, true>), S.grid, s), k, n, r, D, z, p, S.B, S.C, st, hm);
+    else TF_LAUNCH((k_cg_p<T, false>), S.grid, s), k, n, r, D, z, p, S.B, S.C, st, hm);
+}
+template <typename T>
+void launch_cg_spmv(const Csr<T>& A, const T* p, T* q, const Sites<T>& S, const SolverState* st, hipStream_t s) {
+    TF_LAUNCH((k_cg_spmv<T>), S.grid, s), A, p, q, S.D, st);
+}
+template <typename T>
+void launch_cg_update(bool jac, int k, int n, T* x, T* r, const T* p, const T* q, const T* D, const Sites<T>& S,
+                      const SolverState* st, hipStream_t s) {
+    if (jac) TF_LAUNCH((k_cg_update<T, true>), S.grid, s), k, n, x, r, p, q, D, S.D, S.B, st);
+    else TF_LAUNCH((k_cg_update<T, false>), S.grid, s), k, n, x, r, p, q, D, S.D, S.B, st);
+}
+template <typename T>
+void launch_bicg_p(bool jac, int k, int n, const T* r1, const T* r2, const T* D, const T* z1, const T* z2, T* p1,
+                   T* p2, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
+    if (jac) TF_LAUNCH((k_bicg_p<T, true>), S.grid, s), k, n, r1, r2, D, z1, z2, p1, p2, S.B, S.C, st, hm);
+    else TF_LAUNCH((k_bicg_p<T, false>), S.grid, s), k, n, r1, r2, D, z1, z2, p1, p2, S.B, S.C, st, hm);
+}
+template <typename T>
+void launch_bicg_spmv(const Csr<T>& A, const T* p1, const T* p2, T* q1, T* q2, const Sites<T>& S,
+                      const SolverState* st, hipStream_t s) {
+    TF_LAUNCH((k_bicg_spmv<T>), S.grid, s), A, p1, p2, q1, q2, S.D, st);
+}
+template <typename T>
+void launch_bicg_update(bool jac, int k, int n, T* x, T* r1, T* r2, const T* p1, const T* q1, const T* q2,
+                        const T* D, const Sites<T>& S, const SolverState* st, hipStream_t s) {
+    if (jac) TF_LAUNCH((k_bicg_update<T, true>), S.grid, s), k, n, x, r1, r2, p1, q1, q2, D, S.D, S.B, st);
+    else TF_LAUNCH((k_bicg_update<T, false>), S.grid, s), k, n, x, r1, r2, p1, q1, q2, D, S.D, S.B, st);
+}
+template <typename T>
+void launch_cgs_u(bool jac, int k, int n, const T* r1, const T* q1, T* u1, T* u2, const T* D, T* p2,
+                  const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
+    if (jac) TF_LAUNCH((k_cgs_u<T, true>), S.grid, s), k, n, r1, q1, u1, u2, D, p2, S.B, st, hm);
+    else TF_LAUNCH((k_cgs_u<T, false>), S.grid, s), k, n, r1, q1, u1, u2, D, p2, S.B, st, hm);
+}
+template <typename T>
+void launch_cgs_spmv_v(const Csr<T>& A, const T* p2, T* v2, const T* r2, const Sites<T>& S, const SolverState* st,
+                       hipStream_t s) {
+    TF_LAUNCH((k_cgs_spmv_v<T>), S.grid, s), A, p2, v2, r2, S.D, st);
+}
+template <typename T>
+void launch_cgs_q(bool jac, int k, int n, const T* u1, const T* v2, T* q1, const T* D, T* p1, const Sites<T>& S,
+                  const SolverState* st, hipStream_t s) {
+    if (jac) TF_LAUNCH((k_cgs_q<T, true>), S.grid, s), k, n, u1, v2, q1, D, p1, S.D, st);
+    else TF_LAUNCH((k_cgs_q<T, false>), S.grid, s), k, n, u1, v2, q1, D, p1, S.D, st);
+}
+template <typename T>
+void launch_cgs_spmv_r(int k, const Csr<T>& A, const T* p1, T* x, T* r1, const T* r2, const Sites<T>& S,
+                       const SolverState* st, hipStream_t s) {
+    TF_LAUNCH((k_cgs_spmv_r<T>), S.grid, s), k, A, p1, x, r1, r2, S.D, S.B, st);
+}
+template <typename T>
+void launch_spmv(const Csr<T>& A, const T* x, T* y, int grid, hipStream_t s) {
+    TF_LAUNCH((k_spmv<T>), grid, s), A, x, y);
+}
+template <typename T>
+void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s) {
+    TF_LAUNCH((k_resid<T>), grid, s), A, x, r);
+}
+void launch_rebase(int count, const int* in, int* out, hipStream_t s) {
+    if (count > 0) hipLaunchKernelGGL(k_rebase, dim3((count + 255) / 256), dim3(256), 0, s, count, in, out);
+}
+void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t s) {
+    if (count > 0)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
+                           count, in, out);
+}
+
+#define TF_INSTANTIATE(T)                                                                                        \
+    template void launch_setup<T>(const Csr<T>&, int, bool, T*, const T*, const Sites<T>&, hipStream_t);        \
+    template void launch_dic_setup_level<T>(const Csr<T>&, const int*, int, T*, hipStream_t);                   \
+    template void launch_dic_fwd_level<T>(const Csr<T>&, const int*, int, const T*, const T*, T*,               \
+                                          const SolverState*, hipStream_t);                                      \
+    template void launch_dic_bwd_level<T>(const Csr<T>&, const int*, int, const T*, T*, const SolverState*,      \
+                                          hipStream_t);                                                          \
+    template void launch_resid_init<T>(int, const Csr<T>&, const T*, T*, T*, const T*, const Sites<T>&,         \
+                                       SolverState*, HostMirror*, hipStream_t);                                  \
+    template void launch_check<T>(int, const Sites<T>&, SolverState*, HostMirror*, hipStream_t);                 \
+    template void launch_rho<T>(int, const T*, const T*, const Sites<T>&, const SolverState*, hipStream_t);      \
+    template void launch_cg_p<T>(bool, int, int, const T*, const T*, const T*, T*, const Sites<T>&,             \
+                                 SolverState*, HostMirror*, hipStream_t);                                        \
+    template void launch_cg_spmv<T>(const Csr<T>&, const T*, T*, const Sites<T>&, const SolverState*,           \
+                                    hipStream_t);                                                                \
+    template void launch_cg_update<T>(bool, int, int, T*, T*, const T*, const T*, const T*, const Sites<T>&,    \
+                                      const SolverState*, hipStream_t);                                          \
+    template void launch_bicg_p<T>(bool, int, int, const T*, const T*, const T*, const T*, const T*, T*, T*,    \
+                                   const Sites<T>&, SolverState*, HostMirror*, hipStream_t);                     \
+    template void launch_bicg_spmv<T>(const Csr<T>&, const T*, const T*, T*, T*, const Sites<T>&,               \
+                                      const SolverState*, hipStream_t);                                          \
+    template void launch_bicg_update<T>(bool, int, int, T*, T*, T*, const T*, const T*, const T*, const T*,     \
+                                        const Sites<T>&, const SolverState*, hipStream_t);                       \
+    template void launch_cgs_u<T>(bool, int, int, const T*, const T*, T*, T*, const T*, T*, const Sites<T>&,    \
+                                  SolverState*, HostMirror*, hipStream_t);                                       \
+    template void launch_cgs_spmv_v<T>(const Csr<T>&, const T*, T*, const T*, const Sites<T>&,                  \
+                                       const SolverState*, hipStream_t);                                         \
+    template void launch_cgs_q<T>(bool, int, int, const T*, const T*, T*, const T*, T*, const Sites<T>&,        \
+                                  const SolverState*, hipStream_t);                                              \
+    template void launch_cgs_spmv_r<T>(int, const Csr<T>&, const T*, T*, T*, const T*, const Sites<T>&,         \
+                                       const SolverState*, hipStream_t);                                         \
+    template void launch_spmv<T>(const Csr<T>&, const T*, T*, int, hipStream_t);                               \
+    template void launch_resid<T>(const Csr<T>&, const T*, T*, int, hipStream_t);
+
+TF_INSTANTIATE(double)
+TF_INSTANTIATE(float)
+
+}  // namespace tflows

@@ -1,0 +1,338 @@
+// system.hip -- orchestration of the CG / BiCG / CGS solves on one device
+// matrix: pattern upload, preconditioner set-up, level analysis for the
+// DIC sweeps, and the host loop that keeps the stream full of iterations.
+#include "system.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <thread>
+
+namespace tflows {
+
+
+System::System(int n, int nonz, const int* acol, const int* arow, const int* adia, bool device_ptrs,
+               hipStream_t s)
+    : n_(n), nnz_(nonz - 1), grid_(stream_grid(n)) {
+    if (n <= 0) throw std::invalid_argument("N must be positive");
+    if (nonz < n + 1) throw std::invalid_argument("NONZ must be at least N + 1 (diagonal entries + 1)");
+    rowptr_.ensure(n_ + 1);
+    col_.ensure(nnz_);
+    dia_.ensure(n_);
+    h_rowptr_.resize(n_ + 1);
+    h_col_.resize(nnz_);
+    h_dia_.resize(n_);
+    if (device_ptrs) {
+        TF_CHECK(hipMemcpyAsync(h_rowptr_.data(), acol, sizeof(int) * (n_ + 1), hipMemcpyDeviceToHost, s));
+        TF_CHECK(hipMemcpyAsync(h_col_.data(), arow, sizeof(int) * nnz_, hipMemcpyDeviceToHost, s));
+        TF_CHECK(hipMemcpyAsync(h_dia_.data(), adia, sizeof(int) * n_, hipMemcpyDeviceToHost, s));
+        TF_CHECK(hipStreamSynchronize(s));
+    } else {
+        std::memcpy(h_rowptr_.data(), acol, sizeof(int) * (n_ + 1));
+        std::memcpy(h_col_.data(), arow, sizeof(int) * nnz_);
+        std::memcpy(h_dia_.data(), adia, sizeof(int) * n_);
+    }
+    // validate the layout on the host before any kernel sees it: every
+    // index the kernels dereference must stay in bounds.
+    if (h_rowptr_[0] != 1 || h_rowptr_[n_] != nnz_ + 1)
+        throw std::invalid_argument("Acol must start at 1 and end at NONZ (= nnz + 1)");
+    for (int i = 0; i < n_; ++i) {
+        if (h_rowptr_[i + 1] < h_rowptr_[i]) throw std::invalid_argument("Acol is not non-decreasing");
+        if (h_dia_[i] < h_rowptr_[i] || h_dia_[i] >= h_rowptr_[i + 1])
+            throw std::invalid_argument("Adia(i) must point inside row i");
+    }
+    for (int i = 0; i < n_ + 1; ++i) h_rowptr_[i] -= 1;
+    for (int i = 0; i < n_; ++i) h_dia_[i] -= 1;
+    for (int j = 0; j < nnz_; ++j) {
+        h_col_[j] -= 1;
+        if (h_col_[j] < 0 || h_col_[j] >= n_) throw std::invalid_argument("Arow holds a column outside 1..N");
+    }
+    for (int i = 0; i < n_; ++i)
+        if (h_col_[h_dia_[i]] != i) throw std::invalid_argument("Arow(Adia(i)) must equal i");
+    TF_CHECK(hipMemcpyAsync(rowptr_.p, h_rowptr_.data(), sizeof(int) * (n_ + 1), hipMemcpyHostToDevice, s));
+    TF_CHECK(hipMemcpyAsync(col_.p, h_col_.data(), sizeof(int) * nnz_, hipMemcpyHostToDevice, s));
+    TF_CHECK(hipMemcpyAsync(dia_.p, h_dia_.data(), sizeof(int) * n_, hipMemcpyHostToDevice, s));
+    val_.ensure(nnz_);
+    TF_CHECK(hipMemsetAsync(val_.p, 0, sizeof(double) * nnz_, s));
+    partials_.ensure(Sites<double>::elems(grid_));
+    state_.ensure(1);
+    TF_CHECK(hipMemsetAsync(state_.p, 0, sizeof(SolverState), s));
+    TF_CHECK(hipHostMalloc((void**)&mirror_, sizeof(HostMirror), hipHostMallocMapped | hipHostMallocCoherent));
+    mirror_->done = 0;
+    mirror_->iter = 0;
+    TF_CHECK(hipStreamSynchronize(s));
+}
+
+System::~System() {
+    if (mirror_) (void)hipHostFree(mirror_);
+}
+
+void System::set_values(const double* aval, bool device_ptr, hipStream_t s) {
+    TF_CHECK(hipMemcpyAsync(val_.p, aval, sizeof(double) * nnz_,
+                            device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    val32_valid_ = false;
+    if (!device_ptr) TF_CHECK(hipStreamSynchronize(s));
+}
+
+const float* System::values32(hipStream_t s) {
+    if (!val32_valid_) {
+        val32_.ensure(nnz_);
+        launch_f64_to_f32(nnz_, val_.p, val32_.p, s);
+        TF_CHECK(hipGetLastError());
+        val32_valid_ = true;
+    }
+    return val32_.p;
+}
+
+// Level analysis on the host (once per pattern).  Forward level of row i:
+// 1 + the largest level among its lower entries; backward level: the same
+// over upper entries, from the last row up.  Rows inside one level have no
+// dependency on each other, so a level is one parallel step of a sweep.
+const Levels& System::levels() {
+    if (levels_ready_) return levels_;
+    std::vector<int> lev(n_);
+    auto bucket = [&](int nlev, std::vector<int>& ptr, DevArray<int>& rows) {
+        ptr.assign(nlev + 1, 0);
+        for (int i = 0; i < n_; ++i) ptr[lev[i] + 1]++;
+        for (int l = 0; l < nlev; ++l) ptr[l + 1] += ptr[l];
+        std::vector<int> out(n_), fill(ptr.begin(), ptr.end() - 1);
+        for (int i = 0; i < n_; ++i) out[fill[lev[i]]++] = i;
+        rows.ensure(n_);
+        TF_CHECK(hipMemcpy(rows.p, out.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
+    };
+    int nf = 0;
+    for (int i = 0; i < n_; ++i) {
+        int l = 0;
+        for (int j = h_rowptr_[i]; j < h_dia_[i]; ++j) l = std::max(l, lev[h_col_[j]] + 1);
+        lev[i] = l;
+        nf = std::max(nf, l + 1);
+    }
+    levels_.nfwd = nf;
+    bucket(nf, levels_.fwd_ptr, levels_.fwd_rows);
+    int nb = 0;
+    for (int i = n_ - 1; i >= 0; --i) {
+        int l = 0;
+        for (int j = h_dia_[i] + 1; j < h_rowptr_[i + 1]; ++j) l = std::max(l, lev[h_col_[j]] + 1);
+        lev[i] = l;
+        nb = std::max(nb, l + 1);
+    }
+    // the upper loop above reads lev[] of rows below i, which were all
+    // recomputed first because i runs downwards and upper columns are > i
+    levels_.nbwd = nb;
+    bucket(nb, levels_.bwd_ptr, levels_.bwd_rows);
+    levels_ready_ = true;
+    return levels_;
+}
+
+void System::dic_setup(double* D, hipStream_t s) {
+    const Levels& L = levels();
+    Csr<double> A = csr();
+    for (int l = 0; l < L.nfwd; ++l)
+        launch_dic_setup_level(A, L.fwd_rows.p + L.fwd_ptr[l], L.fwd_ptr[l + 1] - L.fwd_ptr[l], D, s);
+}
+
+template <typename T>
+void dic_apply_levels(const Levels& L, const Csr<T>& A, const T* D, const T* u, T* z, const SolverState* st,
+                      hipStream_t s) {
+    for (int l = 0; l < L.nfwd; ++l)
+        launch_dic_fwd_level(A, L.fwd_rows.p + L.fwd_ptr[l], L.fwd_ptr[l + 1] - L.fwd_ptr[l], D, u, z, st, s);
+    for (int l = 0; l < L.nbwd; ++l)
+        launch_dic_bwd_level(A, L.bwd_rows.p + L.bwd_ptr[l], L.bwd_ptr[l + 1] - L.bwd_ptr[l], D, z, st, s);
+}
+
+template <typename T>
+void System::dic_apply(const Csr<T>& A, const T* D, const T* u, T* z, const SolverState* st, hipStream_t s) {
+    dic_apply_levels(levels(), A, D, u, z, st, s);
+}
+
+void System::prec1(int prec, double* d, hipStream_t s) {
+    // D only; the ||b|| partial sums of k_setup run over a zero slot of the
+    // workspace and are not used.
+    ws_.ensure((size_t)V_COUNT * n_);
+    TF_CHECK(hipMemsetAsync(ws_.p, 0, sizeof(double) * n_, s));
+    if (prec == 2) dic_setup(d, s);
+    launch_setup(csr(), prec, prec != 2, d, (const double*)ws_.p, sites(), s);
+    TF_CHECK(hipGetLastError());
+}
+
+Sites<double> System::sites() { return Sites<double>::carve(partials_.p, grid_); }
+
+void System::spmv(const double* x, double* y, hipStream_t s) {
+    launch_spmv(csr(), x, y, grid_, s);
+    TF_CHECK(hipGetLastError());
+}
+void System::spmv32(const float* x, float* y, hipStream_t s) {
+    launch_spmv(csr32(s), x, y, grid_, s);
+    TF_CHECK(hipGetLastError());
+}
+void System::resid(const double* x, double* r, hipStream_t s) {
+    launch_resid(csr(), x, r, grid_, s);
+    TF_CHECK(hipGetLastError());
+}
+
+// One iteration k of `method`, enqueued (CG.f90:113-259, BiCG.f90:100-251,
+// CGS.f90:112-334).
+template <typename T>
+void enqueue_iteration(int method, bool jac, int k, const Csr<T>& A, const KrylovVectors<T>& w, const Sites<T>& S,
+                       SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s) {
+    const int n = A.n;
+    switch (method) {
+        case 0: {  // CG
+            if (!jac) {  // stopping test, z = M^-1 r into q, rho = (r, z)
+                launch_check(k, S, st, hm, s);
+                dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.r1, w.q, st, s);
+                launch_rho(n, (const T*)w.r1, (const T*)w.q, S, st, s);
+            }
+            launch_cg_p(jac, k, n, (const T*)w.r1, (const T*)w.D, (const T*)w.q, w.p, S, st, hm, s);
+            launch_cg_spmv(A, (const T*)w.p, w.q, S, st, s);
+            launch_cg_update(jac, k, n, w.x, w.r1, (const T*)w.p, (const T*)w.q, (const T*)w.D, S, st, s);
+            break;
+        }
+        case 1: {  // BiCG (reference semantics)
+            if (!jac) {
+                launch_check(k, S, st, hm, s);
+                dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.r1, w.q, st, s);
+                dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.r2, w.q2, st, s);
+                launch_rho(n, (const T*)w.q, (const T*)w.r2, S, st, s);
+            }
+            launch_bicg_p(jac, k, n, (const T*)w.r1, (const T*)w.r2, (const T*)w.D, (const T*)w.q, (const T*)w.q2,
+                          w.p, w.p2, S, st, hm, s);
+            launch_bicg_spmv(A, (const T*)w.p, (const T*)w.p2, w.q, w.q2, S, st, s);
+            launch_bicg_update(jac, k, n, w.x, w.r1, w.r2, (const T*)w.p, (const T*)w.q, (const T*)w.q2,
+                               (const T*)w.D, S, st, s);
+            break;
+        }
+        default: {  // CGS: u1, u2, p2 = M^-1 u2, v2 = A p2, q1 (q), p1 (p) = M^-1 (u1+q1)
+            launch_cgs_u(jac, k, n, (const T*)w.r1, (const T*)w.q, w.u1, w.u2, (const T*)w.D, w.p2, S, st, hm, s);
+            if (!jac) dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.u2, w.p2, st, s);
+            launch_cgs_spmv_v(A, (const T*)w.p2, w.v2, (const T*)w.r2, S, st, s);
+            launch_cgs_q(jac, k, n, (const T*)w.u1, (const T*)w.v2, w.q, (const T*)w.D, w.p, S, st, s);
+            if (!jac) dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.p, w.p, st, s);
+            launch_cgs_spmv_r(k, A, (const T*)w.p, w.x, w.r1, (const T*)w.r2, S, st, s);
+            break;
+        }
+    }
+}
+
+// Keep the stream supplied with iterations while the device runs them.
+// The device publishes its progress into coherent host memory; the host
+// stays at most `lookahead` iterations ahead and stops once `done` shows
+// (kernels queued past convergence return immediately).  After iteration
+// `cap` a last stopping test (k = cap + 1) closes the loop.  If the mirror
+// stops moving while the stream is idle, the device state is read
+// directly, so the loop always ends.
+template <typename T>
+void pace_iterations(int method, bool jac, int cap, const Csr<T>& A, const KrylovVectors<T>& w, const Sites<T>& S,
+                     SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s) {
+    const int per_iter = jac ? 4 : 6 + 2 * (L->nfwd + L->nbwd) * (method == 0 ? 1 : 2);
+    const int lookahead = std::max(2, std::min(64, 4096 / std::max(1, per_iter)));
+    int issued = 0, completed_floor = 0;
+    while (issued < cap) {
+        if (hm->done) return;
+        const int completed = std::max((int)hm->iter, completed_floor);
+        if (issued - completed < lookahead) {
+            const int k = std::min(std::max(1, lookahead / 2), cap - issued);
+            for (int i = 0; i < k; ++i) {
+                ++issued;
+                enqueue_iteration<T>(method, jac, issued, A, w, S, st, hm, L, s);
+            }
+            TF_CHECK(hipGetLastError());
+            continue;
+        }
+        if (hipStreamQuery(s) == hipSuccess) {
+            SolverState h;
+            TF_CHECK(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
+            if (h.done) return;
+            completed_floor = issued;  // idle stream: everything issued has run
+            continue;
+        }
+        std::this_thread::yield();
+    }
+    launch_check(std::max(cap, 0) + 1, S, st, hm, s);
+    TF_CHECK(hipGetLastError());
+}
+
+SolveResult System::solve(int method, int prec, double* x, double* r1, int cap, double tol, double stale_error,
+                          hipStream_t s) {
+    if (method < 0 || method > 2) throw std::invalid_argument("method must be 0 (cg), 1 (bicg) or 2 (cgs)");
+    ws_.ensure((size_t)V_COUNT * n_);
+    KrylovVectors<double> w = vectors(x, r1);
+    // iteration 1 overwrites p, p2, u1, u2 before reading them; zero the
+    // workspace anyway so no value of a previous solve lingers
+    TF_CHECK(hipMemsetAsync(ws_.p, 0, sizeof(double) * (size_t)V_COUNT * n_, s));
+
+    SolverState h{};
+    h.tol = tol;
+    h.cap = cap;
+    h.error = stale_error;
+    TF_CHECK(hipMemcpyAsync(state_.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    mirror_->done = 0;
+    mirror_->iter = 0;
+
+    const bool jac = prec != 2;
+    Csr<double> A = csr();
+    Sites<double> S = sites();
+    if (prec == 2) dic_setup(w.D, s);
+    launch_setup(A, prec, jac, w.D, (const double*)r1, S, s);
+    int mode = RHO_NONE_;
+    double* r2 = nullptr;
+    if (method == 0) mode = jac ? RHO_JAC_ : RHO_NONE_;
+    if (method == 1) { mode = jac ? RHO_BICG_ : RHO_NONE_; r2 = w.r2; }
+    if (method == 2) { mode = RHO_RR_; r2 = w.r2; }
+    launch_resid_init(mode, A, (const double*)x, r1, r2, (const double*)w.D, S, state_.p, mirror_, s);
+    TF_CHECK(hipGetLastError());
+    const Levels* L = jac ? nullptr : &levels();
+    static const Levels kNoLevels;
+    pace_iterations<double>(method, jac, cap, A, w, S, state_.p, mirror_, L ? L : &kNoLevels, s);
+    TF_CHECK(hipStreamSynchronize(s));
+    SolverState h2;
+    TF_CHECK(hipMemcpy(&h2, state_.p, sizeof(h2), hipMemcpyDeviceToHost));
+    if (!h2.done) throw HipError("solver finished without reaching a stopping state");
+    SolveResult res;
+    res.niter = h2.niter_out;
+    res.ini_written = h2.ini_written != 0;
+    res.ini_res = h2.ini_res;
+    res.fin_res = h2.error;
+    return res;
+}
+
+KrylovVectors<double> System::vectors(double* x, double* r1) {
+    KrylovVectors<double> w;
+    double* b = ws_.p;
+    auto vec = [&](int k) { return b + (size_t)k * n_; };
+    w.x = x;
+    w.r1 = r1;
+    w.D = vec(V_D);
+    w.p = vec(V_P);
+    w.q = vec(V_Q);
+    w.r2 = vec(V_R2);
+    w.p2 = vec(V_P2);
+    w.q2 = vec(V_Q2);
+    w.u1 = vec(V_U1);
+    w.u2 = vec(V_U2);
+    w.v2 = vec(V_V2);
+    return w;
+}
+
+template void enqueue_iteration<double>(int, bool, int, const Csr<double>&, const KrylovVectors<double>&,
+                                        const Sites<double>&, SolverState*, HostMirror*, const Levels*,
+                                        hipStream_t);
+template void enqueue_iteration<float>(int, bool, int, const Csr<float>&, const KrylovVectors<float>&,
+                                       const Sites<float>&, SolverState*, HostMirror*, const Levels*,
+                                       hipStream_t);
+template void pace_iterations<double>(int, bool, int, const Csr<double>&, const KrylovVectors<double>&,
+                                      const Sites<double>&, SolverState*, HostMirror*, const Levels*,
+                                      hipStream_t);
+template void pace_iterations<float>(int, bool, int, const Csr<float>&, const KrylovVectors<float>&,
+                                     const Sites<float>&, SolverState*, HostMirror*, const Levels*,
+                                     hipStream_t);
+template void dic_apply_levels<double>(const Levels&, const Csr<double>&, const double*, const double*,
+                                       double*, const SolverState*, hipStream_t);
+template void dic_apply_levels<float>(const Levels&, const Csr<float>&, const float*, const float*, float*,
+                                      const SolverState*, hipStream_t);
+template void System::dic_apply<double>(const Csr<double>&, const double*, const double*, double*,
+                                        const SolverState*, hipStream_t);
+template void System::dic_apply<float>(const Csr<float>&, const float*, const float*, float*,
+                                       const SolverState*, hipStream_t);
+
+}  // namespace tflows

@@ -1,0 +1,253 @@
+"""Host-side mirror of the reference solver interface over libtflows_krylov.so.
+
+The reference subroutines (Process/CG.f90:2-3, BiCG.f90:2-3, CGS.f90:2-3,
+Prec1.f90:2, Resid.f90:2) are called here with the same names, argument
+order and meaning:
+
+    niter, IniRes, FinRes = cg(N, NB, NONZ, A, Acol, Arow, Adia, Ab, x, r1,
+                               prec, niter, tol)
+
+``x`` (length NB+N+1, Fortran x(-NB:N)) and ``r1`` (length N) are numpy
+arrays updated in place exactly as the Fortran dummies are; the three
+scalar outputs are returned.  ``IniRes`` is ``None`` when the solver left it
+unwritten (the ||b|| < tol early exit, Process/CG.f90:69-72).
+
+The computation always runs on the GPU through the C ABI; if the shared
+library is missing or no GPU is present the calls raise -- there is no CPU
+fallback.
+
+:class:`DeviceSystem` wraps the device-resident API for callers that keep
+the matrix and vectors in HBM (torch tensors on ``cuda``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libtflows_krylov.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "tflows_krylov.h")
+
+METHODS = {"cg": 0, "bicg": 1, "cgs": 2}
+
+_ip = C.POINTER(C.c_int32)
+_dp = C.POINTER(C.c_double)
+_i32 = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f64 = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_SENTINEL = -1.2345e300
+
+_lib = None
+
+
+class TFlowsError(RuntimeError):
+    pass
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libtflows_krylov.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise TFlowsError(f"{path} is not built; run __graft_entry__.build()")
+    # torch bundles its own libamdhip64.so.7 with the same SONAME as the
+    # system one; whichever loads first serves the whole process.  Load
+    # torch's first so that torch tensors and this library share one HIP
+    # runtime (the other order leaves torch without a usable device).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = C.CDLL(path)
+    for name in ("cg_", "bicg_", "cgs_"):
+        f = getattr(lib, name)
+        f.restype = None
+        f.argtypes = [_ip, _ip, _ip, _f64, _i32, _i32, _i32, _f64, _f64, _f64, _ip, _ip, _dp, _dp, _dp]
+    lib.prec1_.restype = None
+    lib.prec1_.argtypes = [_ip, _ip, _f64, _i32, _i32, _i32, _f64, _ip]
+    lib.resid_.restype = None
+    lib.resid_.argtypes = [_ip, _ip, _ip, _f64, _i32, _i32, _f64, _f64, _f64]
+    lib.tflows_last_error.restype = C.c_char_p
+    lib.tflows_version.restype = C.c_char_p
+    vp = C.c_void_p
+    lib.tflows_system_create.restype = C.c_int
+    lib.tflows_system_create.argtypes = [C.c_int32, C.c_int32, vp, vp, vp, C.c_int32, vp, C.POINTER(vp)]
+    lib.tflows_system_destroy.argtypes = [vp]
+    lib.tflows_system_set_values.argtypes = [vp, vp, C.c_int32, vp]
+    lib.tflows_solve.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, _ip, C.c_double, _dp, _dp, vp]
+    lib.tflows_spmv.argtypes = [vp, vp, vp, vp]
+    lib.tflows_spmv32.argtypes = [vp, vp, vp, vp]
+    lib.tflows_prec1.argtypes = [vp, C.c_int32, vp, vp]
+    lib.tflows_resid.argtypes = [vp, vp, vp, vp]
+    lib.tflows_system_levels.restype = C.c_int32
+    lib.tflows_system_levels.argtypes = [vp]
+    _lib = lib
+    _bind_extensions(lib)
+    return lib
+
+
+def _bind_extensions(lib):
+    """Optional entry points of the mixed-precision, batched and adjoint
+    components (bound when present in the library)."""
+    vp = C.c_void_p
+    if hasattr(lib, "tflows_solve_mixed"):
+        f = lib.tflows_solve_mixed
+        f.restype = C.c_int
+        f.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, _ip, C.c_double, C.c_double, C.c_int32,
+                      _dp, _dp, vp, vp]
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise TFlowsError(load_library().tflows_last_error().decode())
+
+
+def declared_symbols(header: str = HEADER) -> list[str]:
+    """Function names declared in include/tflows_krylov.h."""
+    import re
+    txt = open(header).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt)) -
+                  {"if", "defined", "extern", "sizeof"})
+
+
+# --------------------------------------------------------------------------
+# drop-in interface (host arrays, reference argument lists)
+# --------------------------------------------------------------------------
+def _solve(name, N, NB, NONZ, A, Acol, Arow, Adia, Ab, x, r1, prec, niter, tol):
+    lib = load_library()
+    for arr, dt in ((A, np.float64), (x, np.float64), (r1, np.float64), (Acol, np.int32), (Arow, np.int32),
+                    (Adia, np.int32)):
+        if arr.dtype != dt or not arr.flags.c_contiguous:
+            raise TypeError("arrays must be C-contiguous float64 / int32")
+    if len(x) != NB + N + 1 or len(r1) != N or len(Acol) != N + 1 or len(A) < NONZ:
+        raise ValueError("array lengths do not match N, NB, NONZ")
+    Ab = Ab if Ab is not None and len(Ab) else np.zeros(1)
+    it = C.c_int32(niter)
+    ini, fin = C.c_double(_SENTINEL), C.c_double(0.0)
+    getattr(lib, name)(C.byref(C.c_int32(N)), C.byref(C.c_int32(NB)), C.byref(C.c_int32(NONZ)), A, Acol, Arow,
+                       Adia, np.ascontiguousarray(Ab, np.float64), x, r1, C.byref(C.c_int32(prec)), C.byref(it),
+                       C.byref(C.c_double(tol)), C.byref(ini), C.byref(fin))
+    return it.value, (None if ini.value == _SENTINEL else ini.value), fin.value
+
+
+def cg(N, NB, NONZ, A, Acol, Arow, Adia, Ab, x, r1, prec, niter, tol):
+    """GPU replacement of SUBROUTINE cg (Process/CG.f90:2)."""
+    return _solve("cg_", N, NB, NONZ, A, Acol, Arow, Adia, Ab, x, r1, prec, niter, tol)
+
+
+def bicg(N, NB, NONZ, A, Acol, Arow, Adia, Ab, x, r1, prec, niter, tol):
+    """GPU replacement of SUBROUTINE bicg (Process/BiCG.f90:2)."""
+    return _solve("bicg_", N, NB, NONZ, A, Acol, Arow, Adia, Ab, x, r1, prec, niter, tol)
+
+
+def cgs(N, NB, NONZ, A, Acol, Arow, Adia, Ab, x, r1, prec, niter, tol):
+    """GPU replacement of SUBROUTINE CGS (Process/CGS.f90:2)."""
+    return _solve("cgs_", N, NB, NONZ, A, Acol, Arow, Adia, Ab, x, r1, prec, niter, tol)
+
+
+def prec1(N, NONZ, A, Acol, Arow, Adia, prec):
+    """GPU replacement of SUBROUTINE Prec1 (Process/Prec1.f90:2); returns D."""
+    lib = load_library()
+    D = np.zeros(N)
+    lib.prec1_(C.byref(C.c_int32(N)), C.byref(C.c_int32(NONZ)), A, Acol, Arow, Adia, D, C.byref(C.c_int32(prec)))
+    return D
+
+
+def resid(N, NB, NONZ, A, Acol, Arow, Ab, x, r1):
+    """GPU replacement of SUBROUTINE Resid (Process/Resid.f90:2); r1 in place."""
+    lib = load_library()
+    Ab = Ab if Ab is not None and len(Ab) else np.zeros(1)
+    lib.resid_(C.byref(C.c_int32(N)), C.byref(C.c_int32(NB)), C.byref(C.c_int32(NONZ)), A, Acol, Arow,
+               np.ascontiguousarray(Ab, np.float64), x, r1)
+
+
+def solve_system(method: str, m, b, x0=None, prec=1, niter=2000, tol=0.0, nb=0):
+    """Convenience wrapper over the drop-in for a meshgen.TFlowSMatrix."""
+    x = np.zeros(nb + m.n + 1)
+    if x0 is not None:
+        x[nb + 1:] = x0
+    r1 = np.array(b, dtype=np.float64, copy=True)
+    it, ini, fin = _solve(method + "_", m.n, nb, m.nonz, m.aval, m.acol, m.arow, m.adia, np.zeros(max(nb, 1)),
+                          x, r1, prec, niter, tol)
+    return dict(x=x[nb + 1:].copy(), r=r1, niter=it, ini_res=ini, fin_res=fin)
+
+
+# --------------------------------------------------------------------------
+# device-resident interface (torch tensors in HBM)
+# --------------------------------------------------------------------------
+def _ptr(t) -> int:
+    return int(t.data_ptr())
+
+
+def _stream(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(int(s.cuda_stream))
+
+
+class DeviceSystem:
+    """A matrix resident on the GPU (pattern + FP64 values), solved in place
+    on device vectors.  ``matrix`` is a meshgen.TFlowSMatrix (host arrays)."""
+
+    def __init__(self, matrix, stream=None):
+        import torch
+        self.lib = load_library()
+        self.n, self.nonz = matrix.n, matrix.nonz
+        self.matrix = matrix
+        h = C.c_void_p()
+        dev = torch.device("cuda")
+        self._acol = torch.from_numpy(matrix.acol).to(dev)
+        self._arow = torch.from_numpy(matrix.arow).to(dev)
+        self._adia = torch.from_numpy(matrix.adia).to(dev)
+        self.values = torch.from_numpy(matrix.aval).to(dev)
+        torch.cuda.synchronize()
+        _check(self.lib.tflows_system_create(self.n, self.nonz, _ptr(self._acol), _ptr(self._arow),
+                                             _ptr(self._adia), 1, _stream(stream), C.byref(h)))
+        self.handle = h
+        self.set_values(self.values, stream)
+
+    def set_values(self, aval, stream=None):
+        """aval: cuda float64 tensor of length NONZ (or nnz)."""
+        _check(self.lib.tflows_system_set_values(self.handle, C.c_void_p(_ptr(aval)), 1, _stream(stream)))
+
+    def solve(self, method: str, prec: int, x, r1, niter: int, tol: float, stream=None):
+        """x, r1: cuda float64 tensors of length N, updated in place."""
+        it = C.c_int32(niter)
+        ini, fin = C.c_double(_SENTINEL), C.c_double(0.0)
+        _check(self.lib.tflows_solve(self.handle, METHODS[method], prec, C.c_void_p(_ptr(x)),
+                                     C.c_void_p(_ptr(r1)), C.byref(it), tol, C.byref(ini), C.byref(fin),
+                                     _stream(stream)))
+        return it.value, (None if ini.value == _SENTINEL else ini.value), fin.value
+
+    def spmv(self, x, y, stream=None):
+        _check(self.lib.tflows_spmv(self.handle, C.c_void_p(_ptr(x)), C.c_void_p(_ptr(y)), _stream(stream)))
+
+    def spmv32(self, x, y, stream=None):
+        _check(self.lib.tflows_spmv32(self.handle, C.c_void_p(_ptr(x)), C.c_void_p(_ptr(y)), _stream(stream)))
+
+    def prec1(self, prec: int, d, stream=None):
+        _check(self.lib.tflows_prec1(self.handle, prec, C.c_void_p(_ptr(d)), _stream(stream)))
+
+    def resid(self, x, r, stream=None):
+        _check(self.lib.tflows_resid(self.handle, C.c_void_p(_ptr(x)), C.c_void_p(_ptr(r)), _stream(stream)))
+
+    def levels(self) -> int:
+        return int(self.lib.tflows_system_levels(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None) is not None:
+            self.lib.tflows_system_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,192 @@
+"""GPU parity of the drop-in solvers against the CPU oracle.
+
+The oracle (oracle/krylov_oracle.c) is bitwise equal to the reference
+Fortran solvers (tests/test_oracle.py pins it).  The GPU path differs from it
+only in the order of the dot-product sums (tree instead of sequential), so
+the bar is: x within 1e-10 relative L2 of the oracle's, the same stopping
+state, and niter equal to within one iteration.  Kernels that have no
+reduction (SpMV, Resid, Prec1) must be bitwise equal.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+X_TOL = 1e-10          # relative L2 of x against the oracle
+
+
+def _cases(pkg):
+    mg = pkg.meshgen
+    return {
+        "pressure12": mg.pressure_system(12, seed=11),
+        "pressure9_perm": mg.pressure_system(9, seed=12, perm_seed=7),
+        "momentum10": mg.momentum_system(10, seed=13),
+        "pressure_aniso": mg.pressure_system((20, 6, 3), seed=14, ratio=1.3),
+    }
+
+
+@pytest.fixture(scope="module")
+def cases(pkg):
+    return _cases(pkg)
+
+
+def _rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def _inv_norm2(m):
+    """||A^-1||_2 of a small matrix (dense SVD)."""
+    A = m.to_scipy().toarray()
+    return 1.0 / np.linalg.svd(A, compute_uv=False)[-1]
+
+
+CASES = ["pressure12", "pressure9_perm", "momentum10", "pressure_aniso"]
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("method", ["cg", "bicg", "cgs"])
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_trajectory_matches_oracle(pkg, oracle, cases, case, method, prec):
+    """After a few iterations GPU and oracle iterates agree to rounding: the
+    GPU runs the same recurrences (same preconditioner, same update order)."""
+    s = cases[case]
+    cap = 6
+    ref = oracle.run(method, s.matrix, s.b, prec=prec, niter=cap, tol=s.tol)
+    got = pkg.krylov.solve_system(method, s.matrix, s.b, prec=prec, niter=cap, tol=s.tol)
+    assert got["niter"] == ref["niter"] == cap + 1
+    assert got["ini_res"] == pytest.approx(ref["ini_res"], rel=1e-13)
+    # CGS squares the residual polynomial, which amplifies the rounding
+    # difference of the dot products more than CG does
+    tol_x = 1e-9 if method == "cgs" else 1e-11
+    assert _rel(got["x"], ref["x"]) <= tol_x
+    assert _rel(got["r"], ref["r"]) <= 1e3 * tol_x
+    assert got["fin_res"] == pytest.approx(ref["fin_res"], rel=1e3 * tol_x)
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("method", ["cg", "bicg", "cgs"])
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_dropin_matches_oracle(pkg, oracle, cases, case, method, prec):
+    """Full solves: same stopping state, niter within one, and x within
+    1e-10 relative L2 of the oracle -- or, where the conditioning makes the
+    set of x with ||b - A x|| < tol wider than that, within the rigorous
+    bound ||x_gpu - x_ref|| <= ||A^-1|| (||r_gpu|| + ||r_ref||)."""
+    s = cases[case]
+    m = s.matrix
+    # CG / BiCG stall on the nonsymmetric momentum matrix (as in the
+    # reference); compare a short trajectory there, full solves elsewhere.
+    cap = 25 if (case == "momentum10" and method != "cgs") else 1000
+    ref = oracle.run(method, m, s.b, prec=prec, niter=cap, tol=s.tol)
+    got = pkg.krylov.solve_system(method, m, s.b, prec=prec, niter=cap, tol=s.tol)
+    rel = _rel(got["x"], ref["x"])
+    print(f"{case} {method} prec={prec} niter gpu={got['niter']} oracle={ref['niter']} "
+          f"FinRes gpu={got['fin_res']:.3e} oracle={ref['fin_res']:.3e} rel_x={rel:.2e}", flush=True)
+    assert abs(got["niter"] - ref["niter"]) <= 1
+    assert (got["fin_res"] < s.tol) == (ref["fin_res"] < s.tol)
+    assert got["ini_res"] == pytest.approx(ref["ini_res"], rel=1e-12)
+    # r1 leaves as the final residual: the recurrence residual whose norm is FinRes
+    assert np.linalg.norm(got["r"]) == pytest.approx(got["fin_res"], rel=1e-12)
+    if rel > X_TOL:
+        A = m.to_scipy()
+        rg = np.linalg.norm(s.b - A @ got["x"])
+        rr = np.linalg.norm(s.b - A @ ref["x"])
+        bound = _inv_norm2(m) * (rg + rr) / np.linalg.norm(ref["x"])
+        assert ref["fin_res"] < s.tol, "non-converged trajectories must agree to 1e-10"
+        assert rel <= bound, (rel, bound)
+
+
+def test_bicg_equals_cg_bitwise_on_gpu(pkg, cases):
+    # the reference's BiCG multiplies its shadow by A and starts it at r,
+    # so it computes exactly CG (BiCG.f90:93-95, 196-200)
+    s = cases["momentum10"]
+    for prec in (1, 2):
+        a = pkg.krylov.solve_system("cg", s.matrix, s.b, prec=prec, niter=30, tol=s.tol)
+        b = pkg.krylov.solve_system("bicg", s.matrix, s.b, prec=prec, niter=30, tol=s.tol)
+        assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["r"], b["r"])
+        assert a["niter"] == b["niter"] and a["fin_res"] == b["fin_res"]
+
+
+def test_config1_pressure_32cube(pkg, oracle):
+    """The 32^3 channel pressure system, CG + diagonal preconditioning."""
+    s = pkg.meshgen.pressure_system(32, seed=1)
+    ref = oracle.run("cg", s.matrix, s.b, prec=1, niter=2000, tol=s.tol)
+    got = pkg.krylov.solve_system("cg", s.matrix, s.b, prec=1, niter=2000, tol=s.tol)
+    print(f"config1 niter gpu={got['niter']} oracle={ref['niter']}", flush=True)
+    assert got["fin_res"] < s.tol and ref["fin_res"] < s.tol
+    assert abs(got["niter"] - ref["niter"]) <= 1
+    assert _rel(got["x"], ref["x"]) <= X_TOL
+    A = s.matrix.to_scipy()
+    assert np.linalg.norm(s.b - A @ got["x"]) <= 1.01e-8 * np.linalg.norm(s.b)
+
+
+def test_spmv_resid_prec1_bitwise(pkg, oracle, cases, cuda):
+    import torch
+    for name, s in cases.items():
+        m = s.matrix
+        rng = np.random.default_rng(5)
+        p = rng.standard_normal(m.n)
+        ds = pkg.krylov.DeviceSystem(m)
+        q = torch.zeros(m.n, dtype=torch.float64, device=cuda)
+        ds.spmv(torch.from_numpy(p).to(cuda), q)
+        assert np.array_equal(q.cpu().numpy(), oracle.spmv(m, p)), name
+        # Resid: r1 <- r1 - A x, subtraction in column order
+        x = np.zeros(m.n + 1)
+        x[1:] = p
+        r_ref = s.b.copy()
+        oracle.resid(m.n, 0, m.nonz, m.aval, m.acol, m.arow, np.zeros(1), x, r_ref)
+        r_gpu = s.b.copy()
+        pkg.krylov.resid(m.n, 0, m.nonz, m.aval, m.acol, m.arow, np.zeros(1), x, r_gpu)
+        assert np.array_equal(r_gpu, r_ref), name
+        for prec in (0, 1, 2):
+            d_ref = oracle.prec1(m.n, m.nonz, m.aval, m.acol, m.arow, m.adia, prec)
+            d_gpu = pkg.krylov.prec1(m.n, m.nonz, m.aval, m.acol, m.arow, m.adia, prec)
+            assert np.array_equal(d_gpu, d_ref), (name, prec)
+        ds.close()
+
+
+def test_semantics_early_exit_and_cap(pkg, oracle, cases):
+    s = cases["pressure12"]
+    m = s.matrix
+    # non-convergence: niter = cap + 1 (Fortran DO-loop exit)
+    got = pkg.krylov.solve_system("cg", m, s.b, prec=1, niter=3, tol=s.tol)
+    assert got["niter"] == 4 and got["fin_res"] >= s.tol
+    got0 = pkg.krylov.solve_system("cgs", m, s.b, prec=1, niter=0, tol=s.tol)
+    assert got0["niter"] == 1 and got0["fin_res"] == pytest.approx(got0["ini_res"])
+    # early exit when ||b|| < tol: niter = 0, IniRes untouched, FinRes is the
+    # previous solve's final residual (module variable `error`)
+    prev = pkg.krylov.solve_system("cg", m, s.b, prec=1, niter=5, tol=s.tol)
+    tiny = s.b * 1e-30
+    e = pkg.krylov.solve_system("bicg", m, tiny, prec=1, niter=50, tol=s.tol)
+    assert e["niter"] == 0 and e["ini_res"] is None and e["fin_res"] == prev["fin_res"]
+    assert np.array_equal(e["x"], np.zeros(m.n)) and np.array_equal(e["r"], tiny)
+
+
+def test_initial_guess_and_buffer_offset(pkg, oracle, cases):
+    """x(-NB:N) with NB > 0: only x(1:N) is read and written."""
+    s = cases["pressure9_perm"]
+    m = s.matrix
+    nb = 17
+    x0 = np.random.default_rng(3).standard_normal(m.n)
+    for method in ("cg", "cgs"):
+        ref = oracle.run(method, m, s.b, x0=x0, prec=1, niter=500, tol=s.tol, nb=nb)
+        x = np.full(nb + m.n + 1, 7.0)
+        x[nb + 1:] = x0
+        r1 = s.b.copy()
+        it, ini, fin = getattr(pkg.krylov, method)(m.n, nb, m.nonz, m.aval, m.acol, m.arow, m.adia,
+                                                   np.zeros(nb), x, r1, 1, 500, s.tol)
+        assert np.all(x[: nb + 1] == 7.0)
+        assert abs(it - ref["niter"]) <= 1 and _rel(x[nb + 1:], ref["x"]) <= X_TOL
+
+
+def test_device_api_matches_dropin(pkg, cases, cuda):
+    import torch
+    s = cases["pressure12"]
+    ds = pkg.krylov.DeviceSystem(s.matrix)
+    for method in ("cg", "bicg", "cgs"):
+        x = torch.zeros(s.matrix.n, dtype=torch.float64, device=cuda)
+        r = torch.from_numpy(s.b.copy()).to(cuda)
+        it, ini, fin = ds.solve(method, 1, x, r, 1000, s.tol)
+        d = pkg.krylov.solve_system(method, s.matrix, s.b, prec=1, niter=1000, tol=s.tol)
+        assert it == d["niter"] and np.array_equal(x.cpu().numpy(), d["x"])
+    ds.close()
