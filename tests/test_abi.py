@@ -4,7 +4,6 @@ import ctypes
 import os
 import subprocess
 
-import pytest
 
 
 def test_library_exports_declared_symbols(pkg):
@@ -33,8 +32,6 @@ def test_dropin_symbols_use_fortran_mangling(pkg):
 
 
 def test_code_object_targets_gfx950(pkg):
-    out = subprocess.run(["/opt/rocm/llvm/bin/clang-offload-bundler", "--list", "--type=o",
-                          f"--input={pkg.krylov.LIB_PATH}"], capture_output=True, text=True)
-    if out.returncode != 0:
-        pytest.skip("clang-offload-bundler cannot list this file")
-    assert "gfx950" in out.stdout
+    # the offload bundle embedded in .hip_fatbin names its target triple
+    data = open(pkg.krylov.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in data
