@@ -7,7 +7,10 @@ box, tanh wall stretching ratio 1.1, cell 1 pinned, b ~ U(-1,1) with its mean
 removed, x0 = 0), solved with
 
   * plain FP64 CG + diagonal preconditioning (the reference algorithm), and
-  * FP32-inner CG inside FP64 iterative refinement (when built),
+  * the same CG in FP32 (difference-form SpMV, Jacobi, vector updates and
+    dot products in FP32) with an FP64 refinement step -- x += fp64(x32),
+    true residual b - A x in FP64, residual replacement -- every 50
+    iterations and at convergence,
 
 both to ||b - A x||_2 <= 1e-8 ||b||_2, checked in FP64 after the solve.
 
@@ -41,7 +44,13 @@ def log(msg):
 
 
 def spmv_bytes(n, nnz, word):
-    return (word + 4) * nnz + 4 * (n + 1) + 2 * word * n
+    """Algorithmic HBM bytes of one SpMV launch, every array touched once.
+    FP64 (reference CSR form): 12*nnz + 4*(N+1) + 16*N.
+    FP32 (difference form used by the mixed solver): off-diagonal col+val
+    8*(nnz-N), off-diagonal row pointer 4*(N+1), row sums, x and y 4*N each."""
+    if word == 8:
+        return 12 * nnz + 4 * (n + 1) + 16 * n
+    return 8 * (nnz - n) + 4 * (n + 1) + 12 * n
 
 
 def time_spmv(torch, ds, n, reps=50, fp32=False):
@@ -92,6 +101,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128, help="cells per direction")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--interval", type=int, default=50,
+                    help="FP32 CG iterations between FP64 residual replacements")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -126,11 +137,11 @@ def main():
         xh = x.cpu().numpy()
         return float(np.linalg.norm(s.b - A_sp @ xh) / bnorm)
 
-    have_mixed = hasattr(ds, "solve_mixed") and ds.lib is not None and hasattr(ds.lib, "tflows_solve_mixed")
+    have_mixed = hasattr(ds.lib, "tflows_solve_mixed_reliable")
 
     def run_mixed():
         x.zero_()
-        return ds.solve_mixed("cg", 1, x, b_dev, 20000, s.tol)
+        return ds.solve_mixed_reliable(1, x, b_dev, 40000, s.tol, interval=args.interval)
 
     results = {}
     for name, fn in (("fp64", run_fp64),) + ((("mixed", run_mixed),) if have_mixed else ()):
@@ -147,9 +158,11 @@ def main():
             its = out[0]
         rr = true_rel_res()
         results[name] = dict(ms=float(np.mean(times)) * 1e3, ms_min=float(np.min(times)) * 1e3, iters=its,
-                             true_rel_res=rr)
+                             true_rel_res=rr, outer=(out[3] if name == "mixed" else None))
         log(f"{name}: {results[name]}")
-        assert rr <= 1.0001e-8 * (1 + 1e-6) or name == "fp64" and rr <= 2e-8, f"{name} missed the tolerance"
+        # the mixed solver stops on the FP64 true residual; the FP64 CG on its
+        # recurrence residual, which tracks the true one to rounding
+        assert rr <= (1.0 if name == "mixed" else 1.05) * 1e-8, f"{name} missed the tolerance: {rr}"
 
     # SpMV bandwidth (dominant kernel), timed on the stream it runs on
     t64 = time_spmv(torch, ds, N, fp32=False)
@@ -180,6 +193,8 @@ def main():
         "fp64_true_rel_res": results["fp64"]["true_rel_res"],
         "mixed_ms": round(results["mixed"]["ms"], 3) if "mixed" in results else None,
         "mixed_iters": results["mixed"]["iters"] if "mixed" in results else None,
+        "mixed_outer_steps": results["mixed"]["outer"] if "mixed" in results else None,
+        "mixed_replacement_interval": args.interval if "mixed" in results else None,
         "mixed_true_rel_res": results["mixed"]["true_rel_res"] if "mixed" in results else None,
         "speedup_mixed_vs_fp64": (round(results["fp64"]["ms"] / results["mixed"]["ms"], 3)
                                   if "mixed" in results else None),
@@ -187,7 +202,7 @@ def main():
         "spmv_fp32_GBs": round(b32 / t32 / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_spmv<float>" if head == "mixed" else "k_spmv<double>",
+                     "kernel": "k_spmv<DiffCsr32>" if head == "mixed" else "k_spmv<Csr<double>>",
                      "bytes_per_launch": spmv_b, "us_per_launch": round(spmv_t * 1e6, 2)},
     }
     if not args.no_cpu:

@@ -107,6 +107,36 @@ int tflows_system_set_values(tflows_system s, const double* aval, int32_t mem, v
 int tflows_solve(tflows_system s, int32_t method, int32_t prec, double* x, double* r1, int32_t* niter,
                  double tol, double* ini_res, double* fin_res, void* stream);
 
+/* Mixed precision: the CG / BiCG / CGS loop of `method` in FP32 (SpMV,
+ * preconditioner, vector updates and dot products in FP32) inside an FP64
+ * iterative-refinement loop.  x (device, N doubles) is the initial guess in
+ * and the solution out; b (device) is not modified; r_out (device, may be
+ * NULL) receives the final true residual b - A x.  Stops when the FP64 true
+ * residual satisfies ||b - A x||_2 < tol.  Each inner solve runs to a
+ * relative reduction inner_rtol of its (unit-scaled) right-hand side;
+ * *niter is the cap on the total inner iterations in and their count out
+ * (count + 1 if not converged); at most max_outer refinement steps;
+ * *outer_steps (may be NULL) receives the number taken.  IniRes / FinRes
+ * are the FP64 true-residual norms before and after. */
+int tflows_solve_mixed(tflows_system s, int32_t method, int32_t prec, double* x, const double* b, double* r_out,
+                       int32_t* niter, double tol, double inner_rtol, int32_t max_outer, double* ini_res,
+                       double* fin_res, int32_t* outer_steps, void* stream);
+
+/* Mixed precision, reliable-update form (CG, prec 0 or 1): a single FP32 CG
+ * recurrence (SpMV in difference form, preconditioner, vector updates and
+ * dot products in FP32) with an FP64 refinement step every `interval`
+ * iterations and whenever the FP32 recurrence residual reaches tol: the
+ * FP32 correction is merged into x in FP64, the true residual b - A x is
+ * recomputed in FP64 and replaces the FP32 residual; the search direction
+ * is kept.  Stops when the FP64 true residual satisfies ||b - A x|| < tol,
+ * checked at the refinement steps.  Arguments as tflows_solve_mixed;
+ * *niter is the cap on FP32 iterations in and the count out (+1 if not
+ * converged); *replacements (may be NULL) receives the number of FP64
+ * refinement steps. */
+int tflows_solve_mixed_reliable(tflows_system s, int32_t prec, double* x, const double* b, double* r_out,
+                                int32_t* niter, double tol, int32_t interval, double* ini_res, double* fin_res,
+                                int32_t* replacements, void* stream);
+
 /* y = A x on the device, FP64 (x, y: N doubles). */
 int tflows_spmv(tflows_system s, const double* x, double* y, void* stream);
 /* y = A x with the FP32 copy of the values (x, y: N floats). */

@@ -177,6 +177,36 @@ int tflows_solve(tflows_system s, int32_t method, int32_t prec, double* x, doubl
     });
 }
 
+int tflows_solve_mixed(tflows_system s, int32_t method, int32_t prec, double* x, const double* b, double* r_out,
+                       int32_t* niter, double tol, double inner_rtol, int32_t max_outer, double* ini_res,
+                       double* fin_res, int32_t* outer_steps, void* stream) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        int outer = 0;
+        tflows::SolveResult r = s->sys->solve_mixed(method, prec, x, b, r_out, *niter, tol, inner_rtol, max_outer,
+                                                    &outer, as_stream(stream));
+        *niter = r.niter;
+        if (ini_res) *ini_res = r.ini_res;
+        if (fin_res) *fin_res = r.fin_res;
+        if (outer_steps) *outer_steps = outer;
+    });
+}
+
+int tflows_solve_mixed_reliable(tflows_system s, int32_t prec, double* x, const double* b, double* r_out,
+                                int32_t* niter, double tol, int32_t interval, double* ini_res, double* fin_res,
+                                int32_t* replacements, void* stream) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        int rep = 0;
+        tflows::SolveResult r =
+            s->sys->solve_mixed_reliable(prec, x, b, r_out, *niter, tol, interval, &rep, as_stream(stream));
+        *niter = r.niter;
+        if (ini_res) *ini_res = r.ini_res;
+        if (fin_res) *fin_res = r.fin_res;
+        if (replacements) *replacements = rep;
+    });
+}
+
 int tflows_spmv(tflows_system s, const double* x, double* y, void* stream) {
     return guarded([&] { s->sys->spmv(x, y, as_stream(stream)); });
 }

@@ -1,0 +1,392 @@
+// cg_fused.hip -- CG with the diagonal (or no) preconditioner in two fused
+// kernels per iteration, in FP64 (the drop-in path) and FP32 (the inner
+// solve of the mixed-precision path).
+//
+// Process/CG.f90:113-259 per iteration reads r, forms z = r/D, rho = (r,z),
+// p = z + beta p, q = A p, alfa = rho/(p,q), x += alfa p, r -= alfa q,
+// ||r||.  Here:
+//
+//   k_cgf_spmv    stopping test of the previous iteration; rho; beta;
+//                 q = A p_new with p_new = z + beta p_old formed inside the
+//                 SpMV gathers (so p needs no separate pass); writes p_new
+//                 and q; (p_new, q) -> site D
+//   k_cgf_update  alfa = rho/(p,q); x += alfa p; r -= alfa q; z = r/D;
+//                 ||r||^2 and the next rho = (r,z) -> site B
+//
+// p alternates between two buffers by iteration parity (the SpMV reads the
+// old p at neighbouring rows while it writes the new one).  Every value is
+// formed with the reference's operations in the reference's order; only
+// the dot-product summation order differs (block tree instead of a serial
+// loop).
+//
+// The iteration index lives on the device: k_cgf_update stores the number
+// of completed iterations in `it`, k_cgf_spmv stores the one in progress in
+// `kcur`, and each kernel only reads the field the other writes.
+#include "krylov_kernels.h"
+
+namespace tflows {
+
+namespace {
+__device__ __forceinline__ bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
+}
+
+// Stopping test before iteration kd (CG.f90:93-108 for kd == 1, :248-257
+// otherwise; kd == cap + 1 closes the loop with niter = cap + 1).  In MIXED
+// mode the decision is taken on the FP64 true residual (site E) when the
+// last FP64 residual replacement came right before iteration kd; otherwise
+// the FP32 recurrence residual only pauses the iterations until the next
+// replacement once it reaches tol, or once it has dropped below
+// kReplaceDrop times the FP64 norm of the last replacement (so the FP32
+// recurrence never runs far past the accuracy FP32 can attain).  Returns
+// true when iteration kd must not run.
+constexpr double kReplaceDrop = 1e-2;
+
+template <typename T, bool MIXED>
+__device__ __forceinline__ bool fused_check(int kd, const T* __restrict__ partB, const double* __restrict__ partE,
+                                            SolverState* st, HostMirror* hm, T& rho_next) {
+    T tot[2];
+    site_total<T, 2>(partB, tot);
+    rho_next = tot[1];
+    const double tol = st->tol;
+    const int cap = st->cap;
+    bool authoritative = true;
+    double err;
+    if (MIXED && st->replaced_before == kd) {
+        double e64[1];
+        site_total<double, 1>(partE, e64);
+        err = sqrt(e64[0]);
+    } else {
+        err = sqrt((double)tot[0]);
+        authoritative = !MIXED;
+    }
+    if (leader() && authoritative) {
+        st->error = err;
+        if (MIXED) st->err_rep = err;
+        if (kd == 1) {
+            st->ini_res = err;
+            st->ini_written = 1;
+        }
+    }
+    if (MIXED && !authoritative && err < kReplaceDrop * st->err_rep) {
+        if (leader()) {
+            st->suspend = 1;
+            publish(hm, 0, kd - 1);
+        }
+        return true;
+    }
+    if (err < tol) {
+        if (leader()) {
+            if (authoritative) {
+                st->done = 1;
+                st->niter_out = kd - 1;
+                publish(hm, 1, kd - 1);
+            } else {
+                st->suspend = 1;
+                publish(hm, 0, kd - 1);
+            }
+        }
+        return true;
+    }
+    if (kd > cap) {
+        if (leader()) {
+            st->done = 1;
+            st->niter_out = (cap > 0 ? cap : 0) + 1;
+            publish(hm, 1, kd - 1);
+        }
+        return true;
+    }
+    if (leader()) publish(hm, 0, kd - 1);
+    return false;
+}
+
+template <typename T>
+struct GatherPNew {
+    const T* __restrict__ z;
+    const T* __restrict__ p_old;
+    T beta;
+    bool first;
+    __device__ __forceinline__ T at(int c) const {
+        const T zc = z[c];
+        return first ? zc : zc + beta * p_old[c];   // CG.f90:180, :185
+    }
+    __device__ void operator()(int c, T (&g)[1]) const { g[0] = at(c); }
+};
+
+template <typename T>
+struct PNewEpi {
+    GatherPNew<T> g;
+    T* __restrict__ p_new;
+    T* __restrict__ q;
+    T acc[1];
+    __device__ void operator()(int row, T (&y)[1]) {
+        const T pr = g.at(row);
+        p_new[row] = pr;
+        q[row] = y[0];
+        acc[0] += pr * y[0];                         // (p, q)  CG.f90:216
+    }
+};
+
+template <typename Op, bool MIXED, typename T = typename OpValue<Op>::type>
+__global__ __launch_bounds__(kBlock) void k_cgf_spmv(Op A, const T* __restrict__ z, T* __restrict__ p0,
+                                                     T* __restrict__ p1, T* __restrict__ q,
+                                                     const T* __restrict__ partB, const double* __restrict__ partE,
+                                                     T* __restrict__ partD, SolverState* st, HostMirror* hm) {
+    if (st->done || st->suspend) return;
+    const int kd = st->it + 1;
+    T rho;
+    if (fused_check<T, MIXED>(kd, partB, partE, st, hm, rho)) return;
+    if (leader()) {
+        st->rho[kd & 1] = (double)rho;
+        st->kcur = kd;
+    }
+    const bool first = kd == 1;
+    const T beta = first ? T(0) : rho / (T)st->rho[(kd - 1) & 1];   // CG.f90:183
+    T* p_new = (kd & 1) ? p1 : p0;
+    const T* p_old = (kd & 1) ? p0 : p1;
+    GatherPNew<T> g{z, p_old, beta, first};
+    PNewEpi<T> epi{g, p_new, q, {T(0)}};
+    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    for (int t = t0; t < t1; ++t) op_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+    site_write<T, 1>(epi.acc, partD);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cgf_update(int n, T* __restrict__ x, T* __restrict__ r,
+                                                       T* __restrict__ z, const T* __restrict__ p0,
+                                                       const T* __restrict__ p1, const T* __restrict__ q,
+                                                       const T* __restrict__ D, const T* __restrict__ partD,
+                                                       T* __restrict__ partB, SolverState* st) {
+    if (st->done || st->suspend) return;
+    const int kd = st->kcur;
+    T pq[1];
+    site_total<T, 1>(partD, pq);
+    const T alfa = (T)st->rho[kd & 1] / pq[0];          // CG.f90:219
+    const T* p = (kd & 1) ? p1 : p0;
+    const int ntiles = (n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    const int rb = t0 * kTileRows, re = min(t1 * kTileRows, n);
+    T v[2] = {T(0), T(0)};
+    for (int i = rb + threadIdx.x; i < re; i += kBlock) {
+        x[i] = x[i] + alfa * p[i];                       // CG.f90:227
+        const T ri = r[i] - alfa * q[i];                 // CG.f90:228
+        r[i] = ri;
+        const T zi = ri / D[i];                          // next z, CG.f90:126
+        z[i] = zi;
+        v[0] += ri * ri;                                 // ||r||^2, CG.f90:250
+        v[1] += ri * zi;                                 // next rho, CG.f90:174
+    }
+    site_write<T, 2>(v, partB);
+    if (leader()) st->it = kd;
+}
+
+template <typename T, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_cgf_check(const T* __restrict__ partB, const double* __restrict__ partE,
+                                                      SolverState* st, HostMirror* hm) {
+    if (st->done || st->suspend) return;
+    T rho;
+    fused_check<T, MIXED>(st->it + 1, partB, partE, st, hm, rho);
+}
+
+// z = r / D and the sums (r,r), (r,z) of the initial residual: site B of the
+// first fused iteration (the residual itself is made by k_resid_init or by
+// the FP64 replacement of the mixed path).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cgf_init_z(int n, const T* __restrict__ r, const T* __restrict__ D,
+                                                       T* __restrict__ z, T* __restrict__ partB,
+                                                       const SolverState* st) {
+    const int ntiles = (n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    const int rb = t0 * kTileRows, re = min(t1 * kTileRows, n);
+    T v[2] = {T(0), T(0)};
+    if (!st->done) {
+        for (int i = rb + threadIdx.x; i < re; i += kBlock) {
+            const T ri = r[i];
+            const T zi = ri / D[i];
+            z[i] = zi;
+            v[0] += ri * ri;
+            v[1] += ri * zi;
+        }
+    }
+    site_write<T, 2>(v, partB);
+}
+
+// ---------------------------------------------------------------------------
+// The same iteration in three kernels: the p update is its own streaming
+// pass (z and p only), and the SpMV gathers p alone.  Each element-wise loop
+// is unrolled so that a thread issues the loads of all its rows before it
+// uses any of them.
+// ---------------------------------------------------------------------------
+constexpr int kUnroll = 4;
+
+template <typename T, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_cgj_p(const T* __restrict__ z, T* __restrict__ p,
+                                                  const T* __restrict__ partB, const double* __restrict__ partE,
+                                                  int n, SolverState* st, HostMirror* hm) {
+    if (st->done || st->suspend) return;
+    const int kd = st->it + 1;
+    T rho;
+    if (fused_check<T, MIXED>(kd, partB, partE, st, hm, rho)) return;
+    if (leader()) {
+        st->rho[kd & 1] = (double)rho;
+        st->kcur = kd;
+    }
+    const bool first = kd == 1;
+    const T beta = first ? T(0) : rho / (T)st->rho[(kd - 1) & 1];   // CG.f90:183
+    const int ntiles = (n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    const int rb = t0 * kTileRows, re = min(t1 * kTileRows, n);
+    for (int i0 = rb + threadIdx.x; i0 < re; i0 += kUnroll * kBlock) {
+        T zv[kUnroll], pv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const int i = i0 + u * kBlock;
+            zv[u] = i < re ? z[i] : T(0);
+            pv[u] = (i < re && !first) ? p[i] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const int i = i0 + u * kBlock;
+            if (i < re) p[i] = first ? zv[u] : zv[u] + beta * pv[u];   // CG.f90:180, :185
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cgj_update(int n, T* __restrict__ x, T* __restrict__ r,
+                                                       T* __restrict__ z, const T* __restrict__ p,
+                                                       const T* __restrict__ q, const T* __restrict__ D,
+                                                       const T* __restrict__ partD, T* __restrict__ partB,
+                                                       SolverState* st) {
+    if (st->done || st->suspend) return;
+    const int kd = st->kcur;
+    T pq[1];
+    site_total<T, 1>(partD, pq);
+    const T alfa = (T)st->rho[kd & 1] / pq[0];          // CG.f90:219
+    const int ntiles = (n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    const int rb = t0 * kTileRows, re = min(t1 * kTileRows, n);
+    T v[2] = {T(0), T(0)};
+    for (int i0 = rb + threadIdx.x; i0 < re; i0 += kUnroll * kBlock) {
+        T xv[kUnroll], pv[kUnroll], rv[kUnroll], qv[kUnroll], dv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const int i = i0 + u * kBlock;
+            const bool ok = i < re;
+            xv[u] = ok ? x[i] : T(0);
+            pv[u] = ok ? p[i] : T(0);
+            rv[u] = ok ? r[i] : T(0);
+            qv[u] = ok ? q[i] : T(0);
+            dv[u] = ok ? D[i] : T(1);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const int i = i0 + u * kBlock;
+            if (i < re) {
+                x[i] = xv[u] + alfa * pv[u];                // CG.f90:227
+                const T ri = rv[u] - alfa * qv[u];          // CG.f90:228
+                r[i] = ri;
+                const T zi = ri / dv[u];                    // next z, CG.f90:126
+                z[i] = zi;
+                v[0] += ri * ri;                            // ||r||^2, CG.f90:250
+                v[1] += ri * zi;                            // next rho, CG.f90:174
+            }
+        }
+    }
+    site_write<T, 2>(v, partB);
+    if (leader()) st->it = kd;
+}
+
+template <typename Op, typename T = typename OpValue<Op>::type>
+__global__ __launch_bounds__(kBlock) void k_cgj_spmv(Op A, const T* __restrict__ p, T* __restrict__ q,
+                                                     T* __restrict__ partD, const SolverState* st) {
+    if (st->done || st->suspend) return;
+    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    Gather1<T> g{p};
+    struct Epi {
+        const T* __restrict__ p;
+        T* __restrict__ q;
+        T acc[1];
+        __device__ void operator()(int row, T (&y)[1]) {
+            q[row] = y[0];
+            acc[0] += p[row] * y[0];                     // (p, q)  CG.f90:216
+        }
+    } epi{p, q, {T(0)}};
+    for (int t = t0; t < t1; ++t) op_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+    site_write<T, 1>(epi.acc, partD);
+}
+
+#define TF_FL(k, grid, s) hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s
+
+template <typename Op, typename T>
+void launch_cgf_spmv(bool mixed, const Op& A, const T* z, T* p0, T* p1, T* q, const Sites<T>& S,
+                     const double* partE, SolverState* st, HostMirror* hm, hipStream_t s) {
+    if (mixed) TF_FL((k_cgf_spmv<Op, true>), S.grid, s), A, z, p0, p1, q, S.B, partE, S.D, st, hm);
+    else TF_FL((k_cgf_spmv<Op, false>), S.grid, s), A, z, p0, p1, q, S.B, partE, S.D, st, hm);
+}
+template <typename T>
+void launch_cgf_update(int n, T* x, T* r, T* z, const T* p0, const T* p1, const T* q, const T* D,
+                       const Sites<T>& S, SolverState* st, hipStream_t s) {
+    TF_FL((k_cgf_update<T>), S.grid, s), n, x, r, z, p0, p1, q, D, S.D, S.B, st);
+}
+template <typename T>
+void launch_cgf_check(bool mixed, const Sites<T>& S, const double* partE, SolverState* st, HostMirror* hm,
+                      hipStream_t s) {
+    if (mixed) TF_FL((k_cgf_check<T, true>), S.grid, s), S.B, partE, st, hm);
+    else TF_FL((k_cgf_check<T, false>), S.grid, s), S.B, partE, st, hm);
+}
+template <typename T>
+void launch_cgf_init_z(int n, const T* r, const T* D, T* z, const Sites<T>& S, const SolverState* st,
+                       hipStream_t s) {
+    TF_FL((k_cgf_init_z<T>), S.grid, s), n, r, D, z, S.B, st);
+}
+
+template <typename Op, typename T>
+void launch_cgj(bool mixed, const Op& A, int n, T* x, T* r, T* z, T* p, T* q, const T* D, const Sites<T>& S,
+                const double* partE, SolverState* st, HostMirror* hm, hipStream_t s) {
+    if (mixed) TF_FL((k_cgj_p<T, true>), S.grid, s), (const T*)z, p, S.B, partE, n, st, hm);
+    else TF_FL((k_cgj_p<T, false>), S.grid, s), (const T*)z, p, S.B, partE, n, st, hm);
+    TF_FL((k_cgj_spmv<Op>), S.grid, s), A, (const T*)p, q, S.D, st);
+    TF_FL((k_cgj_update<T>), S.grid, s), n, x, r, z, (const T*)p, (const T*)q, D, S.D, S.B, st);
+}
+template void launch_cgj<Csr<double>, double>(bool, const Csr<double>&, int, double*, double*, double*, double*,
+                                              double*, const double*, const Sites<double>&, const double*,
+                                              SolverState*, HostMirror*, hipStream_t);
+template void launch_cgj<Csr<float>, float>(bool, const Csr<float>&, int, float*, float*, float*, float*, float*,
+                                            const float*, const Sites<float>&, const double*, SolverState*,
+                                            HostMirror*, hipStream_t);
+template void launch_cgj<DiffCsr32, float>(bool, const DiffCsr32&, int, float*, float*, float*, float*, float*,
+                                           const float*, const Sites<float>&, const double*, SolverState*,
+                                           HostMirror*, hipStream_t);
+
+template void launch_cgf_spmv<Csr<double>, double>(bool, const Csr<double>&, const double*, double*, double*,
+                                                   double*, const Sites<double>&, const double*, SolverState*,
+                                                   HostMirror*, hipStream_t);
+template void launch_cgf_spmv<Csr<float>, float>(bool, const Csr<float>&, const float*, float*, float*, float*,
+                                                 const Sites<float>&, const double*, SolverState*, HostMirror*,
+                                                 hipStream_t);
+template void launch_cgf_spmv<DiffCsr32, float>(bool, const DiffCsr32&, const float*, float*, float*, float*,
+                                                const Sites<float>&, const double*, SolverState*, HostMirror*,
+                                                hipStream_t);
+template void launch_cgf_update<double>(int, double*, double*, double*, const double*, const double*,
+                                        const double*, const double*, const Sites<double>&, SolverState*,
+                                        hipStream_t);
+template void launch_cgf_update<float>(int, float*, float*, float*, const float*, const float*, const float*,
+                                       const float*, const Sites<float>&, SolverState*, hipStream_t);
+template void launch_cgf_check<double>(bool, const Sites<double>&, const double*, SolverState*, HostMirror*,
+                                       hipStream_t);
+template void launch_cgf_check<float>(bool, const Sites<float>&, const double*, SolverState*, HostMirror*,
+                                      hipStream_t);
+template void launch_cgf_init_z<double>(int, const double*, const double*, double*, const Sites<double>&,
+                                        const SolverState*, hipStream_t);
+template void launch_cgf_init_z<float>(int, const float*, const float*, float*, const Sites<float>&,
+                                       const SolverState*, hipStream_t);
+
+}  // namespace tflows

@@ -22,6 +22,19 @@ struct SolverState {
     int niter_out;                  // the reference's `niter` on exit
     int cap;                        // iteration cap
     int ini_written;                // 1 when IniRes was set
+    // device-side iteration bookkeeping of the fused CG kernels: `it` is the
+    // number of completed iterations (written by the second kernel of an
+    // iteration), `kcur` the one in progress (written by the first); each is
+    // read only by the other kernel, so no kernel reads what it writes.
+    int it;
+    int kcur;
+    // mixed precision with residual replacement: `suspend` is set when the
+    // FP32 recurrence residual reaches tol (iterations pause until the next
+    // FP64 replacement verifies); `replaced_before` = the iteration that
+    // follows the last replacement, whose stopping test uses the FP64 norm.
+    int suspend;
+    int replaced_before;
+    double err_rep;                 // FP64 residual norm at the last replacement
 };
 
 // Host-visible mirror of the progress, written by the device with
@@ -49,15 +62,35 @@ struct Csr {
     const T* __restrict__ val;       // nnz
 };
 
+// FP32 operator in difference ("flux") form, used by the inner solves of
+// the mixed-precision path:
+//   (A p)_i = sum_{j != i} a_ij (p_j - p_i) + rs_i p_i,   rs_i = sum_j a_ij
+// with the row sums rs_i formed in FP64 and rounded once.  It is the same
+// operator as A, but its FP32 evaluation keeps relative accuracy on smooth
+// vectors: rows of a pressure-correction matrix sum to zero, and the
+// plain form a_ii p_i + sum a_ij p_j cancels terms of size |a_ii p_i| that
+// FP32 cannot resolve against the tiny eigenvalues of the near-null mode.
+struct DiffCsr32 {
+    int n;
+    const int* __restrict__ rowptr;  // n+1, off-diagonal entries only
+    const int* __restrict__ col;     // nnz - n
+    const float* __restrict__ val;   // off-diagonal a_ij
+    const float* __restrict__ rs;    // row sums (from FP64)
+};
+
+template <typename Op> struct OpValue;
+template <typename T> struct OpValue<Csr<T>> { using type = T; };
+template <> struct OpValue<DiffCsr32> { using type = float; };
+
 // Rows per tile of the stream kernels: one row per thread in the summation
 // phase.  Tiles are the unit of work split across blocks.
 constexpr int kTileRows = kBlock;
-// LDS products per right-hand side (per chunk).  A 7-point tile needs
+// LDS entries per right-hand side (per chunk).  A 7-point tile needs
 // 256*7 = 1792; longer rows are handled by chunking.
-template <typename T> struct LdsCap { static constexpr int value = 2048; };
+constexpr int kLdsCap = 2048;
 
 // ---------------------------------------------------------------------------
-// Stream one tile of rows [r0, r1):
+// Stream one tile of rows [r0, r1) of a CSR matrix:
 //   phase 1: every thread loads val/col entries of the tile coalesced and
 //            stores val[e]*g_v(col[e]) for each right-hand side v into LDS;
 //   phase 2: thread t sums the products of local row t sequentially, in
@@ -69,9 +102,9 @@ template <typename T> struct LdsCap { static constexpr int value = 2048; };
 // epi(row, y[NRHS]) is called once per row by its owning thread.
 // ---------------------------------------------------------------------------
 template <typename T, int NRHS, bool SUBTRACT, class Gather, class Init, class Epi>
-__device__ __forceinline__ void stream_tile(const Csr<T>& A, int r0, int r1, Gather& g, Init& init,
-                                            Epi& epi, int* s_rp, T* s_prod) {
-    constexpr int CAP = LdsCap<T>::value;
+__device__ __forceinline__ void stream_tile(const Csr<T>& A, int r0, int r1, Gather& g, Init& init, Epi& epi) {
+    __shared__ int s_rp[kTileRows + 1];
+    __shared__ T s_prod[NRHS * kLdsCap];
     const int tid = threadIdx.x;
     const int nr = r1 - r0;
     __syncthreads();  // the previous tile's readers of s_rp / s_prod are done
@@ -87,15 +120,15 @@ __device__ __forceinline__ void stream_tile(const Csr<T>& A, int r0, int r1, Gat
 #pragma unroll
         for (int v = 0; v < NRHS; ++v) acc[v] = SUBTRACT ? init(r0 + tid, v) : T(0);
     }
-    for (int cs = e0; cs < e1; cs += CAP) {
-        const int ce = min(cs + CAP, e1);
+    for (int cs = e0; cs < e1; cs += kLdsCap) {
+        const int ce = min(cs + kLdsCap, e1);
         for (int e = cs + tid; e < ce; e += kBlock) {
             const int c = A.col[e];
             const T a = A.val[e];
             T gv[NRHS];
             g(c, gv);
 #pragma unroll
-            for (int v = 0; v < NRHS; ++v) s_prod[v * CAP + (e - cs)] = a * gv[v];
+            for (int v = 0; v < NRHS; ++v) s_prod[v * kLdsCap + (e - cs)] = a * gv[v];
         }
         __syncthreads();
         if (has_row) {
@@ -103,14 +136,99 @@ __device__ __forceinline__ void stream_tile(const Csr<T>& A, int r0, int r1, Gat
             for (int k = b; k < e; ++k) {
 #pragma unroll
                 for (int v = 0; v < NRHS; ++v) {
-                    if (SUBTRACT) acc[v] = acc[v] - s_prod[v * CAP + (k - cs)];
-                    else acc[v] = acc[v] + s_prod[v * CAP + (k - cs)];
+                    if (SUBTRACT) acc[v] = acc[v] - s_prod[v * kLdsCap + (k - cs)];
+                    else acc[v] = acc[v] + s_prod[v * kLdsCap + (k - cs)];
                 }
             }
         }
         __syncthreads();
     }
     if (has_row) epi(r0 + tid, acc);
+}
+
+// The same streaming scheme for the difference form: phase 1 stores a_ij
+// and the gathered p_j; phase 2 forms sum a_ij (p_j - p_i) in column order
+// and adds rs_i p_i.
+template <int NRHS, class Gather, class Epi>
+__device__ __forceinline__ void stream_tile_diff(const DiffCsr32& A, int r0, int r1, Gather& g, Epi& epi) {
+    __shared__ int s_rp[kTileRows + 1];
+    __shared__ float s_a[kLdsCap];
+    __shared__ float s_g[NRHS * kLdsCap];
+    const int tid = threadIdx.x;
+    const int nr = r1 - r0;
+    __syncthreads();
+    for (int i = tid; i <= nr; i += kBlock) s_rp[i] = A.rowptr[r0 + i];
+    __syncthreads();
+    const int e0 = s_rp[0], e1 = s_rp[nr];
+    float acc[NRHS], own[NRHS];
+    int my_b = 0, my_e = 0;
+    const bool has_row = tid < nr;
+    if (has_row) {
+        my_b = s_rp[tid];
+        my_e = s_rp[tid + 1];
+        g(r0 + tid, own);
+#pragma unroll
+        for (int v = 0; v < NRHS; ++v) acc[v] = 0.0f;
+    }
+    for (int cs = e0; cs < e1; cs += kLdsCap) {
+        const int ce = min(cs + kLdsCap, e1);
+        for (int e = cs + tid; e < ce; e += kBlock) {
+            const int c = A.col[e];
+            s_a[e - cs] = A.val[e];
+            float gv[NRHS];
+            g(c, gv);
+#pragma unroll
+            for (int v = 0; v < NRHS; ++v) s_g[v * kLdsCap + (e - cs)] = gv[v];
+        }
+        __syncthreads();
+        if (has_row) {
+            const int b = max(my_b, cs), e = min(my_e, ce);
+            for (int k = b; k < e; ++k) {
+                const float a = s_a[k - cs];
+#pragma unroll
+                for (int v = 0; v < NRHS; ++v) acc[v] = acc[v] + a * (s_g[v * kLdsCap + (k - cs)] - own[v]);
+            }
+        }
+        __syncthreads();
+    }
+    if (has_row) {
+        const float rsi = A.rs[r0 + tid];
+#pragma unroll
+        for (int v = 0; v < NRHS; ++v) acc[v] = acc[v] + rsi * own[v];
+        epi(r0 + tid, acc);
+    }
+}
+
+template <typename T>
+struct InitZero {
+    __device__ T operator()(int, int) const { return T(0); }
+};
+template <typename T>
+struct InitVec {
+    const T* __restrict__ b;
+    __device__ T operator()(int r, int) const { return b[r]; }
+};
+template <typename T>
+struct Gather1 {
+    const T* __restrict__ x;
+    __device__ void operator()(int c, T (&g)[1]) const { g[0] = x[c]; }
+};
+template <typename T>
+struct Gather2 {
+    const T* __restrict__ x;
+    const T* __restrict__ y;
+    __device__ void operator()(int c, T (&g)[2]) const { g[0] = x[c]; g[1] = y[c]; }
+};
+
+// q = A p over one tile for either operator (no residual form).
+template <int NRHS, typename T, class Gather, class Epi>
+__device__ __forceinline__ void op_tile(const Csr<T>& A, int r0, int r1, Gather& g, Epi& epi) {
+    InitZero<T> z;
+    stream_tile<T, NRHS, false>(A, r0, r1, g, z, epi);
+}
+template <int NRHS, class Gather, class Epi>
+__device__ __forceinline__ void op_tile(const DiffCsr32& A, int r0, int r1, Gather& g, Epi& epi) {
+    stream_tile_diff<NRHS>(A, r0, r1, g, epi);
 }
 
 }  // namespace tflows

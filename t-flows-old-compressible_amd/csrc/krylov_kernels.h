@@ -33,8 +33,8 @@ template <typename T>
 void launch_dic_bwd_level(const Csr<T>& A, const int* rows, int count, const T* D, T* z,
                           const SolverState* st, hipStream_t s);
 template <typename T>
-void launch_resid_init(int mode, const Csr<T>& A, const T* x, T* r, T* r2, const T* D, const Sites<T>& S,
-                       SolverState* st, HostMirror* hm, hipStream_t s);
+void launch_resid_init(int mode, bool zero_x, const Csr<T>& A, const T* x, T* r, T* r2, const T* D,
+                       const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
 template <typename T>
 void launch_check(int k, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
 template <typename T>
@@ -42,16 +42,16 @@ void launch_rho(int n, const T* a, const T* b, const Sites<T>& S, const SolverSt
 template <typename T>
 void launch_cg_p(bool jac, int k, int n, const T* r, const T* D, const T* z, T* p, const Sites<T>& S,
                  SolverState* st, HostMirror* hm, hipStream_t s);
-template <typename T>
-void launch_cg_spmv(const Csr<T>& A, const T* p, T* q, const Sites<T>& S, const SolverState* st, hipStream_t s);
+template <typename Op, typename T>
+void launch_cg_spmv(const Op& A, const T* p, T* q, const Sites<T>& S, const SolverState* st, hipStream_t s);
 template <typename T>
 void launch_cg_update(bool jac, int k, int n, T* x, T* r, const T* p, const T* q, const T* D, const Sites<T>& S,
                       const SolverState* st, hipStream_t s);
 template <typename T>
 void launch_bicg_p(bool jac, int k, int n, const T* r1, const T* r2, const T* D, const T* z1, const T* z2, T* p1,
                    T* p2, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
-template <typename T>
-void launch_bicg_spmv(const Csr<T>& A, const T* p1, const T* p2, T* q1, T* q2, const Sites<T>& S,
+template <typename Op, typename T>
+void launch_bicg_spmv(const Op& A, const T* p1, const T* p2, T* q1, T* q2, const Sites<T>& S,
                       const SolverState* st, hipStream_t s);
 template <typename T>
 void launch_bicg_update(bool jac, int k, int n, T* x, T* r1, T* r2, const T* p1, const T* q1, const T* q2,
@@ -59,19 +59,37 @@ void launch_bicg_update(bool jac, int k, int n, T* x, T* r1, T* r2, const T* p1,
 template <typename T>
 void launch_cgs_u(bool jac, int k, int n, const T* r1, const T* q1, T* u1, T* u2, const T* D, T* p2,
                   const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
-template <typename T>
-void launch_cgs_spmv_v(const Csr<T>& A, const T* p2, T* v2, const T* r2, const Sites<T>& S, const SolverState* st,
+template <typename Op, typename T>
+void launch_cgs_spmv_v(const Op& A, const T* p2, T* v2, const T* r2, const Sites<T>& S, const SolverState* st,
                        hipStream_t s);
 template <typename T>
 void launch_cgs_q(bool jac, int k, int n, const T* u1, const T* v2, T* q1, const T* D, T* p1, const Sites<T>& S,
                   const SolverState* st, hipStream_t s);
-template <typename T>
-void launch_cgs_spmv_r(int k, const Csr<T>& A, const T* p1, T* x, T* r1, const T* r2, const Sites<T>& S,
+template <typename Op, typename T>
+void launch_cgs_spmv_r(int k, const Op& A, const T* p1, T* x, T* r1, const T* r2, const Sites<T>& S,
                        const SolverState* st, hipStream_t s);
-template <typename T>
-void launch_spmv(const Csr<T>& A, const T* x, T* y, int grid, hipStream_t s);
+template <typename Op, typename T>
+void launch_spmv(const Op& A, const T* x, T* y, int grid, hipStream_t s);
 template <typename T>
 void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s);
+// fused two-kernel CG (cg_fused.hip)
+template <typename Op, typename T>
+void launch_cgf_spmv(bool mixed, const Op& A, const T* z, T* p0, T* p1, T* q, const Sites<T>& S,
+                     const double* partE, SolverState* st, HostMirror* hm, hipStream_t s);
+template <typename T>
+void launch_cgf_update(int n, T* x, T* r, T* z, const T* p0, const T* p1, const T* q, const T* D,
+                       const Sites<T>& S, SolverState* st, hipStream_t s);
+template <typename T>
+void launch_cgf_check(bool mixed, const Sites<T>& S, const double* partE, SolverState* st, HostMirror* hm,
+                      hipStream_t s);
+// three-kernel Jacobi CG with device-side iteration counters (cg_fused.hip)
+template <typename Op, typename T>
+void launch_cgj(bool mixed, const Op& A, int n, T* x, T* r, T* z, T* p, T* q, const T* D, const Sites<T>& S,
+                const double* partE, SolverState* st, HostMirror* hm, hipStream_t s);
+template <typename T>
+void launch_cgf_init_z(int n, const T* r, const T* D, T* z, const Sites<T>& S, const SolverState* st,
+                       hipStream_t s);
+
 void launch_rebase(int count, const int* in, int* out, hipStream_t s);
 void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t s);
 

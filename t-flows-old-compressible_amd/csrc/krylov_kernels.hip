@@ -27,34 +27,6 @@
 
 namespace tflows {
 
-// ---------------------------------------------------------------------------
-// small functors for stream_tile
-// ---------------------------------------------------------------------------
-template <typename T>
-struct Gather1 {
-    const T* __restrict__ x;
-    __device__ void operator()(int c, T (&g)[1]) const { g[0] = x[c]; }
-};
-template <typename T>
-struct Gather2 {
-    const T* __restrict__ x;
-    const T* __restrict__ y;
-    __device__ void operator()(int c, T (&g)[2]) const { g[0] = x[c]; g[1] = y[c]; }
-};
-template <typename T>
-struct InitZero {
-    __device__ T operator()(int, int) const { return T(0); }
-};
-template <typename T>
-struct InitVec {
-    const T* __restrict__ b;
-    __device__ T operator()(int r, int) const { return b[r]; }
-};
-
-#define TFLOWS_STREAM_SMEM(T, NRHS)                  \
-    __shared__ int s_rp[kTileRows + 1];               \
-    __shared__ T s_prod[(NRHS) * LdsCap<T>::value]
-
 // Rows [rb, re) of this block for the element-wise kernels (the same tile
 // split as the stream kernels, so a block touches the rows it streamed).
 #define TFLOWS_BLOCK_ROWS(n)                                  \
@@ -216,12 +188,11 @@ struct ResidEpi {
     }
 };
 
-template <typename T, int MODE>
+template <typename T, int MODE, bool ZERO_X>
 __global__ __launch_bounds__(kBlock) void k_resid_init(Csr<T> A, const T* __restrict__ x, T* __restrict__ r,
                                                        T* __restrict__ r2, const T* __restrict__ D,
                                                        const T* __restrict__ partA, T* __restrict__ partB,
                                                        SolverState* st, HostMirror* hm) {
-    TFLOWS_STREAM_SMEM(T, 1);
     T bb[1];
     site_total<T, 1>(partA, bb);
     const double bn = sqrt((double)bb[0]);
@@ -238,11 +209,19 @@ __global__ __launch_bounds__(kBlock) void k_resid_init(Csr<T> A, const T* __rest
     const int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int t0, t1;
     block_tiles(ntiles, t0, t1);
-    Gather1<T> g{x};
-    InitVec<T> init{r};
     ResidEpi<T, MODE> epi{r, r2, D, {T(0), T(0)}};
-    for (int t = t0; t < t1; ++t)
-        stream_tile<T, 1, true>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi, s_rp, s_prod);
+    if (ZERO_X) {  // x = 0: r = b - A*0 = b, no matrix pass
+        const int rb = t0 * kTileRows, re = min(t1 * kTileRows, A.n);
+        for (int i = rb + threadIdx.x; i < re; i += kBlock) {
+            T y[1] = {r[i]};
+            epi(i, y);
+        }
+    } else {
+        Gather1<T> g{x};
+        InitVec<T> init{r};
+        for (int t = t0; t < t1; ++t)
+            stream_tile<T, 1, true>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi);
+    }
     site_write<T, 2>(epi.acc, partB);
 }
 
@@ -319,19 +298,16 @@ struct DotEpi {
     }
 };
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(Csr<T> A, const T* __restrict__ p, T* __restrict__ q,
+template <typename Op, typename T = typename OpValue<Op>::type>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(Op A, const T* __restrict__ p, T* __restrict__ q,
                                                     T* __restrict__ partD, const SolverState* st) {
     if (st->done) return;
-    TFLOWS_STREAM_SMEM(T, 1);
     const int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int t0, t1;
     block_tiles(ntiles, t0, t1);
     Gather1<T> g{p};
-    InitZero<T> init;
     DotEpi<T> epi{p, q, {T(0)}};
-    for (int t = t0; t < t1; ++t)
-        stream_tile<T, 1, false>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi, s_rp, s_prod);
+    for (int t = t0; t < t1; ++t) op_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
     site_write<T, 1>(epi.acc, partD);
 }
 
@@ -396,21 +372,17 @@ struct BiDotEpi {
     }
 };
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bicg_spmv(Csr<T> A, const T* __restrict__ p1,
-                                                      const T* __restrict__ p2, T* __restrict__ q1,
-                                                      T* __restrict__ q2, T* __restrict__ partD,
-                                                      const SolverState* st) {
+template <typename Op, typename T = typename OpValue<Op>::type>
+__global__ __launch_bounds__(kBlock) void k_bicg_spmv(Op A, const T* __restrict__ p1, const T* __restrict__ p2,
+                                                      T* __restrict__ q1, T* __restrict__ q2,
+                                                      T* __restrict__ partD, const SolverState* st) {
     if (st->done) return;
-    TFLOWS_STREAM_SMEM(T, 2);
     const int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int t0, t1;
     block_tiles(ntiles, t0, t1);
     Gather2<T> g{p1, p2};
-    InitZero<T> init;
     BiDotEpi<T> epi{p2, q1, q2, {T(0)}};
-    for (int t = t0; t < t1; ++t)
-        stream_tile<T, 2, false>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi, s_rp, s_prod);
+    for (int t = t0; t < t1; ++t) op_tile<2>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
     site_write<T, 1>(epi.acc, partD);
 }
 
@@ -474,20 +446,17 @@ __global__ __launch_bounds__(kBlock) void k_cgs_u(int k, int n, const T* __restr
 }
 
 // v2 = A p2 (:193-198) with (r2, v2) -> site D (:215-219)
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cgs_spmv_v(Csr<T> A, const T* __restrict__ p2,
-                                                       T* __restrict__ v2, const T* __restrict__ r2,
-                                                       T* __restrict__ partD, const SolverState* st) {
+template <typename Op, typename T = typename OpValue<Op>::type>
+__global__ __launch_bounds__(kBlock) void k_cgs_spmv_v(Op A, const T* __restrict__ p2, T* __restrict__ v2,
+                                                       const T* __restrict__ r2, T* __restrict__ partD,
+                                                       const SolverState* st) {
     if (st->done) return;
-    TFLOWS_STREAM_SMEM(T, 1);
     const int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int t0, t1;
     block_tiles(ntiles, t0, t1);
     Gather1<T> g{p2};
-    InitZero<T> init;
     DotEpi<T> epi{r2, v2, {T(0)}};
-    for (int t = t0; t < t1; ++t)
-        stream_tile<T, 1, false>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi, s_rp, s_prod);
+    for (int t = t0; t < t1; ++t) op_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
     site_write<T, 1>(epi.acc, partD);
 }
 
@@ -531,13 +500,12 @@ struct CgsREpi {
     }
 };
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cgs_spmv_r(int k, Csr<T> A, const T* __restrict__ p1,
-                                                       T* __restrict__ x, T* __restrict__ r1,
-                                                       const T* __restrict__ r2, const T* __restrict__ partD,
-                                                       T* __restrict__ partB, const SolverState* st) {
+template <typename Op, typename T = typename OpValue<Op>::type>
+__global__ __launch_bounds__(kBlock) void k_cgs_spmv_r(int k, Op A, const T* __restrict__ p1, T* __restrict__ x,
+                                                       T* __restrict__ r1, const T* __restrict__ r2,
+                                                       const T* __restrict__ partD, T* __restrict__ partB,
+                                                       const SolverState* st) {
     if (st->done) return;
-    TFLOWS_STREAM_SMEM(T, 1);
     T rv[1];
     site_total<T, 1>(partD, rv);
     const T alfa = (T)st->rho[k & 1] / rv[0];
@@ -545,10 +513,8 @@ __global__ __launch_bounds__(kBlock) void k_cgs_spmv_r(int k, Csr<T> A, const T*
     int t0, t1;
     block_tiles(ntiles, t0, t1);
     Gather1<T> g{p1};
-    InitZero<T> init;
     CgsREpi<T> epi{x, r1, r2, p1, alfa, {T(0), T(0)}};
-    for (int t = t0; t < t1; ++t)
-        stream_tile<T, 1, false>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi, s_rp, s_prod);
+    for (int t = t0; t < t1; ++t) op_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
     site_write<T, 2>(epi.acc, partB);
 }
 
@@ -562,22 +528,18 @@ struct StoreEpi {
     __device__ void operator()(int row, T (&v)[1]) { y[row] = v[0]; }
 };
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_spmv(Csr<T> A, const T* __restrict__ x, T* __restrict__ y) {
-    TFLOWS_STREAM_SMEM(T, 1);
+template <typename Op, typename T = typename OpValue<Op>::type>
+__global__ __launch_bounds__(kBlock) void k_spmv(Op A, const T* __restrict__ x, T* __restrict__ y) {
     const int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int t0, t1;
     block_tiles(ntiles, t0, t1);
     Gather1<T> g{x};
-    InitZero<T> init;
     StoreEpi<T> epi{y};
-    for (int t = t0; t < t1; ++t)
-        stream_tile<T, 1, false>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi, s_rp, s_prod);
+    for (int t = t0; t < t1; ++t) op_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_resid(Csr<T> A, const T* __restrict__ x, T* __restrict__ r) {
-    TFLOWS_STREAM_SMEM(T, 1);
     const int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int t0, t1;
     block_tiles(ntiles, t0, t1);
@@ -585,7 +547,7 @@ __global__ __launch_bounds__(kBlock) void k_resid(Csr<T> A, const T* __restrict_
     InitVec<T> init{r};
     StoreEpi<T> epi{r};
     for (int t = t0; t < t1; ++t)
-        stream_tile<T, 1, true>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi, s_rp, s_prod);
+        stream_tile<T, 1, true>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi);
 }
 
 // 1-based T-FlowS pattern -> 0-based device copy (TopolM layout).
@@ -601,7 +563,7 @@ __global__ void k_convert(int64_t count, const TI* __restrict__ in, TO* __restri
 }
 
 // ---------------------------------------------------------------------------
-// host-side launchers (explicitly instantiated for double and float)
+// host-side launchers (explicitly instantiated below)
 // ---------------------------------------------------------------------------
 #define TF_LAUNCH(k, grid, stream) hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, stream
 
@@ -623,15 +585,21 @@ void launch_dic_bwd_level(const Csr<T>& A, const int* rows, int count, const T* 
                           const SolverState* st, hipStream_t s) {
     if (count > 0) TF_LAUNCH((k_dic_bwd_level<T>), (count + kBlock - 1) / kBlock, s), A, rows, count, D, z, st);
 }
-template <typename T>
-void launch_resid_init(int mode, const Csr<T>& A, const T* x, T* r, T* r2, const T* D, const Sites<T>& S,
-                       SolverState* st, HostMirror* hm, hipStream_t s) {
+template <typename T, bool Z>
+static void resid_init_mode(int mode, const Csr<T>& A, const T* x, T* r, T* r2, const T* D, const Sites<T>& S,
+                            SolverState* st, HostMirror* hm, hipStream_t s) {
     switch (mode) {
-        case RHO_JAC: TF_LAUNCH((k_resid_init<T, RHO_JAC>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
-        case RHO_BICG: TF_LAUNCH((k_resid_init<T, RHO_BICG>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
-        case RHO_RR: TF_LAUNCH((k_resid_init<T, RHO_RR>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
-        default: TF_LAUNCH((k_resid_init<T, RHO_NONE>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
+        case RHO_JAC: TF_LAUNCH((k_resid_init<T, RHO_JAC, Z>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
+        case RHO_BICG: TF_LAUNCH((k_resid_init<T, RHO_BICG, Z>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
+        case RHO_RR: TF_LAUNCH((k_resid_init<T, RHO_RR, Z>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
+        default: TF_LAUNCH((k_resid_init<T, RHO_NONE, Z>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
     }
+}
+template <typename T>
+void launch_resid_init(int mode, bool zero_x, const Csr<T>& A, const T* x, T* r, T* r2, const T* D,
+                       const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
+    if (zero_x) resid_init_mode<T, true>(mode, A, x, r, r2, D, S, st, hm, s);
+    else resid_init_mode<T, false>(mode, A, x, r, r2, D, S, st, hm, s);
 }
 template <typename T>
 void launch_check(int k, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
@@ -647,9 +615,9 @@ void launch_cg_p(bool jac, int k, int n, const T* r, const T* D, const T* z, T* 
     if (jac) TF_LAUNCH((k_cg_p<T, true>), S.grid, s), k, n, r, D, z, p, S.B, S.C, st, hm);
     else TF_LAUNCH((k_cg_p<T, false>), S.grid, s), k, n, r, D, z, p, S.B, S.C, st, hm);
 }
-template <typename T>
-void launch_cg_spmv(const Csr<T>& A, const T* p, T* q, const Sites<T>& S, const SolverState* st, hipStream_t s) {
-    TF_LAUNCH((k_cg_spmv<T>), S.grid, s), A, p, q, S.D, st);
+template <typename Op, typename T>
+void launch_cg_spmv(const Op& A, const T* p, T* q, const Sites<T>& S, const SolverState* st, hipStream_t s) {
+    TF_LAUNCH((k_cg_spmv<Op>), S.grid, s), A, p, q, S.D, st);
 }
 template <typename T>
 void launch_cg_update(bool jac, int k, int n, T* x, T* r, const T* p, const T* q, const T* D, const Sites<T>& S,
@@ -663,10 +631,10 @@ void launch_bicg_p(bool jac, int k, int n, const T* r1, const T* r2, const T* D,
     if (jac) TF_LAUNCH((k_bicg_p<T, true>), S.grid, s), k, n, r1, r2, D, z1, z2, p1, p2, S.B, S.C, st, hm);
     else TF_LAUNCH((k_bicg_p<T, false>), S.grid, s), k, n, r1, r2, D, z1, z2, p1, p2, S.B, S.C, st, hm);
 }
-template <typename T>
-void launch_bicg_spmv(const Csr<T>& A, const T* p1, const T* p2, T* q1, T* q2, const Sites<T>& S,
+template <typename Op, typename T>
+void launch_bicg_spmv(const Op& A, const T* p1, const T* p2, T* q1, T* q2, const Sites<T>& S,
                       const SolverState* st, hipStream_t s) {
-    TF_LAUNCH((k_bicg_spmv<T>), S.grid, s), A, p1, p2, q1, q2, S.D, st);
+    TF_LAUNCH((k_bicg_spmv<Op>), S.grid, s), A, p1, p2, q1, q2, S.D, st);
 }
 template <typename T>
 void launch_bicg_update(bool jac, int k, int n, T* x, T* r1, T* r2, const T* p1, const T* q1, const T* q2,
@@ -680,10 +648,10 @@ void launch_cgs_u(bool jac, int k, int n, const T* r1, const T* q1, T* u1, T* u2
     if (jac) TF_LAUNCH((k_cgs_u<T, true>), S.grid, s), k, n, r1, q1, u1, u2, D, p2, S.B, st, hm);
     else TF_LAUNCH((k_cgs_u<T, false>), S.grid, s), k, n, r1, q1, u1, u2, D, p2, S.B, st, hm);
 }
-template <typename T>
-void launch_cgs_spmv_v(const Csr<T>& A, const T* p2, T* v2, const T* r2, const Sites<T>& S, const SolverState* st,
+template <typename Op, typename T>
+void launch_cgs_spmv_v(const Op& A, const T* p2, T* v2, const T* r2, const Sites<T>& S, const SolverState* st,
                        hipStream_t s) {
-    TF_LAUNCH((k_cgs_spmv_v<T>), S.grid, s), A, p2, v2, r2, S.D, st);
+    TF_LAUNCH((k_cgs_spmv_v<Op>), S.grid, s), A, p2, v2, r2, S.D, st);
 }
 template <typename T>
 void launch_cgs_q(bool jac, int k, int n, const T* u1, const T* v2, T* q1, const T* D, T* p1, const Sites<T>& S,
@@ -691,14 +659,14 @@ void launch_cgs_q(bool jac, int k, int n, const T* u1, const T* v2, T* q1, const
     if (jac) TF_LAUNCH((k_cgs_q<T, true>), S.grid, s), k, n, u1, v2, q1, D, p1, S.D, st);
     else TF_LAUNCH((k_cgs_q<T, false>), S.grid, s), k, n, u1, v2, q1, D, p1, S.D, st);
 }
-template <typename T>
-void launch_cgs_spmv_r(int k, const Csr<T>& A, const T* p1, T* x, T* r1, const T* r2, const Sites<T>& S,
+template <typename Op, typename T>
+void launch_cgs_spmv_r(int k, const Op& A, const T* p1, T* x, T* r1, const T* r2, const Sites<T>& S,
                        const SolverState* st, hipStream_t s) {
-    TF_LAUNCH((k_cgs_spmv_r<T>), S.grid, s), k, A, p1, x, r1, r2, S.D, S.B, st);
+    TF_LAUNCH((k_cgs_spmv_r<Op>), S.grid, s), k, A, p1, x, r1, r2, S.D, S.B, st);
 }
-template <typename T>
-void launch_spmv(const Csr<T>& A, const T* x, T* y, int grid, hipStream_t s) {
-    TF_LAUNCH((k_spmv<T>), grid, s), A, x, y);
+template <typename Op, typename T>
+void launch_spmv(const Op& A, const T* x, T* y, int grid, hipStream_t s) {
+    TF_LAUNCH((k_spmv<Op>), grid, s), A, x, y);
 }
 template <typename T>
 void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s) {
@@ -720,34 +688,39 @@ void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t 
                                           const SolverState*, hipStream_t);                                      \
     template void launch_dic_bwd_level<T>(const Csr<T>&, const int*, int, const T*, T*, const SolverState*,      \
                                           hipStream_t);                                                          \
-    template void launch_resid_init<T>(int, const Csr<T>&, const T*, T*, T*, const T*, const Sites<T>&,         \
+    template void launch_resid_init<T>(int, bool, const Csr<T>&, const T*, T*, T*, const T*, const Sites<T>&,   \
                                        SolverState*, HostMirror*, hipStream_t);                                  \
     template void launch_check<T>(int, const Sites<T>&, SolverState*, HostMirror*, hipStream_t);                 \
     template void launch_rho<T>(int, const T*, const T*, const Sites<T>&, const SolverState*, hipStream_t);      \
     template void launch_cg_p<T>(bool, int, int, const T*, const T*, const T*, T*, const Sites<T>&,             \
                                  SolverState*, HostMirror*, hipStream_t);                                        \
-    template void launch_cg_spmv<T>(const Csr<T>&, const T*, T*, const Sites<T>&, const SolverState*,           \
-                                    hipStream_t);                                                                \
     template void launch_cg_update<T>(bool, int, int, T*, T*, const T*, const T*, const T*, const Sites<T>&,    \
                                       const SolverState*, hipStream_t);                                          \
     template void launch_bicg_p<T>(bool, int, int, const T*, const T*, const T*, const T*, const T*, T*, T*,    \
                                    const Sites<T>&, SolverState*, HostMirror*, hipStream_t);                     \
-    template void launch_bicg_spmv<T>(const Csr<T>&, const T*, const T*, T*, T*, const Sites<T>&,               \
-                                      const SolverState*, hipStream_t);                                          \
     template void launch_bicg_update<T>(bool, int, int, T*, T*, T*, const T*, const T*, const T*, const T*,     \
                                         const Sites<T>&, const SolverState*, hipStream_t);                       \
     template void launch_cgs_u<T>(bool, int, int, const T*, const T*, T*, T*, const T*, T*, const Sites<T>&,    \
                                   SolverState*, HostMirror*, hipStream_t);                                       \
-    template void launch_cgs_spmv_v<T>(const Csr<T>&, const T*, T*, const T*, const Sites<T>&,                  \
-                                       const SolverState*, hipStream_t);                                         \
     template void launch_cgs_q<T>(bool, int, int, const T*, const T*, T*, const T*, T*, const Sites<T>&,        \
                                   const SolverState*, hipStream_t);                                              \
-    template void launch_cgs_spmv_r<T>(int, const Csr<T>&, const T*, T*, T*, const T*, const Sites<T>&,         \
-                                       const SolverState*, hipStream_t);                                         \
-    template void launch_spmv<T>(const Csr<T>&, const T*, T*, int, hipStream_t);                               \
     template void launch_resid<T>(const Csr<T>&, const T*, T*, int, hipStream_t);
+
+#define TF_INSTANTIATE_OP(Op, T)                                                                                 \
+    template void launch_cg_spmv<Op, T>(const Op&, const T*, T*, const Sites<T>&, const SolverState*,           \
+                                        hipStream_t);                                                            \
+    template void launch_bicg_spmv<Op, T>(const Op&, const T*, const T*, T*, T*, const Sites<T>&,               \
+                                          const SolverState*, hipStream_t);                                      \
+    template void launch_cgs_spmv_v<Op, T>(const Op&, const T*, T*, const T*, const Sites<T>&,                  \
+                                           const SolverState*, hipStream_t);                                     \
+    template void launch_cgs_spmv_r<Op, T>(int, const Op&, const T*, T*, T*, const T*, const Sites<T>&,         \
+                                           const SolverState*, hipStream_t);                                     \
+    template void launch_spmv<Op, T>(const Op&, const T*, T*, int, hipStream_t);
 
 TF_INSTANTIATE(double)
 TF_INSTANTIATE(float)
+TF_INSTANTIATE_OP(Csr<double>, double)
+TF_INSTANTIATE_OP(Csr<float>, float)
+TF_INSTANTIATE_OP(DiffCsr32, float)
 
 }  // namespace tflows

@@ -1,0 +1,374 @@
+// mixed.hip -- FP32 Krylov solves inside FP64 iterative refinement.
+//
+// The reference has only FP64 solves (every build line uses
+// -fdefault-real-8, Process/Makefile:111, 130).  This component keeps the
+// reference algorithm for the inner solve (the same CG / BiCG / CGS
+// kernels, instantiated for float: SpMV, preconditioner, vector updates and
+// dot products in FP32 with FP32 accumulation) and wraps it in an FP64
+// refinement loop in which the true residual b - A x, the update of x and
+// every convergence norm stay FP64:
+//
+//   r = b - A x                                   (FP64, Resid form)
+//   while ||r|| >= tol:
+//       s = ||r||;  r32 = fp32(r / s)             (scaled to unit norm)
+//       solve A32 d = r32 with the FP32 Krylov loop to ||r32_k|| < eps
+//       x = x + s * fp64(d)                       (FP64)
+//       r = b - A x                               (FP64)
+//
+// The stopping test is the reference's absolute ||b - A x||_2 < tol, here
+// evaluated on the true residual rather than on a recurrence.
+//
+// The FP32 SpMVs use the difference form of A (kernels.h, DiffCsr32): the
+// pressure-correction matrix is singular up to one pinned cell, and the
+// plain FP32 product loses the near-null mode (rounding of a_ii against
+// sum a_ij exceeds its eigenvalue), which makes the refinement diverge.
+#include <cmath>
+#include <vector>
+
+#include "system.h"
+
+namespace tflows {
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_norm2(int n, const T* __restrict__ v, T* __restrict__ partials) {
+    const int ntiles = (n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    const int rb = t0 * kTileRows, re = min(t1 * kTileRows, n);
+    T a[1] = {T(0)};
+    for (int i = rb + threadIdx.x; i < re; i += kBlock) a[0] += v[i] * v[i];
+    site_write<T, 1>(a, partials);
+}
+
+// r32 = fp32(r64 * inv_s); x32 = 0
+__global__ __launch_bounds__(kBlock) void k_scale_to_f32(int n, const double* __restrict__ r64, double inv_s,
+                                                         float* __restrict__ r32, float* __restrict__ x32) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        r32[i] = (float)(r64[i] * inv_s);
+        x32[i] = 0.0f;
+    }
+}
+
+// x64 += s * fp64(d32);  r64 = b (ready for the Resid pass)
+__global__ __launch_bounds__(kBlock) void k_refine(int n, double* __restrict__ x64, double s,
+                                                   const float* __restrict__ d32, const double* __restrict__ b,
+                                                   double* __restrict__ r64) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        x64[i] = x64[i] + s * (double)d32[i];
+        r64[i] = b[i];
+    }
+}
+
+// Build the FP32 difference-form operator from the FP64 CSR, one thread per
+// row: off-diagonal entries keep their column order; rs_i is the FP64 sum of
+// the row's values in column order, rounded once; diag32_i = fp32(a_ii).
+__global__ __launch_bounds__(kBlock) void k_build_diff(int n, const int* __restrict__ rowptr,
+                                                       const int* __restrict__ col, const int* __restrict__ dia,
+                                                       const double* __restrict__ val, int* __restrict__ off_rowptr,
+                                                       int* __restrict__ off_col, float* __restrict__ off_val,
+                                                       float* __restrict__ rs, float* __restrict__ diag32) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int b = rowptr[i], e = rowptr[i + 1], d = dia[i];
+    const int ob = b - i;                 // one diagonal entry per earlier row
+    off_rowptr[i] = ob;
+    if (i == n - 1) off_rowptr[n] = e - n;
+    double sum = 0.0;
+    int o = ob;
+    for (int j = b; j < e; ++j) {
+        const double a = val[j];
+        sum += a;
+        if (j == d) continue;
+        off_col[o] = col[j];
+        off_val[o] = (float)a;
+        ++o;
+    }
+    rs[i] = (float)sum;
+    diag32[i] = (float)val[d];
+}
+
+__global__ __launch_bounds__(kBlock) void k_f64_to_f32(int n, const double* __restrict__ a, float* __restrict__ b) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_f32(int n, float v, float* __restrict__ b) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) b[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(int n, const double* __restrict__ a, double* __restrict__ b) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// FP64 refinement update of the reliable-update form: x64 += fp64(x32);
+// x32 = 0.
+__global__ __launch_bounds__(kBlock) void k_rel_merge(int n, double* __restrict__ x64, float* __restrict__ x32) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        x64[i] = x64[i] + (double)x32[i];
+        x32[i] = 0.0f;
+    }
+}
+
+// FP64 residual replacement: r = b - A x64 (Resid form, FP64), then
+// r32 = fp32(r), z32 = r32/D32, with (r32,r32), (r32,z32) -> site B (FP32)
+// and (r,r) -> site E (FP64).  The leader clears `suspend` and marks the
+// next iteration as the one whose stopping test uses the FP64 norm.
+struct RelEpi {
+    float* __restrict__ r32;
+    float* __restrict__ z32;
+    const float* __restrict__ D32;
+    double* __restrict__ r64;
+    float a32[2];
+    double a64[1];
+    __device__ void operator()(int row, double (&y)[1]) {
+        const double ri = y[0];
+        if (r64) r64[row] = ri;
+        const float rf = (float)ri;
+        const float zf = rf / D32[row];
+        r32[row] = rf;
+        z32[row] = zf;
+        a32[0] += rf * rf;
+        a32[1] += rf * zf;
+        a64[0] += ri * ri;
+    }
+};
+
+__global__ __launch_bounds__(kBlock) void k_rel_resid(Csr<double> A, const double* __restrict__ x64,
+                                                      const double* __restrict__ b, float* __restrict__ r32,
+                                                      float* __restrict__ z32, const float* __restrict__ D32,
+                                                      double* __restrict__ r64, float* __restrict__ partB,
+                                                      double* __restrict__ partE, SolverState* st) {
+    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    Gather1<double> g{x64};
+    InitVec<double> init{b};
+    RelEpi epi{r32, z32, D32, r64, {0.0f, 0.0f}, {0.0}};
+    for (int t = t0; t < t1; ++t)
+        stream_tile<double, 1, true>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi);
+    site_write<float, 2>(epi.a32, partB);
+    site_write<double, 1>(epi.a64, partE);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->suspend = 0;
+        st->replaced_before = st->it + 1;
+    }
+}
+
+// Sum of the block partials on the host, in block order.
+static double host_site_sum(const double* d_partials, int grid, hipStream_t s) {
+    std::vector<double> h(grid);
+    TF_CHECK(hipMemcpyAsync(h.data(), d_partials, sizeof(double) * grid, hipMemcpyDeviceToHost, s));
+    TF_CHECK(hipStreamSynchronize(s));
+    double acc = 0.0;
+    for (double v : h) acc += v;
+    return acc;
+}
+
+DiffCsr32 System::diff32(hipStream_t s) {
+    if (!diff32_valid_) {
+        off_rowptr_.ensure(n_ + 1);
+        off_col_.ensure(nnz_ - n_ > 0 ? nnz_ - n_ : 1);
+        off_val_.ensure(nnz_ - n_ > 0 ? nnz_ - n_ : 1);
+        rowsum32_.ensure(n_);
+        diag32_.ensure(n_);
+        hipLaunchKernelGGL(k_build_diff, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n_, rowptr_.p, col_.p,
+                           dia_.p, (const double*)val_.p, off_rowptr_.p, off_col_.p, off_val_.p, rowsum32_.p,
+                           diag32_.p);
+        TF_CHECK(hipGetLastError());
+        diff32_valid_ = true;
+    }
+    return DiffCsr32{n_, off_rowptr_.p, off_col_.p, off_val_.p, rowsum32_.p};
+}
+
+const float* System::diag32(hipStream_t s) {
+    diff32(s);
+    return diag32_.p;
+}
+
+double System::true_residual(const double* x, const double* b, double* r, hipStream_t s) {
+    const int nb = (n_ + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(kBlock), 0, s, n_, b, r);
+    launch_resid(csr(), x, r, grid_, s);
+    hipLaunchKernelGGL((k_norm2<double>), dim3(grid_), dim3(kBlock), 0, s, n_, (const double*)r, partials_.p);
+    TF_CHECK(hipGetLastError());
+    return std::sqrt(host_site_sum(partials_.p, grid_, s));
+}
+
+SolveResult System::solve_mixed(int method, int prec, double* x, const double* b, double* r_out, int cap,
+                                double tol, double inner_rtol, int max_outer, int* outer_steps, hipStream_t s) {
+    if (method < 0 || method > 2) throw std::invalid_argument("method must be 0 (cg), 1 (bicg) or 2 (cgs)");
+    if (!(inner_rtol > 0.0 && inner_rtol < 1.0)) throw std::invalid_argument("inner_rtol must be in (0, 1)");
+    const size_t n = (size_t)n_;
+    ws32_.ensure((V_COUNT + 2) * n);
+    partials32_.ensure(Sites<float>::elems(grid_));
+    state32_.ensure(1);
+    r64_.ensure(n);
+    if (!mirror32_)
+        TF_CHECK(hipHostMalloc((void**)&mirror32_, sizeof(HostMirror), hipHostMallocMapped | hipHostMallocCoherent));
+    ws_.ensure((size_t)V_COUNT * n);
+
+    const DiffCsr32 Aop = diff32(s);
+    // the DIC sweeps of an FP32 inner solve read the plainly rounded
+    // off-diagonal values; the operator of its SpMVs is the difference form
+    const Csr<float> A32 = prec == 2 ? csr32(s) : Csr<float>{n_, rowptr_.p, col_.p, dia_.p, nullptr};
+    KrylovVectors<float> w;
+    float* base = ws32_.p;
+    auto vec = [&](int k) { return base + (size_t)k * n; };
+    w.D = vec(V_D); w.p = vec(V_P); w.q = vec(V_Q); w.r2 = vec(V_R2); w.p2 = vec(V_P2); w.q2 = vec(V_Q2);
+    w.u1 = vec(V_U1); w.u2 = vec(V_U2); w.v2 = vec(V_V2);
+    w.x = vec(V_COUNT);
+    w.r1 = vec(V_COUNT + 1);
+    TF_CHECK(hipMemsetAsync(ws32_.p, 0, sizeof(float) * (V_COUNT + 2) * n, s));
+    Sites<float> S32 = Sites<float>::carve(partials32_.p, grid_);
+    static const Levels kNoLevels;
+    const Levels* L = prec == 2 ? &levels() : &kNoLevels;
+    const int nb = (n_ + kBlock - 1) / kBlock;
+    // the FP32 preconditioner: fp32(diag) for Jacobi, 1 for none, and the
+    // DIC pivots computed in FP64 (Prec1.f90:51-59) then rounded
+    if (prec == 1) {
+        TF_CHECK(hipMemcpyAsync(w.D, diag32(s), sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    } else if (prec == 2) {
+        double* d64 = ws_.p;                        // FP64 scratch slot
+        dic_setup(d64, s);
+        hipLaunchKernelGGL(k_f64_to_f32, dim3(nb), dim3(kBlock), 0, s, n_, (const double*)d64, w.D);
+    } else {
+        hipLaunchKernelGGL(k_fill_f32, dim3(nb), dim3(kBlock), 0, s, n_, 1.0f, w.D);
+    }
+    TF_CHECK(hipGetLastError());
+
+    double* r64 = r_out ? r_out : r64_.p;
+    double err = true_residual(x, b, r64, s);
+    SolveResult res;
+    res.ini_res = err;
+    res.ini_written = true;
+    int total = 0, outer = 0;
+    while (err >= tol && outer < max_outer && total < cap) {
+        ++outer;
+        hipLaunchKernelGGL(k_scale_to_f32, dim3(nb), dim3(kBlock), 0, s, n_, (const double*)r64, 1.0 / err, w.r1,
+                           w.x);
+        SolverState h{};
+        h.tol = inner_rtol;            // ||r32|| = 1 up to rounding: relative target
+        h.cap = cap - total;
+        TF_CHECK(hipMemcpyAsync(state32_.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
+        run_solve<float, DiffCsr32>(method, prec, cap - total, Aop, A32, w, S32, state32_.p, mirror32_, L, s,
+                                    /*d_ready=*/true, /*zero_x=*/true);
+        SolverState h2;
+        TF_CHECK(hipMemcpy(&h2, state32_.p, sizeof(h2), hipMemcpyDeviceToHost));
+        // iterations actually executed: niter on convergence, cap on exhaustion
+        const int done_its = h2.niter_out > h.cap ? h.cap : h2.niter_out;
+        total += done_its;
+        hipLaunchKernelGGL(k_refine, dim3(nb), dim3(kBlock), 0, s, n_, x, err, (const float*)w.x, b, r64);
+        launch_resid(csr(), (const double*)x, r64, grid_, s);
+        hipLaunchKernelGGL((k_norm2<double>), dim3(grid_), dim3(kBlock), 0, s, n_, (const double*)r64,
+                           partials_.p);
+        TF_CHECK(hipGetLastError());
+        err = std::sqrt(host_site_sum(partials_.p, grid_, s));
+        if (done_its == 0) break;      // the inner solve could not make progress
+    }
+    if (outer_steps) *outer_steps = outer;
+    res.niter = err < tol ? total : total + 1;
+    res.fin_res = err;
+    return res;
+}
+
+// Mixed precision with reliable updates (residual replacement): one FP32
+// CG recurrence runs across the whole solve, and every `interval`
+// iterations (and whenever its recurrence residual reaches tol) an FP64
+// refinement step merges the FP32 correction into x (FP64), recomputes the
+// true residual b - A x in FP64 and restarts the FP32 residual from it.
+// The solve stops when the FP64 true residual satisfies ||b - A x|| < tol.
+// Unlike restarted refinement the Krylov directions p survive each
+// refinement step, so the iteration count stays close to the FP64 CG's.
+SolveResult System::solve_mixed_reliable(int prec, double* x, const double* b, double* r_out, int cap, double tol,
+                                         int interval, int* outer_steps, hipStream_t s) {
+    if (prec == 2) throw std::invalid_argument("reliable-update mode supports prec 0 and 1");
+    if (interval < 1) interval = 1;
+    const size_t n = (size_t)n_;
+    ws32_.ensure((V_COUNT + 2) * n);
+    partials32_.ensure(Sites<float>::elems(grid_));
+    state32_.ensure(1);
+    if (!mirror32_)
+        TF_CHECK(hipHostMalloc((void**)&mirror32_, sizeof(HostMirror), hipHostMallocMapped | hipHostMallocCoherent));
+    const DiffCsr32 Aop = diff32(s);
+    float* base = ws32_.p;
+    auto vec = [&](int k) { return base + (size_t)k * n; };
+    float *D32 = vec(V_D), *p0 = vec(V_P), *p1 = vec(V_P2), *q = vec(V_Q), *z = vec(V_U1);
+    float *x32 = vec(V_COUNT), *r32 = vec(V_COUNT + 1);
+    const int nb = (n_ + kBlock - 1) / kBlock;
+    if (prec == 1) TF_CHECK(hipMemcpyAsync(D32, diag32(s), sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    else hipLaunchKernelGGL(k_fill_f32, dim3(nb), dim3(kBlock), 0, s, n_, 1.0f, D32);
+    hipLaunchKernelGGL(k_fill_f32, dim3(nb), dim3(kBlock), 0, s, n_, 0.0f, x32);
+    Sites<float> S32 = Sites<float>::carve(partials32_.p, grid_);
+    double* partE = partials_.p + Sites<double>::elems(grid_);
+    SolverState h{};
+    h.tol = tol;
+    h.cap = cap;
+    TF_CHECK(hipMemcpyAsync(state32_.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    mirror32_->done = 0;
+    mirror32_->iter = 0;
+    const Csr<double> A = csr();
+    auto replace = [&](double* rout) {
+        hipLaunchKernelGGL(k_rel_merge, dim3(nb), dim3(kBlock), 0, s, n_, x, x32);
+        hipLaunchKernelGGL(k_rel_resid, dim3(grid_), dim3(kBlock), 0, s, A, (const double*)x, b, r32, z, D32, rout,
+                           S32.B, partE, state32_.p);
+    };
+    replace(nullptr);
+    TF_CHECK(hipGetLastError());
+
+    hipEvent_t ev[2];
+    TF_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    TF_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    bool pending[2] = {false, false};
+    int slot = 0, issued = 0, replacements = 1;
+    bool done = false;
+    const long long guard = 2LL * (cap > 0 ? cap : 0) + 2LL * interval;
+    while (!done && issued <= guard) {
+        for (int i = 0; i < interval; ++i) {
+            if (cg_form() == 2) {
+                launch_cgf_spmv(true, Aop, (const float*)z, p0, p1, q, S32, (const double*)partE, state32_.p,
+                                mirror32_, s);
+                launch_cgf_update(n_, x32, r32, z, (const float*)p0, (const float*)p1, (const float*)q,
+                                  (const float*)D32, S32, state32_.p, s);
+            } else {
+                launch_cgj(true, Aop, n_, x32, r32, z, p0, q, (const float*)D32, S32, (const double*)partE,
+                           state32_.p, mirror32_, s);
+            }
+        }
+        issued += interval;
+        replace(nullptr);
+        ++replacements;
+        launch_cgf_check(true, S32, (const double*)partE, state32_.p, mirror32_, s);
+        TF_CHECK(hipGetLastError());
+        TF_CHECK(hipEventRecord(ev[slot], s));
+        pending[slot] = true;
+        const int other = slot ^ 1;
+        if (pending[other]) {
+            TF_CHECK(hipEventSynchronize(ev[other]));
+            pending[other] = false;
+            done = mirror32_->done != 0;
+        }
+        slot = other;
+    }
+    // final merge and FP64 residual (also the caller's r_out)
+    replace(r_out);
+    TF_CHECK(hipGetLastError());
+    SolverState h2;
+    TF_CHECK(hipMemcpyAsync(&h2, state32_.p, sizeof(h2), hipMemcpyDeviceToHost, s));
+    const double fin = std::sqrt(host_site_sum(partE, grid_, s));
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    SolveResult res;
+    res.ini_res = h2.ini_res;
+    res.ini_written = h2.ini_written != 0;
+    res.fin_res = fin;
+    res.niter = fin < tol ? h2.it : h2.it + 1;
+    if (outer_steps) *outer_steps = replacements;
+    return res;
+}
+
+}  // namespace tflows

@@ -83,11 +83,26 @@ class System {
 
     Csr<double> csr() const { return Csr<double>{n_, rowptr_.p, col_.p, dia_.p, val_.p}; }
     Csr<float> csr32(hipStream_t s) { return Csr<float>{n_, rowptr_.p, col_.p, dia_.p, values32(s)}; }
+    // FP32 difference-form operator (off-diagonal CSR + FP64 row sums) and
+    // the FP32 diagonal, made on first use after set_values (mixed.hip)
+    DiffCsr32 diff32(hipStream_t s);
+    const float* diag32(hipStream_t s);
 
     // Solve with x, r1 device pointers (x(1:N), r1(1:N)); `stale_error` is
     // the FinRes the early exit hands back.
     SolveResult solve(int method, int prec, double* x, double* r1, int cap, double tol, double stale_error,
                       hipStream_t s);
+
+    // FP32 Krylov inside FP64 iterative refinement (mixed.hip).  b is not
+    // modified; r_out (may be null) receives the final true residual.
+    SolveResult solve_mixed(int method, int prec, double* x, const double* b, double* r_out, int cap, double tol,
+                            double inner_rtol, int max_outer, int* outer_steps, hipStream_t s);
+    // the same with reliable updates (residual replacement every `interval`
+    // FP32 CG iterations); CG with prec 0 / 1
+    SolveResult solve_mixed_reliable(int prec, double* x, const double* b, double* r_out, int cap, double tol,
+                                     int interval, int* outer_steps, hipStream_t s);
+    // r = b - A x (FP64) and its norm
+    double true_residual(const double* x, const double* b, double* r, hipStream_t s);
 
     void prec1(int prec, double* d, hipStream_t s);
     void spmv(const double* x, double* y, hipStream_t s);
@@ -121,20 +136,35 @@ class System {
     DevArray<double> partials_;
     DevArray<SolverState> state_;
     HostMirror* mirror_ = nullptr;  // coherent pinned host memory
+    DevArray<float> ws32_;          // FP32 workspace of the mixed solver
+    DevArray<float> partials32_;
+    DevArray<SolverState> state32_;
+    HostMirror* mirror32_ = nullptr;
+    DevArray<double> r64_;
+    DevArray<int> off_rowptr_, off_col_;
+    DevArray<float> off_val_, rowsum32_, diag32_;
+    bool diff32_valid_ = false;
     std::vector<int> h_rowptr_, h_col_, h_dia_;
     Levels levels_;
     bool levels_ready_ = false;
     int grid_;
 };
 
+int max_blocks_from_env();
+bool fused_cg_enabled();
+int cg_form();
+template <typename T, typename Op>
+void run_solve(int method, int prec, int cap, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
+               const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s, bool d_ready,
+               bool zero_x);
+template <typename T, typename Op>
+void pace_iterations(int method, bool jac, int cap, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
+                     const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s);
+template <typename T, typename Op>
+void enqueue_iteration(int method, bool jac, int k, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
+                       const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s);
 template <typename T>
 void dic_apply_levels(const Levels& L, const Csr<T>& A, const T* D, const T* u, T* z, const SolverState* st,
                       hipStream_t s);
-template <typename T>
-void enqueue_iteration(int method, bool jac, int k, const Csr<T>& A, const KrylovVectors<T>& w, const Sites<T>& S,
-                       SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s);
-template <typename T>
-void pace_iterations(int method, bool jac, int cap, const Csr<T>& A, const KrylovVectors<T>& w, const Sites<T>& S,
-                     SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s);
 
 }  // namespace tflows

@@ -6,14 +6,36 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <cstdlib>
 #include <thread>
 
 namespace tflows {
 
+// Upper bound on the blocks of the row-streaming kernels (default 2048 =
+// 8 per CU); TFLOWS_MAX_BLOCKS overrides it for tuning.
+// Jacobi CG kernel form of the FP64 solve: 0 (default, fastest measured) =
+// host-indexed p update, SpMV, update; 3 = the same three kernels with
+// device iteration counters (the form the mixed-precision path uses);
+// 2 = p formed inside the SpMV gathers.  TFLOWS_CG_FORM selects.
+int cg_form() {
+    static const int f = [] {
+        const char* e = std::getenv("TFLOWS_CG_FORM");
+        return e ? std::atoi(e) : 0;
+    }();
+    return f;
+}
+bool fused_cg_enabled() { return cg_form() != 0; }
+
+int max_blocks_from_env() {
+    const char* e = std::getenv("TFLOWS_MAX_BLOCKS");
+    int v = e ? std::atoi(e) : 2048;
+    return v >= 8 ? v : 2048;
+}
+
 
 System::System(int n, int nonz, const int* acol, const int* arow, const int* adia, bool device_ptrs,
                hipStream_t s)
-    : n_(n), nnz_(nonz - 1), grid_(stream_grid(n)) {
+    : n_(n), nnz_(nonz - 1), grid_(stream_grid(n, max_blocks_from_env())) {
     if (n <= 0) throw std::invalid_argument("N must be positive");
     if (nonz < n + 1) throw std::invalid_argument("NONZ must be at least N + 1 (diagonal entries + 1)");
     rowptr_.ensure(n_ + 1);
@@ -54,7 +76,7 @@ System::System(int n, int nonz, const int* acol, const int* arow, const int* adi
     TF_CHECK(hipMemcpyAsync(dia_.p, h_dia_.data(), sizeof(int) * n_, hipMemcpyHostToDevice, s));
     val_.ensure(nnz_);
     TF_CHECK(hipMemsetAsync(val_.p, 0, sizeof(double) * nnz_, s));
-    partials_.ensure(Sites<double>::elems(grid_));
+    partials_.ensure(Sites<double>::elems(grid_) + 2 * (size_t)grid_);  // + site E
     state_.ensure(1);
     TF_CHECK(hipMemsetAsync(state_.p, 0, sizeof(SolverState), s));
     TF_CHECK(hipHostMalloc((void**)&mirror_, sizeof(HostMirror), hipHostMallocMapped | hipHostMallocCoherent));
@@ -65,12 +87,14 @@ System::System(int n, int nonz, const int* acol, const int* arow, const int* adi
 
 System::~System() {
     if (mirror_) (void)hipHostFree(mirror_);
+    if (mirror32_) (void)hipHostFree(mirror32_);
 }
 
 void System::set_values(const double* aval, bool device_ptr, hipStream_t s) {
     TF_CHECK(hipMemcpyAsync(val_.p, aval, sizeof(double) * nnz_,
                             device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     val32_valid_ = false;
+    diff32_valid_ = false;
     if (!device_ptr) TF_CHECK(hipStreamSynchronize(s));
 }
 
@@ -162,7 +186,7 @@ void System::spmv(const double* x, double* y, hipStream_t s) {
     TF_CHECK(hipGetLastError());
 }
 void System::spmv32(const float* x, float* y, hipStream_t s) {
-    launch_spmv(csr32(s), x, y, grid_, s);
+    launch_spmv(diff32(s), x, y, grid_, s);
     TF_CHECK(hipGetLastError());
 }
 void System::resid(const double* x, double* r, hipStream_t s) {
@@ -172,19 +196,30 @@ void System::resid(const double* x, double* r, hipStream_t s) {
 
 // One iteration k of `method`, enqueued (CG.f90:113-259, BiCG.f90:100-251,
 // CGS.f90:112-334).
-template <typename T>
-void enqueue_iteration(int method, bool jac, int k, const Csr<T>& A, const KrylovVectors<T>& w, const Sites<T>& S,
-                       SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s) {
+template <typename T, typename Op>
+void enqueue_iteration(int method, bool jac, int k, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
+                       const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s) {
     const int n = A.n;
     switch (method) {
         case 0: {  // CG
+            if (jac && cg_form() == 2) {  // p0 = p, p1 = p2, z = u1
+                launch_cgf_spmv(false, Aop, (const T*)w.u1, w.p, w.p2, w.q, S, (const double*)nullptr, st, hm, s);
+                launch_cgf_update(n, w.x, w.r1, w.u1, (const T*)w.p, (const T*)w.p2, (const T*)w.q,
+                                  (const T*)w.D, S, st, s);
+                break;
+            }
+            if (jac && cg_form() == 3) {
+                launch_cgj(false, Aop, n, w.x, w.r1, w.u1, w.p, w.q, (const T*)w.D, S, (const double*)nullptr, st,
+                           hm, s);
+                break;
+            }
             if (!jac) {  // stopping test, z = M^-1 r into q, rho = (r, z)
                 launch_check(k, S, st, hm, s);
                 dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.r1, w.q, st, s);
                 launch_rho(n, (const T*)w.r1, (const T*)w.q, S, st, s);
             }
             launch_cg_p(jac, k, n, (const T*)w.r1, (const T*)w.D, (const T*)w.q, w.p, S, st, hm, s);
-            launch_cg_spmv(A, (const T*)w.p, w.q, S, st, s);
+            launch_cg_spmv(Aop, (const T*)w.p, w.q, S, st, s);
             launch_cg_update(jac, k, n, w.x, w.r1, (const T*)w.p, (const T*)w.q, (const T*)w.D, S, st, s);
             break;
         }
@@ -197,7 +232,7 @@ void enqueue_iteration(int method, bool jac, int k, const Csr<T>& A, const Krylo
             }
             launch_bicg_p(jac, k, n, (const T*)w.r1, (const T*)w.r2, (const T*)w.D, (const T*)w.q, (const T*)w.q2,
                           w.p, w.p2, S, st, hm, s);
-            launch_bicg_spmv(A, (const T*)w.p, (const T*)w.p2, w.q, w.q2, S, st, s);
+            launch_bicg_spmv(Aop, (const T*)w.p, (const T*)w.p2, w.q, w.q2, S, st, s);
             launch_bicg_update(jac, k, n, w.x, w.r1, w.r2, (const T*)w.p, (const T*)w.q, (const T*)w.q2,
                                (const T*)w.D, S, st, s);
             break;
@@ -205,51 +240,88 @@ void enqueue_iteration(int method, bool jac, int k, const Csr<T>& A, const Krylo
         default: {  // CGS: u1, u2, p2 = M^-1 u2, v2 = A p2, q1 (q), p1 (p) = M^-1 (u1+q1)
             launch_cgs_u(jac, k, n, (const T*)w.r1, (const T*)w.q, w.u1, w.u2, (const T*)w.D, w.p2, S, st, hm, s);
             if (!jac) dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.u2, w.p2, st, s);
-            launch_cgs_spmv_v(A, (const T*)w.p2, w.v2, (const T*)w.r2, S, st, s);
+            launch_cgs_spmv_v(Aop, (const T*)w.p2, w.v2, (const T*)w.r2, S, st, s);
             launch_cgs_q(jac, k, n, (const T*)w.u1, (const T*)w.v2, w.q, (const T*)w.D, w.p, S, st, s);
             if (!jac) dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.p, w.p, st, s);
-            launch_cgs_spmv_r(k, A, (const T*)w.p, w.x, w.r1, (const T*)w.r2, S, st, s);
+            launch_cgs_spmv_r(k, Aop, (const T*)w.p, w.x, w.r1, (const T*)w.r2, S, st, s);
             break;
         }
     }
 }
 
 // Keep the stream supplied with iterations while the device runs them.
-// The device publishes its progress into coherent host memory; the host
-// stays at most `lookahead` iterations ahead and stops once `done` shows
-// (kernels queued past convergence return immediately).  After iteration
-// `cap` a last stopping test (k = cap + 1) closes the loop.  If the mirror
-// stops moving while the stream is idle, the device state is read
-// directly, so the loop always ends.
-template <typename T>
-void pace_iterations(int method, bool jac, int cap, const Csr<T>& A, const KrylovVectors<T>& w, const Sites<T>& S,
-                     SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s) {
-    const int per_iter = jac ? 4 : 6 + 2 * (L->nfwd + L->nbwd) * (method == 0 ? 1 : 2);
-    const int lookahead = std::max(2, std::min(64, 4096 / std::max(1, per_iter)));
-    int issued = 0, completed_floor = 0;
-    while (issued < cap) {
-        if (hm->done) return;
-        const int completed = std::max((int)hm->iter, completed_floor);
-        if (issued - completed < lookahead) {
-            const int k = std::min(std::max(1, lookahead / 2), cap - issued);
-            for (int i = 0; i < k; ++i) {
-                ++issued;
-                enqueue_iteration<T>(method, jac, issued, A, w, S, st, hm, L, s);
-            }
-            TF_CHECK(hipGetLastError());
-            continue;
+// Iterations go out in chunks that double in size (so short solves stop
+// early and long ones keep the queue deep); after queueing chunk j the host
+// waits for the event of chunk j-1 and reads the `done` flag the device
+// published, so one chunk is always queued behind the running one.  Kernels
+// queued past convergence return at once.  After iteration `cap` a last
+// stopping test (k = cap + 1) closes the loop.
+template <typename T, typename Op>
+void pace_iterations(int method, bool jac, int cap, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
+                     const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s) {
+    hipEvent_t ev[2];
+    TF_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    TF_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    bool pending[2] = {false, false};
+    const int max_chunk = jac ? 64 : 2;
+    int chunk = jac ? 4 : 1, slot = 0, issued = 0;
+    bool done = false;
+    while (issued < cap && !done) {
+        const int k = std::min(chunk, cap - issued);
+        for (int i = 0; i < k; ++i) {
+            ++issued;
+            enqueue_iteration<T, Op>(method, jac, issued, Aop, A, w, S, st, hm, L, s);
         }
-        if (hipStreamQuery(s) == hipSuccess) {
-            SolverState h;
-            TF_CHECK(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
-            if (h.done) return;
-            completed_floor = issued;  // idle stream: everything issued has run
-            continue;
+        TF_CHECK(hipGetLastError());
+        TF_CHECK(hipEventRecord(ev[slot], s));
+        pending[slot] = true;
+        const int other = slot ^ 1;
+        if (pending[other]) {
+            TF_CHECK(hipEventSynchronize(ev[other]));
+            pending[other] = false;
+            done = hm->done != 0;
         }
-        std::this_thread::yield();
+        slot = other;
+        chunk = std::min(max_chunk, chunk * 2);
     }
-    launch_check(std::max(cap, 0) + 1, S, st, hm, s);
+    if (!done) {
+        if (method == 0 && jac && fused_cg_enabled())
+            launch_cgf_check(false, S, (const double*)nullptr, st, hm, s);
+        else
+            launch_check(std::max(cap, 0) + 1, S, st, hm, s);
+    }
     TF_CHECK(hipGetLastError());
+    TF_CHECK(hipStreamSynchronize(s));
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+}
+
+// Set-up, initial residual and iterations of one solve in precision T.
+// Aop is the operator of the SpMVs; A the CSR the DIC sweeps read.  With
+// d_ready the caller has filled w.D; with zero_x the initial guess is 0 and
+// the initial residual is b without a matrix pass.  The caller has written
+// the initial SolverState (tol, cap, stale error).
+template <typename T, typename Op>
+void run_solve(int method, int prec, int cap, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
+               const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s, bool d_ready,
+               bool zero_x) {
+    const bool jac = prec != 2;
+    hm->done = 0;
+    hm->iter = 0;
+    if (prec == 2 && !d_ready)
+        for (int l = 0; l < L->nfwd; ++l)
+            launch_dic_setup_level(A, L->fwd_rows.p + L->fwd_ptr[l], L->fwd_ptr[l + 1] - L->fwd_ptr[l], w.D, s);
+    launch_setup(A, prec, jac && !d_ready, w.D, (const T*)w.r1, S, s);
+    int mode = RHO_NONE_;
+    T* r2 = nullptr;
+    const bool fused = method == 0 && jac && fused_cg_enabled();
+    if (method == 0) mode = (jac && !fused) ? RHO_JAC_ : RHO_NONE_;
+    if (method == 1) { mode = jac ? RHO_BICG_ : RHO_NONE_; r2 = w.r2; }
+    if (method == 2) { mode = RHO_RR_; r2 = w.r2; }
+    launch_resid_init(mode, zero_x, A, (const T*)w.x, w.r1, r2, (const T*)w.D, S, st, hm, s);
+    if (fused) launch_cgf_init_z(A.n, (const T*)w.r1, (const T*)w.D, w.u1, S, st, s);
+    TF_CHECK(hipGetLastError());
+    pace_iterations<T, Op>(method, jac, cap, Aop, A, w, S, st, hm, L, s);
 }
 
 SolveResult System::solve(int method, int prec, double* x, double* r1, int cap, double tol, double stale_error,
@@ -269,22 +341,10 @@ SolveResult System::solve(int method, int prec, double* x, double* r1, int cap, 
     mirror_->done = 0;
     mirror_->iter = 0;
 
-    const bool jac = prec != 2;
-    Csr<double> A = csr();
-    Sites<double> S = sites();
-    if (prec == 2) dic_setup(w.D, s);
-    launch_setup(A, prec, jac, w.D, (const double*)r1, S, s);
-    int mode = RHO_NONE_;
-    double* r2 = nullptr;
-    if (method == 0) mode = jac ? RHO_JAC_ : RHO_NONE_;
-    if (method == 1) { mode = jac ? RHO_BICG_ : RHO_NONE_; r2 = w.r2; }
-    if (method == 2) { mode = RHO_RR_; r2 = w.r2; }
-    launch_resid_init(mode, A, (const double*)x, r1, r2, (const double*)w.D, S, state_.p, mirror_, s);
-    TF_CHECK(hipGetLastError());
-    const Levels* L = jac ? nullptr : &levels();
     static const Levels kNoLevels;
-    pace_iterations<double>(method, jac, cap, A, w, S, state_.p, mirror_, L ? L : &kNoLevels, s);
-    TF_CHECK(hipStreamSynchronize(s));
+    const Levels* L = prec == 2 ? &levels() : &kNoLevels;
+    const Csr<double> A = csr();
+    run_solve<double, Csr<double>>(method, prec, cap, A, A, w, sites(), state_.p, mirror_, L, s, false, false);
     SolverState h2;
     TF_CHECK(hipMemcpy(&h2, state_.p, sizeof(h2), hipMemcpyDeviceToHost));
     if (!h2.done) throw HipError("solver finished without reaching a stopping state");
@@ -314,18 +374,18 @@ KrylovVectors<double> System::vectors(double* x, double* r1) {
     return w;
 }
 
-template void enqueue_iteration<double>(int, bool, int, const Csr<double>&, const KrylovVectors<double>&,
-                                        const Sites<double>&, SolverState*, HostMirror*, const Levels*,
-                                        hipStream_t);
-template void enqueue_iteration<float>(int, bool, int, const Csr<float>&, const KrylovVectors<float>&,
-                                       const Sites<float>&, SolverState*, HostMirror*, const Levels*,
-                                       hipStream_t);
-template void pace_iterations<double>(int, bool, int, const Csr<double>&, const KrylovVectors<double>&,
-                                      const Sites<double>&, SolverState*, HostMirror*, const Levels*,
-                                      hipStream_t);
-template void pace_iterations<float>(int, bool, int, const Csr<float>&, const KrylovVectors<float>&,
-                                     const Sites<float>&, SolverState*, HostMirror*, const Levels*,
-                                     hipStream_t);
+#define TF_SOLVE_INST(T, Op)                                                                                      \
+    template void enqueue_iteration<T, Op>(int, bool, int, const Op&, const Csr<T>&, const KrylovVectors<T>&,     \
+                                           const Sites<T>&, SolverState*, HostMirror*, const Levels*,             \
+                                           hipStream_t);                                                          \
+    template void pace_iterations<T, Op>(int, bool, int, const Op&, const Csr<T>&, const KrylovVectors<T>&,       \
+                                         const Sites<T>&, SolverState*, HostMirror*, const Levels*, hipStream_t); \
+    template void run_solve<T, Op>(int, int, int, const Op&, const Csr<T>&, const KrylovVectors<T>&,              \
+                                   const Sites<T>&, SolverState*, HostMirror*, const Levels*, hipStream_t, bool, \
+                                   bool);
+TF_SOLVE_INST(double, Csr<double>)
+TF_SOLVE_INST(float, Csr<float>)
+TF_SOLVE_INST(float, DiffCsr32)
 template void dic_apply_levels<double>(const Levels&, const Csr<double>&, const double*, const double*,
                                        double*, const SolverState*, hipStream_t);
 template void dic_apply_levels<float>(const Levels&, const Csr<float>&, const float*, const float*, float*,

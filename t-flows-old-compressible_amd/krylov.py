@@ -100,7 +100,11 @@ def _bind_extensions(lib):
         f = lib.tflows_solve_mixed
         f.restype = C.c_int
         f.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, _ip, C.c_double, C.c_double, C.c_int32,
-                      _dp, _dp, vp, vp]
+                      _dp, _dp, _ip, vp]
+    if hasattr(lib, "tflows_solve_mixed_reliable"):
+        f = lib.tflows_solve_mixed_reliable
+        f.restype = C.c_int
+        f.argtypes = [vp, C.c_int32, vp, vp, vp, _ip, C.c_double, C.c_int32, _dp, _dp, _ip, vp]
 
 
 def _check(rc: int):
@@ -225,6 +229,35 @@ class DeviceSystem:
                                      C.c_void_p(_ptr(r1)), C.byref(it), tol, C.byref(ini), C.byref(fin),
                                      _stream(stream)))
         return it.value, (None if ini.value == _SENTINEL else ini.value), fin.value
+
+    def solve_mixed(self, method: str, prec: int, x, b, niter: int, tol: float, inner_rtol: float = 1e-4,
+                    max_outer: int = 100, r_out=None, stream=None):
+        """FP32 Krylov inside FP64 iterative refinement.  x: cuda float64
+        tensor (guess in, solution out); b: cuda float64 tensor (unchanged).
+        Returns (niter_total_inner, IniRes, FinRes, outer_steps) with the
+        residual norms those of the FP64 true residual b - A x."""
+        it = C.c_int32(niter)
+        ini, fin = C.c_double(0.0), C.c_double(0.0)
+        outer = C.c_int32(0)
+        _check(self.lib.tflows_solve_mixed(self.handle, METHODS[method], prec, C.c_void_p(_ptr(x)),
+                                           C.c_void_p(_ptr(b)),
+                                           C.c_void_p(_ptr(r_out) if r_out is not None else 0), C.byref(it),
+                                           tol, inner_rtol, max_outer, C.byref(ini), C.byref(fin),
+                                           C.byref(outer), _stream(stream)))
+        return it.value, ini.value, fin.value, outer.value
+
+    def solve_mixed_reliable(self, prec: int, x, b, niter: int, tol: float, interval: int = 50, r_out=None,
+                             stream=None):
+        """CG in FP32 with FP64 residual replacement every `interval`
+        iterations; returns (niter, IniRes, FinRes, replacements)."""
+        it = C.c_int32(niter)
+        ini, fin = C.c_double(0.0), C.c_double(0.0)
+        rep = C.c_int32(0)
+        _check(self.lib.tflows_solve_mixed_reliable(
+            self.handle, prec, C.c_void_p(_ptr(x)), C.c_void_p(_ptr(b)),
+            C.c_void_p(_ptr(r_out) if r_out is not None else 0), C.byref(it), tol, interval, C.byref(ini),
+            C.byref(fin), C.byref(rep), _stream(stream)))
+        return it.value, ini.value, fin.value, rep.value
 
     def spmv(self, x, y, stream=None):
         _check(self.lib.tflows_spmv(self.handle, C.c_void_p(_ptr(x)), C.c_void_p(_ptr(y)), _stream(stream)))

@@ -1,0 +1,143 @@
+"""FP32-inner / FP64-refinement solves against the FP64 oracle.
+
+The mixed solver stops on the FP64 true residual ||b - A x|| < tol, so the
+pass bar is that test evaluated independently here, plus the distance of x
+to the oracle's FP64 solution, which both satisfy the same residual bound:
+||x_mixed - x_oracle|| <= ||A^-1|| (||r_mixed|| + ||r_oracle||).  The relative
+error against the oracle is printed per case.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _inv_norm2(m):
+    return 1.0 / np.linalg.svd(m.to_scipy().toarray(), compute_uv=False)[-1]
+
+
+@pytest.mark.parametrize("case,method,prec", [
+    ("pressure12", "cg", 1), ("pressure12", "cg", 2), ("pressure12", "cgs", 1),
+    ("pressure10_perm", "cg", 1), ("momentum10", "cgs", 1), ("momentum10", "cgs", 2),
+    ("pressure10_perm", "bicg", 1),
+])
+def test_mixed_matches_fp64_oracle(pkg, oracle, cuda, case, method, prec):
+    import torch
+    mg = pkg.meshgen
+    s = {"pressure12": lambda: mg.pressure_system(12, seed=51),
+         "pressure10_perm": lambda: mg.pressure_system(10, seed=52, perm_seed=53),
+         "momentum10": lambda: mg.momentum_system(10, seed=54)}[case]()
+    m = s.matrix
+    A = m.to_scipy()
+    ds = pkg.krylov.DeviceSystem(m)
+    b = torch.from_numpy(s.b.copy()).to(cuda)
+    x = torch.zeros(m.n, dtype=torch.float64, device=cuda)
+    r = torch.zeros_like(b)
+    it, ini, fin, outer = ds.solve_mixed(method, prec, x, b, 5000, s.tol, inner_rtol=1e-3, r_out=r)
+    xm = x.cpu().numpy()
+    true_r = np.linalg.norm(s.b - A @ xm)
+    assert np.array_equal(b.cpu().numpy(), s.b)           # b untouched
+    assert ini == pytest.approx(np.linalg.norm(s.b), rel=1e-12)
+    assert true_r < s.tol * (1 + 1e-9) and fin < s.tol
+    assert fin == pytest.approx(true_r, rel=1e-6)
+    assert np.allclose(r.cpu().numpy(), s.b - A @ xm, atol=1e-12 * np.linalg.norm(s.b))
+    ref = oracle.run(method, m, s.b, prec=prec, niter=5000, tol=s.tol)
+    assert ref["fin_res"] < s.tol
+    rel = np.linalg.norm(xm - ref["x"]) / np.linalg.norm(ref["x"])
+    rr = np.linalg.norm(s.b - A @ ref["x"])
+    bound = _inv_norm2(m) * (true_r + rr) / np.linalg.norm(ref["x"])
+    print(f"{case} {method} prec={prec}: inner={it} outer={outer} rel_err_vs_fp64={rel:.2e} bound={bound:.2e}")
+    assert rel <= bound
+    ds.close()
+
+
+def test_mixed_already_converged_and_cap(pkg, cuda):
+    import torch
+    s = pkg.meshgen.pressure_system(8, seed=55)
+    m = s.matrix
+    ds = pkg.krylov.DeviceSystem(m)
+    b = torch.from_numpy(s.b.copy()).to(cuda)
+    # exact solution as the initial guess: no refinement step is taken
+    xs = np.linalg.solve(m.to_scipy().toarray(), s.b)
+    x = torch.from_numpy(xs.copy()).to(cuda)
+    it, ini, fin, outer = ds.solve_mixed("cg", 1, x, b, 100, s.tol)
+    assert outer == 0 and it == 0 and fin < s.tol
+    # a cap that is too small returns count + 1 and a residual above tol
+    x.zero_()
+    it, ini, fin, outer = ds.solve_mixed("cg", 1, x, b, 3, s.tol)
+    assert fin >= s.tol and it == 4
+    ds.close()
+
+
+def test_difference_form_fp32_operator(pkg, cuda):
+    """The FP32 operator of the inner solves is A in difference form:
+    (A p)_i = sum_{j!=i} a_ij (p_j - p_i) + rs_i p_i.  On a random vector it
+    matches the FP64 product to FP32 rounding relative to |A||p|; on a
+    constant vector it is exactly rs * c (rows of the pressure matrix sum
+    to zero except at the pinned cell), which the plain FP32 product is not."""
+    import torch
+    s = pkg.meshgen.pressure_system(20, seed=56)
+    m = s.matrix
+    A = m.to_scipy()
+    ds = pkg.krylov.DeviceSystem(m)
+    p = np.random.default_rng(57).standard_normal(m.n).astype(np.float32)
+    y = torch.zeros(m.n, dtype=torch.float32, device=cuda)
+    ds.spmv32(torch.from_numpy(p).to(cuda), y)
+    ref = A @ p.astype(np.float64)
+    scale = abs(A) @ np.abs(p.astype(np.float64))
+    assert np.all(np.abs(y.cpu().numpy() - ref) <= 16 * 6e-8 * scale + 1e-30)
+    c = np.full(m.n, 0.75, np.float32)
+    ds.spmv32(torch.from_numpy(c).to(cuda), y)
+    rs = np.asarray(A.sum(axis=1)).ravel().astype(np.float32)
+    got = y.cpu().numpy()
+    diag = np.abs(A.diagonal())
+    interior = np.abs(rs) < 1e-6 * diag
+    # rows that sum to zero give zero to FP64 rounding of the row sum, far
+    # below the eps32*|a_ii| that the plain FP32 product leaves
+    assert np.all(np.abs(got[interior]) <= 1e-12 * diag[interior])
+    assert np.allclose(got[~interior], rs[~interior] * np.float32(0.75), rtol=2.5e-7, atol=0)
+    ds.close()
+
+
+@pytest.mark.parametrize("case,prec,interval", [
+    ("pressure12", 1, 50), ("pressure12", 0, 7), ("pressure10_perm", 1, 1), ("pressure10_perm", 1, 1000),
+    ("momentum10", 1, 20),
+])
+def test_mixed_reliable_matches_fp64_oracle(pkg, oracle, cuda, case, prec, interval):
+    """FP32 CG with FP64 residual replacement: the FP64 true residual meets
+    tol, the iteration count stays near the FP64 CG's, and x is within the
+    conditioning bound of the oracle's FP64 solution."""
+    import torch
+    mg = pkg.meshgen
+    s = {"pressure12": lambda: mg.pressure_system(12, seed=51),
+         "pressure10_perm": lambda: mg.pressure_system(10, seed=52, perm_seed=53),
+         "momentum10": lambda: mg.momentum_system(10, seed=54)}[case]()
+    m = s.matrix
+    A = m.to_scipy()
+    ds = pkg.krylov.DeviceSystem(m)
+    b = torch.from_numpy(s.b.copy()).to(cuda)
+    x = torch.zeros(m.n, dtype=torch.float64, device=cuda)
+    r = torch.zeros_like(b)
+    cap = 3000
+    it, ini, fin, rep = ds.solve_mixed_reliable(prec, x, b, cap, s.tol, interval=interval, r_out=r)
+    xm = x.cpu().numpy()
+    true_r = np.linalg.norm(s.b - A @ xm)
+    assert ini == pytest.approx(np.linalg.norm(s.b), rel=1e-12)
+    if case == "momentum10":
+        # CG on the nonsymmetric momentum matrix stalls, as the reference's
+        # does; the solver must report it (count + 1, residual above tol)
+        ref = oracle.run("cg", m, s.b, prec=prec, niter=cap, tol=s.tol)
+        assert ref["fin_res"] >= s.tol and fin >= s.tol and it == cap + 1
+        assert fin == pytest.approx(true_r, rel=1e-6)
+        ds.close()
+        return
+    assert true_r < s.tol and fin < s.tol and fin == pytest.approx(true_r, rel=1e-6)
+    assert np.allclose(r.cpu().numpy(), s.b - A @ xm, atol=1e-12 * np.linalg.norm(s.b))
+    ref = oracle.run("cg", m, s.b, prec=prec, niter=cap, tol=s.tol)
+    rel = np.linalg.norm(xm - ref["x"]) / np.linalg.norm(ref["x"])
+    bound = _inv_norm2(m) * (true_r + np.linalg.norm(s.b - A @ ref["x"])) / np.linalg.norm(ref["x"])
+    print(f"{case} prec={prec} interval={interval}: it={it} (fp64 {ref['niter']}) replacements={rep} "
+          f"rel_err_vs_fp64={rel:.2e} bound={bound:.2e}")
+    assert rel <= bound
+    assert it <= 1.5 * ref["niter"] + interval + 5
+    ds.close()
