@@ -137,6 +137,23 @@ int tflows_solve_mixed_reliable(tflows_system s, int32_t prec, double* x, const 
                                 int32_t* niter, double tol, int32_t interval, double* ini_res, double* fin_res,
                                 int32_t* replacements, void* stream);
 
+/* Adjoint (transposed) solve A^T lambda = g with the Krylov method and
+ * preconditioner of the forward solve, for a structurally symmetric pattern
+ * (the values of A^T are gathered from the mirror positions).  lambda (device,
+ * N doubles): initial guess in, solution out; r1 (device): g in, final
+ * residual out; other arguments as tflows_solve.  No counterpart in the
+ * reference. */
+int tflows_solve_adjoint(tflows_system s, int32_t method, int32_t prec, double* lambda, double* r1, int32_t* niter,
+                         double tol, double* ini_res, double* fin_res, void* stream);
+
+/* Sparse outer-product gradient on the pattern: grad[p] = -lambda_i x_j for
+ * every stored entry p = (i, j), i.e. dL/dA given lambda = A^-T dL/dx;
+ * grad (device) has NONZ doubles, the last one set to 0. */
+int tflows_outer_grad(tflows_system s, const double* lambda, const double* x, double* grad, void* stream);
+
+/* Values of A^T on the same pattern (device, nnz doubles). */
+int tflows_transpose_values(tflows_system s, double* aval_t, void* stream);
+
 /* y = A x on the device, FP64 (x, y: N doubles). */
 int tflows_spmv(tflows_system s, const double* x, double* y, void* stream);
 /* y = A x with the FP32 copy of the values (x, y: N floats). */

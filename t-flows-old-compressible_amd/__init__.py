@@ -5,3 +5,4 @@ The directory name carries hyphens, so it is loaded as the package
 """
 from . import meshgen  # noqa: F401
 from . import krylov  # noqa: F401,E402
+from . import adjoint_harness  # noqa: F401,E402

@@ -72,7 +72,7 @@ def build_fortran(verbose: bool = False) -> str | None:
     out = os.path.join(PKG, "tflows_dropin_driver")
     mod_dir = os.path.join(BUILD, "fmod")
     os.makedirs(mod_dir, exist_ok=True)
-    srcs = [os.path.join(FORTRAN, f) for f in ("tflows_krylov_mod.f90", "dropin_driver.f90")]
+    srcs = [os.path.join(FORTRAN, f) for f in ("tflows_krylov_mod.f90", "dropin_explicit.f90", "dropin_driver.f90")]
     if not all(os.path.exists(s) for s in srcs):
         return None
     cmd = [FLANG, "-O2", "-fdefault-real-8", "-fdefault-double-8", "-module-dir", mod_dir,

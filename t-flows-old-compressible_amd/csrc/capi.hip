@@ -207,6 +207,31 @@ int tflows_solve_mixed_reliable(tflows_system s, int32_t prec, double* x, const 
     });
 }
 
+int tflows_solve_adjoint(tflows_system s, int32_t method, int32_t prec, double* lambda, double* r1, int32_t* niter,
+                         double tol, double* ini_res, double* fin_res, void* stream) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        System& T = s->sys->transposed(as_stream(stream));
+        tflows::SolveResult r = T.solve(method, prec, lambda, r1, *niter, tol, g_sol_error, as_stream(stream));
+        *niter = r.niter;
+        if (r.ini_written && ini_res) *ini_res = r.ini_res;
+        if (fin_res) *fin_res = r.fin_res;
+        g_sol_error = r.fin_res;
+    });
+}
+
+int tflows_outer_grad(tflows_system s, const double* lambda, const double* x, double* grad, void* stream) {
+    return guarded([&] { s->sys->outer_grad(lambda, x, grad, as_stream(stream)); });
+}
+
+int tflows_transpose_values(tflows_system s, double* aval_t, void* stream) {
+    return guarded([&] {
+        System& T = s->sys->transposed(as_stream(stream));
+        TF_CHECK(hipMemcpyAsync(aval_t, T.values(), sizeof(double) * T.nnz(), hipMemcpyDeviceToDevice,
+                                as_stream(stream)));
+    });
+}
+
 int tflows_spmv(tflows_system s, const double* x, double* y, void* stream) {
     return guarded([&] { s->sys->spmv(x, y, as_stream(stream)); });
 }

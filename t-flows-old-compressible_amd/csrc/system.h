@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -104,6 +105,13 @@ class System {
     // r = b - A x (FP64) and its norm
     double true_residual(const double* x, const double* b, double* r, hipStream_t s);
 
+    // adjoint support (adjoint.hip): mirror positions of the structurally
+    // symmetric pattern, a system holding A^T on the same pattern (values
+    // refreshed after set_values), and grad[p] = -lambda_i x_j
+    const int* transpose_map();
+    System& transposed(hipStream_t s);
+    void outer_grad(const double* lambda, const double* x, double* grad, hipStream_t s);
+
     void prec1(int prec, double* d, hipStream_t s);
     void spmv(const double* x, double* y, hipStream_t s);
     void spmv32(const float* x, float* y, hipStream_t s);
@@ -144,6 +152,10 @@ class System {
     DevArray<int> off_rowptr_, off_col_;
     DevArray<float> off_val_, rowsum32_, diag32_;
     bool diff32_valid_ = false;
+    DevArray<int> tpos_;
+    bool tpos_ready_ = false;
+    std::unique_ptr<System> transposed_;
+    bool transposed_valid_ = false;
     std::vector<int> h_rowptr_, h_col_, h_dia_;
     Levels levels_;
     bool levels_ready_ = false;

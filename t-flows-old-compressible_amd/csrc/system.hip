@@ -95,6 +95,7 @@ void System::set_values(const double* aval, bool device_ptr, hipStream_t s) {
                             device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     val32_valid_ = false;
     diff32_valid_ = false;
+    transposed_valid_ = false;
     if (!device_ptr) TF_CHECK(hipStreamSynchronize(s));
 }
 

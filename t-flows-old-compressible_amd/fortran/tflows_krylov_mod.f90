@@ -1,0 +1,110 @@
+!==============================================================================!
+!  tflows_krylov -- ISO_C_BINDING interfaces of libtflows_krylov.so, the
+!  MI355X (gfx950) Krylov solvers.
+!
+!  The drop-in entry points keep the reference's external subroutines
+!  (Process/CG.f90:2-3, BiCG.f90:2-3, CGS.f90:2-3, Prec1.f90:2,
+!  Resid.f90:2): every argument by reference, the same order and meaning,
+!  linked under the names cg_, bicg_, cgs_, prec1_, resid_.  A caller that
+!  keeps its implicit interface (as CalcPS.f90:217 and CalcZmix.f90:394 do)
+!  links to them unchanged; a caller that USEs this module gets explicit,
+!  typed interfaces to the same symbols.  Compile the calling code with
+!  -fdefault-real-8 as the reference does, so that REAL is C double and
+!  INTEGER is a 32-bit C int.
+!
+!  The device-resident API (tflows_system_*, tflows_solve*) is declared
+!  here as well for callers that keep their data in GPU memory.
+!==============================================================================!
+module tflows_krylov
+  use iso_c_binding, only: c_int32_t, c_double, c_ptr, c_char
+  implicit none
+
+  integer(c_int32_t), parameter :: TFLOWS_METHOD_CG = 0, TFLOWS_METHOD_BICG = 1, TFLOWS_METHOD_CGS = 2
+  integer(c_int32_t), parameter :: TFLOWS_MEM_HOST = 0, TFLOWS_MEM_DEVICE = 1
+
+  interface
+    !--- drop-ins (host arrays, reference layout) -----------------------------
+    subroutine tflows_cg(n, nb, nonz, a, acol, arow, adia, ab, x, r1, prec, niter, tol, inires, finres) &
+        bind(C, name="cg_")
+      import :: c_int32_t, c_double
+      integer(c_int32_t), intent(in)    :: n, nb, nonz, prec
+      real(c_double),     intent(in)    :: a(nonz), ab(*), tol
+      integer(c_int32_t), intent(in)    :: acol(n+1), arow(nonz), adia(n)
+      real(c_double),     intent(inout) :: x(-nb:n), r1(n)
+      integer(c_int32_t), intent(inout) :: niter
+      real(c_double),     intent(inout) :: inires, finres
+    end subroutine tflows_cg
+
+    subroutine tflows_bicg(n, nb, nonz, a, acol, arow, adia, ab, x, r1, prec, niter, tol, inires, finres) &
+        bind(C, name="bicg_")
+      import :: c_int32_t, c_double
+      integer(c_int32_t), intent(in)    :: n, nb, nonz, prec
+      real(c_double),     intent(in)    :: a(nonz), ab(*), tol
+      integer(c_int32_t), intent(in)    :: acol(n+1), arow(nonz), adia(n)
+      real(c_double),     intent(inout) :: x(-nb:n), r1(n)
+      integer(c_int32_t), intent(inout) :: niter
+      real(c_double),     intent(inout) :: inires, finres
+    end subroutine tflows_bicg
+
+    subroutine tflows_cgs(n, nb, nonz, a, acol, arow, adia, ab, x, r1, prec, niter, tol, inires, finres) &
+        bind(C, name="cgs_")
+      import :: c_int32_t, c_double
+      integer(c_int32_t), intent(in)    :: n, nb, nonz, prec
+      real(c_double),     intent(in)    :: a(nonz), ab(*), tol
+      integer(c_int32_t), intent(in)    :: acol(n+1), arow(nonz), adia(n)
+      real(c_double),     intent(inout) :: x(-nb:n), r1(n)
+      integer(c_int32_t), intent(inout) :: niter
+      real(c_double),     intent(inout) :: inires, finres
+    end subroutine tflows_cgs
+
+    subroutine tflows_prec1(n, nonz, a, acol, arow, adia, d, prec) bind(C, name="prec1_")
+      import :: c_int32_t, c_double
+      integer(c_int32_t), intent(in)  :: n, nonz, prec
+      real(c_double),     intent(in)  :: a(nonz)
+      integer(c_int32_t), intent(in)  :: acol(n+1), arow(nonz), adia(n)
+      real(c_double),     intent(out) :: d(n)
+    end subroutine tflows_prec1
+
+    subroutine tflows_resid(n, nb, nonz, a, acol, arow, ab, x, r1) bind(C, name="resid_")
+      import :: c_int32_t, c_double
+      integer(c_int32_t), intent(in)    :: n, nb, nonz
+      real(c_double),     intent(in)    :: a(nonz), ab(*), x(-nb:n)
+      integer(c_int32_t), intent(in)    :: acol(n+1), arow(nonz)
+      real(c_double),     intent(inout) :: r1(n)
+    end subroutine tflows_resid
+
+    !--- device-resident API -------------------------------------------------
+    function tflows_system_create(n, nonz, acol, arow, adia, mem, stream, sys) result(rc) &
+        bind(C, name="tflows_system_create")
+      import :: c_int32_t, c_ptr
+      integer(c_int32_t), value :: n, nonz, mem
+      type(c_ptr), value        :: acol, arow, adia, stream
+      type(c_ptr)               :: sys
+      integer(c_int32_t)        :: rc
+    end function tflows_system_create
+
+    function tflows_system_destroy(sys) result(rc) bind(C, name="tflows_system_destroy")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: sys
+      integer(c_int32_t) :: rc
+    end function tflows_system_destroy
+
+    function tflows_system_set_values(sys, aval, mem, stream) result(rc) bind(C, name="tflows_system_set_values")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value        :: sys, aval, stream
+      integer(c_int32_t), value :: mem
+      integer(c_int32_t)        :: rc
+    end function tflows_system_set_values
+
+    function tflows_solve(sys, method, prec, x, r1, niter, tol, inires, finres, stream) result(rc) &
+        bind(C, name="tflows_solve")
+      import :: c_int32_t, c_double, c_ptr
+      type(c_ptr), value        :: sys, x, r1, stream
+      integer(c_int32_t), value :: method, prec
+      integer(c_int32_t)        :: niter
+      real(c_double), value     :: tol
+      real(c_double)            :: inires, finres
+      integer(c_int32_t)        :: rc
+    end function tflows_solve
+  end interface
+end module tflows_krylov

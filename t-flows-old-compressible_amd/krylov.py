@@ -101,6 +101,16 @@ def _bind_extensions(lib):
         f.restype = C.c_int
         f.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, _ip, C.c_double, C.c_double, C.c_int32,
                       _dp, _dp, _ip, vp]
+    for name in ("tflows_solve_adjoint",):
+        if hasattr(lib, name):
+            f = getattr(lib, name)
+            f.restype = C.c_int
+            f.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, _ip, C.c_double, _dp, _dp, vp]
+    if hasattr(lib, "tflows_outer_grad"):
+        lib.tflows_outer_grad.restype = C.c_int
+        lib.tflows_outer_grad.argtypes = [vp, vp, vp, vp, vp]
+        lib.tflows_transpose_values.restype = C.c_int
+        lib.tflows_transpose_values.argtypes = [vp, vp, vp]
     if hasattr(lib, "tflows_solve_mixed_reliable"):
         f = lib.tflows_solve_mixed_reliable
         f.restype = C.c_int
@@ -258,6 +268,24 @@ class DeviceSystem:
             C.c_void_p(_ptr(r_out) if r_out is not None else 0), C.byref(it), tol, interval, C.byref(ini),
             C.byref(fin), C.byref(rep), _stream(stream)))
         return it.value, ini.value, fin.value, rep.value
+
+    def solve_adjoint(self, method: str, prec: int, lam, r1, niter: int, tol: float, stream=None):
+        """A^T lam = r1 (r1 in: right-hand side, out: final residual) with the
+        method and preconditioner of the forward solve."""
+        it = C.c_int32(niter)
+        ini, fin = C.c_double(_SENTINEL), C.c_double(0.0)
+        _check(self.lib.tflows_solve_adjoint(self.handle, METHODS[method], prec, C.c_void_p(_ptr(lam)),
+                                             C.c_void_p(_ptr(r1)), C.byref(it), tol, C.byref(ini), C.byref(fin),
+                                             _stream(stream)))
+        return it.value, (None if ini.value == _SENTINEL else ini.value), fin.value
+
+    def outer_grad(self, lam, x, grad, stream=None):
+        """grad[p] = -lam[i] * x[j] for every stored entry p = (i, j)."""
+        _check(self.lib.tflows_outer_grad(self.handle, C.c_void_p(_ptr(lam)), C.c_void_p(_ptr(x)),
+                                          C.c_void_p(_ptr(grad)), _stream(stream)))
+
+    def transpose_values(self, out, stream=None):
+        _check(self.lib.tflows_transpose_values(self.handle, C.c_void_p(_ptr(out)), _stream(stream)))
 
     def spmv(self, x, y, stream=None):
         _check(self.lib.tflows_spmv(self.handle, C.c_void_p(_ptr(x)), C.c_void_p(_ptr(y)), _stream(stream)))

@@ -532,3 +532,159 @@ def from_scipy(A, meta=None) -> TFlowSMatrix:
     except ValueError:
         m.mirror = None
     return m
+
+
+# --------------------------------------------------------------------------
+# face-based assembly (TopolM) and perturbed geometry
+# --------------------------------------------------------------------------
+@dataclass
+class Topology:
+    """Result of :func:`topolm`: the CSR pattern plus the positions of each
+    face's two off-diagonal entries (SidAij) and of every diagonal."""
+    n: int
+    nonz: int
+    acol: np.ndarray      # int32 (N+1), 1-based
+    arow: np.ndarray      # int32 (NONZ), 1-based, last slot 0
+    adia: np.ndarray      # int32 (N), 1-based
+    sidaij: np.ndarray    # int64 (2, NS) 0-based positions of A(c1,c2), A(c2,c1)
+
+
+def topolm(n: int, c1: np.ndarray, c2: np.ndarray) -> Topology:
+    """Pattern of the system matrix from the face list (Process/TopolM.f90).
+
+    ``c1``, ``c2`` are 1-based cells of the interior faces (SideC(1:2,s)
+    with c2 > 0).  Each row holds the diagonal plus one entry per face of
+    the cell, columns sorted ascending (ISORT, TopolM.f90:72-74); Adia is the
+    diagonal position (:75-77) and SidAij(1,s) / SidAij(2,s) the positions of
+    A(c1,c2) / A(c2,c1) (:81-94).  NONZERO = nnz + 1 (:49).
+    """
+    c1 = np.asarray(c1, np.int64) - 1
+    c2 = np.asarray(c2, np.int64) - 1
+    ns = len(c1)
+    rows = np.concatenate([np.arange(n), c1, c2])
+    cols = np.concatenate([np.arange(n), c2, c1])
+    order = np.lexsort((cols, rows))
+    nnz = len(rows)
+    pos = np.empty(nnz, np.int64)
+    pos[order] = np.arange(nnz)
+    counts = np.bincount(rows, minlength=n)
+    acol = np.empty(n + 1, np.int32)
+    acol[0] = 1
+    acol[1:] = 1 + np.cumsum(counts)
+    arow = np.zeros(nnz + 1, np.int32)
+    arow[:nnz] = cols[order] + 1
+    adia = (pos[:n] + 1).astype(np.int32)
+    sid = np.stack([pos[n:n + ns], pos[n + ns:]])
+    return Topology(n, nnz + 1, acol, arow, adia, sid)
+
+
+def box_faces(shape):
+    """Interior faces (c1 < c2, 1-based, natural numbering) of a box of cells,
+    x-faces, then y-faces, then z-faces."""
+    nx, ny, nz = shape
+    c = np.arange(nx * ny * nz).reshape(nz, ny, nx)
+    f = [(c[:, :, :-1].ravel(), c[:, :, 1:].ravel()),
+         (c[:, :-1, :].ravel(), c[:, 1:, :].ravel()),
+         (c[:-1, :, :].ravel(), c[1:, :, :].ravel())]
+    return (np.concatenate([a for a, _ in f]) + 1, np.concatenate([b for _, b in f]) + 1,
+            np.concatenate([np.full(len(a), d) for d, (a, _) in enumerate(f)]))
+
+
+@dataclass
+class FaceGeometry:
+    shape: tuple
+    c1: np.ndarray            # 1-based low-side cell of each interior face
+    c2: np.ndarray            # 1-based high-side cell
+    coef: np.ndarray          # a_f = |S|^2 / (S.d), FP64
+    centres: np.ndarray       # (N, 3) cell centres
+    pin_coef: float           # boundary term added to the pinned cell
+
+
+def perturbed_box_geometry(n: int, ratio: float = 1.05, perturb: float = 0.2, seed: int = 4) -> FaceGeometry:
+    """Hex box with tanh wall stretching in y and every interior node moved by
+    up to ``perturb`` times the local spacing in each direction (seeded).
+    Face area vectors come from the quad diagonals, S = (n3-n1) x (n4-n2)/2,
+    cell centres are node averages, and the pressure coefficient of a face is
+    a_f = |S|^2/(S.d) with d the centre-to-centre vector -- in FP64, the
+    form of Process/Calc3.f90:24, 46 and CalcPS.f90:77-78."""
+    rng = np.random.default_rng(seed)
+    xs = np.linspace(0.0, 1.0, n + 1)
+    ys = tanh_nodes(n, 1.0, ratio)
+    zs = np.linspace(0.0, 1.0, n + 1)
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")          # (n+1)^3, index [i, j, k]
+    P = np.stack([X, Y, Z], axis=-1)
+    h = np.stack(np.meshgrid(np.gradient(xs), np.gradient(ys), np.gradient(zs), indexing="ij"), axis=-1)
+    disp = rng.uniform(-perturb, perturb, P.shape) * h
+    disp[[0, -1], :, :, 0] = 0.0
+    disp[:, [0, -1], :, 1] = 0.0
+    disp[:, :, [0, -1], 2] = 0.0
+    P = P + disp
+    # cell centres (node average), cell index = i + n*(j + n*k)
+    C = 0.125 * (P[:-1, :-1, :-1] + P[1:, :-1, :-1] + P[:-1, 1:, :-1] + P[1:, 1:, :-1] +
+                 P[:-1, :-1, 1:] + P[1:, :-1, 1:] + P[:-1, 1:, 1:] + P[1:, 1:, 1:])
+    cen = C.transpose(2, 1, 0, 3).reshape(-1, 3)
+
+    def quad_area(a, b, c, d):    # nodes in cyclic order
+        return 0.5 * np.cross(c - a, d - b)
+
+    coef, c1s, c2s = [], [], []
+    cid = np.arange(n ** 3).reshape(n, n, n, order="F")        # [i, j, k] -> i + n j + n^2 k
+    # x-faces at node plane i+1 between cells (i, j, k) and (i+1, j, k)
+    S = quad_area(P[1:-1, :-1, :-1], P[1:-1, 1:, :-1], P[1:-1, 1:, 1:], P[1:-1, :-1, 1:])
+    d = C[1:] - C[:-1]
+    coef.append((np.sum(S * S, -1) / np.sum(S * d, -1)).ravel(order="F"))
+    c1s.append(cid[:-1].ravel(order="F")); c2s.append(cid[1:].ravel(order="F"))
+    # y-faces
+    S = quad_area(P[:-1, 1:-1, :-1], P[:-1, 1:-1, 1:], P[1:, 1:-1, 1:], P[1:, 1:-1, :-1])
+    d = C[:, 1:] - C[:, :-1]
+    coef.append((np.sum(S * S, -1) / np.sum(S * d, -1)).ravel(order="F"))
+    c1s.append(cid[:, :-1].ravel(order="F")); c2s.append(cid[:, 1:].ravel(order="F"))
+    # z-faces
+    S = quad_area(P[:-1, :-1, 1:-1], P[1:, :-1, 1:-1], P[1:, 1:, 1:-1], P[:-1, 1:, 1:-1])
+    d = C[:, :, 1:] - C[:, :, :-1]
+    coef.append((np.sum(S * S, -1) / np.sum(S * d, -1)).ravel(order="F"))
+    c1s.append(cid[:, :, :-1].ravel(order="F")); c2s.append(cid[:, :, 1:].ravel(order="F"))
+    a = np.concatenate(coef)
+    if np.any(a <= 0):
+        raise ValueError("perturbation too strong: a face has S.d <= 0")
+    # pin: one boundary face of cell 1 (its x=0 face), A/(distance to face)
+    S0 = quad_area(P[0, 0, 0], P[0, 1, 0], P[0, 1, 1], P[0, 0, 1])
+    d0 = C[0, 0, 0] - 0.25 * (P[0, 0, 0] + P[0, 1, 0] + P[0, 1, 1] + P[0, 0, 1])
+    pin = float(np.dot(S0, S0) / abs(np.dot(S0, d0)))
+    return FaceGeometry((n, n, n), np.concatenate(c1s) + 1, np.concatenate(c2s) + 1, a, cen, pin)
+
+
+def renumber_faces(geo: FaceGeometry, seed: int) -> tuple[FaceGeometry, np.ndarray]:
+    """Random cell renumbering: natural cell c becomes perm[c]."""
+    n = len(geo.centres)
+    perm = random_permutation(n, seed)
+    cen = np.empty_like(geo.centres)
+    cen[perm] = geo.centres
+    return (FaceGeometry(geo.shape, perm[geo.c1 - 1] + 1, perm[geo.c2 - 1] + 1, geo.coef, cen, geo.pin_coef),
+            perm)
+
+
+def assemble_face_values(topo: Topology, c1, c2, coef, pin_cell: int, pin_coef: float) -> np.ndarray:
+    """Pressure-correction values on a pattern from face coefficients:
+    A(c1,c2) = A(c2,c1) = -a_f and a_f added to both diagonals
+    (CalcPS.f90:77-94), plus pin_coef on the diagonal of pin_cell."""
+    aval = np.zeros(topo.nonz)
+    np.add.at(aval, topo.sidaij[0], -coef)
+    np.add.at(aval, topo.sidaij[1], -coef)
+    np.add.at(aval, topo.adia[np.asarray(c1) - 1] - 1, coef)
+    np.add.at(aval, topo.adia[np.asarray(c2) - 1] - 1, coef)
+    if pin_cell:
+        aval[topo.adia[pin_cell - 1] - 1] += pin_coef
+    return aval
+
+
+def fourier_target(centres: np.ndarray, seed: int, modes: int = 4) -> np.ndarray:
+    """Smooth target field: a sum of `modes` seeded Fourier modes."""
+    rng = np.random.default_rng(seed)
+    t = np.zeros(len(centres))
+    for _ in range(modes):
+        k = rng.integers(1, 4, size=3)
+        ph = rng.uniform(0, 2 * np.pi, size=3)
+        amp = rng.uniform(-1, 1)
+        t += amp * np.prod(np.cos(np.pi * k * centres + ph), axis=1)
+    return t
