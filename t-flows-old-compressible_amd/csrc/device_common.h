@@ -116,6 +116,20 @@ __device__ __forceinline__ void site_total(const T* __restrict__ partials, T (&t
     block_sum<T, NV>(tot, (T*)sm);
 }
 
+// Sum of the partials of blocks [b0, b1) only (one system of a batch).
+template <typename T, int NV>
+__device__ __forceinline__ void site_total_range(const T* __restrict__ partials, int b0, int b1, T (&tot)[NV]) {
+    __shared__ T sm[kWavesPerBlock * NV];
+    const int G = gridDim.x;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        T s = T(0);
+        for (int b = b0 + threadIdx.x; b < b1; b += kBlock) s += partials[k * G + b];
+        tot[k] = s;
+    }
+    block_sum<T, NV>(tot, (T*)sm);
+}
+
 // ---------------------------------------------------------------------------
 // Launch geometry.  The hardware dispatches block b to XCD (b mod 8); the
 // logical block index below gives each XCD one contiguous share of the

@@ -52,6 +52,32 @@ __device__ __forceinline__ void publish(HostMirror* hm, int done, int iter) {
     }
 }
 
+// Launch context of the solver kernels.  For one system every member is
+// null: block b owns an even share of the 256-row tiles (XCD-aware), the
+// leader is block 0, reductions span the whole grid and the iteration index
+// is the launch argument.  For a batch of K systems stored as one
+// block-diagonal matrix, tiles never straddle two systems and:
+//   blk_sys[b]      system of block b
+//   blk_t0[b..b+1]  tiles of block b
+//   tile_r0[t..t+1] rows of tile t (tiles are contiguous in row order)
+//   sys_blk0[k..k+1] blocks of system k (its reduction range; the first is
+//                    its leader)
+// kbase (optional) offsets the launch's iteration index by a device value,
+// so one captured graph of a chunk of iterations can be replayed.
+struct BatchCtx {
+    const int* blk_sys = nullptr;
+    const int* blk_t0 = nullptr;
+    const int* tile_r0 = nullptr;
+    const int* sys_blk0 = nullptr;
+    const int* kbase = nullptr;
+
+    __device__ __forceinline__ int sys() const { return blk_sys ? blk_sys[blockIdx.x] : 0; }
+    __device__ __forceinline__ bool leader() const {
+        return threadIdx.x == 0 && (blk_sys ? (int)blockIdx.x == sys_blk0[sys()] : blockIdx.x == 0);
+    }
+    __device__ __forceinline__ int k(int k_arg) const { return kbase ? *kbase + k_arg : k_arg; }
+};
+
 // Matrix view with 0-based indices.
 template <typename T>
 struct Csr {
@@ -219,6 +245,47 @@ struct Gather2 {
     const T* __restrict__ y;
     __device__ void operator()(int c, T (&g)[2]) const { g[0] = x[c]; g[1] = y[c]; }
 };
+
+// ---------------------------------------------------------------------------
+// Work split under a BatchCtx.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void ctx_tiles(const BatchCtx& c, int n, int& t0, int& t1) {
+    if (!c.blk_sys) {
+        block_tiles((n + kTileRows - 1) / kTileRows, t0, t1);
+    } else {
+        t0 = c.blk_t0[blockIdx.x];
+        t1 = c.blk_t0[blockIdx.x + 1];
+    }
+}
+__device__ __forceinline__ void ctx_tile_rows(const BatchCtx& c, int n, int t, int& r0, int& r1) {
+    if (!c.tile_r0) {
+        r0 = t * kTileRows;
+        r1 = min(r0 + kTileRows, n);
+    } else {
+        r0 = c.tile_r0[t];
+        r1 = c.tile_r0[t + 1];
+    }
+}
+__device__ __forceinline__ void ctx_rows(const BatchCtx& c, int n, int& rb, int& re) {
+    int t0, t1;
+    ctx_tiles(c, n, t0, t1);
+    if (!c.tile_r0) {
+        rb = t0 * kTileRows;
+        re = min(t1 * kTileRows, n);
+    } else {
+        rb = c.tile_r0[t0];
+        re = c.tile_r0[t1];
+    }
+}
+template <typename T, int NV>
+__device__ __forceinline__ void ctx_total(const BatchCtx& c, const T* __restrict__ partials, T (&tot)[NV]) {
+    if (!c.blk_sys) {
+        site_total<T, NV>(partials, tot);
+    } else {
+        const int s = c.sys();
+        site_total_range<T, NV>(partials, c.sys_blk0[s], c.sys_blk0[s + 1], tot);
+    }
+}
 
 // q = A p over one tile for either operator (no residual form).
 template <int NRHS, typename T, class Gather, class Epi>

@@ -22,8 +22,12 @@ struct Sites {
     static size_t elems(int grid) { return (size_t)8 * grid; }
 };
 
+// Solver-kernel launchers.  `c` is the launch context: BatchCtx{} for one
+// system, or the maps of a block-diagonal batch (st and hm then point at
+// the first of the per-system arrays).
 template <typename T>
-void launch_setup(const Csr<T>& A, int prec, bool write_d, T* D, const T* b, const Sites<T>& S, hipStream_t s);
+void launch_setup(const BatchCtx& c, const Csr<T>& A, int prec, bool write_d, T* D, const T* b, const Sites<T>& S,
+                  hipStream_t s);
 template <typename T>
 void launch_dic_setup_level(const Csr<T>& A, const int* rows, int count, T* D, hipStream_t s);
 template <typename T>
@@ -33,41 +37,43 @@ template <typename T>
 void launch_dic_bwd_level(const Csr<T>& A, const int* rows, int count, const T* D, T* z,
                           const SolverState* st, hipStream_t s);
 template <typename T>
-void launch_resid_init(int mode, bool zero_x, const Csr<T>& A, const T* x, T* r, T* r2, const T* D,
-                       const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
+void launch_resid_init(int mode, bool zero_x, const BatchCtx& c, const Csr<T>& A, const T* x, T* r, T* r2,
+                       const T* D, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
 template <typename T>
-void launch_check(int k, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
+void launch_check(const BatchCtx& c, int k, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
 template <typename T>
-void launch_rho(int n, const T* a, const T* b, const Sites<T>& S, const SolverState* st, hipStream_t s);
+void launch_rho(const BatchCtx& c, int n, const T* a, const T* b, const Sites<T>& S, const SolverState* st,
+                hipStream_t s);
 template <typename T>
-void launch_cg_p(bool jac, int k, int n, const T* r, const T* D, const T* z, T* p, const Sites<T>& S,
-                 SolverState* st, HostMirror* hm, hipStream_t s);
+void launch_cg_p(const BatchCtx& c, bool jac, int k, int n, const T* r, const T* D, const T* z, T* p,
+                 const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
 template <typename Op, typename T>
-void launch_cg_spmv(const Op& A, const T* p, T* q, const Sites<T>& S, const SolverState* st, hipStream_t s);
+void launch_cg_spmv(const BatchCtx& c, const Op& A, const T* p, T* q, const Sites<T>& S, const SolverState* st,
+                    hipStream_t s);
 template <typename T>
-void launch_cg_update(bool jac, int k, int n, T* x, T* r, const T* p, const T* q, const T* D, const Sites<T>& S,
+void launch_cg_update(const BatchCtx& c, bool jac, int k, int n, T* x, T* r, const T* p, const T* q, const T* D,
+                      const Sites<T>& S, const SolverState* st, hipStream_t s);
+template <typename T>
+void launch_bicg_p(const BatchCtx& c, bool jac, int k, int n, const T* r1, const T* r2, const T* D, const T* z1,
+                   const T* z2, T* p1, T* p2, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
+template <typename Op, typename T>
+void launch_bicg_spmv(const BatchCtx& c, const Op& A, const T* p1, const T* p2, T* q1, T* q2, const Sites<T>& S,
                       const SolverState* st, hipStream_t s);
 template <typename T>
-void launch_bicg_p(bool jac, int k, int n, const T* r1, const T* r2, const T* D, const T* z1, const T* z2, T* p1,
-                   T* p2, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
-template <typename Op, typename T>
-void launch_bicg_spmv(const Op& A, const T* p1, const T* p2, T* q1, T* q2, const Sites<T>& S,
-                      const SolverState* st, hipStream_t s);
+void launch_bicg_update(const BatchCtx& c, bool jac, int k, int n, T* x, T* r1, T* r2, const T* p1, const T* q1,
+                        const T* q2, const T* D, const Sites<T>& S, const SolverState* st, hipStream_t s);
 template <typename T>
-void launch_bicg_update(bool jac, int k, int n, T* x, T* r1, T* r2, const T* p1, const T* q1, const T* q2,
-                        const T* D, const Sites<T>& S, const SolverState* st, hipStream_t s);
-template <typename T>
-void launch_cgs_u(bool jac, int k, int n, const T* r1, const T* q1, T* u1, T* u2, const T* D, T* p2,
-                  const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
+void launch_cgs_u(const BatchCtx& c, bool jac, int k, int n, const T* r1, const T* q1, T* u1, T* u2, const T* D,
+                  T* p2, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
 template <typename Op, typename T>
-void launch_cgs_spmv_v(const Op& A, const T* p2, T* v2, const T* r2, const Sites<T>& S, const SolverState* st,
-                       hipStream_t s);
-template <typename T>
-void launch_cgs_q(bool jac, int k, int n, const T* u1, const T* v2, T* q1, const T* D, T* p1, const Sites<T>& S,
-                  const SolverState* st, hipStream_t s);
-template <typename Op, typename T>
-void launch_cgs_spmv_r(int k, const Op& A, const T* p1, T* x, T* r1, const T* r2, const Sites<T>& S,
+void launch_cgs_spmv_v(const BatchCtx& c, const Op& A, const T* p2, T* v2, const T* r2, const Sites<T>& S,
                        const SolverState* st, hipStream_t s);
+template <typename T>
+void launch_cgs_q(const BatchCtx& c, bool jac, int k, int n, const T* u1, const T* v2, T* q1, const T* D, T* p1,
+                  const Sites<T>& S, const SolverState* st, hipStream_t s);
+template <typename Op, typename T>
+void launch_cgs_spmv_r(const BatchCtx& c, int k, const Op& A, const T* p1, T* x, T* r1, const T* r2,
+                       const Sites<T>& S, const SolverState* st, hipStream_t s);
 template <typename Op, typename T>
 void launch_spmv(const Op& A, const T* x, T* y, int grid, hipStream_t s);
 template <typename T>
@@ -90,7 +96,6 @@ template <typename T>
 void launch_cgf_init_z(int n, const T* r, const T* D, T* z, const Sites<T>& S, const SolverState* st,
                        hipStream_t s);
 
-void launch_rebase(int count, const int* in, int* out, hipStream_t s);
 void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t s);
 
 // Number of blocks of the row-streaming kernels for n rows: one block per

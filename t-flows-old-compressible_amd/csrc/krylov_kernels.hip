@@ -9,6 +9,11 @@
 // SolverState::done and returns at once after convergence, so the host can
 // run ahead by whole chunks of iterations.
 //
+// Every solver kernel takes a BatchCtx (kernels.h): empty for one system,
+// or the block / tile / system maps of a batch of systems stored as one
+// block-diagonal matrix, in which case each system has its own SolverState,
+// reduction range and leader block, and systems finish independently.
+//
 // Reduction sites (each 2 values x G blocks):
 //   A  ||b||^2                       k_setup          -> k_resid_init
 //   B  ||r||^2, next rho             k_resid_init / *_update / k_cgs_spmv_r
@@ -27,20 +32,15 @@
 
 namespace tflows {
 
-// Rows [rb, re) of this block for the element-wise kernels (the same tile
-// split as the stream kernels, so a block touches the rows it streamed).
-#define TFLOWS_BLOCK_ROWS(n)                                  \
-    const int ntiles_ = ((n) + kTileRows - 1) / kTileRows;    \
-    int t0_, t1_;                                             \
-    block_tiles(ntiles_, t0_, t1_);                           \
-    const int rb = t0_ * kTileRows, re = min(t1_ * kTileRows, (n))
-
-__device__ __forceinline__ bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
-
 // z_i of the Jacobi / identity preconditioner: a true division by D
 // (Process/CG.f90:124-127; prec = 0 uses D = 1, and r/1 == r exactly).
 template <typename T>
 __device__ __forceinline__ T jac(T r, T d) { return r / d; }
+
+#define TF_STATE                                              \
+    SolverState* st = st_base + ctx.sys();                    \
+    HostMirror* hm = hm_base ? hm_base + ctx.sys() : nullptr; \
+    (void)hm
 
 // ---------------------------------------------------------------------------
 // Stopping test at the start of iteration k, from residual site B written
@@ -51,13 +51,14 @@ __device__ __forceinline__ T jac(T r, T d) { return r / d; }
 // records it.  Returns true when the solve is over.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ bool check_stop(int k, const T* __restrict__ partB, SolverState* st, HostMirror* hm,
-                                           T (&tot)[2]) {
-    site_total<T, 2>(partB, tot);
+__device__ __forceinline__ bool check_stop(const BatchCtx& ctx, int k, const T* __restrict__ partB,
+                                           SolverState* st, HostMirror* hm, T (&tot)[2]) {
+    ctx_total<T, 2>(ctx, partB, tot);
     const double err = sqrt((double)tot[0]);
     const double tol = st->tol;
     const int cap = st->cap;
-    if (leader()) {
+    const bool lead = ctx.leader();
+    if (lead) {
         st->error = err;
         if (k == 1) {
             st->ini_res = err;
@@ -65,7 +66,7 @@ __device__ __forceinline__ bool check_stop(int k, const T* __restrict__ partB, S
         }
     }
     if (err < tol) {
-        if (leader()) {
+        if (lead) {
             st->done = 1;
             st->niter_out = k - 1;
             publish(hm, 1, k - 1);
@@ -73,14 +74,14 @@ __device__ __forceinline__ bool check_stop(int k, const T* __restrict__ partB, S
         return true;
     }
     if (k > cap) {
-        if (leader()) {
+        if (lead) {
             st->done = 1;
             st->niter_out = (cap > 0 ? cap : 0) + 1;
             publish(hm, 1, k - 1);
         }
         return true;
     }
-    if (leader()) publish(hm, 0, k - 1);
+    if (lead) publish(hm, 0, k - 1);
     return false;
 }
 
@@ -90,9 +91,10 @@ __device__ __forceinline__ bool check_stop(int k, const T* __restrict__ partB, S
 // computed before this kernel by k_dic_setup_level and write_d is false.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_setup(Csr<T> A, int prec, bool write_d, T* __restrict__ D,
+__global__ __launch_bounds__(kBlock) void k_setup(BatchCtx ctx, Csr<T> A, int prec, bool write_d, T* __restrict__ D,
                                                   const T* __restrict__ b, T* __restrict__ partA) {
-    TFLOWS_BLOCK_ROWS(A.n);
+    int rb, re;
+    ctx_rows(ctx, A.n, rb, re);
     T v[1] = {T(0)};
     for (int i = rb + threadIdx.x; i < re; i += kBlock) {
         if (write_d) D[i] = (prec == 1) ? A.val[A.dia[i]] : T(1);
@@ -125,6 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_dic_setup_level(Csr<T> A, const int*
 // Forward sweep of the DIC apply for one level (CG.f90:135-142):
 //   z(i) = (u(i) - sum_{lower} a_ij z(k)) * D(i)
 // `u` and `z` may alias (each row reads its own u before writing its z).
+// `st` (null for a batch) lets the sweep stop after convergence.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dic_fwd_level(Csr<T> A, const int* __restrict__ rows, int count,
                                                           const T* __restrict__ D, const T* u, T* z,
@@ -189,38 +192,44 @@ struct ResidEpi {
 };
 
 template <typename T, int MODE, bool ZERO_X>
-__global__ __launch_bounds__(kBlock) void k_resid_init(Csr<T> A, const T* __restrict__ x, T* __restrict__ r,
-                                                       T* __restrict__ r2, const T* __restrict__ D,
-                                                       const T* __restrict__ partA, T* __restrict__ partB,
-                                                       SolverState* st, HostMirror* hm) {
+__global__ __launch_bounds__(kBlock) void k_resid_init(BatchCtx ctx, Csr<T> A, const T* __restrict__ x,
+                                                       T* __restrict__ r, T* __restrict__ r2,
+                                                       const T* __restrict__ D, const T* __restrict__ partA,
+                                                       T* __restrict__ partB, SolverState* st_base,
+                                                       HostMirror* hm_base) {
+    TF_STATE;
     T bb[1];
-    site_total<T, 1>(partA, bb);
+    ctx_total<T, 1>(ctx, partA, bb);
     const double bn = sqrt((double)bb[0]);
+    ResidEpi<T, MODE> epi{r, r2, D, {T(0), T(0)}};
     if (bn < st->tol) {
-        if (leader()) {
+        if (ctx.leader()) {
             st->bnrm2 = bn;
             st->done = 1;
             st->niter_out = 0;
             publish(hm, 1, 0);
         }
+        site_write<T, 2>(epi.acc, partB);   // zeros: the system is over
         return;
     }
-    if (leader()) st->bnrm2 = bn;
-    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
-    int t0, t1;
-    block_tiles(ntiles, t0, t1);
-    ResidEpi<T, MODE> epi{r, r2, D, {T(0), T(0)}};
+    if (ctx.leader()) st->bnrm2 = bn;
     if (ZERO_X) {  // x = 0: r = b - A*0 = b, no matrix pass
-        const int rb = t0 * kTileRows, re = min(t1 * kTileRows, A.n);
+        int rb, re;
+        ctx_rows(ctx, A.n, rb, re);
         for (int i = rb + threadIdx.x; i < re; i += kBlock) {
             T y[1] = {r[i]};
             epi(i, y);
         }
     } else {
+        int t0, t1;
+        ctx_tiles(ctx, A.n, t0, t1);
         Gather1<T> g{x};
         InitVec<T> init{r};
-        for (int t = t0; t < t1; ++t)
-            stream_tile<T, 1, true>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi);
+        for (int t = t0; t < t1; ++t) {
+            int r0, r1;
+            ctx_tile_rows(ctx, A.n, t, r0, r1);
+            stream_tile<T, 1, true>(A, r0, r1, g, init, epi);
+        }
     }
     site_write<T, 2>(epi.acc, partB);
 }
@@ -228,41 +237,58 @@ __global__ __launch_bounds__(kBlock) void k_resid_init(Csr<T> A, const T* __rest
 // Stand-alone stopping test (DIC forms start their iteration with it, and
 // every solve ends with it at k = cap + 1).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_check(int k, const T* __restrict__ partB, SolverState* st,
-                                                  HostMirror* hm) {
+__global__ __launch_bounds__(kBlock) void k_check(BatchCtx ctx, int k_arg, const T* __restrict__ partB,
+                                                  SolverState* st_base, HostMirror* hm_base) {
+    TF_STATE;
     if (st->done) return;
     T tot[2];
-    check_stop<T>(k, partB, st, hm, tot);
+    check_stop<T>(ctx, ctx.k(k_arg), partB, st, hm, tot);
 }
 
 // rho = sum a_i*b_i -> site C, for the DIC forms (CG.f90:172-176: (r,z);
 // BiCG.f90:169-173: (z1,r2)); the product is a*b in the reference's order.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_rho(int n, const T* __restrict__ a, const T* __restrict__ b,
-                                                T* __restrict__ partC, const SolverState* st) {
-    if (st->done) return;
-    TFLOWS_BLOCK_ROWS(n);
+__global__ __launch_bounds__(kBlock) void k_rho(BatchCtx ctx, int n, const T* __restrict__ a,
+                                                const T* __restrict__ b, T* __restrict__ partC,
+                                                const SolverState* st_base) {
+    const SolverState* st = st_base + ctx.sys();
     T v[2] = {T(0), T(0)};
-    for (int i = rb + threadIdx.x; i < re; i += kBlock) v[0] += a[i] * b[i];
+    if (!st->done) {
+        int rb, re;
+        ctx_rows(ctx, n, rb, re);
+        for (int i = rb + threadIdx.x; i < re; i += kBlock) v[0] += a[i] * b[i];
+    }
     site_write<T, 2>(v, partC);
 }
 
 // rho_k of iteration k: from site B (Jacobi forms, after the stopping test)
 // or site C (DIC forms); recorded by the leader in st->rho[k & 1].
 template <typename T, bool JAC>
-__device__ __forceinline__ bool iteration_rho(int k, const T* partB, const T* partC, SolverState* st,
-                                              HostMirror* hm, T& rho) {
+__device__ __forceinline__ bool iteration_rho(const BatchCtx& ctx, int k, const T* partB, const T* partC,
+                                              SolverState* st, HostMirror* hm, T& rho) {
     T tot[2];
     if (JAC) {
-        if (check_stop<T>(k, partB, st, hm, tot)) return false;
+        if (check_stop<T>(ctx, k, partB, st, hm, tot)) return false;
         rho = tot[1];
     } else {
-        site_total<T, 2>(partC, tot);
+        ctx_total<T, 2>(ctx, partC, tot);
         rho = tot[0];
     }
-    if (leader()) st->rho[k & 1] = (double)rho;
+    if (ctx.leader()) st->rho[k & 1] = (double)rho;
     return true;
 }
+
+// In a batch, blocks of a finished system still write their (zero) partial
+// sums so that every partial a consumer reads is defined; for one system a
+// finished solve simply returns.
+#define TF_DONE_RETURN(site, NV)                 \
+    if (st->done) {                              \
+        if (ctx.blk_sys) {                       \
+            T z_[NV] = {};                       \
+            site_write<T, NV>(z_, site);         \
+        }                                        \
+        return;                                  \
+    }
 
 // ===========================================================================
 // CG  (Process/CG.f90:113-259)
@@ -270,16 +296,20 @@ __device__ __forceinline__ bool iteration_rho(int k, const T* partB, const T* pa
 // [stopping test,] rho, p = z + beta*p  (z = r/D, or the DIC output z)
 // CG.f90:172-187
 template <typename T, bool JAC>
-__global__ __launch_bounds__(kBlock) void k_cg_p(int k, int n, const T* __restrict__ r, const T* __restrict__ D,
-                                                 const T* __restrict__ z, T* __restrict__ p,
-                                                 const T* __restrict__ partB, const T* __restrict__ partC,
-                                                 SolverState* st, HostMirror* hm) {
+__global__ __launch_bounds__(kBlock) void k_cg_p(BatchCtx ctx, int k_arg, int n, const T* __restrict__ r,
+                                                 const T* __restrict__ D, const T* __restrict__ z,
+                                                 T* __restrict__ p, const T* __restrict__ partB,
+                                                 const T* __restrict__ partC, SolverState* st_base,
+                                                 HostMirror* hm_base) {
+    TF_STATE;
     if (st->done) return;
+    const int k = ctx.k(k_arg);
     T rho;
-    if (!iteration_rho<T, JAC>(k, partB, partC, st, hm, rho)) return;
+    if (!iteration_rho<T, JAC>(ctx, k, partB, partC, st, hm, rho)) return;
     const bool first = k == 1;
     const T beta = first ? T(0) : rho / (T)st->rho[(k - 1) & 1];
-    TFLOWS_BLOCK_ROWS(n);
+    int rb, re;
+    ctx_rows(ctx, n, rb, re);
     for (int i = rb + threadIdx.x; i < re; i += kBlock) {
         const T zi = JAC ? jac(r[i], D[i]) : z[i];
         p[i] = first ? zi : zi + beta * p[i];
@@ -299,30 +329,38 @@ struct DotEpi {
 };
 
 template <typename Op, typename T = typename OpValue<Op>::type>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(Op A, const T* __restrict__ p, T* __restrict__ q,
-                                                    T* __restrict__ partD, const SolverState* st) {
-    if (st->done) return;
-    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(BatchCtx ctx, Op A, const T* __restrict__ p, T* __restrict__ q,
+                                                    T* __restrict__ partD, const SolverState* st_base) {
+    const SolverState* st = st_base + ctx.sys();
+    TF_DONE_RETURN(partD, 1);
     int t0, t1;
-    block_tiles(ntiles, t0, t1);
+    ctx_tiles(ctx, A.n, t0, t1);
     Gather1<T> g{p};
     DotEpi<T> epi{p, q, {T(0)}};
-    for (int t = t0; t < t1; ++t) op_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+    for (int t = t0; t < t1; ++t) {
+        int r0, r1;
+        ctx_tile_rows(ctx, A.n, t, r0, r1);
+        op_tile<1>(A, r0, r1, g, epi);
+    }
     site_write<T, 1>(epi.acc, partD);
 }
 
 // alfa = rho/(p,q) (:219); x += alfa p; r -= alfa q (CG.f90:226-229), with
 // ||r||^2 and, for the Jacobi forms, the next rho = (r, r/D) -> site B.
 template <typename T, bool JAC>
-__global__ __launch_bounds__(kBlock) void k_cg_update(int k, int n, T* __restrict__ x, T* __restrict__ r,
-                                                      const T* __restrict__ p, const T* __restrict__ q,
-                                                      const T* __restrict__ D, const T* __restrict__ partD,
-                                                      T* __restrict__ partB, const SolverState* st) {
-    if (st->done) return;
+__global__ __launch_bounds__(kBlock) void k_cg_update(BatchCtx ctx, int k_arg, int n, T* __restrict__ x,
+                                                      T* __restrict__ r, const T* __restrict__ p,
+                                                      const T* __restrict__ q, const T* __restrict__ D,
+                                                      const T* __restrict__ partD, T* __restrict__ partB,
+                                                      const SolverState* st_base) {
+    const SolverState* st = st_base + ctx.sys();
+    TF_DONE_RETURN(partB, 2);
+    const int k = ctx.k(k_arg);
     T pq[1];
-    site_total<T, 1>(partD, pq);
+    ctx_total<T, 1>(ctx, partD, pq);
     const T alfa = (T)st->rho[k & 1] / pq[0];
-    TFLOWS_BLOCK_ROWS(n);
+    int rb, re;
+    ctx_rows(ctx, n, rb, re);
     T v[2] = {T(0), T(0)};
     for (int i = rb + threadIdx.x; i < re; i += kBlock) {
         x[i] = x[i] + alfa * p[i];
@@ -339,18 +377,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int k, int n, T* __restric
 // (BiCG.f90:196-200) and starts from r2 = r1, exactly as in the reference.
 // ===========================================================================
 template <typename T, bool JAC>
-__global__ __launch_bounds__(kBlock) void k_bicg_p(int k, int n, const T* __restrict__ r1,
+__global__ __launch_bounds__(kBlock) void k_bicg_p(BatchCtx ctx, int k_arg, int n, const T* __restrict__ r1,
                                                    const T* __restrict__ r2, const T* __restrict__ D,
                                                    const T* __restrict__ z1, const T* __restrict__ z2,
                                                    T* __restrict__ p1, T* __restrict__ p2,
                                                    const T* __restrict__ partB, const T* __restrict__ partC,
-                                                   SolverState* st, HostMirror* hm) {
+                                                   SolverState* st_base, HostMirror* hm_base) {
+    TF_STATE;
     if (st->done) return;
+    const int k = ctx.k(k_arg);
     T rho;
-    if (!iteration_rho<T, JAC>(k, partB, partC, st, hm, rho)) return;
+    if (!iteration_rho<T, JAC>(ctx, k, partB, partC, st, hm, rho)) return;
     const bool first = k == 1;
     const T beta = first ? T(0) : rho / (T)st->rho[(k - 1) & 1];
-    TFLOWS_BLOCK_ROWS(n);
+    int rb, re;
+    ctx_rows(ctx, n, rb, re);
     for (int i = rb + threadIdx.x; i < re; i += kBlock) {
         const T a = JAC ? jac(r1[i], D[i]) : z1[i];
         const T b = JAC ? jac(r2[i], D[i]) : z2[i];
@@ -373,30 +414,39 @@ struct BiDotEpi {
 };
 
 template <typename Op, typename T = typename OpValue<Op>::type>
-__global__ __launch_bounds__(kBlock) void k_bicg_spmv(Op A, const T* __restrict__ p1, const T* __restrict__ p2,
-                                                      T* __restrict__ q1, T* __restrict__ q2,
-                                                      T* __restrict__ partD, const SolverState* st) {
-    if (st->done) return;
-    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
+__global__ __launch_bounds__(kBlock) void k_bicg_spmv(BatchCtx ctx, Op A, const T* __restrict__ p1,
+                                                      const T* __restrict__ p2, T* __restrict__ q1,
+                                                      T* __restrict__ q2, T* __restrict__ partD,
+                                                      const SolverState* st_base) {
+    const SolverState* st = st_base + ctx.sys();
+    TF_DONE_RETURN(partD, 1);
     int t0, t1;
-    block_tiles(ntiles, t0, t1);
+    ctx_tiles(ctx, A.n, t0, t1);
     Gather2<T> g{p1, p2};
     BiDotEpi<T> epi{p2, q1, q2, {T(0)}};
-    for (int t = t0; t < t1; ++t) op_tile<2>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+    for (int t = t0; t < t1; ++t) {
+        int r0, r1;
+        ctx_tile_rows(ctx, A.n, t, r0, r1);
+        op_tile<2>(A, r0, r1, g, epi);
+    }
     site_write<T, 1>(epi.acc, partD);
 }
 
 template <typename T, bool JAC>
-__global__ __launch_bounds__(kBlock) void k_bicg_update(int k, int n, T* __restrict__ x, T* __restrict__ r1,
-                                                        T* __restrict__ r2, const T* __restrict__ p1,
-                                                        const T* __restrict__ q1, const T* __restrict__ q2,
-                                                        const T* __restrict__ D, const T* __restrict__ partD,
-                                                        T* __restrict__ partB, const SolverState* st) {
-    if (st->done) return;
+__global__ __launch_bounds__(kBlock) void k_bicg_update(BatchCtx ctx, int k_arg, int n, T* __restrict__ x,
+                                                        T* __restrict__ r1, T* __restrict__ r2,
+                                                        const T* __restrict__ p1, const T* __restrict__ q1,
+                                                        const T* __restrict__ q2, const T* __restrict__ D,
+                                                        const T* __restrict__ partD, T* __restrict__ partB,
+                                                        const SolverState* st_base) {
+    const SolverState* st = st_base + ctx.sys();
+    TF_DONE_RETURN(partB, 2);
+    const int k = ctx.k(k_arg);
     T pq[1];
-    site_total<T, 1>(partD, pq);
+    ctx_total<T, 1>(ctx, partD, pq);
     const T alfa = (T)st->rho[k & 1] / pq[0];        // BiCG.f90:223
-    TFLOWS_BLOCK_ROWS(n);
+    int rb, re;
+    ctx_rows(ctx, n, rb, re);
     T v[2] = {T(0), T(0)};
     for (int i = rb + threadIdx.x; i < re; i += kBlock) {
         x[i] = x[i] + alfa * p1[i];                   // BiCG.f90:230-234
@@ -416,19 +466,22 @@ __global__ __launch_bounds__(kBlock) void k_bicg_update(int k, int n, T* __restr
 // stopping test; rho = (r1, r2) from site B; u1 = r1 + beta q1;
 // u2 = u1 + beta (q1 + beta u2); p2 = u2/D  (:118-148)
 template <typename T, bool JAC>
-__global__ __launch_bounds__(kBlock) void k_cgs_u(int k, int n, const T* __restrict__ r1,
+__global__ __launch_bounds__(kBlock) void k_cgs_u(BatchCtx ctx, int k_arg, int n, const T* __restrict__ r1,
                                                   const T* __restrict__ q1, T* __restrict__ u1,
                                                   T* __restrict__ u2, const T* __restrict__ D,
                                                   T* __restrict__ p2, const T* __restrict__ partB,
-                                                  SolverState* st, HostMirror* hm) {
+                                                  SolverState* st_base, HostMirror* hm_base) {
+    TF_STATE;
     if (st->done) return;
+    const int k = ctx.k(k_arg);
     T tot[2];
-    if (check_stop<T>(k, partB, st, hm, tot)) return;
+    if (check_stop<T>(ctx, k, partB, st, hm, tot)) return;
     const T rho = tot[1];
-    if (leader()) st->rho[k & 1] = (double)rho;
+    if (ctx.leader()) st->rho[k & 1] = (double)rho;
     const bool first = k == 1;
     const T beta = first ? T(0) : rho / (T)st->rho[(k - 1) & 1];
-    TFLOWS_BLOCK_ROWS(n);
+    int rb, re;
+    ctx_rows(ctx, n, rb, re);
     for (int i = rb + threadIdx.x; i < re; i += kBlock) {
         T a, b;
         if (first) {
@@ -447,31 +500,38 @@ __global__ __launch_bounds__(kBlock) void k_cgs_u(int k, int n, const T* __restr
 
 // v2 = A p2 (:193-198) with (r2, v2) -> site D (:215-219)
 template <typename Op, typename T = typename OpValue<Op>::type>
-__global__ __launch_bounds__(kBlock) void k_cgs_spmv_v(Op A, const T* __restrict__ p2, T* __restrict__ v2,
-                                                       const T* __restrict__ r2, T* __restrict__ partD,
-                                                       const SolverState* st) {
-    if (st->done) return;
-    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
+__global__ __launch_bounds__(kBlock) void k_cgs_spmv_v(BatchCtx ctx, Op A, const T* __restrict__ p2,
+                                                       T* __restrict__ v2, const T* __restrict__ r2,
+                                                       T* __restrict__ partD, const SolverState* st_base) {
+    const SolverState* st = st_base + ctx.sys();
+    TF_DONE_RETURN(partD, 1);
     int t0, t1;
-    block_tiles(ntiles, t0, t1);
+    ctx_tiles(ctx, A.n, t0, t1);
     Gather1<T> g{p2};
     DotEpi<T> epi{r2, v2, {T(0)}};
-    for (int t = t0; t < t1; ++t) op_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+    for (int t = t0; t < t1; ++t) {
+        int r0, r1;
+        ctx_tile_rows(ctx, A.n, t, r0, r1);
+        op_tile<1>(A, r0, r1, g, epi);
+    }
     site_write<T, 1>(epi.acc, partD);
 }
 
 // alfa = rho/(r2,v2) (:220); q1 = u1 - alfa v2 (:226-228);
 // p1 = (u1+q1)/D (:238-241), or w = u1+q1 into p1 for the DIC apply.
 template <typename T, bool JAC>
-__global__ __launch_bounds__(kBlock) void k_cgs_q(int k, int n, const T* __restrict__ u1,
+__global__ __launch_bounds__(kBlock) void k_cgs_q(BatchCtx ctx, int k_arg, int n, const T* __restrict__ u1,
                                                   const T* __restrict__ v2, T* __restrict__ q1,
                                                   const T* __restrict__ D, T* __restrict__ p1,
-                                                  const T* __restrict__ partD, const SolverState* st) {
+                                                  const T* __restrict__ partD, const SolverState* st_base) {
+    const SolverState* st = st_base + ctx.sys();
     if (st->done) return;
+    const int k = ctx.k(k_arg);
     T rv[1];
-    site_total<T, 1>(partD, rv);
+    ctx_total<T, 1>(ctx, partD, rv);
     const T alfa = (T)st->rho[k & 1] / rv[0];
-    TFLOWS_BLOCK_ROWS(n);
+    int rb, re;
+    ctx_rows(ctx, n, rb, re);
     for (int i = rb + threadIdx.x; i < re; i += kBlock) {
         const T ui = u1[i];
         const T qi = ui - alfa * v2[i];
@@ -501,20 +561,25 @@ struct CgsREpi {
 };
 
 template <typename Op, typename T = typename OpValue<Op>::type>
-__global__ __launch_bounds__(kBlock) void k_cgs_spmv_r(int k, Op A, const T* __restrict__ p1, T* __restrict__ x,
-                                                       T* __restrict__ r1, const T* __restrict__ r2,
-                                                       const T* __restrict__ partD, T* __restrict__ partB,
-                                                       const SolverState* st) {
-    if (st->done) return;
+__global__ __launch_bounds__(kBlock) void k_cgs_spmv_r(BatchCtx ctx, int k_arg, Op A, const T* __restrict__ p1,
+                                                       T* __restrict__ x, T* __restrict__ r1,
+                                                       const T* __restrict__ r2, const T* __restrict__ partD,
+                                                       T* __restrict__ partB, const SolverState* st_base) {
+    const SolverState* st = st_base + ctx.sys();
+    TF_DONE_RETURN(partB, 2);
+    const int k = ctx.k(k_arg);
     T rv[1];
-    site_total<T, 1>(partD, rv);
+    ctx_total<T, 1>(ctx, partD, rv);
     const T alfa = (T)st->rho[k & 1] / rv[0];
-    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int t0, t1;
-    block_tiles(ntiles, t0, t1);
+    ctx_tiles(ctx, A.n, t0, t1);
     Gather1<T> g{p1};
     CgsREpi<T> epi{x, r1, r2, p1, alfa, {T(0), T(0)}};
-    for (int t = t0; t < t1; ++t) op_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+    for (int t = t0; t < t1; ++t) {
+        int r0, r1_;
+        ctx_tile_rows(ctx, A.n, t, r0, r1_);
+        op_tile<1>(A, r0, r1_, g, epi);
+    }
     site_write<T, 2>(epi.acc, partB);
 }
 
@@ -550,12 +615,6 @@ __global__ __launch_bounds__(kBlock) void k_resid(Csr<T> A, const T* __restrict_
         stream_tile<T, 1, true>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi);
 }
 
-// 1-based T-FlowS pattern -> 0-based device copy (TopolM layout).
-__global__ void k_rebase(int count, const int* __restrict__ in, int* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) out[i] = in[i] - 1;
-}
-
 template <typename TO, typename TI>
 __global__ void k_convert(int64_t count, const TI* __restrict__ in, TO* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -568,8 +627,9 @@ __global__ void k_convert(int64_t count, const TI* __restrict__ in, TO* __restri
 #define TF_LAUNCH(k, grid, stream) hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, stream
 
 template <typename T>
-void launch_setup(const Csr<T>& A, int prec, bool write_d, T* D, const T* b, const Sites<T>& S, hipStream_t s) {
-    TF_LAUNCH((k_setup<T>), S.grid, s), A, prec, write_d, D, b, S.A);
+void launch_setup(const BatchCtx& c, const Csr<T>& A, int prec, bool write_d, T* D, const T* b, const Sites<T>& S,
+                  hipStream_t s) {
+    TF_LAUNCH((k_setup<T>), S.grid, s), c, A, prec, write_d, D, b, S.A);
 }
 template <typename T>
 void launch_dic_setup_level(const Csr<T>& A, const int* rows, int count, T* D, hipStream_t s) {
@@ -586,83 +646,85 @@ void launch_dic_bwd_level(const Csr<T>& A, const int* rows, int count, const T* 
     if (count > 0) TF_LAUNCH((k_dic_bwd_level<T>), (count + kBlock - 1) / kBlock, s), A, rows, count, D, z, st);
 }
 template <typename T, bool Z>
-static void resid_init_mode(int mode, const Csr<T>& A, const T* x, T* r, T* r2, const T* D, const Sites<T>& S,
-                            SolverState* st, HostMirror* hm, hipStream_t s) {
+static void resid_init_mode(int mode, const BatchCtx& c, const Csr<T>& A, const T* x, T* r, T* r2, const T* D,
+                            const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
     switch (mode) {
-        case RHO_JAC: TF_LAUNCH((k_resid_init<T, RHO_JAC, Z>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
-        case RHO_BICG: TF_LAUNCH((k_resid_init<T, RHO_BICG, Z>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
-        case RHO_RR: TF_LAUNCH((k_resid_init<T, RHO_RR, Z>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
-        default: TF_LAUNCH((k_resid_init<T, RHO_NONE, Z>), S.grid, s), A, x, r, r2, D, S.A, S.B, st, hm); break;
+        case RHO_JAC: TF_LAUNCH((k_resid_init<T, RHO_JAC, Z>), S.grid, s), c, A, x, r, r2, D, S.A, S.B, st, hm); break;
+        case RHO_BICG: TF_LAUNCH((k_resid_init<T, RHO_BICG, Z>), S.grid, s), c, A, x, r, r2, D, S.A, S.B, st, hm); break;
+        case RHO_RR: TF_LAUNCH((k_resid_init<T, RHO_RR, Z>), S.grid, s), c, A, x, r, r2, D, S.A, S.B, st, hm); break;
+        default: TF_LAUNCH((k_resid_init<T, RHO_NONE, Z>), S.grid, s), c, A, x, r, r2, D, S.A, S.B, st, hm); break;
     }
 }
 template <typename T>
-void launch_resid_init(int mode, bool zero_x, const Csr<T>& A, const T* x, T* r, T* r2, const T* D,
-                       const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
-    if (zero_x) resid_init_mode<T, true>(mode, A, x, r, r2, D, S, st, hm, s);
-    else resid_init_mode<T, false>(mode, A, x, r, r2, D, S, st, hm, s);
+void launch_resid_init(int mode, bool zero_x, const BatchCtx& c, const Csr<T>& A, const T* x, T* r, T* r2,
+                       const T* D, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
+    if (zero_x) resid_init_mode<T, true>(mode, c, A, x, r, r2, D, S, st, hm, s);
+    else resid_init_mode<T, false>(mode, c, A, x, r, r2, D, S, st, hm, s);
 }
 template <typename T>
-void launch_check(int k, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
-    TF_LAUNCH((k_check<T>), S.grid, s), k, S.B, st, hm);
+void launch_check(const BatchCtx& c, int k, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
+    TF_LAUNCH((k_check<T>), S.grid, s), c, k, S.B, st, hm);
 }
 template <typename T>
-void launch_rho(int n, const T* a, const T* b, const Sites<T>& S, const SolverState* st, hipStream_t s) {
-    TF_LAUNCH((k_rho<T>), S.grid, s), n, a, b, S.C, st);
+void launch_rho(const BatchCtx& c, int n, const T* a, const T* b, const Sites<T>& S, const SolverState* st,
+                hipStream_t s) {
+    TF_LAUNCH((k_rho<T>), S.grid, s), c, n, a, b, S.C, st);
 }
 template <typename T>
-void launch_cg_p(bool jac, int k, int n, const T* r, const T* D, const T* z, T* p, const Sites<T>& S,
-                 SolverState* st, HostMirror* hm, hipStream_t s) {
-    if (jac) TF_LAUNCH((k_cg_p<T, true>), S.grid, s), k, n, r, D, z, p, S.B, S.C, st, hm);
-    else TF_LAUNCH((k_cg_p<T, false>), S.grid, s), k, n, r, D, z, p, S.B, S.C, st, hm);
+void launch_cg_p(const BatchCtx& c, bool jac, int k, int n, const T* r, const T* D, const T* z, T* p,
+                 const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
+    if (jac) TF_LAUNCH((k_cg_p<T, true>), S.grid, s), c, k, n, r, D, z, p, S.B, S.C, st, hm);
+    else TF_LAUNCH((k_cg_p<T, false>), S.grid, s), c, k, n, r, D, z, p, S.B, S.C, st, hm);
 }
 template <typename Op, typename T>
-void launch_cg_spmv(const Op& A, const T* p, T* q, const Sites<T>& S, const SolverState* st, hipStream_t s) {
-    TF_LAUNCH((k_cg_spmv<Op>), S.grid, s), A, p, q, S.D, st);
+void launch_cg_spmv(const BatchCtx& c, const Op& A, const T* p, T* q, const Sites<T>& S, const SolverState* st,
+                    hipStream_t s) {
+    TF_LAUNCH((k_cg_spmv<Op>), S.grid, s), c, A, p, q, S.D, st);
 }
 template <typename T>
-void launch_cg_update(bool jac, int k, int n, T* x, T* r, const T* p, const T* q, const T* D, const Sites<T>& S,
+void launch_cg_update(const BatchCtx& c, bool jac, int k, int n, T* x, T* r, const T* p, const T* q, const T* D,
+                      const Sites<T>& S, const SolverState* st, hipStream_t s) {
+    if (jac) TF_LAUNCH((k_cg_update<T, true>), S.grid, s), c, k, n, x, r, p, q, D, S.D, S.B, st);
+    else TF_LAUNCH((k_cg_update<T, false>), S.grid, s), c, k, n, x, r, p, q, D, S.D, S.B, st);
+}
+template <typename T>
+void launch_bicg_p(const BatchCtx& c, bool jac, int k, int n, const T* r1, const T* r2, const T* D, const T* z1,
+                   const T* z2, T* p1, T* p2, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
+    if (jac) TF_LAUNCH((k_bicg_p<T, true>), S.grid, s), c, k, n, r1, r2, D, z1, z2, p1, p2, S.B, S.C, st, hm);
+    else TF_LAUNCH((k_bicg_p<T, false>), S.grid, s), c, k, n, r1, r2, D, z1, z2, p1, p2, S.B, S.C, st, hm);
+}
+template <typename Op, typename T>
+void launch_bicg_spmv(const BatchCtx& c, const Op& A, const T* p1, const T* p2, T* q1, T* q2, const Sites<T>& S,
                       const SolverState* st, hipStream_t s) {
-    if (jac) TF_LAUNCH((k_cg_update<T, true>), S.grid, s), k, n, x, r, p, q, D, S.D, S.B, st);
-    else TF_LAUNCH((k_cg_update<T, false>), S.grid, s), k, n, x, r, p, q, D, S.D, S.B, st);
+    TF_LAUNCH((k_bicg_spmv<Op>), S.grid, s), c, A, p1, p2, q1, q2, S.D, st);
 }
 template <typename T>
-void launch_bicg_p(bool jac, int k, int n, const T* r1, const T* r2, const T* D, const T* z1, const T* z2, T* p1,
-                   T* p2, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
-    if (jac) TF_LAUNCH((k_bicg_p<T, true>), S.grid, s), k, n, r1, r2, D, z1, z2, p1, p2, S.B, S.C, st, hm);
-    else TF_LAUNCH((k_bicg_p<T, false>), S.grid, s), k, n, r1, r2, D, z1, z2, p1, p2, S.B, S.C, st, hm);
+void launch_bicg_update(const BatchCtx& c, bool jac, int k, int n, T* x, T* r1, T* r2, const T* p1, const T* q1,
+                        const T* q2, const T* D, const Sites<T>& S, const SolverState* st, hipStream_t s) {
+    if (jac) TF_LAUNCH((k_bicg_update<T, true>), S.grid, s), c, k, n, x, r1, r2, p1, q1, q2, D, S.D, S.B, st);
+    else TF_LAUNCH((k_bicg_update<T, false>), S.grid, s), c, k, n, x, r1, r2, p1, q1, q2, D, S.D, S.B, st);
+}
+template <typename T>
+void launch_cgs_u(const BatchCtx& c, bool jac, int k, int n, const T* r1, const T* q1, T* u1, T* u2, const T* D,
+                  T* p2, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
+    if (jac) TF_LAUNCH((k_cgs_u<T, true>), S.grid, s), c, k, n, r1, q1, u1, u2, D, p2, S.B, st, hm);
+    else TF_LAUNCH((k_cgs_u<T, false>), S.grid, s), c, k, n, r1, q1, u1, u2, D, p2, S.B, st, hm);
 }
 template <typename Op, typename T>
-void launch_bicg_spmv(const Op& A, const T* p1, const T* p2, T* q1, T* q2, const Sites<T>& S,
-                      const SolverState* st, hipStream_t s) {
-    TF_LAUNCH((k_bicg_spmv<Op>), S.grid, s), A, p1, p2, q1, q2, S.D, st);
-}
-template <typename T>
-void launch_bicg_update(bool jac, int k, int n, T* x, T* r1, T* r2, const T* p1, const T* q1, const T* q2,
-                        const T* D, const Sites<T>& S, const SolverState* st, hipStream_t s) {
-    if (jac) TF_LAUNCH((k_bicg_update<T, true>), S.grid, s), k, n, x, r1, r2, p1, q1, q2, D, S.D, S.B, st);
-    else TF_LAUNCH((k_bicg_update<T, false>), S.grid, s), k, n, x, r1, r2, p1, q1, q2, D, S.D, S.B, st);
-}
-template <typename T>
-void launch_cgs_u(bool jac, int k, int n, const T* r1, const T* q1, T* u1, T* u2, const T* D, T* p2,
-                  const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
-    if (jac) TF_LAUNCH((k_cgs_u<T, true>), S.grid, s), k, n, r1, q1, u1, u2, D, p2, S.B, st, hm);
-    else TF_LAUNCH((k_cgs_u<T, false>), S.grid, s), k, n, r1, q1, u1, u2, D, p2, S.B, st, hm);
-}
-template <typename Op, typename T>
-void launch_cgs_spmv_v(const Op& A, const T* p2, T* v2, const T* r2, const Sites<T>& S, const SolverState* st,
-                       hipStream_t s) {
-    TF_LAUNCH((k_cgs_spmv_v<Op>), S.grid, s), A, p2, v2, r2, S.D, st);
-}
-template <typename T>
-void launch_cgs_q(bool jac, int k, int n, const T* u1, const T* v2, T* q1, const T* D, T* p1, const Sites<T>& S,
-                  const SolverState* st, hipStream_t s) {
-    if (jac) TF_LAUNCH((k_cgs_q<T, true>), S.grid, s), k, n, u1, v2, q1, D, p1, S.D, st);
-    else TF_LAUNCH((k_cgs_q<T, false>), S.grid, s), k, n, u1, v2, q1, D, p1, S.D, st);
-}
-template <typename Op, typename T>
-void launch_cgs_spmv_r(int k, const Op& A, const T* p1, T* x, T* r1, const T* r2, const Sites<T>& S,
+void launch_cgs_spmv_v(const BatchCtx& c, const Op& A, const T* p2, T* v2, const T* r2, const Sites<T>& S,
                        const SolverState* st, hipStream_t s) {
-    TF_LAUNCH((k_cgs_spmv_r<Op>), S.grid, s), k, A, p1, x, r1, r2, S.D, S.B, st);
+    TF_LAUNCH((k_cgs_spmv_v<Op>), S.grid, s), c, A, p2, v2, r2, S.D, st);
+}
+template <typename T>
+void launch_cgs_q(const BatchCtx& c, bool jac, int k, int n, const T* u1, const T* v2, T* q1, const T* D, T* p1,
+                  const Sites<T>& S, const SolverState* st, hipStream_t s) {
+    if (jac) TF_LAUNCH((k_cgs_q<T, true>), S.grid, s), c, k, n, u1, v2, q1, D, p1, S.D, st);
+    else TF_LAUNCH((k_cgs_q<T, false>), S.grid, s), c, k, n, u1, v2, q1, D, p1, S.D, st);
+}
+template <typename Op, typename T>
+void launch_cgs_spmv_r(const BatchCtx& c, int k, const Op& A, const T* p1, T* x, T* r1, const T* r2,
+                       const Sites<T>& S, const SolverState* st, hipStream_t s) {
+    TF_LAUNCH((k_cgs_spmv_r<Op>), S.grid, s), c, k, A, p1, x, r1, r2, S.D, S.B, st);
 }
 template <typename Op, typename T>
 void launch_spmv(const Op& A, const T* x, T* y, int grid, hipStream_t s) {
@@ -672,9 +734,6 @@ template <typename T>
 void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s) {
     TF_LAUNCH((k_resid<T>), grid, s), A, x, r);
 }
-void launch_rebase(int count, const int* in, int* out, hipStream_t s) {
-    if (count > 0) hipLaunchKernelGGL(k_rebase, dim3((count + 255) / 256), dim3(256), 0, s, count, in, out);
-}
 void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t s) {
     if (count > 0)
         hipLaunchKernelGGL((k_convert<float, double>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
@@ -682,39 +741,42 @@ void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t 
 }
 
 #define TF_INSTANTIATE(T)                                                                                        \
-    template void launch_setup<T>(const Csr<T>&, int, bool, T*, const T*, const Sites<T>&, hipStream_t);        \
+    template void launch_setup<T>(const BatchCtx&, const Csr<T>&, int, bool, T*, const T*, const Sites<T>&,     \
+                                  hipStream_t);                                                                  \
     template void launch_dic_setup_level<T>(const Csr<T>&, const int*, int, T*, hipStream_t);                   \
     template void launch_dic_fwd_level<T>(const Csr<T>&, const int*, int, const T*, const T*, T*,               \
                                           const SolverState*, hipStream_t);                                      \
     template void launch_dic_bwd_level<T>(const Csr<T>&, const int*, int, const T*, T*, const SolverState*,      \
                                           hipStream_t);                                                          \
-    template void launch_resid_init<T>(int, bool, const Csr<T>&, const T*, T*, T*, const T*, const Sites<T>&,   \
-                                       SolverState*, HostMirror*, hipStream_t);                                  \
-    template void launch_check<T>(int, const Sites<T>&, SolverState*, HostMirror*, hipStream_t);                 \
-    template void launch_rho<T>(int, const T*, const T*, const Sites<T>&, const SolverState*, hipStream_t);      \
-    template void launch_cg_p<T>(bool, int, int, const T*, const T*, const T*, T*, const Sites<T>&,             \
-                                 SolverState*, HostMirror*, hipStream_t);                                        \
-    template void launch_cg_update<T>(bool, int, int, T*, T*, const T*, const T*, const T*, const Sites<T>&,    \
-                                      const SolverState*, hipStream_t);                                          \
-    template void launch_bicg_p<T>(bool, int, int, const T*, const T*, const T*, const T*, const T*, T*, T*,    \
-                                   const Sites<T>&, SolverState*, HostMirror*, hipStream_t);                     \
-    template void launch_bicg_update<T>(bool, int, int, T*, T*, T*, const T*, const T*, const T*, const T*,     \
-                                        const Sites<T>&, const SolverState*, hipStream_t);                       \
-    template void launch_cgs_u<T>(bool, int, int, const T*, const T*, T*, T*, const T*, T*, const Sites<T>&,    \
-                                  SolverState*, HostMirror*, hipStream_t);                                       \
-    template void launch_cgs_q<T>(bool, int, int, const T*, const T*, T*, const T*, T*, const Sites<T>&,        \
-                                  const SolverState*, hipStream_t);                                              \
+    template void launch_resid_init<T>(int, bool, const BatchCtx&, const Csr<T>&, const T*, T*, T*, const T*,   \
+                                       const Sites<T>&, SolverState*, HostMirror*, hipStream_t);                 \
+    template void launch_check<T>(const BatchCtx&, int, const Sites<T>&, SolverState*, HostMirror*,             \
+                                  hipStream_t);                                                                  \
+    template void launch_rho<T>(const BatchCtx&, int, const T*, const T*, const Sites<T>&, const SolverState*,  \
+                                hipStream_t);                                                                    \
+    template void launch_cg_p<T>(const BatchCtx&, bool, int, int, const T*, const T*, const T*, T*,             \
+                                 const Sites<T>&, SolverState*, HostMirror*, hipStream_t);                       \
+    template void launch_cg_update<T>(const BatchCtx&, bool, int, int, T*, T*, const T*, const T*, const T*,    \
+                                      const Sites<T>&, const SolverState*, hipStream_t);                         \
+    template void launch_bicg_p<T>(const BatchCtx&, bool, int, int, const T*, const T*, const T*, const T*,     \
+                                   const T*, T*, T*, const Sites<T>&, SolverState*, HostMirror*, hipStream_t);   \
+    template void launch_bicg_update<T>(const BatchCtx&, bool, int, int, T*, T*, T*, const T*, const T*,        \
+                                        const T*, const T*, const Sites<T>&, const SolverState*, hipStream_t);   \
+    template void launch_cgs_u<T>(const BatchCtx&, bool, int, int, const T*, const T*, T*, T*, const T*, T*,    \
+                                  const Sites<T>&, SolverState*, HostMirror*, hipStream_t);                      \
+    template void launch_cgs_q<T>(const BatchCtx&, bool, int, int, const T*, const T*, T*, const T*, T*,        \
+                                  const Sites<T>&, const SolverState*, hipStream_t);                             \
     template void launch_resid<T>(const Csr<T>&, const T*, T*, int, hipStream_t);
 
 #define TF_INSTANTIATE_OP(Op, T)                                                                                 \
-    template void launch_cg_spmv<Op, T>(const Op&, const T*, T*, const Sites<T>&, const SolverState*,           \
-                                        hipStream_t);                                                            \
-    template void launch_bicg_spmv<Op, T>(const Op&, const T*, const T*, T*, T*, const Sites<T>&,               \
-                                          const SolverState*, hipStream_t);                                      \
-    template void launch_cgs_spmv_v<Op, T>(const Op&, const T*, T*, const T*, const Sites<T>&,                  \
+    template void launch_cg_spmv<Op, T>(const BatchCtx&, const Op&, const T*, T*, const Sites<T>&,              \
+                                        const SolverState*, hipStream_t);                                        \
+    template void launch_bicg_spmv<Op, T>(const BatchCtx&, const Op&, const T*, const T*, T*, T*,               \
+                                          const Sites<T>&, const SolverState*, hipStream_t);                     \
+    template void launch_cgs_spmv_v<Op, T>(const BatchCtx&, const Op&, const T*, T*, const T*, const Sites<T>&, \
                                            const SolverState*, hipStream_t);                                     \
-    template void launch_cgs_spmv_r<Op, T>(int, const Op&, const T*, T*, T*, const T*, const Sites<T>&,         \
-                                           const SolverState*, hipStream_t);                                     \
+    template void launch_cgs_spmv_r<Op, T>(const BatchCtx&, int, const Op&, const T*, T*, T*, const T*,         \
+                                           const Sites<T>&, const SolverState*, hipStream_t);                    \
     template void launch_spmv<Op, T>(const Op&, const T*, T*, int, hipStream_t);
 
 TF_INSTANTIATE(double)

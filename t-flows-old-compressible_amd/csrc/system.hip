@@ -176,7 +176,7 @@ void System::prec1(int prec, double* d, hipStream_t s) {
     ws_.ensure((size_t)V_COUNT * n_);
     TF_CHECK(hipMemsetAsync(ws_.p, 0, sizeof(double) * n_, s));
     if (prec == 2) dic_setup(d, s);
-    launch_setup(csr(), prec, prec != 2, d, (const double*)ws_.p, sites(), s);
+    launch_setup(BatchCtx{}, csr(), prec, prec != 2, d, (const double*)ws_.p, sites(), s);
     TF_CHECK(hipGetLastError());
 }
 
@@ -215,36 +215,36 @@ void enqueue_iteration(int method, bool jac, int k, const Op& Aop, const Csr<T>&
                 break;
             }
             if (!jac) {  // stopping test, z = M^-1 r into q, rho = (r, z)
-                launch_check(k, S, st, hm, s);
+                launch_check(BatchCtx{}, k, S, st, hm, s);
                 dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.r1, w.q, st, s);
-                launch_rho(n, (const T*)w.r1, (const T*)w.q, S, st, s);
+                launch_rho(BatchCtx{}, n, (const T*)w.r1, (const T*)w.q, S, st, s);
             }
-            launch_cg_p(jac, k, n, (const T*)w.r1, (const T*)w.D, (const T*)w.q, w.p, S, st, hm, s);
-            launch_cg_spmv(Aop, (const T*)w.p, w.q, S, st, s);
-            launch_cg_update(jac, k, n, w.x, w.r1, (const T*)w.p, (const T*)w.q, (const T*)w.D, S, st, s);
+            launch_cg_p(BatchCtx{}, jac, k, n, (const T*)w.r1, (const T*)w.D, (const T*)w.q, w.p, S, st, hm, s);
+            launch_cg_spmv(BatchCtx{}, Aop, (const T*)w.p, w.q, S, st, s);
+            launch_cg_update(BatchCtx{}, jac, k, n, w.x, w.r1, (const T*)w.p, (const T*)w.q, (const T*)w.D, S, st, s);
             break;
         }
         case 1: {  // BiCG (reference semantics)
             if (!jac) {
-                launch_check(k, S, st, hm, s);
+                launch_check(BatchCtx{}, k, S, st, hm, s);
                 dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.r1, w.q, st, s);
                 dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.r2, w.q2, st, s);
-                launch_rho(n, (const T*)w.q, (const T*)w.r2, S, st, s);
+                launch_rho(BatchCtx{}, n, (const T*)w.q, (const T*)w.r2, S, st, s);
             }
-            launch_bicg_p(jac, k, n, (const T*)w.r1, (const T*)w.r2, (const T*)w.D, (const T*)w.q, (const T*)w.q2,
+            launch_bicg_p(BatchCtx{}, jac, k, n, (const T*)w.r1, (const T*)w.r2, (const T*)w.D, (const T*)w.q, (const T*)w.q2,
                           w.p, w.p2, S, st, hm, s);
-            launch_bicg_spmv(Aop, (const T*)w.p, (const T*)w.p2, w.q, w.q2, S, st, s);
-            launch_bicg_update(jac, k, n, w.x, w.r1, w.r2, (const T*)w.p, (const T*)w.q, (const T*)w.q2,
+            launch_bicg_spmv(BatchCtx{}, Aop, (const T*)w.p, (const T*)w.p2, w.q, w.q2, S, st, s);
+            launch_bicg_update(BatchCtx{}, jac, k, n, w.x, w.r1, w.r2, (const T*)w.p, (const T*)w.q, (const T*)w.q2,
                                (const T*)w.D, S, st, s);
             break;
         }
         default: {  // CGS: u1, u2, p2 = M^-1 u2, v2 = A p2, q1 (q), p1 (p) = M^-1 (u1+q1)
-            launch_cgs_u(jac, k, n, (const T*)w.r1, (const T*)w.q, w.u1, w.u2, (const T*)w.D, w.p2, S, st, hm, s);
+            launch_cgs_u(BatchCtx{}, jac, k, n, (const T*)w.r1, (const T*)w.q, w.u1, w.u2, (const T*)w.D, w.p2, S, st, hm, s);
             if (!jac) dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.u2, w.p2, st, s);
-            launch_cgs_spmv_v(Aop, (const T*)w.p2, w.v2, (const T*)w.r2, S, st, s);
-            launch_cgs_q(jac, k, n, (const T*)w.u1, (const T*)w.v2, w.q, (const T*)w.D, w.p, S, st, s);
+            launch_cgs_spmv_v(BatchCtx{}, Aop, (const T*)w.p2, w.v2, (const T*)w.r2, S, st, s);
+            launch_cgs_q(BatchCtx{}, jac, k, n, (const T*)w.u1, (const T*)w.v2, w.q, (const T*)w.D, w.p, S, st, s);
             if (!jac) dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.p, w.p, st, s);
-            launch_cgs_spmv_r(k, Aop, (const T*)w.p, w.x, w.r1, (const T*)w.r2, S, st, s);
+            launch_cgs_spmv_r(BatchCtx{}, k, Aop, (const T*)w.p, w.x, w.r1, (const T*)w.r2, S, st, s);
             break;
         }
     }
@@ -289,7 +289,7 @@ void pace_iterations(int method, bool jac, int cap, const Op& Aop, const Csr<T>&
         if (method == 0 && jac && fused_cg_enabled())
             launch_cgf_check(false, S, (const double*)nullptr, st, hm, s);
         else
-            launch_check(std::max(cap, 0) + 1, S, st, hm, s);
+            launch_check(BatchCtx{}, std::max(cap, 0) + 1, S, st, hm, s);
     }
     TF_CHECK(hipGetLastError());
     TF_CHECK(hipStreamSynchronize(s));
@@ -312,14 +312,14 @@ void run_solve(int method, int prec, int cap, const Op& Aop, const Csr<T>& A, co
     if (prec == 2 && !d_ready)
         for (int l = 0; l < L->nfwd; ++l)
             launch_dic_setup_level(A, L->fwd_rows.p + L->fwd_ptr[l], L->fwd_ptr[l + 1] - L->fwd_ptr[l], w.D, s);
-    launch_setup(A, prec, jac && !d_ready, w.D, (const T*)w.r1, S, s);
+    launch_setup(BatchCtx{}, A, prec, jac && !d_ready, w.D, (const T*)w.r1, S, s);
     int mode = RHO_NONE_;
     T* r2 = nullptr;
     const bool fused = method == 0 && jac && fused_cg_enabled();
     if (method == 0) mode = (jac && !fused) ? RHO_JAC_ : RHO_NONE_;
     if (method == 1) { mode = jac ? RHO_BICG_ : RHO_NONE_; r2 = w.r2; }
     if (method == 2) { mode = RHO_RR_; r2 = w.r2; }
-    launch_resid_init(mode, zero_x, A, (const T*)w.x, w.r1, r2, (const T*)w.D, S, st, hm, s);
+    launch_resid_init(mode, zero_x, BatchCtx{}, A, (const T*)w.x, w.r1, r2, (const T*)w.D, S, st, hm, s);
     if (fused) launch_cgf_init_z(A.n, (const T*)w.r1, (const T*)w.D, w.u1, S, st, s);
     TF_CHECK(hipGetLastError());
     pace_iterations<T, Op>(method, jac, cap, Aop, A, w, S, st, hm, L, s);
