@@ -167,6 +167,57 @@ int tflows_resid(tflows_system s, const double* x, double* r, void* stream);
  * (0 before the first prec = 2 use); for reporting. */
 int32_t tflows_system_levels(tflows_system s);
 
+/* SpMV form of the system's solves and products: lanes per matrix row
+ * (1 = one row per thread with LDS-staged products, 64 rows per wavefront;
+ * 2, 4, 8 = lane groups streaming the row, 32 / 16 / 8 rows per
+ * wavefront).  Every form sums each row in column order (the reference's
+ * order, CG.f90:193-199), so results are identical.  No counterpart in the
+ * reference. */
+int tflows_system_set_lanes(tflows_system s, int32_t lanes);
+
+/* ------------------------------------------------------------------------
+ * Batches of independent systems (no counterpart in the reference, whose
+ * callers solve one system per call: NewUVW.f90:506 once per velocity
+ * component, CalcZmix.f90:394 per scalar).  K systems of any sizes and
+ * patterns are solved together: every kernel of an iteration runs once for
+ * all of them, each system follows the reference loop on its own matrix
+ * with its own stopping test, cap and result, and the iterations are
+ * replayed from captured HIP graphs.  Vectors are device arrays holding the
+ * K systems' rows back to back (system k starts at row_offsets[k]).
+ * ------------------------------------------------------------------------ */
+typedef struct tflows_batch_s* tflows_batch;
+
+/* K systems, each in the T-FlowS layout (host arrays acol[k], arow[k],
+ * adia[k] of system k, sizes n[k], nonz[k]). */
+int tflows_batch_create(int32_t k, const int32_t* n, const int32_t* nonz, const int32_t* const* acol,
+                        const int32_t* const* arow, const int32_t* const* adia, tflows_batch* out);
+int tflows_batch_destroy(tflows_batch b);
+/* Values A(1:NONZ-1) of system `sys`, or of all systems back to back
+ * (sys = -1); host or device memory (mem). */
+int tflows_batch_set_values(tflows_batch b, int32_t sys, const double* aval, int32_t mem, void* stream);
+/* Total rows and entries, and the K+1 row offsets (any output may be NULL). */
+int tflows_batch_layout(tflows_batch b, int32_t* rows, int64_t* entries, int32_t* row_offsets);
+/* Launch configuration: blocks per launch, lanes per row, iterations per
+ * graph replay (0 = chosen by the autotuner / default), and whether blocks
+ * of finished systems are handed to running ones (rebalance != 0). */
+int tflows_batch_configure(tflows_batch b, int32_t grid, int32_t lanes, int32_t chunk, int32_t rebalance);
+/* Configuration in use for (bits = 64 or 32, method, prec) and the number
+ * of rebalances so far. */
+int tflows_batch_config(tflows_batch b, int32_t bits, int32_t method, int32_t prec, int32_t* grid, int32_t* lanes,
+                        int32_t* chunk, int32_t* rebalances);
+/* FP64 solve of all K systems with `method` / `prec` (as tflows_solve):
+ * x, r1 device arrays (r1 in: b, out: final residual); cap[k], tol[k] per
+ * system (host); niter[k], ini_res[k] (written unless the system took the
+ * ||b|| < tol exit), fin_res[k] (in: the value that exit returns) out. */
+int tflows_batch_solve(tflows_batch b, int32_t method, int32_t prec, double* x, double* r1, const int32_t* cap,
+                       const double* tol, int32_t* niter, double* ini_res, double* fin_res, void* stream);
+/* FP32 inner solves inside FP64 iterative refinement, per system (as
+ * tflows_solve_mixed); rhs is not modified; r_out may be NULL. */
+int tflows_batch_solve_mixed(tflows_batch b, int32_t method, int32_t prec, double* x, const double* rhs,
+                             double* r_out, const int32_t* cap, const double* tol, double inner_rtol,
+                             int32_t max_outer, int32_t* niter, double* ini_res, double* fin_res,
+                             int32_t* outer_steps, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

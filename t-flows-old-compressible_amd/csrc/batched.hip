@@ -1,0 +1,701 @@
+// batched.hip -- K independent systems solved by one set of launches.
+//
+// The K matrices are concatenated block-diagonally into one System (rows
+// and entries of system k follow those of system k-1).  Every kernel of the
+// CG / BiCG / CGS iteration (krylov_kernels.hip) runs under a BatchCtx that
+// maps each block to one system and a range of that system's 256-row tiles;
+// reductions, alfa/beta, the stopping test and the SolverState are per
+// system, so every system follows exactly the reference loop
+// (Process/CG.f90:113-262, BiCG.f90:100-254, CGS.f90:112-337) on its own
+// matrix while they share the launches.
+//
+// Launch machinery:
+//  * a chunk of C iterations is captured once into a HIP graph; the
+//    iteration index of each kernel is a launch constant plus a device
+//    counter (kbase) that the chunk's last node advances, so the same graph
+//    is replayed until every system has stopped;
+//  * the host keeps one replay queued behind the running one and reads the
+//    per-system `done` flags the leaders publish into pinned host memory;
+//  * when the finished systems leave enough idle blocks, the block maps are
+//    rebuilt for the running systems only (the graph reads the maps from
+//    device memory, so it stays valid); the one reduction site that crosses
+//    an iteration boundary (site B) is collapsed to one value per system
+//    under the old maps and re-expanded under the new ones;
+//  * the grid size and the SpMV form (lanes per row) come from a small
+//    autotuner that times a few iterations of every candidate on the
+//    caller's own systems, once per (precision, method, preconditioner).
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <numeric>
+
+#include "batched.h"
+
+namespace tflows {
+
+namespace {
+constexpr int kMaxGrid = 4096;
+// divergence guard of the FP32 inner solves (their residual starts at norm
+// 1): beyond 1e6 the rounding of FP32 recurrences (~1e-7 relative) exceeds
+// the unit right-hand side and the correction is worthless
+constexpr double kInnerDivLimit = 1e6;
+int round8(int v) { return (v + kNumXcd - 1) / kNumXcd * kNumXcd; }
+const Levels kNoLevels{};
+}  // namespace
+
+__global__ void k_set_int(int* p, int v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+__global__ void k_advance(int* kbase, int c) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *kbase += c;
+}
+
+// Site B of each running system summed under the current maps into
+// st->aux[0..1] (by its leader).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_site_collapse(BatchCtx c, const T* __restrict__ partB,
+                                                          SolverState* st_base) {
+    SolverState* st = st_base + c.sys();
+    if (st->done) return;
+    T tot[2];
+    ctx_total<T, 2>(c, partB, tot);
+    if (c.leader()) {
+        st->aux[0] = (double)tot[0];
+        st->aux[1] = (double)tot[1];
+    }
+}
+
+// Site B re-expanded under the new maps: the system's first block carries
+// the collapsed totals, its other blocks zeros (x + 0 == x, so the totals
+// read back unchanged).
+template <typename T>
+__global__ void k_site_expand(BatchCtx c, T* __restrict__ partB, const SolverState* st_base) {
+    if (threadIdx.x != 0) return;
+    const int b = blockIdx.x, G = gridDim.x, sys = c.sys();
+    const SolverState* st = st_base + sys;
+    T v0 = T(0), v1 = T(0);
+    if (!st->done && b == c.sys_blk0[sys]) {
+        v0 = (T)st->aux[0];
+        v1 = (T)st->aux[1];
+    }
+    partB[b] = v0;
+    partB[G + b] = v1;
+}
+
+// Per-tile sums of v^2 (a tile never straddles two systems).
+__global__ __launch_bounds__(kBlock) void k_tile_norm2(int ntiles, const int* __restrict__ tile_r0,
+                                                       const double* __restrict__ v, double* __restrict__ out) {
+    __shared__ double sm[kWavesPerBlock];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int r = tile_r0[t] + (int)threadIdx.x;
+        double a[1] = {r < tile_r0[t + 1] ? v[r] * v[r] : 0.0};
+        block_sum<double, 1>(a, sm);
+        if (threadIdx.x == 0) out[t] = a[0];
+    }
+}
+
+// r32 = fp32(r64 * inv[sys]); x32 = 0
+__global__ __launch_bounds__(kBlock) void k_tile_scale(int ntiles, const int* __restrict__ tile_r0,
+                                                       const int* __restrict__ tile_sys,
+                                                       const double* __restrict__ r64,
+                                                       const double* __restrict__ inv, float* __restrict__ r32,
+                                                       float* __restrict__ x32) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int r = tile_r0[t] + (int)threadIdx.x;
+        if (r < tile_r0[t + 1]) {
+            r32[r] = (float)(r64[r] * inv[tile_sys[t]]);
+            x32[r] = 0.0f;
+        }
+    }
+}
+
+// x64 += s[sys] * fp64(d32); r64 = b
+__global__ __launch_bounds__(kBlock) void k_tile_refine(int ntiles, const int* __restrict__ tile_r0,
+                                                        const int* __restrict__ tile_sys, double* __restrict__ x64,
+                                                        const double* __restrict__ scale,
+                                                        const float* __restrict__ d32,
+                                                        const double* __restrict__ b, double* __restrict__ r64) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int r = tile_r0[t] + (int)threadIdx.x;
+        if (r < tile_r0[t + 1]) {
+            x64[r] = x64[r] + scale[tile_sys[t]] * (double)d32[r];
+            r64[r] = b[r];
+        }
+    }
+}
+
+// x = xprev on the systems whose flag is set (a rejected refinement step)
+__global__ __launch_bounds__(kBlock) void k_tile_restore(int ntiles, const int* __restrict__ tile_r0,
+                                                         const int* __restrict__ tile_sys, double* __restrict__ x,
+                                                         const double* __restrict__ xprev,
+                                                         const double* __restrict__ flag) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int r = tile_r0[t] + (int)threadIdx.x;
+        if (r < tile_r0[t + 1] && flag[tile_sys[t]] != 0.0) x[r] = xprev[r];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill32(int n, float v, float* __restrict__ a) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+Batch::Batch(int k, const int* n, const int* nonz, const int* const* acol, const int* const* arow,
+             const int* const* adia)
+    : k_(k) {
+    if (k <= 0) throw std::invalid_argument("a batch needs at least one system");
+    roff_.assign(k + 1, 0);
+    eoff_.assign(k + 1, 0);
+    int64_t rows = 0, ents = 0;
+    for (int i = 0; i < k; ++i) {
+        if (n[i] <= 0) throw std::invalid_argument("every system needs N > 0");
+        if (nonz[i] < n[i] + 1) throw std::invalid_argument("NONZ must be at least N + 1 for every system");
+        if (acol[i][0] != 1 || acol[i][n[i]] != nonz[i])
+            throw std::invalid_argument("Acol of every system must run from 1 to its NONZ");
+        rows += n[i];
+        ents += nonz[i] - 1;
+        if (rows >= (1LL << 31) - 1 || ents >= (1LL << 31) - 2)
+            throw std::invalid_argument("the batch exceeds 2^31 rows or entries");
+        roff_[i + 1] = (int)rows;
+        eoff_[i + 1] = (int)ents;
+    }
+    n_tot_ = (int)rows;
+    nnz_tot_ = ents;
+    // concatenated 1-based layout; each system's columns must stay in its
+    // own block (the System constructor checks the rest of the layout)
+    std::vector<int> cacol(n_tot_ + 1), carow(nnz_tot_), cadia(n_tot_);
+    for (int i = 0; i < k; ++i) {
+        const int r0 = roff_[i], e0 = eoff_[i], ni = n[i], nzi = nonz[i] - 1;
+        for (int r = 0; r < ni; ++r) {
+            cacol[r0 + r] = acol[i][r] + e0;
+            cadia[r0 + r] = adia[i][r] + e0;
+        }
+        for (int e = 0; e < nzi; ++e) {
+            const int c = arow[i][e];
+            if (c < 1 || c > ni) throw std::invalid_argument("Arow of a system holds a column outside 1..N");
+            carow[e0 + e] = c + r0;
+        }
+    }
+    cacol[n_tot_] = (int)nnz_tot_ + 1;
+    TF_CHECK(hipStreamCreateWithFlags(&bs_, hipStreamNonBlocking));
+    sys_.reset(new System(n_tot_, (int)nnz_tot_ + 1, cacol.data(), carow.data(), cadia.data(), false, bs_));
+
+    tiles_.assign(k, 0);
+    tile0_.assign(k + 1, 0);
+    for (int i = 0; i < k; ++i) {
+        tiles_[i] = (n[i] + kTileRows - 1) / kTileRows;
+        tile0_[i + 1] = tile0_[i] + tiles_[i];
+    }
+    const int T = tile0_[k];
+    std::vector<int> tr0(T + 1), tsys(T);
+    for (int i = 0; i < k; ++i)
+        for (int j = 0; j < tiles_[i]; ++j) {
+            tr0[tile0_[i] + j] = roff_[i] + j * kTileRows;
+            tsys[tile0_[i] + j] = i;
+        }
+    tr0[T] = n_tot_;
+    tile_r0_.ensure(T + 1);
+    tile_sys_.ensure(T);
+    TF_CHECK(hipMemcpy(tile_r0_.p, tr0.data(), sizeof(int) * (T + 1), hipMemcpyHostToDevice));
+    TF_CHECK(hipMemcpy(tile_sys_.p, tsys.data(), sizeof(int) * T, hipMemcpyHostToDevice));
+
+    gmax_ = round8(std::max(kMaxGrid, k));
+    const size_t map_ints = 3 * (size_t)gmax_ + k + 1;
+    maps_.ensure(map_ints);
+    TF_CHECK(hipHostMalloc((void**)&h_maps_, sizeof(int) * map_ints, hipHostMallocDefault));
+    kbase_.ensure(1);
+    const size_t nn = (size_t)n_tot_;
+    ws_.ensure((V_COUNT + 3) * nn);
+    ws32_.ensure((V_COUNT + 2) * nn);
+    partials_.ensure(Sites<double>::elems(gmax_));
+    partials32_.ensure(Sites<float>::elems(gmax_));
+    tnorm_.ensure(T);
+    scale_.ensure(k);
+    st_.ensure(k);
+    st32_.ensure(k);
+    TF_CHECK(hipHostMalloc((void**)&hm_, sizeof(HostMirror) * k, hipHostMallocMapped | hipHostMallocCoherent));
+    TF_CHECK(hipHostMalloc((void**)&hm32_, sizeof(HostMirror) * k, hipHostMallocMapped | hipHostMallocCoherent));
+    for (int i = 0; i < 2; ++i) TF_CHECK(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
+    for (int i = 2; i < 4; ++i) TF_CHECK(hipEventCreate(&ev_[i]));
+    TF_CHECK(hipEventCreateWithFlags(&ev_[4], hipEventDisableTiming));
+    TF_CHECK(hipStreamSynchronize(bs_));
+}
+
+Batch::~Batch() {
+    for (auto& g : graphs_) (void)hipGraphExecDestroy(g.second);
+    if (bs_) (void)hipStreamSynchronize(bs_);
+    for (auto& e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (h_maps_) (void)hipHostFree(h_maps_);
+    if (hm_) (void)hipHostFree(hm_);
+    if (hm32_) (void)hipHostFree(hm32_);
+    sys_.reset();
+    if (bs_) (void)hipStreamDestroy(bs_);
+}
+
+void Batch::set_values(int sys, const double* aval, bool device_ptr, hipStream_t s) {
+    if (sys < -1 || sys >= k_) throw std::invalid_argument("system index out of range");
+    if (sys < 0) sys_->set_values(aval, device_ptr, s);
+    else sys_->set_values_range(eoff_[sys], eoff_[sys + 1] - eoff_[sys], aval, device_ptr, s);
+}
+
+void Batch::join(hipStream_t s) {
+    TF_CHECK(hipEventRecord(ev_[4], s));
+    TF_CHECK(hipStreamWaitEvent(bs_, ev_[4], 0));
+}
+
+// Blocks per running system in proportion to its tiles (at least one
+// each); inside a system, the blocks that the hardware sends to the same
+// XCD (block index mod 8) own consecutive tiles.
+void Batch::build_maps(const std::vector<char>& active, int G) {
+    const int K = k_;
+    int* blk_sys = h_maps_;
+    int* blk_tiles = h_maps_ + gmax_;
+    int* sys_blk0 = h_maps_ + 3 * gmax_;
+    std::vector<char> act(active);
+    int64_t tact = 0;
+    int ka = 0;
+    for (int i = 0; i < K; ++i)
+        if (act[i]) {
+            tact += tiles_[i];
+            ++ka;
+        }
+    if (ka == 0) {  // nothing left: keep every system mapped (all return at once)
+        std::fill(act.begin(), act.end(), 1);
+        ka = K;
+        tact = tile0_[K];
+    }
+    if (ka > G) throw std::invalid_argument("the grid must have at least one block per system");
+    std::vector<int> nb(K, 0);
+    std::vector<std::pair<double, int>> frac;
+    const int spare = G - ka;
+    int used = 0;
+    for (int i = 0; i < K; ++i) {
+        if (!act[i]) continue;
+        const double q = (double)spare * tiles_[i] / (double)tact;
+        const int f = (int)q;
+        nb[i] = 1 + f;
+        used += nb[i];
+        frac.push_back({q - f, i});
+    }
+    std::stable_sort(frac.begin(), frac.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    for (size_t j = 0; used < G; ++j, ++used) nb[frac[j % frac.size()].second]++;
+    int b = 0;
+    std::vector<int> order;
+    for (int i = 0; i < K; ++i) {
+        sys_blk0[i] = b;
+        const int cnt = nb[i];
+        order.resize(cnt);
+        std::iota(order.begin(), order.end(), b);
+        std::stable_sort(order.begin(), order.end(),
+                         [](int x, int y) { return (x & (kNumXcd - 1)) < (y & (kNumXcd - 1)); });
+        for (int j = 0; j < cnt; ++j) {
+            const int blk = order[j];
+            blk_sys[blk] = i;
+            blk_tiles[2 * blk] = tile0_[i] + (int)((int64_t)tiles_[i] * j / cnt);
+            blk_tiles[2 * blk + 1] = tile0_[i] + (int)((int64_t)tiles_[i] * (j + 1) / cnt);
+        }
+        b += cnt;
+    }
+    sys_blk0[K] = G;
+    map_grid_ = G;
+}
+
+void Batch::upload_maps(hipStream_t s) {
+    TF_CHECK(hipMemcpyAsync(maps_.p, h_maps_, sizeof(int) * (3 * (size_t)gmax_ + k_ + 1), hipMemcpyHostToDevice,
+                            s));
+}
+
+BatchCtx Batch::ctx(int lanes, bool with_kbase) const {
+    BatchCtx c;
+    c.blk_sys = maps_.p;
+    c.blk_tiles = maps_.p + gmax_;
+    c.tile_r0 = tile_r0_.p;
+    c.sys_blk0 = maps_.p + 3 * gmax_;
+    c.kbase = with_kbase ? kbase_.p : nullptr;
+    c.lanes = lanes;
+    return c;
+}
+
+BatchConfig Batch::config_for(int bits, int method, int prec) const {
+    BatchConfig c;
+    auto it = tuned_.find(Key{bits, method, prec});
+    if (it != tuned_.end()) c = it->second;
+    if (fixed_.grid) c.grid = fixed_.grid;
+    if (fixed_.lanes) c.lanes = fixed_.lanes;
+    if (fixed_.chunk) c.chunk = fixed_.chunk;
+    if (!c.chunk) c.chunk = prec == 2 ? 1 : 8;
+    if (c.grid) c.grid = std::min(gmax_, std::max(round8(k_), round8(c.grid)));
+    return c;
+}
+
+template <typename T, typename Op>
+hipGraphExec_t Batch::graph(const GraphKey& gk, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
+                            T* partials, SolverState* st, HostMirror* hm) {
+    auto it = graphs_.find(gk);
+    if (it != graphs_.end()) return it->second;
+    const Levels* L = gk.prec == 2 ? &sys_->levels() : &kNoLevels;
+    const BatchCtx c = ctx(gk.lanes, true);
+    const Sites<T> S = Sites<T>::carve(partials, gk.grid);
+    hipGraph_t g = nullptr;
+    TF_CHECK(hipStreamBeginCapture(bs_, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int i = 1; i <= gk.chunk; ++i)
+            enqueue_iteration<T, Op>(c, gk.method, gk.prec != 2, i, Aop, A, w, S, st, hm, L, bs_);
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(kWave), 0, bs_, kbase_.p, gk.chunk);
+    } catch (...) {
+        (void)hipStreamEndCapture(bs_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    TF_CHECK(hipStreamEndCapture(bs_, &g));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    TF_CHECK(e);
+    graphs_[gk] = exec;
+    return exec;
+}
+
+template <typename T>
+void Batch::rebalance(const std::vector<char>& active, int grid, T* partials, SolverState* st) {
+    const BatchCtx c = ctx(1, false);
+    const Sites<T> S = Sites<T>::carve(partials, grid);
+    hipLaunchKernelGGL((k_site_collapse<T>), dim3(grid), dim3(kBlock), 0, bs_, c, (const T*)S.B, st);
+    TF_CHECK(hipGetLastError());
+    // the staging buffer is rewritten below: let the previous upload finish
+    TF_CHECK(hipStreamSynchronize(bs_));
+    build_maps(active, grid);
+    upload_maps(bs_);
+    hipLaunchKernelGGL((k_site_expand<T>), dim3(grid), dim3(kWave), 0, bs_, c, S.B, (const SolverState*)st);
+    TF_CHECK(hipGetLastError());
+    ++rebalances_;
+}
+
+// Times a few iterations of every (grid, lanes) candidate on the caller's
+// systems; returns the fastest.
+template <typename T, typename Op>
+BatchConfig Batch::tune(int bits, int method, int prec, const Op& Aop, const Csr<T>& A,
+                        const KrylovVectors<T>& w, T* partials, SolverState* st, HostMirror* hm,
+                        const std::vector<SolverState>& init, bool d_ready, bool zero_x,
+                        const std::function<void()>& prepare) {
+    (void)bits;
+    static const int kGrids[] = {1024, 2048, 4096};
+    static const int kLanes[] = {1, 2, 4, 8};
+    const bool jac = prec != 2;
+    const int probe = jac ? 6 : 2;
+    const Levels* L = prec == 2 ? &sys_->levels() : &kNoLevels;
+    const std::vector<char> all(k_, 1);
+    BatchConfig best;
+    float best_ms = 1e30f;
+    for (int rep = 0; rep < 2; ++rep) {  // the first pass also loads the code objects
+        for (int g0 : kGrids) {
+            const int G = std::min(gmax_, std::max(round8(k_), g0));
+            TF_CHECK(hipStreamSynchronize(bs_));
+            build_maps(all, G);
+            upload_maps(bs_);
+            for (int lanes : kLanes) {
+                const BatchCtx c = ctx(lanes, false);
+                const Sites<T> S = Sites<T>::carve(partials, G);
+                prepare();
+                TF_CHECK(hipMemcpyAsync(st, init.data(), sizeof(SolverState) * k_, hipMemcpyHostToDevice, bs_));
+                start_solve<T, Op>(c, method, prec, Aop, A, w, S, st, hm, L, bs_, d_ready, zero_x);
+                TF_CHECK(hipEventRecord(ev_[2], bs_));
+                for (int i = 1; i <= probe; ++i) enqueue_iteration<T, Op>(c, method, jac, i, Aop, A, w, S, st, hm, L, bs_);
+                TF_CHECK(hipEventRecord(ev_[3], bs_));
+                TF_CHECK(hipEventSynchronize(ev_[3]));
+                float ms = 0.0f;
+                TF_CHECK(hipEventElapsedTime(&ms, ev_[2], ev_[3]));
+                if (rep == 1 && ms < best_ms) {
+                    best_ms = ms;
+                    best.grid = G;
+                    best.lanes = lanes;
+                }
+            }
+        }
+    }
+    return best;
+}
+
+template <typename T, typename Op>
+void Batch::run(int bits, int method, int prec, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
+                T* partials, SolverState* st, HostMirror* hm, const std::vector<SolverState>& init, bool d_ready,
+                bool zero_x, const std::function<void()>& prepare, std::vector<SolverState>& out) {
+    const Key key{bits, method, prec};
+    const bool auto_cfg = !fixed_.grid || !fixed_.lanes;
+    if (auto_cfg && tuned_.find(key) == tuned_.end())
+        tuned_[key] = tune<T, Op>(bits, method, prec, Aop, A, w, partials, st, hm, init, d_ready, zero_x, prepare);
+    const BatchConfig cfg = config_for(bits, method, prec);
+    const int G = cfg.grid ? cfg.grid : std::min(gmax_, std::max(round8(k_), 2048));
+    const int lanes = cfg.lanes ? cfg.lanes : 1;
+    const int C = cfg.chunk;
+    const bool jac = prec != 2;
+    const Levels* L = prec == 2 ? &sys_->levels() : &kNoLevels;
+    hipGraphExec_t exec = graph<T, Op>(GraphKey{bits, method, prec, G, lanes, C}, Aop, A, w, partials, st, hm);
+
+    std::vector<char> act(k_, 1);
+    TF_CHECK(hipStreamSynchronize(bs_));
+    build_maps(act, G);
+    upload_maps(bs_);
+    prepare();
+    TF_CHECK(hipMemcpyAsync(st, init.data(), sizeof(SolverState) * k_, hipMemcpyHostToDevice, bs_));
+    for (int i = 0; i < k_; ++i) {
+        hm[i].done = 0;
+        hm[i].iter = 0;
+    }
+    const BatchCtx c0 = ctx(lanes, false);
+    start_solve<T, Op>(c0, method, prec, Aop, A, w, Sites<T>::carve(partials, G), st, hm, L, bs_, d_ready, zero_x);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(kWave), 0, bs_, kbase_.p, 0);
+    TF_CHECK(hipGetLastError());
+    (void)jac;
+
+    int maxcap = 0;
+    for (const auto& h : init) maxcap = std::max(maxcap, h.cap);
+    const long long limit = (long long)maxcap + 1 + 2LL * C;
+    int64_t mapped = tile0_[k_];
+    bool pending[2] = {false, false};
+    int slot = 0;
+    long long issued = 0;
+    bool all_done = false;
+    while (issued < limit) {
+        TF_CHECK(hipGraphLaunch(exec, bs_));
+        issued += C;
+        TF_CHECK(hipEventRecord(ev_[slot], bs_));
+        pending[slot] = true;
+        const int other = slot ^ 1;
+        if (pending[other]) {
+            TF_CHECK(hipEventSynchronize(ev_[other]));
+            pending[other] = false;
+            int64_t running = 0;
+            all_done = true;
+            for (int i = 0; i < k_; ++i) {
+                act[i] = hm[i].done == 0;
+                if (act[i]) {
+                    all_done = false;
+                    running += tiles_[i];
+                }
+            }
+            if (all_done) break;
+            if (rebalance_ && running * 10 <= mapped * 7) {
+                rebalance<T>(act, G, partials, st);
+                mapped = running;
+            }
+        }
+        slot = other;
+    }
+    TF_CHECK(hipStreamSynchronize(bs_));
+    out.resize(k_);
+    TF_CHECK(hipMemcpy(out.data(), st, sizeof(SolverState) * k_, hipMemcpyDeviceToHost));
+    for (int i = 0; i < k_; ++i)
+        if (!out[i].done) throw HipError("batched solve finished without every system reaching a stopping state");
+}
+
+KrylovVectors<double> Batch::vectors64() {
+    const size_t n = (size_t)n_tot_;
+    double* b = ws_.p;
+    KrylovVectors<double> w;
+    w.D = b + V_D * n;
+    w.p = b + V_P * n;
+    w.q = b + V_Q * n;
+    w.r2 = b + V_R2 * n;
+    w.p2 = b + V_P2 * n;
+    w.q2 = b + V_Q2 * n;
+    w.u1 = b + V_U1 * n;
+    w.u2 = b + V_U2 * n;
+    w.v2 = b + V_V2 * n;
+    w.x = b + V_COUNT * n;
+    w.r1 = b + (V_COUNT + 1) * n;
+    return w;
+}
+
+KrylovVectors<float> Batch::vectors32() {
+    const size_t n = (size_t)n_tot_;
+    float* b = ws32_.p;
+    KrylovVectors<float> w;
+    w.D = b + V_D * n;
+    w.p = b + V_P * n;
+    w.q = b + V_Q * n;
+    w.r2 = b + V_R2 * n;
+    w.p2 = b + V_P2 * n;
+    w.q2 = b + V_Q2 * n;
+    w.u1 = b + V_U1 * n;
+    w.u2 = b + V_U2 * n;
+    w.v2 = b + V_V2 * n;
+    w.x = b + V_COUNT * n;
+    w.r1 = b + (V_COUNT + 1) * n;
+    return w;
+}
+
+static void check_method(int method, int prec) {
+    if (method < 0 || method > 2) throw std::invalid_argument("method must be 0 (cg), 1 (bicg) or 2 (cgs)");
+    if (prec < 0 || prec > 2) throw std::invalid_argument("prec must be 0, 1 or 2");
+}
+
+void Batch::solve(int method, int prec, double* x, double* r1, const int* cap, const double* tol,
+                  SolveResult* res, hipStream_t s) {
+    check_method(method, prec);
+    join(s);
+    const KrylovVectors<double> w = vectors64();
+    const size_t bytes = sizeof(double) * (size_t)n_tot_;
+    std::vector<SolverState> init(k_);
+    for (int i = 0; i < k_; ++i) {
+        init[i] = SolverState{};
+        init[i].tol = tol[i];
+        init[i].cap = cap[i];
+        init[i].error = res[i].fin_res;   // returned on the ||b|| < tol exit
+    }
+    auto prepare = [&] {
+        TF_CHECK(hipMemcpyAsync(w.x, x, bytes, hipMemcpyDeviceToDevice, bs_));
+        TF_CHECK(hipMemcpyAsync(w.r1, r1, bytes, hipMemcpyDeviceToDevice, bs_));
+    };
+    const Csr<double> A = sys_->csr();
+    std::vector<SolverState> out;
+    run<double, Csr<double>>(64, method, prec, A, A, w, partials_.p, st_.p, hm_, init, false, false, prepare, out);
+    TF_CHECK(hipMemcpyAsync(x, w.x, bytes, hipMemcpyDeviceToDevice, bs_));
+    TF_CHECK(hipMemcpyAsync(r1, w.r1, bytes, hipMemcpyDeviceToDevice, bs_));
+    TF_CHECK(hipStreamSynchronize(bs_));
+    for (int i = 0; i < k_; ++i) {
+        res[i].niter = out[i].niter_out;
+        res[i].ini_written = out[i].ini_written != 0;
+        if (res[i].ini_written) res[i].ini_res = out[i].ini_res;
+        res[i].fin_res = out[i].error;
+    }
+}
+
+// Per-system ||v||_2 from per-tile partial sums, added on the host in tile
+// order.
+void Batch::tile_norms(const double* v, std::vector<double>& out) {
+    const int T = tile0_[k_];
+    hipLaunchKernelGGL(k_tile_norm2, dim3(std::min(T, kMaxGrid)), dim3(kBlock), 0, bs_, T, (const int*)tile_r0_.p, v,
+                       tnorm_.p);
+    TF_CHECK(hipGetLastError());
+    std::vector<double> h(T);
+    TF_CHECK(hipMemcpyAsync(h.data(), tnorm_.p, sizeof(double) * T, hipMemcpyDeviceToHost, bs_));
+    TF_CHECK(hipStreamSynchronize(bs_));
+    out.assign(k_, 0.0);
+    for (int i = 0; i < k_; ++i) {
+        double a = 0.0;
+        for (int t = tile0_[i]; t < tile0_[i + 1]; ++t) a += h[t];
+        out[i] = std::sqrt(a);
+    }
+}
+
+// Refinement per system: an inner FP32 solve on the unit-scaled residual,
+// x += ||r|| d, a new FP64 residual.  A step that does not lower the FP64
+// residual (an inner solve that diverged: its recurrence residual left the
+// range FP32 resolves, kInnerDivLimit) is undone and ends that system's
+// refinement, which then reports non-convergence the reference way
+// (niter = iterations + 1, FinRes = ||b - A x||).
+void Batch::solve_mixed(int method, int prec, double* x, const double* b, double* r_out, const int* cap,
+                        const double* tol, double inner_rtol, int max_outer, SolveResult* res, int* outer_steps,
+                        hipStream_t s) {
+    check_method(method, prec);
+    if (!(inner_rtol > 0.0 && inner_rtol < 1.0)) throw std::invalid_argument("inner_rtol must be in (0, 1)");
+    join(s);
+    const int n = n_tot_, T = tile0_[k_];
+    const size_t bytes = sizeof(double) * (size_t)n;
+    double* x64 = ws_.p + (size_t)V_COUNT * n;
+    double* r64 = x64 + n;
+    double* b64 = r64 + n;
+    double* xprev = ws_.p + (size_t)V_P * n;   // FP64 slots are free during a mixed solve
+    TF_CHECK(hipMemcpyAsync(x64, x, bytes, hipMemcpyDeviceToDevice, bs_));
+    TF_CHECK(hipMemcpyAsync(b64, b, bytes, hipMemcpyDeviceToDevice, bs_));
+
+    const DiffCsr32 Aop = sys_->diff32(bs_);
+    const Csr<float> A32 = prec == 2 ? sys_->csr32(bs_) : Csr<float>{n, sys_->rowptr(), sys_->col(), sys_->dia(), nullptr};
+    const KrylovVectors<float> w = vectors32();
+    const int nb = (n + kBlock - 1) / kBlock;
+    if (prec == 1) {
+        TF_CHECK(hipMemcpyAsync(w.D, sys_->diag32(bs_), sizeof(float) * n, hipMemcpyDeviceToDevice, bs_));
+    } else if (prec == 2) {
+        double* d64 = ws_.p;   // FP64 scratch (the V_D slot)
+        sys_->dic_setup(d64, bs_);
+        launch_f64_to_f32(n, d64, w.D, bs_);
+    } else {
+        hipLaunchKernelGGL(k_fill32, dim3(nb), dim3(kBlock), 0, bs_, n, 1.0f, w.D);
+    }
+    TF_CHECK(hipGetLastError());
+    const int tgrid = std::min(T, kMaxGrid);
+    auto residual = [&](std::vector<double>& err) {
+        TF_CHECK(hipMemcpyAsync(r64, b64, bytes, hipMemcpyDeviceToDevice, bs_));
+        sys_->resid(x64, r64, bs_);
+        tile_norms(r64, err);
+    };
+    std::vector<double> err, sc(k_);
+    residual(err);
+    std::vector<int> total(k_, 0);
+    std::vector<char> open(k_, 0);   // still refining
+    for (int i = 0; i < k_; ++i) {
+        res[i].ini_res = err[i];
+        res[i].ini_written = true;
+        open[i] = err[i] >= tol[i] && cap[i] > 0;
+    }
+    int outer = 0;
+    std::vector<SolverState> init(k_), out;
+    while (outer < max_outer && std::any_of(open.begin(), open.end(), [](char c) { return c != 0; })) {
+        ++outer;
+        for (int i = 0; i < k_; ++i) {
+            sc[i] = open[i] ? 1.0 / err[i] : 0.0;   // closed systems: r32 = 0, an immediate exit
+            init[i] = SolverState{};
+            init[i].tol = inner_rtol;               // ||r32|| = 1 up to rounding
+            init[i].cap = open[i] ? cap[i] - total[i] : 0;
+            init[i].div_limit = kInnerDivLimit;
+        }
+        TF_CHECK(hipMemcpyAsync(scale_.p, sc.data(), sizeof(double) * k_, hipMemcpyHostToDevice, bs_));
+        auto prepare = [&] {
+            hipLaunchKernelGGL(k_tile_scale, dim3(tgrid), dim3(kBlock), 0, bs_, T, (const int*)tile_r0_.p,
+                               (const int*)tile_sys_.p, (const double*)r64, (const double*)scale_.p, w.r1, w.x);
+        };
+        run<float, DiffCsr32>(32, method, prec, Aop, A32, w, partials32_.p, st32_.p, hm32_, init, true, true, prepare,
+                              out);
+        std::vector<int> its(k_, 0);
+        for (int i = 0; i < k_; ++i) {
+            sc[i] = 0.0;
+            if (!open[i]) continue;
+            its[i] = out[i].niter_out > init[i].cap ? std::max(init[i].cap, 0) : out[i].niter_out;
+            total[i] += its[i];
+            if (!out[i].diverged) sc[i] = err[i];
+        }
+        TF_CHECK(hipMemcpyAsync(xprev, x64, bytes, hipMemcpyDeviceToDevice, bs_));
+        TF_CHECK(hipMemcpyAsync(scale_.p, sc.data(), sizeof(double) * k_, hipMemcpyHostToDevice, bs_));
+        hipLaunchKernelGGL(k_tile_refine, dim3(tgrid), dim3(kBlock), 0, bs_, T, (const int*)tile_r0_.p,
+                           (const int*)tile_sys_.p, x64, (const double*)scale_.p, (const float*)w.x,
+                           (const double*)b64, r64);
+        TF_CHECK(hipGetLastError());
+        sys_->resid(x64, r64, bs_);
+        std::vector<double> err_new;
+        tile_norms(r64, err_new);
+        std::vector<double> reject(k_, 0.0);
+        bool any_reject = false;
+        for (int i = 0; i < k_; ++i)
+            if (open[i] && !(err_new[i] < err[i])) {
+                reject[i] = 1.0;
+                any_reject = true;
+            }
+        if (any_reject) {   // undo those steps and recompute their residuals
+            TF_CHECK(hipMemcpyAsync(scale_.p, reject.data(), sizeof(double) * k_, hipMemcpyHostToDevice, bs_));
+            hipLaunchKernelGGL(k_tile_restore, dim3(tgrid), dim3(kBlock), 0, bs_, T, (const int*)tile_r0_.p,
+                               (const int*)tile_sys_.p, x64, (const double*)xprev, (const double*)scale_.p);
+            TF_CHECK(hipGetLastError());
+            TF_CHECK(hipMemcpyAsync(r64, b64, bytes, hipMemcpyDeviceToDevice, bs_));
+            sys_->resid(x64, r64, bs_);
+            tile_norms(r64, err_new);
+        }
+        err = err_new;
+        for (int i = 0; i < k_; ++i) {
+            if (!open[i]) continue;
+            if (err[i] < tol[i] || its[i] == 0 || total[i] >= cap[i] || reject[i] != 0.0) open[i] = 0;
+        }
+    }
+    TF_CHECK(hipMemcpyAsync(x, x64, bytes, hipMemcpyDeviceToDevice, bs_));
+    if (r_out) TF_CHECK(hipMemcpyAsync(r_out, r64, bytes, hipMemcpyDeviceToDevice, bs_));
+    TF_CHECK(hipStreamSynchronize(bs_));
+    for (int i = 0; i < k_; ++i) {
+        res[i].niter = err[i] < tol[i] ? total[i] : total[i] + 1;
+        res[i].fin_res = err[i];
+    }
+    if (outer_steps) *outer_steps = outer;
+}
+
+}  // namespace tflows

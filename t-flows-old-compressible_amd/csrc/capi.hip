@@ -8,12 +8,16 @@
 #include <string>
 
 #include "../../include/tflows_krylov.h"
+#include "batched.h"
 #include "system.h"
 
 using tflows::System;
 
 struct tflows_system_s {
     std::unique_ptr<System> sys;
+};
+struct tflows_batch_s {
+    std::unique_ptr<tflows::Batch> batch;
 };
 
 namespace {
@@ -248,6 +252,101 @@ int32_t tflows_system_levels(tflows_system s) {
     int32_t v = 0;
     if (guarded([&] { v = s->sys->levels().nfwd; })) return -1;
     return v;
+}
+
+int tflows_system_set_lanes(tflows_system s, int32_t lanes) {
+    return guarded([&] { s->sys->set_lanes(lanes); });
+}
+
+// ---- batches of independent systems ---------------------------------------
+int tflows_batch_create(int32_t k, const int32_t* n, const int32_t* nonz, const int32_t* const* acol,
+                        const int32_t* const* arow, const int32_t* const* adia, tflows_batch* out) {
+    return guarded([&] {
+        if (!out || !n || !nonz || !acol || !arow || !adia) throw std::invalid_argument("NULL argument");
+        auto h = std::make_unique<tflows_batch_s>();
+        h->batch.reset(new tflows::Batch(k, n, nonz, acol, arow, adia));
+        *out = h.release();
+    });
+}
+
+int tflows_batch_destroy(tflows_batch b) {
+    return guarded([&] { delete b; });
+}
+
+int tflows_batch_set_values(tflows_batch b, int32_t sys, const double* aval, int32_t mem, void* stream) {
+    return guarded([&] { b->batch->set_values(sys, aval, mem == TFLOWS_MEM_DEVICE, as_stream(stream)); });
+}
+
+int tflows_batch_layout(tflows_batch b, int32_t* rows, int64_t* entries, int32_t* row_offsets) {
+    return guarded([&] {
+        if (rows) *rows = b->batch->rows();
+        if (entries) *entries = b->batch->entries();
+        if (row_offsets) {
+            const auto& r = b->batch->row_offsets();
+            std::copy(r.begin(), r.end(), row_offsets);
+        }
+    });
+}
+
+int tflows_batch_configure(tflows_batch b, int32_t grid, int32_t lanes, int32_t chunk, int32_t rebalance) {
+    return guarded([&] {
+        if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8)
+            throw std::invalid_argument("lanes per row must be 0 (auto), 1, 2, 4 or 8");
+        if (grid < 0 || chunk < 0) throw std::invalid_argument("grid and chunk must be >= 0");
+        tflows::BatchConfig c;
+        c.grid = grid;
+        c.lanes = lanes;
+        c.chunk = chunk;
+        b->batch->configure(c);
+        b->batch->set_rebalance(rebalance != 0);
+    });
+}
+
+int tflows_batch_config(tflows_batch b, int32_t bits, int32_t method, int32_t prec, int32_t* grid, int32_t* lanes,
+                        int32_t* chunk, int32_t* rebalances) {
+    return guarded([&] {
+        const tflows::BatchConfig c = b->batch->config_for(bits, method, prec);
+        if (grid) *grid = c.grid;
+        if (lanes) *lanes = c.lanes;
+        if (chunk) *chunk = c.chunk;
+        if (rebalances) *rebalances = b->batch->rebalances();
+    });
+}
+
+int tflows_batch_solve(tflows_batch b, int32_t method, int32_t prec, double* x, double* r1, const int32_t* cap,
+                       const double* tol, int32_t* niter, double* ini_res, double* fin_res, void* stream) {
+    return guarded([&] {
+        const int k = b->batch->count();
+        if (!cap || !tol || !niter || !fin_res) throw std::invalid_argument("NULL argument");
+        std::vector<tflows::SolveResult> res(k);
+        for (int i = 0; i < k; ++i) res[i].fin_res = fin_res[i];
+        b->batch->solve(method, prec, x, r1, cap, tol, res.data(), as_stream(stream));
+        for (int i = 0; i < k; ++i) {
+            niter[i] = res[i].niter;
+            if (res[i].ini_written && ini_res) ini_res[i] = res[i].ini_res;
+            fin_res[i] = res[i].fin_res;
+        }
+    });
+}
+
+int tflows_batch_solve_mixed(tflows_batch b, int32_t method, int32_t prec, double* x, const double* rhs,
+                             double* r_out, const int32_t* cap, const double* tol, double inner_rtol,
+                             int32_t max_outer, int32_t* niter, double* ini_res, double* fin_res,
+                             int32_t* outer_steps, void* stream) {
+    return guarded([&] {
+        const int k = b->batch->count();
+        if (!cap || !tol || !niter || !fin_res) throw std::invalid_argument("NULL argument");
+        std::vector<tflows::SolveResult> res(k);
+        int outer = 0;
+        b->batch->solve_mixed(method, prec, x, rhs, r_out, cap, tol, inner_rtol, max_outer, res.data(), &outer,
+                              as_stream(stream));
+        for (int i = 0; i < k; ++i) {
+            niter[i] = res[i].niter;
+            if (ini_res) ini_res[i] = res[i].ini_res;
+            fin_res[i] = res[i].fin_res;
+        }
+        if (outer_steps) *outer_steps = outer;
+    });
 }
 
 }  // extern "C"

@@ -35,6 +35,11 @@ struct SolverState {
     int suspend;
     int replaced_before;
     double err_rep;                 // FP64 residual norm at the last replacement
+    // divergence guard of the FP32 inner solves of the mixed-precision
+    // path (0 = off, the reference behaviour): the solve stops, marked
+    // `diverged`, once ||r|| exceeds div_limit or is not finite.
+    double div_limit;
+    int diverged;
 };
 
 // Host-visible mirror of the progress, written by the device with
@@ -58,18 +63,24 @@ __device__ __forceinline__ void publish(HostMirror* hm, int done, int iter) {
 // is the launch argument.  For a batch of K systems stored as one
 // block-diagonal matrix, tiles never straddle two systems and:
 //   blk_sys[b]      system of block b
-//   blk_t0[b..b+1]  tiles of block b
+//   blk_tiles[2b], blk_tiles[2b+1]  tiles [t0, t1) of block b
 //   tile_r0[t..t+1] rows of tile t (tiles are contiguous in row order)
 //   sys_blk0[k..k+1] blocks of system k (its reduction range; the first is
 //                    its leader)
 // kbase (optional) offsets the launch's iteration index by a device value,
 // so one captured graph of a chunk of iterations can be replayed.
+// `lanes` selects the SpMV form of the launch: 1 = one row per thread with
+// the tile's products staged in LDS (64 rows per wavefront), 2/4/8 = that
+// many lanes per row streaming col/val directly (32/16/8 rows per
+// wavefront).  Every form sums a row in column order, so all give the same
+// bits.
 struct BatchCtx {
     const int* blk_sys = nullptr;
-    const int* blk_t0 = nullptr;
+    const int* blk_tiles = nullptr;
     const int* tile_r0 = nullptr;
     const int* sys_blk0 = nullptr;
     const int* kbase = nullptr;
+    int lanes = 1;
 
     __device__ __forceinline__ int sys() const { return blk_sys ? blk_sys[blockIdx.x] : 0; }
     __device__ __forceinline__ bool leader() const {
@@ -253,8 +264,8 @@ __device__ __forceinline__ void ctx_tiles(const BatchCtx& c, int n, int& t0, int
     if (!c.blk_sys) {
         block_tiles((n + kTileRows - 1) / kTileRows, t0, t1);
     } else {
-        t0 = c.blk_t0[blockIdx.x];
-        t1 = c.blk_t0[blockIdx.x + 1];
+        t0 = c.blk_tiles[2 * blockIdx.x];
+        t1 = c.blk_tiles[2 * blockIdx.x + 1];
     }
 }
 __device__ __forceinline__ void ctx_tile_rows(const BatchCtx& c, int n, int t, int& r0, int& r1) {
@@ -287,15 +298,138 @@ __device__ __forceinline__ void ctx_total(const BatchCtx& c, const T* __restrict
     }
 }
 
-// q = A p over one tile for either operator (no residual form).
-template <int NRHS, typename T, class Gather, class Epi>
+// ---------------------------------------------------------------------------
+// Lane-group forms: L lanes (L = 2, 4, 8; aligned groups inside a 16-lane
+// DPP row) share one row.  Lane j of the group loads entries e0+j, e0+j+L,
+// ... of the row, so a wavefront's col/val loads cover 64/L consecutive rows
+// contiguously with no LDS staging and no barrier.  The products of each
+// chunk of L entries are folded into the row sum left to right: after the
+// DPP row_shr:1 step i, lane i holds acc + p_0 + ... + p_i, so the row is
+// summed in column order exactly as the thread-per-row form does.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T shr1(T v);
+template <>
+__device__ __forceinline__ double shr1<double>(double v) { return dpp_d<0x111>(v); }
+template <>
+__device__ __forceinline__ float shr1<float>(float v) { return dpp_f<0x111>(v); }
+
+template <typename T>
+__device__ __forceinline__ T group_last(T v, int L) {   // value of the group's last lane
+    const int src = (int)(threadIdx.x & (kWave - 1) & ~(L - 1)) + L - 1;
+    return __shfl(v, src, kWave);
+}
+
+template <int L, typename T, int NRHS, bool SUBTRACT, class Gather, class Init, class Epi>
+__device__ __forceinline__ void group_tile(const Csr<T>& A, int r0, int r1, Gather& g, Init& init, Epi& epi) {
+    constexpr int kRowsPerPass = kBlock / L;
+    const int j = threadIdx.x & (L - 1);
+    for (int rb = r0; rb < r1; rb += kRowsPerPass) {
+        const int row = rb + (int)threadIdx.x / L;
+        if (row >= r1) continue;                  // the whole group leaves together
+        const int e0 = A.rowptr[row], e1 = A.rowptr[row + 1];
+        T acc[NRHS];
+#pragma unroll
+        for (int v = 0; v < NRHS; ++v) acc[v] = SUBTRACT ? init(row, v) : T(0);
+        for (int cs = e0; cs < e1; cs += L) {
+            const int e = cs + j;
+            const bool ok = e < e1;
+            T prod[NRHS];
+            if (ok) {
+                T gv[NRHS];
+                const T a = A.val[e];
+                g(A.col[e], gv);
+#pragma unroll
+                for (int v = 0; v < NRHS; ++v) prod[v] = a * gv[v];
+            } else {
+#pragma unroll
+                for (int v = 0; v < NRHS; ++v) prod[v] = T(0);
+            }
+            T s[NRHS];
+#pragma unroll
+            for (int v = 0; v < NRHS; ++v) s[v] = ok ? (SUBTRACT ? acc[v] - prod[v] : acc[v] + prod[v]) : acc[v];
+#pragma unroll
+            for (int i = 1; i < L; ++i) {
+#pragma unroll
+                for (int v = 0; v < NRHS; ++v) {
+                    const T t = shr1(s[v]);
+                    if (j == i) s[v] = ok ? (SUBTRACT ? t - prod[v] : t + prod[v]) : t;
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < NRHS; ++v) acc[v] = group_last(s[v], L);
+        }
+        if (j == 0) epi(row, acc);
+    }
+}
+
+template <int L, int NRHS, class Gather, class Epi>
+__device__ __forceinline__ void group_tile_diff(const DiffCsr32& A, int r0, int r1, Gather& g, Epi& epi) {
+    constexpr int kRowsPerPass = kBlock / L;
+    const int j = threadIdx.x & (L - 1);
+    for (int rb = r0; rb < r1; rb += kRowsPerPass) {
+        const int row = rb + (int)threadIdx.x / L;
+        if (row >= r1) continue;
+        const int e0 = A.rowptr[row], e1 = A.rowptr[row + 1];
+        float own[NRHS], acc[NRHS];
+        g(row, own);
+#pragma unroll
+        for (int v = 0; v < NRHS; ++v) acc[v] = 0.0f;
+        for (int cs = e0; cs < e1; cs += L) {
+            const int e = cs + j;
+            const bool ok = e < e1;
+            float term[NRHS];
+            if (ok) {
+                float gv[NRHS];
+                const float a = A.val[e];
+                g(A.col[e], gv);
+#pragma unroll
+                for (int v = 0; v < NRHS; ++v) term[v] = a * (gv[v] - own[v]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < NRHS; ++v) term[v] = 0.0f;
+            }
+            float s[NRHS];
+#pragma unroll
+            for (int v = 0; v < NRHS; ++v) s[v] = ok ? acc[v] + term[v] : acc[v];
+#pragma unroll
+            for (int i = 1; i < L; ++i) {
+#pragma unroll
+                for (int v = 0; v < NRHS; ++v) {
+                    const float t = shr1(s[v]);
+                    if (j == i) s[v] = ok ? t + term[v] : t;
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < NRHS; ++v) acc[v] = group_last(s[v], L);
+        }
+        if (j == 0) {
+            const float rsi = A.rs[row];
+#pragma unroll
+            for (int v = 0; v < NRHS; ++v) acc[v] = acc[v] + rsi * own[v];
+            epi(row, acc);
+        }
+    }
+}
+
+// q = A p over one tile for either operator; L = lanes per row (1 = the
+// LDS-staged thread-per-row form).
+template <int NRHS, int L = 1, typename T, class Gather, class Epi>
 __device__ __forceinline__ void op_tile(const Csr<T>& A, int r0, int r1, Gather& g, Epi& epi) {
     InitZero<T> z;
-    stream_tile<T, NRHS, false>(A, r0, r1, g, z, epi);
+    if constexpr (L == 1) stream_tile<T, NRHS, false>(A, r0, r1, g, z, epi);
+    else group_tile<L, T, NRHS, false>(A, r0, r1, g, z, epi);
 }
-template <int NRHS, class Gather, class Epi>
+template <int NRHS, int L = 1, class Gather, class Epi>
 __device__ __forceinline__ void op_tile(const DiffCsr32& A, int r0, int r1, Gather& g, Epi& epi) {
-    stream_tile_diff<NRHS>(A, r0, r1, g, epi);
+    if constexpr (L == 1) stream_tile_diff<NRHS>(A, r0, r1, g, epi);
+    else group_tile_diff<L, NRHS>(A, r0, r1, g, epi);
+}
+// r = b - A x over one tile (the residual form: subtraction from b).
+template <int L, typename T, class Gather, class Init, class Epi>
+__device__ __forceinline__ void resid_tile(const Csr<T>& A, int r0, int r1, Gather& g, Init& init, Epi& epi) {
+    if constexpr (L == 1) stream_tile<T, 1, true>(A, r0, r1, g, init, epi);
+    else group_tile<L, T, 1, true>(A, r0, r1, g, init, epi);
 }
 
 }  // namespace tflows

@@ -74,10 +74,11 @@ void launch_cgs_q(const BatchCtx& c, bool jac, int k, int n, const T* u1, const 
 template <typename Op, typename T>
 void launch_cgs_spmv_r(const BatchCtx& c, int k, const Op& A, const T* p1, T* x, T* r1, const T* r2,
                        const Sites<T>& S, const SolverState* st, hipStream_t s);
+// stand-alone q = A p and r -= A x; `lanes` as in BatchCtx
 template <typename Op, typename T>
-void launch_spmv(const Op& A, const T* x, T* y, int grid, hipStream_t s);
+void launch_spmv(const Op& A, const T* x, T* y, int grid, hipStream_t s, int lanes = 1);
 template <typename T>
-void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s);
+void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s, int lanes = 1);
 // fused two-kernel CG (cg_fused.hip)
 template <typename Op, typename T>
 void launch_cgf_spmv(bool mixed, const Op& A, const T* z, T* p0, T* p1, T* q, const Sites<T>& S,

@@ -28,6 +28,8 @@
 //   BiCG   Process/BiCG.f90:50-254          -> k_bicg_p, k_bicg_spmv, k_bicg_update
 //   CGS    Process/CGS.f90:62-337           -> k_cgs_u, k_cgs_spmv_v, k_cgs_q, k_cgs_spmv_r
 //   DIC    Process/CG.f90:135-156           -> k_dic_fwd_level, k_dic_bwd_level
+#include <type_traits>
+
 #include "krylov_kernels.h"
 
 namespace tflows {
@@ -68,6 +70,15 @@ __device__ __forceinline__ bool check_stop(const BatchCtx& ctx, int k, const T* 
     if (err < tol) {
         if (lead) {
             st->done = 1;
+            st->niter_out = k - 1;
+            publish(hm, 1, k - 1);
+        }
+        return true;
+    }
+    if (st->div_limit > 0.0 && !(err <= st->div_limit)) {
+        if (lead) {
+            st->done = 1;
+            st->diverged = 1;
             st->niter_out = k - 1;
             publish(hm, 1, k - 1);
         }
@@ -191,7 +202,7 @@ struct ResidEpi {
     }
 };
 
-template <typename T, int MODE, bool ZERO_X>
+template <typename T, int MODE, bool ZERO_X, int L>
 __global__ __launch_bounds__(kBlock) void k_resid_init(BatchCtx ctx, Csr<T> A, const T* __restrict__ x,
                                                        T* __restrict__ r, T* __restrict__ r2,
                                                        const T* __restrict__ D, const T* __restrict__ partA,
@@ -228,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_resid_init(BatchCtx ctx, Csr<T> A, c
         for (int t = t0; t < t1; ++t) {
             int r0, r1;
             ctx_tile_rows(ctx, A.n, t, r0, r1);
-            stream_tile<T, 1, true>(A, r0, r1, g, init, epi);
+            resid_tile<L>(A, r0, r1, g, init, epi);
         }
     }
     site_write<T, 2>(epi.acc, partB);
@@ -328,7 +339,7 @@ struct DotEpi {
     }
 };
 
-template <typename Op, typename T = typename OpValue<Op>::type>
+template <typename Op, int L, typename T = typename OpValue<Op>::type>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(BatchCtx ctx, Op A, const T* __restrict__ p, T* __restrict__ q,
                                                     T* __restrict__ partD, const SolverState* st_base) {
     const SolverState* st = st_base + ctx.sys();
@@ -340,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(BatchCtx ctx, Op A, const T*
     for (int t = t0; t < t1; ++t) {
         int r0, r1;
         ctx_tile_rows(ctx, A.n, t, r0, r1);
-        op_tile<1>(A, r0, r1, g, epi);
+        op_tile<1, L>(A, r0, r1, g, epi);
     }
     site_write<T, 1>(epi.acc, partD);
 }
@@ -413,7 +424,7 @@ struct BiDotEpi {
     }
 };
 
-template <typename Op, typename T = typename OpValue<Op>::type>
+template <typename Op, int L, typename T = typename OpValue<Op>::type>
 __global__ __launch_bounds__(kBlock) void k_bicg_spmv(BatchCtx ctx, Op A, const T* __restrict__ p1,
                                                       const T* __restrict__ p2, T* __restrict__ q1,
                                                       T* __restrict__ q2, T* __restrict__ partD,
@@ -427,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_spmv(BatchCtx ctx, Op A, const 
     for (int t = t0; t < t1; ++t) {
         int r0, r1;
         ctx_tile_rows(ctx, A.n, t, r0, r1);
-        op_tile<2>(A, r0, r1, g, epi);
+        op_tile<2, L>(A, r0, r1, g, epi);
     }
     site_write<T, 1>(epi.acc, partD);
 }
@@ -499,7 +510,7 @@ __global__ __launch_bounds__(kBlock) void k_cgs_u(BatchCtx ctx, int k_arg, int n
 }
 
 // v2 = A p2 (:193-198) with (r2, v2) -> site D (:215-219)
-template <typename Op, typename T = typename OpValue<Op>::type>
+template <typename Op, int L, typename T = typename OpValue<Op>::type>
 __global__ __launch_bounds__(kBlock) void k_cgs_spmv_v(BatchCtx ctx, Op A, const T* __restrict__ p2,
                                                        T* __restrict__ v2, const T* __restrict__ r2,
                                                        T* __restrict__ partD, const SolverState* st_base) {
@@ -512,7 +523,7 @@ __global__ __launch_bounds__(kBlock) void k_cgs_spmv_v(BatchCtx ctx, Op A, const
     for (int t = t0; t < t1; ++t) {
         int r0, r1;
         ctx_tile_rows(ctx, A.n, t, r0, r1);
-        op_tile<1>(A, r0, r1, g, epi);
+        op_tile<1, L>(A, r0, r1, g, epi);
     }
     site_write<T, 1>(epi.acc, partD);
 }
@@ -560,7 +571,7 @@ struct CgsREpi {
     }
 };
 
-template <typename Op, typename T = typename OpValue<Op>::type>
+template <typename Op, int L, typename T = typename OpValue<Op>::type>
 __global__ __launch_bounds__(kBlock) void k_cgs_spmv_r(BatchCtx ctx, int k_arg, Op A, const T* __restrict__ p1,
                                                        T* __restrict__ x, T* __restrict__ r1,
                                                        const T* __restrict__ r2, const T* __restrict__ partD,
@@ -578,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_cgs_spmv_r(BatchCtx ctx, int k_arg, 
     for (int t = t0; t < t1; ++t) {
         int r0, r1_;
         ctx_tile_rows(ctx, A.n, t, r0, r1_);
-        op_tile<1>(A, r0, r1_, g, epi);
+        op_tile<1, L>(A, r0, r1_, g, epi);
     }
     site_write<T, 2>(epi.acc, partB);
 }
@@ -593,17 +604,17 @@ struct StoreEpi {
     __device__ void operator()(int row, T (&v)[1]) { y[row] = v[0]; }
 };
 
-template <typename Op, typename T = typename OpValue<Op>::type>
+template <typename Op, int L, typename T = typename OpValue<Op>::type>
 __global__ __launch_bounds__(kBlock) void k_spmv(Op A, const T* __restrict__ x, T* __restrict__ y) {
     const int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int t0, t1;
     block_tiles(ntiles, t0, t1);
     Gather1<T> g{x};
     StoreEpi<T> epi{y};
-    for (int t = t0; t < t1; ++t) op_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+    for (int t = t0; t < t1; ++t) op_tile<1, L>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
 }
 
-template <typename T>
+template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_resid(Csr<T> A, const T* __restrict__ x, T* __restrict__ r) {
     const int ntiles = (A.n + kTileRows - 1) / kTileRows;
     int t0, t1;
@@ -612,7 +623,7 @@ __global__ __launch_bounds__(kBlock) void k_resid(Csr<T> A, const T* __restrict_
     InitVec<T> init{r};
     StoreEpi<T> epi{r};
     for (int t = t0; t < t1; ++t)
-        stream_tile<T, 1, true>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi);
+        resid_tile<L>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi);
 }
 
 template <typename TO, typename TI>
@@ -625,6 +636,18 @@ __global__ void k_convert(int64_t count, const TI* __restrict__ in, TO* __restri
 // host-side launchers (explicitly instantiated below)
 // ---------------------------------------------------------------------------
 #define TF_LAUNCH(k, grid, stream) hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, stream
+
+// Calls f(std::integral_constant<int, L>) for the SpMV form `lanes`.
+template <class F>
+static void with_lanes(int lanes, F&& f) {
+    switch (lanes) {
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        default: f(std::integral_constant<int, 1>{}); break;
+    }
+}
+#define TF_L constexpr int L = decltype(lc_)::value
 
 template <typename T>
 void launch_setup(const BatchCtx& c, const Csr<T>& A, int prec, bool write_d, T* D, const T* b, const Sites<T>& S,
@@ -645,21 +668,27 @@ void launch_dic_bwd_level(const Csr<T>& A, const int* rows, int count, const T* 
                           const SolverState* st, hipStream_t s) {
     if (count > 0) TF_LAUNCH((k_dic_bwd_level<T>), (count + kBlock - 1) / kBlock, s), A, rows, count, D, z, st);
 }
-template <typename T, bool Z>
+template <typename T, bool Z, int L>
 static void resid_init_mode(int mode, const BatchCtx& c, const Csr<T>& A, const T* x, T* r, T* r2, const T* D,
                             const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
     switch (mode) {
-        case RHO_JAC: TF_LAUNCH((k_resid_init<T, RHO_JAC, Z>), S.grid, s), c, A, x, r, r2, D, S.A, S.B, st, hm); break;
-        case RHO_BICG: TF_LAUNCH((k_resid_init<T, RHO_BICG, Z>), S.grid, s), c, A, x, r, r2, D, S.A, S.B, st, hm); break;
-        case RHO_RR: TF_LAUNCH((k_resid_init<T, RHO_RR, Z>), S.grid, s), c, A, x, r, r2, D, S.A, S.B, st, hm); break;
-        default: TF_LAUNCH((k_resid_init<T, RHO_NONE, Z>), S.grid, s), c, A, x, r, r2, D, S.A, S.B, st, hm); break;
+        case RHO_JAC: TF_LAUNCH((k_resid_init<T, RHO_JAC, Z, L>), S.grid, s), c, A, x, r, r2, D, S.A, S.B, st, hm); break;
+        case RHO_BICG: TF_LAUNCH((k_resid_init<T, RHO_BICG, Z, L>), S.grid, s), c, A, x, r, r2, D, S.A, S.B, st, hm); break;
+        case RHO_RR: TF_LAUNCH((k_resid_init<T, RHO_RR, Z, L>), S.grid, s), c, A, x, r, r2, D, S.A, S.B, st, hm); break;
+        default: TF_LAUNCH((k_resid_init<T, RHO_NONE, Z, L>), S.grid, s), c, A, x, r, r2, D, S.A, S.B, st, hm); break;
     }
 }
 template <typename T>
 void launch_resid_init(int mode, bool zero_x, const BatchCtx& c, const Csr<T>& A, const T* x, T* r, T* r2,
                        const T* D, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
-    if (zero_x) resid_init_mode<T, true>(mode, c, A, x, r, r2, D, S, st, hm, s);
-    else resid_init_mode<T, false>(mode, c, A, x, r, r2, D, S, st, hm, s);
+    if (zero_x) {  // no matrix pass: one form suffices
+        resid_init_mode<T, true, 1>(mode, c, A, x, r, r2, D, S, st, hm, s);
+        return;
+    }
+    with_lanes(c.lanes, [&](auto lc_) {
+        TF_L;
+        resid_init_mode<T, false, L>(mode, c, A, x, r, r2, D, S, st, hm, s);
+    });
 }
 template <typename T>
 void launch_check(const BatchCtx& c, int k, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
@@ -679,7 +708,10 @@ void launch_cg_p(const BatchCtx& c, bool jac, int k, int n, const T* r, const T*
 template <typename Op, typename T>
 void launch_cg_spmv(const BatchCtx& c, const Op& A, const T* p, T* q, const Sites<T>& S, const SolverState* st,
                     hipStream_t s) {
-    TF_LAUNCH((k_cg_spmv<Op>), S.grid, s), c, A, p, q, S.D, st);
+    with_lanes(c.lanes, [&](auto lc_) {
+        TF_L;
+        TF_LAUNCH((k_cg_spmv<Op, L>), S.grid, s), c, A, p, q, S.D, st);
+    });
 }
 template <typename T>
 void launch_cg_update(const BatchCtx& c, bool jac, int k, int n, T* x, T* r, const T* p, const T* q, const T* D,
@@ -696,7 +728,10 @@ void launch_bicg_p(const BatchCtx& c, bool jac, int k, int n, const T* r1, const
 template <typename Op, typename T>
 void launch_bicg_spmv(const BatchCtx& c, const Op& A, const T* p1, const T* p2, T* q1, T* q2, const Sites<T>& S,
                       const SolverState* st, hipStream_t s) {
-    TF_LAUNCH((k_bicg_spmv<Op>), S.grid, s), c, A, p1, p2, q1, q2, S.D, st);
+    with_lanes(c.lanes, [&](auto lc_) {
+        TF_L;
+        TF_LAUNCH((k_bicg_spmv<Op, L>), S.grid, s), c, A, p1, p2, q1, q2, S.D, st);
+    });
 }
 template <typename T>
 void launch_bicg_update(const BatchCtx& c, bool jac, int k, int n, T* x, T* r1, T* r2, const T* p1, const T* q1,
@@ -713,7 +748,10 @@ void launch_cgs_u(const BatchCtx& c, bool jac, int k, int n, const T* r1, const 
 template <typename Op, typename T>
 void launch_cgs_spmv_v(const BatchCtx& c, const Op& A, const T* p2, T* v2, const T* r2, const Sites<T>& S,
                        const SolverState* st, hipStream_t s) {
-    TF_LAUNCH((k_cgs_spmv_v<Op>), S.grid, s), c, A, p2, v2, r2, S.D, st);
+    with_lanes(c.lanes, [&](auto lc_) {
+        TF_L;
+        TF_LAUNCH((k_cgs_spmv_v<Op, L>), S.grid, s), c, A, p2, v2, r2, S.D, st);
+    });
 }
 template <typename T>
 void launch_cgs_q(const BatchCtx& c, bool jac, int k, int n, const T* u1, const T* v2, T* q1, const T* D, T* p1,
@@ -724,15 +762,24 @@ void launch_cgs_q(const BatchCtx& c, bool jac, int k, int n, const T* u1, const 
 template <typename Op, typename T>
 void launch_cgs_spmv_r(const BatchCtx& c, int k, const Op& A, const T* p1, T* x, T* r1, const T* r2,
                        const Sites<T>& S, const SolverState* st, hipStream_t s) {
-    TF_LAUNCH((k_cgs_spmv_r<Op>), S.grid, s), c, k, A, p1, x, r1, r2, S.D, S.B, st);
+    with_lanes(c.lanes, [&](auto lc_) {
+        TF_L;
+        TF_LAUNCH((k_cgs_spmv_r<Op, L>), S.grid, s), c, k, A, p1, x, r1, r2, S.D, S.B, st);
+    });
 }
 template <typename Op, typename T>
-void launch_spmv(const Op& A, const T* x, T* y, int grid, hipStream_t s) {
-    TF_LAUNCH((k_spmv<Op>), grid, s), A, x, y);
+void launch_spmv(const Op& A, const T* x, T* y, int grid, hipStream_t s, int lanes) {
+    with_lanes(lanes, [&](auto lc_) {
+        TF_L;
+        TF_LAUNCH((k_spmv<Op, L>), grid, s), A, x, y);
+    });
 }
 template <typename T>
-void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s) {
-    TF_LAUNCH((k_resid<T>), grid, s), A, x, r);
+void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s, int lanes) {
+    with_lanes(lanes, [&](auto lc_) {
+        TF_L;
+        TF_LAUNCH((k_resid<T, L>), grid, s), A, x, r);
+    });
 }
 void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t s) {
     if (count > 0)
@@ -766,7 +813,7 @@ void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t 
                                   const Sites<T>&, SolverState*, HostMirror*, hipStream_t);                      \
     template void launch_cgs_q<T>(const BatchCtx&, bool, int, int, const T*, const T*, T*, const T*, T*,        \
                                   const Sites<T>&, const SolverState*, hipStream_t);                             \
-    template void launch_resid<T>(const Csr<T>&, const T*, T*, int, hipStream_t);
+    template void launch_resid<T>(const Csr<T>&, const T*, T*, int, hipStream_t, int);
 
 #define TF_INSTANTIATE_OP(Op, T)                                                                                 \
     template void launch_cg_spmv<Op, T>(const BatchCtx&, const Op&, const T*, T*, const Sites<T>&,              \
@@ -777,7 +824,7 @@ void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t 
                                            const SolverState*, hipStream_t);                                     \
     template void launch_cgs_spmv_r<Op, T>(const BatchCtx&, int, const Op&, const T*, T*, T*, const T*,         \
                                            const Sites<T>&, const SolverState*, hipStream_t);                    \
-    template void launch_spmv<Op, T>(const Op&, const T*, T*, int, hipStream_t);
+    template void launch_spmv<Op, T>(const Op&, const T*, T*, int, hipStream_t, int);
 
 TF_INSTANTIATE(double)
 TF_INSTANTIATE(float)

@@ -160,6 +160,11 @@ __global__ __launch_bounds__(kBlock) void k_rel_resid(Csr<double> A, const doubl
 }
 
 // Sum of the block partials on the host, in block order.
+// divergence guard of the FP32 inner solves (their residual starts at norm
+// 1): beyond 1e6 the rounding of FP32 recurrences (~1e-7 relative) exceeds
+// the unit right-hand side and the correction is worthless
+constexpr double kInnerDivLimit = 1e6;
+
 static double host_site_sum(const double* d_partials, int grid, hipStream_t s) {
     std::vector<double> h(grid);
     TF_CHECK(hipMemcpyAsync(h.data(), d_partials, sizeof(double) * grid, hipMemcpyDeviceToHost, s));
@@ -193,7 +198,7 @@ const float* System::diag32(hipStream_t s) {
 double System::true_residual(const double* x, const double* b, double* r, hipStream_t s) {
     const int nb = (n_ + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_copy, dim3(nb), dim3(kBlock), 0, s, n_, b, r);
-    launch_resid(csr(), x, r, grid_, s);
+    launch_resid(csr(), x, r, grid_, s, lanes_);
     hipLaunchKernelGGL((k_norm2<double>), dim3(grid_), dim3(kBlock), 0, s, n_, (const double*)r, partials_.p);
     TF_CHECK(hipGetLastError());
     return std::sqrt(host_site_sum(partials_.p, grid_, s));
@@ -242,6 +247,7 @@ SolveResult System::solve_mixed(int method, int prec, double* x, const double* b
     TF_CHECK(hipGetLastError());
 
     double* r64 = r_out ? r_out : r64_.p;
+    double* xprev = ws_.p + (size_t)V_P * n;   // FP64 slot free during a mixed solve
     double err = true_residual(x, b, r64, s);
     SolveResult res;
     res.ini_res = err;
@@ -254,20 +260,29 @@ SolveResult System::solve_mixed(int method, int prec, double* x, const double* b
         SolverState h{};
         h.tol = inner_rtol;            // ||r32|| = 1 up to rounding: relative target
         h.cap = cap - total;
+        h.div_limit = kInnerDivLimit;
         TF_CHECK(hipMemcpyAsync(state32_.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
-        run_solve<float, DiffCsr32>(method, prec, cap - total, Aop, A32, w, S32, state32_.p, mirror32_, L, s,
+        run_solve<float, DiffCsr32>(ctx(), method, prec, cap - total, Aop, A32, w, S32, state32_.p, mirror32_, L, s,
                                     /*d_ready=*/true, /*zero_x=*/true);
         SolverState h2;
         TF_CHECK(hipMemcpy(&h2, state32_.p, sizeof(h2), hipMemcpyDeviceToHost));
         // iterations actually executed: niter on convergence, cap on exhaustion
         const int done_its = h2.niter_out > h.cap ? h.cap : h2.niter_out;
         total += done_its;
+        if (h2.diverged) break;        // the correction is worthless; x is kept
+        TF_CHECK(hipMemcpyAsync(xprev, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL(k_refine, dim3(nb), dim3(kBlock), 0, s, n_, x, err, (const float*)w.x, b, r64);
-        launch_resid(csr(), (const double*)x, r64, grid_, s);
+        launch_resid(csr(), (const double*)x, r64, grid_, s, lanes_);
         hipLaunchKernelGGL((k_norm2<double>), dim3(grid_), dim3(kBlock), 0, s, n_, (const double*)r64,
                            partials_.p);
         TF_CHECK(hipGetLastError());
-        err = std::sqrt(host_site_sum(partials_.p, grid_, s));
+        const double err_new = std::sqrt(host_site_sum(partials_.p, grid_, s));
+        if (!(err_new < err)) {        // a step that does not help is undone
+            TF_CHECK(hipMemcpyAsync(x, xprev, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            err = true_residual(x, b, r64, s);
+            break;
+        }
+        err = err_new;
         if (done_its == 0) break;      // the inner solve could not make progress
     }
     if (outer_steps) *outer_steps = outer;

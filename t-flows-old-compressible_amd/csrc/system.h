@@ -79,6 +79,8 @@ class System {
     int nnz() const { return nnz_; }
 
     void set_values(const double* aval, bool device_ptr, hipStream_t s);
+    // values of entries [offset, offset + count) only (0-based)
+    void set_values_range(int64_t offset, int64_t count, const double* aval, bool device_ptr, hipStream_t s);
     const double* values() const { return val_.p; }
     const float* values32(hipStream_t s);   // FP32 copy, made on first use
 
@@ -119,6 +121,14 @@ class System {
 
     const Levels& levels();
     int grid() const { return grid_; }
+    // SpMV form of the solves (lanes per row: 1, 2, 4 or 8)
+    int lanes() const { return lanes_; }
+    void set_lanes(int lanes);
+    BatchCtx ctx() const {
+        BatchCtx c;
+        c.lanes = lanes_;
+        return c;
+    }
 
     // raw pattern access for other components (batching, adjoint)
     const int* rowptr() const { return rowptr_.p; }
@@ -160,21 +170,28 @@ class System {
     Levels levels_;
     bool levels_ready_ = false;
     int grid_;
+    int lanes_ = 1;
 };
 
 int max_blocks_from_env();
 bool fused_cg_enabled();
 int cg_form();
 template <typename T, typename Op>
-void run_solve(int method, int prec, int cap, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
-               const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s, bool d_ready,
-               bool zero_x);
+void start_solve(const BatchCtx& c, int method, int prec, const Op& Aop, const Csr<T>& A,
+                 const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L,
+                 hipStream_t s, bool d_ready, bool zero_x);
 template <typename T, typename Op>
-void pace_iterations(int method, bool jac, int cap, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
-                     const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s);
+void run_solve(const BatchCtx& c, int method, int prec, int cap, const Op& Aop, const Csr<T>& A,
+               const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L,
+               hipStream_t s, bool d_ready, bool zero_x);
 template <typename T, typename Op>
-void enqueue_iteration(int method, bool jac, int k, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
-                       const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s);
+void pace_iterations(const BatchCtx& c, int method, bool jac, int cap, const Op& Aop, const Csr<T>& A,
+                     const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm,
+                     const Levels* L, hipStream_t s);
+template <typename T, typename Op>
+void enqueue_iteration(const BatchCtx& c, int method, bool jac, int k, const Op& Aop, const Csr<T>& A,
+                       const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm,
+                       const Levels* L, hipStream_t s);
 template <typename T>
 void dic_apply_levels(const Levels& L, const Csr<T>& A, const T* D, const T* u, T* z, const SolverState* st,
                       hipStream_t s);

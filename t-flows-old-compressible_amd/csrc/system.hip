@@ -26,6 +26,20 @@ int cg_form() {
 }
 bool fused_cg_enabled() { return cg_form() != 0; }
 
+// Lanes per row of the SpMV kernels (1, 2, 4, 8); TFLOWS_LANES sets the
+// default of new systems.
+static int lanes_from_env() {
+    const char* e = std::getenv("TFLOWS_LANES");
+    const int v = e ? std::atoi(e) : 1;
+    return (v == 2 || v == 4 || v == 8) ? v : 1;
+}
+
+void System::set_lanes(int lanes) {
+    if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8)
+        throw std::invalid_argument("lanes per row must be 1, 2, 4 or 8");
+    lanes_ = lanes;
+}
+
 int max_blocks_from_env() {
     const char* e = std::getenv("TFLOWS_MAX_BLOCKS");
     int v = e ? std::atoi(e) : 2048;
@@ -35,7 +49,7 @@ int max_blocks_from_env() {
 
 System::System(int n, int nonz, const int* acol, const int* arow, const int* adia, bool device_ptrs,
                hipStream_t s)
-    : n_(n), nnz_(nonz - 1), grid_(stream_grid(n, max_blocks_from_env())) {
+    : n_(n), nnz_(nonz - 1), grid_(stream_grid(n, max_blocks_from_env())), lanes_(lanes_from_env()) {
     if (n <= 0) throw std::invalid_argument("N must be positive");
     if (nonz < n + 1) throw std::invalid_argument("NONZ must be at least N + 1 (diagonal entries + 1)");
     rowptr_.ensure(n_ + 1);
@@ -92,6 +106,17 @@ System::~System() {
 
 void System::set_values(const double* aval, bool device_ptr, hipStream_t s) {
     TF_CHECK(hipMemcpyAsync(val_.p, aval, sizeof(double) * nnz_,
+                            device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    val32_valid_ = false;
+    diff32_valid_ = false;
+    transposed_valid_ = false;
+    if (!device_ptr) TF_CHECK(hipStreamSynchronize(s));
+}
+
+void System::set_values_range(int64_t offset, int64_t count, const double* aval, bool device_ptr,
+                              hipStream_t s) {
+    if (offset < 0 || count < 0 || offset + count > nnz_) throw std::invalid_argument("value range out of bounds");
+    TF_CHECK(hipMemcpyAsync(val_.p + offset, aval, sizeof(double) * count,
                             device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     val32_valid_ = false;
     diff32_valid_ = false;
@@ -176,75 +201,81 @@ void System::prec1(int prec, double* d, hipStream_t s) {
     ws_.ensure((size_t)V_COUNT * n_);
     TF_CHECK(hipMemsetAsync(ws_.p, 0, sizeof(double) * n_, s));
     if (prec == 2) dic_setup(d, s);
-    launch_setup(BatchCtx{}, csr(), prec, prec != 2, d, (const double*)ws_.p, sites(), s);
+    launch_setup(ctx(), csr(), prec, prec != 2, d, (const double*)ws_.p, sites(), s);
     TF_CHECK(hipGetLastError());
 }
 
 Sites<double> System::sites() { return Sites<double>::carve(partials_.p, grid_); }
 
 void System::spmv(const double* x, double* y, hipStream_t s) {
-    launch_spmv(csr(), x, y, grid_, s);
+    launch_spmv(csr(), x, y, grid_, s, lanes_);
     TF_CHECK(hipGetLastError());
 }
 void System::spmv32(const float* x, float* y, hipStream_t s) {
-    launch_spmv(diff32(s), x, y, grid_, s);
+    launch_spmv(diff32(s), x, y, grid_, s, lanes_);
     TF_CHECK(hipGetLastError());
 }
 void System::resid(const double* x, double* r, hipStream_t s) {
-    launch_resid(csr(), x, r, grid_, s);
+    launch_resid(csr(), x, r, grid_, s, lanes_);
     TF_CHECK(hipGetLastError());
 }
 
 // One iteration k of `method`, enqueued (CG.f90:113-259, BiCG.f90:100-251,
 // CGS.f90:112-334).
+// Under a batch context st / hm are the per-system arrays, k is relative
+// to c.kbase, and the DIC sweeps run without the early return (a finished
+// system's sweeps only touch its workspace).
 template <typename T, typename Op>
-void enqueue_iteration(int method, bool jac, int k, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
-                       const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s) {
+void enqueue_iteration(const BatchCtx& c, int method, bool jac, int k, const Op& Aop, const Csr<T>& A,
+                       const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm,
+                       const Levels* L, hipStream_t s) {
     const int n = A.n;
+    const bool single = c.blk_sys == nullptr;
+    const SolverState* dst = single ? st : nullptr;
     switch (method) {
         case 0: {  // CG
-            if (jac && cg_form() == 2) {  // p0 = p, p1 = p2, z = u1
+            if (single && jac && cg_form() == 2) {  // p0 = p, p1 = p2, z = u1
                 launch_cgf_spmv(false, Aop, (const T*)w.u1, w.p, w.p2, w.q, S, (const double*)nullptr, st, hm, s);
                 launch_cgf_update(n, w.x, w.r1, w.u1, (const T*)w.p, (const T*)w.p2, (const T*)w.q,
                                   (const T*)w.D, S, st, s);
                 break;
             }
-            if (jac && cg_form() == 3) {
+            if (single && jac && cg_form() == 3) {
                 launch_cgj(false, Aop, n, w.x, w.r1, w.u1, w.p, w.q, (const T*)w.D, S, (const double*)nullptr, st,
                            hm, s);
                 break;
             }
             if (!jac) {  // stopping test, z = M^-1 r into q, rho = (r, z)
-                launch_check(BatchCtx{}, k, S, st, hm, s);
-                dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.r1, w.q, st, s);
-                launch_rho(BatchCtx{}, n, (const T*)w.r1, (const T*)w.q, S, st, s);
+                launch_check(c, k, S, st, hm, s);
+                dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.r1, w.q, dst, s);
+                launch_rho(c, n, (const T*)w.r1, (const T*)w.q, S, st, s);
             }
-            launch_cg_p(BatchCtx{}, jac, k, n, (const T*)w.r1, (const T*)w.D, (const T*)w.q, w.p, S, st, hm, s);
-            launch_cg_spmv(BatchCtx{}, Aop, (const T*)w.p, w.q, S, st, s);
-            launch_cg_update(BatchCtx{}, jac, k, n, w.x, w.r1, (const T*)w.p, (const T*)w.q, (const T*)w.D, S, st, s);
+            launch_cg_p(c, jac, k, n, (const T*)w.r1, (const T*)w.D, (const T*)w.q, w.p, S, st, hm, s);
+            launch_cg_spmv(c, Aop, (const T*)w.p, w.q, S, st, s);
+            launch_cg_update(c, jac, k, n, w.x, w.r1, (const T*)w.p, (const T*)w.q, (const T*)w.D, S, st, s);
             break;
         }
         case 1: {  // BiCG (reference semantics)
             if (!jac) {
-                launch_check(BatchCtx{}, k, S, st, hm, s);
-                dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.r1, w.q, st, s);
-                dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.r2, w.q2, st, s);
-                launch_rho(BatchCtx{}, n, (const T*)w.q, (const T*)w.r2, S, st, s);
+                launch_check(c, k, S, st, hm, s);
+                dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.r1, w.q, dst, s);
+                dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.r2, w.q2, dst, s);
+                launch_rho(c, n, (const T*)w.q, (const T*)w.r2, S, st, s);
             }
-            launch_bicg_p(BatchCtx{}, jac, k, n, (const T*)w.r1, (const T*)w.r2, (const T*)w.D, (const T*)w.q, (const T*)w.q2,
+            launch_bicg_p(c, jac, k, n, (const T*)w.r1, (const T*)w.r2, (const T*)w.D, (const T*)w.q, (const T*)w.q2,
                           w.p, w.p2, S, st, hm, s);
-            launch_bicg_spmv(BatchCtx{}, Aop, (const T*)w.p, (const T*)w.p2, w.q, w.q2, S, st, s);
-            launch_bicg_update(BatchCtx{}, jac, k, n, w.x, w.r1, w.r2, (const T*)w.p, (const T*)w.q, (const T*)w.q2,
+            launch_bicg_spmv(c, Aop, (const T*)w.p, (const T*)w.p2, w.q, w.q2, S, st, s);
+            launch_bicg_update(c, jac, k, n, w.x, w.r1, w.r2, (const T*)w.p, (const T*)w.q, (const T*)w.q2,
                                (const T*)w.D, S, st, s);
             break;
         }
         default: {  // CGS: u1, u2, p2 = M^-1 u2, v2 = A p2, q1 (q), p1 (p) = M^-1 (u1+q1)
-            launch_cgs_u(BatchCtx{}, jac, k, n, (const T*)w.r1, (const T*)w.q, w.u1, w.u2, (const T*)w.D, w.p2, S, st, hm, s);
-            if (!jac) dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.u2, w.p2, st, s);
-            launch_cgs_spmv_v(BatchCtx{}, Aop, (const T*)w.p2, w.v2, (const T*)w.r2, S, st, s);
-            launch_cgs_q(BatchCtx{}, jac, k, n, (const T*)w.u1, (const T*)w.v2, w.q, (const T*)w.D, w.p, S, st, s);
-            if (!jac) dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.p, w.p, st, s);
-            launch_cgs_spmv_r(BatchCtx{}, k, Aop, (const T*)w.p, w.x, w.r1, (const T*)w.r2, S, st, s);
+            launch_cgs_u(c, jac, k, n, (const T*)w.r1, (const T*)w.q, w.u1, w.u2, (const T*)w.D, w.p2, S, st, hm, s);
+            if (!jac) dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.u2, w.p2, dst, s);
+            launch_cgs_spmv_v(c, Aop, (const T*)w.p2, w.v2, (const T*)w.r2, S, st, s);
+            launch_cgs_q(c, jac, k, n, (const T*)w.u1, (const T*)w.v2, w.q, (const T*)w.D, w.p, S, st, s);
+            if (!jac) dic_apply_levels(*L, A, (const T*)w.D, (const T*)w.p, w.p, dst, s);
+            launch_cgs_spmv_r(c, k, Aop, (const T*)w.p, w.x, w.r1, (const T*)w.r2, S, st, s);
             break;
         }
     }
@@ -258,8 +289,9 @@ void enqueue_iteration(int method, bool jac, int k, const Op& Aop, const Csr<T>&
 // queued past convergence return at once.  After iteration `cap` a last
 // stopping test (k = cap + 1) closes the loop.
 template <typename T, typename Op>
-void pace_iterations(int method, bool jac, int cap, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
-                     const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s) {
+void pace_iterations(const BatchCtx& c, int method, bool jac, int cap, const Op& Aop, const Csr<T>& A,
+                     const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm,
+                     const Levels* L, hipStream_t s) {
     hipEvent_t ev[2];
     TF_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     TF_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
@@ -271,7 +303,7 @@ void pace_iterations(int method, bool jac, int cap, const Op& Aop, const Csr<T>&
         const int k = std::min(chunk, cap - issued);
         for (int i = 0; i < k; ++i) {
             ++issued;
-            enqueue_iteration<T, Op>(method, jac, issued, Aop, A, w, S, st, hm, L, s);
+            enqueue_iteration<T, Op>(c, method, jac, issued, Aop, A, w, S, st, hm, L, s);
         }
         TF_CHECK(hipGetLastError());
         TF_CHECK(hipEventRecord(ev[slot], s));
@@ -289,7 +321,7 @@ void pace_iterations(int method, bool jac, int cap, const Op& Aop, const Csr<T>&
         if (method == 0 && jac && fused_cg_enabled())
             launch_cgf_check(false, S, (const double*)nullptr, st, hm, s);
         else
-            launch_check(BatchCtx{}, std::max(cap, 0) + 1, S, st, hm, s);
+            launch_check(c, std::max(cap, 0) + 1, S, st, hm, s);
     }
     TF_CHECK(hipGetLastError());
     TF_CHECK(hipStreamSynchronize(s));
@@ -297,32 +329,42 @@ void pace_iterations(int method, bool jac, int cap, const Op& Aop, const Csr<T>&
     (void)hipEventDestroy(ev[1]);
 }
 
-// Set-up, initial residual and iterations of one solve in precision T.
-// Aop is the operator of the SpMVs; A the CSR the DIC sweeps read.  With
-// d_ready the caller has filled w.D; with zero_x the initial guess is 0 and
-// the initial residual is b without a matrix pass.  The caller has written
-// the initial SolverState (tol, cap, stale error).
+// Set-up and initial residual of a solve in precision T (one system, or a
+// batch under its context).  Aop is the operator of the SpMVs; A the CSR
+// the DIC sweeps read.  With d_ready the caller has filled w.D; with
+// zero_x the initial guess is 0 and the initial residual is b without a
+// matrix pass.  The caller has written the initial SolverState(s) (tol,
+// cap, stale error).
 template <typename T, typename Op>
-void run_solve(int method, int prec, int cap, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
-               const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L, hipStream_t s, bool d_ready,
-               bool zero_x) {
+void start_solve(const BatchCtx& c, int method, int prec, const Op& Aop, const Csr<T>& A,
+                 const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L,
+                 hipStream_t s, bool d_ready, bool zero_x) {
+    (void)Aop;
     const bool jac = prec != 2;
-    hm->done = 0;
-    hm->iter = 0;
     if (prec == 2 && !d_ready)
         for (int l = 0; l < L->nfwd; ++l)
             launch_dic_setup_level(A, L->fwd_rows.p + L->fwd_ptr[l], L->fwd_ptr[l + 1] - L->fwd_ptr[l], w.D, s);
-    launch_setup(BatchCtx{}, A, prec, jac && !d_ready, w.D, (const T*)w.r1, S, s);
+    launch_setup(c, A, prec, jac && !d_ready, w.D, (const T*)w.r1, S, s);
     int mode = RHO_NONE_;
     T* r2 = nullptr;
-    const bool fused = method == 0 && jac && fused_cg_enabled();
+    const bool fused = c.blk_sys == nullptr && method == 0 && jac && fused_cg_enabled();
     if (method == 0) mode = (jac && !fused) ? RHO_JAC_ : RHO_NONE_;
     if (method == 1) { mode = jac ? RHO_BICG_ : RHO_NONE_; r2 = w.r2; }
     if (method == 2) { mode = RHO_RR_; r2 = w.r2; }
-    launch_resid_init(mode, zero_x, BatchCtx{}, A, (const T*)w.x, w.r1, r2, (const T*)w.D, S, st, hm, s);
+    launch_resid_init(mode, zero_x, c, A, (const T*)w.x, w.r1, r2, (const T*)w.D, S, st, hm, s);
     if (fused) launch_cgf_init_z(A.n, (const T*)w.r1, (const T*)w.D, w.u1, S, st, s);
     TF_CHECK(hipGetLastError());
-    pace_iterations<T, Op>(method, jac, cap, Aop, A, w, S, st, hm, L, s);
+}
+
+// One complete solve of one system: set-up, initial residual, iterations.
+template <typename T, typename Op>
+void run_solve(const BatchCtx& c, int method, int prec, int cap, const Op& Aop, const Csr<T>& A,
+               const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L,
+               hipStream_t s, bool d_ready, bool zero_x) {
+    hm->done = 0;
+    hm->iter = 0;
+    start_solve<T, Op>(c, method, prec, Aop, A, w, S, st, hm, L, s, d_ready, zero_x);
+    pace_iterations<T, Op>(c, method, prec != 2, cap, Aop, A, w, S, st, hm, L, s);
 }
 
 SolveResult System::solve(int method, int prec, double* x, double* r1, int cap, double tol, double stale_error,
@@ -345,7 +387,8 @@ SolveResult System::solve(int method, int prec, double* x, double* r1, int cap, 
     static const Levels kNoLevels;
     const Levels* L = prec == 2 ? &levels() : &kNoLevels;
     const Csr<double> A = csr();
-    run_solve<double, Csr<double>>(method, prec, cap, A, A, w, sites(), state_.p, mirror_, L, s, false, false);
+    run_solve<double, Csr<double>>(ctx(), method, prec, cap, A, A, w, sites(), state_.p, mirror_, L, s, false,
+                                   false);
     SolverState h2;
     TF_CHECK(hipMemcpy(&h2, state_.p, sizeof(h2), hipMemcpyDeviceToHost));
     if (!h2.done) throw HipError("solver finished without reaching a stopping state");
@@ -375,15 +418,19 @@ KrylovVectors<double> System::vectors(double* x, double* r1) {
     return w;
 }
 
-#define TF_SOLVE_INST(T, Op)                                                                                      \
-    template void enqueue_iteration<T, Op>(int, bool, int, const Op&, const Csr<T>&, const KrylovVectors<T>&,     \
-                                           const Sites<T>&, SolverState*, HostMirror*, const Levels*,             \
-                                           hipStream_t);                                                          \
-    template void pace_iterations<T, Op>(int, bool, int, const Op&, const Csr<T>&, const KrylovVectors<T>&,       \
-                                         const Sites<T>&, SolverState*, HostMirror*, const Levels*, hipStream_t); \
-    template void run_solve<T, Op>(int, int, int, const Op&, const Csr<T>&, const KrylovVectors<T>&,              \
-                                   const Sites<T>&, SolverState*, HostMirror*, const Levels*, hipStream_t, bool, \
-                                   bool);
+#define TF_SOLVE_INST(T, Op)                                                                                  \
+    template void enqueue_iteration<T, Op>(const BatchCtx&, int, bool, int, const Op&, const Csr<T>&,         \
+                                           const KrylovVectors<T>&, const Sites<T>&, SolverState*,            \
+                                           HostMirror*, const Levels*, hipStream_t);                          \
+    template void pace_iterations<T, Op>(const BatchCtx&, int, bool, int, const Op&, const Csr<T>&,           \
+                                         const KrylovVectors<T>&, const Sites<T>&, SolverState*, HostMirror*, \
+                                         const Levels*, hipStream_t);                                         \
+    template void start_solve<T, Op>(const BatchCtx&, int, int, const Op&, const Csr<T>&,                     \
+                                     const KrylovVectors<T>&, const Sites<T>&, SolverState*, HostMirror*,     \
+                                     const Levels*, hipStream_t, bool, bool);                                 \
+    template void run_solve<T, Op>(const BatchCtx&, int, int, int, const Op&, const Csr<T>&,                  \
+                                   const KrylovVectors<T>&, const Sites<T>&, SolverState*, HostMirror*,       \
+                                   const Levels*, hipStream_t, bool, bool);
 TF_SOLVE_INST(double, Csr<double>)
 TF_SOLVE_INST(float, Csr<float>)
 TF_SOLVE_INST(float, DiffCsr32)

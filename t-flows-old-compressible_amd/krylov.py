@@ -115,6 +115,28 @@ def _bind_extensions(lib):
         f = lib.tflows_solve_mixed_reliable
         f.restype = C.c_int
         f.argtypes = [vp, C.c_int32, vp, vp, vp, _ip, C.c_double, C.c_int32, _dp, _dp, _ip, vp]
+    if hasattr(lib, "tflows_system_set_lanes"):
+        lib.tflows_system_set_lanes.restype = C.c_int
+        lib.tflows_system_set_lanes.argtypes = [vp, C.c_int32]
+    if hasattr(lib, "tflows_batch_create"):
+        pp = C.POINTER(vp)
+        lib.tflows_batch_create.restype = C.c_int
+        lib.tflows_batch_create.argtypes = [C.c_int32, vp, vp, pp, pp, pp, pp]
+        lib.tflows_batch_destroy.restype = C.c_int
+        lib.tflows_batch_destroy.argtypes = [vp]
+        lib.tflows_batch_set_values.restype = C.c_int
+        lib.tflows_batch_set_values.argtypes = [vp, C.c_int32, vp, C.c_int32, vp]
+        lib.tflows_batch_layout.restype = C.c_int
+        lib.tflows_batch_layout.argtypes = [vp, _ip, C.POINTER(C.c_int64), vp]
+        lib.tflows_batch_configure.restype = C.c_int
+        lib.tflows_batch_configure.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+        lib.tflows_batch_config.restype = C.c_int
+        lib.tflows_batch_config.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _ip, _ip, _ip, _ip]
+        lib.tflows_batch_solve.restype = C.c_int
+        lib.tflows_batch_solve.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp]
+        lib.tflows_batch_solve_mixed.restype = C.c_int
+        lib.tflows_batch_solve_mixed.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, C.c_double,
+                                                 C.c_int32, vp, vp, vp, _ip, vp]
 
 
 def _check(rc: int):
@@ -302,9 +324,114 @@ class DeviceSystem:
     def levels(self) -> int:
         return int(self.lib.tflows_system_levels(self.handle))
 
+    def set_lanes(self, lanes: int):
+        """SpMV form: lanes per row (1, 2, 4 or 8); results are identical."""
+        _check(self.lib.tflows_system_set_lanes(self.handle, lanes))
+
     def close(self):
         if getattr(self, "handle", None) is not None:
             self.lib.tflows_system_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBatch:
+    """K independent systems solved together on the GPU.
+
+    ``matrices`` is a list of meshgen.TFlowSMatrix (any sizes and
+    patterns).  Vectors are cuda float64 tensors of ``rows`` entries holding
+    the systems back to back; ``offsets[k]`` is the first row of system k.
+    """
+
+    def __init__(self, matrices, stream=None):
+        self.lib = load_library()
+        self.k = len(matrices)
+        self.matrices = list(matrices)
+        n = np.array([m.n for m in matrices], dtype=np.int32)
+        nonz = np.array([m.nonz for m in matrices], dtype=np.int32)
+        keep = [(np.ascontiguousarray(m.acol, np.int32), np.ascontiguousarray(m.arow, np.int32),
+                 np.ascontiguousarray(m.adia, np.int32)) for m in matrices]
+        arr = C.c_void_p * self.k
+        acol = arr(*[a[0].ctypes.data for a in keep])
+        arow = arr(*[a[1].ctypes.data for a in keep])
+        adia = arr(*[a[2].ctypes.data for a in keep])
+        h = C.c_void_p()
+        _check(self.lib.tflows_batch_create(self.k, n.ctypes.data, nonz.ctypes.data,
+                                            C.cast(acol, C.POINTER(C.c_void_p)),
+                                            C.cast(arow, C.POINTER(C.c_void_p)),
+                                            C.cast(adia, C.POINTER(C.c_void_p)), C.byref(h)))
+        self.handle = h
+        rows = C.c_int32(0)
+        ents = C.c_int64(0)
+        offs = np.zeros(self.k + 1, dtype=np.int32)
+        _check(self.lib.tflows_batch_layout(h, C.byref(rows), C.byref(ents), offs.ctypes.data))
+        self.rows, self.entries, self.offsets = rows.value, ents.value, offs
+        for k, m in enumerate(matrices):
+            self.set_values(k, np.ascontiguousarray(m.aval[:m.nnz], np.float64))
+
+    def set_values(self, k: int, aval, stream=None):
+        """Values of system k (k = -1: all systems back to back); a numpy
+        array (host) or a cuda float64 tensor (device)."""
+        if isinstance(aval, np.ndarray):
+            a = np.ascontiguousarray(aval, np.float64)
+            _check(self.lib.tflows_batch_set_values(self.handle, k, a.ctypes.data, 0, None))
+        else:
+            _check(self.lib.tflows_batch_set_values(self.handle, k, C.c_void_p(_ptr(aval)), 1, _stream(stream)))
+
+    def configure(self, grid: int = 0, lanes: int = 0, chunk: int = 0, rebalance: bool = True):
+        _check(self.lib.tflows_batch_configure(self.handle, grid, lanes, chunk, 1 if rebalance else 0))
+
+    def config(self, method: str = "cg", prec: int = 1, bits: int = 64) -> dict:
+        g, l, c, r = (C.c_int32(0) for _ in range(4))
+        _check(self.lib.tflows_batch_config(self.handle, bits, METHODS[method], prec, C.byref(g), C.byref(l),
+                                            C.byref(c), C.byref(r)))
+        return dict(grid=g.value, lanes=l.value, chunk=c.value, rebalances=r.value)
+
+    def _arrays(self, cap, tol, fin_in):
+        cap = np.ascontiguousarray(np.broadcast_to(np.asarray(cap, np.int32), (self.k,)))
+        tol = np.ascontiguousarray(np.broadcast_to(np.asarray(tol, np.float64), (self.k,)))
+        fin = np.ascontiguousarray(np.broadcast_to(np.asarray(fin_in, np.float64), (self.k,))).copy()
+        ini = np.full(self.k, _SENTINEL)
+        nit = np.zeros(self.k, np.int32)
+        return cap, tol, fin, ini, nit
+
+    def solve(self, method: str, prec: int, x, r1, cap, tol, fin_in=0.0, stream=None):
+        """FP64 solve of every system; x, r1 cuda float64 tensors (r1 in: b,
+        out: final residual).  cap, tol: scalars or per-system arrays.
+        Returns dict of per-system arrays niter, ini_res (nan where not
+        written), fin_res."""
+        cap, tol, fin, ini, nit = self._arrays(cap, tol, fin_in)
+        _check(self.lib.tflows_batch_solve(self.handle, METHODS[method], prec, C.c_void_p(_ptr(x)),
+                                           C.c_void_p(_ptr(r1)), cap.ctypes.data, tol.ctypes.data,
+                                           nit.ctypes.data, ini.ctypes.data, fin.ctypes.data, _stream(stream)))
+        ini = np.where(ini == _SENTINEL, np.nan, ini)
+        return dict(niter=nit, ini_res=ini, fin_res=fin)
+
+    def solve_mixed(self, method: str, prec: int, x, b, cap, tol, inner_rtol: float = 1e-4,
+                    max_outer: int = 100, r_out=None, stream=None):
+        """FP32 inner solves inside FP64 refinement, per system; b unchanged.
+        Returns dict niter (total inner iterations), ini_res, fin_res (FP64
+        true residual norms), outer (refinement steps of the batch)."""
+        cap, tol, fin, ini, nit = self._arrays(cap, tol, 0.0)
+        outer = C.c_int32(0)
+        _check(self.lib.tflows_batch_solve_mixed(
+            self.handle, METHODS[method], prec, C.c_void_p(_ptr(x)), C.c_void_p(_ptr(b)),
+            C.c_void_p(_ptr(r_out) if r_out is not None else 0), cap.ctypes.data, tol.ctypes.data, inner_rtol,
+            max_outer, nit.ctypes.data, ini.ctypes.data, fin.ctypes.data, C.byref(outer), _stream(stream)))
+        return dict(niter=nit, ini_res=ini, fin_res=fin, outer=outer.value)
+
+    def split(self, v):
+        """Per-system views of a batch vector."""
+        return [v[self.offsets[k]:self.offsets[k + 1]] for k in range(self.k)]
+
+    def close(self):
+        if getattr(self, "handle", None) is not None:
+            self.lib.tflows_batch_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

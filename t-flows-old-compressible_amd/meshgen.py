@@ -507,6 +507,66 @@ def momentum_system(n: int | tuple, seed: int = 2, gamma: float = 1e-3, prec: in
                              generator_version=GENERATOR_VERSION))
 
 
+def transport_batch(n: int = 96, count: int = 5, seed: int = 3, prec: int = 1, niter: int = 2000,
+                    rel_tol: float = 1e-8, renumber: bool = True) -> list[LinearSystem]:
+    """Systems sharing one pattern: the momentum components and scalars of
+    one mesh, as NewUVW.f90:506 (u, v, w) and CalcZmix.f90:394 (Zmix, Y)
+    solve them one call each.
+
+    One n^3 unit-cube mesh and one random cell renumbering (seed 100*seed);
+    system k (seed 100*seed + k) has its own divergence-free convecting
+    velocity, its own diffusion coefficient Gamma_k = 10**U(-3.5, -2.5) and
+    its own right-hand side b ~ U(-1,1); CFL 0.5 time term as in
+    transport_matrix.
+    """
+    mesh = BoxMesh(np.linspace(0, 1, n + 1), np.linspace(0, 1, n + 1), np.linspace(0, 1, n + 1))
+    base = 100 * seed
+    perm = random_permutation(mesh.n_cells, base) if renumber else None
+    names = ("u", "v", "w", "zmix", "y")
+    out = []
+    for k in range(count):
+        sk = base + k
+        gamma = float(10.0 ** np.random.default_rng(sk).uniform(-3.5, -2.5))
+        m = transport_matrix(mesh, gamma, sk, perm=perm)
+        b = seeded_rhs(m.n, sk, False, perm)
+        tol = rel_tol * float(np.sqrt(np.dot(b, b)))
+        out.append(LinearSystem(m, b, np.zeros(m.n), tol, prec, niter,
+                                dict(recipe="transport_batch", name=names[k % len(names)], shape=(n, n, n),
+                                     seed=sk, gamma=gamma, rel_tol=rel_tol, renumber=renumber,
+                                     generator_version=GENERATOR_VERSION)))
+    return out
+
+
+def uneven_pressure_batch(count: int = 64, seed: int = 3, n_min: float = 1e4, n_max: float = 5e5,
+                          prec: int = 1, niter: int = 5000, rel_tol: float = 1e-8) -> list[LinearSystem]:
+    """Independent pressure-correction systems of uneven sizes.
+
+    Cell counts N_k are log-uniform on [n_min, n_max] and the box of system
+    k has seeded aspect ratios a ~ U(1,4)^3 (rng seed ``seed``): its
+    dimensions are round(a_i * (N_k / prod a)^(1/3)), at least 2.  Unit
+    cubic cells, cell 1 pinned (pressure_matrix), b ~ U(-1,1) from seed
+    100*seed + k with its mean removed, tol = rel_tol * ||b||.
+    """
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(n_min), np.log(n_max), size=count))
+    aspect = rng.uniform(1.0, 4.0, size=(count, 3))
+    out = []
+    for k in range(count):
+        a = aspect[k]
+        s = (sizes[k] / float(np.prod(a))) ** (1.0 / 3.0)
+        nx, ny, nz = (max(2, int(round(v * s))) for v in a)
+        mesh = BoxMesh(np.arange(nx + 1, dtype=np.float64), np.arange(ny + 1, dtype=np.float64),
+                       np.arange(nz + 1, dtype=np.float64))
+        m = pressure_matrix(mesh, pin_cell=1)
+        sk = 100 * seed + k
+        b = seeded_rhs(m.n, sk, True)
+        tol = rel_tol * float(np.sqrt(np.dot(b, b)))
+        out.append(LinearSystem(m, b, np.zeros(m.n), tol, prec, niter,
+                                dict(recipe="uneven_pressure_batch", shape=(nx, ny, nz), seed=sk,
+                                     rel_tol=rel_tol, generator_version=GENERATOR_VERSION)))
+    return out
+
+
 def from_scipy(A, meta=None) -> TFlowSMatrix:
     """Wrap a square scipy sparse matrix (structurally symmetric pattern with a
     stored diagonal) in the T-FlowS layout, columns sorted per row."""
