@@ -71,6 +71,44 @@ def time_spmv(torch, ds, n, reps=50, fp32=False):
     return e0.elapsed_time(e1) / reps * 1e-3  # seconds per launch
 
 
+COPY_BYTES = 512 * 1024 * 1024
+
+
+def profile_spmv(torch, ds, n, reps=20):
+    """The launches the traffic counters are read on: `reps` FP64 and FP32
+    SpMVs of the workload and `reps` copies of COPY_BYTES (a kernel whose
+    HBM traffic is known: COPY_BYTES read, COPY_BYTES written), which
+    calibrates FETCH_SIZE / WRITE_SIZE."""
+    for dt, f in ((torch.float64, ds.spmv), (torch.float32, ds.spmv32)):
+        x = torch.randn(n, dtype=dt, device="cuda")
+        y = torch.empty(n, dtype=dt, device="cuda")
+        for _ in range(reps):
+            f(x, y)
+    src = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"profile-spmv: {reps} x (spmv fp64, spmv fp32, copy {COPY_BYTES} B) done")
+
+
+def measured_traffic(kernel_tag):
+    """HBM bytes per launch of the SpMV kernel from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy calibration),
+    or None when no summary for this workload is present."""
+    path = os.path.join(REPO, "profiles", "spmv_traffic_pressure128.md")
+    if not os.path.exists(path):
+        return None
+    key = f"traffic_bytes_per_launch[{kernel_tag}]"
+    for line in open(path):
+        if line.strip().startswith(key):
+            try:
+                return int(float(line.split("=")[1].strip().strip("`")))
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(sysm, b, tol, full_iters, budget_s=20.0):
     """Reference Fortran CG (or its C restatement) on one core, bounded sample."""
     from oracle import oracle as O
@@ -104,6 +142,9 @@ def main():
     ap.add_argument("--interval", type=int, default=50,
                     help="FP32 CG iterations between FP64 residual replacements")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--profile-spmv", action="store_true",
+                    help="only run the SpMV launches (FP64 and FP32, 20 each) and a 512 MiB device copy, "
+                         "for hardware-counter runs; prints nothing on stdout")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note "
@@ -122,6 +163,10 @@ def main():
     N, nnz = m.n, m.nnz
     log(f"N={N} nnz={nnz}")
     ds = pkg.krylov.DeviceSystem(m)
+    if args.profile_spmv:
+        profile_spmv(torch, ds, N)
+        ds.close()
+        return
     b_dev = torch.from_numpy(s.b).cuda()
     x = torch.zeros(N, dtype=torch.float64, device="cuda")
     r = torch.empty_like(b_dev)
@@ -201,7 +246,8 @@ def main():
         "spmv_fp64_GBs": round(b64 / t64 / 1e9, 1),
         "spmv_fp32_GBs": round(b32 / t32 / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic("fp32" if head == "mixed" else "fp64") if n == 128 else None,
                      "kernel": "k_spmv<DiffCsr32>" if head == "mixed" else "k_spmv<Csr<double>>",
                      "bytes_per_launch": spmv_b, "us_per_launch": round(spmv_t * 1e6, 2)},
     }

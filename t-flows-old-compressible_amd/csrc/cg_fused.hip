@@ -304,11 +304,11 @@ __global__ __launch_bounds__(kBlock) void k_cgj_update(int n, T* __restrict__ x,
 
 template <typename Op, typename T = typename OpValue<Op>::type>
 __global__ __launch_bounds__(kBlock) void k_cgj_spmv(Op A, const T* __restrict__ p, T* __restrict__ q,
-                                                     T* __restrict__ partD, const SolverState* st) {
+                                                     T* __restrict__ partD, const SolverState* st, int strided) {
     if (st->done || st->suspend) return;
-    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
-    int t0, t1;
-    block_tiles(ntiles, t0, t1);
+    BatchCtx one;
+    one.strided = strided;
+    const TileSpan sp = ctx_span(one, A.n);
     Gather1<T> g{p};
     struct Epi {
         const T* __restrict__ p;
@@ -319,7 +319,8 @@ __global__ __launch_bounds__(kBlock) void k_cgj_spmv(Op A, const T* __restrict__
             acc[0] += p[row] * y[0];                     // (p, q)  CG.f90:216
         }
     } epi{p, q, {T(0)}};
-    for (int t = t0; t < t1; ++t) op_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+    for (int t = sp.first; t < sp.end; t += sp.step)
+        op_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
     site_write<T, 1>(epi.acc, partD);
 }
 
@@ -349,20 +350,20 @@ void launch_cgf_init_z(int n, const T* r, const T* D, T* z, const Sites<T>& S, c
 }
 
 template <typename Op, typename T>
-void launch_cgj(bool mixed, const Op& A, int n, T* x, T* r, T* z, T* p, T* q, const T* D, const Sites<T>& S,
-                const double* partE, SolverState* st, HostMirror* hm, hipStream_t s) {
+void launch_cgj(bool mixed, bool strided, const Op& A, int n, T* x, T* r, T* z, T* p, T* q, const T* D,
+                const Sites<T>& S, const double* partE, SolverState* st, HostMirror* hm, hipStream_t s) {
     if (mixed) TF_FL((k_cgj_p<T, true>), S.grid, s), (const T*)z, p, S.B, partE, n, st, hm);
     else TF_FL((k_cgj_p<T, false>), S.grid, s), (const T*)z, p, S.B, partE, n, st, hm);
-    TF_FL((k_cgj_spmv<Op>), S.grid, s), A, (const T*)p, q, S.D, st);
+    TF_FL((k_cgj_spmv<Op>), S.grid, s), A, (const T*)p, q, S.D, st, strided ? 1 : 0);
     TF_FL((k_cgj_update<T>), S.grid, s), n, x, r, z, (const T*)p, (const T*)q, D, S.D, S.B, st);
 }
-template void launch_cgj<Csr<double>, double>(bool, const Csr<double>&, int, double*, double*, double*, double*,
+template void launch_cgj<Csr<double>, double>(bool, bool, const Csr<double>&, int, double*, double*, double*, double*,
                                               double*, const double*, const Sites<double>&, const double*,
                                               SolverState*, HostMirror*, hipStream_t);
-template void launch_cgj<Csr<float>, float>(bool, const Csr<float>&, int, float*, float*, float*, float*, float*,
+template void launch_cgj<Csr<float>, float>(bool, bool, const Csr<float>&, int, float*, float*, float*, float*, float*,
                                             const float*, const Sites<float>&, const double*, SolverState*,
                                             HostMirror*, hipStream_t);
-template void launch_cgj<DiffCsr32, float>(bool, const DiffCsr32&, int, float*, float*, float*, float*, float*,
+template void launch_cgj<DiffCsr32, float>(bool, bool, const DiffCsr32&, int, float*, float*, float*, float*, float*,
                                            const float*, const Sites<float>&, const double*, SolverState*,
                                            HostMirror*, hipStream_t);
 

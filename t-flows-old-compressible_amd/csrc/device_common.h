@@ -142,6 +142,24 @@ __device__ __forceinline__ int logical_block() {
     return (b & (kNumXcd - 1)) * (G / kNumXcd) + b / kNumXcd;
 }
 
+// Tiles of a block, visited as t = first, first + step, ... < end.
+struct TileSpan {
+    int first, end, step;
+};
+
+// Strided span for the SpMV kernels: XCD x (blocks b with b mod 8 == x)
+// owns one contiguous eighth of the tiles, and its G/8 blocks walk that
+// chunk with stride G/8, so at any moment an XCD works on a window of G/8
+// consecutive tiles.  Rows at +-(one plane) of a structured mesh were then
+// gathered by the previous or the next window, still in that XCD's L2.
+__device__ __forceinline__ TileSpan block_span_strided(int ntiles) {
+    const int G = gridDim.x, b = blockIdx.x;
+    if (G < kNumXcd || (G & (kNumXcd - 1))) return TileSpan{b, ntiles, G};
+    const int x = b & (kNumXcd - 1), per = G / kNumXcd;
+    const int c0 = (int)(((int64_t)x * ntiles) / kNumXcd), c1 = (int)(((int64_t)(x + 1) * ntiles) / kNumXcd);
+    return TileSpan{c0 + b / kNumXcd, c1, per};
+}
+
 // Tiles [t0, t1) owned by this block when `ntiles` tiles are split evenly.
 __device__ __forceinline__ void block_tiles(int ntiles, int& t0, int& t1) {
     const int G = gridDim.x, lb = logical_block();

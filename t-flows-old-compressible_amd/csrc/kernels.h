@@ -81,6 +81,9 @@ struct BatchCtx {
     const int* sys_blk0 = nullptr;
     const int* kbase = nullptr;
     int lanes = 1;
+    // tile order of the SpMV kernels of one system: 0 = each block a
+    // contiguous range, 1 = strided inside each XCD's chunk
+    int strided = 0;
 
     __device__ __forceinline__ int sys() const { return blk_sys ? blk_sys[blockIdx.x] : 0; }
     __device__ __forceinline__ bool leader() const {
@@ -267,6 +270,17 @@ __device__ __forceinline__ void ctx_tiles(const BatchCtx& c, int n, int& t0, int
         t0 = c.blk_tiles[2 * blockIdx.x];
         t1 = c.blk_tiles[2 * blockIdx.x + 1];
     }
+}
+// Tiles of this block for the SpMV kernels.
+__device__ __forceinline__ TileSpan ctx_span(const BatchCtx& c, int n) {
+    if (!c.blk_sys) {
+        const int ntiles = (n + kTileRows - 1) / kTileRows;
+        if (c.strided) return block_span_strided(ntiles);
+        int t0, t1;
+        block_tiles(ntiles, t0, t1);
+        return TileSpan{t0, t1, 1};
+    }
+    return TileSpan{c.blk_tiles[2 * blockIdx.x], c.blk_tiles[2 * blockIdx.x + 1], 1};
 }
 __device__ __forceinline__ void ctx_tile_rows(const BatchCtx& c, int n, int t, int& r0, int& r1) {
     if (!c.tile_r0) {

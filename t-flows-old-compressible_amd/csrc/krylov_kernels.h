@@ -74,11 +74,12 @@ void launch_cgs_q(const BatchCtx& c, bool jac, int k, int n, const T* u1, const 
 template <typename Op, typename T>
 void launch_cgs_spmv_r(const BatchCtx& c, int k, const Op& A, const T* p1, T* x, T* r1, const T* r2,
                        const Sites<T>& S, const SolverState* st, hipStream_t s);
-// stand-alone q = A p and r -= A x; `lanes` as in BatchCtx
+// stand-alone q = A p and r -= A x of one system; the context selects the
+// SpMV form (lanes) and tile order
 template <typename Op, typename T>
-void launch_spmv(const Op& A, const T* x, T* y, int grid, hipStream_t s, int lanes = 1);
+void launch_spmv(const Op& A, const T* x, T* y, int grid, hipStream_t s, const BatchCtx& c = BatchCtx{});
 template <typename T>
-void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s, int lanes = 1);
+void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s, const BatchCtx& c = BatchCtx{});
 // fused two-kernel CG (cg_fused.hip)
 template <typename Op, typename T>
 void launch_cgf_spmv(bool mixed, const Op& A, const T* z, T* p0, T* p1, T* q, const Sites<T>& S,
@@ -91,7 +92,7 @@ void launch_cgf_check(bool mixed, const Sites<T>& S, const double* partE, Solver
                       hipStream_t s);
 // three-kernel Jacobi CG with device-side iteration counters (cg_fused.hip)
 template <typename Op, typename T>
-void launch_cgj(bool mixed, const Op& A, int n, T* x, T* r, T* z, T* p, T* q, const T* D, const Sites<T>& S,
+void launch_cgj(bool mixed, bool strided, const Op& A, int n, T* x, T* r, T* z, T* p, T* q, const T* D, const Sites<T>& S,
                 const double* partE, SolverState* st, HostMirror* hm, hipStream_t s);
 template <typename T>
 void launch_cgf_init_z(int n, const T* r, const T* D, T* z, const Sites<T>& S, const SolverState* st,

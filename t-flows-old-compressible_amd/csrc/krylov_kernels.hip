@@ -28,6 +28,7 @@
 //   BiCG   Process/BiCG.f90:50-254          -> k_bicg_p, k_bicg_spmv, k_bicg_update
 //   CGS    Process/CGS.f90:62-337           -> k_cgs_u, k_cgs_spmv_v, k_cgs_q, k_cgs_spmv_r
 //   DIC    Process/CG.f90:135-156           -> k_dic_fwd_level, k_dic_bwd_level
+#include <stdexcept>
 #include <type_traits>
 
 #include "krylov_kernels.h"
@@ -232,11 +233,10 @@ __global__ __launch_bounds__(kBlock) void k_resid_init(BatchCtx ctx, Csr<T> A, c
             epi(i, y);
         }
     } else {
-        int t0, t1;
-        ctx_tiles(ctx, A.n, t0, t1);
+        const TileSpan sp = ctx_span(ctx, A.n);
         Gather1<T> g{x};
         InitVec<T> init{r};
-        for (int t = t0; t < t1; ++t) {
+        for (int t = sp.first; t < sp.end; t += sp.step) {
             int r0, r1;
             ctx_tile_rows(ctx, A.n, t, r0, r1);
             resid_tile<L>(A, r0, r1, g, init, epi);
@@ -344,11 +344,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(BatchCtx ctx, Op A, const T*
                                                     T* __restrict__ partD, const SolverState* st_base) {
     const SolverState* st = st_base + ctx.sys();
     TF_DONE_RETURN(partD, 1);
-    int t0, t1;
-    ctx_tiles(ctx, A.n, t0, t1);
+    const TileSpan sp = ctx_span(ctx, A.n);
     Gather1<T> g{p};
     DotEpi<T> epi{p, q, {T(0)}};
-    for (int t = t0; t < t1; ++t) {
+    for (int t = sp.first; t < sp.end; t += sp.step) {
         int r0, r1;
         ctx_tile_rows(ctx, A.n, t, r0, r1);
         op_tile<1, L>(A, r0, r1, g, epi);
@@ -431,11 +430,10 @@ __global__ __launch_bounds__(kBlock) void k_bicg_spmv(BatchCtx ctx, Op A, const 
                                                       const SolverState* st_base) {
     const SolverState* st = st_base + ctx.sys();
     TF_DONE_RETURN(partD, 1);
-    int t0, t1;
-    ctx_tiles(ctx, A.n, t0, t1);
+    const TileSpan sp = ctx_span(ctx, A.n);
     Gather2<T> g{p1, p2};
     BiDotEpi<T> epi{p2, q1, q2, {T(0)}};
-    for (int t = t0; t < t1; ++t) {
+    for (int t = sp.first; t < sp.end; t += sp.step) {
         int r0, r1;
         ctx_tile_rows(ctx, A.n, t, r0, r1);
         op_tile<2, L>(A, r0, r1, g, epi);
@@ -516,11 +514,10 @@ __global__ __launch_bounds__(kBlock) void k_cgs_spmv_v(BatchCtx ctx, Op A, const
                                                        T* __restrict__ partD, const SolverState* st_base) {
     const SolverState* st = st_base + ctx.sys();
     TF_DONE_RETURN(partD, 1);
-    int t0, t1;
-    ctx_tiles(ctx, A.n, t0, t1);
+    const TileSpan sp = ctx_span(ctx, A.n);
     Gather1<T> g{p2};
     DotEpi<T> epi{r2, v2, {T(0)}};
-    for (int t = t0; t < t1; ++t) {
+    for (int t = sp.first; t < sp.end; t += sp.step) {
         int r0, r1;
         ctx_tile_rows(ctx, A.n, t, r0, r1);
         op_tile<1, L>(A, r0, r1, g, epi);
@@ -582,11 +579,10 @@ __global__ __launch_bounds__(kBlock) void k_cgs_spmv_r(BatchCtx ctx, int k_arg, 
     T rv[1];
     ctx_total<T, 1>(ctx, partD, rv);
     const T alfa = (T)st->rho[k & 1] / rv[0];
-    int t0, t1;
-    ctx_tiles(ctx, A.n, t0, t1);
+    const TileSpan sp = ctx_span(ctx, A.n);
     Gather1<T> g{p1};
     CgsREpi<T> epi{x, r1, r2, p1, alfa, {T(0), T(0)}};
-    for (int t = t0; t < t1; ++t) {
+    for (int t = sp.first; t < sp.end; t += sp.step) {
         int r0, r1_;
         ctx_tile_rows(ctx, A.n, t, r0, r1_);
         op_tile<1, L>(A, r0, r1_, g, epi);
@@ -605,24 +601,22 @@ struct StoreEpi {
 };
 
 template <typename Op, int L, typename T = typename OpValue<Op>::type>
-__global__ __launch_bounds__(kBlock) void k_spmv(Op A, const T* __restrict__ x, T* __restrict__ y) {
-    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
-    int t0, t1;
-    block_tiles(ntiles, t0, t1);
+__global__ __launch_bounds__(kBlock) void k_spmv(BatchCtx ctx, Op A, const T* __restrict__ x, T* __restrict__ y) {
+    const TileSpan sp = ctx_span(ctx, A.n);
     Gather1<T> g{x};
     StoreEpi<T> epi{y};
-    for (int t = t0; t < t1; ++t) op_tile<1, L>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+    for (int t = sp.first; t < sp.end; t += sp.step)
+        op_tile<1, L>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
 }
 
 template <typename T, int L>
-__global__ __launch_bounds__(kBlock) void k_resid(Csr<T> A, const T* __restrict__ x, T* __restrict__ r) {
-    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
-    int t0, t1;
-    block_tiles(ntiles, t0, t1);
+__global__ __launch_bounds__(kBlock) void k_resid(BatchCtx ctx, Csr<T> A, const T* __restrict__ x,
+                                                  T* __restrict__ r) {
+    const TileSpan sp = ctx_span(ctx, A.n);
     Gather1<T> g{x};
     InitVec<T> init{r};
     StoreEpi<T> epi{r};
-    for (int t = t0; t < t1; ++t)
+    for (int t = sp.first; t < sp.end; t += sp.step)
         resid_tile<L>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi);
 }
 
@@ -768,17 +762,19 @@ void launch_cgs_spmv_r(const BatchCtx& c, int k, const Op& A, const T* p1, T* x,
     });
 }
 template <typename Op, typename T>
-void launch_spmv(const Op& A, const T* x, T* y, int grid, hipStream_t s, int lanes) {
-    with_lanes(lanes, [&](auto lc_) {
+void launch_spmv(const Op& A, const T* x, T* y, int grid, hipStream_t s, const BatchCtx& c) {
+    if (c.blk_sys) throw std::invalid_argument("stand-alone SpMV takes a one-system context");
+    with_lanes(c.lanes, [&](auto lc_) {
         TF_L;
-        TF_LAUNCH((k_spmv<Op, L>), grid, s), A, x, y);
+        TF_LAUNCH((k_spmv<Op, L>), grid, s), c, A, x, y);
     });
 }
 template <typename T>
-void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s, int lanes) {
-    with_lanes(lanes, [&](auto lc_) {
+void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s, const BatchCtx& c) {
+    if (c.blk_sys) throw std::invalid_argument("stand-alone residual takes a one-system context");
+    with_lanes(c.lanes, [&](auto lc_) {
         TF_L;
-        TF_LAUNCH((k_resid<T, L>), grid, s), A, x, r);
+        TF_LAUNCH((k_resid<T, L>), grid, s), c, A, x, r);
     });
 }
 void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t s) {
@@ -813,7 +809,7 @@ void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t 
                                   const Sites<T>&, SolverState*, HostMirror*, hipStream_t);                      \
     template void launch_cgs_q<T>(const BatchCtx&, bool, int, int, const T*, const T*, T*, const T*, T*,        \
                                   const Sites<T>&, const SolverState*, hipStream_t);                             \
-    template void launch_resid<T>(const Csr<T>&, const T*, T*, int, hipStream_t, int);
+    template void launch_resid<T>(const Csr<T>&, const T*, T*, int, hipStream_t, const BatchCtx&);
 
 #define TF_INSTANTIATE_OP(Op, T)                                                                                 \
     template void launch_cg_spmv<Op, T>(const BatchCtx&, const Op&, const T*, T*, const Sites<T>&,              \
@@ -824,7 +820,7 @@ void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t 
                                            const SolverState*, hipStream_t);                                     \
     template void launch_cgs_spmv_r<Op, T>(const BatchCtx&, int, const Op&, const T*, T*, T*, const T*,         \
                                            const Sites<T>&, const SolverState*, hipStream_t);                    \
-    template void launch_spmv<Op, T>(const Op&, const T*, T*, int, hipStream_t, int);
+    template void launch_spmv<Op, T>(const Op&, const T*, T*, int, hipStream_t, const BatchCtx&);
 
 TF_INSTANTIATE(double)
 TF_INSTANTIATE(float)

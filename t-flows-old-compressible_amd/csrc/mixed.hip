@@ -198,7 +198,7 @@ const float* System::diag32(hipStream_t s) {
 double System::true_residual(const double* x, const double* b, double* r, hipStream_t s) {
     const int nb = (n_ + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_copy, dim3(nb), dim3(kBlock), 0, s, n_, b, r);
-    launch_resid(csr(), x, r, grid_, s, lanes_);
+    launch_resid(csr(), x, r, grid_, s, ctx());
     hipLaunchKernelGGL((k_norm2<double>), dim3(grid_), dim3(kBlock), 0, s, n_, (const double*)r, partials_.p);
     TF_CHECK(hipGetLastError());
     return std::sqrt(host_site_sum(partials_.p, grid_, s));
@@ -272,7 +272,7 @@ SolveResult System::solve_mixed(int method, int prec, double* x, const double* b
         if (h2.diverged) break;        // the correction is worthless; x is kept
         TF_CHECK(hipMemcpyAsync(xprev, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL(k_refine, dim3(nb), dim3(kBlock), 0, s, n_, x, err, (const float*)w.x, b, r64);
-        launch_resid(csr(), (const double*)x, r64, grid_, s, lanes_);
+        launch_resid(csr(), (const double*)x, r64, grid_, s, ctx());
         hipLaunchKernelGGL((k_norm2<double>), dim3(grid_), dim3(kBlock), 0, s, n_, (const double*)r64,
                            partials_.p);
         TF_CHECK(hipGetLastError());
@@ -350,7 +350,7 @@ SolveResult System::solve_mixed_reliable(int prec, double* x, const double* b, d
                 launch_cgf_update(n_, x32, r32, z, (const float*)p0, (const float*)p1, (const float*)q,
                                   (const float*)D32, S32, state32_.p, s);
             } else {
-                launch_cgj(true, Aop, n_, x32, r32, z, p0, q, (const float*)D32, S32, (const double*)partE,
+                launch_cgj(true, ctx().strided != 0, Aop, n_, x32, r32, z, p0, q, (const float*)D32, S32, (const double*)partE,
                            state32_.p, mirror32_, s);
             }
         }

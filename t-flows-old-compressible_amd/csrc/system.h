@@ -124,9 +124,13 @@ class System {
     // SpMV form of the solves (lanes per row: 1, 2, 4 or 8)
     int lanes() const { return lanes_; }
     void set_lanes(int lanes);
+    // tile order of the SpMV kernels (0 contiguous per block, 1 strided
+    // inside each XCD's chunk)
+    void set_tile_order(int strided) { strided_ = strided ? 1 : 0; }
     BatchCtx ctx() const {
         BatchCtx c;
         c.lanes = lanes_;
+        c.strided = strided_;
         return c;
     }
 
@@ -171,6 +175,7 @@ class System {
     bool levels_ready_ = false;
     int grid_;
     int lanes_ = 1;
+    int strided_ = 0;
 };
 
 int max_blocks_from_env();

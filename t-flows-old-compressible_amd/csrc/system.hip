@@ -34,6 +34,12 @@ static int lanes_from_env() {
     return (v == 2 || v == 4 || v == 8) ? v : 1;
 }
 
+// Tile order of the SpMV kernels; TFLOWS_TILE_ORDER sets the default.
+static int tile_order_from_env() {
+    const char* e = std::getenv("TFLOWS_TILE_ORDER");
+    return e ? (std::atoi(e) != 0) : 1;
+}
+
 void System::set_lanes(int lanes) {
     if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8)
         throw std::invalid_argument("lanes per row must be 1, 2, 4 or 8");
@@ -49,7 +55,8 @@ int max_blocks_from_env() {
 
 System::System(int n, int nonz, const int* acol, const int* arow, const int* adia, bool device_ptrs,
                hipStream_t s)
-    : n_(n), nnz_(nonz - 1), grid_(stream_grid(n, max_blocks_from_env())), lanes_(lanes_from_env()) {
+    : n_(n), nnz_(nonz - 1), grid_(stream_grid(n, max_blocks_from_env())), lanes_(lanes_from_env()),
+      strided_(tile_order_from_env()) {
     if (n <= 0) throw std::invalid_argument("N must be positive");
     if (nonz < n + 1) throw std::invalid_argument("NONZ must be at least N + 1 (diagonal entries + 1)");
     rowptr_.ensure(n_ + 1);
@@ -208,15 +215,15 @@ void System::prec1(int prec, double* d, hipStream_t s) {
 Sites<double> System::sites() { return Sites<double>::carve(partials_.p, grid_); }
 
 void System::spmv(const double* x, double* y, hipStream_t s) {
-    launch_spmv(csr(), x, y, grid_, s, lanes_);
+    launch_spmv(csr(), x, y, grid_, s, ctx());
     TF_CHECK(hipGetLastError());
 }
 void System::spmv32(const float* x, float* y, hipStream_t s) {
-    launch_spmv(diff32(s), x, y, grid_, s, lanes_);
+    launch_spmv(diff32(s), x, y, grid_, s, ctx());
     TF_CHECK(hipGetLastError());
 }
 void System::resid(const double* x, double* r, hipStream_t s) {
-    launch_resid(csr(), x, r, grid_, s, lanes_);
+    launch_resid(csr(), x, r, grid_, s, ctx());
     TF_CHECK(hipGetLastError());
 }
 
@@ -241,7 +248,7 @@ void enqueue_iteration(const BatchCtx& c, int method, bool jac, int k, const Op&
                 break;
             }
             if (single && jac && cg_form() == 3) {
-                launch_cgj(false, Aop, n, w.x, w.r1, w.u1, w.p, w.q, (const T*)w.D, S, (const double*)nullptr, st,
+                launch_cgj(false, c.strided != 0, Aop, n, w.x, w.r1, w.u1, w.p, w.q, (const T*)w.D, S, (const double*)nullptr, st,
                            hm, s);
                 break;
             }
