@@ -35,10 +35,13 @@ namespace tflows {
 
 namespace {
 constexpr int kMaxGrid = 4096;
-// divergence guard of the FP32 inner solves (their residual starts at norm
-// 1): beyond 1e6 the rounding of FP32 recurrences (~1e-7 relative) exceeds
-// the unit right-hand side and the correction is worthless
+// guards and inner-target back-off of the FP32 inner solves, as in the
+// single-system mixed solve: diverged above 1e6 (unit start), stalled when
+// the smallest residual is older than max(100, its iteration); a failed
+// step is retried with a 10x looser inner target, up to 0.1
 constexpr double kInnerDivLimit = 1e6;
+constexpr int kInnerStallMin = 100;
+constexpr double kLoosestInnerRtol = 0.1;
 int round8(int v) { return (v + kNumXcd - 1) / kNumXcd * kNumXcd; }
 const Levels kNoLevels{};
 }  // namespace
@@ -583,10 +586,11 @@ void Batch::tile_norms(const double* v, std::vector<double>& out) {
 
 // Refinement per system: an inner FP32 solve on the unit-scaled residual,
 // x += ||r|| d, a new FP64 residual.  A step that does not lower the FP64
-// residual (an inner solve that diverged: its recurrence residual left the
-// range FP32 resolves, kInnerDivLimit) is undone and ends that system's
-// refinement, which then reports non-convergence the reference way
-// (niter = iterations + 1, FinRes = ||b - A x||).
+// residual, or whose inner solve diverged (its residual rose above
+// kInnerDivLimit), is undone and retried with a 10x
+// looser inner target; once the target reaches kLoosestInnerRtol a failed
+// step ends that system's refinement, which then reports non-convergence
+// the reference way (niter = iterations + 1, FinRes = ||b - A x||).
 void Batch::solve_mixed(int method, int prec, double* x, const double* b, double* r_out, const int* cap,
                         const double* tol, double inner_rtol, int max_outer, SolveResult* res, int* outer_steps,
                         hipStream_t s) {
@@ -633,14 +637,17 @@ void Batch::solve_mixed(int method, int prec, double* x, const double* b, double
     }
     int outer = 0;
     std::vector<SolverState> init(k_), out;
+    std::vector<double> rtol(k_, inner_rtol);
     while (outer < max_outer && std::any_of(open.begin(), open.end(), [](char c) { return c != 0; })) {
         ++outer;
         for (int i = 0; i < k_; ++i) {
             sc[i] = open[i] ? 1.0 / err[i] : 0.0;   // closed systems: r32 = 0, an immediate exit
             init[i] = SolverState{};
-            init[i].tol = inner_rtol;               // ||r32|| = 1 up to rounding
+            init[i].tol = rtol[i];                  // ||r32|| = 1 up to rounding
             init[i].cap = open[i] ? cap[i] - total[i] : 0;
             init[i].div_limit = kInnerDivLimit;
+            init[i].stall_min = kInnerStallMin;
+            init[i].err_min[0] = HUGE_VAL;
         }
         TF_CHECK(hipMemcpyAsync(scale_.p, sc.data(), sizeof(double) * k_, hipMemcpyHostToDevice, bs_));
         auto prepare = [&] {
@@ -685,7 +692,12 @@ void Batch::solve_mixed(int method, int prec, double* x, const double* b, double
         err = err_new;
         for (int i = 0; i < k_; ++i) {
             if (!open[i]) continue;
-            if (err[i] < tol[i] || its[i] == 0 || total[i] >= cap[i] || reject[i] != 0.0) open[i] = 0;
+            const bool failed = reject[i] != 0.0 || out[i].diverged;
+            if (failed && rtol[i] < kLoosestInnerRtol && total[i] < cap[i]) {
+                rtol[i] = std::min(kLoosestInnerRtol, rtol[i] * 10.0);   // retry the step
+                continue;
+            }
+            if (err[i] < tol[i] || its[i] == 0 || total[i] >= cap[i] || failed) open[i] = 0;
         }
     }
     TF_CHECK(hipMemcpyAsync(x, x64, bytes, hipMemcpyDeviceToDevice, bs_));

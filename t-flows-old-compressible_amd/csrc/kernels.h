@@ -35,10 +35,17 @@ struct SolverState {
     int suspend;
     int replaced_before;
     double err_rep;                 // FP64 residual norm at the last replacement
-    // divergence guard of the FP32 inner solves of the mixed-precision
-    // path (0 = off, the reference behaviour): the solve stops, marked
-    // `diverged`, once ||r|| exceeds div_limit or is not finite.
+    // guards of the FP32 inner solves of the mixed-precision path (0 = off,
+    // the reference behaviour): the solve stops, marked `diverged`, once
+    // ||r|| exceeds div_limit or is not finite, and stops (stalled) once
+    // the smallest ||r|| so far is more than max(stall_min, its iteration)
+    // iterations old.  err_min / k_min (the smallest ||r|| and where) are
+    // kept by iteration parity (read [(k-1)&1], written [k&1]); the host
+    // sets err_min[0] = +inf.
     double div_limit;
+    double err_min[2];
+    int k_min[2];
+    int stall_min;
     int diverged;
 };
 

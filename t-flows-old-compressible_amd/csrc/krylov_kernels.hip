@@ -76,14 +76,25 @@ __device__ __forceinline__ bool check_stop(const BatchCtx& ctx, int k, const T* 
         }
         return true;
     }
-    if (st->div_limit > 0.0 && !(err <= st->div_limit)) {
+    if (st->div_limit > 0.0) {
+        const double prev = st->err_min[(k - 1) & 1];
+        const bool lower = err < prev;
+        const int kmin = lower ? k : st->k_min[(k - 1) & 1];
         if (lead) {
-            st->done = 1;
-            st->diverged = 1;
-            st->niter_out = k - 1;
-            publish(hm, 1, k - 1);
+            st->err_min[k & 1] = lower ? err : prev;
+            st->k_min[k & 1] = kmin;
         }
-        return true;
+        const bool diverged = !(err <= st->div_limit);
+        const int window = st->stall_min > kmin ? st->stall_min : kmin;
+        if (diverged || (st->stall_min > 0 && k - kmin > window)) {
+            if (lead) {
+                st->done = 1;
+                st->diverged = diverged ? 1 : 0;
+                st->niter_out = k - 1;
+                publish(hm, 1, k - 1);
+            }
+            return true;
+        }
     }
     if (k > cap) {
         if (lead) {

@@ -22,6 +22,7 @@
 // pressure-correction matrix is singular up to one pinned cell, and the
 // plain FP32 product loses the near-null mode (rounding of a_ii against
 // sum a_ij exceeds its eigenvalue), which makes the refinement diverge.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -159,11 +160,22 @@ __global__ __launch_bounds__(kBlock) void k_rel_resid(Csr<double> A, const doubl
     }
 }
 
-// Sum of the block partials on the host, in block order.
-// divergence guard of the FP32 inner solves (their residual starts at norm
-// 1): beyond 1e6 the rounding of FP32 recurrences (~1e-7 relative) exceeds
-// the unit right-hand side and the correction is worthless
+// Guards of the FP32 inner solves (their residual starts at norm 1):
+//  * above 1e6 the rounding of FP32 recurrences (~1e-7 of the peak)
+//    exceeds the unit right-hand side: the solve stops as diverged and its
+//    correction is discarded;
+//  * a solve whose smallest residual is older than max(100, the iteration
+//    that reached it) has stalled: it stops and its correction is tried.
+// A failed step (a diverged inner solve, or a correction that does not
+// lower the FP64 residual) is undone and retried with a 10x looser inner
+// target, up to 0.1.  CGS, whose residual polynomial is squared, needs
+// this on transport matrices; a solve that still fails reports
+// non-convergence.
 constexpr double kInnerDivLimit = 1e6;
+constexpr int kInnerStallMin = 100;
+constexpr double kLoosestInnerRtol = 0.1;
+
+// Sum of the block partials on the host, in block order.
 
 static double host_site_sum(const double* d_partials, int grid, hipStream_t s) {
     std::vector<double> h(grid);
@@ -253,14 +265,17 @@ SolveResult System::solve_mixed(int method, int prec, double* x, const double* b
     res.ini_res = err;
     res.ini_written = true;
     int total = 0, outer = 0;
+    double rtol = inner_rtol;
     while (err >= tol && outer < max_outer && total < cap) {
         ++outer;
         hipLaunchKernelGGL(k_scale_to_f32, dim3(nb), dim3(kBlock), 0, s, n_, (const double*)r64, 1.0 / err, w.r1,
                            w.x);
         SolverState h{};
-        h.tol = inner_rtol;            // ||r32|| = 1 up to rounding: relative target
+        h.tol = rtol;                  // ||r32|| = 1 up to rounding: relative target
         h.cap = cap - total;
         h.div_limit = kInnerDivLimit;
+        h.stall_min = kInnerStallMin;
+        h.err_min[0] = HUGE_VAL;
         TF_CHECK(hipMemcpyAsync(state32_.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
         run_solve<float, DiffCsr32>(ctx(), method, prec, cap - total, Aop, A32, w, S32, state32_.p, mirror32_, L, s,
                                     /*d_ready=*/true, /*zero_x=*/true);
@@ -269,20 +284,30 @@ SolveResult System::solve_mixed(int method, int prec, double* x, const double* b
         // iterations actually executed: niter on convergence, cap on exhaustion
         const int done_its = h2.niter_out > h.cap ? h.cap : h2.niter_out;
         total += done_its;
-        if (h2.diverged) break;        // the correction is worthless; x is kept
-        TF_CHECK(hipMemcpyAsync(xprev, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        hipLaunchKernelGGL(k_refine, dim3(nb), dim3(kBlock), 0, s, n_, x, err, (const float*)w.x, b, r64);
-        launch_resid(csr(), (const double*)x, r64, grid_, s, ctx());
-        hipLaunchKernelGGL((k_norm2<double>), dim3(grid_), dim3(kBlock), 0, s, n_, (const double*)r64,
-                           partials_.p);
-        TF_CHECK(hipGetLastError());
-        const double err_new = std::sqrt(host_site_sum(partials_.p, grid_, s));
-        if (!(err_new < err)) {        // a step that does not help is undone
-            TF_CHECK(hipMemcpyAsync(x, xprev, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            err = true_residual(x, b, r64, s);
+        bool failed = h2.diverged != 0;   // a diverged correction is not applied
+        if (!failed) {
+            TF_CHECK(hipMemcpyAsync(xprev, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_refine, dim3(nb), dim3(kBlock), 0, s, n_, x, err, (const float*)w.x, b, r64);
+            launch_resid(csr(), (const double*)x, r64, grid_, s, ctx());
+            hipLaunchKernelGGL((k_norm2<double>), dim3(grid_), dim3(kBlock), 0, s, n_, (const double*)r64,
+                               partials_.p);
+            TF_CHECK(hipGetLastError());
+            const double err_new = std::sqrt(host_site_sum(partials_.p, grid_, s));
+            if (!(err_new < err)) {    // a step that does not help is undone
+                TF_CHECK(hipMemcpyAsync(x, xprev, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+                err = true_residual(x, b, r64, s);
+                failed = true;
+            } else {
+                err = err_new;
+            }
+        }
+        if (failed) {
+            if (rtol < kLoosestInnerRtol) {
+                rtol = std::min(kLoosestInnerRtol, rtol * 10.0);
+                continue;
+            }
             break;
         }
-        err = err_new;
         if (done_its == 0) break;      // the inner solve could not make progress
     }
     if (outer_steps) *outer_steps = outer;

@@ -485,6 +485,31 @@ def pressure_system(n: int | tuple, seed: int = 1, ratio: float = 1.1, prec: int
                              generator_version=GENERATOR_VERSION))
 
 
+def wall_pressure_system(n: int | tuple, seed: int = 5, ratio: float = 1.05, max_aspect: float = 50.0,
+                         prec: int = 2, niter: int = 20000, rel_tol: float = 1e-8) -> LinearSystem:
+    """Channel pressure-correction system with strong wall clustering.
+
+    2*pi x 2 x pi box; y-nodes grow geometrically by ``ratio`` from each wall
+    (geometric_wall_nodes) with a first cell of dx / ``max_aspect``, so the
+    wall cells reach an aspect ratio of about ``max_aspect``:1 against the
+    streamwise spacing dx = 2*pi/nx.  Cell 1 pinned, b ~ U(-1,1) with its
+    mean removed, DIC preconditioning by default.
+    """
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+    dx = 2.0 * np.pi / nx
+    mesh = BoxMesh(np.linspace(0.0, 2.0 * np.pi, nx + 1),
+                   geometric_wall_nodes(ny, 2.0, ratio, max_aspect, dx),
+                   np.linspace(0.0, np.pi, nz + 1))
+    m = pressure_matrix(mesh, pin_cell=1)
+    b = seeded_rhs(m.n, seed, True)
+    tol = rel_tol * float(np.sqrt(np.dot(b, b)))
+    dy = np.diff(mesh.yn)
+    return LinearSystem(m, b, np.zeros(m.n), tol, prec, niter,
+                        dict(recipe="wall_pressure", shape=(nx, ny, nz), seed=seed, ratio=ratio,
+                             max_aspect=float(dx / dy.min()), rel_tol=rel_tol,
+                             generator_version=GENERATOR_VERSION))
+
+
 def momentum_system(n: int | tuple, seed: int = 2, gamma: float = 1e-3, prec: int = 1,
                     niter: int = 2000, rel_tol: float = 1e-8, renumber: bool = True,
                     flux_seed: int | None = None) -> LinearSystem:
