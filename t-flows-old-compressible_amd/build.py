@@ -24,7 +24,7 @@ FORTRAN = os.path.join(PKG, "fortran")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 ARCH = "gfx950"
-SOURCES = ["krylov_kernels.hip", "cg_fused.hip", "system.hip", "capi.hip", "mixed.hip", "batched.hip",
+SOURCES = ["krylov_kernels.hip", "cg_fused.hip", "system.hip", "reorder.hip", "capi.hip", "mixed.hip", "batched.hip",
            "adjoint.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

@@ -67,19 +67,24 @@ System& System::transposed(hipStream_t s) {
         transposed_valid_ = false;
     }
     if (!transposed_valid_) {
+        // A^T values in the caller's entry order, then stored the way the
+        // transposed system keeps its own values
         const int64_t nnz = nnz_;
+        tvals_.ensure(nnz_);
         hipLaunchKernelGGL(k_mirror_gather, dim3((unsigned)((nnz + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nnz,
-                           mir, (const double*)val_.p, transposed_->val_.p);
+                           mir, values_original(), tvals_.p);
         TF_CHECK(hipGetLastError());
-        transposed_->val32_valid_ = false;
-        transposed_->diff32_valid_ = false;
+        transposed_->set_values(tvals_.p, true, s);
         transposed_valid_ = true;
     }
     return *transposed_;
 }
 
 void System::outer_grad(const double* lambda, const double* x, double* grad, hipStream_t s) {
-    hipLaunchKernelGGL(k_outer_grad, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n_, rowptr_.p, col_.p,
+    // over the caller's pattern and numbering (lambda, x, grad all in it)
+    const int* rp = reordered_ ? orowptr_.p : rowptr_.p;
+    const int* cl = reordered_ ? ocol_.p : col_.p;
+    hipLaunchKernelGGL(k_outer_grad, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n_, rp, cl,
                        lambda, x, grad);
     TF_CHECK(hipMemsetAsync(grad + nnz_, 0, sizeof(double), s));   // the NONZ-th slot
     TF_CHECK(hipGetLastError());

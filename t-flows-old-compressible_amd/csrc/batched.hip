@@ -182,7 +182,8 @@ Batch::Batch(int k, const int* n, const int* nonz, const int* const* acol, const
     }
     cacol[n_tot_] = (int)nnz_tot_ + 1;
     TF_CHECK(hipStreamCreateWithFlags(&bs_, hipStreamNonBlocking));
-    sys_.reset(new System(n_tot_, (int)nnz_tot_ + 1, cacol.data(), carow.data(), cadia.data(), false, bs_));
+    sys_.reset(new System(n_tot_, (int)nnz_tot_ + 1, cacol.data(), carow.data(), cadia.data(), false, bs_,
+                          /*allow_reorder=*/false));
 
     tiles_.assign(k, 0);
     tile0_.assign(k + 1, 0);

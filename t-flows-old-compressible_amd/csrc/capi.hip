@@ -231,7 +231,7 @@ int tflows_outer_grad(tflows_system s, const double* lambda, const double* x, do
 int tflows_transpose_values(tflows_system s, double* aval_t, void* stream) {
     return guarded([&] {
         System& T = s->sys->transposed(as_stream(stream));
-        TF_CHECK(hipMemcpyAsync(aval_t, T.values(), sizeof(double) * T.nnz(), hipMemcpyDeviceToDevice,
+        TF_CHECK(hipMemcpyAsync(aval_t, T.values_original(), sizeof(double) * T.nnz(), hipMemcpyDeviceToDevice,
                                 as_stream(stream)));
     });
 }

@@ -218,6 +218,20 @@ double System::true_residual(const double* x, const double* b, double* r, hipStr
 
 SolveResult System::solve_mixed(int method, int prec, double* x, const double* b, double* r_out, int cap,
                                 double tol, double inner_rtol, int max_outer, int* outer_steps, hipStream_t s) {
+    if (!reordered_) return solve_mixed_impl(method, prec, x, b, r_out, cap, tol, inner_rtol, max_outer, outer_steps, s);
+    double *xi = ext_buffer(0), *bi = ext_buffer(1), *ri = ext_buffer(2);
+    to_internal((const double*)x, xi, s);
+    to_internal(b, bi, s);
+    const SolveResult res = solve_mixed_impl(method, prec, xi, bi, ri, cap, tol, inner_rtol, max_outer, outer_steps, s);
+    to_external((const double*)xi, x, s);
+    if (r_out) to_external((const double*)ri, r_out, s);
+    TF_CHECK(hipStreamSynchronize(s));
+    return res;
+}
+
+SolveResult System::solve_mixed_impl(int method, int prec, double* x, const double* b, double* r_out, int cap,
+                                     double tol, double inner_rtol, int max_outer, int* outer_steps,
+                                     hipStream_t s) {
     if (method < 0 || method > 2) throw std::invalid_argument("method must be 0 (cg), 1 (bicg) or 2 (cgs)");
     if (!(inner_rtol > 0.0 && inner_rtol < 1.0)) throw std::invalid_argument("inner_rtol must be in (0, 1)");
     const size_t n = (size_t)n_;
@@ -326,6 +340,20 @@ SolveResult System::solve_mixed(int method, int prec, double* x, const double* b
 // refinement step, so the iteration count stays close to the FP64 CG's.
 SolveResult System::solve_mixed_reliable(int prec, double* x, const double* b, double* r_out, int cap, double tol,
                                          int interval, int* outer_steps, hipStream_t s) {
+    if (!reordered_) return solve_mixed_reliable_impl(prec, x, b, r_out, cap, tol, interval, outer_steps, s);
+    double *xi = ext_buffer(0), *bi = ext_buffer(1), *ri = ext_buffer(2);
+    to_internal((const double*)x, xi, s);
+    to_internal(b, bi, s);
+    const SolveResult res = solve_mixed_reliable_impl(prec, xi, bi, r_out ? ri : nullptr, cap, tol, interval,
+                                                      outer_steps, s);
+    to_external((const double*)xi, x, s);
+    if (r_out) to_external((const double*)ri, r_out, s);
+    TF_CHECK(hipStreamSynchronize(s));
+    return res;
+}
+
+SolveResult System::solve_mixed_reliable_impl(int prec, double* x, const double* b, double* r_out, int cap,
+                                              double tol, int interval, int* outer_steps, hipStream_t s) {
     if (prec == 2) throw std::invalid_argument("reliable-update mode supports prec 0 and 1");
     if (interval < 1) interval = 1;
     const size_t n = (size_t)n_;

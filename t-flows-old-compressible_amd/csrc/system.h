@@ -70,9 +70,16 @@ struct Levels {
     DevArray<int> fwd_rows, bwd_rows;    // device row lists, level by level
 };
 
+// Every vector argument of the public solve / product entry points is in
+// the caller's numbering.  The matrix itself may be stored permuted for
+// locality (reorder.hip): then the pattern arrays returned by csr() /
+// rowptr() / col() / dia() and every workspace vector are in the internal
+// order, and the entry points gather their inputs in and scatter their
+// outputs back.
 class System {
    public:
-    System(int n, int nonz, const int* acol, const int* arow, const int* adia, bool device_ptrs, hipStream_t s);
+    System(int n, int nonz, const int* acol, const int* arow, const int* adia, bool device_ptrs, hipStream_t s,
+           bool allow_reorder = true);
     ~System();
 
     int n() const { return n_; }
@@ -81,7 +88,14 @@ class System {
     void set_values(const double* aval, bool device_ptr, hipStream_t s);
     // values of entries [offset, offset + count) only (0-based)
     void set_values_range(int64_t offset, int64_t count, const double* aval, bool device_ptr, hipStream_t s);
-    const double* values() const { return val_.p; }
+    // values in the caller's entry order
+    const double* values_original() const { return reordered_ ? val_orig_.p : val_.p; }
+    bool reordered() const { return reordered_; }
+    // caller numbering <-> internal numbering of an N-vector
+    template <typename T>
+    void to_internal(const T* ext, T* in, hipStream_t s) const;
+    template <typename T>
+    void to_external(const T* in, T* ext, hipStream_t s) const;
     const float* values32(hipStream_t s);   // FP32 copy, made on first use
 
     Csr<double> csr() const { return Csr<double>{n_, rowptr_.p, col_.p, dia_.p, val_.p}; }
@@ -104,7 +118,7 @@ class System {
     // FP32 CG iterations); CG with prec 0 / 1
     SolveResult solve_mixed_reliable(int prec, double* x, const double* b, double* r_out, int cap, double tol,
                                      int interval, int* outer_steps, hipStream_t s);
-    // r = b - A x (FP64) and its norm
+    // r = b - A x (FP64) and its norm; internal numbering
     double true_residual(const double* x, const double* b, double* r, hipStream_t s);
 
     // adjoint support (adjoint.hip): mirror positions of the structurally
@@ -148,6 +162,15 @@ class System {
     void dic_apply(const Csr<T>& A, const T* D, const T* u, T* z, const SolverState* st, hipStream_t s);
 
    private:
+    SolveResult solve_impl(int method, int prec, double* x, double* r1, int cap, double tol, double stale_error,
+                           hipStream_t s);
+    SolveResult solve_mixed_impl(int method, int prec, double* x, const double* b, double* r_out, int cap,
+                                 double tol, double inner_rtol, int max_outer, int* outer_steps, hipStream_t s);
+    SolveResult solve_mixed_reliable_impl(int prec, double* x, const double* b, double* r_out, int cap, double tol,
+                                          int interval, int* outer_steps, hipStream_t s);
+    void build_storage(bool allow_reorder, hipStream_t s);
+    void permute_values(hipStream_t s);
+    double* ext_buffer(int k);   // k-th N-vector of the gather / scatter buffer
 
     int n_, nnz_;
     DevArray<int> rowptr_, col_, dia_;
@@ -176,6 +199,14 @@ class System {
     int grid_;
     int lanes_ = 1;
     int strided_ = 0;
+    // locality reordering (reorder.hip): perm_ new -> old rows, eperm_
+    // new -> old entries, the original pattern for the adjoint outer
+    // product, the values in the caller's order
+    bool reordered_ = false;
+    std::vector<int> iperm_h_;
+    DevArray<int> perm_, eperm_, orowptr_, ocol_;
+    DevArray<double> val_orig_, extbuf_, tvals_;
+    DevArray<float> extbuf32_;
 };
 
 int max_blocks_from_env();

@@ -54,7 +54,7 @@ int max_blocks_from_env() {
 
 
 System::System(int n, int nonz, const int* acol, const int* arow, const int* adia, bool device_ptrs,
-               hipStream_t s)
+               hipStream_t s, bool allow_reorder)
     : n_(n), nnz_(nonz - 1), grid_(stream_grid(n, max_blocks_from_env())), lanes_(lanes_from_env()),
       strided_(tile_order_from_env()) {
     if (n <= 0) throw std::invalid_argument("N must be positive");
@@ -92,9 +92,7 @@ System::System(int n, int nonz, const int* acol, const int* arow, const int* adi
     }
     for (int i = 0; i < n_; ++i)
         if (h_col_[h_dia_[i]] != i) throw std::invalid_argument("Arow(Adia(i)) must equal i");
-    TF_CHECK(hipMemcpyAsync(rowptr_.p, h_rowptr_.data(), sizeof(int) * (n_ + 1), hipMemcpyHostToDevice, s));
-    TF_CHECK(hipMemcpyAsync(col_.p, h_col_.data(), sizeof(int) * nnz_, hipMemcpyHostToDevice, s));
-    TF_CHECK(hipMemcpyAsync(dia_.p, h_dia_.data(), sizeof(int) * n_, hipMemcpyHostToDevice, s));
+    build_storage(allow_reorder, s);
     val_.ensure(nnz_);
     TF_CHECK(hipMemsetAsync(val_.p, 0, sizeof(double) * nnz_, s));
     partials_.ensure(Sites<double>::elems(grid_) + 2 * (size_t)grid_);  // + site E
@@ -112,8 +110,9 @@ System::~System() {
 }
 
 void System::set_values(const double* aval, bool device_ptr, hipStream_t s) {
-    TF_CHECK(hipMemcpyAsync(val_.p, aval, sizeof(double) * nnz_,
+    TF_CHECK(hipMemcpyAsync(reordered_ ? val_orig_.p : val_.p, aval, sizeof(double) * nnz_,
                             device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    if (reordered_) permute_values(s);
     val32_valid_ = false;
     diff32_valid_ = false;
     transposed_valid_ = false;
@@ -123,6 +122,7 @@ void System::set_values(const double* aval, bool device_ptr, hipStream_t s) {
 void System::set_values_range(int64_t offset, int64_t count, const double* aval, bool device_ptr,
                               hipStream_t s) {
     if (offset < 0 || count < 0 || offset + count > nnz_) throw std::invalid_argument("value range out of bounds");
+    if (reordered_) throw std::logic_error("partial value updates need the caller's storage order");
     TF_CHECK(hipMemcpyAsync(val_.p + offset, aval, sizeof(double) * count,
                             device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     val32_valid_ = false;
@@ -153,7 +153,9 @@ const Levels& System::levels() {
         for (int i = 0; i < n_; ++i) ptr[lev[i] + 1]++;
         for (int l = 0; l < nlev; ++l) ptr[l + 1] += ptr[l];
         std::vector<int> out(n_), fill(ptr.begin(), ptr.end() - 1);
-        for (int i = 0; i < n_; ++i) out[fill[lev[i]]++] = i;
+        // levels are found on the caller's pattern; the lists hold the rows
+        // in the internal numbering the kernels use
+        for (int i = 0; i < n_; ++i) out[fill[lev[i]]++] = reordered_ ? iperm_h_[i] : i;
         rows.ensure(n_);
         TF_CHECK(hipMemcpy(rows.p, out.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
     };
@@ -202,29 +204,63 @@ void System::dic_apply(const Csr<T>& A, const T* D, const T* u, T* z, const Solv
     dic_apply_levels(levels(), A, D, u, z, st, s);
 }
 
-void System::prec1(int prec, double* d, hipStream_t s) {
+void System::prec1(int prec, double* d_ext, hipStream_t s) {
     // D only; the ||b|| partial sums of k_setup run over a zero slot of the
     // workspace and are not used.
     ws_.ensure((size_t)V_COUNT * n_);
     TF_CHECK(hipMemsetAsync(ws_.p, 0, sizeof(double) * n_, s));
+    double* d = reordered_ ? ext_buffer(0) : d_ext;
     if (prec == 2) dic_setup(d, s);
     launch_setup(ctx(), csr(), prec, prec != 2, d, (const double*)ws_.p, sites(), s);
     TF_CHECK(hipGetLastError());
+    if (reordered_) to_external(d, d_ext, s);
+}
+
+double* System::ext_buffer(int k) {
+    extbuf_.ensure(3 * (size_t)n_);
+    return extbuf_.p + (size_t)k * n_;
 }
 
 Sites<double> System::sites() { return Sites<double>::carve(partials_.p, grid_); }
 
 void System::spmv(const double* x, double* y, hipStream_t s) {
-    launch_spmv(csr(), x, y, grid_, s, ctx());
+    if (!reordered_) {
+        launch_spmv(csr(), x, y, grid_, s, ctx());
+        TF_CHECK(hipGetLastError());
+        return;
+    }
+    double *xi = ext_buffer(0), *yi = ext_buffer(1);
+    to_internal(x, xi, s);
+    launch_spmv(csr(), (const double*)xi, yi, grid_, s, ctx());
     TF_CHECK(hipGetLastError());
+    to_external((const double*)yi, y, s);
 }
 void System::spmv32(const float* x, float* y, hipStream_t s) {
-    launch_spmv(diff32(s), x, y, grid_, s, ctx());
+    const DiffCsr32 A = diff32(s);
+    if (!reordered_) {
+        launch_spmv(A, x, y, grid_, s, ctx());
+        TF_CHECK(hipGetLastError());
+        return;
+    }
+    extbuf32_.ensure(2 * (size_t)n_);
+    float *xi = extbuf32_.p, *yi = extbuf32_.p + n_;
+    to_internal(x, xi, s);
+    launch_spmv(A, (const float*)xi, yi, grid_, s, ctx());
     TF_CHECK(hipGetLastError());
+    to_external((const float*)yi, y, s);
 }
 void System::resid(const double* x, double* r, hipStream_t s) {
-    launch_resid(csr(), x, r, grid_, s, ctx());
+    if (!reordered_) {
+        launch_resid(csr(), x, r, grid_, s, ctx());
+        TF_CHECK(hipGetLastError());
+        return;
+    }
+    double *xi = ext_buffer(0), *ri = ext_buffer(1);
+    to_internal(x, xi, s);
+    to_internal((const double*)r, ri, s);
+    launch_resid(csr(), (const double*)xi, ri, grid_, s, ctx());
     TF_CHECK(hipGetLastError());
+    to_external((const double*)ri, r, s);
 }
 
 // One iteration k of `method`, enqueued (CG.f90:113-259, BiCG.f90:100-251,
@@ -376,6 +412,19 @@ void run_solve(const BatchCtx& c, int method, int prec, int cap, const Op& Aop, 
 
 SolveResult System::solve(int method, int prec, double* x, double* r1, int cap, double tol, double stale_error,
                           hipStream_t s) {
+    if (!reordered_) return solve_impl(method, prec, x, r1, cap, tol, stale_error, s);
+    double *xi = ext_buffer(0), *ri = ext_buffer(1);
+    to_internal((const double*)x, xi, s);
+    to_internal((const double*)r1, ri, s);
+    const SolveResult res = solve_impl(method, prec, xi, ri, cap, tol, stale_error, s);
+    to_external((const double*)xi, x, s);
+    to_external((const double*)ri, r1, s);
+    TF_CHECK(hipStreamSynchronize(s));
+    return res;
+}
+
+SolveResult System::solve_impl(int method, int prec, double* x, double* r1, int cap, double tol,
+                               double stale_error, hipStream_t s) {
     if (method < 0 || method > 2) throw std::invalid_argument("method must be 0 (cg), 1 (bicg) or 2 (cgs)");
     ws_.ensure((size_t)V_COUNT * n_);
     KrylovVectors<double> w = vectors(x, r1);

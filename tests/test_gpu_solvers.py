@@ -176,7 +176,12 @@ def test_initial_guess_and_buffer_offset(pkg, oracle, cases):
         it, ini, fin = getattr(pkg.krylov, method)(m.n, nb, m.nonz, m.aval, m.acol, m.arow, m.adia,
                                                    np.zeros(nb), x, r1, 1, 500, s.tol)
         assert np.all(x[: nb + 1] == 7.0)
-        assert abs(it - ref["niter"]) <= 1 and _rel(x[nb + 1:], ref["x"]) <= X_TOL
+        assert abs(it - ref["niter"]) <= 1
+        # the parity bar of test_dropin_matches_oracle: 1e-10, or the
+        # conditioning bound where the tolerance admits a wider set of x
+        bound = _inv_norm2(m) * (np.linalg.norm(r1) + np.linalg.norm(ref["r"]))
+        assert (_rel(x[nb + 1:], ref["x"]) <= X_TOL or
+                np.linalg.norm(x[nb + 1:] - ref["x"]) <= 1.01 * bound)
 
 
 def test_device_api_matches_dropin(pkg, cases, cuda):
