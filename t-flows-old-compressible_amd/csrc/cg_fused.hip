@@ -228,6 +228,20 @@ __global__ __launch_bounds__(kBlock) void k_cgj_p(const T* __restrict__ z, T* __
                                                   int n, SolverState* st, HostMirror* hm) {
     if (st->done || st->suspend) return;
     const int kd = st->it + 1;
+    const int ntiles = (n + kTileRows - 1) / kTileRows;
+    int t0, t1;
+    block_tiles(ntiles, t0, t1);
+    const int rb = t0 * kTileRows, re = min(t1 * kTileRows, n);
+    // the first group of rows is loaded before the grid reduction of the
+    // stopping test, so its memory latency overlaps the reduction's
+    int i0 = rb + (int)threadIdx.x;
+    T zv[kUnroll], pv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const int i = i0 + u * kBlock;
+        zv[u] = i < re ? z[i] : T(0);
+        pv[u] = i < re ? p[i] : T(0);
+    }
     T rho;
     if (fused_check<T, MIXED>(kd, partB, partE, st, hm, rho)) return;
     if (leader()) {
@@ -236,22 +250,19 @@ __global__ __launch_bounds__(kBlock) void k_cgj_p(const T* __restrict__ z, T* __
     }
     const bool first = kd == 1;
     const T beta = first ? T(0) : rho / (T)st->rho[(kd - 1) & 1];   // CG.f90:183
-    const int ntiles = (n + kTileRows - 1) / kTileRows;
-    int t0, t1;
-    block_tiles(ntiles, t0, t1);
-    const int rb = t0 * kTileRows, re = min(t1 * kTileRows, n);
-    for (int i0 = rb + threadIdx.x; i0 < re; i0 += kUnroll * kBlock) {
-        T zv[kUnroll], pv[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            const int i = i0 + u * kBlock;
-            zv[u] = i < re ? z[i] : T(0);
-            pv[u] = (i < re && !first) ? p[i] : T(0);
-        }
+    for (;;) {
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             const int i = i0 + u * kBlock;
             if (i < re) p[i] = first ? zv[u] : zv[u] + beta * pv[u];   // CG.f90:180, :185
+        }
+        i0 += kUnroll * kBlock;
+        if (i0 >= re) break;
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const int i = i0 + u * kBlock;
+            zv[u] = i < re ? z[i] : T(0);
+            pv[u] = i < re ? p[i] : T(0);
         }
     }
 }
@@ -264,16 +275,15 @@ __global__ __launch_bounds__(kBlock) void k_cgj_update(int n, T* __restrict__ x,
                                                        SolverState* st) {
     if (st->done || st->suspend) return;
     const int kd = st->kcur;
-    T pq[1];
-    site_total<T, 1>(partD, pq);
-    const T alfa = (T)st->rho[kd & 1] / pq[0];          // CG.f90:219
     const int ntiles = (n + kTileRows - 1) / kTileRows;
     int t0, t1;
     block_tiles(ntiles, t0, t1);
     const int rb = t0 * kTileRows, re = min(t1 * kTileRows, n);
     T v[2] = {T(0), T(0)};
-    for (int i0 = rb + threadIdx.x; i0 < re; i0 += kUnroll * kBlock) {
-        T xv[kUnroll], pv[kUnroll], rv[kUnroll], qv[kUnroll], dv[kUnroll];
+    T xv[kUnroll], pv[kUnroll], rv[kUnroll], qv[kUnroll], dv[kUnroll];
+    // loads of the next group of rows; the first is issued before the grid
+    // reduction of (p,q), so its latency overlaps the reduction's
+    auto load = [&](int i0) {
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             const int i = i0 + u * kBlock;
@@ -284,6 +294,13 @@ __global__ __launch_bounds__(kBlock) void k_cgj_update(int n, T* __restrict__ x,
             qv[u] = ok ? q[i] : T(0);
             dv[u] = ok ? D[i] : T(1);
         }
+    };
+    load(rb + (int)threadIdx.x);
+    T pq[1];
+    site_total<T, 1>(partD, pq);
+    const T alfa = (T)st->rho[kd & 1] / pq[0];          // CG.f90:219
+    for (int i0 = rb + (int)threadIdx.x; i0 < re; i0 += kUnroll * kBlock) {
+        if (i0 != rb + (int)threadIdx.x) load(i0);
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             const int i = i0 + u * kBlock;

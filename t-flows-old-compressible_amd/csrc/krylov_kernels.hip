@@ -40,6 +40,10 @@ namespace tflows {
 template <typename T>
 __device__ __forceinline__ T jac(T r, T d) { return r / d; }
 
+// rows per thread whose operands the element-wise kernels load before they
+// wait on a grid reduction
+constexpr int kPrefetch = 4;
+
 #define TF_STATE                                              \
     SolverState* st = st_base + ctx.sys();                    \
     HostMirror* hm = hm_base ? hm_base + ctx.sys() : nullptr; \
@@ -326,13 +330,32 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(BatchCtx ctx, int k_arg, int n,
     TF_STATE;
     if (st->done) return;
     const int k = ctx.k(k_arg);
+    int rb, re;
+    ctx_rows(ctx, n, rb, re);
+    // the first kPrefetch rows of this thread are loaded before the grid
+    // reduction, so their memory latency overlaps its latency
+    T av[kPrefetch], dv[kPrefetch], pv[kPrefetch];
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+        const int i = rb + (int)threadIdx.x + u * kBlock;
+        const bool ok = i < re;
+        av[u] = ok ? (JAC ? r[i] : z[i]) : T(0);
+        dv[u] = (JAC && ok) ? D[i] : T(1);
+        pv[u] = ok ? p[i] : T(0);
+    }
     T rho;
     if (!iteration_rho<T, JAC>(ctx, k, partB, partC, st, hm, rho)) return;
     const bool first = k == 1;
     const T beta = first ? T(0) : rho / (T)st->rho[(k - 1) & 1];
-    int rb, re;
-    ctx_rows(ctx, n, rb, re);
-    for (int i = rb + threadIdx.x; i < re; i += kBlock) {
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+        const int i = rb + (int)threadIdx.x + u * kBlock;
+        if (i < re) {
+            const T zi = JAC ? jac(av[u], dv[u]) : av[u];
+            p[i] = first ? zi : zi + beta * pv[u];
+        }
+    }
+    for (int i = rb + (int)threadIdx.x + kPrefetch * kBlock; i < re; i += kBlock) {
         const T zi = JAC ? jac(r[i], D[i]) : z[i];
         p[i] = first ? zi : zi + beta * p[i];
     }
@@ -377,13 +400,35 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(BatchCtx ctx, int k_arg, i
     const SolverState* st = st_base + ctx.sys();
     TF_DONE_RETURN(partB, 2);
     const int k = ctx.k(k_arg);
+    int rb, re;
+    ctx_rows(ctx, n, rb, re);
+    T xv[kPrefetch], pv[kPrefetch], rv[kPrefetch], qv[kPrefetch], dv[kPrefetch];
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+        const int i = rb + (int)threadIdx.x + u * kBlock;
+        const bool ok = i < re;
+        xv[u] = ok ? x[i] : T(0);
+        pv[u] = ok ? p[i] : T(0);
+        rv[u] = ok ? r[i] : T(0);
+        qv[u] = ok ? q[i] : T(0);
+        dv[u] = (JAC && ok) ? D[i] : T(1);
+    }
     T pq[1];
     ctx_total<T, 1>(ctx, partD, pq);
     const T alfa = (T)st->rho[k & 1] / pq[0];
-    int rb, re;
-    ctx_rows(ctx, n, rb, re);
     T v[2] = {T(0), T(0)};
-    for (int i = rb + threadIdx.x; i < re; i += kBlock) {
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+        const int i = rb + (int)threadIdx.x + u * kBlock;
+        if (i < re) {
+            x[i] = xv[u] + alfa * pv[u];
+            const T ri = rv[u] - alfa * qv[u];
+            r[i] = ri;
+            v[0] += ri * ri;
+            if (JAC) v[1] += ri * jac(ri, dv[u]);
+        }
+    }
+    for (int i = rb + (int)threadIdx.x + kPrefetch * kBlock; i < re; i += kBlock) {
         x[i] = x[i] + alfa * p[i];
         const T ri = r[i] - alfa * q[i];
         r[i] = ri;
