@@ -46,12 +46,6 @@ int round8(int v) { return (v + kNumXcd - 1) / kNumXcd * kNumXcd; }
 const Levels kNoLevels{};
 }  // namespace
 
-__global__ void k_set_int(int* p, int v) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
-}
-__global__ void k_advance(int* kbase, int c) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *kbase += c;
-}
 
 // Site B of each running system summed under the current maps into
 // st->aux[0..1] (by its leader).
@@ -347,7 +341,7 @@ hipGraphExec_t Batch::graph(const GraphKey& gk, const Op& Aop, const Csr<T>& A, 
     try {
         for (int i = 1; i <= gk.chunk; ++i)
             enqueue_iteration<T, Op>(c, gk.method, gk.prec != 2, i, Aop, A, w, S, st, hm, L, bs_);
-        hipLaunchKernelGGL(k_advance, dim3(1), dim3(kWave), 0, bs_, kbase_.p, gk.chunk);
+        launch_advance(kbase_.p, gk.chunk, bs_);
     } catch (...) {
         (void)hipStreamEndCapture(bs_, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -450,7 +444,7 @@ void Batch::run(int bits, int method, int prec, const Op& Aop, const Csr<T>& A, 
     }
     const BatchCtx c0 = ctx(lanes, false);
     start_solve<T, Op>(c0, method, prec, Aop, A, w, Sites<T>::carve(partials, G), st, hm, L, bs_, d_ready, zero_x);
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(kWave), 0, bs_, kbase_.p, 0);
+    launch_set_int(kbase_.p, 0, bs_);
     TF_CHECK(hipGetLastError());
     (void)jac;
 

@@ -99,6 +99,8 @@ void launch_cgf_init_z(int n, const T* r, const T* D, T* z, const Sites<T>& S, c
                        hipStream_t s);
 
 void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t s);
+void launch_set_int(int* p, int v, hipStream_t s);
+void launch_advance(int* kbase, int c, hipStream_t s);
 
 // Number of blocks of the row-streaming kernels for n rows: one block per
 // 256-row tile up to 8 blocks per CU (2048 on MI355X), rounded to a

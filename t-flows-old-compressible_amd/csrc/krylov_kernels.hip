@@ -676,6 +676,16 @@ __global__ __launch_bounds__(kBlock) void k_resid(BatchCtx ctx, Csr<T> A, const 
         resid_tile<L>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi);
 }
 
+// Device iteration base of replayed graphs: set, and advanced by a chunk
+// (the last node of each captured chunk).  Kernels, not memsets, so that
+// they can sit inside a graph.
+__global__ void k_set_int(int* p, int v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+__global__ void k_advance(int* kbase, int c) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *kbase += c;
+}
+
 template <typename TO, typename TI>
 __global__ void k_convert(int64_t count, const TI* __restrict__ in, TO* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -833,6 +843,13 @@ void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s, co
         TF_LAUNCH((k_resid<T, L>), grid, s), c, A, x, r);
     });
 }
+void launch_set_int(int* p, int v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(kWave), 0, s, p, v);
+}
+void launch_advance(int* kbase, int c, hipStream_t s) {
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(kWave), 0, s, kbase, c);
+}
+
 void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t s) {
     if (count > 0)
         hipLaunchKernelGGL((k_convert<float, double>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,

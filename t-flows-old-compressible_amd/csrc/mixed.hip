@@ -292,7 +292,7 @@ SolveResult System::solve_mixed_impl(int method, int prec, double* x, const doub
         h.err_min[0] = HUGE_VAL;
         TF_CHECK(hipMemcpyAsync(state32_.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
         run_solve<float, DiffCsr32>(ctx(), method, prec, cap - total, Aop, A32, w, S32, state32_.p, mirror32_, L, s,
-                                    /*d_ready=*/true, /*zero_x=*/true);
+                                    /*d_ready=*/true, /*zero_x=*/true, &replay_);
         SolverState h2;
         TF_CHECK(hipMemcpy(&h2, state32_.p, sizeof(h2), hipMemcpyDeviceToHost));
         // iterations actually executed: niter on convergence, cap on exhaustion

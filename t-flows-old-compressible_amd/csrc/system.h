@@ -76,6 +76,30 @@ struct Levels {
 // rowptr() / col() / dia() and every workspace vector are in the internal
 // order, and the entry points gather their inputs in and scatter their
 // outputs back.
+// Graph replay of one system's DIC iterations.  A DIC iteration is one
+// launch per level per sweep (some 1500 dependent launches at 256^3), so
+// the host, not the GPU, limits the rate at which they are issued; one
+// iteration is captured into a HIP graph on a stream of the replayer's
+// own (the caller's may be the legacy stream, which cannot be captured)
+// and replayed with its iteration index read from a device counter.
+// Graphs are kept per set of buffers they were captured with.
+struct Replay {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[3] = {};
+    DevArray<int> kbase;
+    struct Entry {
+        std::string key;
+        hipGraphExec_t exec;
+    };
+    std::vector<Entry> cache;
+    Replay() = default;
+    Replay(const Replay&) = delete;
+    Replay& operator=(const Replay&) = delete;
+    ~Replay();
+    void ensure();
+};
+bool dic_graph_enabled();
+
 class System {
    public:
     System(int n, int nonz, const int* acol, const int* arow, const int* adia, bool device_ptrs, hipStream_t s,
@@ -199,6 +223,7 @@ class System {
     int grid_;
     int lanes_ = 1;
     int strided_ = 0;
+    Replay replay_;
     // locality reordering (reorder.hip): perm_ new -> old rows, eperm_
     // new -> old entries, the original pattern for the adjoint outer
     // product, the values in the caller's order
@@ -219,11 +244,11 @@ void start_solve(const BatchCtx& c, int method, int prec, const Op& Aop, const C
 template <typename T, typename Op>
 void run_solve(const BatchCtx& c, int method, int prec, int cap, const Op& Aop, const Csr<T>& A,
                const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L,
-               hipStream_t s, bool d_ready, bool zero_x);
+               hipStream_t s, bool d_ready, bool zero_x, Replay* rp = nullptr);
 template <typename T, typename Op>
 void pace_iterations(const BatchCtx& c, int method, bool jac, int cap, const Op& Aop, const Csr<T>& A,
                      const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm,
-                     const Levels* L, hipStream_t s);
+                     const Levels* L, hipStream_t s, Replay* rp = nullptr);
 template <typename T, typename Op>
 void enqueue_iteration(const BatchCtx& c, int method, bool jac, int k, const Op& Aop, const Csr<T>& A,
                        const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm,

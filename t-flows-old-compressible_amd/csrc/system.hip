@@ -331,10 +331,110 @@ void enqueue_iteration(const BatchCtx& c, int method, bool jac, int k, const Op&
 // published, so one chunk is always queued behind the running one.  Kernels
 // queued past convergence return at once.  After iteration `cap` a last
 // stopping test (k = cap + 1) closes the loop.
+Replay::~Replay() {
+    for (auto& e : cache) (void)hipGraphExecDestroy(e.exec);
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Replay::ensure() {
+    if (stream) return;
+    TF_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto& e : ev) TF_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    kbase.ensure(1);
+}
+
+// TFLOWS_DIC_GRAPH=0 issues DIC iterations from the host instead.
+bool dic_graph_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("TFLOWS_DIC_GRAPH");
+        return !e || std::atoi(e) != 0;
+    }();
+    return on;
+}
+
+static const void* op_key(const Csr<double>& a) { return a.val; }
+static const void* op_key(const Csr<float>& a) { return a.val; }
+static const void* op_key(const DiffCsr32& a) { return a.val; }
+
+// One DIC iteration of one system replayed from a graph until the system
+// stops (or `cap` iterations were issued; then the k = cap + 1 test).
+template <typename T, typename Op>
+static void pace_graph(const BatchCtx& c, int method, int cap, const Op& Aop, const Csr<T>& A,
+                       const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm,
+                       const Levels* L, hipStream_t s, Replay& rp) {
+    rp.ensure();
+    hipStream_t g = rp.stream;
+    TF_CHECK(hipEventRecord(rp.ev[2], s));   // the replayer's stream follows the caller's work
+    TF_CHECK(hipStreamWaitEvent(g, rp.ev[2], 0));
+    // the graph is bound to every buffer it was captured with
+    std::string key;
+    auto add = [&](const void* p) { key.append(reinterpret_cast<const char*>(&p), sizeof(p)); };
+    key.append(reinterpret_cast<const char*>(&method), sizeof(method));
+    key.append(reinterpret_cast<const char*>(&c.lanes), sizeof(c.lanes));
+    key.append(reinterpret_cast<const char*>(&c.strided), sizeof(c.strided));
+    add(op_key(Aop));
+    add(A.rowptr), add(A.col), add(A.dia), add(A.val);
+    add(w.x), add(w.r1), add(w.D), add(w.p), add(w.q), add(w.r2), add(w.p2), add(w.q2), add(w.u1), add(w.u2);
+    add(w.v2), add(S.A), add(S.B), add(S.C), add(S.D), add(st), add(hm);
+    hipGraphExec_t exec = nullptr;
+    for (auto& e : rp.cache)
+        if (e.key == key) exec = e.exec;
+    if (!exec) {
+        if (rp.cache.size() >= 8) {   // keep a few buffer sets
+            (void)hipGraphExecDestroy(rp.cache.front().exec);
+            rp.cache.erase(rp.cache.begin());
+        }
+        BatchCtx ck = c;
+        ck.kbase = rp.kbase.p;
+        hipGraph_t gr = nullptr;
+        TF_CHECK(hipStreamBeginCapture(g, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_iteration<T, Op>(ck, method, false, 1, Aop, A, w, S, st, hm, L, g);
+            launch_advance(rp.kbase.p, 1, g);
+        } catch (...) {
+            (void)hipStreamEndCapture(g, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
+            throw;
+        }
+        TF_CHECK(hipStreamEndCapture(g, &gr));
+        const hipError_t e = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        TF_CHECK(e);
+        rp.cache.push_back({key, exec});
+    }
+    launch_set_int(rp.kbase.p, 0, g);
+    bool pending[2] = {false, false};
+    int slot = 0, issued = 0;
+    bool done = false;
+    while (issued < cap && !done) {
+        TF_CHECK(hipGraphLaunch(exec, g));
+        ++issued;
+        TF_CHECK(hipEventRecord(rp.ev[slot], g));
+        pending[slot] = true;
+        const int other = slot ^ 1;
+        if (pending[other]) {
+            TF_CHECK(hipEventSynchronize(rp.ev[other]));
+            pending[other] = false;
+            done = hm->done != 0;
+        }
+        slot = other;
+    }
+    if (!done) launch_check(c, std::max(cap, 0) + 1, S, st, hm, g);
+    TF_CHECK(hipGetLastError());
+    TF_CHECK(hipStreamSynchronize(g));
+}
+
 template <typename T, typename Op>
 void pace_iterations(const BatchCtx& c, int method, bool jac, int cap, const Op& Aop, const Csr<T>& A,
                      const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm,
-                     const Levels* L, hipStream_t s) {
+                     const Levels* L, hipStream_t s, Replay* rp) {
+    if (!jac && rp && !c.blk_sys && dic_graph_enabled()) {
+        pace_graph<T, Op>(c, method, cap, Aop, A, w, S, st, hm, L, s, *rp);
+        return;
+    }
     hipEvent_t ev[2];
     TF_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     TF_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
@@ -403,11 +503,11 @@ void start_solve(const BatchCtx& c, int method, int prec, const Op& Aop, const C
 template <typename T, typename Op>
 void run_solve(const BatchCtx& c, int method, int prec, int cap, const Op& Aop, const Csr<T>& A,
                const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L,
-               hipStream_t s, bool d_ready, bool zero_x) {
+               hipStream_t s, bool d_ready, bool zero_x, Replay* rp) {
     hm->done = 0;
     hm->iter = 0;
     start_solve<T, Op>(c, method, prec, Aop, A, w, S, st, hm, L, s, d_ready, zero_x);
-    pace_iterations<T, Op>(c, method, prec != 2, cap, Aop, A, w, S, st, hm, L, s);
+    pace_iterations<T, Op>(c, method, prec != 2, cap, Aop, A, w, S, st, hm, L, s, rp);
 }
 
 SolveResult System::solve(int method, int prec, double* x, double* r1, int cap, double tol, double stale_error,
@@ -444,7 +544,7 @@ SolveResult System::solve_impl(int method, int prec, double* x, double* r1, int 
     const Levels* L = prec == 2 ? &levels() : &kNoLevels;
     const Csr<double> A = csr();
     run_solve<double, Csr<double>>(ctx(), method, prec, cap, A, A, w, sites(), state_.p, mirror_, L, s, false,
-                                   false);
+                                   false, &replay_);
     SolverState h2;
     TF_CHECK(hipMemcpy(&h2, state_.p, sizeof(h2), hipMemcpyDeviceToHost));
     if (!h2.done) throw HipError("solver finished without reaching a stopping state");
@@ -480,13 +580,13 @@ KrylovVectors<double> System::vectors(double* x, double* r1) {
                                            HostMirror*, const Levels*, hipStream_t);                          \
     template void pace_iterations<T, Op>(const BatchCtx&, int, bool, int, const Op&, const Csr<T>&,           \
                                          const KrylovVectors<T>&, const Sites<T>&, SolverState*, HostMirror*, \
-                                         const Levels*, hipStream_t);                                         \
+                                         const Levels*, hipStream_t, Replay*);                                \
     template void start_solve<T, Op>(const BatchCtx&, int, int, const Op&, const Csr<T>&,                     \
                                      const KrylovVectors<T>&, const Sites<T>&, SolverState*, HostMirror*,     \
                                      const Levels*, hipStream_t, bool, bool);                                 \
     template void run_solve<T, Op>(const BatchCtx&, int, int, int, const Op&, const Csr<T>&,                  \
                                    const KrylovVectors<T>&, const Sites<T>&, SolverState*, HostMirror*,       \
-                                   const Levels*, hipStream_t, bool, bool);
+                                   const Levels*, hipStream_t, bool, bool, Replay*);
 TF_SOLVE_INST(double, Csr<double>)
 TF_SOLVE_INST(float, Csr<float>)
 TF_SOLVE_INST(float, DiffCsr32)

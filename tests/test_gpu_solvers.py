@@ -195,3 +195,35 @@ def test_device_api_matches_dropin(pkg, cases, cuda):
         d = pkg.krylov.solve_system(method, s.matrix, s.b, prec=1, niter=1000, tol=s.tol)
         assert it == d["niter"] and np.array_equal(x.cpu().numpy(), d["x"])
     ds.close()
+
+
+@pytest.mark.parametrize("method", ["cg", "bicg", "cgs"])
+def test_dic_graph_replay_repeats(pkg, oracle, cases, cuda, method):
+    """DIC iterations are replayed from a captured graph: repeated solves on
+    one system, with the same and with new vector buffers (a new graph),
+    and a capped solve (the cap + 1 test after the replays) all follow the
+    oracle and repeat bit for bit."""
+    import torch
+    s = cases["pressure12"]
+    m = s.matrix
+    ds = pkg.krylov.DeviceSystem(m)
+    ref = oracle.run(method, m, s.b, prec=2, niter=1000, tol=s.tol)
+    outs = []
+    for fresh in (False, False, True):
+        x = torch.zeros(m.n, dtype=torch.float64, device=cuda)
+        r = torch.from_numpy(s.b.copy()).to(cuda)
+        if fresh:
+            x, r = x.clone(), r.clone()
+        it, ini, fin = ds.solve(method, 2, x, r, 1000, s.tol)
+        outs.append((it, fin, x.cpu().numpy()))
+        assert abs(it - ref["niter"]) <= 1 and fin < s.tol
+        assert _rel(outs[-1][2], ref["x"]) <= 1e-10
+    for o in outs[1:]:
+        assert o[0] == outs[0][0] and o[1] == outs[0][1] and np.array_equal(o[2], outs[0][2])
+    capped = oracle.run(method, m, s.b, prec=2, niter=4, tol=s.tol)
+    x = torch.zeros(m.n, dtype=torch.float64, device=cuda)
+    r = torch.from_numpy(s.b.copy()).to(cuda)
+    it, ini, fin = ds.solve(method, 2, x, r, 4, s.tol)
+    assert it == capped["niter"] == 5
+    assert _rel(x.cpu().numpy(), capped["x"]) <= (1e-9 if method == "cgs" else 1e-11)
+    ds.close()
