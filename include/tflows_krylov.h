@@ -175,6 +175,16 @@ int32_t tflows_system_levels(tflows_system s);
  * reference. */
 int tflows_system_set_lanes(tflows_system s, int32_t lanes);
 
+/* 1 when the system stores its matrix in a locality (reverse
+ * Cuthill-McKee) order internally, 0 when in the caller's order, -1 on
+ * error.  The order is chosen at creation (environment TFLOWS_REORDER:
+ * 0 never, 1 always, unset: when it shortens the mean column offset at
+ * least 4x).  Entries keep their order within each row, so every row sum,
+ * and every result of tflows_spmv / tflows_resid / tflows_prec1, is the
+ * same bit for bit; vectors are always exchanged in the caller's order.
+ * No counterpart in the reference. */
+int32_t tflows_system_reordered(tflows_system s);
+
 /* ------------------------------------------------------------------------
  * Batches of independent systems (no counterpart in the reference, whose
  * callers solve one system per call: NewUVW.f90:506 once per velocity

@@ -254,6 +254,12 @@ int32_t tflows_system_levels(tflows_system s) {
     return v;
 }
 
+int32_t tflows_system_reordered(tflows_system s) {
+    int32_t v = 0;
+    if (guarded([&] { v = s->sys->reordered() ? 1 : 0; })) return -1;
+    return v;
+}
+
 int tflows_system_set_lanes(tflows_system s, int32_t lanes) {
     return guarded([&] { s->sys->set_lanes(lanes); });
 }

@@ -118,6 +118,9 @@ def _bind_extensions(lib):
     if hasattr(lib, "tflows_system_set_lanes"):
         lib.tflows_system_set_lanes.restype = C.c_int
         lib.tflows_system_set_lanes.argtypes = [vp, C.c_int32]
+    if hasattr(lib, "tflows_system_reordered"):
+        lib.tflows_system_reordered.restype = C.c_int32
+        lib.tflows_system_reordered.argtypes = [vp]
     if hasattr(lib, "tflows_batch_create"):
         pp = C.POINTER(vp)
         lib.tflows_batch_create.restype = C.c_int
@@ -323,6 +326,11 @@ class DeviceSystem:
 
     def levels(self) -> int:
         return int(self.lib.tflows_system_levels(self.handle))
+
+    @property
+    def reordered(self) -> bool:
+        """True when the matrix is stored in a locality order internally."""
+        return bool(self.lib.tflows_system_reordered(self.handle))
 
     def set_lanes(self, lanes: int):
         """SpMV form: lanes per row (1, 2, 4 or 8); results are identical."""
