@@ -132,6 +132,156 @@ def cpu_baseline(sysm, b, tol, full_iters, budget_s=20.0):
                 ms_per_iteration=round(per_it * 1e3, 3))
 
 
+# ---------------------------------------------------------------------------
+# Secondary workloads (--workload): the other configurations of the path,
+# measured the same way (inputs resident in HBM, wall time of complete
+# solves).  Each prints one JSON line of its own; the default run is the
+# 128^3 pressure benchmark above.
+# ---------------------------------------------------------------------------
+def _timed(torch, fn, steps):
+    best, out = None, None
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return best * 1e3, out
+
+
+def workload_momentum(args, torch, pkg):
+    """128^3 renumbered momentum system: BiCG (reference semantics, cap 2000)
+    and CGS, FP64 and FP32-inner refinement; error against the FP64 GPU
+    solution and the true residual."""
+    n = args.n
+    s = pkg.meshgen.momentum_system(n, seed=2)
+    m = s.matrix
+    ds = pkg.krylov.DeviceSystem(m)
+    A = m.to_scipy()
+    b = torch.from_numpy(s.b).cuda()
+    bn = float(np.linalg.norm(s.b))
+    out = {"metric": "time to 1e-8 relative residual, renumbered momentum system", "unit": "ms",
+           "workload": f"momentum {n}^3 (N={m.n}, nnz={m.nnz}), random cell renumbering, Jacobi",
+           "reordered": ds.reordered}
+    for method in ("cgs", "bicg"):
+        x = torch.zeros(m.n, dtype=torch.float64, device="cuda")
+        r = torch.empty_like(b)
+
+        def f64():
+            x.zero_(); r.copy_(b)
+            return ds.solve(method, 1, x, r, 2000, s.tol)
+        ms, (it, ini, fin) = _timed(torch, f64, args.steps)
+        x64 = x.cpu().numpy()
+        xm = torch.zeros_like(x)
+
+        def mixed():
+            xm.zero_()
+            return ds.solve_mixed(method, 1, xm, b, 2000, s.tol, inner_rtol=args.inner_rtol)
+        msm, (itm, inim, finm, outer) = _timed(torch, mixed, args.steps)
+        xmh = xm.cpu().numpy()
+        out[method] = dict(
+            fp64_ms=round(ms, 2), fp64_iters=it, fp64_true_rel_res=float(np.linalg.norm(s.b - A @ x64) / bn),
+            mixed_ms=round(msm, 2), mixed_iters=itm, mixed_outer=outer,
+            mixed_true_rel_res=float(np.linalg.norm(s.b - A @ xmh) / bn),
+            mixed_rel_err_vs_fp64=float(np.linalg.norm(xmh - x64) / max(np.linalg.norm(x64), 1e-300)))
+    ds.close()
+    return out
+
+
+def workload_batch(args, torch, pkg, kind):
+    """Batched solves (one launch per kernel for all systems, graph replay)
+    against the same systems solved one by one on the GPU."""
+    mg = pkg.meshgen
+    if kind == "transport":
+        systems, method, cap = mg.transport_batch(n=96, count=5, seed=3), args.method or "cgs", 2000
+    else:
+        systems, method, cap = mg.uneven_pressure_batch(count=64, seed=3), args.method or "cg", 5000
+    B = pkg.krylov.DeviceBatch([t.matrix for t in systems])
+    b = torch.from_numpy(np.concatenate([t.b for t in systems])).cuda()
+    tol = np.array([t.tol for t in systems])
+    x = torch.zeros(B.rows, dtype=torch.float64, device="cuda")
+    r = torch.empty_like(b)
+
+    def batched():
+        x.zero_(); r.copy_(b)
+        return B.solve(method, 1, x, r, cap, tol)
+    ms, res = _timed(torch, batched, args.steps + 1)   # the first call also tunes and captures
+    cfg = B.config(method, 1)
+    B.close()
+    Ds = [pkg.krylov.DeviceSystem(t.matrix) for t in systems]
+    bs = [torch.from_numpy(t.b).cuda() for t in systems]
+
+    def serial():
+        its = []
+        for D, t, bb in zip(Ds, systems, bs):
+            xx = torch.zeros(t.matrix.n, dtype=torch.float64, device="cuda")
+            rr = bb.clone()
+            its.append(D.solve(method, 1, xx, rr, cap, t.tol)[0])
+        return its
+    mss, its = _timed(torch, serial, args.steps)
+    for D in Ds:
+        D.close()
+    return {"metric": "batched solve time vs one GPU solve per system", "unit": "ms",
+            "workload": f"{len(systems)} {'transport (shared 96^3 pattern)' if kind == 'transport' else 'pressure'} "
+                        f"systems, {sum(t.matrix.n for t in systems)} rows, {method} + Jacobi, cap {cap}",
+            "batched_ms": round(ms, 2), "serial_ms": round(mss, 2), "speedup": round(mss / ms, 3),
+            "iterations_sum": int(np.sum(res["niter"])), "iterations_max": int(np.max(res["niter"])),
+            "converged": int(np.sum(res["fin_res"] < tol)), "config": cfg}
+
+
+def workload_wall(args, torch, pkg):
+    """256^3 wall-clustered pressure system (aspect up to 50:1): FP64 CG
+    with Jacobi and with DIC, mixed CG; SpMV and DIC-iteration bandwidth."""
+    n = args.n
+    s = pkg.meshgen.wall_pressure_system(n)
+    m = s.matrix
+    ds = pkg.krylov.DeviceSystem(m)
+    b = torch.from_numpy(s.b).cuda()
+    x = torch.zeros(m.n, dtype=torch.float64, device="cuda")
+    r = torch.empty_like(b)
+    out = {"metric": "time to 1e-8 relative residual, wall-clustered pressure", "unit": "ms",
+           "workload": f"pressure {n}^3 (N={m.n}, nnz={m.nnz}), max aspect {s.meta['max_aspect']:.1f}"}
+    for prec, cap in ((1, 40000), (2, args.dic_cap)):
+        def f():
+            x.zero_(); r.copy_(b)
+            return ds.solve("cg", prec, x, r, cap, s.tol)
+        ms, (it, ini, fin) = _timed(torch, f, 1)
+        out[f"cg_prec{prec}"] = dict(ms=round(ms, 1), iters=it, converged=bool(fin < s.tol),
+                                     ms_per_iteration=round(ms / max(1, min(it, cap)), 3))
+    out["dic_levels"] = ds.levels()
+    xm = torch.zeros_like(x)
+
+    def mixed():
+        xm.zero_()
+        return ds.solve_mixed_reliable(1, xm, b, 40000, s.tol, interval=args.interval)
+    ms, (it, ini, fin, rep) = _timed(torch, mixed, 1)
+    out["mixed_cg_prec1"] = dict(ms=round(ms, 1), iters=it, converged=bool(fin < s.tol), replacements=rep)
+    t = time_spmv(torch, ds, m.n, reps=20)
+    out["spmv_fp64_GBs"] = round(spmv_bytes(m.n, m.nnz, 8) / t / 1e9, 1)
+    # one DIC iteration = two level-scheduled sweeps over the triangles plus
+    # the Jacobi iteration's kernels; algorithmic bytes of one sweep:
+    # 12 F + 4 (N+1) + 28 N each, F = (nnz - N) / 2
+    dic = out["cg_prec2"]["ms_per_iteration"] - out["cg_prec1"]["ms_per_iteration"]
+    F = (m.nnz - m.n) // 2
+    sweep_bytes = 12 * F + 4 * (m.n + 1) + 28 * m.n
+    out["dic_sweep_ms"] = round(dic / 2, 3)
+    out["dic_sweep_GBs"] = round(sweep_bytes / (dic / 2 * 1e-3) / 1e9, 1) if dic > 0 else None
+    ds.close()
+    return out
+
+
+def workload_adjoint(args, torch, pkg):
+    """96^3 perturbed, renumbered pressure problem: adjoint gradients vs
+    central differences and an FP64 CPU gradient."""
+    t0 = time.perf_counter()
+    res = pkg.adjoint_harness.run(pkg, n=args.n, log=lambda *a: None)
+    res = {k: (float(v) if np.isscalar(v) else v) for k, v in res.items() if np.isscalar(v)}
+    return {"metric": "adjoint gradient error (relative) and harness time", "unit": "s",
+            "workload": f"perturbed renumbered pressure {args.n}^3", "seconds": round(time.perf_counter() - t0, 1),
+            **res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +292,13 @@ def main():
     ap.add_argument("--interval", type=int, default=50,
                     help="FP32 CG iterations between FP64 residual replacements")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--workload", default="pressure",
+                    choices=["pressure", "momentum", "batch-transport", "batch-pressure", "wall", "adjoint"],
+                    help="pressure: the headline 128^3 benchmark; the others measure the remaining "
+                         "configurations and print their own JSON line")
+    ap.add_argument("--method", default=None, help="Krylov method of the batch workloads")
+    ap.add_argument("--inner-rtol", type=float, default=1e-3, help="inner tolerance of FP32 refinement steps")
+    ap.add_argument("--dic-cap", type=int, default=4000, help="iteration cap of the DIC solve (wall workload)")
     ap.add_argument("--profile-spmv", action="store_true",
                     help="only run the SpMV launches (FP64 and FP32, 20 each) and a 512 MiB device copy, "
                          "for hardware-counter runs; prints nothing on stdout")
@@ -155,6 +312,23 @@ def main():
     import __graft_entry__ as ge
     pkg = ge.load()
     torch.cuda.set_device(0)
+
+    if args.workload != "pressure":
+        if args.workload == "momentum":
+            out = workload_momentum(args, torch, pkg)
+        elif args.workload.startswith("batch"):
+            out = workload_batch(args, torch, pkg, args.workload.split("-")[1])
+        elif args.workload == "wall":
+            if args.n == 128:
+                args.n = 256
+            out = workload_wall(args, torch, pkg)
+        else:
+            if args.n == 128:
+                args.n = 96
+            out = workload_adjoint(args, torch, pkg)
+        out.update(n_gpus=1, steps=args.steps, data="synthetic")
+        print(json.dumps(out), flush=True)
+        return
 
     n = args.n
     log(f"generating {n}^3 pressure system")

@@ -96,6 +96,9 @@ class Batch {
     template <typename T>
     void rebalance(const std::vector<char>& active, int grid, T* partials, SolverState* st);
     void tile_norms(const double* v, std::vector<double>& out);
+    // caller order <-> stored order of a batch vector (on the batch stream)
+    void gather_in(const double* ext, double* in);
+    void scatter_out(const double* in, double* ext);
 
     int k_, n_tot_;
     int64_t nnz_tot_;
@@ -115,6 +118,12 @@ class Batch {
     std::map<Key, BatchConfig> tuned_;
     std::map<GraphKey, hipGraphExec_t> graphs_;
     bool rebalance_ = true;
+    // locality order of renumbered systems: perm_ stored row -> caller row,
+    // eperm_ stored entry -> caller entry (whole batch); val_orig_ the
+    // values in the callers' order
+    bool reordered_ = false;
+    DevArray<int> perm_, eperm_;
+    DevArray<double> val_orig_, val_tmp_;
     int rebalances_ = 0;
     hipEvent_t ev_[5] = {};            // pacing (2), tuner timing (2), join
 };

@@ -160,21 +160,55 @@ Batch::Batch(int k, const int* n, const int* nonz, const int* const* acol, const
     n_tot_ = (int)rows;
     nnz_tot_ = ents;
     // concatenated 1-based layout; each system's columns must stay in its
-    // own block (the System constructor checks the rest of the layout)
+    // own block (the System constructor checks the rest of the layout).  A
+    // renumbered system is stored in its locality order (reorder.hip): its
+    // rows permuted inside its own block, each row's entries kept in their
+    // order, so every row sum is unchanged.
     std::vector<int> cacol(n_tot_ + 1), carow(nnz_tot_), cadia(n_tot_);
+    std::vector<int> gperm(n_tot_), geperm(nnz_tot_);
+    bool any = false;
     for (int i = 0; i < k; ++i) {
         const int r0 = roff_[i], e0 = eoff_[i], ni = n[i], nzi = nonz[i] - 1;
-        for (int r = 0; r < ni; ++r) {
-            cacol[r0 + r] = acol[i][r] + e0;
-            cadia[r0 + r] = adia[i][r] + e0;
-        }
+        std::vector<int> rp(ni + 1), cl(nzi);
+        for (int r = 0; r <= ni; ++r) rp[r] = acol[i][r] - 1;
+        for (int r = 0; r < ni; ++r)
+            if (acol[i][r + 1] < acol[i][r] || adia[i][r] < acol[i][r] || adia[i][r] >= acol[i][r + 1])
+                throw std::invalid_argument("Acol / Adia of a system are not a valid row layout");
         for (int e = 0; e < nzi; ++e) {
             const int c = arow[i][e];
             if (c < 1 || c > ni) throw std::invalid_argument("Arow of a system holds a column outside 1..N");
-            carow[e0 + e] = c + r0;
+            cl[e] = c - 1;
+        }
+        const std::vector<int> perm = locality_order(ni, rp, cl, true);
+        std::vector<int> inv;
+        if (!perm.empty()) {
+            any = true;
+            inv.assign(ni, 0);
+            for (int j = 0; j < ni; ++j) inv[perm[j]] = j;
+        }
+        int pos = e0;
+        for (int j = 0; j < ni; ++j) {
+            const int o = perm.empty() ? j : perm[j];
+            gperm[r0 + j] = r0 + o;
+            cacol[r0 + j] = pos + 1;
+            cadia[r0 + j] = pos + (adia[i][o] - acol[i][o]) + 1;
+            for (int e = rp[o]; e < rp[o + 1]; ++e, ++pos) {
+                carow[pos] = (perm.empty() ? cl[e] : inv[cl[e]]) + r0 + 1;
+                geperm[pos] = e0 + e;
+            }
         }
     }
     cacol[n_tot_] = (int)nnz_tot_ + 1;
+    if (any) {
+        perm_.ensure(n_tot_);
+        eperm_.ensure(nnz_tot_);
+        val_orig_.ensure(nnz_tot_);
+        val_tmp_.ensure(nnz_tot_);
+        TF_CHECK(hipMemcpy(perm_.p, gperm.data(), sizeof(int) * n_tot_, hipMemcpyHostToDevice));
+        TF_CHECK(hipMemcpy(eperm_.p, geperm.data(), sizeof(int) * nnz_tot_, hipMemcpyHostToDevice));
+        TF_CHECK(hipMemset(val_orig_.p, 0, sizeof(double) * nnz_tot_));
+        reordered_ = true;
+    }
     TF_CHECK(hipStreamCreateWithFlags(&bs_, hipStreamNonBlocking));
     sys_.reset(new System(n_tot_, (int)nnz_tot_ + 1, cacol.data(), carow.data(), cadia.data(), false, bs_,
                           /*allow_reorder=*/false));
@@ -234,8 +268,28 @@ Batch::~Batch() {
 
 void Batch::set_values(int sys, const double* aval, bool device_ptr, hipStream_t s) {
     if (sys < -1 || sys >= k_) throw std::invalid_argument("system index out of range");
-    if (sys < 0) sys_->set_values(aval, device_ptr, s);
-    else sys_->set_values_range(eoff_[sys], eoff_[sys + 1] - eoff_[sys], aval, device_ptr, s);
+    if (!reordered_) {
+        if (sys < 0) sys_->set_values(aval, device_ptr, s);
+        else sys_->set_values_range(eoff_[sys], eoff_[sys + 1] - eoff_[sys], aval, device_ptr, s);
+        return;
+    }
+    // values arrive in the callers' entry order; the stored matrix holds
+    // them in the locality order
+    const int64_t e0 = sys < 0 ? 0 : eoff_[sys], cnt = sys < 0 ? nnz_tot_ : eoff_[sys + 1] - eoff_[sys];
+    TF_CHECK(hipMemcpyAsync(val_orig_.p + e0, aval, sizeof(double) * cnt,
+                            device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    launch_gather((int)nnz_tot_, eperm_.p, val_orig_.p, val_tmp_.p, s);
+    sys_->set_values(val_tmp_.p, true, s);
+    if (!device_ptr) TF_CHECK(hipStreamSynchronize(s));
+}
+
+void Batch::gather_in(const double* ext, double* in) {
+    if (reordered_) launch_gather(n_tot_, perm_.p, ext, in, bs_);
+    else TF_CHECK(hipMemcpyAsync(in, ext, sizeof(double) * (size_t)n_tot_, hipMemcpyDeviceToDevice, bs_));
+}
+void Batch::scatter_out(const double* in, double* ext) {
+    if (reordered_) launch_scatter(n_tot_, perm_.p, in, ext, bs_);
+    else TF_CHECK(hipMemcpyAsync(ext, in, sizeof(double) * (size_t)n_tot_, hipMemcpyDeviceToDevice, bs_));
 }
 
 void Batch::join(hipStream_t s) {
@@ -544,14 +598,14 @@ void Batch::solve(int method, int prec, double* x, double* r1, const int* cap, c
         init[i].error = res[i].fin_res;   // returned on the ||b|| < tol exit
     }
     auto prepare = [&] {
-        TF_CHECK(hipMemcpyAsync(w.x, x, bytes, hipMemcpyDeviceToDevice, bs_));
-        TF_CHECK(hipMemcpyAsync(w.r1, r1, bytes, hipMemcpyDeviceToDevice, bs_));
+        gather_in(x, w.x);
+        gather_in(r1, w.r1);
     };
     const Csr<double> A = sys_->csr();
     std::vector<SolverState> out;
     run<double, Csr<double>>(64, method, prec, A, A, w, partials_.p, st_.p, hm_, init, false, false, prepare, out);
-    TF_CHECK(hipMemcpyAsync(x, w.x, bytes, hipMemcpyDeviceToDevice, bs_));
-    TF_CHECK(hipMemcpyAsync(r1, w.r1, bytes, hipMemcpyDeviceToDevice, bs_));
+    scatter_out(w.x, x);
+    scatter_out(w.r1, r1);
     TF_CHECK(hipStreamSynchronize(bs_));
     for (int i = 0; i < k_; ++i) {
         res[i].niter = out[i].niter_out;
@@ -598,8 +652,8 @@ void Batch::solve_mixed(int method, int prec, double* x, const double* b, double
     double* r64 = x64 + n;
     double* b64 = r64 + n;
     double* xprev = ws_.p + (size_t)V_P * n;   // FP64 slots are free during a mixed solve
-    TF_CHECK(hipMemcpyAsync(x64, x, bytes, hipMemcpyDeviceToDevice, bs_));
-    TF_CHECK(hipMemcpyAsync(b64, b, bytes, hipMemcpyDeviceToDevice, bs_));
+    gather_in(x, x64);
+    gather_in(b, b64);
 
     const DiffCsr32 Aop = sys_->diff32(bs_);
     const Csr<float> A32 = prec == 2 ? sys_->csr32(bs_) : Csr<float>{n, sys_->rowptr(), sys_->col(), sys_->dia(), nullptr};
@@ -695,8 +749,8 @@ void Batch::solve_mixed(int method, int prec, double* x, const double* b, double
             if (err[i] < tol[i] || its[i] == 0 || total[i] >= cap[i] || failed) open[i] = 0;
         }
     }
-    TF_CHECK(hipMemcpyAsync(x, x64, bytes, hipMemcpyDeviceToDevice, bs_));
-    if (r_out) TF_CHECK(hipMemcpyAsync(r_out, r64, bytes, hipMemcpyDeviceToDevice, bs_));
+    scatter_out(x64, x);
+    if (r_out) scatter_out(r64, r_out);
     TF_CHECK(hipStreamSynchronize(bs_));
     for (int i = 0; i < k_; ++i) {
         res[i].niter = err[i] < tol[i] ? total[i] : total[i] + 1;

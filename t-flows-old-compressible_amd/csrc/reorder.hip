@@ -151,19 +151,37 @@ static int reorder_mode() {
     return std::atoi(e) == 0 ? 0 : (std::atoi(e) == 1 ? 1 : 2);
 }
 
+std::vector<int> locality_order(int n, const std::vector<int>& rowptr, const std::vector<int>& col, bool allow) {
+    const int mode = allow ? reorder_mode() : 0;
+    if (mode == 0 || n <= 1) return {};
+    std::vector<int> perm = rcm_order(n, rowptr, col);
+    std::vector<int> inv(n);
+    for (int i = 0; i < n; ++i) inv[perm[i]] = i;
+    const double before = mean_offset(n, rowptr, col, nullptr);
+    const double after = mean_offset(n, rowptr, col, inv.data());
+    if (mode == 1 || before > 4.0 * after) return perm;
+    return {};
+}
+
+void launch_gather(int n, const int* perm, const double* src, double* dst, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_gather<double>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, perm, src, dst);
+    TF_CHECK(hipGetLastError());
+}
+void launch_scatter(int n, const int* perm, const double* src, double* dst, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_scatter<double>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, perm, src, dst);
+    TF_CHECK(hipGetLastError());
+}
+
 // Decide on and build the internal (reordered) storage from the original
 // 0-based host pattern.  Fills the device pattern arrays either way.
 void System::build_storage(bool allow_reorder, hipStream_t s) {
-    const int mode = allow_reorder ? reorder_mode() : 0;
-    std::vector<int> perm;
-    if (mode != 0 && n_ > 1) {
-        perm = rcm_order(n_, h_rowptr_, h_col_);
-        std::vector<int> inv(n_);
-        for (int i = 0; i < n_; ++i) inv[perm[i]] = i;
-        const double before = mean_offset(n_, h_rowptr_, h_col_, nullptr);
-        const double after = mean_offset(n_, h_rowptr_, h_col_, inv.data());
-        reordered_ = mode == 1 || before > 4.0 * after;
-        if (reordered_) iperm_h_ = std::move(inv);
+    std::vector<int> perm = locality_order(n_, h_rowptr_, h_col_, allow_reorder);
+    reordered_ = !perm.empty();
+    if (reordered_) {
+        iperm_h_.assign(n_, 0);
+        for (int i = 0; i < n_; ++i) iperm_h_[perm[i]] = i;
     }
     if (!reordered_) {
         TF_CHECK(hipMemcpyAsync(rowptr_.p, h_rowptr_.data(), sizeof(int) * (n_ + 1), hipMemcpyHostToDevice, s));

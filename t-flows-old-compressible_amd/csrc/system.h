@@ -100,6 +100,14 @@ struct Replay {
 };
 bool dic_graph_enabled();
 
+// Locality (reverse Cuthill-McKee) order of a 0-based pattern, new -> old,
+// when the environment (TFLOWS_REORDER) and the pattern call for one (it
+// must shorten the mean column offset at least 4x); empty otherwise.
+std::vector<int> locality_order(int n, const std::vector<int>& rowptr, const std::vector<int>& col, bool allow);
+// dst[i] = src[perm[i]]  /  dst[perm[i]] = src[i]
+void launch_gather(int n, const int* perm, const double* src, double* dst, hipStream_t s);
+void launch_scatter(int n, const int* perm, const double* src, double* dst, hipStream_t s);
+
 class System {
    public:
     System(int n, int nonz, const int* acol, const int* arow, const int* adia, bool device_ptrs, hipStream_t s,

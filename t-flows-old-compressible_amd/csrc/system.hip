@@ -141,13 +141,52 @@ const float* System::values32(hipStream_t s) {
     return val32_.p;
 }
 
-// Level analysis on the host (once per pattern).  Forward level of row i:
-// 1 + the largest level among its lower entries; backward level: the same
-// over upper entries, from the last row up.  Rows inside one level have no
-// dependency on each other, so a level is one parallel step of a sweep.
+// Level analysis on the host (once per pattern).  Row i of the forward
+// sweep depends on the rows of its lower entries (those before Adia(i)),
+// row i of the backward sweep on the rows of its upper entries; the level
+// of a row is the length of the longest dependency chain that ends in it,
+// so the rows of one level are independent and a level is one parallel
+// step of a sweep.  Levels are found by a topological walk of each
+// dependency graph, which holds in any numbering of the rows (a batch
+// stores its renumbered systems permuted, with each row's entries in their
+// original order, so "lower" there is no longer "column < row").
+static int dependency_levels(int n, const std::vector<int>& rowptr, const std::vector<int>& col,
+                             const std::vector<int>& dia, bool lower, std::vector<int>& lev) {
+    auto first = [&](int i) { return lower ? rowptr[i] : dia[i] + 1; };
+    auto last = [&](int i) { return lower ? dia[i] : rowptr[i + 1]; };
+    // dependents of each row (transpose of the dependency lists)
+    std::vector<int> cnt(n + 1, 0), indeg(n);
+    for (int i = 0; i < n; ++i) {
+        indeg[i] = last(i) - first(i);
+        for (int j = first(i); j < last(i); ++j) cnt[col[j] + 1]++;
+    }
+    for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int> dep(cnt[n]), fill(cnt.begin(), cnt.end() - 1);
+    for (int i = 0; i < n; ++i)
+        for (int j = first(i); j < last(i); ++j) dep[fill[col[j]]++] = i;
+    std::vector<int> queue;
+    queue.reserve(n);
+    lev.assign(n, 0);
+    for (int i = 0; i < n; ++i)
+        if (indeg[i] == 0) queue.push_back(i);
+    int nlev = 0;
+    for (size_t h = 0; h < queue.size(); ++h) {
+        const int v = queue[h];
+        nlev = std::max(nlev, lev[v] + 1);
+        for (int k = cnt[v]; k < cnt[v + 1]; ++k) {
+            const int d = dep[k];
+            lev[d] = std::max(lev[d], lev[v] + 1);
+            if (--indeg[d] == 0) queue.push_back(d);
+        }
+    }
+    if ((int)queue.size() != n)
+        throw std::invalid_argument("the triangular dependencies of the pattern form a cycle");
+    return nlev;
+}
+
 const Levels& System::levels() {
     if (levels_ready_) return levels_;
-    std::vector<int> lev(n_);
+    std::vector<int> lev;
     auto bucket = [&](int nlev, std::vector<int>& ptr, DevArray<int>& rows) {
         ptr.assign(nlev + 1, 0);
         for (int i = 0; i < n_; ++i) ptr[lev[i] + 1]++;
@@ -159,26 +198,10 @@ const Levels& System::levels() {
         rows.ensure(n_);
         TF_CHECK(hipMemcpy(rows.p, out.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
     };
-    int nf = 0;
-    for (int i = 0; i < n_; ++i) {
-        int l = 0;
-        for (int j = h_rowptr_[i]; j < h_dia_[i]; ++j) l = std::max(l, lev[h_col_[j]] + 1);
-        lev[i] = l;
-        nf = std::max(nf, l + 1);
-    }
-    levels_.nfwd = nf;
-    bucket(nf, levels_.fwd_ptr, levels_.fwd_rows);
-    int nb = 0;
-    for (int i = n_ - 1; i >= 0; --i) {
-        int l = 0;
-        for (int j = h_dia_[i] + 1; j < h_rowptr_[i + 1]; ++j) l = std::max(l, lev[h_col_[j]] + 1);
-        lev[i] = l;
-        nb = std::max(nb, l + 1);
-    }
-    // the upper loop above reads lev[] of rows below i, which were all
-    // recomputed first because i runs downwards and upper columns are > i
-    levels_.nbwd = nb;
-    bucket(nb, levels_.bwd_ptr, levels_.bwd_rows);
+    levels_.nfwd = dependency_levels(n_, h_rowptr_, h_col_, h_dia_, true, lev);
+    bucket(levels_.nfwd, levels_.fwd_ptr, levels_.fwd_rows);
+    levels_.nbwd = dependency_levels(n_, h_rowptr_, h_col_, h_dia_, false, lev);
+    bucket(levels_.nbwd, levels_.bwd_ptr, levels_.bwd_rows);
     levels_ready_ = true;
     return levels_;
 }
