@@ -195,9 +195,59 @@ __device__ __forceinline__ void stream_tile(const Csr<T>& A, int r0, int r1, Gat
 
 // The same streaming scheme for the difference form: phase 1 stores a_ij
 // and the gathered p_j; phase 2 forms sum a_ij (p_j - p_i) in column order
-// and adds rs_i p_i.
+// and adds rs_i p_i.  With one right-hand side the pair (a_ij, p_j) is
+// stored as one 8-byte LDS word, so each entry costs one LDS write and one
+// read.
+template <class Gather, class Epi>
+__device__ __forceinline__ void stream_tile_diff1(const DiffCsr32& A, int r0, int r1, Gather& g, Epi& epi) {
+    __shared__ int s_rp[kTileRows + 1];
+    __shared__ float2 s_ag[kLdsCap];
+    const int tid = threadIdx.x;
+    const int nr = r1 - r0;
+    __syncthreads();
+    for (int i = tid; i <= nr; i += kBlock) s_rp[i] = A.rowptr[r0 + i];
+    __syncthreads();
+    const int e0 = s_rp[0], e1 = s_rp[nr];
+    float acc[1] = {0.0f}, own[1] = {0.0f};
+    int my_b = 0, my_e = 0;
+    const bool has_row = tid < nr;
+    if (has_row) {
+        my_b = s_rp[tid];
+        my_e = s_rp[tid + 1];
+        g(r0 + tid, own);
+    }
+    for (int cs = e0; cs < e1; cs += kLdsCap) {
+        const int ce = min(cs + kLdsCap, e1);
+        for (int e = cs + tid; e < ce; e += kBlock) {
+            const int c = A.col[e];
+            const float a = A.val[e];
+            float gv[1];
+            g(c, gv);
+            s_ag[e - cs] = make_float2(a, gv[0]);
+        }
+        __syncthreads();
+        if (has_row) {
+            const int b = max(my_b, cs), e = min(my_e, ce);
+            for (int k = b; k < e; ++k) {
+                const float2 ag = s_ag[k - cs];
+                acc[0] = acc[0] + ag.x * (ag.y - own[0]);
+            }
+        }
+        __syncthreads();
+    }
+    if (has_row) {
+        const float rsi = A.rs[r0 + tid];
+        acc[0] = acc[0] + rsi * own[0];
+        epi(r0 + tid, acc);
+    }
+}
+
 template <int NRHS, class Gather, class Epi>
 __device__ __forceinline__ void stream_tile_diff(const DiffCsr32& A, int r0, int r1, Gather& g, Epi& epi) {
+    if constexpr (NRHS == 1) {
+        stream_tile_diff1(A, r0, r1, g, epi);
+        return;
+    }
     __shared__ int s_rp[kTileRows + 1];
     __shared__ float s_a[kLdsCap];
     __shared__ float s_g[NRHS * kLdsCap];
