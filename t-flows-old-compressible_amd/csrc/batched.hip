@@ -587,6 +587,7 @@ static void check_method(int method, int prec) {
 void Batch::solve(int method, int prec, double* x, double* r1, const int* cap, const double* tol,
                   SolveResult* res, hipStream_t s) {
     check_method(method, prec);
+    method = kernel_method(method);
     join(s);
     const KrylovVectors<double> w = vectors64();
     const size_t bytes = sizeof(double) * (size_t)n_tot_;
@@ -644,6 +645,7 @@ void Batch::solve_mixed(int method, int prec, double* x, const double* b, double
                         const double* tol, double inner_rtol, int max_outer, SolveResult* res, int* outer_steps,
                         hipStream_t s) {
     check_method(method, prec);
+    method = kernel_method(method);
     if (!(inner_rtol > 0.0 && inner_rtol < 1.0)) throw std::invalid_argument("inner_rtol must be in (0, 1)");
     join(s);
     const int n = n_tot_, T = tile0_[k_];

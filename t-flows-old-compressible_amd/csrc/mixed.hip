@@ -234,6 +234,7 @@ SolveResult System::solve_mixed_impl(int method, int prec, double* x, const doub
                                      hipStream_t s) {
     if (method < 0 || method > 2) throw std::invalid_argument("method must be 0 (cg), 1 (bicg) or 2 (cgs)");
     if (!(inner_rtol > 0.0 && inner_rtol < 1.0)) throw std::invalid_argument("inner_rtol must be in (0, 1)");
+    method = kernel_method(method);
     const size_t n = (size_t)n_;
     ws32_.ensure((V_COUNT + 2) * n);
     partials32_.ensure(Sites<float>::elems(grid_));

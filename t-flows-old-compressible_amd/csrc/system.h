@@ -100,6 +100,15 @@ struct Replay {
 };
 bool dic_graph_enabled();
 
+// The reference's BiCG multiplies its shadow by A and starts it at r
+// (BiCG.f90:93-95, 196-200), so its shadow vectors equal the primal ones bit
+// for bit at every step (r2 = r1, p2 = p1, q2 = q1; its rho = (z1, r2) is
+// CG's (r, z) with the factors of each product swapped, which IEEE
+// multiplication does not distinguish).  Its results are CG's results, and
+// method 1 runs the CG kernels, which move half the bytes.
+// TFLOWS_BICG_SHADOW=1 (read at every solve) runs the two-vector kernels.
+int kernel_method(int method);
+
 // Locality (reverse Cuthill-McKee) order of a 0-based pattern, new -> old,
 // when the environment (TFLOWS_REORDER) and the pattern call for one (it
 // must shorten the mean column offset at least 4x); empty otherwise.

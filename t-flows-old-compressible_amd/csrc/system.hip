@@ -546,9 +546,16 @@ SolveResult System::solve(int method, int prec, double* x, double* r1, int cap, 
     return res;
 }
 
+int kernel_method(int method) {
+    if (method != 1) return method;
+    const char* e = std::getenv("TFLOWS_BICG_SHADOW");
+    return (e && std::atoi(e) != 0) ? 1 : 0;
+}
+
 SolveResult System::solve_impl(int method, int prec, double* x, double* r1, int cap, double tol,
                                double stale_error, hipStream_t s) {
     if (method < 0 || method > 2) throw std::invalid_argument("method must be 0 (cg), 1 (bicg) or 2 (cgs)");
+    method = kernel_method(method);
     ws_.ensure((size_t)V_COUNT * n_);
     KrylovVectors<double> w = vectors(x, r1);
     // iteration 1 overwrites p, p2, u1, u2 before reading them; zero the

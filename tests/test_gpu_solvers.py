@@ -227,3 +227,34 @@ def test_dic_graph_replay_repeats(pkg, oracle, cases, cuda, method):
     assert it == capped["niter"] == 5
     assert _rel(x.cpu().numpy(), capped["x"]) <= (1e-9 if method == "cgs" else 1e-11)
     ds.close()
+
+
+@pytest.mark.parametrize("case", ["momentum10", "pressure9_perm"])
+@pytest.mark.parametrize("prec", [1, 2])
+def test_bicg_runs_as_cg_bitwise(pkg, oracle, cases, cuda, case, prec):
+    """BiCG's shadow equals its primal bit for bit (the reference multiplies
+    it by A and starts it at r), so the library runs BiCG through the CG
+    kernels; the two-vector kernels (TFLOWS_BICG_SHADOW=1) must give the
+    same bits, and both the oracle's CG result."""
+    import os
+    import torch
+    s = cases[case]
+    m = s.matrix
+    ds = pkg.krylov.DeviceSystem(m)
+    out = {}
+    for shadow in ("0", "1"):
+        os.environ["TFLOWS_BICG_SHADOW"] = shadow
+        try:
+            x = torch.zeros(m.n, dtype=torch.float64, device=cuda)
+            r = torch.from_numpy(s.b.copy()).to(cuda)
+            it, ini, fin = ds.solve("bicg", prec, x, r, 8, s.tol)
+        finally:
+            os.environ.pop("TFLOWS_BICG_SHADOW", None)
+        out[shadow] = (it, ini, fin, x.cpu().numpy(), r.cpu().numpy())
+    a, b = out["0"], out["1"]
+    assert a[0] == b[0] and a[1] == b[1] and a[2] == b[2]
+    assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
+    ref = oracle.run("cg", m, s.b, prec=prec, niter=8, tol=s.tol)
+    assert a[0] == ref["niter"] == 9
+    assert _rel(a[3], ref["x"]) <= 1e-11
+    ds.close()
