@@ -475,6 +475,7 @@ void Batch::run(int bits, int method, int prec, const Op& Aop, const Csr<T>& A, 
                 T* partials, SolverState* st, HostMirror* hm, const std::vector<SolverState>& init, bool d_ready,
                 bool zero_x, const std::function<void()>& prepare, std::vector<SolverState>& out) {
     const Key key{bits, method, prec};
+    if (prec == 2) sys_->prepare_dic(bits == 32, bs_);
     const bool auto_cfg = !fixed_.grid || !fixed_.lanes;
     if (auto_cfg && tuned_.find(key) == tuned_.end())
         tuned_[key] = tune<T, Op>(bits, method, prec, Aop, A, w, partials, st, hm, init, d_ready, zero_x, prepare);

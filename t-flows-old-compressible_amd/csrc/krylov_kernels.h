@@ -100,6 +100,19 @@ void launch_cgf_init_z(int n, const T* r, const T* D, T* z, const Sites<T>& S, c
 
 void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t s);
 void launch_set_int(int* p, int v, hipStream_t s);
+// level-major DIC sweeps (see Levels in system.h)
+template <typename T>
+void launch_plan_in(int n, const int* order, const T* u, const T* D, T* up, T* dp, const SolverState* st,
+                    hipStream_t s);
+template <typename T>
+void launch_plan_out(int n, const int* order, const T* zp, T* z, const SolverState* st, hipStream_t s);
+template <typename T>
+void launch_plan_fwd(int a, int b, const int* lptr, const int* lcol, const T* lval, const T* dp, const T* up, T* zp,
+                     const SolverState* st, hipStream_t s);
+template <typename T>
+void launch_plan_bwd(const int* pos, int count, const int* uptr, const int* ucol, const T* uval, const T* dp, T* zp,
+                     const SolverState* st, hipStream_t s);
+void launch_entry_gather(size_t count, const int* map, const double* src, double* dst, hipStream_t s);
 void launch_advance(int* kbase, int c, hipStream_t s);
 
 // Number of blocks of the row-streaming kernels for n rows: one block per

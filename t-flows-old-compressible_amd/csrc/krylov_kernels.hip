@@ -189,6 +189,61 @@ __global__ __launch_bounds__(kBlock) void k_dic_bwd_level(Csr<T> A, const int* _
     z[i] = sum1 * di;
 }
 
+// The same sweeps over the level-major copy of the triangles (rows of a
+// forward level at consecutive positions): gather u and D into position
+// order, forward and backward sweeps with the arithmetic of
+// k_dic_fwd_level / k_dic_bwd_level, scatter z back.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_plan_in(int n, const int* __restrict__ order, const T* __restrict__ u,
+                                                    const T* __restrict__ D, T* __restrict__ up,
+                                                    T* __restrict__ dp, const SolverState* st) {
+    if (st && st->done) return;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const int i = order[k];
+    up[k] = u[i];
+    dp[k] = D[i];
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_plan_out(int n, const int* __restrict__ order, const T* __restrict__ zp,
+                                                     T* __restrict__ z, const SolverState* st) {
+    if (st && st->done) return;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < n) z[order[k]] = zp[k];
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_plan_fwd(int a, int b, const int* __restrict__ lptr,
+                                                     const int* __restrict__ lcol, const T* __restrict__ lval,
+                                                     const T* __restrict__ dp, const T* __restrict__ up, T* zp,
+                                                     const SolverState* st) {
+    if (st && st->done) return;
+    const int k = a + blockIdx.x * kBlock + threadIdx.x;
+    if (k >= b) return;
+    T sum1 = up[k];
+    for (int e = lptr[k]; e < lptr[k + 1]; ++e) sum1 = sum1 - lval[e] * zp[lcol[e]];
+    zp[k] = sum1 * dp[k];
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_plan_bwd(const int* __restrict__ pos, int count,
+                                                     const int* __restrict__ uptr, const int* __restrict__ ucol,
+                                                     const T* __restrict__ uval, const T* __restrict__ dp, T* zp,
+                                                     const SolverState* st) {
+    if (st && st->done) return;
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    const int k = pos[t];
+    const T di = dp[k];
+    T sum1 = zp[k] / (di + T(kTiny));
+    for (int e = uptr[k]; e < uptr[k + 1]; ++e) sum1 = sum1 - uval[e] * zp[ucol[e]];
+    zp[k] = sum1 * di;
+}
+
+__global__ __launch_bounds__(kBlock) void k_entry_gather(int count, const int* __restrict__ map,
+                                                         const double* __restrict__ src, double* __restrict__ dst) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e < count) dst[e] = src[map[e]];
+}
+
 // ---------------------------------------------------------------------------
 // Early exit on ||b|| < tol (CG.f90:69-72) from site A, then the initial
 // residual r = b - A x in place (Resid.f90:29-34: subtraction in column
@@ -728,6 +783,32 @@ void launch_dic_bwd_level(const Csr<T>& A, const int* rows, int count, const T* 
                           const SolverState* st, hipStream_t s) {
     if (count > 0) TF_LAUNCH((k_dic_bwd_level<T>), (count + kBlock - 1) / kBlock, s), A, rows, count, D, z, st);
 }
+template <typename T>
+void launch_plan_in(int n, const int* order, const T* u, const T* D, T* up, T* dp, const SolverState* st,
+                    hipStream_t s) {
+    if (n > 0) TF_LAUNCH((k_plan_in<T>), (n + kBlock - 1) / kBlock, s), n, order, u, D, up, dp, st);
+}
+template <typename T>
+void launch_plan_out(int n, const int* order, const T* zp, T* z, const SolverState* st, hipStream_t s) {
+    if (n > 0) TF_LAUNCH((k_plan_out<T>), (n + kBlock - 1) / kBlock, s), n, order, zp, z, st);
+}
+template <typename T>
+void launch_plan_fwd(int a, int b, const int* lptr, const int* lcol, const T* lval, const T* dp, const T* up, T* zp,
+                     const SolverState* st, hipStream_t s) {
+    if (b > a) TF_LAUNCH((k_plan_fwd<T>), (b - a + kBlock - 1) / kBlock, s), a, b, lptr, lcol, lval, dp, up, zp, st);
+}
+template <typename T>
+void launch_plan_bwd(const int* pos, int count, const int* uptr, const int* ucol, const T* uval, const T* dp, T* zp,
+                     const SolverState* st, hipStream_t s) {
+    if (count > 0)
+        TF_LAUNCH((k_plan_bwd<T>), (count + kBlock - 1) / kBlock, s), pos, count, uptr, ucol, uval, dp, zp, st);
+}
+void launch_entry_gather(size_t count, const int* map, const double* src, double* dst, hipStream_t s) {
+    if (count > 0)
+        hipLaunchKernelGGL(k_entry_gather, dim3((unsigned)((count + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                           (int)count, map, src, dst);
+}
+
 template <typename T, bool Z, int L>
 static void resid_init_mode(int mode, const BatchCtx& c, const Csr<T>& A, const T* x, T* r, T* r2, const T* D,
                             const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
@@ -855,6 +936,16 @@ void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t 
         hipLaunchKernelGGL((k_convert<float, double>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
                            count, in, out);
 }
+
+#define TF_INSTANTIATE_PLAN(T)                                                                                   \
+    template void launch_plan_in<T>(int, const int*, const T*, const T*, T*, T*, const SolverState*, hipStream_t); \
+    template void launch_plan_out<T>(int, const int*, const T*, T*, const SolverState*, hipStream_t);            \
+    template void launch_plan_fwd<T>(int, int, const int*, const int*, const T*, const T*, const T*, T*,         \
+                                     const SolverState*, hipStream_t);                                          \
+    template void launch_plan_bwd<T>(const int*, int, const int*, const int*, const T*, const T*, T*,           \
+                                     const SolverState*, hipStream_t);
+TF_INSTANTIATE_PLAN(double)
+TF_INSTANTIATE_PLAN(float)
 
 #define TF_INSTANTIATE(T)                                                                                        \
     template void launch_setup<T>(const BatchCtx&, const Csr<T>&, int, bool, T*, const T*, const Sites<T>&,     \

@@ -267,6 +267,7 @@ SolveResult System::solve_mixed_impl(int method, int prec, double* x, const doub
     } else if (prec == 2) {
         double* d64 = ws_.p;                        // FP64 scratch slot
         dic_setup(d64, s);
+        prepare_dic(true, s);
         hipLaunchKernelGGL(k_f64_to_f32, dim3(nb), dim3(kBlock), 0, s, n_, (const double*)d64, w.D);
     } else {
         hipLaunchKernelGGL(k_fill_f32, dim3(nb), dim3(kBlock), 0, s, n_, 1.0f, w.D);

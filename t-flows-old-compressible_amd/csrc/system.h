@@ -68,7 +68,29 @@ struct Levels {
     int nfwd = 0, nbwd = 0;
     std::vector<int> fwd_ptr, bwd_ptr;   // host offsets into the row lists
     DevArray<int> fwd_rows, bwd_rows;    // device row lists, level by level
+    std::vector<int> fwd_rows_h, bwd_rows_h;
+
+    // Level-major copy of the triangles for the sweeps (DIC apply): rows in
+    // forward-level order, so each forward level is a contiguous range of
+    // positions and its loads are coalesced; the lower and upper entries of
+    // each row keep their order (the same products summed in the same
+    // order as the sweeps over the matrix itself).  The values are gathered
+    // from the matrix through the entry maps whenever it changes.
+    bool plan = false;
+    size_t nlow = 0, nup = 0;            // lower / upper entries
+    DevArray<int> order;                 // position -> row
+    DevArray<int> lptr, lcol, lmap;      // lower entries: position-based CSR, source entry
+    DevArray<int> uptr, ucol, umap;      // upper entries
+    DevArray<int> bwd_pos;               // backward levels as positions
+    mutable DevArray<double> lval, uval, up, zp, dp;
+    mutable DevArray<float> lval32, uval32, up32, zp32, dp32;
+    mutable bool vals64 = false, vals32 = false;
 };
+
+// TFLOWS_DIC_PLAN=0 (read when a system first builds its levels) keeps
+// the DIC sweeps on the matrix rows through the level lists instead of the
+// level-major copy.
+bool dic_plan_enabled();
 
 // Every vector argument of the public solve / product entry points is in
 // the caller's numbering.  The matrix itself may be stored permuted for
@@ -199,6 +221,9 @@ class System {
     SolverState* state() { return state_.p; }
     HostMirror* mirror() { return mirror_; }
     void dic_setup(double* D, hipStream_t s);
+    // values of the level-major triangles for the next DIC sweeps (FP64,
+    // and FP32 rounded from them when fp32)
+    void prepare_dic(bool fp32, hipStream_t s);
     template <typename T>
     void dic_apply(const Csr<T>& A, const T* D, const T* u, T* z, const SolverState* st, hipStream_t s);
 
@@ -210,6 +235,7 @@ class System {
     SolveResult solve_mixed_reliable_impl(int prec, double* x, const double* b, double* r_out, int cap, double tol,
                                           int interval, int* outer_steps, hipStream_t s);
     void build_storage(bool allow_reorder, hipStream_t s);
+    void build_dic_plan();
     void permute_values(hipStream_t s);
     double* ext_buffer(int k);   // k-th N-vector of the gather / scatter buffer
 

@@ -113,6 +113,7 @@ void System::set_values(const double* aval, bool device_ptr, hipStream_t s) {
     TF_CHECK(hipMemcpyAsync(reordered_ ? val_orig_.p : val_.p, aval, sizeof(double) * nnz_,
                             device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     if (reordered_) permute_values(s);
+    levels_.vals64 = levels_.vals32 = false;
     val32_valid_ = false;
     diff32_valid_ = false;
     transposed_valid_ = false;
@@ -125,6 +126,7 @@ void System::set_values_range(int64_t offset, int64_t count, const double* aval,
     if (reordered_) throw std::logic_error("partial value updates need the caller's storage order");
     TF_CHECK(hipMemcpyAsync(val_.p + offset, aval, sizeof(double) * count,
                             device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    levels_.vals64 = levels_.vals32 = false;
     val32_valid_ = false;
     diff32_valid_ = false;
     transposed_valid_ = false;
@@ -197,13 +199,96 @@ const Levels& System::levels() {
         for (int i = 0; i < n_; ++i) out[fill[lev[i]]++] = reordered_ ? iperm_h_[i] : i;
         rows.ensure(n_);
         TF_CHECK(hipMemcpy(rows.p, out.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
+        return out;
     };
     levels_.nfwd = dependency_levels(n_, h_rowptr_, h_col_, h_dia_, true, lev);
-    bucket(levels_.nfwd, levels_.fwd_ptr, levels_.fwd_rows);
+    levels_.fwd_rows_h = bucket(levels_.nfwd, levels_.fwd_ptr, levels_.fwd_rows);
     levels_.nbwd = dependency_levels(n_, h_rowptr_, h_col_, h_dia_, false, lev);
-    bucket(levels_.nbwd, levels_.bwd_ptr, levels_.bwd_rows);
+    levels_.bwd_rows_h = bucket(levels_.nbwd, levels_.bwd_ptr, levels_.bwd_rows);
+    build_dic_plan();
     levels_ready_ = true;
     return levels_;
+}
+
+// The level-major triangles (Levels::plan) from the stored pattern.
+void System::build_dic_plan() {
+    Levels& L = levels_;
+    if (!dic_plan_enabled()) return;
+    std::vector<int> rp(n_ + 1), cl(nnz_), dg(n_);
+    TF_CHECK(hipMemcpy(rp.data(), rowptr_.p, sizeof(int) * (n_ + 1), hipMemcpyDeviceToHost));
+    TF_CHECK(hipMemcpy(cl.data(), col_.p, sizeof(int) * nnz_, hipMemcpyDeviceToHost));
+    TF_CHECK(hipMemcpy(dg.data(), dia_.p, sizeof(int) * n_, hipMemcpyDeviceToHost));
+    const std::vector<int>& order = L.fwd_rows_h;
+    std::vector<int> pos(n_);
+    for (int k = 0; k < n_; ++k) pos[order[k]] = k;
+    std::vector<int> lptr(n_ + 1, 0), uptr(n_ + 1, 0);
+    for (int k = 0; k < n_; ++k) {
+        const int i = order[k];
+        lptr[k + 1] = lptr[k] + (dg[i] - rp[i]);
+        uptr[k + 1] = uptr[k] + (rp[i + 1] - dg[i] - 1);
+    }
+    std::vector<int> lcol(lptr[n_]), lmap(lptr[n_]), ucol(uptr[n_]), umap(uptr[n_]), bpos(n_);
+    for (int k = 0; k < n_; ++k) {
+        const int i = order[k];
+        int o = lptr[k];
+        for (int j = rp[i]; j < dg[i]; ++j, ++o) {
+            lcol[o] = pos[cl[j]];
+            lmap[o] = j;
+        }
+        o = uptr[k];
+        for (int j = dg[i] + 1; j < rp[i + 1]; ++j, ++o) {
+            ucol[o] = pos[cl[j]];
+            umap[o] = j;
+        }
+    }
+    for (int k = 0; k < n_; ++k) bpos[k] = pos[L.bwd_rows_h[k]];
+    auto up = [](DevArray<int>& d, const std::vector<int>& h) {
+        d.ensure(h.size());
+        if (!h.empty()) TF_CHECK(hipMemcpy(d.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
+    };
+    up(L.order, order);
+    up(L.lptr, lptr);
+    up(L.lcol, lcol);
+    up(L.lmap, lmap);
+    up(L.uptr, uptr);
+    up(L.ucol, ucol);
+    up(L.umap, umap);
+    up(L.bwd_pos, bpos);
+    L.nlow = lptr[n_];
+    L.nup = uptr[n_];
+    L.plan = true;
+    L.vals64 = L.vals32 = false;
+}
+
+bool dic_plan_enabled() {
+    const char* e = std::getenv("TFLOWS_DIC_PLAN");
+    return !e || std::atoi(e) != 0;
+}
+
+void System::prepare_dic(bool fp32, hipStream_t s) {
+    const Levels& L = levels();
+    if (!L.plan) return;
+    if (!L.vals64) {
+        L.lval.ensure(L.nlow);
+        L.uval.ensure(L.nup);
+        L.up.ensure(n_);
+        L.zp.ensure(n_);
+        L.dp.ensure(n_);
+        launch_entry_gather(L.nlow, L.lmap.p, val_.p, L.lval.p, s);
+        launch_entry_gather(L.nup, L.umap.p, val_.p, L.uval.p, s);
+        L.vals64 = true;
+        L.vals32 = false;
+    }
+    if (fp32 && !L.vals32) {
+        L.lval32.ensure(L.nlow);
+        L.uval32.ensure(L.nup);
+        L.up32.ensure(n_);
+        L.zp32.ensure(n_);
+        L.dp32.ensure(n_);
+        launch_f64_to_f32(L.nlow, L.lval.p, L.lval32.p, s);
+        launch_f64_to_f32(L.nup, L.uval.p, L.uval32.p, s);
+        L.vals32 = true;
+    }
 }
 
 void System::dic_setup(double* D, hipStream_t s) {
@@ -216,6 +301,24 @@ void System::dic_setup(double* D, hipStream_t s) {
 template <typename T>
 void dic_apply_levels(const Levels& L, const Csr<T>& A, const T* D, const T* u, T* z, const SolverState* st,
                       hipStream_t s) {
+    const bool f32 = sizeof(T) == sizeof(float);
+    if (L.plan && (f32 ? L.vals32 : L.vals64)) {
+        T* up = f32 ? (T*)L.up32.p : (T*)L.up.p;
+        T* zp = f32 ? (T*)L.zp32.p : (T*)L.zp.p;
+        T* dp = f32 ? (T*)L.dp32.p : (T*)L.dp.p;
+        const T* lv = f32 ? (const T*)L.lval32.p : (const T*)L.lval.p;
+        const T* uv = f32 ? (const T*)L.uval32.p : (const T*)L.uval.p;
+        const int n = A.n;
+        launch_plan_in(n, L.order.p, u, D, up, dp, st, s);
+        for (int l = 0; l < L.nfwd; ++l)
+            launch_plan_fwd(L.fwd_ptr[l], L.fwd_ptr[l + 1], L.lptr.p, L.lcol.p, lv, (const T*)dp, (const T*)up, zp,
+                            st, s);
+        for (int l = 0; l < L.nbwd; ++l)
+            launch_plan_bwd(L.bwd_pos.p + L.bwd_ptr[l], L.bwd_ptr[l + 1] - L.bwd_ptr[l], L.uptr.p, L.ucol.p, uv,
+                            (const T*)dp, zp, st, s);
+        launch_plan_out(n, L.order.p, (const T*)zp, z, st, s);
+        return;
+    }
     for (int l = 0; l < L.nfwd; ++l)
         launch_dic_fwd_level(A, L.fwd_rows.p + L.fwd_ptr[l], L.fwd_ptr[l + 1] - L.fwd_ptr[l], D, u, z, st, s);
     for (int l = 0; l < L.nbwd; ++l)
@@ -572,6 +675,7 @@ SolveResult System::solve_impl(int method, int prec, double* x, double* r1, int 
 
     static const Levels kNoLevels;
     const Levels* L = prec == 2 ? &levels() : &kNoLevels;
+    if (prec == 2) prepare_dic(false, s);
     const Csr<double> A = csr();
     run_solve<double, Csr<double>>(ctx(), method, prec, cap, A, A, w, sites(), state_.p, mirror_, L, s, false,
                                    false, &replay_);
