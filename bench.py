@@ -43,14 +43,17 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def spmv_bytes(n, nnz, word):
+def spmv_bytes(n, nnz, word, sliced=False):
     """Algorithmic HBM bytes of one SpMV launch, every array touched once.
     FP64 (reference CSR form): 12*nnz + 4*(N+1) + 16*N.
     FP32 (difference form used by the mixed solver): off-diagonal col+val
-    8*(nnz-N), off-diagonal row pointer 4*(N+1), row sums, x and y 4*N each."""
+    8*(nnz-N), row sums, x and y 4*N each, and the row index: the CSR row
+    pointer 4*(N+1), or for the sliced ELL layout its slice offsets
+    4*(ceil(N/64)+1); the slices' padding is not counted."""
     if word == 8:
         return 12 * nnz + 4 * (n + 1) + 16 * n
-    return 8 * (nnz - n) + 4 * (n + 1) + 12 * n
+    index = 4 * ((n + 63) // 64 + 1) if sliced else 4 * (n + 1)
+    return 8 * (nnz - n) + index + 12 * n
 
 
 def time_spmv(torch, ds, n, reps=50, fp32=False):
@@ -74,12 +77,16 @@ def time_spmv(torch, ds, n, reps=50, fp32=False):
 COPY_BYTES = 512 * 1024 * 1024
 
 
-def profile_spmv(torch, ds, n, reps=20):
+def profile_spmv(torch, ds, n, reps=20, ds_csr=None):
     """The launches the traffic counters are read on: `reps` FP64 and FP32
-    SpMVs of the workload and `reps` copies of COPY_BYTES (a kernel whose
-    HBM traffic is known: COPY_BYTES read, COPY_BYTES written), which
+    SpMVs of the workload (FP32 in the system's layout, and in the CSR
+    layout when ds_csr is given) and `reps` copies of COPY_BYTES (a kernel
+    whose HBM traffic is known: COPY_BYTES read, COPY_BYTES written), which
     calibrates FETCH_SIZE / WRITE_SIZE."""
-    for dt, f in ((torch.float64, ds.spmv), (torch.float32, ds.spmv32)):
+    ops = [(torch.float64, ds.spmv), (torch.float32, ds.spmv32)]
+    if ds_csr is not None:
+        ops.append((torch.float32, ds_csr.spmv32))
+    for dt, f in ops:
         x = torch.randn(n, dtype=dt, device="cuda")
         y = torch.empty(n, dtype=dt, device="cuda")
         for _ in range(reps):
@@ -89,7 +96,7 @@ def profile_spmv(torch, ds, n, reps=20):
     for _ in range(reps):
         dst.copy_(src)
     torch.cuda.synchronize()
-    log(f"profile-spmv: {reps} x (spmv fp64, spmv fp32, copy {COPY_BYTES} B) done")
+    log(f"profile-spmv: {reps} x (spmv fp64, spmv fp32 x{len(ops) - 1}, copy {COPY_BYTES} B) done")
 
 
 def measured_traffic(kernel_tag):
@@ -230,6 +237,47 @@ def workload_batch(args, torch, pkg, kind):
             "converged": int(np.sum(res["fin_res"] < tol)), "config": cfg}
 
 
+def dic_sweep_cost(torch, ds, s, iters=50, steps=3):
+    """Cost of one DIC apply: CG with Jacobi and with DIC, each capped at
+    `iters` iterations (best of `steps`, after a warm-up that builds the
+    levels and the level-major triangles), differ per iteration by the two
+    triangular sweeps.  Algorithmic bytes of one sweep: 12 F + 4 (N+1) +
+    28 N with F = (nnz - N) / 2 (triangle col + val, its row pointer, u,
+    D, z and the level-major staging of u and D)."""
+    m = s.matrix
+    b = torch.from_numpy(s.b).cuda()
+    x = torch.zeros(m.n, dtype=torch.float64, device="cuda")
+    r = torch.empty_like(b)
+    per_it = {}
+    for prec in (1, 2):
+        def f():
+            x.zero_(); r.copy_(b)
+            return ds.solve("cg", prec, x, r, iters, 0.0)
+        f()
+        ms, _ = _timed(torch, f, steps)
+        per_it[prec] = ms / iters
+    sweep = (per_it[2] - per_it[1]) / 2
+    F = (m.nnz - m.n) // 2
+    sweep_bytes = 12 * F + 4 * (m.n + 1) + 28 * m.n
+    return dict(jacobi_ms_per_iteration=round(per_it[1], 4), dic_ms_per_iteration=round(per_it[2], 4),
+                sweep_ms=round(sweep, 4), sweep_bytes=sweep_bytes,
+                sweep_GBs=round(sweep_bytes / (sweep * 1e-3) / 1e9, 1) if sweep > 0 else None)
+
+
+def workload_dic_sweep(args, torch, pkg):
+    """The DIC apply alone on the wall-clustered system (capped solves, a
+    short run for kernel profiles)."""
+    n = args.n
+    s = pkg.meshgen.wall_pressure_system(n)
+    m = s.matrix
+    ds = pkg.krylov.DeviceSystem(m)
+    out = {"metric": "DIC apply (two triangular sweeps) per CG iteration", "unit": "ms",
+           "workload": f"wall-clustered pressure {n}^3 (N={m.n}, nnz={m.nnz}), CG capped at {args.sweep_iters}",
+           "dic_levels": ds.levels(), **dic_sweep_cost(torch, ds, s, args.sweep_iters, args.steps)}
+    ds.close()
+    return out
+
+
 def workload_wall(args, torch, pkg):
     """256^3 wall-clustered pressure system (aspect up to 50:1): FP64 CG
     with Jacobi and with DIC, mixed CG; SpMV and DIC-iteration bandwidth."""
@@ -242,6 +290,14 @@ def workload_wall(args, torch, pkg):
     r = torch.empty_like(b)
     out = {"metric": "time to 1e-8 relative residual, wall-clustered pressure", "unit": "ms",
            "workload": f"pressure {n}^3 (N={m.n}, nnz={m.nnz}), max aspect {s.meta['max_aspect']:.1f}"}
+    # once per pattern: level analysis and the level-major triangles, then
+    # their values (one capped DIC solve), outside the timed solves
+    t0 = time.perf_counter()
+    out["dic_levels"] = ds.levels()
+    x.zero_(); r.copy_(b)
+    ds.solve("cg", 2, x, r, 1, s.tol)
+    torch.cuda.synchronize()
+    out["dic_setup_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
     for prec, cap in ((1, 40000), (2, args.dic_cap)):
         def f():
             x.zero_(); r.copy_(b)
@@ -249,7 +305,6 @@ def workload_wall(args, torch, pkg):
         ms, (it, ini, fin) = _timed(torch, f, 1)
         out[f"cg_prec{prec}"] = dict(ms=round(ms, 1), iters=it, converged=bool(fin < s.tol),
                                      ms_per_iteration=round(ms / max(1, min(it, cap)), 3))
-    out["dic_levels"] = ds.levels()
     xm = torch.zeros_like(x)
 
     def mixed():
@@ -257,16 +312,18 @@ def workload_wall(args, torch, pkg):
         return ds.solve_mixed_reliable(1, xm, b, 40000, s.tol, interval=args.interval)
     ms, (it, ini, fin, rep) = _timed(torch, mixed, 1)
     out["mixed_cg_prec1"] = dict(ms=round(ms, 1), iters=it, converged=bool(fin < s.tol), replacements=rep)
+
+    def mixed_dic():
+        xm.zero_()
+        return ds.solve_mixed("cg", 2, xm, b, args.dic_cap, s.tol, inner_rtol=args.inner_rtol)
+    mixed_dic()   # FP32 copies of the values
+    ms, (it, ini, fin, outer) = _timed(torch, mixed_dic, 1)
+    out["mixed_cg_prec2"] = dict(ms=round(ms, 1), iters=it, converged=bool(fin < s.tol), outer=outer)
     t = time_spmv(torch, ds, m.n, reps=20)
     out["spmv_fp64_GBs"] = round(spmv_bytes(m.n, m.nnz, 8) / t / 1e9, 1)
-    # one DIC iteration = two level-scheduled sweeps over the triangles plus
-    # the Jacobi iteration's kernels; algorithmic bytes of one sweep:
-    # 12 F + 4 (N+1) + 28 N each, F = (nnz - N) / 2
-    dic = out["cg_prec2"]["ms_per_iteration"] - out["cg_prec1"]["ms_per_iteration"]
-    F = (m.nnz - m.n) // 2
-    sweep_bytes = 12 * F + 4 * (m.n + 1) + 28 * m.n
-    out["dic_sweep_ms"] = round(dic / 2, 3)
-    out["dic_sweep_GBs"] = round(sweep_bytes / (dic / 2 * 1e-3) / 1e9, 1) if dic > 0 else None
+    sw = dic_sweep_cost(torch, ds, s, args.sweep_iters)
+    out["dic_sweep_ms"] = sw["sweep_ms"]
+    out["dic_sweep_GBs"] = sw["sweep_GBs"]
     ds.close()
     return out
 
@@ -293,12 +350,15 @@ def main():
                     help="FP32 CG iterations between FP64 residual replacements")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--workload", default="pressure",
-                    choices=["pressure", "momentum", "batch-transport", "batch-pressure", "wall", "adjoint"],
+                    choices=["pressure", "momentum", "batch-transport", "batch-pressure", "wall", "dic-sweep",
+                             "adjoint"],
                     help="pressure: the headline 128^3 benchmark; the others measure the remaining "
                          "configurations and print their own JSON line")
     ap.add_argument("--method", default=None, help="Krylov method of the batch workloads")
     ap.add_argument("--inner-rtol", type=float, default=1e-3, help="inner tolerance of FP32 refinement steps")
     ap.add_argument("--dic-cap", type=int, default=4000, help="iteration cap of the DIC solve (wall workload)")
+    ap.add_argument("--sweep-iters", type=int, default=50,
+                    help="capped iterations that time the DIC apply (wall, dic-sweep workloads)")
     ap.add_argument("--profile-spmv", action="store_true",
                     help="only run the SpMV launches (FP64 and FP32, 20 each) and a 512 MiB device copy, "
                          "for hardware-counter runs; prints nothing on stdout")
@@ -318,10 +378,10 @@ def main():
             out = workload_momentum(args, torch, pkg)
         elif args.workload.startswith("batch"):
             out = workload_batch(args, torch, pkg, args.workload.split("-")[1])
-        elif args.workload == "wall":
+        elif args.workload in ("wall", "dic-sweep"):
             if args.n == 128:
                 args.n = 256
-            out = workload_wall(args, torch, pkg)
+            out = (workload_wall if args.workload == "wall" else workload_dic_sweep)(args, torch, pkg)
         else:
             if args.n == 128:
                 args.n = 96
@@ -338,7 +398,11 @@ def main():
     log(f"N={N} nnz={nnz}")
     ds = pkg.krylov.DeviceSystem(m)
     if args.profile_spmv:
-        profile_spmv(torch, ds, N)
+        os.environ["TFLOWS_FP32_OP"] = "csr"
+        ds_csr = pkg.krylov.DeviceSystem(m)
+        del os.environ["TFLOWS_FP32_OP"]
+        profile_spmv(torch, ds, N, ds_csr=ds_csr)
+        ds_csr.close()
         ds.close()
         return
     b_dev = torch.from_numpy(s.b).cuda()
@@ -387,7 +451,9 @@ def main():
     t64 = time_spmv(torch, ds, N, fp32=False)
     b64 = spmv_bytes(N, nnz, 8)
     t32 = time_spmv(torch, ds, N, fp32=True)
-    b32 = spmv_bytes(N, nnz, 4)
+    layout = ds.fp32_layout()
+    b32 = spmv_bytes(N, nnz, 4, sliced=layout["sliced"])
+    k32 = "k_spmv<DiffSell32>" if layout["sliced"] else "k_spmv<DiffCsr32>"
     head = "mixed" if "mixed" in results else "fp64"
     spmv_t, spmv_b = (t32, b32) if head == "mixed" else (t64, b64)
     ach = spmv_b / spmv_t / 1e9
@@ -421,9 +487,12 @@ def main():
         "spmv_fp32_GBs": round(b32 / t32 / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic("fp32" if head == "mixed" else "fp64") if n == 128 else None,
-                     "kernel": "k_spmv<DiffCsr32>" if head == "mixed" else "k_spmv<Csr<double>>",
+                     "traffic": (measured_traffic(("fp32_sliced" if layout["sliced"] else "fp32_csr")
+                                                  if head == "mixed" else "fp64") if n == 128 else None),
+                     "kernel": k32 if head == "mixed" else "k_spmv<Csr<double>>",
                      "bytes_per_launch": spmv_b, "us_per_launch": round(spmv_t * 1e6, 2)},
+        "fp32_operator": {"layout": "sliced ELL (64-row slices)" if layout["sliced"] else "CSR",
+                          "stored_entries": layout["stored_entries"], "off_diagonal_entries": nnz - N},
     }
     if not args.no_cpu:
         log("timing the CPU baseline")

@@ -185,6 +185,15 @@ int tflows_system_set_lanes(tflows_system s, int32_t lanes);
  * No counterpart in the reference. */
 int32_t tflows_system_reordered(tflows_system s);
 
+/* Layout of the FP32 inner operator of the mixed-precision solves (the
+ * difference form of A: off-diagonal a_ij and FP64 row sums rounded to
+ * FP32).  *sliced = 1: sliced ELL, 64-row slices stored slot-major and
+ * padded to the slice's longest row (the default); 0: CSR
+ * (environment TFLOWS_FP32_OP=csr when the system was created).
+ * *stored_entries: off-diagonal entries held, padding included.  Builds
+ * the operator if no solve has yet.  No counterpart in the reference. */
+int tflows_system_fp32_layout(tflows_system s, int32_t* sliced, int64_t* stored_entries, void* stream);
+
 /* ------------------------------------------------------------------------
  * Batches of independent systems (no counterpart in the reference, whose
  * callers solve one system per call: NewUVW.f90:506 once per velocity

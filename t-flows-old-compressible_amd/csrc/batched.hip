@@ -542,6 +542,7 @@ void Batch::run(int bits, int method, int prec, const Op& Aop, const Csr<T>& A, 
     TF_CHECK(hipMemcpy(out.data(), st, sizeof(SolverState) * k_, hipMemcpyDeviceToHost));
     for (int i = 0; i < k_; ++i)
         if (!out[i].done) throw HipError("batched solve finished without every system reaching a stopping state");
+    if (prec == 2) sys_->check_dic(bs_);
 }
 
 KrylovVectors<double> Batch::vectors64() {
@@ -658,7 +659,6 @@ void Batch::solve_mixed(int method, int prec, double* x, const double* b, double
     gather_in(x, x64);
     gather_in(b, b64);
 
-    const DiffCsr32 Aop = sys_->diff32(bs_);
     const Csr<float> A32 = prec == 2 ? sys_->csr32(bs_) : Csr<float>{n, sys_->rowptr(), sys_->col(), sys_->dia(), nullptr};
     const KrylovVectors<float> w = vectors32();
     const int nb = (n + kBlock - 1) / kBlock;
@@ -706,8 +706,10 @@ void Batch::solve_mixed(int method, int prec, double* x, const double* b, double
             hipLaunchKernelGGL(k_tile_scale, dim3(tgrid), dim3(kBlock), 0, bs_, T, (const int*)tile_r0_.p,
                                (const int*)tile_sys_.p, (const double*)r64, (const double*)scale_.p, w.r1, w.x);
         };
-        run<float, DiffCsr32>(32, method, prec, Aop, A32, w, partials32_.p, st32_.p, hm32_, init, true, true, prepare,
-                              out);
+        sys_->with_op32(bs_, [&](const auto& Aop) {
+            run<float, std::decay_t<decltype(Aop)>>(32, method, prec, Aop, A32, w, partials32_.p, st32_.p, hm32_, init,
+                                                    true, true, prepare, out);
+        });
         std::vector<int> its(k_, 0);
         for (int i = 0; i < k_; ++i) {
             sc[i] = 0.0;

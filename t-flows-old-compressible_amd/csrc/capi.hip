@@ -260,6 +260,16 @@ int32_t tflows_system_reordered(tflows_system s) {
     return v;
 }
 
+int tflows_system_fp32_layout(tflows_system s, int32_t* sliced, int64_t* stored_entries, void* stream) {
+    return guarded([&] {
+        if (!sliced || !stored_entries) throw std::invalid_argument("null output pointer");
+        const hipStream_t st = as_stream(stream);
+        *sliced = s->sys->op32_sliced() ? 1 : 0;
+        *stored_entries = s->sys->op32_sliced() ? s->sys->sell32_entries(st) : s->sys->nnz() - s->sys->n();
+        TF_CHECK(hipStreamSynchronize(st));
+    });
+}
+
 int tflows_system_set_lanes(tflows_system s, int32_t lanes) {
     return guarded([&] { s->sys->set_lanes(lanes); });
 }

@@ -384,6 +384,12 @@ template void launch_cgj<DiffCsr32, float>(bool, bool, const DiffCsr32&, int, fl
                                            const float*, const Sites<float>&, const double*, SolverState*,
                                            HostMirror*, hipStream_t);
 
+template void launch_cgj<DiffSell32, float>(bool, bool, const DiffSell32&, int, float*, float*, float*, float*,
+                                            float*, const float*, const Sites<float>&, const double*, SolverState*,
+                                            HostMirror*, hipStream_t);
+template void launch_cgf_spmv<DiffSell32, float>(bool, const DiffSell32&, const float*, float*, float*, float*,
+                                                 const Sites<float>&, const double*, SolverState*, HostMirror*,
+                                                 hipStream_t);
 template void launch_cgf_spmv<Csr<double>, double>(bool, const Csr<double>&, const double*, double*, double*,
                                                    double*, const Sites<double>&, const double*, SolverState*,
                                                    HostMirror*, hipStream_t);

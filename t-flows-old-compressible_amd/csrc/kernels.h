@@ -125,9 +125,25 @@ struct DiffCsr32 {
     const float* __restrict__ rs;    // row sums (from FP64)
 };
 
+// The same FP32 difference-form operator in sliced ELL layout: rows in
+// slices of 64 (one wavefront), the off-diagonal entries of a slice stored
+// slot-major -- slot j of row 64 s + l at off[s] + 64 j + l -- and padded
+// to the longest row of the slice with (column = own row, a = 0), which
+// adds exactly 0.  One thread per row: every column and value load of a
+// wavefront is one contiguous segment, with no LDS staging.
+struct DiffSell32 {
+    int n;
+    const int* __restrict__ off;     // ceil(n/64)+1 slice offsets
+    const int* __restrict__ col;
+    const float* __restrict__ val;   // off-diagonal a_ij, 0 in padding
+    const float* __restrict__ rs;    // row sums (from FP64)
+};
+constexpr int kSlice = 64;
+
 template <typename Op> struct OpValue;
 template <typename T> struct OpValue<Csr<T>> { using type = T; };
 template <> struct OpValue<DiffCsr32> { using type = float; };
+template <> struct OpValue<DiffSell32> { using type = float; };
 
 // Rows per tile of the stream kernels: one row per thread in the summation
 // phase.  Tiles are the unit of work split across blocks.
@@ -496,6 +512,41 @@ __device__ __forceinline__ void op_tile(const DiffCsr32& A, int r0, int r1, Gath
     if constexpr (L == 1) stream_tile_diff<NRHS>(A, r0, r1, g, epi);
     else group_tile_diff<L, NRHS>(A, r0, r1, g, epi);
 }
+// Rows [r0, r1) of the sliced operator, row r0 + t on thread t (no early
+// return: the caller's block reductions follow).
+template <int NRHS, class Gather, class Epi>
+__device__ __forceinline__ void sell_tile(const DiffSell32& A, int r0, int r1, Gather& g, Epi& epi) {
+    const int row = r0 + (int)threadIdx.x;
+    if (row < r1) {
+        const int sl = row / kSlice;
+        const int b = A.off[sl];
+        const int w = (A.off[sl + 1] - b) / kSlice;
+        const int* __restrict__ cp = A.col + b + (row - sl * kSlice);
+        const float* __restrict__ vp = A.val + b + (row - sl * kSlice);
+        float own[NRHS], acc[NRHS];
+        g(row, own);
+#pragma unroll
+        for (int v = 0; v < NRHS; ++v) acc[v] = 0.0f;
+#pragma unroll 2
+        for (int j = 0; j < w; ++j) {
+            const int c = cp[j * kSlice];
+            const float a = vp[j * kSlice];
+            float gv[NRHS];
+            g(c, gv);
+#pragma unroll
+            for (int v = 0; v < NRHS; ++v) acc[v] = acc[v] + a * (gv[v] - own[v]);
+        }
+        const float rsi = A.rs[row];
+#pragma unroll
+        for (int v = 0; v < NRHS; ++v) acc[v] = acc[v] + rsi * own[v];
+        epi(row, acc);
+    }
+}
+template <int NRHS, int L = 1, class Gather, class Epi>
+__device__ __forceinline__ void op_tile(const DiffSell32& A, int r0, int r1, Gather& g, Epi& epi) {
+    sell_tile<NRHS>(A, r0, r1, g, epi);
+}
+
 // r = b - A x over one tile (the residual form: subtraction from b).
 template <int L, typename T, class Gather, class Init, class Epi>
 __device__ __forceinline__ void resid_tile(const Csr<T>& A, int r0, int r1, Gather& g, Init& init, Epi& epi) {

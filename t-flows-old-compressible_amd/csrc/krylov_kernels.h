@@ -107,6 +107,15 @@ void launch_plan_in(int n, const int* order, const T* u, const T* D, T* up, T* d
 template <typename T>
 void launch_plan_out(int n, const int* order, const T* zp, T* z, const SolverState* st, hipStream_t s);
 template <typename T>
+void launch_flow_reset(int n, T* zf, hipStream_t s);
+template <typename T>
+void launch_flow_fwd(int n, const int* order, const T* u, const T* D, const int* lptr, const int* lcol,
+                     const T* lval, T* dp, T* zf, T* zb, int* fault, uint64_t wait_ticks, const SolverState* st,
+                     hipStream_t s);
+template <typename T>
+void launch_flow_bwd(int n, const int* order, T* z, const int* uptr, const int* ucol, const T* uval, const T* dp,
+                     T* zf, T* zb, int* fault, uint64_t wait_ticks, const SolverState* st, hipStream_t s);
+template <typename T>
 void launch_plan_fwd(int a, int b, const int* lptr, const int* lcol, const T* lval, const T* dp, const T* up, T* zp,
                      const SolverState* st, hipStream_t s);
 template <typename T>

@@ -28,7 +28,9 @@
 //   BiCG   Process/BiCG.f90:50-254          -> k_bicg_p, k_bicg_spmv, k_bicg_update
 //   CGS    Process/CGS.f90:62-337           -> k_cgs_u, k_cgs_spmv_v, k_cgs_q, k_cgs_spmv_r
 //   DIC    Process/CG.f90:135-156           -> k_dic_fwd_level, k_dic_bwd_level
+#include <algorithm>
 #include <stdexcept>
+#include <string>
 #include <type_traits>
 
 #include "krylov_kernels.h"
@@ -190,9 +192,10 @@ __global__ __launch_bounds__(kBlock) void k_dic_bwd_level(Csr<T> A, const int* _
 }
 
 // The same sweeps over the level-major copy of the triangles (rows of a
-// forward level at consecutive positions): gather u and D into position
-// order, forward and backward sweeps with the arithmetic of
-// k_dic_fwd_level / k_dic_bwd_level, scatter z back.
+// forward level at consecutive positions, padding positions with row -1
+// and no entries): gather u and D into position order, forward and
+// backward sweeps with the arithmetic of k_dic_fwd_level /
+// k_dic_bwd_level, scatter z back.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_plan_in(int n, const int* __restrict__ order, const T* __restrict__ u,
                                                     const T* __restrict__ D, T* __restrict__ up,
@@ -201,15 +204,204 @@ __global__ __launch_bounds__(kBlock) void k_plan_in(int n, const int* __restrict
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= n) return;
     const int i = order[k];
-    up[k] = u[i];
-    dp[k] = D[i];
+    up[k] = i < 0 ? T(0) : u[i];
+    dp[k] = i < 0 ? T(0) : D[i];
 }
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_plan_out(int n, const int* __restrict__ order, const T* __restrict__ zp,
                                                      T* __restrict__ z, const SolverState* st) {
     if (st && st->done) return;
     const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k < n) z[order[k]] = zp[k];
+    if (k >= n) return;
+    const int i = order[k];
+    if (i >= 0) z[i] = zp[k];
+}
+
+// ---------------------------------------------------------------------------
+// One launch per triangle.  A value slot holds kEmpty until its row is
+// solved; a row polls the slots of its dependencies with device-coherent
+// loads and publishes its own value with a device-coherent store (the
+// value is its own flag, so no fence is needed).  kEmpty is a signalling
+// NaN pattern: arithmetic produces only quiet NaNs, so a solved value
+// never equals it.  The grid is persistent and no larger than the blocks
+// that fit on the device at once; block b takes the position chunks b,
+// b + G, b + 2G, ... in that order, so each wavefront visits its 64-row
+// pieces in increasing position order.  Rows only wait on smaller
+// positions, and the rows of one wavefront piece share one forward level
+// and never wait on each other, so the unfinished piece of smallest
+// position always has its inputs and its wavefront at hand: every wait
+// ends.
+template <typename T>
+struct Slot;
+template <>
+struct Slot<double> {
+    using U = unsigned long long;
+    static constexpr U kEmpty = 0x7FF4A5A5A5A5A5A5ull;
+    static constexpr U kGaveUp = 0x7FF8000000000000ull;
+};
+template <>
+struct Slot<float> {
+    using U = unsigned int;
+    static constexpr U kEmpty = 0x7FA5A5A5u;
+    static constexpr U kGaveUp = 0x7FC00000u;
+};
+
+// poll interval of a waiting row (s_sleep units of 64 clocks) and the
+// share of the resident blocks a one-launch sweep uses (1/kFlowShare)
+#ifndef TFLOWS_FLOW_SLEEP
+#define TFLOWS_FLOW_SLEEP 1
+#endif
+#ifndef TFLOWS_FLOW_SHARE
+#define TFLOWS_FLOW_SHARE 2
+#endif
+constexpr int kFlowShare = TFLOWS_FLOW_SHARE;
+
+template <typename T>
+__device__ __forceinline__ void publish(T* p, T v) {
+    using U = typename Slot<T>::U;
+    __hip_atomic_store(reinterpret_cast<U*>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__device__ __forceinline__ typename Slot<T>::U peek(const T* p) {
+    using U = typename Slot<T>::U;
+    return __hip_atomic_load(reinterpret_cast<const U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The value of slot p once solved, given a first reading v.  Every 64
+// polls the wait checks the fault word and the time spent; past wait_ticks
+// (or once another row gave up) it raises the fault word and returns a
+// quiet NaN.
+template <typename T>
+__device__ __forceinline__ T settle(typename Slot<T>::U v, const T* p, int* fault, uint64_t t0,
+                                    uint64_t wait_ticks) {
+    using U = typename Slot<T>::U;
+    const U* q = reinterpret_cast<const U*>(p);
+    unsigned polls = 0;
+    while (v == Slot<T>::kEmpty) {
+        __builtin_amdgcn_s_sleep(TFLOWS_FLOW_SLEEP);
+        if ((++polls & 63u) == 0) {
+            const bool late = (uint64_t)wall_clock64() - t0 > wait_ticks;
+            if (late || __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                v = Slot<T>::kGaveUp;
+                break;
+            }
+        }
+        v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return __builtin_bit_cast(T, v);
+}
+
+// Every value slot of zf back to kEmpty (before the first sweep of a
+// solve; afterwards the backward sweep empties each slot it has read).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_flow_reset(int n, T* __restrict__ zf) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < n) zf[k] = __builtin_bit_cast(T, Slot<T>::kEmpty);
+}
+
+// sum1 - sum_e a_e z_{c_e} over entries [e0, e1), in entry order.  Rows
+// of up to kFlowRegs entries load their columns and values, then the
+// first readings of all their dependencies, before the first wait.
+constexpr int kFlowRegs = 4;
+template <typename T>
+__device__ __forceinline__ T flow_row(T sum1, int e0, int e1, const int* __restrict__ col,
+                                      const T* __restrict__ val, const T* z, int* fault, uint64_t t0,
+                                      uint64_t wait_ticks) {
+    using U = typename Slot<T>::U;
+    if (e1 - e0 <= kFlowRegs) {
+        int c[kFlowRegs];
+        T a[kFlowRegs];
+        U v[kFlowRegs];
+#pragma unroll
+        for (int j = 0; j < kFlowRegs; ++j)
+            if (e0 + j < e1) {
+                c[j] = col[e0 + j];
+                a[j] = val[e0 + j];
+            }
+#pragma unroll
+        for (int j = 0; j < kFlowRegs; ++j)
+            if (e0 + j < e1) v[j] = peek(z + c[j]);
+#pragma unroll
+        for (int j = 0; j < kFlowRegs; ++j)
+            if (e0 + j < e1) sum1 = sum1 - a[j] * settle(v[j], z + c[j], fault, t0, wait_ticks);
+        return sum1;
+    }
+    for (int e = e0; e < e1; ++e) {
+        const T* p = z + col[e];
+        sum1 = sum1 - val[e] * settle(peek(p), p, fault, t0, wait_ticks);
+    }
+    return sum1;
+}
+
+// forward sweep: z_k = (u_k - sum_{lower} a_kj z_j) * D_k, in entry order,
+// with u and D gathered from the caller's rows (position k holds row
+// order[k], -1 for padding); D is kept in position order for the backward
+// sweep, and every zb slot is emptied for it (no one reads zb now).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_flow_fwd(int n, const int* __restrict__ order, const T* __restrict__ u,
+                                                     const T* __restrict__ D, const int* __restrict__ lptr,
+                                                     const int* __restrict__ lcol, const T* __restrict__ lval,
+                                                     T* __restrict__ dp, T* zf, T* __restrict__ zb, int* fault,
+                                                     uint64_t wait_ticks, const SolverState* st) {
+    if (st && st->done) return;
+    const uint64_t t0 = (uint64_t)wall_clock64();
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+        const int i = order[k];
+        const T uk = i < 0 ? T(0) : u[i];
+        const T dk = i < 0 ? T(0) : D[i];
+        dp[k] = dk;
+        zb[k] = __builtin_bit_cast(T, Slot<T>::kEmpty);
+        const T sum1 = flow_row(uk, lptr[k], lptr[k + 1], lcol, lval, (const T*)zf, fault, t0, wait_ticks);
+        publish(zf + k, sum1 * dk);
+    }
+}
+
+// backward sweep over the positions in reverse:
+//   z_k = (zf_k / (D_k + TINY) - sum_{upper} a_kj z_j) * D_k
+// each row empties its zf slot (read only by itself here) for the next
+// forward sweep and stores its result in the caller's row as well.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_flow_bwd(int n, const int* __restrict__ order, T* __restrict__ z,
+                                                     const int* __restrict__ uptr, const int* __restrict__ ucol,
+                                                     const T* __restrict__ uval, const T* __restrict__ dp,
+                                                     T* __restrict__ zf, T* zb, int* fault, uint64_t wait_ticks,
+                                                     const SolverState* st) {
+    if (st && st->done) return;
+    const uint64_t t0 = (uint64_t)wall_clock64();
+    for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+        const int k = n - 1 - t;
+        const T di = dp[k];
+        const T zk = zf[k];
+        zf[k] = __builtin_bit_cast(T, Slot<T>::kEmpty);
+        const T sum1 = flow_row(zk / (di + T(kTiny)), uptr[k], uptr[k + 1], ucol, uval, (const T*)zb, fault, t0,
+                                wait_ticks);
+        const T v = sum1 * di;
+        publish(zb + k, v);
+        const int i = order[k];
+        if (i >= 0) z[i] = v;
+    }
+}
+
+// Blocks of a one-launch sweep: a share of those that fit on the device
+// at once (the rest is headroom), and no more than the position chunks.
+template <typename T>
+int flow_grid(int n) {
+    static int resident = [] {
+        auto check = [](hipError_t e) {
+            if (e != hipSuccess) throw std::runtime_error(std::string("sweep grid: ") + hipGetErrorString(e));
+        };
+        int dev = 0, cus = 0, per_cu = 0, per_cu_b = 0;
+        check(hipGetDevice(&dev));
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_flow_fwd<T>, kBlock, 0));
+        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_b, k_flow_bwd<T>, kBlock, 0));
+        per_cu = std::min(per_cu, per_cu_b);
+        return std::max(1, cus * per_cu / kFlowShare);
+    }();
+    return std::max(1, std::min(resident, (n + kBlock - 1) / kBlock));
 }
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_plan_fwd(int a, int b, const int* __restrict__ lptr,
@@ -789,6 +981,25 @@ void launch_plan_in(int n, const int* order, const T* u, const T* D, T* up, T* d
     if (n > 0) TF_LAUNCH((k_plan_in<T>), (n + kBlock - 1) / kBlock, s), n, order, u, D, up, dp, st);
 }
 template <typename T>
+void launch_flow_reset(int n, T* zf, hipStream_t s) {
+    if (n > 0) TF_LAUNCH((k_flow_reset<T>), (n + kBlock - 1) / kBlock, s), n, zf);
+}
+template <typename T>
+void launch_flow_fwd(int n, const int* order, const T* u, const T* D, const int* lptr, const int* lcol,
+                     const T* lval, T* dp, T* zf, T* zb, int* fault, uint64_t wait_ticks, const SolverState* st,
+                     hipStream_t s) {
+    if (n > 0)
+        TF_LAUNCH((k_flow_fwd<T>), flow_grid<T>(n), s), n, order, u, D, lptr, lcol, lval, dp, zf, zb, fault,
+            wait_ticks, st);
+}
+template <typename T>
+void launch_flow_bwd(int n, const int* order, T* z, const int* uptr, const int* ucol, const T* uval, const T* dp,
+                     T* zf, T* zb, int* fault, uint64_t wait_ticks, const SolverState* st, hipStream_t s) {
+    if (n > 0)
+        TF_LAUNCH((k_flow_bwd<T>), flow_grid<T>(n), s), n, order, z, uptr, ucol, uval, dp, zf, zb, fault,
+            wait_ticks, st);
+}
+template <typename T>
 void launch_plan_out(int n, const int* order, const T* zp, T* z, const SolverState* st, hipStream_t s) {
     if (n > 0) TF_LAUNCH((k_plan_out<T>), (n + kBlock - 1) / kBlock, s), n, order, zp, z, st);
 }
@@ -943,7 +1154,12 @@ void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t 
     template void launch_plan_fwd<T>(int, int, const int*, const int*, const T*, const T*, const T*, T*,         \
                                      const SolverState*, hipStream_t);                                          \
     template void launch_plan_bwd<T>(const int*, int, const int*, const int*, const T*, const T*, T*,           \
-                                     const SolverState*, hipStream_t);
+                                     const SolverState*, hipStream_t);                                          \
+    template void launch_flow_reset<T>(int, T*, hipStream_t);                                                   \
+    template void launch_flow_fwd<T>(int, const int*, const T*, const T*, const int*, const int*, const T*, T*,  \
+                                     T*, T*, int*, uint64_t, const SolverState*, hipStream_t);                  \
+    template void launch_flow_bwd<T>(int, const int*, T*, const int*, const int*, const T*, const T*, T*, T*,    \
+                                     int*, uint64_t, const SolverState*, hipStream_t);
 TF_INSTANTIATE_PLAN(double)
 TF_INSTANTIATE_PLAN(float)
 
@@ -991,5 +1207,6 @@ TF_INSTANTIATE(float)
 TF_INSTANTIATE_OP(Csr<double>, double)
 TF_INSTANTIATE_OP(Csr<float>, float)
 TF_INSTANTIATE_OP(DiffCsr32, float)
+TF_INSTANTIATE_OP(DiffSell32, float)
 
 }  // namespace tflows

@@ -202,6 +202,67 @@ DiffCsr32 System::diff32(hipStream_t s) {
     return DiffCsr32{n_, off_rowptr_.p, off_col_.p, off_val_.p, rowsum32_.p};
 }
 
+__global__ __launch_bounds__(kBlock) void k_sell_values(int64_t count, const int* __restrict__ map,
+                                                        const float* __restrict__ src, float* __restrict__ dst) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e < count) dst[e] = map[e] < 0 ? 0.0f : src[map[e]];
+}
+
+DiffSell32 System::sell32(hipStream_t s) {
+    const DiffCsr32 d = diff32(s);
+    if (!sell_built_) {
+        // slice layout from the off-diagonal pattern (once per pattern)
+        const int64_t noff = nnz_ - n_;
+        std::vector<int> orp(n_ + 1), ocol(noff > 0 ? noff : 0);
+        TF_CHECK(hipMemcpyAsync(orp.data(), d.rowptr, sizeof(int) * (n_ + 1), hipMemcpyDeviceToHost, s));
+        if (noff > 0)
+            TF_CHECK(hipMemcpyAsync(ocol.data(), d.col, sizeof(int) * noff, hipMemcpyDeviceToHost, s));
+        TF_CHECK(hipStreamSynchronize(s));
+        const int nsl = (n_ + kSlice - 1) / kSlice;
+        std::vector<int> off(nsl + 1, 0);
+        for (int sl = 0; sl < nsl; ++sl) {
+            int w = 0;
+            for (int i = sl * kSlice; i < std::min(n_, (sl + 1) * kSlice); ++i) w = std::max(w, orp[i + 1] - orp[i]);
+            if ((int64_t)off[sl] + (int64_t)w * kSlice > INT32_MAX)
+                throw std::invalid_argument("sliced operator exceeds 2^31 entries");
+            off[sl + 1] = off[sl] + w * kSlice;
+        }
+        std::vector<int> col(off[nsl]), map(off[nsl]);
+        for (int sl = 0; sl < nsl; ++sl) {
+            const int w = (off[sl + 1] - off[sl]) / kSlice;
+            for (int l = 0; l < kSlice; ++l) {
+                const int i = sl * kSlice + l;
+                const int len = i < n_ ? orp[i + 1] - orp[i] : 0;
+                for (int j = 0; j < w; ++j) {
+                    const int e = off[sl] + j * kSlice + l;
+                    col[e] = j < len ? ocol[orp[i] + j] : (i < n_ ? i : 0);
+                    map[e] = j < len ? orp[i] + j : -1;
+                }
+            }
+        }
+        sell_off_.ensure(nsl + 1);
+        sell_col_.ensure(col.size());
+        sell_map_.ensure(map.size());
+        sell_val_.ensure(col.size());
+        TF_CHECK(hipMemcpy(sell_off_.p, off.data(), sizeof(int) * (nsl + 1), hipMemcpyHostToDevice));
+        if (!col.empty()) {
+            TF_CHECK(hipMemcpy(sell_col_.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
+            TF_CHECK(hipMemcpy(sell_map_.p, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));
+        }
+        sell_entries_ = (int64_t)col.size();
+        sell_built_ = true;
+        sell_valid_ = false;
+    }
+    if (!sell_valid_) {
+        if (sell_entries_ > 0)
+            hipLaunchKernelGGL(k_sell_values, dim3((unsigned)((sell_entries_ + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                               s, sell_entries_, (const int*)sell_map_.p, d.val, sell_val_.p);
+        TF_CHECK(hipGetLastError());
+        sell_valid_ = true;
+    }
+    return DiffSell32{n_, sell_off_.p, sell_col_.p, sell_val_.p, d.rs};
+}
+
 const float* System::diag32(hipStream_t s) {
     diff32(s);
     return diag32_.p;
@@ -244,7 +305,6 @@ SolveResult System::solve_mixed_impl(int method, int prec, double* x, const doub
         TF_CHECK(hipHostMalloc((void**)&mirror32_, sizeof(HostMirror), hipHostMallocMapped | hipHostMallocCoherent));
     ws_.ensure((size_t)V_COUNT * n);
 
-    const DiffCsr32 Aop = diff32(s);
     // the DIC sweeps of an FP32 inner solve read the plainly rounded
     // off-diagonal values; the operator of its SpMVs is the difference form
     const Csr<float> A32 = prec == 2 ? csr32(s) : Csr<float>{n_, rowptr_.p, col_.p, dia_.p, nullptr};
@@ -293,8 +353,11 @@ SolveResult System::solve_mixed_impl(int method, int prec, double* x, const doub
         h.stall_min = kInnerStallMin;
         h.err_min[0] = HUGE_VAL;
         TF_CHECK(hipMemcpyAsync(state32_.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
-        run_solve<float, DiffCsr32>(ctx(), method, prec, cap - total, Aop, A32, w, S32, state32_.p, mirror32_, L, s,
-                                    /*d_ready=*/true, /*zero_x=*/true, &replay_);
+        with_op32(s, [&](const auto& Aop) {
+            run_solve<float, std::decay_t<decltype(Aop)>>(ctx(), method, prec, cap - total, Aop, A32, w, S32,
+                                                          state32_.p, mirror32_, L, s, /*d_ready=*/true,
+                                                          /*zero_x=*/true, &replay_);
+        });
         SolverState h2;
         TF_CHECK(hipMemcpy(&h2, state32_.p, sizeof(h2), hipMemcpyDeviceToHost));
         // iterations actually executed: niter on convergence, cap on exhaustion
@@ -327,6 +390,7 @@ SolveResult System::solve_mixed_impl(int method, int prec, double* x, const doub
         if (done_its == 0) break;      // the inner solve could not make progress
     }
     if (outer_steps) *outer_steps = outer;
+    if (prec == 2) check_dic(s);
     res.niter = err < tol ? total : total + 1;
     res.fin_res = err;
     return res;
@@ -364,7 +428,6 @@ SolveResult System::solve_mixed_reliable_impl(int prec, double* x, const double*
     state32_.ensure(1);
     if (!mirror32_)
         TF_CHECK(hipHostMalloc((void**)&mirror32_, sizeof(HostMirror), hipHostMallocMapped | hipHostMallocCoherent));
-    const DiffCsr32 Aop = diff32(s);
     float* base = ws32_.p;
     auto vec = [&](int k) { return base + (size_t)k * n; };
     float *D32 = vec(V_D), *p0 = vec(V_P), *p1 = vec(V_P2), *q = vec(V_Q), *z = vec(V_U1);
@@ -397,7 +460,7 @@ SolveResult System::solve_mixed_reliable_impl(int prec, double* x, const double*
     int slot = 0, issued = 0, replacements = 1;
     bool done = false;
     const long long guard = 2LL * (cap > 0 ? cap : 0) + 2LL * interval;
-    while (!done && issued <= guard) {
+    auto iterations = [&](const auto& Aop) {
         for (int i = 0; i < interval; ++i) {
             if (cg_form() == 2) {
                 launch_cgf_spmv(true, Aop, (const float*)z, p0, p1, q, S32, (const double*)partE, state32_.p,
@@ -405,10 +468,13 @@ SolveResult System::solve_mixed_reliable_impl(int prec, double* x, const double*
                 launch_cgf_update(n_, x32, r32, z, (const float*)p0, (const float*)p1, (const float*)q,
                                   (const float*)D32, S32, state32_.p, s);
             } else {
-                launch_cgj(true, ctx().strided != 0, Aop, n_, x32, r32, z, p0, q, (const float*)D32, S32, (const double*)partE,
-                           state32_.p, mirror32_, s);
+                launch_cgj(true, ctx().strided != 0, Aop, n_, x32, r32, z, p0, q, (const float*)D32, S32,
+                           (const double*)partE, state32_.p, mirror32_, s);
             }
         }
+    };
+    while (!done && issued <= guard) {
+        with_op32(s, iterations);
         issued += interval;
         replace(nullptr);
         ++replacements;

@@ -76,21 +76,38 @@ struct Levels {
     // each row keep their order (the same products summed in the same
     // order as the sweeps over the matrix itself).  The values are gathered
     // from the matrix through the entry maps whenever it changes.
+    //
+    // mode 1 sweeps one level per launch.  mode 2 sweeps each triangle in
+    // one launch of a persistent grid that walks the positions in order,
+    // each row waiting for the values of its dependencies, which start as
+    // an "empty" NaN pattern that no arithmetic produces (a
+    // synchronisation-free triangular solve).  Every wavefront holds rows
+    // of one forward level, so no row waits on a row of its own
+    // wavefront; the backward sweep runs over the positions in reverse.
+    // A row that waits longer than wait_ticks gives up, raises the fault
+    // word and leaves NaN, which the host turns into an error.
     bool plan = false;
+    int mode = 0;                        // 0 matrix rows by level lists, 1 / 2 as above
+    int npos = 0;                        // positions (rows and padding)
+    std::vector<int> pos_ptr;            // host: position range of each forward level
     size_t nlow = 0, nup = 0;            // lower / upper entries
-    DevArray<int> order;                 // position -> row
+    DevArray<int> fault;                 // raised by a row that gave up waiting
+    uint64_t wait_ticks = 0;
+    DevArray<int> order;                 // position -> row (-1: padding)
     DevArray<int> lptr, lcol, lmap;      // lower entries: position-based CSR, source entry
     DevArray<int> uptr, ucol, umap;      // upper entries
     DevArray<int> bwd_pos;               // backward levels as positions
-    mutable DevArray<double> lval, uval, up, zp, dp;
-    mutable DevArray<float> lval32, uval32, up32, zp32, dp32;
+    mutable DevArray<double> lval, uval, up, zp, zb, dp;
+    mutable DevArray<float> lval32, uval32, up32, zp32, zb32, dp32;
     mutable bool vals64 = false, vals32 = false;
 };
 
-// TFLOWS_DIC_PLAN=0 (read when a system first builds its levels) keeps
-// the DIC sweeps on the matrix rows through the level lists instead of the
-// level-major copy.
-bool dic_plan_enabled();
+// TFLOWS_DIC_SWEEP (read when a system first builds its levels): 0 sweeps
+// the matrix rows level by level, 1 the level-major copy level by level,
+// 2 (default) the level-major copy one launch per triangle.  All three
+// give the same bits.  TFLOWS_DIC_WAIT_MS bounds a row's wait in form 2.
+int dic_sweep_mode();
+uint64_t dic_wait_ms();
 
 // Every vector argument of the public solve / product entry points is in
 // the caller's numbering.  The matrix itself may be stored permuted for
@@ -166,6 +183,21 @@ class System {
     // FP32 difference-form operator (off-diagonal CSR + FP64 row sums) and
     // the FP32 diagonal, made on first use after set_values (mixed.hip)
     DiffCsr32 diff32(hipStream_t s);
+    // the same operator in sliced ELL layout (built on first use, values
+    // refreshed after set_values)
+    DiffSell32 sell32(hipStream_t s);
+    // Calls f with the FP32 inner operator of this system: the sliced form
+    // unless TFLOWS_FP32_OP=csr was set when the system was created.
+    template <class F>
+    void with_op32(hipStream_t s, F&& f) {
+        if (op32_sliced_) f(sell32(s));
+        else f(diff32(s));
+    }
+    bool op32_sliced() const { return op32_sliced_; }
+    int64_t sell32_entries(hipStream_t s) {
+        sell32(s);
+        return sell_entries_;
+    }
     const float* diag32(hipStream_t s);
 
     // Solve with x, r1 device pointers (x(1:N), r1(1:N)); `stale_error` is
@@ -224,6 +256,8 @@ class System {
     // values of the level-major triangles for the next DIC sweeps (FP64,
     // and FP32 rounded from them when fp32)
     void prepare_dic(bool fp32, hipStream_t s);
+    // throws when a one-launch sweep of the last solve gave up waiting
+    void check_dic(hipStream_t s) const;
     template <typename T>
     void dic_apply(const Csr<T>& A, const T* D, const T* u, T* z, const SolverState* st, hipStream_t s);
 
@@ -254,6 +288,10 @@ class System {
     HostMirror* mirror32_ = nullptr;
     DevArray<double> r64_;
     DevArray<int> off_rowptr_, off_col_;
+    DevArray<int> sell_off_, sell_col_, sell_map_;
+    DevArray<float> sell_val_;
+    int64_t sell_entries_ = 0;
+    bool sell_built_ = false, sell_valid_ = false, op32_sliced_ = true;
     DevArray<float> off_val_, rowsum32_, diag32_;
     bool diff32_valid_ = false;
     DevArray<int> tpos_;

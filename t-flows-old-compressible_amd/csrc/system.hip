@@ -34,6 +34,13 @@ static int lanes_from_env() {
     return (v == 2 || v == 4 || v == 8) ? v : 1;
 }
 
+// Layout of the FP32 inner operator: TFLOWS_FP32_OP=csr keeps the CSR
+// difference form, anything else the sliced ELL form.
+static bool op32_sliced_from_env() {
+    const char* e = std::getenv("TFLOWS_FP32_OP");
+    return !(e && std::string(e) == "csr");
+}
+
 // Tile order of the SpMV kernels; TFLOWS_TILE_ORDER sets the default.
 static int tile_order_from_env() {
     const char* e = std::getenv("TFLOWS_TILE_ORDER");
@@ -56,7 +63,7 @@ int max_blocks_from_env() {
 System::System(int n, int nonz, const int* acol, const int* arow, const int* adia, bool device_ptrs,
                hipStream_t s, bool allow_reorder)
     : n_(n), nnz_(nonz - 1), grid_(stream_grid(n, max_blocks_from_env())), lanes_(lanes_from_env()),
-      strided_(tile_order_from_env()) {
+      strided_(tile_order_from_env()), op32_sliced_(op32_sliced_from_env()) {
     if (n <= 0) throw std::invalid_argument("N must be positive");
     if (nonz < n + 1) throw std::invalid_argument("NONZ must be at least N + 1 (diagonal entries + 1)");
     rowptr_.ensure(n_ + 1);
@@ -116,6 +123,7 @@ void System::set_values(const double* aval, bool device_ptr, hipStream_t s) {
     levels_.vals64 = levels_.vals32 = false;
     val32_valid_ = false;
     diff32_valid_ = false;
+    sell_valid_ = false;
     transposed_valid_ = false;
     if (!device_ptr) TF_CHECK(hipStreamSynchronize(s));
 }
@@ -129,6 +137,7 @@ void System::set_values_range(int64_t offset, int64_t count, const double* aval,
     levels_.vals64 = levels_.vals32 = false;
     val32_valid_ = false;
     diff32_valid_ = false;
+    sell_valid_ = false;
     transposed_valid_ = false;
     if (!device_ptr) TF_CHECK(hipStreamSynchronize(s));
 }
@@ -213,23 +222,47 @@ const Levels& System::levels() {
 // The level-major triangles (Levels::plan) from the stored pattern.
 void System::build_dic_plan() {
     Levels& L = levels_;
-    if (!dic_plan_enabled()) return;
+    L.mode = dic_sweep_mode();
+    if (L.mode == 0) return;
     std::vector<int> rp(n_ + 1), cl(nnz_), dg(n_);
     TF_CHECK(hipMemcpy(rp.data(), rowptr_.p, sizeof(int) * (n_ + 1), hipMemcpyDeviceToHost));
     TF_CHECK(hipMemcpy(cl.data(), col_.p, sizeof(int) * nnz_, hipMemcpyDeviceToHost));
     TF_CHECK(hipMemcpy(dg.data(), dia_.p, sizeof(int) * n_, hipMemcpyDeviceToHost));
-    const std::vector<int>& order = L.fwd_rows_h;
-    std::vector<int> pos(n_);
-    for (int k = 0; k < n_; ++k) pos[order[k]] = k;
-    std::vector<int> lptr(n_ + 1, 0), uptr(n_ + 1, 0);
-    for (int k = 0; k < n_; ++k) {
-        const int i = order[k];
-        lptr[k + 1] = lptr[k] + (dg[i] - rp[i]);
-        uptr[k + 1] = uptr[k] + (rp[i + 1] - dg[i] - 1);
+    // positions: forward level l occupies [pos_ptr[l], pos_ptr[l+1]), its
+    // start a multiple of the wavefront size; the padding positions (row
+    // -1) have no entries
+    L.pos_ptr.assign(L.nfwd + 1, 0);
+    for (int l = 0; l < L.nfwd; ++l) {
+        const int len = L.fwd_ptr[l + 1] - L.fwd_ptr[l];
+        L.pos_ptr[l + 1] = L.pos_ptr[l] + (len + kWave - 1) / kWave * kWave;
     }
-    std::vector<int> lcol(lptr[n_]), lmap(lptr[n_]), ucol(uptr[n_]), umap(uptr[n_]), bpos(n_);
-    for (int k = 0; k < n_; ++k) {
+    const int np = L.pos_ptr[L.nfwd];
+    std::vector<int> order(np, -1), pos(n_), plev(np, 0);
+    for (int l = 0; l < L.nfwd; ++l)
+        for (int t = L.fwd_ptr[l]; t < L.fwd_ptr[l + 1]; ++t) {
+            const int k = L.pos_ptr[l] + (t - L.fwd_ptr[l]);
+            order[k] = L.fwd_rows_h[t];
+            pos[order[k]] = k;
+        }
+    for (int l = 0; l < L.nfwd; ++l)
+        for (int k = L.pos_ptr[l]; k < L.pos_ptr[l + 1]; ++k) plev[k] = l;
+    std::vector<int> lptr(np + 1, 0), uptr(np + 1, 0);
+    for (int k = 0; k < np; ++k) {
         const int i = order[k];
+        lptr[k + 1] = lptr[k] + (i < 0 ? 0 : dg[i] - rp[i]);
+        uptr[k + 1] = uptr[k] + (i < 0 ? 0 : rp[i + 1] - dg[i] - 1);
+    }
+    std::vector<int> lcol(lptr[np]), lmap(lptr[np]), ucol(uptr[np]), umap(uptr[np]), bpos(n_);
+    // one launch per triangle needs every wavefront's rows free of mutual
+    // dependencies in both sweeps: rows of one forward level are (their
+    // lower entries lie in earlier levels by construction); for the
+    // backward sweep, run over the positions in reverse, the upper entries
+    // must lie in later forward levels, which holds for a structurally
+    // symmetric pattern (TopolM.f90:65-66)
+    bool upper_later = true;
+    for (int k = 0; k < np; ++k) {
+        const int i = order[k];
+        if (i < 0) continue;
         int o = lptr[k];
         for (int j = rp[i]; j < dg[i]; ++j, ++o) {
             lcol[o] = pos[cl[j]];
@@ -239,9 +272,10 @@ void System::build_dic_plan() {
         for (int j = dg[i] + 1; j < rp[i + 1]; ++j, ++o) {
             ucol[o] = pos[cl[j]];
             umap[o] = j;
+            upper_later = upper_later && plev[ucol[o]] > plev[k];
         }
     }
-    for (int k = 0; k < n_; ++k) bpos[k] = pos[L.bwd_rows_h[k]];
+    for (int t = 0; t < n_; ++t) bpos[t] = pos[L.bwd_rows_h[t]];
     auto up = [](DevArray<int>& d, const std::vector<int>& h) {
         d.ensure(h.size());
         if (!h.empty()) TF_CHECK(hipMemcpy(d.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
@@ -254,15 +288,43 @@ void System::build_dic_plan() {
     up(L.ucol, ucol);
     up(L.umap, umap);
     up(L.bwd_pos, bpos);
-    L.nlow = lptr[n_];
-    L.nup = uptr[n_];
+    L.npos = np;
+    L.nlow = lptr[np];
+    L.nup = uptr[np];
+    if (L.mode == 2 && !upper_later) L.mode = 1;
+    if (L.mode == 2) {
+        L.fault.ensure(1);
+        TF_CHECK(hipMemset(L.fault.p, 0, sizeof(int)));
+        int dev = 0, khz = 0;
+        TF_CHECK(hipGetDevice(&dev));
+        TF_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        L.wait_ticks = (uint64_t)(khz > 0 ? khz : 100000) * dic_wait_ms();
+    }
     L.plan = true;
     L.vals64 = L.vals32 = false;
 }
 
-bool dic_plan_enabled() {
-    const char* e = std::getenv("TFLOWS_DIC_PLAN");
-    return !e || std::atoi(e) != 0;
+int dic_sweep_mode() {
+    const char* e = std::getenv("TFLOWS_DIC_SWEEP");
+    if (!e) return 2;
+    const int m = std::atoi(e);
+    return m < 0 ? 0 : (m > 2 ? 2 : m);
+}
+
+uint64_t dic_wait_ms() {
+    const char* e = std::getenv("TFLOWS_DIC_WAIT_MS");
+    const long v = e ? std::atol(e) : 2000;
+    return v < 0 ? 0 : (uint64_t)v;
+}
+
+void System::check_dic(hipStream_t s) const {
+    if (!levels_ready_ || levels_.mode != 2 || !levels_.plan) return;
+    int fault = 0;
+    TF_CHECK(hipMemcpyAsync(&fault, levels_.fault.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    TF_CHECK(hipStreamSynchronize(s));
+    if (fault)
+        throw HipError("DIC sweep: a row waited on its dependencies beyond TFLOWS_DIC_WAIT_MS; the "
+                       "preconditioned vector is invalid");
 }
 
 void System::prepare_dic(bool fp32, hipStream_t s) {
@@ -271,9 +333,10 @@ void System::prepare_dic(bool fp32, hipStream_t s) {
     if (!L.vals64) {
         L.lval.ensure(L.nlow);
         L.uval.ensure(L.nup);
-        L.up.ensure(n_);
-        L.zp.ensure(n_);
-        L.dp.ensure(n_);
+        L.up.ensure(L.npos);
+        L.zp.ensure(L.npos);
+        L.zb.ensure(L.npos);
+        L.dp.ensure(L.npos);
         launch_entry_gather(L.nlow, L.lmap.p, val_.p, L.lval.p, s);
         launch_entry_gather(L.nup, L.umap.p, val_.p, L.uval.p, s);
         L.vals64 = true;
@@ -282,12 +345,18 @@ void System::prepare_dic(bool fp32, hipStream_t s) {
     if (fp32 && !L.vals32) {
         L.lval32.ensure(L.nlow);
         L.uval32.ensure(L.nup);
-        L.up32.ensure(n_);
-        L.zp32.ensure(n_);
-        L.dp32.ensure(n_);
+        L.up32.ensure(L.npos);
+        L.zp32.ensure(L.npos);
+        L.zb32.ensure(L.npos);
+        L.dp32.ensure(L.npos);
         launch_f64_to_f32(L.nlow, L.lval.p, L.lval32.p, s);
         launch_f64_to_f32(L.nup, L.uval.p, L.uval32.p, s);
         L.vals32 = true;
+    }
+    if (L.mode == 2) {   // a solve starts: no fault, every forward slot empty
+        launch_set_int(L.fault.p, 0, s);
+        launch_flow_reset(L.npos, L.zp.p, s);
+        if (fp32) launch_flow_reset(L.npos, L.zp32.p, s);
     }
 }
 
@@ -305,18 +374,25 @@ void dic_apply_levels(const Levels& L, const Csr<T>& A, const T* D, const T* u, 
     if (L.plan && (f32 ? L.vals32 : L.vals64)) {
         T* up = f32 ? (T*)L.up32.p : (T*)L.up.p;
         T* zp = f32 ? (T*)L.zp32.p : (T*)L.zp.p;
+        T* zb = f32 ? (T*)L.zb32.p : (T*)L.zb.p;
         T* dp = f32 ? (T*)L.dp32.p : (T*)L.dp.p;
         const T* lv = f32 ? (const T*)L.lval32.p : (const T*)L.lval.p;
         const T* uv = f32 ? (const T*)L.uval32.p : (const T*)L.uval.p;
-        const int n = A.n;
-        launch_plan_in(n, L.order.p, u, D, up, dp, st, s);
+        const int np = L.npos;
+        if (L.mode == 2) {
+            launch_flow_fwd(np, L.order.p, u, D, L.lptr.p, L.lcol.p, lv, dp, zp, zb, L.fault.p, L.wait_ticks, st, s);
+            launch_flow_bwd(np, L.order.p, z, L.uptr.p, L.ucol.p, uv, (const T*)dp, zp, zb, L.fault.p, L.wait_ticks,
+                            st, s);
+            return;
+        }
+        launch_plan_in(np, L.order.p, u, D, up, dp, st, s);
         for (int l = 0; l < L.nfwd; ++l)
-            launch_plan_fwd(L.fwd_ptr[l], L.fwd_ptr[l + 1], L.lptr.p, L.lcol.p, lv, (const T*)dp, (const T*)up, zp,
+            launch_plan_fwd(L.pos_ptr[l], L.pos_ptr[l + 1], L.lptr.p, L.lcol.p, lv, (const T*)dp, (const T*)up, zp,
                             st, s);
         for (int l = 0; l < L.nbwd; ++l)
             launch_plan_bwd(L.bwd_pos.p + L.bwd_ptr[l], L.bwd_ptr[l + 1] - L.bwd_ptr[l], L.uptr.p, L.ucol.p, uv,
                             (const T*)dp, zp, st, s);
-        launch_plan_out(n, L.order.p, (const T*)zp, z, st, s);
+        launch_plan_out(np, L.order.p, (const T*)zp, z, st, s);
         return;
     }
     for (int l = 0; l < L.nfwd; ++l)
@@ -362,18 +438,19 @@ void System::spmv(const double* x, double* y, hipStream_t s) {
     to_external((const double*)yi, y, s);
 }
 void System::spmv32(const float* x, float* y, hipStream_t s) {
-    const DiffCsr32 A = diff32(s);
-    if (!reordered_) {
-        launch_spmv(A, x, y, grid_, s, ctx());
+    with_op32(s, [&](const auto& A) {
+        if (!reordered_) {
+            launch_spmv(A, x, y, grid_, s, ctx());
+            TF_CHECK(hipGetLastError());
+            return;
+        }
+        extbuf32_.ensure(2 * (size_t)n_);
+        float *xi = extbuf32_.p, *yi = extbuf32_.p + n_;
+        to_internal(x, xi, s);
+        launch_spmv(A, (const float*)xi, yi, grid_, s, ctx());
         TF_CHECK(hipGetLastError());
-        return;
-    }
-    extbuf32_.ensure(2 * (size_t)n_);
-    float *xi = extbuf32_.p, *yi = extbuf32_.p + n_;
-    to_internal(x, xi, s);
-    launch_spmv(A, (const float*)xi, yi, grid_, s, ctx());
-    TF_CHECK(hipGetLastError());
-    to_external((const float*)yi, y, s);
+        to_external((const float*)yi, y, s);
+    });
 }
 void System::resid(const double* x, double* r, hipStream_t s) {
     if (!reordered_) {
@@ -484,6 +561,7 @@ bool dic_graph_enabled() {
 static const void* op_key(const Csr<double>& a) { return a.val; }
 static const void* op_key(const Csr<float>& a) { return a.val; }
 static const void* op_key(const DiffCsr32& a) { return a.val; }
+static const void* op_key(const DiffSell32& a) { return a.val; }
 
 // One DIC iteration of one system replayed from a graph until the system
 // stops (or `cap` iterations were issued; then the k = cap + 1 test).
@@ -682,6 +760,7 @@ SolveResult System::solve_impl(int method, int prec, double* x, double* r1, int 
     SolverState h2;
     TF_CHECK(hipMemcpy(&h2, state_.p, sizeof(h2), hipMemcpyDeviceToHost));
     if (!h2.done) throw HipError("solver finished without reaching a stopping state");
+    if (prec == 2) check_dic(s);
     SolveResult res;
     res.niter = h2.niter_out;
     res.ini_written = h2.ini_written != 0;
@@ -724,6 +803,7 @@ KrylovVectors<double> System::vectors(double* x, double* r1) {
 TF_SOLVE_INST(double, Csr<double>)
 TF_SOLVE_INST(float, Csr<float>)
 TF_SOLVE_INST(float, DiffCsr32)
+TF_SOLVE_INST(float, DiffSell32)
 template void dic_apply_levels<double>(const Levels&, const Csr<double>&, const double*, const double*,
                                        double*, const SolverState*, hipStream_t);
 template void dic_apply_levels<float>(const Levels&, const Csr<float>&, const float*, const float*, float*,

@@ -121,6 +121,9 @@ def _bind_extensions(lib):
     if hasattr(lib, "tflows_system_reordered"):
         lib.tflows_system_reordered.restype = C.c_int32
         lib.tflows_system_reordered.argtypes = [vp]
+    if hasattr(lib, "tflows_system_fp32_layout"):
+        lib.tflows_system_fp32_layout.restype = C.c_int
+        lib.tflows_system_fp32_layout.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), vp]
     if hasattr(lib, "tflows_batch_create"):
         pp = C.POINTER(vp)
         lib.tflows_batch_create.restype = C.c_int
@@ -331,6 +334,13 @@ class DeviceSystem:
     def reordered(self) -> bool:
         """True when the matrix is stored in a locality order internally."""
         return bool(self.lib.tflows_system_reordered(self.handle))
+
+    def fp32_layout(self, stream=None) -> dict:
+        """Layout of the FP32 inner operator: {"sliced": bool,
+        "stored_entries": off-diagonal entries held, padding included}."""
+        sl, ent = C.c_int32(0), C.c_int64(0)
+        _check(self.lib.tflows_system_fp32_layout(self.handle, C.byref(sl), C.byref(ent), _stream(stream)))
+        return {"sliced": bool(sl.value), "stored_entries": int(ent.value)}
 
     def set_lanes(self, lanes: int):
         """SpMV form: lanes per row (1, 2, 4 or 8); results are identical."""
