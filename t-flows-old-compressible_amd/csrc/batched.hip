@@ -447,7 +447,10 @@ BatchConfig Batch::tune(int bits, int method, int prec, const Op& Aop, const Csr
             TF_CHECK(hipStreamSynchronize(bs_));
             build_maps(all, G);
             upload_maps(bs_);
+            // the sliced operators have one form; the CSR ones four
+            constexpr bool lane_forms = std::is_same<Op, Csr<T>>::value || std::is_same<Op, DiffCsr32>::value;
             for (int lanes : kLanes) {
+                if (!lane_forms && lanes != 1) continue;
                 const BatchCtx c = ctx(lanes, false);
                 const Sites<T> S = Sites<T>::carve(partials, G);
                 prepare();
@@ -606,7 +609,10 @@ void Batch::solve(int method, int prec, double* x, double* r1, const int* cap, c
     };
     const Csr<double> A = sys_->csr();
     std::vector<SolverState> out;
-    run<double, Csr<double>>(64, method, prec, A, A, w, partials_.p, st_.p, hm_, init, false, false, prepare, out);
+    sys_->with_op64(bs_, [&](const auto& Aop) {
+        run<double, std::decay_t<decltype(Aop)>>(64, method, prec, Aop, A, w, partials_.p, st_.p, hm_, init, false,
+                                                 false, prepare, out);
+    });
     scatter_out(w.x, x);
     scatter_out(w.r1, r1);
     TF_CHECK(hipStreamSynchronize(bs_));

@@ -384,6 +384,12 @@ template void launch_cgj<DiffCsr32, float>(bool, bool, const DiffCsr32&, int, fl
                                            const float*, const Sites<float>&, const double*, SolverState*,
                                            HostMirror*, hipStream_t);
 
+template void launch_cgj<Sell<double>, double>(bool, bool, const Sell<double>&, int, double*, double*, double*,
+                                               double*, double*, const double*, const Sites<double>&, const double*,
+                                               SolverState*, HostMirror*, hipStream_t);
+template void launch_cgf_spmv<Sell<double>, double>(bool, const Sell<double>&, const double*, double*, double*,
+                                                    double*, const Sites<double>&, const double*, SolverState*,
+                                                    HostMirror*, hipStream_t);
 template void launch_cgj<DiffSell32, float>(bool, bool, const DiffSell32&, int, float*, float*, float*, float*,
                                             float*, const float*, const Sites<float>&, const double*, SolverState*,
                                             HostMirror*, hipStream_t);

@@ -140,8 +140,24 @@ struct DiffSell32 {
 };
 constexpr int kSlice = 64;
 
+// A matrix in sliced ELL layout: rows in slices of 64 (one wavefront), the
+// entries of a slice stored slot-major -- slot j of row 64 s + l at
+// off[s] + 64 j + l -- each row's entries in their CSR order (diagonal
+// included where it stands), padded to the slice's longest row with column
+// -1, which is skipped.  One thread per row sums its products in that
+// order, as the reference loop does, so the result is the CSR stream
+// form's bit for bit; the loads of a wavefront are contiguous segments.
+template <typename T>
+struct Sell {
+    int n;
+    const int* __restrict__ off;     // ceil(n/64)+1 slice offsets
+    const int* __restrict__ col;     // -1: padding
+    const T* __restrict__ val;
+};
+
 template <typename Op> struct OpValue;
 template <typename T> struct OpValue<Csr<T>> { using type = T; };
+template <typename T> struct OpValue<Sell<T>> { using type = T; };
 template <> struct OpValue<DiffCsr32> { using type = float; };
 template <> struct OpValue<DiffSell32> { using type = float; };
 
@@ -545,6 +561,43 @@ __device__ __forceinline__ void sell_tile(const DiffSell32& A, int r0, int r1, G
 template <int NRHS, int L = 1, class Gather, class Epi>
 __device__ __forceinline__ void op_tile(const DiffSell32& A, int r0, int r1, Gather& g, Epi& epi) {
     sell_tile<NRHS>(A, r0, r1, g, epi);
+}
+
+// Rows [r0, r1) of a sliced matrix, row r0 + t on thread t: products
+// rounded, then added to 0 (or subtracted from init) in CSR order.
+template <typename T, int NRHS, bool SUBTRACT, class Gather, class Init, class Epi>
+__device__ __forceinline__ void sell_rows(const Sell<T>& A, int r0, int r1, Gather& g, Init& init, Epi& epi) {
+    const int row = r0 + (int)threadIdx.x;
+    if (row < r1) {
+        const int sl = row / kSlice;
+        const int b = A.off[sl];
+        const int w = (A.off[sl + 1] - b) / kSlice;
+        const int* __restrict__ cp = A.col + b + (row - sl * kSlice);
+        const T* __restrict__ vp = A.val + b + (row - sl * kSlice);
+        T acc[NRHS];
+#pragma unroll
+        for (int v = 0; v < NRHS; ++v) acc[v] = SUBTRACT ? init(row, v) : T(0);
+#pragma unroll 2
+        for (int j = 0; j < w; ++j) {
+            const int c = cp[j * kSlice];
+            if (c >= 0) {
+                const T a = vp[j * kSlice];
+                T gv[NRHS];
+                g(c, gv);
+#pragma unroll
+                for (int v = 0; v < NRHS; ++v) {
+                    const T prod = a * gv[v];
+                    acc[v] = SUBTRACT ? acc[v] - prod : acc[v] + prod;
+                }
+            }
+        }
+        epi(row, acc);
+    }
+}
+template <int NRHS, int L = 1, typename T, class Gather, class Epi>
+__device__ __forceinline__ void op_tile(const Sell<T>& A, int r0, int r1, Gather& g, Epi& epi) {
+    InitZero<T> z;
+    sell_rows<T, NRHS, false>(A, r0, r1, g, z, epi);
 }
 
 // r = b - A x over one tile (the residual form: subtraction from b).

@@ -1208,5 +1208,6 @@ TF_INSTANTIATE_OP(Csr<double>, double)
 TF_INSTANTIATE_OP(Csr<float>, float)
 TF_INSTANTIATE_OP(DiffCsr32, float)
 TF_INSTANTIATE_OP(DiffSell32, float)
+TF_INSTANTIATE_OP(Sell<double>, double)
 
 }  // namespace tflows

@@ -211,40 +211,20 @@ __global__ __launch_bounds__(kBlock) void k_sell_values(int64_t count, const int
 DiffSell32 System::sell32(hipStream_t s) {
     const DiffCsr32 d = diff32(s);
     if (!sell_built_) {
-        // slice layout from the off-diagonal pattern (once per pattern)
+        // slice layout of the off-diagonal pattern (once per pattern);
+        // padding slots point at the row itself with a = 0
         const int64_t noff = nnz_ - n_;
-        std::vector<int> orp(n_ + 1), ocol(noff > 0 ? noff : 0);
+        std::vector<int> orp(n_ + 1), ocol(noff > 0 ? noff : 0), off, col, map;
         TF_CHECK(hipMemcpyAsync(orp.data(), d.rowptr, sizeof(int) * (n_ + 1), hipMemcpyDeviceToHost, s));
         if (noff > 0)
             TF_CHECK(hipMemcpyAsync(ocol.data(), d.col, sizeof(int) * noff, hipMemcpyDeviceToHost, s));
         TF_CHECK(hipStreamSynchronize(s));
-        const int nsl = (n_ + kSlice - 1) / kSlice;
-        std::vector<int> off(nsl + 1, 0);
-        for (int sl = 0; sl < nsl; ++sl) {
-            int w = 0;
-            for (int i = sl * kSlice; i < std::min(n_, (sl + 1) * kSlice); ++i) w = std::max(w, orp[i + 1] - orp[i]);
-            if ((int64_t)off[sl] + (int64_t)w * kSlice > INT32_MAX)
-                throw std::invalid_argument("sliced operator exceeds 2^31 entries");
-            off[sl + 1] = off[sl] + w * kSlice;
-        }
-        std::vector<int> col(off[nsl]), map(off[nsl]);
-        for (int sl = 0; sl < nsl; ++sl) {
-            const int w = (off[sl + 1] - off[sl]) / kSlice;
-            for (int l = 0; l < kSlice; ++l) {
-                const int i = sl * kSlice + l;
-                const int len = i < n_ ? orp[i + 1] - orp[i] : 0;
-                for (int j = 0; j < w; ++j) {
-                    const int e = off[sl] + j * kSlice + l;
-                    col[e] = j < len ? ocol[orp[i] + j] : (i < n_ ? i : 0);
-                    map[e] = j < len ? orp[i] + j : -1;
-                }
-            }
-        }
-        sell_off_.ensure(nsl + 1);
+        slice_layout(n_, orp, ocol, true, off, col, map);
+        sell_off_.ensure(off.size());
         sell_col_.ensure(col.size());
         sell_map_.ensure(map.size());
         sell_val_.ensure(col.size());
-        TF_CHECK(hipMemcpy(sell_off_.p, off.data(), sizeof(int) * (nsl + 1), hipMemcpyHostToDevice));
+        TF_CHECK(hipMemcpy(sell_off_.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
         if (!col.empty()) {
             TF_CHECK(hipMemcpy(sell_col_.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
             TF_CHECK(hipMemcpy(sell_map_.p, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));

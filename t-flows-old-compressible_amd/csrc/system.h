@@ -102,6 +102,13 @@ struct Levels {
     mutable bool vals64 = false, vals32 = false;
 };
 
+// Sliced ELL layout (kernels.h, Sell / DiffSell32) of a CSR pattern with
+// row pointer rp and columns cl: slice offsets, the column of every slot
+// and the CSR entry it holds (-1 for padding).  Padding slots take the
+// row's own index when pad_own, else -1.
+void slice_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl, bool pad_own, std::vector<int>& off,
+                  std::vector<int>& col, std::vector<int>& map);
+
 // TFLOWS_DIC_SWEEP (read when a system first builds its levels): 0 sweeps
 // the matrix rows level by level, 1 the level-major copy level by level,
 // 2 (default) the level-major copy one launch per triangle.  All three
@@ -194,6 +201,16 @@ class System {
         else f(diff32(s));
     }
     bool op32_sliced() const { return op32_sliced_; }
+    // The FP64 operator of the solver iterations: A in sliced ELL layout
+    // (same row sums bit for bit), or the CSR arrays themselves when
+    // TFLOWS_FP64_OP=csr was set when the system was created.
+    Sell<double> sell64(hipStream_t s);
+    template <class F>
+    void with_op64(hipStream_t s, F&& f) {
+        if (op64_sliced_) f(sell64(s));
+        else f(csr());
+    }
+    bool op64_sliced() const { return op64_sliced_; }
     int64_t sell32_entries(hipStream_t s) {
         sell32(s);
         return sell_entries_;
@@ -292,6 +309,10 @@ class System {
     DevArray<float> sell_val_;
     int64_t sell_entries_ = 0;
     bool sell_built_ = false, sell_valid_ = false, op32_sliced_ = true;
+    DevArray<int> sell64_off_, sell64_col_, sell64_map_;
+    DevArray<double> sell64_val_;
+    int64_t sell64_entries_ = 0;
+    bool sell64_built_ = false, sell64_valid_ = false, op64_sliced_ = true;
     DevArray<float> off_val_, rowsum32_, diag32_;
     bool diff32_valid_ = false;
     DevArray<int> tpos_;

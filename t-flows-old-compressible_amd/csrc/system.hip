@@ -40,6 +40,77 @@ static bool op32_sliced_from_env() {
     const char* e = std::getenv("TFLOWS_FP32_OP");
     return !(e && std::string(e) == "csr");
 }
+// The same for the FP64 operator (TFLOWS_FP64_OP=csr).
+static bool op64_sliced_from_env() {
+    const char* e = std::getenv("TFLOWS_FP64_OP");
+    return !(e && std::string(e) == "csr");
+}
+
+void slice_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl, bool pad_own, std::vector<int>& off,
+                  std::vector<int>& col, std::vector<int>& map) {
+    const int nsl = (n + kSlice - 1) / kSlice;
+    off.assign(nsl + 1, 0);
+    for (int sl = 0; sl < nsl; ++sl) {
+        int w = 0;
+        for (int i = sl * kSlice; i < std::min(n, (sl + 1) * kSlice); ++i) w = std::max(w, rp[i + 1] - rp[i]);
+        if ((int64_t)off[sl] + (int64_t)w * kSlice > INT32_MAX)
+            throw std::invalid_argument("sliced layout exceeds 2^31 slots");
+        off[sl + 1] = off[sl] + w * kSlice;
+    }
+    col.assign(off[nsl], 0);
+    map.assign(off[nsl], -1);
+    for (int sl = 0; sl < nsl; ++sl) {
+        const int w = (off[sl + 1] - off[sl]) / kSlice;
+        for (int l = 0; l < kSlice; ++l) {
+            const int i = sl * kSlice + l;
+            const int len = i < n ? rp[i + 1] - rp[i] : 0;
+            for (int j = 0; j < w; ++j) {
+                const int e = off[sl] + j * kSlice + l;
+                if (j < len) {
+                    col[e] = cl[rp[i] + j];
+                    map[e] = rp[i] + j;
+                } else {
+                    col[e] = pad_own ? (i < n ? i : 0) : -1;
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_slot_values(int64_t count, const int* __restrict__ map,
+                                                        const T* __restrict__ src, T* __restrict__ dst) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e < count) dst[e] = map[e] < 0 ? T(0) : src[map[e]];
+}
+
+Sell<double> System::sell64(hipStream_t s) {
+    if (!sell64_built_) {
+        std::vector<int> rp(n_ + 1), cl(nnz_), off, col, map;
+        TF_CHECK(hipMemcpyAsync(rp.data(), rowptr_.p, sizeof(int) * (n_ + 1), hipMemcpyDeviceToHost, s));
+        TF_CHECK(hipMemcpyAsync(cl.data(), col_.p, sizeof(int) * nnz_, hipMemcpyDeviceToHost, s));
+        TF_CHECK(hipStreamSynchronize(s));
+        slice_layout(n_, rp, cl, false, off, col, map);
+        sell64_off_.ensure(off.size());
+        sell64_col_.ensure(col.size());
+        sell64_map_.ensure(map.size());
+        sell64_val_.ensure(col.size());
+        TF_CHECK(hipMemcpy(sell64_off_.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+        TF_CHECK(hipMemcpy(sell64_col_.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
+        TF_CHECK(hipMemcpy(sell64_map_.p, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));
+        sell64_entries_ = (int64_t)col.size();
+        sell64_built_ = true;
+        sell64_valid_ = false;
+    }
+    if (!sell64_valid_) {
+        hipLaunchKernelGGL((k_slot_values<double>), dim3((unsigned)((sell64_entries_ + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, s, sell64_entries_, (const int*)sell64_map_.p, (const double*)val_.p,
+                           sell64_val_.p);
+        TF_CHECK(hipGetLastError());
+        sell64_valid_ = true;
+    }
+    return Sell<double>{n_, sell64_off_.p, sell64_col_.p, sell64_val_.p};
+}
 
 // Tile order of the SpMV kernels; TFLOWS_TILE_ORDER sets the default.
 static int tile_order_from_env() {
@@ -63,7 +134,7 @@ int max_blocks_from_env() {
 System::System(int n, int nonz, const int* acol, const int* arow, const int* adia, bool device_ptrs,
                hipStream_t s, bool allow_reorder)
     : n_(n), nnz_(nonz - 1), grid_(stream_grid(n, max_blocks_from_env())), lanes_(lanes_from_env()),
-      strided_(tile_order_from_env()), op32_sliced_(op32_sliced_from_env()) {
+      strided_(tile_order_from_env()), op32_sliced_(op32_sliced_from_env()), op64_sliced_(op64_sliced_from_env()) {
     if (n <= 0) throw std::invalid_argument("N must be positive");
     if (nonz < n + 1) throw std::invalid_argument("NONZ must be at least N + 1 (diagonal entries + 1)");
     rowptr_.ensure(n_ + 1);
@@ -124,6 +195,7 @@ void System::set_values(const double* aval, bool device_ptr, hipStream_t s) {
     val32_valid_ = false;
     diff32_valid_ = false;
     sell_valid_ = false;
+    sell64_valid_ = false;
     transposed_valid_ = false;
     if (!device_ptr) TF_CHECK(hipStreamSynchronize(s));
 }
@@ -138,6 +210,7 @@ void System::set_values_range(int64_t offset, int64_t count, const double* aval,
     val32_valid_ = false;
     diff32_valid_ = false;
     sell_valid_ = false;
+    sell64_valid_ = false;
     transposed_valid_ = false;
     if (!device_ptr) TF_CHECK(hipStreamSynchronize(s));
 }
@@ -426,16 +499,18 @@ double* System::ext_buffer(int k) {
 Sites<double> System::sites() { return Sites<double>::carve(partials_.p, grid_); }
 
 void System::spmv(const double* x, double* y, hipStream_t s) {
-    if (!reordered_) {
-        launch_spmv(csr(), x, y, grid_, s, ctx());
+    with_op64(s, [&](const auto& A) {
+        if (!reordered_) {
+            launch_spmv(A, x, y, grid_, s, ctx());
+            TF_CHECK(hipGetLastError());
+            return;
+        }
+        double *xi = ext_buffer(0), *yi = ext_buffer(1);
+        to_internal(x, xi, s);
+        launch_spmv(A, (const double*)xi, yi, grid_, s, ctx());
         TF_CHECK(hipGetLastError());
-        return;
-    }
-    double *xi = ext_buffer(0), *yi = ext_buffer(1);
-    to_internal(x, xi, s);
-    launch_spmv(csr(), (const double*)xi, yi, grid_, s, ctx());
-    TF_CHECK(hipGetLastError());
-    to_external((const double*)yi, y, s);
+        to_external((const double*)yi, y, s);
+    });
 }
 void System::spmv32(const float* x, float* y, hipStream_t s) {
     with_op32(s, [&](const auto& A) {
@@ -562,6 +637,7 @@ static const void* op_key(const Csr<double>& a) { return a.val; }
 static const void* op_key(const Csr<float>& a) { return a.val; }
 static const void* op_key(const DiffCsr32& a) { return a.val; }
 static const void* op_key(const DiffSell32& a) { return a.val; }
+static const void* op_key(const Sell<double>& a) { return a.val; }
 
 // One DIC iteration of one system replayed from a graph until the system
 // stops (or `cap` iterations were issued; then the k = cap + 1 test).
@@ -755,8 +831,10 @@ SolveResult System::solve_impl(int method, int prec, double* x, double* r1, int 
     const Levels* L = prec == 2 ? &levels() : &kNoLevels;
     if (prec == 2) prepare_dic(false, s);
     const Csr<double> A = csr();
-    run_solve<double, Csr<double>>(ctx(), method, prec, cap, A, A, w, sites(), state_.p, mirror_, L, s, false,
-                                   false, &replay_);
+    with_op64(s, [&](const auto& Aop) {
+        run_solve<double, std::decay_t<decltype(Aop)>>(ctx(), method, prec, cap, Aop, A, w, sites(), state_.p,
+                                                       mirror_, L, s, false, false, &replay_);
+    });
     SolverState h2;
     TF_CHECK(hipMemcpy(&h2, state_.p, sizeof(h2), hipMemcpyDeviceToHost));
     if (!h2.done) throw HipError("solver finished without reaching a stopping state");
@@ -804,6 +882,7 @@ TF_SOLVE_INST(double, Csr<double>)
 TF_SOLVE_INST(float, Csr<float>)
 TF_SOLVE_INST(float, DiffCsr32)
 TF_SOLVE_INST(float, DiffSell32)
+TF_SOLVE_INST(double, Sell<double>)
 template void dic_apply_levels<double>(const Levels&, const Csr<double>&, const double*, const double*,
                                        double*, const SolverState*, hipStream_t);
 template void dic_apply_levels<float>(const Levels&, const Csr<float>&, const float*, const float*, float*,

@@ -120,11 +120,23 @@ def test_batch_per_system_caps_and_early_exit(pkg, oracle, cuda, mixed_sizes):
     assert _rel(x0, ref0["x"]) <= 1e-11
 
 
-def test_spmv_lane_forms_bitwise(pkg, oracle, cuda):
-    """q = A p is bitwise identical for 1, 2, 4, 8 lanes per row and equal to
-    the oracle's sequential row sums (CG.f90:193-199); the residual form
-    likewise."""
+def _fp64_op(monkeypatch, form):
+    """Systems created from here on use the FP64 operator `form`: "csr"
+    (the LDS-staged CSR kernels, which have the lane-group forms) or
+    "sliced" (the default sliced ELL kernels, one form)."""
+    if form == "csr":
+        monkeypatch.setenv("TFLOWS_FP64_OP", "csr")
+    else:
+        monkeypatch.delenv("TFLOWS_FP64_OP", raising=False)
+
+
+@pytest.mark.parametrize("form", ["csr", "sliced"])
+def test_spmv_lane_forms_bitwise(pkg, oracle, cuda, monkeypatch, form):
+    """q = A p is bitwise identical for 1, 2, 4, 8 lanes per row (CSR
+    operator) and for the sliced operator, and equal to the oracle's
+    sequential row sums (CG.f90:193-199); the residual form likewise."""
     import torch
+    _fp64_op(monkeypatch, form)
     mg = pkg.meshgen
     for s in (mg.pressure_system(24, seed=31, perm_seed=5), mg.momentum_system(17, seed=32)):
         m = s.matrix
@@ -148,11 +160,13 @@ def test_spmv_lane_forms_bitwise(pkg, oracle, cuda):
         D.close()
 
 
-def test_single_solve_lane_forms(pkg, oracle, cuda):
+@pytest.mark.parametrize("form", ["csr", "sliced"])
+def test_single_solve_lane_forms(pkg, oracle, cuda, monkeypatch, form):
     """A single-system solve in every SpMV form: the same stopping state and
     x within the parity bar of the oracle; the same form twice gives the
     same bits."""
     import torch
+    _fp64_op(monkeypatch, form)
     s = pkg.meshgen.pressure_system(20, seed=33)
     ref = oracle.run("cg", s.matrix, s.b, prec=1, niter=1000, tol=s.tol)
     D = pkg.krylov.DeviceSystem(s.matrix)
@@ -171,10 +185,12 @@ def test_single_solve_lane_forms(pkg, oracle, cuda):
     D.close()
 
 
-def test_batch_lane_forms(pkg, oracle, cuda, mixed_sizes):
+@pytest.mark.parametrize("form", ["csr", "sliced"])
+def test_batch_lane_forms(pkg, oracle, cuda, mixed_sizes, monkeypatch, form):
     """A batched solve in every SpMV form stays within the parity bar;
     repeated solves in one configuration (graph replays) give identical
     bits."""
+    _fp64_op(monkeypatch, form)
     systems = mixed_sizes[:5]
     refs = [oracle.run("bicg", s.matrix, s.b, prec=1, niter=1000, tol=s.tol) for s in systems]
     B = None
