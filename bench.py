@@ -278,6 +278,89 @@ def workload_dic_sweep(args, torch, pkg):
     return out
 
 
+def calcps_bytes(case):
+    """Algorithmic bytes of one pressure-correction assembly: every input
+    read once (per face SideC(1:2), SidAij(1:2), the face kind, S, d, f:
+    73 B; eleven (-NbC:NC) cell arrays, volume and the ten fields), every
+    output written once (Aval, b, Flux)."""
+    ncell = case.nbc + case.nc + 1
+    return 73 * case.ns + 8 * 11 * ncell + 8 * case.nonz + 8 * case.nc + 8 * case.ns
+
+
+def workload_calcps(args, torch, pkg):
+    """Pressure-correction assembly (CalcPS.f90:51-202) of the n^3 channel
+    box on the GPU, into the device system, then the pressure solve
+    (cap 800, CalcPS.f90:214) -- against assembling on the host and
+    uploading the values for each solve."""
+    n = args.n
+    case = pkg.meshgen.calcps_case((n, n, n), seed=1)
+    dc = pkg.krylov.DeviceCalcPS(case)
+    fields = {k: torch.from_numpy(getattr(case, k)).cuda() for k in pkg.krylov.CALCPS_FIELDS}
+    out = None
+    for _ in range(3):
+        got = dc.assemble(fields, out=out)
+        out = {k: got[k] for k in ("aval", "b", "flux", "abou")}
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record(s)
+    for _ in range(reps):
+        dc.assemble(fields, out=out)
+    e1.record(s)
+    e1.synchronize()
+    t_asm = e0.elapsed_time(e1) / reps
+    nbytes = calcps_bytes(case)
+    # solve on the assembled values, device resident (x0 = 0, cap 800)
+    m = case.matrix(out["aval"].cpu().numpy())
+    ds = pkg.krylov.DeviceSystem(m)
+    # the pressure correction is not solved to 1e-8 in the SIMPLE loop
+    # (CalcPS.f90:173-178); 1e-6 of ||b|| within the cap of 800
+    tol = 1e-6 * float(torch.linalg.norm(out["b"]))
+    x = torch.zeros(case.nc, dtype=torch.float64, device="cuda")
+    r = torch.empty_like(out["b"])
+
+    def gpu_path():
+        dc.assemble(fields, out=out)
+        ds.set_values(out["aval"])
+        x.zero_(); r.copy_(out["b"])
+        return ds.solve("cg", 1, x, r, 800, tol)
+
+    host_aval = out["aval"].cpu().numpy()
+    host_b = out["b"].cpu().numpy()
+
+    def upload_path():
+        # the values as a host assembly would hand them over: one upload of
+        # Aval and b per solve
+        ds.lib.tflows_system_set_values(ds.handle, host_aval.ctypes.data, 0, pkg.krylov._stream())
+        x.zero_(); r.copy_(torch.from_numpy(host_b).cuda())
+        return ds.solve("cg", 1, x, r, 800, tol)
+    gpu_path()
+    ms_gpu, (it, ini, fin) = _timed(torch, gpu_path, args.steps)
+    upload_path()
+    ms_up, (it2, _, fin2) = _timed(torch, upload_path, args.steps)
+    res = {"metric": "pressure-correction assembly on the GPU, and assembly + pressure solve", "unit": "ms",
+           "workload": f"channel {n}^3 (NC={case.nc}, NS={case.ns}, NbC={case.nbc}, NONZ={case.nonz}), "
+                       f"inflow/outflow, walls, symmetry",
+           "assembly_ms": round(t_asm, 4), "assembly_bytes": nbytes,
+           "assembly_GBs": round(nbytes / (t_asm * 1e-3) / 1e9, 1),
+           "roofline": {"bound": "hbm", "achieved": round(nbytes / (t_asm * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(nbytes / (t_asm * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+           "assemble_and_solve_ms": round(ms_gpu, 2), "solve_iters": it, "converged": bool(fin < tol),
+           "upload_and_solve_ms": round(ms_up, 2), "upload_iters": it2}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        orc = O.Oracle()
+        t0 = time.perf_counter()
+        orc.calcps(case)
+        el = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(el * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": f"one assembly of the same {n}^3 case by oracle/calcps_oracle.c, 1 thread"}
+    ds.close()
+    dc.close()
+    return res
+
+
 def workload_wall(args, torch, pkg):
     """256^3 wall-clustered pressure system (aspect up to 50:1): FP64 CG
     with Jacobi and with DIC, mixed CG; SpMV and DIC-iteration bandwidth."""
@@ -351,7 +434,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--workload", default="pressure",
                     choices=["pressure", "momentum", "batch-transport", "batch-pressure", "wall", "dic-sweep",
-                             "adjoint"],
+                             "adjoint", "calcps"],
                     help="pressure: the headline 128^3 benchmark; the others measure the remaining "
                          "configurations and print their own JSON line")
     ap.add_argument("--method", default=None, help="Krylov method of the batch workloads")
@@ -378,6 +461,8 @@ def main():
             out = workload_momentum(args, torch, pkg)
         elif args.workload.startswith("batch"):
             out = workload_batch(args, torch, pkg, args.workload.split("-")[1])
+        elif args.workload == "calcps":
+            out = workload_calcps(args, torch, pkg)
         elif args.workload in ("wall", "dic-sweep"):
             if args.n == 128:
                 args.n = 256

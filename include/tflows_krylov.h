@@ -237,6 +237,43 @@ int tflows_batch_solve_mixed(tflows_batch b, int32_t method, int32_t prec, doubl
                              int32_t max_outer, int32_t* niter, double* ini_res, double* fin_res,
                              int32_t* outer_steps, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Pressure-correction assembly on the device: the step before the pressure
+ * solve (Process/CalcPS.f90:51-202), which fills Aval, the mass source b and
+ * the face fluxes from the face list, the geometry and the cell fields.
+ * The reference keeps these in modules (all_mod, pro_mod); here they are
+ * passed with the reference's array bases:
+ *   sidec   SideC(0:2,NS), column-major: sidec[3*(s-1) + k] = SideC(k,s)
+ *   sidaij  SidAij(1:2,NS): sidaij[2*(s-1) + k-1] (1-based positions in
+ *           Aval; only interior faces are read)
+ *   adia    Adia(NC)
+ *   typebc  TypeBC(-NbC:NC): typebc[NbC + c]   (codes of allp_mod.f90:16-24)
+ *   sx..dz, f   (NS)
+ *   volume and every cell field: (-NbC:NC), x[NbC + c]
+ *   Abou    (-NbC:-1): abou[NbC + c], NbC entries
+ * tflows_calcps_create keeps the mesh part on the device (once, after
+ * TopolM); tflows_calcps_assemble takes the fields of one outer iteration
+ * and writes Aval(NONZ), b(NC), Flux(NS) and Abou like CalcPS.f90:51-202,
+ * with errmax (:172-175) and the removed mean bc_sum/vol (:201-204).
+ * Diagonals, b before the mean, off-diagonals, Flux, Abou and errmax are
+ * the reference's bit for bit; bc_sum is summed as a tree, so the mean and
+ * b differ from a sequential sum by its rounding.  With mem =
+ * TFLOWS_MEM_DEVICE every array argument is a device pointer, so the
+ * values go straight to tflows_system_set_values and the solve; with
+ * TFLOWS_MEM_HOST they are copied.  Replaces the face and cell loops of
+ * CalcPS; the solve (CalcPS.f90:214-219) stays tflows_solve / cg_. */
+typedef struct tflows_calcps_s* tflows_calcps;
+int tflows_calcps_create(int32_t nc, int32_t nbc, int32_t ns, int32_t nonz, const int32_t* sidec,
+                         const int32_t* sidaij, const int32_t* adia, const int32_t* typebc, const double* sx,
+                         const double* sy, const double* sz, const double* dx, const double* dy, const double* dz,
+                         const double* f, const double* volume, int32_t mem, void* stream, tflows_calcps* out);
+int tflows_calcps_destroy(tflows_calcps h);
+int tflows_calcps_assemble(tflows_calcps h, int32_t nmat, double dt, const double* asave, const double* rho_n,
+                           const double* rho_o, const double* rho_oo, const double* u, const double* v,
+                           const double* w, const double* p, const double* px, const double* py, const double* pz,
+                           double* aval, double* b, double* flux, double* abou, double* errmax, double* mean,
+                           int32_t mem, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

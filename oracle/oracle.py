@@ -143,6 +143,34 @@ class Oracle(_Solvers):
                                    np.ascontiguousarray(u, np.float64), z)
         return z
 
+    def calcps(self, case, fields=None, nmat=None, dt=None):
+        """Pressure-correction assembly (oracle/calcps_oracle.c, after
+        Process/CalcPS.f90:51-202) of a meshgen.CalcPSCase; ``fields``
+        overrides the case's per-call arrays.  Returns dict aval (NONZ),
+        b (NC), flux (NS), abou (NbC, Abou(-NbC:-1)), errmax, mean, and
+        b_pre (b before the mean is removed)."""
+        f = dict(case.fields(), **(fields or {}))
+        fn = self.lib.oracle_calcps
+        if not getattr(fn, "_typed", False):
+            fn.restype = None
+            fn.argtypes = ([C.c_int] * 5 + [_i32p] * 4 + [_f64p] * 8 + [_f64p] * 11 + [C.c_double] +
+                           [_f64p] * 4 + [C.POINTER(C.c_double)] * 2 + [_f64p])
+            fn._typed = True
+        aval = np.zeros(case.nonz)
+        b = np.zeros(case.nc)
+        flux = np.zeros(case.ns)
+        abou = np.zeros(max(case.nbc, 1))
+        em, mn = C.c_double(0.0), C.c_double(0.0)
+        b_pre = np.zeros(case.nc)
+        c32 = lambda a: np.ascontiguousarray(a, np.int32)   # noqa: E731
+        c64 = lambda a: np.ascontiguousarray(a, np.float64)  # noqa: E731
+        fn(case.nc, case.nbc, case.ns, case.nonz, case.nmat if nmat is None else nmat,
+           c32(case.sidec.T), c32(case.sidaij.T), c32(case.adia), c32(case.typebc),
+           *(c64(getattr(case, k)) for k in ("sx", "sy", "sz", "dx", "dy", "dz", "f", "volume")),
+           *(c64(f[k]) for k in ("asave", "rho_n", "rho_o", "rho_oo", "u", "v", "w", "p", "px", "py", "pz")),
+           case.dt if dt is None else dt, aval, b, flux, abou, C.byref(em), C.byref(mn), b_pre)
+        return dict(aval=aval, b=b, flux=flux, abou=abou[:case.nbc], errmax=em.value, mean=mn.value, b_pre=b_pre)
+
 
 class Reference(_Solvers):
     """The reference Fortran solvers, compiled from their own sources."""

@@ -9,6 +9,7 @@
 
 #include "../../include/tflows_krylov.h"
 #include "batched.h"
+#include "calcps.h"
 #include "system.h"
 
 using tflows::System;
@@ -18,6 +19,9 @@ struct tflows_system_s {
 };
 struct tflows_batch_s {
     std::unique_ptr<tflows::Batch> batch;
+};
+struct tflows_calcps_s {
+    std::unique_ptr<tflows::CalcPS> cps;
 };
 
 namespace {
@@ -366,3 +370,74 @@ int tflows_batch_solve_mixed(tflows_batch b, int32_t method, int32_t prec, doubl
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// pressure-correction assembly
+// ---------------------------------------------------------------------------
+int tflows_calcps_create(int32_t nc, int32_t nbc, int32_t ns, int32_t nonz, const int32_t* sidec,
+                         const int32_t* sidaij, const int32_t* adia, const int32_t* typebc, const double* sx,
+                         const double* sy, const double* sz, const double* dx, const double* dy, const double* dz,
+                         const double* f, const double* volume, int32_t mem, void* stream, tflows_calcps* out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("out is NULL");
+        if (!sidec || !adia || !typebc || !volume || (ns > 0 && (!sidaij || !sx || !sy || !sz || !dx || !dy ||
+                                                                 !dz || !f)))
+            throw std::invalid_argument("null array argument");
+        const double* geom[7] = {sx, sy, sz, dx, dy, dz, f};
+        auto h = std::make_unique<tflows_calcps_s>();
+        h->cps = std::make_unique<tflows::CalcPS>(nc, nbc, ns, nonz, sidec, sidaij, adia, typebc, geom, volume,
+                                                  mem == TFLOWS_MEM_DEVICE, as_stream(stream));
+        *out = h.release();
+    });
+}
+
+int tflows_calcps_destroy(tflows_calcps h) {
+    return guarded([&] { delete h; });
+}
+
+int tflows_calcps_assemble(tflows_calcps h, int32_t nmat, double dt, const double* asave, const double* rho_n,
+                           const double* rho_o, const double* rho_oo, const double* u, const double* v,
+                           const double* w, const double* p, const double* px, const double* py, const double* pz,
+                           double* aval, double* b, double* flux, double* abou, double* errmax, double* mean,
+                           int32_t mem, void* stream) {
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("null handle");
+        const double* in[11] = {asave, rho_n, rho_o, rho_oo, u, v, w, p, px, py, pz};
+        for (const double* q : in)
+            if (!q) throw std::invalid_argument("null field");
+        tflows::CalcPS& C = *h->cps;
+        if (!aval || !b || (C.ns() > 0 && !flux) || (C.nbc() > 0 && !abou))
+            throw std::invalid_argument("null output");
+        const hipStream_t st = as_stream(stream);
+        const size_t ncell = (size_t)C.nbc() + C.nc() + 1;
+        if (mem == TFLOWS_MEM_DEVICE) {
+            const tflows::CalcPSFields F{asave, rho_n, rho_o, rho_oo, u, v, w, p, px, py, pz};
+            C.assemble(nmat, dt, F, aval, b, flux, abou, errmax, mean, st);
+            return;
+        }
+        // host arrays: one device staging area, fields in, results out;
+        // Flux and Abou enter with the caller's values (kept where the
+        // reference leaves them)
+        const size_t total = 11 * ncell + (size_t)C.nonz() + C.nc() + C.ns() + C.nbc();
+        double* d = C.staging(total);
+        const double* dev[11];
+        for (int q = 0; q < 11; ++q) {
+            TF_CHECK(hipMemcpyAsync(d + q * ncell, in[q], sizeof(double) * ncell, hipMemcpyHostToDevice, st));
+            dev[q] = d + q * ncell;
+        }
+        double* daval = d + 11 * ncell;
+        double* db = daval + C.nonz();
+        double* dflux = db + C.nc();
+        double* dabou = dflux + C.ns();
+        if (C.ns() > 0) TF_CHECK(hipMemcpyAsync(dflux, flux, sizeof(double) * C.ns(), hipMemcpyHostToDevice, st));
+        if (C.nbc() > 0) TF_CHECK(hipMemcpyAsync(dabou, abou, sizeof(double) * C.nbc(), hipMemcpyHostToDevice, st));
+        const tflows::CalcPSFields F{dev[0], dev[1], dev[2], dev[3], dev[4], dev[5], dev[6], dev[7], dev[8], dev[9],
+                                     dev[10]};
+        C.assemble(nmat, dt, F, daval, db, dflux, dabou, errmax, mean, st);
+        TF_CHECK(hipMemcpyAsync(aval, daval, sizeof(double) * C.nonz(), hipMemcpyDeviceToHost, st));
+        TF_CHECK(hipMemcpyAsync(b, db, sizeof(double) * C.nc(), hipMemcpyDeviceToHost, st));
+        if (C.ns() > 0) TF_CHECK(hipMemcpyAsync(flux, dflux, sizeof(double) * C.ns(), hipMemcpyDeviceToHost, st));
+        if (C.nbc() > 0) TF_CHECK(hipMemcpyAsync(abou, dabou, sizeof(double) * C.nbc(), hipMemcpyDeviceToHost, st));
+        TF_CHECK(hipStreamSynchronize(st));
+    });
+}

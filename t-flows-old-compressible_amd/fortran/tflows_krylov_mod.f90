@@ -106,5 +106,34 @@ module tflows_krylov
       real(c_double)            :: inires, finres
       integer(c_int32_t)        :: rc
     end function tflows_solve
+
+    !--- pressure-correction assembly (CalcPS.f90:51-202) ------------------
+    ! Array arguments by address (c_loc of the module arrays, which keep
+    ! their reference bases: SideC(0:2,NS), TypeBC(-NbC:NC), volume(-NbC:NC))
+    function tflows_calcps_create(nc, nbc, ns, nonz, sidec, sidaij, adia, typebc, sx, sy, sz, dx, dy, dz, f, &
+                                  volume, mem, stream, h) result(rc) bind(C, name="tflows_calcps_create")
+      import :: c_int32_t, c_ptr
+      integer(c_int32_t), value :: nc, nbc, ns, nonz, mem
+      type(c_ptr), value        :: sidec, sidaij, adia, typebc, sx, sy, sz, dx, dy, dz, f, volume, stream
+      type(c_ptr)               :: h
+      integer(c_int32_t)        :: rc
+    end function tflows_calcps_create
+
+    function tflows_calcps_destroy(h) result(rc) bind(C, name="tflows_calcps_destroy")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t) :: rc
+    end function tflows_calcps_destroy
+
+    function tflows_calcps_assemble(h, nmat, dt, asave, rho_n, rho_o, rho_oo, u, v, w, p, px, py, pz, aval, b, &
+                                    flux, abou, errmax, mean, mem, stream) result(rc) &
+        bind(C, name="tflows_calcps_assemble")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value        :: h, asave, rho_n, rho_o, rho_oo, u, v, w, p, px, py, pz, aval, b, flux, abou, stream
+      integer(c_int32_t), value :: nmat, mem
+      real(c_double), value     :: dt
+      real(c_double)            :: errmax, mean
+      integer(c_int32_t)        :: rc
+    end function tflows_calcps_assemble
   end interface
 end module tflows_krylov

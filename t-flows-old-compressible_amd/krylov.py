@@ -143,6 +143,14 @@ def _bind_extensions(lib):
         lib.tflows_batch_solve_mixed.restype = C.c_int
         lib.tflows_batch_solve_mixed.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, C.c_double,
                                                  C.c_int32, vp, vp, vp, _ip, vp]
+    if hasattr(lib, "tflows_calcps_create"):
+        lib.tflows_calcps_create.restype = C.c_int
+        lib.tflows_calcps_create.argtypes = [C.c_int32] * 4 + [vp] * 12 + [C.c_int32, vp, C.POINTER(vp)]
+        lib.tflows_calcps_destroy.restype = C.c_int
+        lib.tflows_calcps_destroy.argtypes = [vp]
+        lib.tflows_calcps_assemble.restype = C.c_int
+        lib.tflows_calcps_assemble.argtypes = ([vp, C.c_int32, C.c_double] + [vp] * 11 + [vp] * 4 +
+                                               [C.POINTER(C.c_double)] * 2 + [C.c_int32, vp])
 
 
 def _check(rc: int):
@@ -450,6 +458,75 @@ class DeviceBatch:
     def close(self):
         if getattr(self, "handle", None) is not None:
             self.lib.tflows_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+CALCPS_FIELDS = ("asave", "rho_n", "rho_o", "rho_oo", "u", "v", "w", "p", "px", "py", "pz")
+
+
+class DeviceCalcPS:
+    """The pressure-correction assembly of one mesh (Process/CalcPS.f90:51-202)
+    on the GPU.  ``case`` is a meshgen.CalcPSCase (reference layout); the
+    face list and geometry go to the device once."""
+
+    def __init__(self, case, stream=None):
+        import torch
+        self.lib = load_library()
+        self.case = case
+        dev = torch.device("cuda")
+        t32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(dev)    # noqa: E731
+        t64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)  # noqa: E731
+        keep = [t32(case.sidec.T), t32(case.sidaij.T), t32(case.adia), t32(case.typebc)]
+        keep += [t64(getattr(case, k)) for k in ("sx", "sy", "sz", "dx", "dy", "dz", "f", "volume")]
+        torch.cuda.synchronize()
+        h = C.c_void_p()
+        _check(self.lib.tflows_calcps_create(case.nc, case.nbc, case.ns, case.nonz, *(_ptr(t) for t in keep), 1,
+                                             _stream(stream), C.byref(h)))
+        self.handle = h
+
+    def assemble(self, fields=None, nmat=None, dt=None, out=None, stream=None):
+        """Assemble from cuda float64 field tensors ((-NbC:NC) layout; the
+        case's own fields by default).  Returns dict of cuda tensors aval,
+        b, flux, abou and floats errmax, mean; ``out`` reuses tensors."""
+        import torch
+        c = self.case
+        if fields is None:
+            fields = {k: torch.from_numpy(getattr(c, k)).cuda() for k in CALCPS_FIELDS}
+        if out is None:
+            dev = torch.device("cuda")
+            out = dict(aval=torch.empty(c.nonz, dtype=torch.float64, device=dev),
+                       b=torch.empty(c.nc, dtype=torch.float64, device=dev),
+                       flux=torch.zeros(max(c.ns, 1), dtype=torch.float64, device=dev),
+                       abou=torch.zeros(max(c.nbc, 1), dtype=torch.float64, device=dev))
+        em, mn = C.c_double(0.0), C.c_double(0.0)
+        _check(self.lib.tflows_calcps_assemble(
+            self.handle, c.nmat if nmat is None else nmat, c.dt if dt is None else dt,
+            *(_ptr(fields[k]) for k in CALCPS_FIELDS), _ptr(out["aval"]), _ptr(out["b"]), _ptr(out["flux"]),
+            _ptr(out["abou"]), C.byref(em), C.byref(mn), 1, _stream(stream)))
+        return dict(out, errmax=em.value, mean=mn.value)
+
+    def assemble_host(self, nmat=None, dt=None):
+        """The same from the case's host arrays (copied in and out)."""
+        c = self.case
+        aval, b = np.empty(c.nonz), np.empty(c.nc)
+        flux, abou = np.zeros(max(c.ns, 1)), np.zeros(max(c.nbc, 1))
+        em, mn = C.c_double(0.0), C.c_double(0.0)
+        fl = [np.ascontiguousarray(getattr(c, k), np.float64) for k in CALCPS_FIELDS]
+        _check(self.lib.tflows_calcps_assemble(
+            self.handle, c.nmat if nmat is None else nmat, c.dt if dt is None else dt,
+            *(a.ctypes.data for a in fl), aval.ctypes.data, b.ctypes.data, flux.ctypes.data, abou.ctypes.data,
+            C.byref(em), C.byref(mn), 0, None))
+        return dict(aval=aval, b=b, flux=flux[:c.ns], abou=abou[:c.nbc], errmax=em.value, mean=mn.value)
+
+    def close(self):
+        if getattr(self, "handle", None) is not None:
+            self.lib.tflows_calcps_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

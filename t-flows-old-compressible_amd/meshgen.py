@@ -773,3 +773,207 @@ def fourier_target(centres: np.ndarray, seed: int, modes: int = 4) -> np.ndarray
         amp = rng.uniform(-1, 1)
         t += amp * np.prod(np.cos(np.pi * k * centres + ph), axis=1)
     return t
+
+
+# --------------------------------------------------------------------------
+# inputs of the pressure-correction assembly (Process/CalcPS.f90:51-202)
+# --------------------------------------------------------------------------
+BC_CODES = {"inflow": 1, "wall": 2, "outflow": 3, "symmetry": 4, "convect": 5, "wallfl": 6, "buffer": 11,
+            "pressure": 12}   # Library/allp_mod.f90:16-24
+
+CHANNEL_BC = {"xmin": "inflow", "xmax": "outflow", "ymin": "wall", "ymax": "wall", "zmin": "symmetry",
+              "zmax": "symmetry"}
+
+
+@dataclass
+class CalcPSCase:
+    """Everything the pressure-correction assembly reads, in the reference's
+    layout: ``sidec`` is SideC(0:2,NS) as a (3, NS) int32 array (row 0
+    unused), ``sidaij`` SidAij(1:2,NS) as (2, NS) int32, 1-based, 0 on
+    boundary faces; the TopolM pattern (acol, arow, adia); per face Sx..Dz
+    and f; and the cell arrays declared (-NbC:NC) as arrays of NbC+NC+1
+    entries with x(c) at index c + NbC (boundary cells c = -1..-NbC, one
+    per boundary face, numbered in face order)."""
+    shape: tuple
+    nc: int
+    nbc: int
+    ns: int
+    nonz: int
+    nmat: int
+    dt: float
+    sidec: np.ndarray
+    sidaij: np.ndarray
+    acol: np.ndarray
+    arow: np.ndarray
+    adia: np.ndarray
+    typebc: np.ndarray
+    sx: np.ndarray
+    sy: np.ndarray
+    sz: np.ndarray
+    dx: np.ndarray
+    dy: np.ndarray
+    dz: np.ndarray
+    f: np.ndarray
+    volume: np.ndarray
+    asave: np.ndarray
+    rho_n: np.ndarray
+    rho_o: np.ndarray
+    rho_oo: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    p: np.ndarray
+    px: np.ndarray
+    py: np.ndarray
+    pz: np.ndarray
+
+    def matrix(self, aval: np.ndarray) -> TFlowSMatrix:
+        """The assembled system matrix with values ``aval`` (NONZ)."""
+        return TFlowSMatrix(self.nc, self.nonz, self.acol, self.arow, self.adia,
+                            np.ascontiguousarray(aval, np.float64), None, {"kind": "calcps", "shape": self.shape})
+
+    def fields(self) -> dict:
+        return {k: getattr(self, k) for k in ("asave", "rho_n", "rho_o", "rho_oo", "u", "v", "w", "p", "px",
+                                              "py", "pz")}
+
+
+def calcps_case(shape, seed: int = 7, ratio: float = 1.05, perturb: float = 0.15, bc: dict | None = None,
+                face_seed: int | None = None, nmat: int = 1, dt: float = 1e-2) -> CalcPSCase:
+    """A channel box (2 pi x 2 x pi, tanh wall stretching in y, interior
+    nodes moved by up to ``perturb`` of the local spacing) with its faces,
+    boundary faces on the six sides with the types of ``bc`` (keys xmin ..
+    zmax, values from BC_CODES), and smooth seeded fields.
+
+    Faces: S is the quad-diagonal area vector pointing from c1 to c2, d the
+    centre-to-centre vector (face centre for a boundary cell), f the weight
+    of c1 in the face interpolation (1 on boundary faces); volumes from the
+    divergence theorem over the same faces.  Interior faces come first
+    (x, y, z), then the boundary faces side by side, unless ``face_seed``
+    shuffles all of them.  BUFFER cells get a volume and Asave like an
+    interior cell's; other boundary cells have zero volume and Asave.
+    """
+    nx, ny, nz = shape
+    bc = dict(CHANNEL_BC, **(bc or {}))
+    rng = np.random.default_rng(seed)
+    xs = np.linspace(0.0, 2.0 * np.pi, nx + 1)
+    ys = tanh_nodes(ny, 2.0, ratio)
+    zs = np.linspace(0.0, np.pi, nz + 1)
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    P = np.stack([X, Y, Z], axis=-1)
+    h = np.stack(np.meshgrid(np.gradient(xs), np.gradient(ys), np.gradient(zs), indexing="ij"), axis=-1)
+    disp = rng.uniform(-perturb, perturb, P.shape) * h
+    disp[[0, -1], :, :, 0] = 0.0
+    disp[:, [0, -1], :, 1] = 0.0
+    disp[:, :, [0, -1], 2] = 0.0
+    P = P + disp
+    C = 0.125 * (P[:-1, :-1, :-1] + P[1:, :-1, :-1] + P[:-1, 1:, :-1] + P[1:, 1:, :-1] +
+                 P[:-1, :-1, 1:] + P[1:, :-1, 1:] + P[:-1, 1:, 1:] + P[1:, 1:, 1:])
+    cid = np.arange(nx * ny * nz).reshape(nx, ny, nz, order="F")     # [i, j, k] -> i + nx (j + ny k)
+    cen = C.reshape(-1, 3, order="F")
+
+    def quad(a, b, c, d):
+        return 0.5 * np.cross(c - a, d - b), 0.25 * (a + b + c + d)
+
+    def xplane(i):   # +x area vectors of node plane i, (ny, nz, 3)
+        return quad(P[i, :-1, :-1], P[i, 1:, :-1], P[i, 1:, 1:], P[i, :-1, 1:])
+
+    def yplane(j):
+        return quad(P[:-1, j, :-1], P[:-1, j, 1:], P[1:, j, 1:], P[1:, j, :-1])
+
+    def zplane(k):
+        return quad(P[:-1, :-1, k], P[1:, :-1, k], P[1:, 1:, k], P[:-1, 1:, k])
+
+    S_l, xf_l, c1_l, c2_l = [], [], [], []
+    # interior faces
+    S, xf = quad(P[1:-1, :-1, :-1], P[1:-1, 1:, :-1], P[1:-1, 1:, 1:], P[1:-1, :-1, 1:])
+    S_l.append(S.reshape(-1, 3, order="F")); xf_l.append(xf.reshape(-1, 3, order="F"))
+    c1_l.append(cid[:-1].ravel(order="F")); c2_l.append(cid[1:].ravel(order="F"))
+    S, xf = quad(P[:-1, 1:-1, :-1], P[:-1, 1:-1, 1:], P[1:, 1:-1, 1:], P[1:, 1:-1, :-1])
+    S_l.append(S.reshape(-1, 3, order="F")); xf_l.append(xf.reshape(-1, 3, order="F"))
+    c1_l.append(cid[:, :-1].ravel(order="F")); c2_l.append(cid[:, 1:].ravel(order="F"))
+    S, xf = quad(P[:-1, :-1, 1:-1], P[1:, :-1, 1:-1], P[1:, 1:, 1:-1], P[:-1, 1:, 1:-1])
+    S_l.append(S.reshape(-1, 3, order="F")); xf_l.append(xf.reshape(-1, 3, order="F"))
+    c1_l.append(cid[:, :, :-1].ravel(order="F")); c2_l.append(cid[:, :, 1:].ravel(order="F"))
+    n_int = sum(len(a) for a in c1_l)
+    # boundary faces, outward from the adjacent cell
+    sides = []
+    for name, (S, xf), cells, sign in (
+            ("xmin", xplane(0), cid[0], -1.0), ("xmax", xplane(nx), cid[-1], 1.0),
+            ("ymin", yplane(0), cid[:, 0], -1.0), ("ymax", yplane(ny), cid[:, -1], 1.0),
+            ("zmin", zplane(0), cid[:, :, 0], -1.0), ("zmax", zplane(nz), cid[:, :, -1], 1.0)):
+        S_l.append(sign * S.reshape(-1, 3, order="F")); xf_l.append(xf.reshape(-1, 3, order="F"))
+        c1_l.append(cells.ravel(order="F")); c2_l.append(np.full(cells.size, -1))
+        sides.append(np.full(cells.size, BC_CODES[bc[name]]))
+    S = np.concatenate(S_l); xf = np.concatenate(xf_l)
+    c1 = np.concatenate(c1_l); c2 = np.concatenate(c2_l)
+    bcode = np.concatenate([np.zeros(n_int, np.int64)] + sides)
+    ns = len(c1)
+    order = np.random.default_rng(face_seed).permutation(ns) if face_seed is not None else np.arange(ns)
+    S, xf, c1, c2, bcode = S[order], xf[order], c1[order], c2[order], bcode[order]
+    bnd = c2 < 0
+    nbc = int(bnd.sum())
+    nc = nx * ny * nz
+    # boundary cells numbered -1 .. -NbC in face order
+    c2 = c2.copy()
+    c2[bnd] = -np.arange(1, nbc + 1)
+    c1f = c1 + 1                                   # 1-based
+    c2f = np.where(bnd, c2, c2 + 1)
+    # geometry
+    xc1 = cen[c1]
+    xc2 = np.where(bnd[:, None], xf, cen[np.where(bnd, 0, c2)])
+    d = xc2 - xc1
+    a1 = np.linalg.norm(xf - xc1, axis=1)
+    a2 = np.linalg.norm(xc2 - xf, axis=1)
+    f = np.where(bnd, 1.0, a2 / np.where(a1 + a2 > 0, a1 + a2, 1.0))
+    third = np.einsum("ij,ij->i", xf, S) / 3.0
+    vol_cells = np.bincount(c1, weights=third, minlength=nc) - np.bincount(c2[~bnd], weights=third[~bnd],
+                                                                            minlength=nc)
+    # pattern of the interior faces (TopolM), in face order
+    interior = ~bnd
+    topo = topolm(nc, c1f[interior], c2f[interior])
+    sidaij = np.zeros((2, ns), np.int32)
+    sidaij[:, interior] = (topo.sidaij + 1).astype(np.int32)
+    sidec = np.zeros((3, ns), np.int32)
+    sidec[1] = c1f
+    sidec[2] = c2f
+    # cell arrays (-NbC:NC)
+    ntot = nbc + nc + 1
+    at = lambda c: c + nbc                         # noqa: E731  index of cell c
+    typebc = np.zeros(ntot, np.int32)
+    typebc[at(c2[bnd])] = bcode[bnd]
+    volume = np.zeros(ntot)
+    volume[at(np.arange(1, nc + 1))] = vol_cells
+    is_buf = bnd & (bcode == BC_CODES["buffer"])
+    volume[at(c2[is_buf])] = vol_cells[c1[is_buf]] * rng.uniform(0.9, 1.1, int(is_buf.sum()))
+    bpos = np.zeros((nbc, 3))                      # positions of the cells (-NbC:NC)
+    bpos[-c2[bnd] - 1] = xf[bnd]
+    allc = np.concatenate([bpos[::-1], np.zeros((1, 3)), cen])
+
+    def smooth(mean, amp, k):
+        g = np.full(ntot, float(mean))
+        for _ in range(3):
+            kv = rng.uniform(0.5, 2.0, 3) * k
+            ph = rng.uniform(0, 2 * np.pi)
+            g += amp * np.sin(allc @ kv + ph) / 3.0
+        return g
+
+    rho_n = smooth(1.0, 0.05, 1.0)
+    rho_o = rho_n * (1.0 + 1e-3 * rng.standard_normal(ntot))
+    rho_oo = rho_n * (1.0 + 1e-3 * rng.standard_normal(ntot))
+    u = smooth(1.0, 0.3, 1.0)
+    v = smooth(0.0, 0.2, 1.3)
+    w = smooth(0.0, 0.2, 0.7)
+    p = smooth(0.0, 0.5, 1.1)
+    px, py, pz = smooth(0.0, 0.4, 1.0), smooth(0.0, 0.4, 1.2), smooth(0.0, 0.4, 0.9)
+    asave = volume / dt * (1.5 + 0.2 * rng.uniform(0, 1, ntot))
+    nonbuf_bc = np.zeros(ntot, bool)
+    nonbuf_bc[at(c2[bnd & ~is_buf])] = True
+    asave[nonbuf_bc] = 0.0
+    volume[nonbuf_bc] = 0.0
+    asave[nbc] = 0.0                              # cell 0 is not a cell
+    for a in (rho_n, rho_o, rho_oo, u, v, w, p, px, py, pz):
+        a[nbc] = 0.0
+    return CalcPSCase(tuple(shape), nc, nbc, ns, topo.nonz, nmat, dt, sidec, sidaij, topo.acol, topo.arow,
+                      topo.adia, typebc, np.ascontiguousarray(S[:, 0]), np.ascontiguousarray(S[:, 1]),
+                      np.ascontiguousarray(S[:, 2]), np.ascontiguousarray(d[:, 0]), np.ascontiguousarray(d[:, 1]),
+                      np.ascontiguousarray(d[:, 2]), f, volume, asave, rho_n, rho_o, rho_oo, u, v, w, p, px, py, pz)
