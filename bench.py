@@ -315,7 +315,7 @@ def workload_calcps(args, torch, pkg):
     m = case.matrix(out["aval"].cpu().numpy())
     ds = pkg.krylov.DeviceSystem(m)
     # the pressure correction is not solved to 1e-8 in the SIMPLE loop
-    # (CalcPS.f90:173-178); 1e-6 of ||b|| within the cap of 800
+    # (CalcPS.f90:177-179); 1e-6 of ||b|| within the cap of 800
     tol = 1e-6 * float(torch.linalg.norm(out["b"]))
     x = torch.zeros(case.nc, dtype=torch.float64, device="cuda")
     r = torch.empty_like(out["b"])
