@@ -274,6 +274,19 @@ int tflows_calcps_assemble(tflows_calcps h, int32_t nmat, double dt, const doubl
                            double* aval, double* b, double* flux, double* abou, double* errmax, double* mean,
                            int32_t mem, void* stream);
 
+/* The correction after the pressure solve, Process/CorUVW.f90:36-121 (the
+ * CFL / Peclet monitor of :126-147 is not computed): U, V, W (-NbC:NC) are
+ * corrected in place with the pressure gradient Px, Py, Pz and copied to
+ * PRESSURE boundary cells; Flux(NS) is corrected in place with the pressure
+ * correction pp (-NbC:NC) through Aval(SidAij(1,s)) and Abou; b(NC)
+ * receives the mass error per unit volume and *errmax its maximum (CorUVW's
+ * value, :183).  Bitwise the reference's.  Same bases and mem rule as
+ * tflows_calcps_assemble. */
+int tflows_calcps_correct(tflows_calcps h, double dt, const double* pp, const double* aval, const double* abou,
+                          const double* asave, const double* rho_n, const double* rho_o, const double* rho_oo,
+                          const double* px, const double* py, const double* pz, double* u, double* v, double* w,
+                          double* flux, double* b, double* errmax, int32_t mem, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -139,3 +139,71 @@ void oracle_calcps(int nc, int nbc, int ns, int nonz, int nmat, const int *sidec
     *mean = mn;
 #undef C
 }
+
+/*
+ * oracle_coruvw -- Process/CorUVW.f90:36-121 and :183 (the correction
+ * after the pressure solve; the CFL / Peclet monitor of :126-147 only feeds
+ * the printed monitor lines and is left out):
+ *
+ *   :36-46    U = U - Px V / Asave / rho (both algorithm branches compute
+ *             the same values), V, W likewise, cells 1..NC
+ *   :48-59    PRESSURE boundary cells take the corrected velocity of c1
+ *   :76-87    Flux(s) += (PP(c2) - PP(c1)) * Aval(SidAij(1,s)) for interior
+ *             faces, * Abou(c2) for BUFFER faces
+ *   :97-115   b = 0; b(c1) -= Flux, b(c2) += Flux (interior) or b(c1) -= Flux
+ *             (every boundary face); b -= V/dt (1.5 rho^n - 2 rho^o +
+ *             0.5 rho^oo); b /= V
+ *   :117-121  errmax = max |b| (glomax is a no-op in one domain), which is
+ *             the function's value (:183)
+ *
+ * Same array bases as oracle_calcps; u, v, w and flux are updated in place.
+ */
+double oracle_coruvw(int nc, int nbc, int ns, const int *sidec, const int *sidaij, const int *typebc,
+                     const double *volume, const double *asave, const double *rho_n, const double *rho_o,
+                     const double *rho_oo, const double *px, const double *py, const double *pz, const double *pp,
+                     const double *aval, const double *abou, double dt, double *u, double *v, double *w,
+                     double *flux, double *b)
+{
+#define C(x, c) (x)[nbc + (c)]
+    for (int c = 1; c <= nc; ++c) {                                     /* :36-46 */
+        C(u, c) = C(u, c) - ((C(px, c) * C(volume, c)) / C(asave, c)) / C(rho_n, c);
+        C(v, c) = C(v, c) - ((C(py, c) * C(volume, c)) / C(asave, c)) / C(rho_n, c);
+        C(w, c) = C(w, c) - ((C(pz, c) * C(volume, c)) / C(asave, c)) / C(rho_n, c);
+    }
+    for (int s = 1; s <= ns; ++s) {                                     /* :48-59 */
+        const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
+        if (c2 < 0 && C(typebc, c2) == BC_PRESSURE) {
+            C(u, c2) = C(u, c1);
+            C(v, c2) = C(v, c1);
+            C(w, c2) = C(w, c1);
+        }
+    }
+    for (int s = 1; s <= ns; ++s) {                                     /* :76-87 */
+        const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
+        if (c2 > 0 || (c2 < 0 && C(typebc, c2) == BC_BUFFER)) {
+            if (c2 > 0)
+                flux[s - 1] = flux[s - 1] + (C(pp, c2) - C(pp, c1)) * aval[sidaij[2 * (s - 1)] - 1];
+            else
+                flux[s - 1] = flux[s - 1] + (C(pp, c2) - C(pp, c1)) * C(abou, c2);
+        }
+    }
+    for (int c = 0; c < nc; ++c) b[c] = 0.0;                            /* :97-99 */
+    for (int s = 1; s <= ns; ++s) {                                     /* :101-110 */
+        const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
+        if (c2 > 0 || (c2 < 0 && C(typebc, c2) == BC_BUFFER)) {
+            b[c1 - 1] = b[c1 - 1] - flux[s - 1];
+            if (c2 > 0) b[c2 - 1] = b[c2 - 1] + flux[s - 1];
+        } else {
+            b[c1 - 1] = b[c1 - 1] - flux[s - 1];
+        }
+    }
+    for (int c = 1; c <= nc; ++c) {                                     /* :112-115 */
+        b[c - 1] = b[c - 1] - (C(volume, c) / dt) *
+                                  ((1.5 * C(rho_n, c) - 2.0 * C(rho_o, c)) + 0.5 * C(rho_oo, c));
+        b[c - 1] = b[c - 1] / C(volume, c);
+    }
+    double em = 0.0;                                                    /* :117-121 */
+    for (int c = 1; c <= nc; ++c) em = fmax(em, fabs(b[c - 1]));
+    return em;
+#undef C
+}

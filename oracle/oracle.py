@@ -171,6 +171,28 @@ class Oracle(_Solvers):
            case.dt if dt is None else dt, aval, b, flux, abou, C.byref(em), C.byref(mn), b_pre)
         return dict(aval=aval, b=b, flux=flux, abou=abou[:case.nbc], errmax=em.value, mean=mn.value, b_pre=b_pre)
 
+    def coruvw(self, case, pp, aval, abou, flux, fields=None, dt=None):
+        """The correction after the pressure solve (oracle/calcps_oracle.c,
+        after Process/CorUVW.f90:36-121).  pp: (-NbC:NC); aval (NONZ); abou
+        (NbC); flux (NS).  Returns dict u, v, w ((-NbC:NC)), flux, b (NC),
+        errmax; the inputs are not modified."""
+        f = dict(case.fields(), **(fields or {}))
+        fn = self.lib.oracle_coruvw
+        if not getattr(fn, "_typed", False):
+            fn.restype = C.c_double
+            fn.argtypes = [C.c_int] * 3 + [_i32p] * 3 + [_f64p] * 11 + [C.c_double] + [_f64p] * 5
+            fn._typed = True
+        c64 = lambda a: np.ascontiguousarray(a, np.float64).copy()   # noqa: E731
+        u, v, w, fl = c64(f["u"]), c64(f["v"]), c64(f["w"]), c64(flux)
+        b = np.zeros(case.nc)
+        ab = np.zeros(max(case.nbc, 1))
+        ab[:case.nbc] = abou
+        em = fn(case.nc, case.nbc, case.ns, np.ascontiguousarray(case.sidec.T, np.int32),
+                np.ascontiguousarray(case.sidaij.T, np.int32), np.ascontiguousarray(case.typebc, np.int32),
+                c64(case.volume), *(c64(f[k]) for k in ("asave", "rho_n", "rho_o", "rho_oo", "px", "py", "pz")),
+                c64(pp), c64(aval), ab, case.dt if dt is None else dt, u, v, w, fl, b)
+        return dict(u=u, v=v, w=w, flux=fl, b=b, errmax=em)
+
 
 class Reference(_Solvers):
     """The reference Fortran solvers, compiled from their own sources."""

@@ -19,7 +19,7 @@ struct CalcPSMesh {
     const unsigned char* kind = nullptr;  // what the face does to its cells
     const int* adia = nullptr;          // Adia(c) - 1
     const int* cptr = nullptr;          // cell -> faces (NC + 1)
-    const int* cface = nullptr;         // 2 s + (the cell is c2), ascending s
+    const int* cface = nullptr;         // 2 s + (the cell is c2), ascending s, every face
     const double *sx = nullptr, *sy = nullptr, *sz = nullptr;
     const double *dx = nullptr, *dy = nullptr, *dz = nullptr, *f = nullptr;
     const double* volume = nullptr;     // (-NbC:NC): volume(c) at [NbC + c]
@@ -45,6 +45,12 @@ public:
     // pointers.  errmax, mean: host (may be null).
     void assemble(int nmat, double dt, const CalcPSFields& F, double* aval, double* b, double* flux, double* abou,
                   double* errmax, double* mean, hipStream_t s);
+    // The correction after the pressure solve (Process/CorUVW.f90:36-121):
+    // U, V, W ((-NbC:NC)) and Flux(NS) in place, the mass error per unit
+    // volume b(NC); returns max |b|.  F supplies asave, rho_n/o/oo, px, py,
+    // pz (the other members are not read).  Device pointers.
+    double correct(double dt, const CalcPSFields& F, const double* pp, const double* aval, const double* abou,
+                   double* u, double* v, double* w, double* flux, double* b, hipStream_t s);
 
     int nc() const { return nc_; }
     int nbc() const { return nbc_; }

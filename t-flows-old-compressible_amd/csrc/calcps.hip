@@ -23,6 +23,13 @@
 // The reference sums b V and V over the cells one after another; here the
 // sum is a tree over blocks, so the mean and b differ from the reference by
 // the rounding of that sum (deterministic from run to run).
+//
+// After the pressure solve, CorUVW (Process/CorUVW.f90:36-121) corrects the
+// velocities (k_cor_velocity, one thread per cell), copies them to PRESSURE
+// boundary cells and corrects the face fluxes with the pressure correction
+// (k_cor_faces, one thread per face), and forms the mass error per unit
+// volume over the same cell-to-face lists (k_cor_cells) and its maximum,
+// all bitwise the reference's.
 #include <algorithm>
 #include <cstring>
 #include <stdexcept>
@@ -123,6 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_calcps_cells(CalcPSMesh M, CalcPSFie
                 const int code = M.cface[e];
                 const int s = code >> 1;
                 const unsigned char k = M.kind[s];
+                if (k == kNone) continue;                              // no flux in the assembly
                 if (adds_diag(k)) diag = diag + fa12[s];
                 if (code & 1) bb = bb + flux[s];                       // the cell is c2
                 else bb = bb - flux[s];
@@ -210,6 +218,90 @@ __global__ __launch_bounds__(kBlock) void k_calcps_mean(int nc, const double* __
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < nc; i += gridDim.x * kBlock) b[i] = b[i] - mean;
 }
 
+// ---------------------------------------------------------------------------
+// The correction after the pressure solve (Process/CorUVW.f90:36-121).
+// ---------------------------------------------------------------------------
+// :36-46  U = U - Px V / Asave / rho, cells 1..NC (V, W likewise)
+__global__ __launch_bounds__(kBlock) void k_cor_velocity(CalcPSMesh M, CalcPSFields F, double* __restrict__ u,
+                                                          double* __restrict__ v, double* __restrict__ w) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= M.nc) return;
+    const int c = M.nbc + i + 1;
+    const double V = M.volume[c], As = F.asave[c], rho = F.rho_n[c];
+    u[c] = u[c] - ((F.px[c] * V) / As) / rho;
+    v[c] = v[c] - ((F.py[c] * V) / As) / rho;
+    w[c] = w[c] - ((F.pz[c] * V) / As) / rho;
+}
+
+// :48-59 PRESSURE boundary cells take the corrected velocity of c1;
+// :76-87 Flux += (PP(c2) - PP(c1)) * Aval(SidAij(1,s)) (interior) or
+// * Abou(c2) (BUFFER)
+__global__ __launch_bounds__(kBlock) void k_cor_faces(CalcPSMesh M, const double* __restrict__ pp,
+                                                       const double* __restrict__ aval,
+                                                       const double* __restrict__ abou, double* __restrict__ u,
+                                                       double* __restrict__ v, double* __restrict__ w,
+                                                       double* __restrict__ flux) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= M.ns) return;
+    const unsigned char k = M.kind[s];
+    const int nb = M.nbc;
+    const int c1 = nb + M.c1[s], c2 = nb + M.c2[s];
+    if (k == kPressure) {
+        u[c2] = u[c1];
+        v[c2] = v[c1];
+        w[c2] = w[c1];
+    } else if (k == kInterior) {
+        flux[s] = flux[s] + (pp[c2] - pp[c1]) * aval[M.sid1[s]];
+    } else if (k == kBuffer) {
+        flux[s] = flux[s] + (pp[c2] - pp[c1]) * abou[c2];
+    }
+}
+
+// :97-115 the mass error per unit volume of every cell, summed over its
+// faces in ascending face order; per-block max |b| (:117-120)
+__global__ __launch_bounds__(kBlock) void k_cor_cells(CalcPSMesh M, CalcPSFields F, double dt,
+                                                       const double* __restrict__ flux, double* __restrict__ b,
+                                                       double* __restrict__ part) {
+    double vmax = 0.0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < M.nc; i += gridDim.x * kBlock) {
+        double bb = 0.0;
+        for (int e = M.cptr[i]; e < M.cptr[i + 1]; ++e) {
+            const int code = M.cface[e];
+            if (code & 1) bb = bb + flux[code >> 1];
+            else bb = bb - flux[code >> 1];
+        }
+        const int c = M.nbc + i + 1;
+        const double V = M.volume[c];
+        bb = bb - (V / dt) * ((1.5 * F.rho_n[c] - 2.0 * F.rho_o[c]) + 0.5 * F.rho_oo[c]);
+        bb = bb / V;
+        b[i] = bb;
+        vmax = fmax(vmax, fabs(bb));
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) vmax = fmax(vmax, __shfl_down(vmax, off, kWave));
+    __shared__ double sm[kWavesPerBlock];
+    if (threadIdx.x % kWave == 0) sm[threadIdx.x / kWave] = vmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sm[0];
+        for (int q = 1; q < kWavesPerBlock; ++q) t = fmax(t, sm[q]);
+        part[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cor_total(int parts, const double* __restrict__ part,
+                                                       double* __restrict__ out) {
+    __shared__ double sm[kBlock];
+    double t = 0.0;
+    for (int q = threadIdx.x; q < parts; q += kBlock) t = fmax(t, part[q]);
+    sm[threadIdx.x] = t;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sm[0];
+}
+
 template <typename T>
 T* upload(std::vector<void*>& owned, const T* src, size_t count, bool device_src, hipStream_t s) {
     T* d = nullptr;
@@ -269,16 +361,16 @@ CalcPS::CalcPS(int nc, int nbc, int ns, int nonz, const int* sidec, const int* s
                     : t == kBcPressure ? kPressure
                     : kNone;
         }
-        if (kind[q] != kNone) count[c1[q]]++;
+        count[c1[q]]++;
         if (kind[q] == kInterior) count[c2[q]]++;
     }
-    // cell -> faces, ascending face index; entry = 2 s + (cell is c2)
+    // cell -> faces (every face of the cell), ascending face index;
+    // entry = 2 s + (cell is c2)
     std::vector<int> cptr(nc + 1, 0), cface;
     for (int i = 0; i < nc; ++i) cptr[i + 1] = cptr[i] + count[i + 1];
     cface.resize(cptr[nc]);
     std::vector<int> fill(cptr.begin(), cptr.end() - 1);
     for (int q = 0; q < ns; ++q) {
-        if (kind[q] == kNone) continue;
         cface[fill[c1[q] - 1]++] = 2 * q;
         if (kind[q] == kInterior) cface[fill[c2[q] - 1]++] = 2 * q + 1;
     }
@@ -307,6 +399,22 @@ CalcPS::CalcPS(int nc, int nbc, int ns, int nonz, const int* sidec, const int* s
     TF_CHECK(hipMalloc(&part_, sizeof(double) * 3 * grid_));
     TF_CHECK(hipMalloc(&out2_, sizeof(double) * 2));
     TF_CHECK(hipStreamSynchronize(s));
+}
+
+double CalcPS::correct(double dt, const CalcPSFields& F, const double* pp, const double* aval, const double* abou,
+                       double* u, double* v, double* w, double* flux, double* b, hipStream_t s) {
+    if (!(dt != 0.0)) throw std::invalid_argument("dt must be non-zero");
+    hipLaunchKernelGGL(k_cor_velocity, dim3((nc_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, M_, F, u, v, w);
+    if (ns_ > 0)
+        hipLaunchKernelGGL(k_cor_faces, dim3((ns_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, M_, pp, aval, abou, u,
+                           v, w, flux);
+    hipLaunchKernelGGL(k_cor_cells, dim3(grid_), dim3(kBlock), 0, s, M_, F, dt, (const double*)flux, b, part_);
+    hipLaunchKernelGGL(k_cor_total, dim3(1), dim3(kBlock), 0, s, grid_, (const double*)part_, out2_);
+    TF_CHECK(hipGetLastError());
+    double em = 0.0;
+    TF_CHECK(hipMemcpyAsync(&em, out2_, sizeof(double), hipMemcpyDeviceToHost, s));
+    TF_CHECK(hipStreamSynchronize(s));
+    return em;
 }
 
 double* CalcPS::staging(size_t doubles) {

@@ -268,8 +268,9 @@ int tflows_system_fp32_layout(tflows_system s, int32_t* sliced, int64_t* stored_
     return guarded([&] {
         if (!sliced || !stored_entries) throw std::invalid_argument("null output pointer");
         const hipStream_t st = as_stream(stream);
-        *sliced = s->sys->op32_sliced() ? 1 : 0;
-        *stored_entries = s->sys->op32_sliced() ? s->sys->sell32_entries(st) : s->sys->nnz() - s->sys->n();
+        const bool sl = s->sys->op32_sliced(st);
+        *sliced = sl ? 1 : 0;
+        *stored_entries = sl ? s->sys->sell32_entries(st) : s->sys->nnz() - s->sys->n();
         TF_CHECK(hipStreamSynchronize(st));
     });
 }
@@ -439,5 +440,53 @@ int tflows_calcps_assemble(tflows_calcps h, int32_t nmat, double dt, const doubl
         if (C.ns() > 0) TF_CHECK(hipMemcpyAsync(flux, dflux, sizeof(double) * C.ns(), hipMemcpyDeviceToHost, st));
         if (C.nbc() > 0) TF_CHECK(hipMemcpyAsync(abou, dabou, sizeof(double) * C.nbc(), hipMemcpyDeviceToHost, st));
         TF_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+int tflows_calcps_correct(tflows_calcps h, double dt, const double* pp, const double* aval, const double* abou,
+                          const double* asave, const double* rho_n, const double* rho_o, const double* rho_oo,
+                          const double* px, const double* py, const double* pz, double* u, double* v, double* w,
+                          double* flux, double* b, double* errmax, int32_t mem, void* stream) {
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("null handle");
+        const double* in[8] = {pp, asave, rho_n, rho_o, rho_oo, px, py, pz};
+        for (const double* q : in)
+            if (!q) throw std::invalid_argument("null field");
+        tflows::CalcPS& C = *h->cps;
+        if (!aval || !u || !v || !w || !b || (C.ns() > 0 && !flux) || (C.nbc() > 0 && !abou))
+            throw std::invalid_argument("null array");
+        const hipStream_t st = as_stream(stream);
+        const size_t ncell = (size_t)C.nbc() + C.nc() + 1;
+        double em = 0.0;
+        if (mem == TFLOWS_MEM_DEVICE) {
+            const tflows::CalcPSFields F{asave, rho_n, rho_o, rho_oo, nullptr, nullptr, nullptr, nullptr, px, py, pz};
+            em = C.correct(dt, F, pp, aval, abou, u, v, w, flux, b, st);
+        } else {
+            const size_t total = 11 * ncell + (size_t)C.nonz() + C.nbc() + C.ns() + C.nc();
+            double* d = C.staging(total);
+            double* dv[11];
+            const double* src[11] = {pp, asave, rho_n, rho_o, rho_oo, px, py, pz, u, v, w};
+            for (int q = 0; q < 11; ++q) {
+                dv[q] = d + q * ncell;
+                TF_CHECK(hipMemcpyAsync(dv[q], src[q], sizeof(double) * ncell, hipMemcpyHostToDevice, st));
+            }
+            double* daval = d + 11 * ncell;
+            double* dabou = daval + C.nonz();
+            double* dflux = dabou + C.nbc();
+            double* db = dflux + C.ns();
+            TF_CHECK(hipMemcpyAsync(daval, aval, sizeof(double) * C.nonz(), hipMemcpyHostToDevice, st));
+            if (C.nbc() > 0) TF_CHECK(hipMemcpyAsync(dabou, abou, sizeof(double) * C.nbc(), hipMemcpyHostToDevice, st));
+            if (C.ns() > 0) TF_CHECK(hipMemcpyAsync(dflux, flux, sizeof(double) * C.ns(), hipMemcpyHostToDevice, st));
+            const tflows::CalcPSFields F{dv[1], dv[2], dv[3], dv[4], nullptr, nullptr, nullptr, nullptr, dv[5], dv[6],
+                                         dv[7]};
+            em = C.correct(dt, F, dv[0], daval, dabou, dv[8], dv[9], dv[10], dflux, db, st);
+            double* outs[3] = {u, v, w};
+            for (int q = 0; q < 3; ++q)
+                TF_CHECK(hipMemcpyAsync(outs[q], dv[8 + q], sizeof(double) * ncell, hipMemcpyDeviceToHost, st));
+            if (C.ns() > 0) TF_CHECK(hipMemcpyAsync(flux, dflux, sizeof(double) * C.ns(), hipMemcpyDeviceToHost, st));
+            TF_CHECK(hipMemcpyAsync(b, db, sizeof(double) * C.nc(), hipMemcpyDeviceToHost, st));
+            TF_CHECK(hipStreamSynchronize(st));
+        }
+        if (errmax) *errmax = em;
     });
 }

@@ -193,24 +193,27 @@ class System {
     // the same operator in sliced ELL layout (built on first use, values
     // refreshed after set_values)
     DiffSell32 sell32(hipStream_t s);
-    // Calls f with the FP32 inner operator of this system: the sliced form
-    // unless TFLOWS_FP32_OP=csr was set when the system was created.
+    // Calls f with the FP32 inner operator of this system: the sliced or
+    // the CSR form.  TFLOWS_FP32_OP=csr / =sliced (read when the system is
+    // created) fixes the form; otherwise the first use times both and keeps
+    // the faster (they give the same values).
     template <class F>
     void with_op32(hipStream_t s, F&& f) {
-        if (op32_sliced_) f(sell32(s));
+        if (op32_sliced(s)) f(sell32(s));
         else f(diff32(s));
     }
-    bool op32_sliced() const { return op32_sliced_; }
-    // The FP64 operator of the solver iterations: A in sliced ELL layout
-    // (same row sums bit for bit), or the CSR arrays themselves when
-    // TFLOWS_FP64_OP=csr was set when the system was created.
+    bool op32_sliced(hipStream_t s);
+    // The FP64 operator of the solver iterations: A in sliced ELL layout or
+    // the CSR arrays themselves (the same row sums bit for bit).
+    // TFLOWS_FP64_OP=csr / =sliced fixes the form; otherwise the first use
+    // times both and keeps the faster.
     Sell<double> sell64(hipStream_t s);
     template <class F>
     void with_op64(hipStream_t s, F&& f) {
-        if (op64_sliced_) f(sell64(s));
+        if (op64_sliced(s)) f(sell64(s));
         else f(csr());
     }
-    bool op64_sliced() const { return op64_sliced_; }
+    bool op64_sliced(hipStream_t s);
     int64_t sell32_entries(hipStream_t s) {
         sell32(s);
         return sell_entries_;
@@ -308,11 +311,13 @@ class System {
     DevArray<int> sell_off_, sell_col_, sell_map_;
     DevArray<float> sell_val_;
     int64_t sell_entries_ = 0;
-    bool sell_built_ = false, sell_valid_ = false, op32_sliced_ = true;
+    bool sell_built_ = false, sell_valid_ = false;
+    int op32_mode_ = 2;                  // 0 CSR, 1 sliced, 2 decide by timing on first use
     DevArray<int> sell64_off_, sell64_col_, sell64_map_;
     DevArray<double> sell64_val_;
     int64_t sell64_entries_ = 0;
-    bool sell64_built_ = false, sell64_valid_ = false, op64_sliced_ = true;
+    bool sell64_built_ = false, sell64_valid_ = false;
+    int op64_mode_ = 2;
     DevArray<float> off_val_, rowsum32_, diag32_;
     bool diff32_valid_ = false;
     DevArray<int> tpos_;

@@ -34,16 +34,62 @@ static int lanes_from_env() {
     return (v == 2 || v == 4 || v == 8) ? v : 1;
 }
 
-// Layout of the FP32 inner operator: TFLOWS_FP32_OP=csr keeps the CSR
-// difference form, anything else the sliced ELL form.
-static bool op32_sliced_from_env() {
-    const char* e = std::getenv("TFLOWS_FP32_OP");
-    return !(e && std::string(e) == "csr");
+// Layout of an operator from the environment: "csr" 0, "sliced" 1,
+// otherwise 2 (time both on first use).
+static int op_mode_from_env(const char* name) {
+    const char* e = std::getenv(name);
+    if (e && std::string(e) == "csr") return 0;
+    if (e && std::string(e) == "sliced") return 1;
+    return 2;
 }
-// The same for the FP64 operator (TFLOWS_FP64_OP=csr).
-static bool op64_sliced_from_env() {
-    const char* e = std::getenv("TFLOWS_FP64_OP");
-    return !(e && std::string(e) == "csr");
+
+// Mean time of `reps` launches of f on stream s (after two warm-up ones).
+template <class F>
+static float time_launches(hipStream_t s, int reps, F&& f) {
+    hipEvent_t e0, e1;
+    TF_CHECK(hipEventCreate(&e0));
+    TF_CHECK(hipEventCreate(&e1));
+    f();
+    f();
+    TF_CHECK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) f();
+    TF_CHECK(hipEventRecord(e1, s));
+    TF_CHECK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    TF_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / reps;
+}
+
+bool System::op64_sliced(hipStream_t s) {
+    if (op64_mode_ == 2) {
+        ws_.ensure((size_t)V_COUNT * n_);
+        double* x = ws_.p + (size_t)V_P * n_;
+        double* y = ws_.p + (size_t)V_Q * n_;
+        TF_CHECK(hipMemsetAsync(x, 0, sizeof(double) * n_, s));
+        const Sell<double> S = sell64(s);
+        const Csr<double> A = csr();
+        const float ts = time_launches(s, 8, [&] { launch_spmv(S, (const double*)x, y, grid_, s, ctx()); });
+        const float tc = time_launches(s, 8, [&] { launch_spmv(A, (const double*)x, y, grid_, s, ctx()); });
+        op64_mode_ = ts <= tc ? 1 : 0;
+    }
+    return op64_mode_ == 1;
+}
+
+bool System::op32_sliced(hipStream_t s) {
+    if (op32_mode_ == 2) {
+        extbuf32_.ensure(2 * (size_t)n_);
+        float* x = extbuf32_.p;
+        float* y = extbuf32_.p + n_;
+        TF_CHECK(hipMemsetAsync(x, 0, sizeof(float) * n_, s));
+        const DiffSell32 S = sell32(s);
+        const DiffCsr32 A = diff32(s);
+        const float ts = time_launches(s, 8, [&] { launch_spmv(S, (const float*)x, y, grid_, s, ctx()); });
+        const float tc = time_launches(s, 8, [&] { launch_spmv(A, (const float*)x, y, grid_, s, ctx()); });
+        op32_mode_ = ts <= tc ? 1 : 0;
+    }
+    return op32_mode_ == 1;
 }
 
 void slice_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl, bool pad_own, std::vector<int>& off,
@@ -134,7 +180,8 @@ int max_blocks_from_env() {
 System::System(int n, int nonz, const int* acol, const int* arow, const int* adia, bool device_ptrs,
                hipStream_t s, bool allow_reorder)
     : n_(n), nnz_(nonz - 1), grid_(stream_grid(n, max_blocks_from_env())), lanes_(lanes_from_env()),
-      strided_(tile_order_from_env()), op32_sliced_(op32_sliced_from_env()), op64_sliced_(op64_sliced_from_env()) {
+      strided_(tile_order_from_env()), op32_mode_(op_mode_from_env("TFLOWS_FP32_OP")),
+      op64_mode_(op_mode_from_env("TFLOWS_FP64_OP")) {
     if (n <= 0) throw std::invalid_argument("N must be positive");
     if (nonz < n + 1) throw std::invalid_argument("NONZ must be at least N + 1 (diagonal entries + 1)");
     rowptr_.ensure(n_ + 1);

@@ -151,6 +151,9 @@ def _bind_extensions(lib):
         lib.tflows_calcps_assemble.restype = C.c_int
         lib.tflows_calcps_assemble.argtypes = ([vp, C.c_int32, C.c_double] + [vp] * 11 + [vp] * 4 +
                                                [C.POINTER(C.c_double)] * 2 + [C.c_int32, vp])
+        lib.tflows_calcps_correct.restype = C.c_int
+        lib.tflows_calcps_correct.argtypes = ([vp, C.c_double] + [vp] * 10 + [vp] * 5 + [C.POINTER(C.c_double)] +
+                                              [C.c_int32, vp])
 
 
 def _check(rc: int):
@@ -523,6 +526,23 @@ class DeviceCalcPS:
             *(a.ctypes.data for a in fl), aval.ctypes.data, b.ctypes.data, flux.ctypes.data, abou.ctypes.data,
             C.byref(em), C.byref(mn), 0, None))
         return dict(aval=aval, b=b, flux=flux[:c.ns], abou=abou[:c.nbc], errmax=em.value, mean=mn.value)
+
+    def correct(self, pp, aval, abou, u, v, w, flux, fields=None, dt=None, b=None, stream=None):
+        """The correction after the pressure solve (CorUVW.f90:36-121) on
+        cuda float64 tensors: u, v, w ((-NbC:NC)) and flux (NS) in place;
+        returns (b, errmax), b the mass error per unit volume (NC)."""
+        import torch
+        c = self.case
+        if fields is None:
+            fields = {k: torch.from_numpy(getattr(c, k)).cuda() for k in CALCPS_FIELDS}
+        if b is None:
+            b = torch.empty(c.nc, dtype=torch.float64, device="cuda")
+        em = C.c_double(0.0)
+        _check(self.lib.tflows_calcps_correct(
+            self.handle, c.dt if dt is None else dt, _ptr(pp), _ptr(aval), _ptr(abou),
+            *(_ptr(fields[k]) for k in ("asave", "rho_n", "rho_o", "rho_oo", "px", "py", "pz")),
+            _ptr(u), _ptr(v), _ptr(w), _ptr(flux), _ptr(b), C.byref(em), 1, _stream(stream)))
+        return b, em.value
 
     def close(self):
         if getattr(self, "handle", None) is not None:

@@ -123,11 +123,8 @@ def test_batch_per_system_caps_and_early_exit(pkg, oracle, cuda, mixed_sizes):
 def _fp64_op(monkeypatch, form):
     """Systems created from here on use the FP64 operator `form`: "csr"
     (the LDS-staged CSR kernels, which have the lane-group forms) or
-    "sliced" (the default sliced ELL kernels, one form)."""
-    if form == "csr":
-        monkeypatch.setenv("TFLOWS_FP64_OP", "csr")
-    else:
-        monkeypatch.delenv("TFLOWS_FP64_OP", raising=False)
+    "sliced" (the sliced ELL kernels, one form)."""
+    monkeypatch.setenv("TFLOWS_FP64_OP", form)
 
 
 @pytest.mark.parametrize("form", ["csr", "sliced"])

@@ -101,9 +101,9 @@ def test_difference_form_fp32_operator(pkg, cuda):
 
 @pytest.mark.parametrize("case", ["pressure", "pressure_perm", "momentum"])
 def test_sliced_fp32_operator_matches_csr_form(pkg, cuda, monkeypatch, case):
-    """The FP32 inner operator in sliced ELL layout (the default) sums each
-    row's terms in the same order as the CSR difference form
-    (TFLOWS_FP32_OP=csr, read when a system is created); its padding adds
+    """The FP32 inner operator in sliced ELL layout sums each row's terms in
+    the same order as the CSR difference form (TFLOWS_FP32_OP=sliced / =csr,
+    read when a system is created); its padding adds
     0 * (p_i - p_i).  Products, and the mixed solves built on them, agree
     with the CSR form's (== comparison: the padding can only turn a -0 row
     sum into +0)."""
@@ -118,10 +118,7 @@ def test_sliced_fp32_operator_matches_csr_form(pkg, cuda, monkeypatch, case):
     method = "cgs" if case == "momentum" else "cg"
     out = {}
     for form in ("csr", "sliced"):
-        if form == "csr":
-            monkeypatch.setenv("TFLOWS_FP32_OP", "csr")
-        else:
-            monkeypatch.delenv("TFLOWS_FP32_OP")
+        monkeypatch.setenv("TFLOWS_FP32_OP", form)
         ds = pkg.krylov.DeviceSystem(m)
         y = torch.zeros(m.n, dtype=torch.float32, device=cuda)
         ds.spmv32(p, y)

@@ -3,9 +3,11 @@
 The FP64 iterations run their SpMVs on A in sliced ELL layout (64-row
 slices, slot-major, each row's entries in CSR order, padding skipped); every
 row sum is the reference loop's (Process/CG.f90:193-199) bit for bit, so
-solves on the sliced operator and on the CSR arrays (TFLOWS_FP64_OP=csr,
-read when a system is created) agree bit for bit: every method and
-preconditioner, natural and locality order, batches.
+solves on the sliced operator and on the CSR arrays (TFLOWS_FP64_OP=csr /
+=sliced, read when a system is created) agree bit for bit: every method and
+preconditioner, natural and locality order, batches.  Unset, a system times
+both forms on its first use and keeps the faster; its results are the same
+bits whichever it keeps.
 """
 import numpy as np
 import pytest
@@ -13,14 +15,16 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _both(monkeypatch, make):
+def _both(monkeypatch, make, with_auto=False):
     out = {}
-    for form in ("csr", "sliced"):
-        if form == "csr":
-            monkeypatch.setenv("TFLOWS_FP64_OP", "csr")
-        else:
+    for form in ("csr", "sliced") + (("auto",) if with_auto else ()):
+        if form == "auto":
             monkeypatch.delenv("TFLOWS_FP64_OP", raising=False)
+        else:
+            monkeypatch.setenv("TFLOWS_FP64_OP", form)
         out[form] = make()
+    if with_auto:
+        return out["csr"], out["sliced"], out["auto"]
     return out["csr"], out["sliced"]
 
 
@@ -48,10 +52,11 @@ def test_sliced_solves_bitwise(pkg, oracle, cuda, monkeypatch, case, method, pre
         ds.close()
         return out
 
-    a, b = _both(monkeypatch, make)
+    a, b, c = _both(monkeypatch, make, with_auto=True)
     assert np.array_equal(a[0], b[0]) and np.array_equal(b[0], oracle.spmv(m, p.cpu().numpy()))
-    assert a[1] == b[1]
+    assert a[1] == b[1] == c[1]
     assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+    assert np.array_equal(a[2], c[2]) and np.array_equal(a[0], c[0])
 
 
 def test_sliced_batch_bitwise(pkg, cuda, monkeypatch):
