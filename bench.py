@@ -398,10 +398,9 @@ def workload_wall(args, torch, pkg):
 
     def mixed_dic():
         xm.zero_()
-        return ds.solve_mixed("cg", 2, xm, b, args.dic_cap, s.tol, inner_rtol=args.inner_rtol)
-    mixed_dic()   # FP32 copies of the values
-    ms, (it, ini, fin, outer) = _timed(torch, mixed_dic, 1)
-    out["mixed_cg_prec2"] = dict(ms=round(ms, 1), iters=it, converged=bool(fin < s.tol), outer=outer)
+        return ds.solve_mixed_reliable(2, xm, b, args.dic_cap, s.tol, interval=args.interval)
+    ms, (it, ini, fin, rep) = _timed(torch, mixed_dic, 1)
+    out["mixed_cg_prec2"] = dict(ms=round(ms, 1), iters=it, converged=bool(fin < s.tol), replacements=rep)
     t = time_spmv(torch, ds, m.n, reps=20)
     out["spmv_fp64_GBs"] = round(spmv_bytes(m.n, m.nnz, 8) / t / 1e9, 1)
     sw = dic_sweep_cost(torch, ds, s, args.sweep_iters)

@@ -122,10 +122,11 @@ int tflows_solve_mixed(tflows_system s, int32_t method, int32_t prec, double* x,
                        int32_t* niter, double tol, double inner_rtol, int32_t max_outer, double* ini_res,
                        double* fin_res, int32_t* outer_steps, void* stream);
 
-/* Mixed precision, reliable-update form (CG, prec 0 or 1): a single FP32 CG
- * recurrence (SpMV in difference form, preconditioner, vector updates and
- * dot products in FP32) with an FP64 refinement step every `interval`
- * iterations and whenever the FP32 recurrence residual reaches tol: the
+/* Mixed precision, reliable-update form (CG, prec 0, 1 or 2): a single FP32
+ * CG recurrence (SpMV in difference form, preconditioner -- DIC sweeps in
+ * FP32 for prec 2 -- vector updates and dot products in FP32) with an FP64
+ * refinement step every `interval` iterations and whenever the FP32
+ * recurrence residual reaches tol or falls 100x below the last FP64 norm: the
  * FP32 correction is merged into x in FP64, the true residual b - A x is
  * recomputed in FP64 and replaces the FP32 residual; the search direction
  * is kept.  Stops when the FP64 true residual satisfies ||b - A x|| < tol,

@@ -398,10 +398,118 @@ SolveResult System::solve_mixed_reliable(int prec, double* x, const double* b, d
     return res;
 }
 
+// The FP32 recurrence of the reliable-update forms pauses for a residual
+// replacement once it has dropped this far below the last FP64 norm.
+constexpr double kRelDrop = 1e-2;
+
+// (r32, r32) of the replaced FP32 residual -> site B, so that the stopping
+// test of the next iteration reads it (the second value, rho of the Jacobi
+// forms, is not used by the DIC iteration).
+__global__ __launch_bounds__(kBlock) void k_rel_site_b(int n, const float* __restrict__ r, float* __restrict__ partB) {
+    float v[2] = {0.0f, 0.0f};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) v[0] += r[i] * r[i];
+    site_write<float, 2>(v, partB);
+}
+
+// The reliable-update form with DIC (prec = 2).  The FP32 recurrence is
+// the reference's preconditioned CG (CG.f90:113-259) with the DIC sweeps in
+// FP32 (pivots computed in FP64, Prec1.f90:51-59, then rounded), run in
+// chunks of at most `interval` iterations; after each chunk x64 +=
+// fp64(x32), the FP64 true residual b - A x64 is formed and, unless it
+// meets tol, replaces the FP32 residual while the search direction p
+// carries on.  A chunk also ends once the FP32 recurrence norm has dropped
+// below max(tol, 1e-2 * the last FP64 norm), as in the Jacobi form: the
+// replacement then comes while the recurrence still tracks the true
+// residual, so the next beta = rho_new / rho_old stays sound (replacing a
+// recurrence that ran far below its true residual stalls the solve).  The
+// solve stops when the FP64 true residual meets tol, or after `cap` FP32
+// iterations (niter = cap + 1).
+SolveResult System::solve_mixed_reliable_dic(double* x, const double* b, double* r_out, int cap, double tol,
+                                             int interval, int* outer_steps, hipStream_t s) {
+    const size_t n = (size_t)n_;
+    ws32_.ensure((V_COUNT + 2) * n);
+    partials32_.ensure(Sites<float>::elems(grid_));
+    state32_.ensure(1);
+    r64_.ensure(n);
+    if (!mirror32_)
+        TF_CHECK(hipHostMalloc((void**)&mirror32_, sizeof(HostMirror), hipHostMallocMapped | hipHostMallocCoherent));
+    ws_.ensure((size_t)V_COUNT * n);
+    const Csr<float> A32 = csr32(s);
+    KrylovVectors<float> w;
+    float* base = ws32_.p;
+    auto vec = [&](int k) { return base + (size_t)k * n; };
+    w.D = vec(V_D); w.p = vec(V_P); w.q = vec(V_Q); w.r2 = vec(V_R2); w.p2 = vec(V_P2); w.q2 = vec(V_Q2);
+    w.u1 = vec(V_U1); w.u2 = vec(V_U2); w.v2 = vec(V_V2);
+    w.x = vec(V_COUNT);
+    w.r1 = vec(V_COUNT + 1);
+    TF_CHECK(hipMemsetAsync(ws32_.p, 0, sizeof(float) * (V_COUNT + 2) * n, s));
+    const Sites<float> S32 = Sites<float>::carve(partials32_.p, grid_);
+    const Levels* L = &levels();
+    const int nb = (n_ + kBlock - 1) / kBlock;
+    double* d64 = ws_.p;                            // FP64 scratch slot
+    dic_setup(d64, s);
+    prepare_dic(true, s);
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(nb), dim3(kBlock), 0, s, n_, (const double*)d64, w.D);
+
+    double* r64 = r_out ? r_out : r64_.p;
+    double err = true_residual(x, b, r64, s);
+    SolveResult res;
+    res.ini_res = err;
+    res.ini_written = true;
+    int replacements = 0, k_next = 1, ran = 0;
+    if (err >= tol && cap > 0) {
+        hipLaunchKernelGGL(k_scale_to_f32, dim3(nb), dim3(kBlock), 0, s, n_, (const double*)r64, 1.0, w.r1, w.x);
+        SolverState h{};
+        h.tol = tol;
+        h.cap = cap;
+        TF_CHECK(hipMemcpyAsync(state32_.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
+        with_op32(s, [&](const auto& Aop) {
+            using Op = std::decay_t<decltype(Aop)>;
+            start_solve<float, Op>(ctx(), 0, 2, Aop, A32, w, S32, state32_.p, mirror32_, L, s, true, true);
+            for (;;) {
+                // this chunk's pause target for the FP32 recurrence
+                const double target = std::max(tol, kRelDrop * err);
+                TF_CHECK(hipMemcpy(&state32_.p->tol, &target, sizeof(double), hipMemcpyHostToDevice));
+                for (int i = 0; i < interval && k_next + i <= cap + 1; ++i)
+                    enqueue_iteration<float, Op>(ctx(), 0, false, k_next + i, Aop, A32, w, S32, state32_.p,
+                                                 mirror32_, L, s);
+                const int k_end = std::min(k_next + interval - 1, cap + 1);
+                SolverState h2;
+                TF_CHECK(hipMemcpyAsync(&h2, state32_.p, sizeof(h2), hipMemcpyDeviceToHost, s));
+                TF_CHECK(hipStreamSynchronize(s));
+                // iterations that ran: up to the stopping test that fired, or the chunk
+                const int last = h2.done ? h2.niter_out : k_end;
+                ran = std::min(last, cap);
+                hipLaunchKernelGGL(k_rel_merge, dim3(nb), dim3(kBlock), 0, s, n_, x, w.x);
+                err = true_residual(x, b, r64, s);
+                ++replacements;
+                if (err < tol || ran >= cap) break;
+                // the FP32 residual restarts from the FP64 one; the next
+                // iteration's stopping test reads its norm
+                hipLaunchKernelGGL(k_scale_to_f32, dim3(nb), dim3(kBlock), 0, s, n_, (const double*)r64, 1.0, w.r1,
+                                   w.x);
+                hipLaunchKernelGGL(k_rel_site_b, dim3(grid_), dim3(kBlock), 0, s, n_, (const float*)w.r1, S32.B);
+                // a pause that ran no iteration cannot be resumed: the FP32
+                // norm of the replaced residual is below tol, its FP64 norm
+                // is not (they differ by FP32 rounding); report as it stands
+                if (h2.done && last < k_next) break;
+                if (h2.done) launch_set_int(&state32_.p->done, 0, s);
+                k_next = last + 1;
+            }
+        });
+        check_dic(s);
+    }
+    TF_CHECK(hipGetLastError());
+    if (outer_steps) *outer_steps = replacements;
+    res.niter = err < tol ? ran : ran + 1;
+    res.fin_res = err;
+    return res;
+}
+
 SolveResult System::solve_mixed_reliable_impl(int prec, double* x, const double* b, double* r_out, int cap,
                                               double tol, int interval, int* outer_steps, hipStream_t s) {
-    if (prec == 2) throw std::invalid_argument("reliable-update mode supports prec 0 and 1");
     if (interval < 1) interval = 1;
+    if (prec == 2) return solve_mixed_reliable_dic(x, b, r_out, cap, tol, interval, outer_steps, s);
     const size_t n = (size_t)n_;
     ws32_.ensure((V_COUNT + 2) * n);
     partials32_.ensure(Sites<float>::elems(grid_));

@@ -286,6 +286,8 @@ class System {
                            hipStream_t s);
     SolveResult solve_mixed_impl(int method, int prec, double* x, const double* b, double* r_out, int cap,
                                  double tol, double inner_rtol, int max_outer, int* outer_steps, hipStream_t s);
+    SolveResult solve_mixed_reliable_dic(double* x, const double* b, double* r_out, int cap, double tol,
+                                         int interval, int* outer_steps, hipStream_t s);
     SolveResult solve_mixed_reliable_impl(int prec, double* x, const double* b, double* r_out, int cap, double tol,
                                           int interval, int* outer_steps, hipStream_t s);
     void build_storage(bool allow_reorder, hipStream_t s);

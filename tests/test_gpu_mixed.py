@@ -139,6 +139,9 @@ def test_sliced_fp32_operator_matches_csr_form(pkg, cuda, monkeypatch, case):
 @pytest.mark.parametrize("case,prec,interval", [
     ("pressure12", 1, 50), ("pressure12", 0, 7), ("pressure10_perm", 1, 1), ("pressure10_perm", 1, 1000),
     ("momentum10", 1, 20),
+    # DIC: the generic FP32 CG with FP32 sweeps, a replacement every
+    # `interval` iterations
+    ("pressure12", 2, 50), ("pressure10_perm", 2, 7), ("wall20", 2, 25),
 ])
 def test_mixed_reliable_matches_fp64_oracle(pkg, oracle, cuda, case, prec, interval):
     """FP32 CG with FP64 residual replacement: the FP64 true residual meets
@@ -148,7 +151,8 @@ def test_mixed_reliable_matches_fp64_oracle(pkg, oracle, cuda, case, prec, inter
     mg = pkg.meshgen
     s = {"pressure12": lambda: mg.pressure_system(12, seed=51),
          "pressure10_perm": lambda: mg.pressure_system(10, seed=52, perm_seed=53),
-         "momentum10": lambda: mg.momentum_system(10, seed=54)}[case]()
+         "momentum10": lambda: mg.momentum_system(10, seed=54),
+         "wall20": lambda: mg.wall_pressure_system(20, seed=55)}[case]()
     m = s.matrix
     A = m.to_scipy()
     ds = pkg.krylov.DeviceSystem(m)
