@@ -361,6 +361,87 @@ def workload_calcps(args, torch, pkg):
     return res
 
 
+def uvw_bytes(case, var):
+    """Algorithmic bytes of one momentum assembly (NewUVW.f90:137-499):
+    every input read once -- per face SideC(1:2), SidAij(1:2), the face's
+    boundary kind, S, D, f, fF, Scoef, Flux_u (97 B); per cell (-NbC:NC)
+    rho n/o/oo, Ui n/o/oo, source, Pi, seven gradients, volume and the
+    material (132 B) -- and every output written once (Aval, b, rhoUi,
+    Abou, Asave for U)."""
+    ncell = case.nbc + case.nc + 1
+    return (97 * case.ns + 132 * ncell + 8 * case.nonz + 8 * case.nc + 8 * ncell + 8 * case.nbc +
+            (8 * case.nc if var == 1 else 0))
+
+
+def workload_uvw(args, torch, pkg):
+    """Momentum assembly (NewUVW.f90:137-499) of U, V and W on the n^3
+    channel box on the GPU, then the NewUVW step: the three BiCG solves
+    from rhoUi (cap 200, :505) and the update (:526)."""
+    n = args.n
+    mg = pkg.meshgen
+    log(f"generating the {n}^3 momentum case")
+    case = mg.calcps_case((n, n, n), seed=1)
+    uf = mg.uvw_fields(case, seed=2)
+    dc = pkg.krylov.DeviceCalcPS(case)
+    dc.uvw_setup(uf)
+    params = [mg.uvw_params(var=v, blend=[10], blend_tem=[10], urf=0.7) for v in (1, 2, 3)]
+    inputs = [dc.uvw_inputs(mg.uvw_args(case, uf, v)) for v in (1, 2, 3)]
+    outs = [dc.assemble_uvw(params[v], inputs[v]) for v in range(3)]
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    t_var = []
+    for v in range(3):
+        e0.record(s)
+        for _ in range(reps):
+            dc.assemble_uvw(params[v], inputs[v], out=outs[v])
+        e1.record(s)
+        e1.synchronize()
+        t_var.append(e0.elapsed_time(e1) / reps)
+    nbytes = [uvw_bytes(case, v) for v in (1, 2, 3)]
+    m = case.matrix(outs[0]["aval"].cpu().numpy())
+    systems = [pkg.krylov.DeviceSystem(m) for _ in range(3)]
+    nb = case.nbc
+    rho = torch.from_numpy(case.rho_n).cuda()
+    ui = [t[3].clone() for t in inputs]
+    tol = [1e-8 * float(torch.linalg.norm(o["b"])) for o in outs]
+
+    def step():
+        its = []
+        for v in range(3):
+            o = dc.assemble_uvw(params[v], inputs[v], out=outs[v])
+            systems[v].set_values(o["aval"])
+            x = o["rhoui"][nb + 1:]
+            its.append(systems[v].solve("bicg", 1, x, o["b"], 200, tol[v])[0])
+            dc.uvw_update(o["rhoui"], rho, ui[v])
+        return its
+    step()
+    ms_step, its = _timed(torch, step, args.steps)
+    gbs = [b / (t * 1e-3) / 1e9 for b, t in zip(nbytes, t_var)]
+    res = {"metric": "momentum assembly of U, V, W on the GPU, and the NewUVW step", "unit": "ms",
+           "workload": f"channel {n}^3 (NC={case.nc}, NS={case.ns}, NbC={case.nbc}, NONZ={case.nonz}), "
+                       f"central convection with the upwind part, LIN inertia, URF 0.7",
+           "assembly_ms": [round(t, 4) for t in t_var], "assembly_bytes": nbytes,
+           "assembly_GBs": [round(g, 1) for g in gbs],
+           "roofline": {"bound": "hbm", "achieved": round(gbs[0], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs[0] / HBM_PEAK_GBS, 4)},
+           "step_ms": round(ms_step, 2), "bicg_iters": its}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        orc = O.Oracle()
+        t0 = time.perf_counter()
+        orc.newuvw(case, uf, params[0])
+        el = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(el * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": f"one U assembly of the same {n}^3 case by oracle/calcps_oracle.c, "
+                                         f"1 thread"}
+    for q in systems:
+        q.close()
+    dc.close()
+    return res
+
+
 def workload_wall(args, torch, pkg):
     """256^3 wall-clustered pressure system (aspect up to 50:1): FP64 CG
     with Jacobi and with DIC, mixed CG; SpMV and DIC-iteration bandwidth."""
@@ -433,7 +514,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--workload", default="pressure",
                     choices=["pressure", "momentum", "batch-transport", "batch-pressure", "wall", "dic-sweep",
-                             "adjoint", "calcps"],
+                             "adjoint", "calcps", "uvw"],
                     help="pressure: the headline 128^3 benchmark; the others measure the remaining "
                          "configurations and print their own JSON line")
     ap.add_argument("--method", default=None, help="Krylov method of the batch workloads")
@@ -462,6 +543,8 @@ def main():
             out = workload_batch(args, torch, pkg, args.workload.split("-")[1])
         elif args.workload == "calcps":
             out = workload_calcps(args, torch, pkg)
+        elif args.workload == "uvw":
+            out = workload_uvw(args, torch, pkg)
         elif args.workload in ("wall", "dic-sweep"):
             if args.n == 128:
                 args.n = 256

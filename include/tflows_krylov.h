@@ -288,6 +288,45 @@ int tflows_calcps_correct(tflows_calcps h, double dt, const double* pp, const do
                           const double* px, const double* py, const double* pz, double* u, double* v, double* w,
                           double* flux, double* b, double* errmax, int32_t mem, void* stream);
 
+/* The momentum assembly of one velocity component, Process/NewUVW.f90:137-499
+ * (the reference's `newuvw` up to its BiCG call at :505-507), on the mesh of
+ * a tflows_calcps handle.  Covered: CONVEC, DIFFUS (CN or FI), CROSS = FI or
+ * not; INERT = LIN or PAR; BLEND = YES or NO (central convection; the TVD
+ * face schemes of Process/ConvScheme.f90 are refused with an error);
+ * BLEND_TEM upwind part; every boundary type of :383-408; Asave (var = 1);
+ * under-relaxation with U%URF.  VISeff = VISc (the default of :287-295).
+ * Diagonal, off-diagonals, b, Abou, Asave and rhoUi are the reference's bit
+ * for bit.  The caller keeps the old-level shift (:154-157) and supplies
+ * the source (:180-199); the solve is tflows_solve / bicg_ with rhoUi(1:NC)
+ * as the starting point, then tflows_calcps_uvw_update.
+ *
+ * tflows_calcps_uvw_setup: Scoef(NS), fF(NS) (Process/Calc3.f90:23-70) and
+ * material(-NbC:NC) with values 1..nmaterials (at most 8), once per mesh.
+ * tflows_calcps_uvw_assemble: flags and constants in *p (codes of
+ * Process/IniPar.f90:22-32; per-material arrays at [m - 1]); var selects
+ * Si..Dk from the handle's Sx..Dz as Process/Processor.f90:314-340 does; the
+ * cell arrays (-NbC:NC) rho_n/o/oo, Ui n/o/oo, source, Pi and the seven
+ * gradients in newuvw's argument order (NewUVW.f90:1-41); Flux_u(NS).
+ * Writes Aval(NONZ), b(NC), Abou(-NbC:-1), Asave (cells 1..NC of a
+ * (-NbC:NC) array, var = 1, may be NULL) and rhoUi(-NbC:NC).
+ * tflows_calcps_uvw_update: Ui%n(1:NC) = rhoUi / rho%n (:526). */
+typedef struct tflows_uvw_params {
+    int32_t var, convec, diffus, cross, inert;
+    double visc, dt, urf;
+    int32_t blend[8], blend_tem[8];
+    double urfc[8];
+} tflows_uvw_params;
+int tflows_calcps_uvw_setup(tflows_calcps h, const double* scoef, const double* fF, const int32_t* material,
+                            int32_t nmaterials, int32_t mem, void* stream);
+int tflows_calcps_uvw_assemble(tflows_calcps h, const tflows_uvw_params* p, const double* rho_n,
+                               const double* rho_o, const double* rho_oo, const double* ui_n, const double* ui_o,
+                               const double* ui_oo, const double* source, const double* pi, const double* dUidi,
+                               const double* dUidj, const double* dUidk, const double* dUjdi, const double* dUkdi,
+                               const double* dUjdj, const double* dUkdk, const double* flux_u, double* aval,
+                               double* b, double* abou, double* asave, double* rhoui, int32_t mem, void* stream);
+int tflows_calcps_uvw_update(tflows_calcps h, const double* rhoui, const double* rho_n, double* ui_n, int32_t mem,
+                             void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -207,3 +207,162 @@ double oracle_coruvw(int nc, int nbc, int ns, const int *sidec, const int *sidai
     return em;
 #undef C
 }
+
+/*
+ * oracle_newuvw -- the momentum assembly of one velocity component,
+ * Process/NewUVW.f90:137-499 (everything before the BiCG call at :507; the
+ * shift of the old levels at :154-157 and the source terms at :180-199 are
+ * the caller's, the source arrives as an array):
+ *
+ *   :137-138  rhoUi = rho^n Ui^n on (-NbC:NC)
+ *   :146-149  Abou = 0, Aval = 0, b = 0
+ *   :175-176  C = 0, X = 0
+ *   :205-240  convection face loop: rhoUiS = f rhoUi(c1) + (1-f) rhoUi(c2),
+ *             left as it is by ConvScheme for BLEND = YES
+ *             (Process/ConvScheme.f90:107-108); C(c1) -= Flux_u rhoUiS,
+ *             C(c2) += ...; the upwind part X when BLEND_TEM /= NO
+ *   :252-257  CONVEC = FI: b += URFC(material) (C - X)
+ *   :264-266  X = 0 on the cells
+ *   :277-416  diffusion face loop: the cross-diffusion flux Fex - Fim into
+ *             X; D12 = A0/rho(c2), D21 = A0/rho(c1) for DIFFUS = FI; the
+ *             upwind A12, A21 when BLEND /= NO; the "m1" matrix fill for
+ *             interior faces and the boundary branches (WALL, CONVECT,
+ *             INFLOW, WALLFL; OUTFLOW; BUFFER into Abou)
+ *   :434-438  CROSS = FI: b += X
+ *   :447-456  inertia, INERT = LIN (two levels) or PAR (three levels)
+ *   :484      b -= Pi V
+ *   :497-499  Asave = diagonal (var = 1); under-relaxation with U%URF
+ *
+ * The flag and boundary codes are those of Process/IniPar.f90:24-32 and
+ * Library/allp_mod.f90:16-24.  Per-material arrays BLEND, BLEND_TEM, URFC
+ * are 1-based (material m at [m-1]); cell arrays (-NbC:NC) as elsewhere.
+ * VISeff = VISc (the default branch of :287-295).  A face whose materials'
+ * BLEND values have max(...) other than YES calls a ConvScheme branch this
+ * restatement does not cover (a TVD scheme, or NO mixed with YES, for which
+ * the reference leaves Phij undefined): returns -1.  abou is (-NbC:-1);
+ * asave (NC) may be NULL; cbuf and xbuf hold NbC+NC+1 doubles.  Returns 0.
+ */
+enum { FL_FI = 3, FL_CN = 2, IN_LIN = 4, IN_PAR = 5, BL_YES = 10, BL_NO = 11 };
+
+int oracle_newuvw(int nc, int nbc, int ns, int nonz, int var, int convec, int diffus, int cross, int inert,
+                  const int *sidec, const int *sidaij, const int *adia, const int *typebc, const int *material,
+                  const int *blend, const int *blend_tem, const double *urfc, const double *f, const double *fF,
+                  const double *scoef, const double *flux_u, const double *si, const double *sj, const double *sk,
+                  const double *di, const double *dj, const double *dk, const double *volume, const double *rho_n,
+                  const double *rho_o, const double *rho_oo, const double *ui_n, const double *ui_o,
+                  const double *ui_oo, const double *source, const double *pi, const double *dUidi,
+                  const double *dUidj, const double *dUidk, const double *dUjdi, const double *dUkdi,
+                  const double *dUjdj, const double *dUkdk, double visc, double dt, double urf, double *aval,
+                  double *b, double *abou, double *asave, double *rhoui, double *cbuf, double *xbuf)
+{
+#define C(x, c) (x)[nbc + (c)]
+#define MAT(arr, c) (arr)[C(material, c) - 1]
+    const int nall = nbc + nc + 1;
+    for (int c = -nbc; c <= nc; ++c) C(rhoui, c) = C(rho_n, c) * C(ui_n, c);   /* :137-138 */
+    for (int c = 0; c < nbc; ++c) abou[c] = 0.0;                        /* :146 */
+    for (int j = 0; j < nonz; ++j) aval[j] = 0.0;                       /* :148 */
+    for (int c = 0; c < nc; ++c) b[c] = 0.0;                            /* :149 */
+    for (int c = 0; c < nall; ++c) { cbuf[c] = 0.0; xbuf[c] = 0.0; }   /* :175-176 */
+
+    for (int s = 1; s <= ns; ++s) {                                     /* :205-240 */
+        const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
+        const double rhoUiS = f[s - 1] * C(rhoui, c1) + (1.0 - f[s - 1]) * C(rhoui, c2);
+        if (MAT(blend, c1) != BL_NO || MAT(blend, c2) != BL_NO) {
+            const int bl = MAT(blend, c1) > MAT(blend, c2) ? MAT(blend, c1) : MAT(blend, c2);
+            if (bl != BL_YES) return -1;                                /* ConvScheme: not covered */
+        }
+        const double F = flux_u[s - 1];
+        if (c2 > 0) {
+            C(cbuf, c1) = C(cbuf, c1) - F * rhoUiS;
+            C(cbuf, c2) = C(cbuf, c2) + F * rhoUiS;
+        } else {
+            C(cbuf, c1) = C(cbuf, c1) - F * rhoUiS;
+        }
+        if (MAT(blend_tem, c1) != BL_NO || MAT(blend_tem, c2) != BL_NO) {
+            if (F < 0) {
+                C(xbuf, c1) = C(xbuf, c1) - F * C(rhoui, c2);
+                if (c2 > 0) C(xbuf, c2) = C(xbuf, c2) + F * C(rhoui, c2);
+            } else {
+                C(xbuf, c1) = C(xbuf, c1) - F * C(rhoui, c1);
+                if (c2 > 0) C(xbuf, c2) = C(xbuf, c2) + F * C(rhoui, c1);
+            }
+        }
+    }
+    if (convec == FL_FI)                                                /* :252-257 */
+        for (int c = 1; c <= nc; ++c)
+            b[c - 1] = b[c - 1] + MAT(urfc, c) * (C(cbuf, c) - C(xbuf, c));
+    for (int c = 1; c <= nc; ++c) C(xbuf, c) = 0.0;                     /* :264-266 */
+
+    for (int s = 1; s <= ns; ++s) {                                     /* :277-416 */
+        double A12 = 0.0, A21 = 0.0, D12 = 0.0, D21 = 0.0;
+        const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
+        const double ff = fF[s - 1], gg = 1.0 - fF[s - 1];
+        const double VISeff = visc;
+        const double dUidiS = ff * C(dUidi, c1) + gg * C(dUidi, c2);
+        const double dUidjS = ff * C(dUidj, c1) + gg * C(dUidj, c2);
+        const double dUidkS = ff * C(dUidk, c1) + gg * C(dUidk, c2);
+        const double dUjdiS = ff * C(dUjdi, c1) + gg * C(dUjdi, c2);
+        const double dUkdiS = ff * C(dUkdi, c1) + gg * C(dUkdi, c2);
+        const double dUjdjS = ff * C(dUjdj, c1) + gg * C(dUjdj, c2);
+        const double dUkdkS = ff * C(dUkdk, c1) + gg * C(dUkdk, c2);
+        const double Fex = VISeff * ((((4.0 / 3.0) * dUidiS * si[s - 1] + (dUidjS + dUjdiS) * sj[s - 1]) +
+                                      (dUidkS + dUkdiS) * sk[s - 1]) -
+                                     (2.0 / 3.0) * (dUjdjS + dUkdkS) * si[s - 1]);
+        const double A0 = VISeff * scoef[s - 1];
+        const double Fim = A0 * ((dUidiS * di[s - 1] + dUidjS * dj[s - 1]) + dUidkS * dk[s - 1]);
+        C(xbuf, c1) = C(xbuf, c1) + Fex - Fim;                          /* :338-341 */
+        if (c2 > 0) C(xbuf, c2) = C(xbuf, c2) - Fex + Fim;
+        if (diffus == FL_CN || diffus == FL_FI) {
+            if (diffus == FL_FI) {
+                D12 = D12 + A0 / C(rho_n, c2);
+                D21 = D21 + A0 / C(rho_n, c1);
+            }
+            if (MAT(blend, c1) != BL_NO || MAT(blend, c2) != BL_NO) {
+                const double F = flux_u[s - 1];
+                A12 = A12 - (F < 0.0 ? F : 0.0);
+                A21 = A21 + (F > 0.0 ? F : 0.0);
+            }
+            if (c2 > 0) {                                               /* "m1" */
+                aval[adia[c1 - 1] - 1] = aval[adia[c1 - 1] - 1] + A21 + D21;
+                aval[sidaij[2 * (s - 1) + 1] - 1] = aval[sidaij[2 * (s - 1) + 1] - 1] - A21 - D21;
+                aval[adia[c2 - 1] - 1] = aval[adia[c2 - 1] - 1] + A12 + D12;
+                aval[sidaij[2 * (s - 1)] - 1] = aval[sidaij[2 * (s - 1)] - 1] - A12 - D12;
+            } else if (c2 < 0) {
+                const int t = C(typebc, c2);
+                if (t == BC_WALL || t == BC_CONVECT || t == BC_INFLOW || t == BC_WALLFL) {
+                    aval[adia[c1 - 1] - 1] = aval[adia[c1 - 1] - 1] + A21 + D21;
+                    b[c1 - 1] = b[c1 - 1] + (A12 + D12) * C(rhoui, c2);
+                } else if (t == BC_OUTFLOW) {
+                    aval[adia[c1 - 1] - 1] = aval[adia[c1 - 1] - 1] + A21;
+                    b[c1 - 1] = b[c1 - 1] + A12 * C(rhoui, c2);
+                } else if (t == BC_BUFFER) {
+                    aval[adia[c1 - 1] - 1] = aval[adia[c1 - 1] - 1] + A21 + D21;
+                    C(abou, c2) = -A12 - D12;
+                }
+            }
+        }
+    }
+    if (cross == FL_FI)                                                 /* :434-438 */
+        for (int c = 1; c <= nc; ++c) b[c - 1] = b[c - 1] + C(xbuf, c);
+    if (inert == IN_LIN)                                                /* :447-450 */
+        for (int c = 1; c <= nc; ++c) {
+            aval[adia[c - 1] - 1] = aval[adia[c - 1] - 1] + C(volume, c) / dt;
+            b[c - 1] = b[c - 1] + C(volume, c) / dt * C(rho_o, c) * C(ui_o, c) + C(source, c) * C(volume, c);
+        }
+    if (inert == IN_PAR)                                                /* :453-456 */
+        for (int c = 1; c <= nc; ++c) {
+            aval[adia[c - 1] - 1] = aval[adia[c - 1] - 1] + 1.5 * C(volume, c) / dt;
+            b[c - 1] = b[c - 1] + 2.0 * C(volume, c) / dt * C(rho_o, c) * C(ui_o, c) -
+                       0.5 * C(volume, c) / dt * C(rho_oo, c) * C(ui_oo, c) + C(source, c) * C(volume, c);
+        }
+    for (int c = 1; c <= nc; ++c) b[c - 1] = b[c - 1] - C(pi, c) * C(volume, c);   /* :484 */
+    if (var == 1 && asave)                                              /* :497 */
+        for (int c = 1; c <= nc; ++c) asave[c - 1] = aval[adia[c - 1] - 1];
+    for (int c = 1; c <= nc; ++c) {                                     /* :498-499 */
+        b[c - 1] = b[c - 1] + aval[adia[c - 1] - 1] * (1.0 - urf) * C(rhoui, c) / urf;
+        aval[adia[c - 1] - 1] = aval[adia[c - 1] - 1] / urf;
+    }
+    return 0;
+#undef MAT
+#undef C
+}

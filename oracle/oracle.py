@@ -194,6 +194,56 @@ class Oracle(_Solvers):
         return dict(u=u, v=v, w=w, flux=fl, b=b, errmax=em)
 
 
+    def newuvw(self, case, uf, params):
+        """Momentum assembly of one component (oracle/calcps_oracle.c, after
+        Process/NewUVW.f90:137-499) on a meshgen.CalcPSCase with the inputs
+        of meshgen.uvw_fields and the flags of meshgen.uvw_params.  Returns
+        dict aval (NONZ), b (NC), abou (NbC), asave ((-NbC:NC), cells 1..NC
+        written for var = 1, zero elsewhere), rhoui ((-NbC:NC)); raises
+        ValueError for a BLEND this restatement does not cover."""
+        fn = self.lib.oracle_newuvw
+        if not getattr(fn, "_typed", False):
+            fn.restype = C.c_int
+            fn.argtypes = [C.c_int] * 9 + [_i32p] * 7 + [_f64p] * 27 + [C.c_double] * 3 + [_f64p] * 7
+            fn._typed = True
+        var = int(params["var"])
+        a = _uvw_args(case, uf, var)
+        nall = case.nbc + case.nc + 1
+        c32 = lambda x: np.ascontiguousarray(x, np.int32)     # noqa: E731
+        c64 = lambda x: np.ascontiguousarray(x, np.float64)   # noqa: E731
+        aval = np.zeros(case.nonz)
+        b = np.zeros(case.nc)
+        abou = np.zeros(max(case.nbc, 1))
+        asave = np.zeros(nall)
+        rhoui = np.zeros(nall)
+        cbuf, xbuf = np.zeros(nall), np.zeros(nall)
+        rc = fn(case.nc, case.nbc, case.ns, case.nonz, var, params["convec"], params["diffus"], params["cross"],
+                params["inert"], c32(case.sidec.T), c32(case.sidaij.T), c32(case.adia), c32(case.typebc),
+                c32(uf["material"]), c32(params["blend"]), c32(params["blend_tem"]), c64(params["urfc"]),
+                c64(case.f), c64(uf["ff"]), c64(uf["scoef"]), c64(a["flux_u"]), *(c64(x) for x in a["S"]),
+                *(c64(x) for x in a["D"]), c64(case.volume), c64(a["rho_n"]), c64(a["rho_o"]), c64(a["rho_oo"]),
+                c64(a["ui_n"]), c64(a["ui_o"]), c64(a["ui_oo"]), c64(a["source"]), c64(a["pi"]),
+                *(c64(x) for x in a["g"]), float(params["visc"]), float(params["dt"]), float(params["urf"]),
+                aval, b, abou, asave[case.nbc + 1:], rhoui, cbuf, xbuf)
+        if rc != 0:
+            raise ValueError("BLEND not covered by the restatement (a ConvScheme face scheme)")
+        return dict(aval=aval, b=b, abou=abou[:case.nbc], asave=asave, rhoui=rhoui)
+
+
+def _uvw_args(case, uf, var):
+    """newuvw's arrays for var, as Process/Processor.f90:314-340 passes them."""
+    grads = {1: ("ux", "uy", "uz", "vx", "wx", "vy", "wz"), 2: ("vy", "vx", "vz", "uy", "wy", "ux", "wz"),
+             3: ("wz", "wx", "wy", "uz", "vz", "ux", "vy")}[var]
+    order = {1: (0, 1, 2), 2: (1, 0, 2), 3: (2, 0, 1)}[var]
+    comp = "uvw"[var - 1]
+    S = (case.sx, case.sy, case.sz)
+    D = (case.dx, case.dy, case.dz)
+    return {"rho_n": case.rho_n, "rho_o": case.rho_o, "rho_oo": case.rho_oo,
+            "ui_n": uf.get(comp, getattr(case, comp)), "ui_o": uf[comp + "_o"], "ui_oo": uf[comp + "_oo"],
+            "source": uf["src_" + comp], "pi": getattr(case, "p" + "xyz"[var - 1]), "g": [uf[k] for k in grads],
+            "S": [S[q] for q in order], "D": [D[q] for q in order], "flux_u": uf["flux_u"]}
+
+
 class Reference(_Solvers):
     """The reference Fortran solvers, compiled from their own sources."""
     _prefix = ""

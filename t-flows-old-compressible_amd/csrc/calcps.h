@@ -30,6 +30,34 @@ struct CalcPSFields {
     const double *asave, *rho_n, *rho_o, *rho_oo, *u, *v, *w, *p, *px, *py, *pz;
 };
 
+// Momentum assembly (Process/NewUVW.f90:137-499): the flags and constants
+// of one call.  Codes as Process/IniPar.f90:22-32 (CN 2, FI 3, LIN 4, PAR 5,
+// YES 10, NO 11); per-material BLEND, BLEND_TEM and URFC for materials
+// 1..nmaterials at [m - 1].
+constexpr int kMaxMaterials = 8;
+struct UvwParams {
+    int var = 1;                        // 1 U, 2 V, 3 W: picks Si..Dk as Process/Processor.f90:314-340
+    int convec = 3, diffus = 3, cross = 3, inert = 4;
+    double visc = 0.0, dt = 1.0, urf = 1.0;
+    int blend[kMaxMaterials] = {}, blend_tem[kMaxMaterials] = {};
+    double urfc[kMaxMaterials] = {};
+};
+
+// The per-call momentum fields, device pointers; cell arrays (-NbC:NC).
+struct UvwFields {
+    const double *rho_n, *rho_o, *rho_oo;
+    const double *ui_n, *ui_o, *ui_oo, *source, *pi;
+    const double* g[7];                 // dUidi, dUidj, dUidk, dUjdi, dUkdi, dUjdj, dUkdk
+    const double* flux_u;               // Flux_u(NS)
+};
+
+// Geometry the momentum assembly adds to the mesh (device).
+struct UvwMesh {
+    const unsigned char* ukind = nullptr;  // what a face does in the diffusion loop
+    const double *scoef = nullptr, *ff = nullptr;
+    const int* material = nullptr;      // (-NbC:NC), 1-based
+};
+
 class CalcPS {
 public:
     // sidec: SideC(0:2,NS), sidaij: SidAij(1:2,NS), adia: Adia(NC),
@@ -52,6 +80,20 @@ public:
     double correct(double dt, const CalcPSFields& F, const double* pp, const double* aval, const double* abou,
                    double* u, double* v, double* w, double* flux, double* b, hipStream_t s);
 
+    // The momentum mesh data: Scoef(NS), fF(NS) (Process/Calc3.f90:23-70)
+    // and material(-NbC:NC) with values 1..nmaterials; host or device.
+    void uvw_setup(const double* scoef, const double* ff, const int* material, int nmaterials, bool device_ptrs,
+                   hipStream_t s);
+    // One velocity component's system (NewUVW.f90:137-499): Aval(NONZ),
+    // b(NC), Abou(-NbC:-1), Asave(-NbC:NC) (cells 1..NC, var = 1 only; may
+    // be null) and rhoUi(-NbC:NC), the solve's starting point.  Device.
+    void uvw_assemble(const UvwParams& P, const UvwFields& F, double* aval, double* b, double* abou, double* asave,
+                      double* rhoui, hipStream_t s);
+    // Ui%n(1:NC) = rhoUi(1:NC) / rho%n(1:NC) after the solve (:526).
+    void uvw_update(const double* rhoui, const double* rho_n, double* ui_n, hipStream_t s);
+    bool uvw_ready() const { return U_.ff != nullptr; }
+    int nmaterials() const { return nmaterials_; }
+
     int nc() const { return nc_; }
     int nbc() const { return nbc_; }
     int ns() const { return ns_; }
@@ -70,6 +112,13 @@ private:
     double* fa12_ = nullptr;
     double* part_ = nullptr;
     double* out2_ = nullptr;
+    // momentum assembly
+    UvwMesh U_;
+    std::vector<unsigned char> ukind_host_;
+    std::vector<int> c1_host_, c2_host_;
+    bool offdiag_shared_ = false;       // two faces write one off-diagonal
+    int nmaterials_ = 0;
+    unsigned pairs_[kMaxMaterials] = {};  // material pairs that meet at a face
 };
 
 }  // namespace tflows

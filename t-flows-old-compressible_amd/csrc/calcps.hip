@@ -42,7 +42,8 @@ namespace tflows {
 
 namespace {
 
-constexpr int kBcInflow = 1, kBcOutflow = 3, kBcConvect = 5, kBcBuffer = 11, kBcPressure = 12;  // allp_mod.f90
+constexpr int kBcInflow = 1, kBcWall = 2, kBcOutflow = 3, kBcConvect = 5, kBcWallfl = 6, kBcBuffer = 11,
+              kBcPressure = 12;  // allp_mod.f90:16-24
 
 // what a face does to its cells (fixed by the face list and TypeBC)
 enum : unsigned char {
@@ -52,6 +53,16 @@ enum : unsigned char {
     kInflow = 3,    // b(c1) -= F only (:120-133)
     kDiagOut = 4,   // OUTFLOW / CONVECT / PRESSURE: diagonal of c1, b(c1) -= F (:134-159)
     kPressure = 5,  // as kDiagOut, with the cell's own mass velocity (:150-152)
+};
+
+// what a face does in the diffusion loop of the momentum assembly
+// (NewUVW.f90:362-410)
+enum : unsigned char {
+    kUvwNone = 0,      // SYMMETRY, PRESSURE, ...: convection only
+    kUvwInterior = 1,  // the "m1" fill of both rows (:364-367)
+    kUvwWallLike = 2,  // WALL, CONVECT, INFLOW, WALLFL: diagonal of c1, b(c1) (:385-392)
+    kUvwOutflow = 3,   // OUTFLOW: A21 to the diagonal, A12 rhoUi(c2) to b (:396-399)
+    kUvwBuffer = 4,    // BUFFER: diagonal of c1, Abou(c2) (:403-406)
 };
 
 __device__ __forceinline__ bool adds_diag(unsigned char k) {
@@ -302,6 +313,196 @@ __global__ __launch_bounds__(kBlock) void k_cor_total(int parts, const double* _
     if (threadIdx.x == 0) out[0] = sm[0];
 }
 
+// ---------------------------------------------------------------------------
+// The momentum assembly of one velocity component (Process/NewUVW.f90:137-499).
+// ---------------------------------------------------------------------------
+constexpr int kFlagCN = 2, kFlagFI = 3, kInertLin = 4, kInertPar = 5, kBlendYes = 10,
+              kBlendNo = 11;   // IniPar.f90:22-32
+
+struct UvwGeom {        // Si, Sj, Sk, Di, Dj, Dk of the component (Processor.f90:314-340)
+    const double *si, *sj, *sk, *di, *dj, *dk;
+};
+
+// What the face loops read of one cell.
+struct UvwCell {
+    double rho, ru;     // rho%n and rhoUi = rho%n Ui%n (:138)
+    double g[7];        // the seven gradients
+    int mat;            // material - 1
+};
+
+__device__ __forceinline__ UvwCell uvw_cell(const UvwMesh& U, const UvwFields& F, int idx) {
+    UvwCell v;
+    v.rho = F.rho_n[idx];
+    v.ru = v.rho * F.ui_n[idx];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v.g[k] = F.g[k][idx];
+    v.mat = U.material[idx] - 1;
+    return v;
+}
+
+// The quantities of face s that the reference's two face loops compute
+// (:211-238, :280-360), in its operation order; a and b are the cells
+// SideC(1,s) and SideC(2,s).
+struct UvwFace {
+    double conv;        // Flux_u rhoUiS (:220-223)
+    double up;          // Flux_u rhoUi(upwind cell) (:228-236)
+    bool upwind;        // BLEND_TEM /= NO on either side
+    double fex, fim;    // the cross-diffusion parts (:312-330)
+    double a12, a21, d12, d21;
+};
+
+__device__ __forceinline__ UvwFace uvw_face(const CalcPSMesh& M, const UvwMesh& U, const UvwGeom& G,
+                                            const UvwParams& P, const UvwFields& F, int s, const UvwCell& a,
+                                            const UvwCell& b) {
+    UvwFace q;
+    const double Fu = F.flux_u[s];
+    const double fs = M.f[s];
+    const double rhoUiS = fs * a.ru + (1.0 - fs) * b.ru;                             // :211
+    q.conv = Fu * rhoUiS;
+    q.upwind = P.blend_tem[a.mat] != kBlendNo || P.blend_tem[b.mat] != kBlendNo;
+    q.up = Fu * (Fu < 0 ? b.ru : a.ru);
+    const double ff = U.ff[s], gg = 1.0 - U.ff[s];
+    double gS[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) gS[k] = ff * a.g[k] + gg * b.g[k];                   // :300-308
+    const double Si = G.si[s], Sj = G.sj[s], Sk = G.sk[s];
+    const double visc = P.visc;                                                      // VISeff (:294)
+    q.fex = visc * ((((4.0 / 3.0) * gS[0] * Si + (gS[1] + gS[3]) * Sj) + (gS[2] + gS[4]) * Sk) -
+                    (2.0 / 3.0) * (gS[5] + gS[6]) * Si);                               // :312-315
+    const double A0 = visc * U.scoef[s];                                             // :323
+    q.fim = A0 * ((gS[0] * G.di[s] + gS[1] * G.dj[s]) + gS[2] * G.dk[s]);            // :325-327
+    q.a12 = q.a21 = q.d12 = q.d21 = 0.0;
+    if (P.diffus == kFlagFI) {                                                       // :351-354
+        q.d12 = 0.0 + A0 / b.rho;
+        q.d21 = 0.0 + A0 / a.rho;
+    }
+    if (P.blend[a.mat] != kBlendNo || P.blend[b.mat] != kBlendNo) {                 // :356-359
+        q.a12 = 0.0 - (Fu < 0.0 ? Fu : 0.0);
+        q.a21 = 0.0 + (Fu > 0.0 ? Fu : 0.0);
+    }
+    return q;
+}
+
+__device__ __forceinline__ UvwCell pick(bool first, const UvwCell& x, const UvwCell& y) {
+    UvwCell v;
+    v.rho = first ? x.rho : y.rho;
+    v.ru = first ? x.ru : y.ru;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v.g[k] = first ? x.g[k] : y.g[k];
+    v.mat = first ? x.mat : y.mat;
+    return v;
+}
+
+// One thread per cell: the two face loops walked over the cell's faces in
+// ascending face order (the order in which the reference's loops reach the
+// cell), then the cell loops (:252-257, :434-499).  The cell's own values
+// are read once; per face only the other cell's and the face's.  Each
+// off-diagonal of row c is written by the thread of cell c (SHARED: several
+// faces write one position, so it accumulates on a zeroed Aval); the
+// diagonal and b stay in registers.
+template <bool SHARED>
+__global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, UvwGeom G, UvwParams P,
+                                                       UvwFields F, double* __restrict__ aval,
+                                                       double* __restrict__ b, double* __restrict__ abou,
+                                                       double* __restrict__ asave, double* __restrict__ rhoui) {
+    const int nb = M.nbc;
+    const int stride = gridDim.x * kBlock;
+    const int t0 = blockIdx.x * kBlock + threadIdx.x;
+    for (int q = t0; q <= nb; q += stride) rhoui[q] = F.rho_n[q] * F.ui_n[q];       // boundary cells and 0
+    const bool diff = P.diffus == kFlagCN || P.diffus == kFlagFI;
+    for (int i = t0; i < M.nc; i += stride) {
+        const int c = i + 1, ic = nb + c;
+        const UvwCell own = uvw_cell(U, F, ic);
+        rhoui[ic] = own.ru;
+        const int e0 = M.cptr[i], e1 = M.cptr[i + 1];
+        double Cs = 0.0, Xc = 0.0, Xd = 0.0, diag = 0.0;
+        for (int e = e0; e < e1; ++e) {
+            const int code = M.cface[e];
+            const int s = code >> 1;
+            const bool second = code & 1;                                            // the cell is c2
+            const int other = second ? M.c1[s] : M.c2[s];
+            const UvwCell oth = uvw_cell(U, F, nb + other);
+            const UvwFace f = uvw_face(M, U, G, P, F, s, pick(!second, own, oth), pick(!second, oth, own));
+            if (!second) {
+                Cs = Cs - f.conv;
+                if (f.upwind) Xc = Xc - f.up;
+                Xd = Xd + f.fex - f.fim;                                             // :338
+            } else {
+                Cs = Cs + f.conv;
+                if (f.upwind) Xc = Xc + f.up;
+                Xd = Xd - f.fex + f.fim;                                             // :340
+            }
+            if (!diff) continue;
+            const unsigned char k = U.ukind[s];
+            if (second) {                                                            // :366-367
+                diag = diag + f.a12 + f.d12;
+                const int pos = M.sid2[s];
+                aval[pos] = (SHARED ? aval[pos] : 0.0) - f.a21 - f.d21;
+            } else if (k == kUvwInterior) {                                          // :364-365, :367
+                diag = diag + f.a21 + f.d21;
+                const int pos = M.sid1[s];
+                aval[pos] = (SHARED ? aval[pos] : 0.0) - f.a12 - f.d12;
+            } else if (k == kUvwWallLike) {
+                diag = diag + f.a21 + f.d21;
+            } else if (k == kUvwOutflow) {
+                diag = diag + f.a21;
+            } else if (k == kUvwBuffer) {
+                diag = diag + f.a21 + f.d21;
+                abou[nb + other] = -f.a12 - f.d12;                                   // Abou(-NbC:-1)
+            }
+        }
+        double bb = 0.0;
+        if (P.convec == kFlagFI) bb = bb + P.urfc[own.mat] * (Cs - Xc);              // :252-257
+        if (diff) {
+            // b(c1) of WALL-like and OUTFLOW faces (A12 and D12 of :356-359),
+            // after the convection term
+            for (int e = e0; e < e1; ++e) {
+                const int code = M.cface[e];
+                if (code & 1) continue;
+                const int s = code >> 1;
+                const unsigned char k = U.ukind[s];
+                if (k != kUvwWallLike && k != kUvwOutflow) continue;
+                const int i2 = nb + M.c2[s];
+                const double rho2 = F.rho_n[i2], ru2 = rho2 * F.ui_n[i2];
+                const double Fu = F.flux_u[s];
+                const int m2 = U.material[i2] - 1;
+                const double a12 = (P.blend[own.mat] != kBlendNo || P.blend[m2] != kBlendNo)
+                                       ? 0.0 - (Fu < 0.0 ? Fu : 0.0) : 0.0;
+                if (k == kUvwWallLike) {
+                    const double d12 = P.diffus == kFlagFI ? 0.0 + (P.visc * U.scoef[s]) / rho2 : 0.0;
+                    bb = bb + (a12 + d12) * ru2;                                     // :392
+                } else {
+                    bb = bb + a12 * ru2;                                             // :399
+                }
+            }
+        }
+        if (P.cross == kFlagFI) bb = bb + Xd;                                        // :434-438
+        const double V = M.volume[ic];
+        if (P.inert == kInertLin) {                                                  // :447-450
+            diag = diag + V / P.dt;
+            bb = bb + V / P.dt * F.rho_o[ic] * F.ui_o[ic] + F.source[ic] * V;
+        }
+        if (P.inert == kInertPar) {                                                  // :453-456
+            diag = diag + 1.5 * V / P.dt;
+            bb = bb + 2.0 * V / P.dt * F.rho_o[ic] * F.ui_o[ic] - 0.5 * V / P.dt * F.rho_oo[ic] * F.ui_oo[ic] +
+                 F.source[ic] * V;
+        }
+        bb = bb - F.pi[ic] * V;                                                      // :484
+        if (P.var == 1 && asave) asave[ic] = diag;                                   // :497
+        bb = bb + diag * (1.0 - P.urf) * own.ru / P.urf;                             // :498
+        aval[M.adia[i]] = diag / P.urf;                                              // :499
+        b[i] = bb;
+    }
+}
+
+// Ui%n(1:NC) = rhoUi(1:NC) / rho%n(1:NC) (:526)
+__global__ __launch_bounds__(kBlock) void k_uvw_update(int nc, int nb, const double* __restrict__ rhoui,
+                                                        const double* __restrict__ rho_n,
+                                                        double* __restrict__ ui_n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < nc) ui_n[nb + i + 1] = rhoui[nb + i + 1] / rho_n[nb + i + 1];
+}
+
 template <typename T>
 T* upload(std::vector<void*>& owned, const T* src, size_t count, bool device_src, hipStream_t s) {
     T* d = nullptr;
@@ -332,12 +533,13 @@ CalcPS::CalcPS(int nc, int nbc, int ns, int nonz, const int* sidec, const int* s
     fetch(h_adia.data(), adia, sizeof(int) * h_adia.size());
     fetch(h_type.data(), typebc, sizeof(int) * h_type.size());
     std::vector<int> c1(ns), c2(ns), s1(ns, 0), s2(ns, 0), ad0(nc);
-    std::vector<unsigned char> kind(ns, kNone);
+    std::vector<unsigned char> kind(ns, kNone), ukind(ns, kUvwNone);
     std::vector<char> covered(nonz, 0);
     for (int i = 0; i < nc; ++i) {
         if (h_adia[i] < 1 || h_adia[i] > nonz - 1) throw std::invalid_argument("Adia out of range");
         ad0[i] = h_adia[i] - 1;
-        covered[ad0[i]] = 1;
+        if (covered[ad0[i]]) throw std::invalid_argument("two cells share a diagonal position");
+        covered[ad0[i]] = 2;
     }
     std::vector<int> count(nc + 1, 0);
     for (int q = 0; q < ns; ++q) {
@@ -347,10 +549,13 @@ CalcPS::CalcPS(int nc, int nbc, int ns, int nonz, const int* sidec, const int* s
         if (c2[q] > nc || c2[q] < -nbc || c2[q] == 0) throw std::invalid_argument("SideC(2,s) out of range");
         if (c2[q] > 0) {
             kind[q] = kInterior;
+            ukind[q] = kUvwInterior;
             for (int r = 0; r < 2; ++r) {
                 const int pos = h_sid[2 * (size_t)q + r];
                 if (pos < 1 || pos > nonz - 1) throw std::invalid_argument("SidAij out of range");
+                if (covered[pos - 1] == 2) throw std::invalid_argument("SidAij names a diagonal position");
                 (r ? s2 : s1)[q] = pos - 1;
+                if (covered[pos - 1]) offdiag_shared_ = true;
                 covered[pos - 1] = 1;
             }
         } else {
@@ -360,6 +565,10 @@ CalcPS::CalcPS(int nc, int nbc, int ns, int nonz, const int* sidec, const int* s
                     : (t == kBcOutflow || t == kBcConvect) ? kDiagOut
                     : t == kBcPressure ? kPressure
                     : kNone;
+            ukind[q] = (t == kBcWall || t == kBcConvect || t == kBcInflow || t == kBcWallfl) ? kUvwWallLike
+                     : t == kBcOutflow ? kUvwOutflow
+                     : t == kBcBuffer ? kUvwBuffer
+                     : kUvwNone;
         }
         count[c1[q]]++;
         if (kind[q] == kInterior) count[c2[q]]++;
@@ -387,6 +596,10 @@ CalcPS::CalcPS(int nc, int nbc, int ns, int nonz, const int* sidec, const int* s
     M_.sid1 = upload(owned_, s1.data(), s1.size(), false, s);
     M_.sid2 = upload(owned_, s2.data(), s2.size(), false, s);
     M_.kind = upload(owned_, kind.data(), kind.size(), false, s);
+    U_.ukind = upload(owned_, ukind.data(), ukind.size(), false, s);
+    ukind_host_ = ukind;
+    c1_host_ = c1;
+    c2_host_ = c2;
     M_.adia = upload(owned_, ad0.data(), ad0.size(), false, s);
     M_.cptr = upload(owned_, cptr.data(), cptr.size(), false, s);
     M_.cface = upload(owned_, cface.data(), cface.size(), false, s);
@@ -462,6 +675,97 @@ void CalcPS::assemble(int nmat, double dt, const CalcPSFields& F, double* aval, 
     TF_CHECK(hipStreamSynchronize(s));
     if (errmax) *errmax = h[0];
     if (mean) *mean = h[1];
+}
+
+void CalcPS::uvw_setup(const double* scoef, const double* ff, const int* material, int nmaterials, bool device_ptrs,
+                       hipStream_t s) {
+    if (nmaterials < 1 || nmaterials > kMaxMaterials)
+        throw std::invalid_argument("the momentum assembly takes 1 to 8 materials");
+    if (ns_ > 0 && (!scoef || !ff)) throw std::invalid_argument("null Scoef or fF");
+    if (!material) throw std::invalid_argument("null material");
+    const size_t ncell = (size_t)nbc_ + nc_ + 1;
+    std::vector<int> mat(ncell);
+    if (device_ptrs) TF_CHECK(hipMemcpy(mat.data(), material, sizeof(int) * ncell, hipMemcpyDeviceToHost));
+    else std::memcpy(mat.data(), material, sizeof(int) * ncell);
+    // cells that no face reaches (cell 0) are never read; the rest must
+    // name a material
+    std::vector<char> used(ncell, 0);
+    for (int q = 0; q < ns_; ++q) used[nbc_ + c1_host_[q]] = used[nbc_ + c2_host_[q]] = 1;
+    for (int c = 1; c <= nc_; ++c) used[nbc_ + c] = 1;
+    for (size_t c = 0; c < ncell; ++c) {
+        if (!used[c]) mat[c] = 1;
+        else if (mat[c] < 1 || mat[c] > nmaterials) throw std::invalid_argument("material out of range");
+    }
+    unsigned pairs[kMaxMaterials] = {};
+    for (int q = 0; q < ns_; ++q) pairs[mat[nbc_ + c1_host_[q]] - 1] |= 1u << (mat[nbc_ + c2_host_[q]] - 1);
+    if (!U_.ff) {
+        double* d = nullptr;
+        TF_CHECK(hipMalloc(&d, sizeof(double) * 2 * (ns_ ? ns_ : 1)));
+        owned_.push_back(d);
+        U_.scoef = d;
+        U_.ff = d + ns_;
+        int* m = nullptr;
+        TF_CHECK(hipMalloc(&m, sizeof(int) * ncell));
+        owned_.push_back(m);
+        U_.material = m;
+    }
+    const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (ns_ > 0) {
+        TF_CHECK(hipMemcpyAsync((double*)U_.scoef, scoef, sizeof(double) * ns_, kind, s));
+        TF_CHECK(hipMemcpyAsync((double*)U_.ff, ff, sizeof(double) * ns_, kind, s));
+    }
+    TF_CHECK(hipMemcpyAsync((int*)U_.material, mat.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, s));
+    TF_CHECK(hipStreamSynchronize(s));
+    nmaterials_ = nmaterials;
+    std::memcpy(pairs_, pairs, sizeof(pairs));
+}
+
+void CalcPS::uvw_assemble(const UvwParams& P, const UvwFields& F, double* aval, double* b, double* abou,
+                          double* asave, double* rhoui, hipStream_t s) {
+    if (!uvw_ready()) throw std::invalid_argument("the momentum geometry is not set (uvw_setup)");
+    if (P.var < 1 || P.var > 3) throw std::invalid_argument("var must be 1, 2 or 3");
+    if (!(P.dt != 0.0) && (P.inert == kInertLin || P.inert == kInertPar))
+        throw std::invalid_argument("dt must be non-zero");
+    if (!(P.urf != 0.0)) throw std::invalid_argument("URF must be non-zero");
+    // ConvScheme (Process/ConvScheme.f90) leaves rhoUiS as it is only for
+    // BLEND = YES; a face whose materials give max(BLEND) another value
+    // would need a TVD face value (or, for YES next to NO, an undefined one)
+    for (int a = 0; a < nmaterials_; ++a)
+        for (int c = 0; c < nmaterials_; ++c)
+            if (pairs_[a] >> c & 1u) {
+                const int ba = P.blend[a], bc = P.blend[c];
+                if ((ba != kBlendNo || bc != kBlendNo) && std::max(ba, bc) != kBlendYes)
+                    throw std::invalid_argument("BLEND other than YES/NO (a ConvScheme face scheme) is not "
+                                                "covered by the momentum assembly");
+            }
+    const UvwGeom G = P.var == 1 ? UvwGeom{M_.sx, M_.sy, M_.sz, M_.dx, M_.dy, M_.dz}
+                    : P.var == 2 ? UvwGeom{M_.sy, M_.sx, M_.sz, M_.dy, M_.dx, M_.dz}
+                                 : UvwGeom{M_.sz, M_.sx, M_.sy, M_.dz, M_.dx, M_.dy};
+    const bool diff = P.diffus == kFlagCN || P.diffus == kFlagFI;
+    // Aval = 0 (:148): the whole array where the cell kernel accumulates
+    // into off-diagonals or writes none, else the positions it never writes
+    if (offdiag_shared_ || !diff) {
+        hipLaunchKernelGGL(k_calcps_zero_all, dim3((nonz_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nonz_, aval);
+    } else if (n_zero_ > 0) {
+        hipLaunchKernelGGL(k_calcps_zero, dim3((n_zero_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n_zero_,
+                           (const int*)zero_idx_, aval);
+    }
+    if (nbc_ > 0)                                                      // :146
+        hipLaunchKernelGGL(k_calcps_zero_all, dim3((nbc_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nbc_, abou);
+    const int grid = (std::max(nc_, nbc_ + 1) + kBlock - 1) / kBlock;
+    if (offdiag_shared_)
+        hipLaunchKernelGGL(k_uvw_cells<true>, dim3(grid), dim3(kBlock), 0, s, M_, U_, G, P, F, aval, b, abou, asave,
+                           rhoui);
+    else
+        hipLaunchKernelGGL(k_uvw_cells<false>, dim3(grid), dim3(kBlock), 0, s, M_, U_, G, P, F, aval, b, abou, asave,
+                           rhoui);
+    TF_CHECK(hipGetLastError());
+}
+
+void CalcPS::uvw_update(const double* rhoui, const double* rho_n, double* ui_n, hipStream_t s) {
+    hipLaunchKernelGGL(k_uvw_update, dim3((nc_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nc_, nbc_, rhoui, rho_n,
+                       ui_n);
+    TF_CHECK(hipGetLastError());
 }
 
 }  // namespace tflows

@@ -490,3 +490,100 @@ int tflows_calcps_correct(tflows_calcps h, double dt, const double* pp, const do
         if (errmax) *errmax = em;
     });
 }
+
+// ---------------------------------------------------------------------------
+// momentum assembly
+// ---------------------------------------------------------------------------
+int tflows_calcps_uvw_setup(tflows_calcps h, const double* scoef, const double* fF, const int32_t* material,
+                            int32_t nmaterials, int32_t mem, void* stream) {
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("null handle");
+        h->cps->uvw_setup(scoef, fF, material, nmaterials, mem == TFLOWS_MEM_DEVICE, as_stream(stream));
+    });
+}
+
+int tflows_calcps_uvw_assemble(tflows_calcps h, const tflows_uvw_params* p, const double* rho_n,
+                               const double* rho_o, const double* rho_oo, const double* ui_n, const double* ui_o,
+                               const double* ui_oo, const double* source, const double* pi, const double* dUidi,
+                               const double* dUidj, const double* dUidk, const double* dUjdi, const double* dUkdi,
+                               const double* dUjdj, const double* dUkdk, const double* flux_u, double* aval,
+                               double* b, double* abou, double* asave, double* rhoui, int32_t mem, void* stream) {
+    return guarded([&] {
+        if (!h || !p) throw std::invalid_argument("null handle or parameters");
+        const double* in[15] = {rho_n, rho_o, rho_oo, ui_n,  ui_o,  ui_oo, source, pi,
+                                dUidi, dUidj, dUidk,  dUjdi, dUkdi, dUjdj, dUkdk};
+        for (const double* q : in)
+            if (!q) throw std::invalid_argument("null field");
+        tflows::CalcPS& C = *h->cps;
+        if (!aval || !b || !rhoui || (C.ns() > 0 && !flux_u) || (C.nbc() > 0 && !abou))
+            throw std::invalid_argument("null array");
+        tflows::UvwParams P;
+        P.var = p->var;
+        P.convec = p->convec;
+        P.diffus = p->diffus;
+        P.cross = p->cross;
+        P.inert = p->inert;
+        P.visc = p->visc;
+        P.dt = p->dt;
+        P.urf = p->urf;
+        for (int m = 0; m < tflows::kMaxMaterials; ++m) {
+            P.blend[m] = p->blend[m];
+            P.blend_tem[m] = p->blend_tem[m];
+            P.urfc[m] = p->urfc[m];
+        }
+        const hipStream_t st = as_stream(stream);
+        const size_t ncell = (size_t)C.nbc() + C.nc() + 1;
+        if (mem == TFLOWS_MEM_DEVICE) {
+            const tflows::UvwFields F{rho_n, rho_o, rho_oo, ui_n, ui_o, ui_oo, source, pi,
+                                      {dUidi, dUidj, dUidk, dUjdi, dUkdi, dUjdj, dUkdk}, flux_u};
+            C.uvw_assemble(P, F, aval, b, abou, asave, rhoui, st);
+            return;
+        }
+        const size_t total = 17 * ncell + (size_t)C.ns() + C.nonz() + C.nc() + C.nbc();
+        double* d = C.staging(total);
+        const double* dv[15];
+        for (int q = 0; q < 15; ++q) {
+            TF_CHECK(hipMemcpyAsync(d + q * ncell, in[q], sizeof(double) * ncell, hipMemcpyHostToDevice, st));
+            dv[q] = d + q * ncell;
+        }
+        double* dasave = d + 15 * ncell;
+        double* drhoui = dasave + ncell;
+        double* dflux = drhoui + ncell;
+        double* daval = dflux + C.ns();
+        double* db = daval + C.nonz();
+        double* dabou = db + C.nc();
+        if (C.ns() > 0) TF_CHECK(hipMemcpyAsync(dflux, flux_u, sizeof(double) * C.ns(), hipMemcpyHostToDevice, st));
+        // Asave keeps the caller's values outside 1..NC
+        if (asave) TF_CHECK(hipMemcpyAsync(dasave, asave, sizeof(double) * ncell, hipMemcpyHostToDevice, st));
+        const tflows::UvwFields F{dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7],
+                                  {dv[8], dv[9], dv[10], dv[11], dv[12], dv[13], dv[14]}, dflux};
+        C.uvw_assemble(P, F, daval, db, dabou, asave ? dasave : nullptr, drhoui, st);
+        TF_CHECK(hipMemcpyAsync(aval, daval, sizeof(double) * C.nonz(), hipMemcpyDeviceToHost, st));
+        TF_CHECK(hipMemcpyAsync(b, db, sizeof(double) * C.nc(), hipMemcpyDeviceToHost, st));
+        TF_CHECK(hipMemcpyAsync(rhoui, drhoui, sizeof(double) * ncell, hipMemcpyDeviceToHost, st));
+        if (C.nbc() > 0) TF_CHECK(hipMemcpyAsync(abou, dabou, sizeof(double) * C.nbc(), hipMemcpyDeviceToHost, st));
+        if (asave) TF_CHECK(hipMemcpyAsync(asave, dasave, sizeof(double) * ncell, hipMemcpyDeviceToHost, st));
+        TF_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+int tflows_calcps_uvw_update(tflows_calcps h, const double* rhoui, const double* rho_n, double* ui_n, int32_t mem,
+                             void* stream) {
+    return guarded([&] {
+        if (!h || !rhoui || !rho_n || !ui_n) throw std::invalid_argument("null argument");
+        tflows::CalcPS& C = *h->cps;
+        const hipStream_t st = as_stream(stream);
+        if (mem == TFLOWS_MEM_DEVICE) {
+            C.uvw_update(rhoui, rho_n, ui_n, st);
+            return;
+        }
+        const size_t ncell = (size_t)C.nbc() + C.nc() + 1;
+        double* d = C.staging(3 * ncell);
+        TF_CHECK(hipMemcpyAsync(d, rhoui, sizeof(double) * ncell, hipMemcpyHostToDevice, st));
+        TF_CHECK(hipMemcpyAsync(d + ncell, rho_n, sizeof(double) * ncell, hipMemcpyHostToDevice, st));
+        TF_CHECK(hipMemcpyAsync(d + 2 * ncell, ui_n, sizeof(double) * ncell, hipMemcpyHostToDevice, st));
+        C.uvw_update(d, d + ncell, d + 2 * ncell, st);
+        TF_CHECK(hipMemcpyAsync(ui_n, d + 2 * ncell, sizeof(double) * ncell, hipMemcpyDeviceToHost, st));
+        TF_CHECK(hipStreamSynchronize(st));
+    });
+}

@@ -151,6 +151,12 @@ def _bind_extensions(lib):
         lib.tflows_calcps_assemble.restype = C.c_int
         lib.tflows_calcps_assemble.argtypes = ([vp, C.c_int32, C.c_double] + [vp] * 11 + [vp] * 4 +
                                                [C.POINTER(C.c_double)] * 2 + [C.c_int32, vp])
+        lib.tflows_calcps_uvw_setup.restype = C.c_int
+        lib.tflows_calcps_uvw_setup.argtypes = [vp, vp, vp, vp, C.c_int32, C.c_int32, vp]
+        lib.tflows_calcps_uvw_assemble.restype = C.c_int
+        lib.tflows_calcps_uvw_assemble.argtypes = [vp, C.POINTER(UvwParams)] + [vp] * 16 + [vp] * 5 + [C.c_int32, vp]
+        lib.tflows_calcps_uvw_update.restype = C.c_int
+        lib.tflows_calcps_uvw_update.argtypes = [vp, vp, vp, vp, C.c_int32, vp]
         lib.tflows_calcps_correct.restype = C.c_int
         lib.tflows_calcps_correct.argtypes = ([vp, C.c_double] + [vp] * 10 + [vp] * 5 + [C.POINTER(C.c_double)] +
                                               [C.c_int32, vp])
@@ -473,6 +479,28 @@ class DeviceBatch:
 CALCPS_FIELDS = ("asave", "rho_n", "rho_o", "rho_oo", "u", "v", "w", "p", "px", "py", "pz")
 
 
+class UvwParams(C.Structure):
+    """tflows_uvw_params: the flags and constants of one newuvw call."""
+    _fields_ = [("var", C.c_int32), ("convec", C.c_int32), ("diffus", C.c_int32), ("cross", C.c_int32),
+                ("inert", C.c_int32), ("visc", C.c_double), ("dt", C.c_double), ("urf", C.c_double),
+                ("blend", C.c_int32 * 8), ("blend_tem", C.c_int32 * 8), ("urfc", C.c_double * 8)]
+
+    @classmethod
+    def from_dict(cls, p):
+        q = cls()
+        for k in ("var", "convec", "diffus", "cross", "inert"):
+            setattr(q, k, int(p[k]))
+        for k in ("visc", "dt", "urf"):
+            setattr(q, k, float(p[k]))
+        if len(p["blend"]) > 8:
+            raise ValueError("at most 8 materials")
+        for m in range(len(p["blend"])):
+            q.blend[m] = int(p["blend"][m])
+            q.blend_tem[m] = int(p["blend_tem"][m])
+            q.urfc[m] = float(p["urfc"][m])
+        return q
+
+
 class DeviceCalcPS:
     """The pressure-correction assembly of one mesh (Process/CalcPS.f90:51-202)
     on the GPU.  ``case`` is a meshgen.CalcPSCase (reference layout); the
@@ -543,6 +571,68 @@ class DeviceCalcPS:
             *(_ptr(fields[k]) for k in ("asave", "rho_n", "rho_o", "rho_oo", "px", "py", "pz")),
             _ptr(u), _ptr(v), _ptr(w), _ptr(flux), _ptr(b), C.byref(em), 1, _stream(stream)))
         return b, em.value
+
+    # -- momentum assembly (Process/NewUVW.f90:137-499) --------------------
+    def uvw_setup(self, uf):
+        """Scoef, fF and the cell materials of meshgen.uvw_fields, once."""
+        import torch
+        t64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()   # noqa: E731
+        keep = [t64(uf["scoef"]), t64(uf["ff"]),
+                torch.from_numpy(np.ascontiguousarray(uf["material"], np.int32)).cuda()]
+        torch.cuda.synchronize()
+        _check(self.lib.tflows_calcps_uvw_setup(self.handle, _ptr(keep[0]), _ptr(keep[1]), _ptr(keep[2]),
+                                                int(uf["nmaterials"]), 1, None))
+
+    @staticmethod
+    def uvw_inputs(args):
+        """cuda float64 tensors of meshgen.uvw_args(...) in the order of
+        tflows_calcps_uvw_assemble: 15 cell arrays, then Flux_u."""
+        import torch
+        t64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()   # noqa: E731
+        cells = [args[k] for k in ("rho_n", "rho_o", "rho_oo", "ui_n", "ui_o", "ui_oo", "source", "pi")]
+        return [t64(a) for a in cells + list(args["g"])] + [t64(args["flux_u"])]
+
+    def assemble_uvw(self, params, inputs, out=None, stream=None):
+        """One component's system from cuda tensors ``inputs`` (uvw_inputs);
+        ``params`` a meshgen.uvw_params dict.  Returns dict of cuda tensors
+        aval, b, abou, asave ((-NbC:NC), cells 1..NC written for var = 1),
+        rhoui ((-NbC:NC))."""
+        import torch
+        c = self.case
+        if out is None:
+            dev = torch.device("cuda")
+            ncell = c.nbc + c.nc + 1
+            out = dict(aval=torch.empty(c.nonz, dtype=torch.float64, device=dev),
+                       b=torch.empty(c.nc, dtype=torch.float64, device=dev),
+                       abou=torch.zeros(max(c.nbc, 1), dtype=torch.float64, device=dev),
+                       asave=torch.zeros(ncell, dtype=torch.float64, device=dev),
+                       rhoui=torch.empty(ncell, dtype=torch.float64, device=dev))
+        P = UvwParams.from_dict(params)
+        _check(self.lib.tflows_calcps_uvw_assemble(
+            self.handle, C.byref(P), *(_ptr(t) for t in inputs), _ptr(out["aval"]), _ptr(out["b"]),
+            _ptr(out["abou"]), _ptr(out["asave"]), _ptr(out["rhoui"]), 1, _stream(stream)))
+        return out
+
+    def assemble_uvw_host(self, params, args):
+        """The same from host arrays (meshgen.uvw_args), copied in and out."""
+        c = self.case
+        ncell = c.nbc + c.nc + 1
+        cells = [np.ascontiguousarray(args[k], np.float64)
+                 for k in ("rho_n", "rho_o", "rho_oo", "ui_n", "ui_o", "ui_oo", "source", "pi")]
+        cells += [np.ascontiguousarray(a, np.float64) for a in args["g"]]
+        fl = np.ascontiguousarray(args["flux_u"], np.float64)
+        aval, b = np.empty(c.nonz), np.empty(c.nc)
+        abou, asave, rhoui = np.zeros(max(c.nbc, 1)), np.zeros(ncell), np.empty(ncell)
+        P = UvwParams.from_dict(params)
+        _check(self.lib.tflows_calcps_uvw_assemble(
+            self.handle, C.byref(P), *(a.ctypes.data for a in cells), fl.ctypes.data, aval.ctypes.data,
+            b.ctypes.data, abou.ctypes.data, asave.ctypes.data, rhoui.ctypes.data, 0, None))
+        return dict(aval=aval, b=b, abou=abou[:c.nbc], asave=asave, rhoui=rhoui)
+
+    def uvw_update(self, rhoui, rho_n, ui_n, stream=None):
+        """Ui%n(1:NC) = rhoUi / rho%n on cuda tensors (NewUVW.f90:526)."""
+        _check(self.lib.tflows_calcps_uvw_update(self.handle, _ptr(rhoui), _ptr(rho_n), _ptr(ui_n), 1,
+                                                 _stream(stream)))
 
     def close(self):
         if getattr(self, "handle", None) is not None:

@@ -826,6 +826,7 @@ class CalcPSCase:
     px: np.ndarray
     py: np.ndarray
     pz: np.ndarray
+    xc: np.ndarray | None = None      # (NbC+NC+1, 3) positions of the cells (face centres for boundary cells)
 
     def matrix(self, aval: np.ndarray) -> TFlowSMatrix:
         """The assembled system matrix with values ``aval`` (NONZ)."""
@@ -976,4 +977,100 @@ def calcps_case(shape, seed: int = 7, ratio: float = 1.05, perturb: float = 0.15
     return CalcPSCase(tuple(shape), nc, nbc, ns, topo.nonz, nmat, dt, sidec, sidaij, topo.acol, topo.arow,
                       topo.adia, typebc, np.ascontiguousarray(S[:, 0]), np.ascontiguousarray(S[:, 1]),
                       np.ascontiguousarray(S[:, 2]), np.ascontiguousarray(d[:, 0]), np.ascontiguousarray(d[:, 1]),
-                      np.ascontiguousarray(d[:, 2]), f, volume, asave, rho_n, rho_o, rho_oo, u, v, w, p, px, py, pz)
+                      np.ascontiguousarray(d[:, 2]), f, volume, asave, rho_n, rho_o, rho_oo, u, v, w, p, px, py, pz,
+                      allc)
+
+
+# --------------------------------------------------------------------------
+# Momentum assembly inputs (Process/NewUVW.f90)
+# --------------------------------------------------------------------------
+FLAGS = {"AB": 1, "CN": 2, "FI": 3, "LIN": 4, "PAR": 5, "YES": 10, "NO": 11}   # Process/IniPar.f90:22-32
+
+UVW_GRADS = ("ux", "uy", "uz", "vx", "vy", "vz", "wx", "wy", "wz")
+
+# newuvw's arguments for var = 1, 2, 3 as Process/Processor.f90:314-340
+# passes them: the component, its seven gradients (dUidi, dUidj, dUidk,
+# dUjdi, dUkdi, dUjdj, dUkdk), the order of the S and D components, Pi
+UVW_ARGS = {
+    1: ("u", ("ux", "uy", "uz", "vx", "wx", "vy", "wz"), (0, 1, 2), "px"),
+    2: ("v", ("vy", "vx", "vz", "uy", "wy", "ux", "wz"), (1, 0, 2), "py"),
+    3: ("w", ("wz", "wx", "wy", "uz", "vz", "ux", "vy"), (2, 0, 1), "pz"),
+}
+
+
+def uvw_fields(case: CalcPSCase, seed: int = 5, nmaterials: int = 1, linear: np.ndarray | None = None,
+               zero_flux_types=("wall", "wallfl", "symmetry")) -> dict:
+    """What the momentum assembly reads beyond a CalcPSCase: Scoef and fF
+    (as Process/Calc3.f90:23-70 forms them from the face geometry), the
+    material of every cell (1 .. nmaterials, slabs in x; a boundary cell
+    takes its face's cell's), a face flux Flux_u of the case's velocity
+    (zero on the faces of ``zero_flux_types``), the nine velocity gradients
+    ux .. wz, the old levels u_o, u_oo, .. and sources src_u, .. (cell
+    arrays (-NbC:NC)).  With ``linear`` (3x3 G, rows u, v, w) the velocity
+    n-level is the linear field G x (keys u, v, w) and the gradients are the
+    constants G[i, j]; otherwise they are smooth seeded fields."""
+    rng = np.random.default_rng(seed)
+    nbc, nc, ns = case.nbc, case.nc, case.ns
+    ntot = nbc + nc + 1
+    sx, sy, sz, dx, dy, dz = case.sx, case.sy, case.sz, case.dx, case.dy, case.dz
+    scoef = ((sx * sx + sy * sy) + sz * sz) / ((dx * sx + dy * sy) + dz * sz)
+    c1 = case.sidec[1].astype(np.int64)
+    c2 = case.sidec[2].astype(np.int64)
+    bnd = c2 < 0
+    ff = case.f.copy()
+    tb = case.typebc[nbc + c2]
+    ff[bnd & (tb != BC_CODES["buffer"])] = 1.0
+    xc = case.xc
+    material = np.ones(ntot, np.int32)
+    if nmaterials > 1:
+        edges = np.linspace(xc[nbc + 1:, 0].min(), xc[nbc + 1:, 0].max() + 1e-12, nmaterials + 1)
+        material[nbc + 1:] = np.clip(np.searchsorted(edges, xc[nbc + 1:, 0], side="right"), 1, nmaterials)
+        material[nbc + c2[bnd]] = material[nbc + c1[bnd]]
+    out = {"scoef": scoef, "ff": ff, "material": material, "nmaterials": nmaterials}
+    if linear is not None:
+        G = np.asarray(linear, float)
+        vel = {k: xc @ G[i] for i, k in enumerate("uvw")}
+        for i, a in enumerate("uvw"):
+            for j, b in enumerate("xyz"):
+                out[a + b] = np.full(ntot, G[i, j])
+        out.update(vel)
+    else:
+        vel = {"u": case.u, "v": case.v, "w": case.w}
+        for k in UVW_GRADS:
+            out[k] = 0.3 * rng.standard_normal(ntot) + rng.uniform(-1, 1)
+    f = case.f
+    ue = [f * vel[k][nbc + c1] + (1.0 - f) * vel[k][nbc + c2] for k in "uvw"]
+    flux = (ue[0] * sx + ue[1] * sy) + ue[2] * sz
+    for t in zero_flux_types:
+        flux[bnd & (tb == BC_CODES[t])] = 0.0
+    out["flux_u"] = flux
+    for k in "uvw":
+        out[k + "_o"] = vel[k] * (1.0 + 0.02 * rng.standard_normal(ntot))
+        out[k + "_oo"] = vel[k] * (1.0 + 0.02 * rng.standard_normal(ntot))
+        out["src_" + k] = 0.1 * rng.standard_normal(ntot)
+    return out
+
+
+def uvw_params(nmaterials: int = 1, **kw) -> dict:
+    """Flags and constants of one newuvw call (defaults: fully implicit
+    convection, diffusion and cross diffusion, two-level inertia, central
+    convection without the upwind part, URF 0.8)."""
+    p = {"var": 1, "convec": FLAGS["FI"], "diffus": FLAGS["FI"], "cross": FLAGS["FI"], "inert": FLAGS["LIN"],
+         "visc": 1e-2, "dt": 1e-2, "urf": 0.8, "blend": [FLAGS["NO"]] * nmaterials,
+         "blend_tem": [FLAGS["NO"]] * nmaterials, "urfc": [1.0] * nmaterials}
+    p.update(kw)
+    return p
+
+
+def uvw_args(case: CalcPSCase, uf: dict, var: int) -> dict:
+    """The cell and face arrays of newuvw(var, ...) in its argument order:
+    rho_n/o/oo, ui_n/o/oo, source, pi, g (the seven gradients), S and D
+    (the three components each, in the order Process/Processor.f90:314-340
+    passes them), flux_u."""
+    comp, grads, order, pname = UVW_ARGS[var]
+    S = (case.sx, case.sy, case.sz)
+    D = (case.dx, case.dy, case.dz)
+    return {"rho_n": case.rho_n, "rho_o": case.rho_o, "rho_oo": case.rho_oo,
+            "ui_n": uf.get(comp, getattr(case, comp)), "ui_o": uf[comp + "_o"], "ui_oo": uf[comp + "_oo"],
+            "source": uf["src_" + comp], "pi": getattr(case, pname), "g": [uf[k] for k in grads],
+            "S": [S[q] for q in order], "D": [D[q] for q in order], "flux_u": uf["flux_u"]}

@@ -1,0 +1,269 @@
+"""CPU pins of the momentum-assembly oracle (oracle/calcps_oracle.c,
+oracle_newuvw, after Process/NewUVW.f90:137-499).
+
+The reference routine needs the whole solver's modules (all_mod, pro_mod,
+Moin_Problem_Mod, CalMinMax, bicg) and cannot be compiled on its own, and the
+reference holds no vectors for it, so the restatement is pinned by:
+  * hand values on a two-cell mesh with every boundary type (convection with
+    the upwind part and relaxation; diffusion with cross diffusion);
+  * identities the discretisation fixes:
+      - convection: with the implicit upwind matrix and the deferred
+        correction URFC (C - X), b - A rhoU^n is the central convective flux
+        sum of every cell whatever the upwind part is;
+      - diffusion: for a linear velocity field with its exact gradients,
+        b - A rhoU^n vanishes (the implicit and explicit normal parts cancel
+        and the stress flux of a constant gradient sums to zero over a closed
+        cell) -- for U, V and W, so the argument order of
+        Process/Processor.f90:314-340 is pinned too;
+      - the stress flux against the tensor form
+        tau = G + G^T - 2/3 tr(G) I on a single interior face;
+      - inertia and relaxation: the diagonal-only system solves to the
+        Euler / BDF2 step blended with the old value by URF;
+  * the BLEND values it refuses.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from test_calcps_oracle import _two_cell_case
+
+YES, NO, FI, CN, LIN, PAR = 10, 11, 3, 2, 4, 5
+
+
+def _hand_case(pkg):
+    """The two-cell mesh of the pressure tests (faces: 1-2 interior, OUTFLOW
+    (1,-1), INFLOW (2,-2), PRESSURE (2,-3), SYMMETRY (1,-4), BUFFER (1,-5))
+    with a density on every boundary cell."""
+    case = _two_cell_case(pkg)
+    at = lambda c: c + case.nbc                                         # noqa: E731
+    rho = np.zeros(case.nbc + case.nc + 1)
+    rho[at(1)], rho[at(2)] = 1.0, 2.0
+    for c in range(1, case.nbc + 1):
+        rho[at(-c)] = 1.0
+    u = np.zeros_like(rho)
+    u[at(1)], u[at(2)], u[at(-2)], u[at(-1)] = 2.0, 1.0, 1.5, 0.75
+    return dataclasses.replace(case, rho_n=rho, u=u)
+
+
+def _hand_fields(case, flux, grads=None):
+    n = case.nbc + case.nc + 1
+    at = lambda c: c + case.nbc                                         # noqa: E731
+    sx, sy, sz, dx, dy, dz = case.sx, case.sy, case.sz, case.dx, case.dy, case.dz
+    with np.errstate(invalid="ignore", divide="ignore"):
+        scoef = ((sx * sx + sy * sy) + sz * sz) / ((dx * sx + dy * sy) + dz * sz)
+    uf = {"scoef": scoef,
+          "ff": np.array([0.5, 1.0, 1.0, 1.0, 1.0, 0.5]), "material": np.ones(n, np.int32), "nmaterials": 1,
+          "flux_u": np.asarray(flux, float)}
+    for k in ("ux", "uy", "uz", "vx", "vy", "vz", "wx", "wy", "wz"):
+        uf[k] = np.full(n, (grads or {}).get(k, 0.0))
+    for k in "uvw":
+        uf[k + "_o"], uf[k + "_oo"], uf["src_" + k] = np.zeros(n), np.zeros(n), np.zeros(n)
+    uf["u_o"][at(1)], uf["u_o"][at(2)] = 1.0, -1.0
+    uf["src_u"][at(1)] = 0.5
+    return uf
+
+
+def test_hand_values_convection(pkg, oracle):
+    """visc = 0, BLEND = BLEND_TEM = YES, URFC = 1, LIN inertia (dt 0.5,
+    V = 1), Px = (1, 3), URF = 0.5.  rhoU: cell 1 2, cell 2 2, cell -2 1.5.
+    Faces (Flux_u; rhoUiS): 1-2 (3; 2), OUTFLOW (1; 2), INFLOW (-2; 1.75),
+    PRESSURE (0.5; 2), SYMMETRY (0; 2), BUFFER (-1; 1).
+    C = (-6 - 2 - 0 + 1, 6 + 3.5 - 1) = (-7, 8.5); upwind X = (-6 - 2, 6 + 3
+    - 1) = (-8, 8); b = C - X = (1, 0.5).  Matrix: face 1-2 A21 = 3 -> diag1
+    3, A(2,1) = -3, A(1,2) = 0; OUTFLOW diag1 += 1; INFLOW A12 = 2 -> b2 +=
+    2 * 1.5; BUFFER A12 = 1 -> Abou(-5) = -1.  Inertia: diag += 2, b1 += 2 +
+    0.5, b2 += -2; pressure b -= (1, 3): b = (2.5, -1.5), Asave = (6, 2).
+    Relaxation: b += diag * 0.5 * 2 / 0.5 = (12, 4), diag / 0.5."""
+    case = _hand_case(pkg)
+    uf = _hand_fields(case, [3.0, 1.0, -2.0, 0.5, 0.0, -1.0])
+    px = np.zeros(case.nbc + case.nc + 1)
+    px[case.nbc + 1], px[case.nbc + 2] = 1.0, 3.0
+    case = dataclasses.replace(case, px=px, dt=0.5)
+    p = pkg.meshgen.uvw_params(visc=0.0, dt=0.5, urf=0.5, blend=[YES], blend_tem=[YES])
+    out = oracle.newuvw(case, uf, p)
+    # row 1: [diag, (1,2)], row 2: [(2,1), diag], trailing slot
+    assert out["aval"].tolist() == [12.0, 0.0, -3.0, 4.0, 0.0]
+    assert out["b"].tolist() == [14.5, 2.5]
+    assert out["abou"].tolist() == [-1.0, 0.0, 0.0, 0.0, 0.0]
+    assert out["asave"][case.nbc + 1:].tolist() == [6.0, 2.0]
+    assert out["rhoui"].tolist() == [0.0, 0.0, 0.0, 1.5, 0.75, 0.0, 2.0, 2.0]
+
+
+def test_hand_values_diffusion(pkg, oracle):
+    """No flux, visc = 0.5, BLEND = NO, no inertia, URF 1 (rhoU(-1) = 0.75
+    at the OUTFLOW face, which takes no diffusion term, :396-399); constant
+    gradients ux = 1, uy = 2, vx = 3, wz = -1, so for U the stress flux is
+    Fex = visc (4/3 + 2/3) Sx + visc (2 + 3) Sy = Sx + 2.5 Sy and
+    Fim = visc Scoef (Dx + 2 Dy).  Scoef = (2, 2, 1, 2, 1, 1).
+    X: face 1-2 2 - 1 = 1 (cell 1), -1 (cell 2); OUTFLOW 10 - 4; INFLOW 0;
+    PRESSURE 1 - 0.5; SYMMETRY 0; BUFFER -1 + 0.5: X = (6.5, -0.5).
+    D21/D12: face 1-2 A0 = 1: diag1 1, A(2,1) -1, diag2 0.5, A(1,2) -0.5;
+    INFLOW A0 = 0.5: diag2 += 0.25, b2 += 0.5 * 1.5; BUFFER: diag1 += 0.5,
+    Abou(-5) = -0.5.  b = (6.5, 0.75 - 0.5) - Px V = (6.5 - 1, 0.25 - 3)."""
+    case = _hand_case(pkg)
+    uf = _hand_fields(case, np.zeros(6), {"ux": 1.0, "uy": 2.0, "vx": 3.0, "wz": -1.0})
+    p = pkg.meshgen.uvw_params(visc=0.5, inert=0, urf=1.0)
+    out = oracle.newuvw(case, uf, p)
+    np.testing.assert_allclose(out["aval"], [1.5, -0.5, -1.0, 0.75, 0.0], rtol=1e-15, atol=0)
+    np.testing.assert_allclose(out["b"], [5.5, -2.75], rtol=1e-14, atol=0)
+    assert out["abou"].tolist() == [-0.5, 0.0, 0.0, 0.0, 0.0]
+
+
+def _apply(case, out, x):
+    """A x with the assembled values (rows 1..NC), plus Abou(c2) x(c2) for
+    BUFFER faces; x on (-NbC:NC)."""
+    m = case.matrix(out["aval"]).to_scipy()
+    y = m @ x[case.nbc + 1:]
+    c1, c2 = case.sidec[1], case.sidec[2]
+    buf = (c2 < 0) & (case.typebc[case.nbc + np.minimum(c2, 0)] == 11)
+    np.add.at(y, c1[buf] - 1, out["abou"][case.nbc + c2[buf]] * x[case.nbc + c2[buf]])
+    return y
+
+
+CLOSED_BC = {"xmin": "inflow", "xmax": "convect", "ymin": "wall", "ymax": "wallfl", "zmin": "outflow",
+             "zmax": "buffer"}
+# every boundary face with implicit diffusion (OUTFLOW has none, :396-399)
+DIFFUSIVE_BC = dict(CLOSED_BC, zmin="wall")
+
+
+@pytest.mark.parametrize("nmat", [1, 2])
+def test_convection_identity(pkg, oracle, nmat):
+    """With the upwind matrix (BLEND = YES) and its explicit counterpart in b
+    (BLEND_TEM = YES), URFC (C - X): b - A rhoU^n = URFC C + (1 - URFC) X per
+    cell, C the central flux sum -sum_s sign Flux_u rhoUiS and X the upwind
+    one, both summed here with numpy over the face list."""
+    mg = pkg.meshgen
+    case = mg.calcps_case((7, 6, 5), seed=41, bc=CLOSED_BC, face_seed=3)
+    case = dataclasses.replace(case, px=np.zeros_like(case.px))
+    uf = mg.uvw_fields(case, seed=42, nmaterials=nmat)
+    urfc = [1.0, 0.625][:nmat]
+    p = mg.uvw_params(nmaterials=nmat, visc=0.0, inert=0, urf=1.0, blend=[YES] * nmat, blend_tem=[YES] * nmat,
+                      urfc=urfc)
+    out = oracle.newuvw(case, uf, p)
+    nb = case.nbc
+    ru = case.rho_n * case.u
+    c1, c2 = case.sidec[1].astype(int), case.sidec[2].astype(int)
+    F = uf["flux_u"]
+    cen = F * (case.f * ru[nb + c1] + (1 - case.f) * ru[nb + c2])
+    upw = F * np.where(F < 0, ru[nb + c2], ru[nb + c1])
+    Cs, Xs = np.zeros(case.nc), np.zeros(case.nc)
+    inner = c2 > 0
+    np.add.at(Cs, c1 - 1, -cen)
+    np.add.at(Cs, c2[inner] - 1, cen[inner])
+    np.add.at(Xs, c1 - 1, -upw)
+    np.add.at(Xs, c2[inner] - 1, upw[inner])
+    w = np.asarray(urfc)[uf["material"][nb + 1:] - 1]
+    lhs = out["b"] - _apply(case, out, ru)
+    scale = np.abs(cen).max() * 8
+    np.testing.assert_allclose(lhs, w * Cs + (1 - w) * Xs, rtol=0, atol=1e-13 * scale)
+    assert np.abs(Cs).max() > 1e-2 * scale / 8       # the identity is not vacuous
+
+
+@pytest.mark.parametrize("var", [1, 2, 3])
+def test_diffusion_identity_linear_field(pkg, oracle, var):
+    """Linear velocity u_i = G_i . x with its exact (constant) gradients,
+    varying density, no flux: the implicit diffusion (D12 = A0/rho(c2),
+    D21 = A0/rho(c1)) applied to rhoU gives A0 (u1 - u2) = -Fim on each face,
+    which cancels the -Fim in the cross-diffusion part, and the stress flux
+    of a constant gradient sums to zero over a closed cell: b - A rhoU^n = 0
+    to rounding.  A wrong pairing of Si..Dk with the gradients (var), a
+    swapped D12/D21, or a sign in X breaks it."""
+    mg = pkg.meshgen
+    case = mg.calcps_case((6, 7, 5), seed=43, bc=DIFFUSIVE_BC, face_seed=5)
+    case = dataclasses.replace(case, px=np.zeros_like(case.px), py=np.zeros_like(case.py),
+                               pz=np.zeros_like(case.pz))
+    G = np.random.default_rng(44).uniform(-1, 1, (3, 3))
+    uf = mg.uvw_fields(case, seed=45, linear=G)
+    uf["flux_u"] = np.zeros(case.ns)
+    p = mg.uvw_params(var=var, visc=0.3, inert=0, urf=1.0)
+    out = oracle.newuvw(case, uf, p)
+    comp = "uvw"[var - 1]
+    ru = case.rho_n * uf[comp]
+    lhs = out["b"] - _apply(case, out, ru)
+    scale = np.abs(out["aval"]).max() * np.abs(uf[comp]).max()
+    assert np.abs(lhs).max() <= 1e-12 * scale
+    # the stress part is there: without the cross diffusion the residual is
+    # the stress flux, which a constant gradient cancels only over the
+    # whole cell (not face by face)
+    p0 = dict(p, cross=0)
+    out0 = oracle.newuvw(case, uf, p0)
+    assert np.abs(out0["b"] - _apply(case, out0, ru)).max() > 1e-6 * scale
+
+
+@pytest.mark.parametrize("var", [1, 2, 3])
+def test_stress_flux_tensor_form(pkg, oracle, var):
+    """On the two-cell mesh with only the interior face carrying a stress
+    flux (other faces have zero S and D), cell 1's b = Fex - Fim with
+    Fex = visc (tau S)_i, tau = G + G^T - 2/3 tr(G) I, Fim = visc Scoef
+    (G_i . D) -- the tensor written out with numpy, independent of the
+    argument permutation."""
+    mg = pkg.meshgen
+    case = _hand_case(pkg)
+    S = np.zeros((6, 3))
+    D = np.zeros((6, 3))
+    S[0] = [0.7, -0.4, 1.3]
+    D[0] = [1.1, 0.2, -0.3]
+    for k in range(1, 6):                     # boundary faces: no area, no distance
+        D[k] = [1.0, 0.0, 0.0]
+    case = dataclasses.replace(case, sx=S[:, 0].copy(), sy=S[:, 1].copy(), sz=S[:, 2].copy(),
+                               dx=D[:, 0].copy(), dy=D[:, 1].copy(), dz=D[:, 2].copy(), px=np.zeros_like(case.px))
+    G = np.array([[0.3, -1.2, 0.5], [0.8, 0.1, -0.7], [1.5, 0.4, -0.9]])
+    grads = {a + b: G[i, j] for i, a in enumerate("uvw") for j, b in enumerate("xyz")}
+    uf = _hand_fields(case, np.zeros(6), grads)
+    uf["scoef"][1:] = 0.0                     # no area on the boundary faces
+    visc = 0.37
+    p = mg.uvw_params(var=var, visc=visc, inert=0, urf=1.0, diffus=0)
+    out = oracle.newuvw(case, uf, p)
+    i = var - 1
+    tau = G + G.T - 2.0 / 3.0 * np.trace(G) * np.eye(3)
+    fex = visc * tau[i] @ S[0]
+    scoef = (S[0] @ S[0]) / (S[0] @ D[0])
+    fim = visc * scoef * (G[i] @ D[0])
+    np.testing.assert_allclose(out["b"], [fex - fim, -(fex - fim)], rtol=1e-13)
+
+
+@pytest.mark.parametrize("inert", [LIN, PAR])
+def test_inertia_relaxation_closed_form(pkg, oracle, inert):
+    """No flux and no viscosity: the system is diagonal and solves to
+    x = URF x* + (1 - URF) rhoU^n, x* the Euler step rho_o u_o + (s - Pi) dt
+    (LIN) or the BDF2 step (2 rho_o u_o - rho_oo u_oo / 2 + (s - Pi) dt) / 1.5
+    (PAR); Asave is V/dt or 1.5 V/dt."""
+    mg = pkg.meshgen
+    case = mg.calcps_case((5, 6, 4), seed=46, bc=CLOSED_BC)
+    uf = mg.uvw_fields(case, seed=47)
+    uf["flux_u"] = np.zeros(case.ns)
+    dt, urf = 0.02, 0.7
+    p = mg.uvw_params(visc=0.0, inert=inert, urf=urf, dt=dt)
+    out = oracle.newuvw(case, uf, p)
+    nb = case.nbc
+    cells = slice(nb + 1, None)
+    diag = out["aval"][case.adia - 1]
+    off = np.delete(out["aval"], case.adia - 1)
+    assert np.all(off == 0.0) and np.all(out["abou"] == 0.0)
+    x = out["b"] / diag
+    ro, uo = case.rho_o[cells], uf["u_o"][cells]
+    net = (uf["src_u"][cells] - case.px[cells]) * dt
+    if inert == LIN:
+        xs = ro * uo + net
+        V = case.volume[cells] / dt
+    else:
+        xs = (2.0 * ro * uo - 0.5 * case.rho_oo[cells] * uf["u_oo"][cells] + net) / 1.5
+        V = 1.5 * case.volume[cells] / dt
+    want = urf * xs + (1 - urf) * (case.rho_n * case.u)[cells]
+    np.testing.assert_allclose(x, want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
+    np.testing.assert_allclose(out["asave"][cells], V, rtol=1e-15)
+
+
+def test_blend_schemes_refused(pkg, oracle):
+    """A TVD face scheme (QUICK = 21) and YES next to NO at a face (where
+    the reference's ConvScheme leaves Phij undefined) are refused; YES on
+    both materials is not."""
+    mg = pkg.meshgen
+    case = mg.calcps_case((6, 4, 3), seed=48)
+    uf = mg.uvw_fields(case, seed=49, nmaterials=2)
+    with pytest.raises(ValueError):
+        oracle.newuvw(case, uf, mg.uvw_params(nmaterials=2, blend=[21, 21]))
+    with pytest.raises(ValueError):
+        oracle.newuvw(case, uf, mg.uvw_params(nmaterials=2, blend=[YES, NO]))
+    oracle.newuvw(case, uf, mg.uvw_params(nmaterials=2, blend=[YES, YES]))
