@@ -373,69 +373,131 @@ def uvw_bytes(case, var):
             (8 * case.nc if var == 1 else 0))
 
 
+def zmix_bytes(case):
+    """Algorithmic bytes of one mixture-fraction assembly (CalcZmix.f90:
+    100-387): per face as the momentum's (97 B); per cell rho n/o/oo, Z
+    n/o/oo, source, three gradients, volume and the material (84 B); Aval,
+    b, rhoZ and Abou written once."""
+    ncell = case.nbc + case.nc + 1
+    return 97 * case.ns + 84 * ncell + 8 * case.nonz + 8 * case.nc + 8 * ncell + 8 * case.nbc
+
+
 def workload_uvw(args, torch, pkg):
-    """Momentum assembly (NewUVW.f90:137-499) of U, V and W on the n^3
-    channel box on the GPU, then the NewUVW step: the three BiCG solves
-    from rhoUi (cap 200, :505) and the update (:526)."""
+    """Momentum assembly of U, V, W (NewUVW.f90:137-499) and the
+    mixture-fraction assembly (CalcZmix.f90:100-387) on the n^3 channel box
+    on the GPU, then the step they feed: BiCG for U, V, W (cap 200, :505),
+    CG for Zmix (cap 20, :388-394) and the updates -- the four systems one
+    after another, and as one batch on their shared pattern."""
     n = args.n
     mg = pkg.meshgen
     log(f"generating the {n}^3 momentum case")
     case = mg.calcps_case((n, n, n), seed=1)
     uf = mg.uvw_fields(case, seed=2)
+    zf = mg.zmix_fields(case, seed=3)
+    zf.update({k: uf[k] for k in ("scoef", "ff", "material", "flux_u")})
     dc = pkg.krylov.DeviceCalcPS(case)
     dc.uvw_setup(uf)
     params = [mg.uvw_params(var=v, blend=[10], blend_tem=[10], urf=0.7) for v in (1, 2, 3)]
+    zparams = mg.uvw_params(blend_tem=[10], urf=0.8, visc=2e-3)
     inputs = [dc.uvw_inputs(mg.uvw_args(case, uf, v)) for v in (1, 2, 3)]
+    zinputs = dc.zmix_inputs(mg.zmix_args(case, zf))
     outs = [dc.assemble_uvw(params[v], inputs[v]) for v in range(3)]
+    zout = dc.assemble_zmix(zparams, zinputs)
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     reps = 20
-    t_var = []
-    for v in range(3):
+
+    def event_ms(fn):
         e0.record(s)
         for _ in range(reps):
-            dc.assemble_uvw(params[v], inputs[v], out=outs[v])
+            fn()
         e1.record(s)
         e1.synchronize()
-        t_var.append(e0.elapsed_time(e1) / reps)
+        return e0.elapsed_time(e1) / reps
+    t_var = [event_ms(lambda v=v: dc.assemble_uvw(params[v], inputs[v], out=outs[v])) for v in range(3)]
+    t_z = event_ms(lambda: dc.assemble_zmix(zparams, zinputs, out=zout))
     nbytes = [uvw_bytes(case, v) for v in (1, 2, 3)]
+    zb = zmix_bytes(case)
     m = case.matrix(outs[0]["aval"].cpu().numpy())
-    systems = [pkg.krylov.DeviceSystem(m) for _ in range(3)]
+    systems = [pkg.krylov.DeviceSystem(m) for _ in range(4)]
     nb = case.nbc
     rho = torch.from_numpy(case.rho_n).cuda()
     ui = [t[3].clone() for t in inputs]
-    tol = [1e-8 * float(torch.linalg.norm(o["b"])) for o in outs]
+    z = zinputs[3].clone()
+    eqs = outs + [zout]
+    xkey = ["rhoui"] * 3 + ["rhoz"]
+    caps = [200, 200, 200, 20]
+    tol = [1e-8 * float(torch.linalg.norm(o["b"])) for o in eqs]
 
-    def step():
-        its = []
+    def assemble_all():
         for v in range(3):
-            o = dc.assemble_uvw(params[v], inputs[v], out=outs[v])
-            systems[v].set_values(o["aval"])
-            x = o["rhoui"][nb + 1:]
-            its.append(systems[v].solve("bicg", 1, x, o["b"], 200, tol[v])[0])
-            dc.uvw_update(o["rhoui"], rho, ui[v])
+            dc.assemble_uvw(params[v], inputs[v], out=outs[v])
+        dc.assemble_zmix(zparams, zinputs, out=zout)
+
+    def update_all():
+        for v in range(3):
+            dc.uvw_update(outs[v]["rhoui"], rho, ui[v])
+        dc.zmix_update(zout["rhoz"], rho, z)
+
+    def step_sequential():
+        assemble_all()
+        its = []
+        for k in range(4):
+            o = eqs[k]
+            systems[k].set_values(o["aval"])
+            its.append(systems[k].solve("bicg" if k < 3 else "cg", 1, o[xkey[k]][nb + 1:], o["b"], caps[k],
+                                        tol[k])[0])
+        update_all()
         return its
-    step()
-    ms_step, its = _timed(torch, step, args.steps)
+
+    batch = pkg.krylov.DeviceBatch([m] * 4)
+    xb = torch.zeros(batch.rows, dtype=torch.float64, device="cuda")
+    rb = torch.zeros_like(xb)
+    xs, rs = batch.split(xb), batch.split(rb)
+
+    def step_batched():
+        # BiCG runs through the CG kernels (its results are CG's), so one
+        # batched CG serves U, V, W and Zmix
+        assemble_all()
+        for k in range(4):
+            batch.set_values(k, eqs[k]["aval"])
+            xs[k].copy_(eqs[k][xkey[k]][nb + 1:])
+            rs[k].copy_(eqs[k]["b"])
+        res = batch.solve("cg", 1, xb, rb, np.array(caps), np.array(tol))
+        for k in range(4):
+            eqs[k][xkey[k]][nb + 1:].copy_(xs[k])
+        update_all()
+        return [int(i) for i in res["niter"]]
+    step_sequential()
+    ms_seq, its = _timed(torch, step_sequential, args.steps)
+    step_batched()
+    ms_bat, its_b = _timed(torch, step_batched, args.steps)
     gbs = [b / (t * 1e-3) / 1e9 for b, t in zip(nbytes, t_var)]
-    res = {"metric": "momentum assembly of U, V, W on the GPU, and the NewUVW step", "unit": "ms",
+    res = {"metric": "momentum (U, V, W) and mixture-fraction assembly on the GPU, and the step they feed",
+           "unit": "ms",
            "workload": f"channel {n}^3 (NC={case.nc}, NS={case.ns}, NbC={case.nbc}, NONZ={case.nonz}), "
-                       f"central convection with the upwind part, LIN inertia, URF 0.7",
+                       f"central convection with the upwind part, LIN inertia",
            "assembly_ms": [round(t, 4) for t in t_var], "assembly_bytes": nbytes,
            "assembly_GBs": [round(g, 1) for g in gbs],
+           "zmix_assembly_ms": round(t_z, 4), "zmix_bytes": zb, "zmix_GBs": round(zb / (t_z * 1e-3) / 1e9, 1),
            "roofline": {"bound": "hbm", "achieved": round(gbs[0], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(gbs[0] / HBM_PEAK_GBS, 4)},
-           "step_ms": round(ms_step, 2), "bicg_iters": its}
+           "step_sequential_ms": round(ms_seq, 2), "iters": its,
+           "step_batched_ms": round(ms_bat, 2), "iters_batched": its_b}
     if not args.no_cpu:
         from oracle import oracle as O
         orc = O.Oracle()
         t0 = time.perf_counter()
         orc.newuvw(case, uf, params[0])
         el = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        orc.calczmix(case, zf, zparams)
+        elz = time.perf_counter() - t0
         res["cpu_baseline"] = {"value": round(el * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
                                "sample": f"one U assembly of the same {n}^3 case by oracle/calcps_oracle.c, "
-                                         f"1 thread"}
+                                         f"1 thread (Zmix: {elz * 1e3:.1f} ms)"}
+    batch.close()
     for q in systems:
         q.close()
     dc.close()

@@ -327,6 +327,25 @@ int tflows_calcps_uvw_assemble(tflows_calcps h, const tflows_uvw_params* p, cons
 int tflows_calcps_uvw_update(tflows_calcps h, const double* rhoui, const double* rho_n, double* ui_n, int32_t mem,
                              void* stream);
 
+/* The mixture-fraction assembly, Process/CalcZmix.f90:100-387 (the
+ * reference's `CalcZmix` up to its CG call at :394), on the same handle
+ * after tflows_calcps_uvw_setup.  In *p: blend_tem = BLEND_TEM (it drives
+ * the face scheme, the upwind matrix and the upwind part), urfc = URFC_Tem,
+ * visc = VIScZmix, urf = Zmix%URF, convec/diffus/cross/inert/dt as for the
+ * momentum (var and blend are not read).  Cell arrays (-NbC:NC): rho_n/o/oo,
+ * Z n/o/oo, the source and dZ/dx, dZ/dy, dZ/dz; Flux_u(NS).  Writes
+ * Aval(NONZ), b(NC), Abou(-NbC:-1) and rhoZ(-NbC:NC), bit for bit the
+ * reference's; BLEND_TEM schemes other than YES/NO are refused.
+ * tflows_calcps_zmix_update: Zmix%n(1:NC) = rhoZ / rho%n limited to [0, 1]
+ * (:401-405). */
+int tflows_calcps_zmix_assemble(tflows_calcps h, const tflows_uvw_params* p, const double* rho_n,
+                                const double* rho_o, const double* rho_oo, const double* z_n, const double* z_o,
+                                const double* z_oo, const double* source, const double* dZdx, const double* dZdy,
+                                const double* dZdz, const double* flux_u, double* aval, double* b, double* abou,
+                                double* rhoz, int32_t mem, void* stream);
+int tflows_calcps_zmix_update(tflows_calcps h, const double* rhoz, const double* rho_n, double* z_n, int32_t mem,
+                              void* stream);
+
 #ifdef __cplusplus
 }
 #endif

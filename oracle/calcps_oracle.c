@@ -366,3 +366,157 @@ int oracle_newuvw(int nc, int nbc, int ns, int nonz, int var, int convec, int di
 #undef MAT
 #undef C
 }
+
+/*
+ * oracle_calczmix -- the assembly of the mixture-fraction equation,
+ * Process/CalcZmix.f90:100-387 (everything before the CG call at :394; the
+ * old-level shift :110-113 and the source :141-151 are the caller's):
+ *
+ *   :100      rhoZ = rho^n Z^n on (-NbC:NC)
+ *   :104-107  Abou = 0, Aval = 0, b = 0
+ *   :134-135  C = 0, X = 0
+ *   :155-189  convection face loop: rhoZS = f rhoZ(c1) + (1-f) rhoZ(c2),
+ *             left as it is by ConvScheme for BLEND_TEM = YES; C, and the
+ *             upwind part X when BLEND_TEM /= NO
+ *   :201-207  CONVEC = FI: b += URFC_Tem(material) (C - X)
+ *   :217-219  X = 0 on the cells
+ *   :224-341  diffusion face loop: Fex = VISeff grad(Z)_f . S,
+ *             Fim = A0 grad(Z)_f . D, X += Fex - Fim; D12 = A0/rho(c2),
+ *             D21 = A0/rho(c1) for DIFFUS = FI; the upwind A12, A21 when
+ *             BLEND_TEM /= NO; the "m1" fill and the boundary branches
+ *   :359-361  CROSS = FI: b += X
+ *   :369-378  inertia, LIN: b += V/dt Z_o rho_o + S V (in that order);
+ *             PAR: b += 2 V/dt rho_o Z_o - V/(2 dt) rho_oo Z_oo + S V
+ *   :386-387  under-relaxation with Zmix%URF
+ *
+ * VISeff = VIScZmix (the default of :236-242).  Per-material BLEND_TEM and
+ * URFC_Tem at [m-1].  Returns -1 where a face's max(BLEND_TEM) other than
+ * YES calls a ConvScheme branch not covered; abou is (-NbC:-1); cbuf, xbuf
+ * hold NbC+NC+1 doubles.  Returns 0.
+ */
+int oracle_calczmix(int nc, int nbc, int ns, int nonz, int convec, int diffus, int cross, int inert,
+                    const int *sidec, const int *sidaij, const int *adia, const int *typebc, const int *material,
+                    const int *blend_tem, const double *urfc_tem, const double *f, const double *fF,
+                    const double *scoef, const double *flux_u, const double *sx, const double *sy, const double *sz,
+                    const double *dx, const double *dy, const double *dz, const double *volume,
+                    const double *rho_n, const double *rho_o, const double *rho_oo, const double *z_n,
+                    const double *z_o, const double *z_oo, const double *source, const double *dZdx,
+                    const double *dZdy, const double *dZdz, double visc, double dt, double urf, double *aval,
+                    double *b, double *abou, double *rhoz, double *cbuf, double *xbuf)
+{
+#define C(x, c) (x)[nbc + (c)]
+#define MAT(arr, c) (arr)[C(material, c) - 1]
+    const int nall = nbc + nc + 1;
+    for (int c = -nbc; c <= nc; ++c) C(rhoz, c) = C(rho_n, c) * C(z_n, c);    /* :100 */
+    for (int c = 0; c < nbc; ++c) abou[c] = 0.0;                        /* :104 */
+    for (int j = 0; j < nonz; ++j) aval[j] = 0.0;                       /* :106 */
+    for (int c = 0; c < nc; ++c) b[c] = 0.0;                            /* :107 */
+    for (int c = 0; c < nall; ++c) { cbuf[c] = 0.0; xbuf[c] = 0.0; }   /* :134-135 */
+
+    for (int s = 1; s <= ns; ++s) {                                     /* :155-189 */
+        const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
+        const double rhoZS = f[s - 1] * C(rhoz, c1) + (1.0 - f[s - 1]) * C(rhoz, c2);
+        const int up = MAT(blend_tem, c1) != BL_NO || MAT(blend_tem, c2) != BL_NO;
+        if (up) {
+            const int bl = MAT(blend_tem, c1) > MAT(blend_tem, c2) ? MAT(blend_tem, c1) : MAT(blend_tem, c2);
+            if (bl != BL_YES) return -1;                                /* ConvScheme: not covered */
+        }
+        const double F = flux_u[s - 1];
+        if (c2 > 0) {
+            C(cbuf, c1) = C(cbuf, c1) - F * rhoZS;
+            C(cbuf, c2) = C(cbuf, c2) + F * rhoZS;
+        } else {
+            C(cbuf, c1) = C(cbuf, c1) - F * rhoZS;
+        }
+        if (up) {
+            if (F < 0) {
+                C(xbuf, c1) = C(xbuf, c1) - F * C(rhoz, c2);
+                if (c2 > 0) C(xbuf, c2) = C(xbuf, c2) + F * C(rhoz, c2);
+            } else {
+                C(xbuf, c1) = C(xbuf, c1) - F * C(rhoz, c1);
+                if (c2 > 0) C(xbuf, c2) = C(xbuf, c2) + F * C(rhoz, c1);
+            }
+        }
+    }
+    if (convec == FL_FI)                                                /* :201-207 */
+        for (int c = 1; c <= nc; ++c)
+            b[c - 1] = b[c - 1] + MAT(urfc_tem, c) * (C(cbuf, c) - C(xbuf, c));
+    for (int c = 1; c <= nc; ++c) C(xbuf, c) = 0.0;                     /* :217-219 */
+
+    for (int s = 1; s <= ns; ++s) {                                     /* :224-341 */
+        double A12 = 0.0, A21 = 0.0, D12 = 0.0, D21 = 0.0;
+        const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
+        const double ff = fF[s - 1], gg = 1.0 - fF[s - 1];
+        const double VISeff = visc;
+        const double dZdxS = ff * C(dZdx, c1) + gg * C(dZdx, c2);
+        const double dZdyS = ff * C(dZdy, c1) + gg * C(dZdy, c2);
+        const double dZdzS = ff * C(dZdz, c1) + gg * C(dZdz, c2);
+        const double Fex = VISeff * ((dZdxS * sx[s - 1] + dZdyS * sy[s - 1]) + dZdzS * sz[s - 1]);   /* :250 */
+        const double A0 = VISeff * scoef[s - 1];
+        const double Fim = A0 * ((dZdxS * dx[s - 1] + dZdyS * dy[s - 1]) + dZdzS * dz[s - 1]);     /* :255-258 */
+        C(xbuf, c1) = C(xbuf, c1) + Fex - Fim;                          /* :261-264 */
+        if (c2 > 0) C(xbuf, c2) = C(xbuf, c2) - Fex + Fim;
+        if (diffus == FL_CN || diffus == FL_FI) {
+            if (diffus == FL_FI) {
+                D12 = D12 + A0 / C(rho_n, c2);
+                D21 = D21 + A0 / C(rho_n, c1);
+            }
+            if (MAT(blend_tem, c1) != BL_NO || MAT(blend_tem, c2) != BL_NO) {
+                const double F = flux_u[s - 1];
+                A12 = A12 - (F < 0.0 ? F : 0.0);
+                A21 = A21 + (F > 0.0 ? F : 0.0);
+            }
+            if (c2 > 0) {                                               /* "m1" */
+                aval[adia[c1 - 1] - 1] = aval[adia[c1 - 1] - 1] + A21 + D21;
+                aval[sidaij[2 * (s - 1) + 1] - 1] = aval[sidaij[2 * (s - 1) + 1] - 1] - A21 - D21;
+                aval[adia[c2 - 1] - 1] = aval[adia[c2 - 1] - 1] + A12 + D12;
+                aval[sidaij[2 * (s - 1)] - 1] = aval[sidaij[2 * (s - 1)] - 1] - A12 - D12;
+            } else if (c2 < 0) {
+                const int t = C(typebc, c2);
+                if (t == BC_INFLOW || t == BC_WALL || t == BC_CONVECT || t == BC_WALLFL) {
+                    aval[adia[c1 - 1] - 1] = aval[adia[c1 - 1] - 1] + A21 + D21;
+                    b[c1 - 1] = b[c1 - 1] + (A12 + D12) * C(rhoz, c2);
+                } else if (t == BC_OUTFLOW) {
+                    aval[adia[c1 - 1] - 1] = aval[adia[c1 - 1] - 1] + A21;
+                    b[c1 - 1] = b[c1 - 1] + A12 * C(rhoz, c2);
+                } else if (t == BC_BUFFER) {
+                    aval[adia[c1 - 1] - 1] = aval[adia[c1 - 1] - 1] + A21 + D21;
+                    C(abou, c2) = -A12 - D12;
+                }
+            }
+        }
+    }
+    if (cross == FL_FI)                                                 /* :359-361 */
+        for (int c = 1; c <= nc; ++c) b[c - 1] = b[c - 1] + C(xbuf, c);
+    if (inert == IN_LIN)                                                /* :369-372 */
+        for (int c = 1; c <= nc; ++c) {
+            aval[adia[c - 1] - 1] = aval[adia[c - 1] - 1] + C(volume, c) / dt;
+            b[c - 1] = b[c - 1] + C(volume, c) / dt * C(z_o, c) * C(rho_o, c) + C(source, c) * C(volume, c);
+        }
+    if (inert == IN_PAR)                                                /* :375-378 */
+        for (int c = 1; c <= nc; ++c) {
+            aval[adia[c - 1] - 1] = aval[adia[c - 1] - 1] + 1.5 * C(volume, c) / dt;
+            b[c - 1] = b[c - 1] + 2.0 * C(volume, c) / dt * C(rho_o, c) * C(z_o, c) -
+                       0.5 * C(volume, c) / dt * C(rho_oo, c) * C(z_oo, c) + C(source, c) * C(volume, c);
+        }
+    for (int c = 1; c <= nc; ++c) {                                     /* :386-387 */
+        b[c - 1] = b[c - 1] + aval[adia[c - 1] - 1] * (1.0 - urf) * C(rhoz, c) / urf;
+        aval[adia[c - 1] - 1] = aval[adia[c - 1] - 1] / urf;
+    }
+    return 0;
+#undef MAT
+#undef C
+}
+
+/* Zmix%n(1:NC) = rhoZ / rho%n, then limited to [0, 1] (CalcZmix.f90:401-405):
+ * z = min(max(rhoZ / rho, 0), 1), with max and min taken as (a < b ? b : a)
+ * and (a > b ? b : a). */
+void oracle_zmix_update(int nc, int nbc, const double *rhoz, const double *rho_n, double *z_n)
+{
+    for (int c = 1; c <= nc; ++c) {
+        double z = rhoz[nbc + c] / rho_n[nbc + c];
+        z = z < 0.0 ? 0.0 : z;
+        z = z > 1.0 ? 1.0 : z;
+        z_n[nbc + c] = z;
+    }
+}

@@ -230,6 +230,54 @@ class Oracle(_Solvers):
         return dict(aval=aval, b=b, abou=abou[:case.nbc], asave=asave, rhoui=rhoui)
 
 
+def _calczmix(self, case, zf, params):
+    """Mixture-fraction assembly (oracle/calcps_oracle.c, after
+    Process/CalcZmix.f90:100-387) of a meshgen.CalcPSCase with the inputs of
+    meshgen.zmix_fields; params as meshgen.uvw_params with BLEND_TEM in
+    "blend_tem", URFC_Tem in "urfc", VIScZmix in "visc", Zmix%URF in "urf".
+    Returns dict aval, b, abou (NbC), rhoz ((-NbC:NC))."""
+    fn = self.lib.oracle_calczmix
+    if not getattr(fn, "_typed", False):
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_int] * 8 + [_i32p] * 6 + [_f64p] * 22 + [C.c_double] * 3 + [_f64p] * 6
+        fn._typed = True
+    nall = case.nbc + case.nc + 1
+    c32 = lambda x: np.ascontiguousarray(x, np.int32)     # noqa: E731
+    c64 = lambda x: np.ascontiguousarray(x, np.float64)   # noqa: E731
+    aval = np.zeros(case.nonz)
+    b = np.zeros(case.nc)
+    abou = np.zeros(max(case.nbc, 1))
+    rhoz = np.zeros(nall)
+    cbuf, xbuf = np.zeros(nall), np.zeros(nall)
+    rc = fn(case.nc, case.nbc, case.ns, case.nonz, params["convec"], params["diffus"], params["cross"],
+            params["inert"], c32(case.sidec.T), c32(case.sidaij.T), c32(case.adia), c32(case.typebc),
+            c32(zf["material"]), c32(params["blend_tem"]), c64(params["urfc"]), c64(case.f), c64(zf["ff"]),
+            c64(zf["scoef"]), c64(zf["flux_u"]), c64(case.sx), c64(case.sy), c64(case.sz), c64(case.dx),
+            c64(case.dy), c64(case.dz), c64(case.volume), c64(case.rho_n), c64(case.rho_o), c64(case.rho_oo),
+            c64(zf["z"]), c64(zf["z_o"]), c64(zf["z_oo"]), c64(zf["src_z"]), c64(zf["zx"]), c64(zf["zy"]),
+            c64(zf["zz"]), float(params["visc"]), float(params["dt"]), float(params["urf"]), aval, b, abou, rhoz,
+            cbuf, xbuf)
+    if rc != 0:
+        raise ValueError("BLEND_TEM not covered by the restatement (a ConvScheme face scheme)")
+    return dict(aval=aval, b=b, abou=abou[:case.nbc], rhoz=rhoz)
+
+
+def _zmix_update(self, case, rhoz, rho_n, z_n):
+    """CalcZmix.f90:401-405 on copies: returns the new z ((-NbC:NC))."""
+    fn = self.lib.oracle_zmix_update
+    if not getattr(fn, "_typed", False):
+        fn.restype = None
+        fn.argtypes = [C.c_int, C.c_int, _f64p, _f64p, _f64p]
+        fn._typed = True
+    z = np.array(z_n, dtype=np.float64, copy=True)
+    fn(case.nc, case.nbc, np.ascontiguousarray(rhoz, np.float64), np.ascontiguousarray(rho_n, np.float64), z)
+    return z
+
+
+Oracle.calczmix = _calczmix
+Oracle.zmix_update = _zmix_update
+
+
 def _uvw_args(case, uf, var):
     """newuvw's arrays for var, as Process/Processor.f90:314-340 passes them."""
     grads = {1: ("ux", "uy", "uz", "vx", "wx", "vy", "wz"), 2: ("vy", "vx", "vz", "uy", "wy", "ux", "wz"),

@@ -53,6 +53,7 @@ struct UvwFields {
 
 // Geometry the momentum assembly adds to the mesh (device).
 struct UvwMesh {
+    const int2* inc = nullptr;          // per cface entry: (2 s + role, the face's other cell)
     const unsigned char* ukind = nullptr;  // what a face does in the diffusion loop
     const double *scoef = nullptr, *ff = nullptr;
     const int* material = nullptr;      // (-NbC:NC), 1-based
@@ -91,6 +92,14 @@ public:
                       double* rhoui, hipStream_t s);
     // Ui%n(1:NC) = rhoUi(1:NC) / rho%n(1:NC) after the solve (:526).
     void uvw_update(const double* rhoui, const double* rho_n, double* ui_n, hipStream_t s);
+    // The mixture-fraction system (Process/CalcZmix.f90:100-387): F.ui_* are
+    // Z n/o/oo, F.g[0..2] dZ/dx, dZ/dy, dZ/dz, F.pi unused; P.blend_tem is
+    // BLEND_TEM, P.urfc URFC_Tem, P.visc VIScZmix, P.urf Zmix%URF (P.var and
+    // P.blend are not read).  Aval, b, Abou, rhoZ as for uvw_assemble.
+    void zmix_assemble(const UvwParams& P, const UvwFields& F, double* aval, double* b, double* abou, double* rhoz,
+                       hipStream_t s);
+    // Zmix%n(1:NC) = rhoZ / rho%n limited to [0, 1] (:401-405).
+    void zmix_update(const double* rhoz, const double* rho_n, double* z_n, hipStream_t s);
     bool uvw_ready() const { return U_.ff != nullptr; }
     int nmaterials() const { return nmaterials_; }
 
@@ -102,6 +111,8 @@ public:
     double* staging(size_t doubles);
 
 private:
+    void assemble_equation(const UvwParams& P, const UvwFields& F, double* aval, double* b, double* abou,
+                           double* asave, double* rhoui, bool scalar, hipStream_t s);
     int nc_, nbc_, ns_, nonz_;
     CalcPSMesh M_;
     std::vector<void*> owned_, staging_;

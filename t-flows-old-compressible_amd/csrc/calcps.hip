@@ -330,12 +330,13 @@ struct UvwCell {
     int mat;            // material - 1
 };
 
+template <int NG>
 __device__ __forceinline__ UvwCell uvw_cell(const UvwMesh& U, const UvwFields& F, int idx) {
     UvwCell v;
     v.rho = F.rho_n[idx];
     v.ru = v.rho * F.ui_n[idx];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) v.g[k] = F.g[k][idx];
+    for (int k = 0; k < 7; ++k) v.g[k] = k < NG ? F.g[k][idx] : 0.0;
     v.mat = U.material[idx] - 1;
     return v;
 }
@@ -351,6 +352,7 @@ struct UvwFace {
     double a12, a21, d12, d21;
 };
 
+template <bool SCALAR>
 __device__ __forceinline__ UvwFace uvw_face(const CalcPSMesh& M, const UvwMesh& U, const UvwGeom& G,
                                             const UvwParams& P, const UvwFields& F, int s, const UvwCell& a,
                                             const UvwCell& b) {
@@ -362,13 +364,17 @@ __device__ __forceinline__ UvwFace uvw_face(const CalcPSMesh& M, const UvwMesh& 
     q.upwind = P.blend_tem[a.mat] != kBlendNo || P.blend_tem[b.mat] != kBlendNo;
     q.up = Fu * (Fu < 0 ? b.ru : a.ru);
     const double ff = U.ff[s], gg = 1.0 - U.ff[s];
+    constexpr int NG = SCALAR ? 3 : 7;
     double gS[7];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) gS[k] = ff * a.g[k] + gg * b.g[k];                   // :300-308
+    for (int k = 0; k < NG; ++k) gS[k] = ff * a.g[k] + gg * b.g[k];                  // :300-308
     const double Si = G.si[s], Sj = G.sj[s], Sk = G.sk[s];
     const double visc = P.visc;                                                      // VISeff (:294)
-    q.fex = visc * ((((4.0 / 3.0) * gS[0] * Si + (gS[1] + gS[3]) * Sj) + (gS[2] + gS[4]) * Sk) -
-                    (2.0 / 3.0) * (gS[5] + gS[6]) * Si);                               // :312-315
+    if (SCALAR)                                                                      // CalcZmix.f90:250
+        q.fex = visc * ((gS[0] * Si + gS[1] * Sj) + gS[2] * Sk);
+    else                                                                             // :312-315
+        q.fex = visc * ((((4.0 / 3.0) * gS[0] * Si + (gS[1] + gS[3]) * Sj) + (gS[2] + gS[4]) * Sk) -
+                        (2.0 / 3.0) * (gS[5] + gS[6]) * Si);
     const double A0 = visc * U.scoef[s];                                             // :323
     q.fim = A0 * ((gS[0] * G.di[s] + gS[1] * G.dj[s]) + gS[2] * G.dk[s]);            // :325-327
     q.a12 = q.a21 = q.d12 = q.d21 = 0.0;
@@ -383,24 +389,60 @@ __device__ __forceinline__ UvwFace uvw_face(const CalcPSMesh& M, const UvwMesh& 
     return q;
 }
 
+template <int NG>
 __device__ __forceinline__ UvwCell pick(bool first, const UvwCell& x, const UvwCell& y) {
     UvwCell v;
     v.rho = first ? x.rho : y.rho;
     v.ru = first ? x.ru : y.ru;
 #pragma unroll
-    for (int k = 0; k < 7; ++k) v.g[k] = first ? x.g[k] : y.g[k];
+    for (int k = 0; k < 7; ++k) v.g[k] = k < NG ? (first ? x.g[k] : y.g[k]) : 0.0;
     v.mat = first ? x.mat : y.mat;
     return v;
 }
 
+// The off-diagonal pair of every interior face, (0 - A12) - D12 at
+// SidAij(1,s) and (0 - A21) - D21 at SidAij(2,s) (:364-367), and Abou(c2) of
+// BUFFER faces (:406), one thread per face; used where no two faces share a
+// position.
+__global__ __launch_bounds__(kBlock) void k_uvw_faces(CalcPSMesh M, UvwMesh U, UvwParams P, UvwFields F,
+                                                       double* __restrict__ aval, double* __restrict__ abou) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= M.ns) return;
+    const unsigned char k = U.ukind[s];
+    if (k != kUvwInterior && k != kUvwBuffer) return;
+    const int nb = M.nbc;
+    const int i1 = nb + M.c1[s], i2 = nb + M.c2[s];
+    const double Fu = F.flux_u[s];
+    const double A0 = P.visc * U.scoef[s];
+    double a12 = 0.0, a21 = 0.0, d12 = 0.0, d21 = 0.0;
+    if (P.diffus == kFlagFI) {
+        d12 = 0.0 + A0 / F.rho_n[i2];
+        d21 = 0.0 + A0 / F.rho_n[i1];
+    }
+    if (P.blend[U.material[i1] - 1] != kBlendNo || P.blend[U.material[i2] - 1] != kBlendNo) {
+        a12 = 0.0 - (Fu < 0.0 ? Fu : 0.0);
+        a21 = 0.0 + (Fu > 0.0 ? Fu : 0.0);
+    }
+    if (k == kUvwInterior) {
+        aval[M.sid1[s]] = 0.0 - a12 - d12;
+        aval[M.sid2[s]] = 0.0 - a21 - d21;
+    } else {
+        abou[i2] = -a12 - d12;
+    }
+}
+
 // One thread per cell: the two face loops walked over the cell's faces in
 // ascending face order (the order in which the reference's loops reach the
-// cell), then the cell loops (:252-257, :434-499).  The cell's own values
-// are read once; per face only the other cell's and the face's.  Each
-// off-diagonal of row c is written by the thread of cell c (SHARED: several
-// faces write one position, so it accumulates on a zeroed Aval); the
-// diagonal and b stay in registers.
-template <bool SHARED>
+// cell), then the cell loops (:252-257, :434-499).  SCALAR is the
+// mixture-fraction equation (Process/CalcZmix.f90:100-387): three gradients,
+// the flux VISc grad(Z) . S, the two-level inertia product in its order, no
+// pressure term and no Asave.  The cell's own values
+// are read once; per face only the other cell's and the face's.  The
+// diagonal and b stay in registers.  The off-diagonals and Abou come from
+// k_uvw_faces, except where several faces write one position (SHARED): then
+// each off-diagonal of row c is accumulated, in face order, on a zeroed Aval
+// by the thread of cell c.
+template <bool SHARED, bool SCALAR>
 __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, UvwGeom G, UvwParams P,
                                                        UvwFields F, double* __restrict__ aval,
                                                        double* __restrict__ b, double* __restrict__ abou,
@@ -410,19 +452,25 @@ __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, U
     const int t0 = blockIdx.x * kBlock + threadIdx.x;
     for (int q = t0; q <= nb; q += stride) rhoui[q] = F.rho_n[q] * F.ui_n[q];       // boundary cells and 0
     const bool diff = P.diffus == kFlagCN || P.diffus == kFlagFI;
-    for (int i = t0; i < M.nc; i += stride) {
+    // 256-cell tiles; each XCD walks one contiguous eighth of them
+    const TileSpan sp = block_span_strided((M.nc + kBlock - 1) / kBlock);
+    for (int t = sp.first; t < sp.end; t += sp.step) {
+        const int i = t * kBlock + threadIdx.x;
+        if (i >= M.nc) continue;
         const int c = i + 1, ic = nb + c;
-        const UvwCell own = uvw_cell(U, F, ic);
+        constexpr int NG = SCALAR ? 3 : 7;
+        const UvwCell own = uvw_cell<NG>(U, F, ic);
         rhoui[ic] = own.ru;
         const int e0 = M.cptr[i], e1 = M.cptr[i + 1];
         double Cs = 0.0, Xc = 0.0, Xd = 0.0, diag = 0.0;
         for (int e = e0; e < e1; ++e) {
-            const int code = M.cface[e];
-            const int s = code >> 1;
-            const bool second = code & 1;                                            // the cell is c2
-            const int other = second ? M.c1[s] : M.c2[s];
-            const UvwCell oth = uvw_cell(U, F, nb + other);
-            const UvwFace f = uvw_face(M, U, G, P, F, s, pick(!second, own, oth), pick(!second, oth, own));
+            const int2 ie = U.inc[e];
+            const int s = ie.x >> 1;
+            const bool second = ie.x & 1;                                            // the cell is c2
+            const int other = ie.y;
+            const UvwCell oth = uvw_cell<NG>(U, F, nb + other);
+            const UvwFace f =
+                uvw_face<SCALAR>(M, U, G, P, F, s, pick<NG>(!second, own, oth), pick<NG>(!second, oth, own));
             if (!second) {
                 Cs = Cs - f.conv;
                 if (f.upwind) Xc = Xc - f.up;
@@ -436,19 +484,21 @@ __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, U
             const unsigned char k = U.ukind[s];
             if (second) {                                                            // :366-367
                 diag = diag + f.a12 + f.d12;
-                const int pos = M.sid2[s];
-                aval[pos] = (SHARED ? aval[pos] : 0.0) - f.a21 - f.d21;
+                if (SHARED) {
+                    const int pos = M.sid2[s];
+                    aval[pos] = aval[pos] - f.a21 - f.d21;
+                }
             } else if (k == kUvwInterior) {                                          // :364-365, :367
                 diag = diag + f.a21 + f.d21;
-                const int pos = M.sid1[s];
-                aval[pos] = (SHARED ? aval[pos] : 0.0) - f.a12 - f.d12;
-            } else if (k == kUvwWallLike) {
+                if (SHARED) {
+                    const int pos = M.sid1[s];
+                    aval[pos] = aval[pos] - f.a12 - f.d12;
+                }
+            } else if (k == kUvwWallLike || k == kUvwBuffer) {                       // :391, :405
                 diag = diag + f.a21 + f.d21;
+                if (SHARED && k == kUvwBuffer) abou[nb + other] = -f.a12 - f.d12;   // Abou(-NbC:-1)
             } else if (k == kUvwOutflow) {
                 diag = diag + f.a21;
-            } else if (k == kUvwBuffer) {
-                diag = diag + f.a21 + f.d21;
-                abou[nb + other] = -f.a12 - f.d12;                                   // Abou(-NbC:-1)
             }
         }
         double bb = 0.0;
@@ -457,12 +507,12 @@ __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, U
             // b(c1) of WALL-like and OUTFLOW faces (A12 and D12 of :356-359),
             // after the convection term
             for (int e = e0; e < e1; ++e) {
-                const int code = M.cface[e];
-                if (code & 1) continue;
-                const int s = code >> 1;
+                const int2 ie = U.inc[e];
+                if (ie.x & 1) continue;
+                const int s = ie.x >> 1;
                 const unsigned char k = U.ukind[s];
                 if (k != kUvwWallLike && k != kUvwOutflow) continue;
-                const int i2 = nb + M.c2[s];
+                const int i2 = nb + ie.y;
                 const double rho2 = F.rho_n[i2], ru2 = rho2 * F.ui_n[i2];
                 const double Fu = F.flux_u[s];
                 const int m2 = U.material[i2] - 1;
@@ -480,19 +530,34 @@ __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, U
         const double V = M.volume[ic];
         if (P.inert == kInertLin) {                                                  // :447-450
             diag = diag + V / P.dt;
-            bb = bb + V / P.dt * F.rho_o[ic] * F.ui_o[ic] + F.source[ic] * V;
+            if (SCALAR)                                                              // CalcZmix.f90:371
+                bb = bb + V / P.dt * F.ui_o[ic] * F.rho_o[ic] + F.source[ic] * V;
+            else
+                bb = bb + V / P.dt * F.rho_o[ic] * F.ui_o[ic] + F.source[ic] * V;
         }
         if (P.inert == kInertPar) {                                                  // :453-456
             diag = diag + 1.5 * V / P.dt;
             bb = bb + 2.0 * V / P.dt * F.rho_o[ic] * F.ui_o[ic] - 0.5 * V / P.dt * F.rho_oo[ic] * F.ui_oo[ic] +
                  F.source[ic] * V;
         }
-        bb = bb - F.pi[ic] * V;                                                      // :484
-        if (P.var == 1 && asave) asave[ic] = diag;                                   // :497
+        if (!SCALAR) bb = bb - F.pi[ic] * V;                                         // :484
+        if (!SCALAR && P.var == 1 && asave) asave[ic] = diag;                        // :497
         bb = bb + diag * (1.0 - P.urf) * own.ru / P.urf;                             // :498
         aval[M.adia[i]] = diag / P.urf;                                              // :499
         b[i] = bb;
     }
+}
+
+// Zmix%n(1:NC) = rhoZ / rho%n, limited to [0, 1] (CalcZmix.f90:401-405)
+__global__ __launch_bounds__(kBlock) void k_zmix_update(int nc, int nb, const double* __restrict__ rhoz,
+                                                         const double* __restrict__ rho_n,
+                                                         double* __restrict__ z_n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nc) return;
+    double z = rhoz[nb + i + 1] / rho_n[nb + i + 1];
+    z = z < 0.0 ? 0.0 : z;
+    z = z > 1.0 ? 1.0 : z;
+    z_n[nb + i + 1] = z;
 }
 
 // Ui%n(1:NC) = rhoUi(1:NC) / rho%n(1:NC) (:526)
@@ -603,6 +668,14 @@ CalcPS::CalcPS(int nc, int nbc, int ns, int nonz, const int* sidec, const int* s
     M_.adia = upload(owned_, ad0.data(), ad0.size(), false, s);
     M_.cptr = upload(owned_, cptr.data(), cptr.size(), false, s);
     M_.cface = upload(owned_, cface.data(), cface.size(), false, s);
+    {
+        std::vector<int2> inc(cface.size());
+        for (size_t e = 0; e < cface.size(); ++e) {
+            const int q = cface[e] >> 1;
+            inc[e] = make_int2(cface[e], (cface[e] & 1) ? c1[q] : c2[q]);
+        }
+        U_.inc = upload(owned_, inc.data(), inc.size(), false, s);
+    }
     zero_idx_ = upload(owned_, zeros.data(), zeros.size(), false, s);
     const double** g[7] = {&M_.sx, &M_.sy, &M_.sz, &M_.dx, &M_.dy, &M_.dz, &M_.f};
     for (int q = 0; q < 7; ++q) *g[q] = upload(owned_, geom[q], (size_t)ns, device_ptrs, s);
@@ -722,8 +795,13 @@ void CalcPS::uvw_setup(const double* scoef, const double* ff, const int* materia
 
 void CalcPS::uvw_assemble(const UvwParams& P, const UvwFields& F, double* aval, double* b, double* abou,
                           double* asave, double* rhoui, hipStream_t s) {
-    if (!uvw_ready()) throw std::invalid_argument("the momentum geometry is not set (uvw_setup)");
     if (P.var < 1 || P.var > 3) throw std::invalid_argument("var must be 1, 2 or 3");
+    assemble_equation(P, F, aval, b, abou, asave, rhoui, false, s);
+}
+
+void CalcPS::assemble_equation(const UvwParams& P, const UvwFields& F, double* aval, double* b, double* abou,
+                               double* asave, double* rhoui, bool scalar, hipStream_t s) {
+    if (!uvw_ready()) throw std::invalid_argument("the momentum geometry is not set (uvw_setup)");
     if (!(P.dt != 0.0) && (P.inert == kInertLin || P.inert == kInertPar))
         throw std::invalid_argument("dt must be non-zero");
     if (!(P.urf != 0.0)) throw std::invalid_argument("URF must be non-zero");
@@ -735,8 +813,10 @@ void CalcPS::uvw_assemble(const UvwParams& P, const UvwFields& F, double* aval, 
             if (pairs_[a] >> c & 1u) {
                 const int ba = P.blend[a], bc = P.blend[c];
                 if ((ba != kBlendNo || bc != kBlendNo) && std::max(ba, bc) != kBlendYes)
-                    throw std::invalid_argument("BLEND other than YES/NO (a ConvScheme face scheme) is not "
-                                                "covered by the momentum assembly");
+                    throw std::invalid_argument(scalar ? "BLEND_TEM other than YES/NO (a ConvScheme face scheme) "
+                                                         "is not covered by the scalar assembly"
+                                                       : "BLEND other than YES/NO (a ConvScheme face scheme) is not "
+                                                         "covered by the momentum assembly");
             }
     const UvwGeom G = P.var == 1 ? UvwGeom{M_.sx, M_.sy, M_.sz, M_.dx, M_.dy, M_.dz}
                     : P.var == 2 ? UvwGeom{M_.sy, M_.sx, M_.sz, M_.dy, M_.dx, M_.dz}
@@ -752,13 +832,44 @@ void CalcPS::uvw_assemble(const UvwParams& P, const UvwFields& F, double* aval, 
     }
     if (nbc_ > 0)                                                      // :146
         hipLaunchKernelGGL(k_calcps_zero_all, dim3((nbc_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nbc_, abou);
-    const int grid = (std::max(nc_, nbc_ + 1) + kBlock - 1) / kBlock;
-    if (offdiag_shared_)
-        hipLaunchKernelGGL(k_uvw_cells<true>, dim3(grid), dim3(kBlock), 0, s, M_, U_, G, P, F, aval, b, abou, asave,
-                           rhoui);
-    else
-        hipLaunchKernelGGL(k_uvw_cells<false>, dim3(grid), dim3(kBlock), 0, s, M_, U_, G, P, F, aval, b, abou, asave,
-                           rhoui);
+    // at most 2048 blocks, a multiple of 8 (the XCD-aware tile walk)
+    const int tiles = (nc_ + kBlock - 1) / kBlock;
+    const int grid = tiles < kNumXcd ? tiles : std::min(2048, (tiles + kNumXcd - 1) / kNumXcd * kNumXcd);
+    if (offdiag_shared_) {
+        if (scalar)
+            hipLaunchKernelGGL((k_uvw_cells<true, true>), dim3(grid), dim3(kBlock), 0, s, M_, U_, G, P, F, aval, b,
+                               abou, asave, rhoui);
+        else
+            hipLaunchKernelGGL((k_uvw_cells<true, false>), dim3(grid), dim3(kBlock), 0, s, M_, U_, G, P, F, aval, b,
+                               abou, asave, rhoui);
+    } else {
+        if (diff && ns_ > 0)
+            hipLaunchKernelGGL(k_uvw_faces, dim3((ns_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, M_, U_, P, F, aval,
+                               abou);
+        if (scalar)
+            hipLaunchKernelGGL((k_uvw_cells<false, true>), dim3(grid), dim3(kBlock), 0, s, M_, U_, G, P, F, aval, b,
+                               abou, asave, rhoui);
+        else
+            hipLaunchKernelGGL((k_uvw_cells<false, false>), dim3(grid), dim3(kBlock), 0, s, M_, U_, G, P, F, aval, b,
+                               abou, asave, rhoui);
+    }
+    TF_CHECK(hipGetLastError());
+}
+
+void CalcPS::zmix_assemble(const UvwParams& Pin, const UvwFields& F, double* aval, double* b, double* abou,
+                           double* rhoz, hipStream_t s) {
+    // BLEND_TEM drives the face scheme, the upwind matrix and the upwind
+    // part (CalcZmix.f90:161-164, :176, :278); the gradients and S, D are
+    // taken in x, y, z order; no pressure term, no Asave
+    UvwParams P = Pin;
+    P.var = 1;
+    for (int m = 0; m < kMaxMaterials; ++m) P.blend[m] = P.blend_tem[m];
+    assemble_equation(P, F, aval, b, abou, nullptr, rhoz, true, s);
+}
+
+void CalcPS::zmix_update(const double* rhoz, const double* rho_n, double* z_n, hipStream_t s) {
+    hipLaunchKernelGGL(k_zmix_update, dim3((nc_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nc_, nbc_, rhoz, rho_n,
+                       z_n);
     TF_CHECK(hipGetLastError());
 }
 

@@ -587,3 +587,84 @@ int tflows_calcps_uvw_update(tflows_calcps h, const double* rhoui, const double*
         TF_CHECK(hipStreamSynchronize(st));
     });
 }
+
+// ---------------------------------------------------------------------------
+// mixture-fraction assembly
+// ---------------------------------------------------------------------------
+int tflows_calcps_zmix_assemble(tflows_calcps h, const tflows_uvw_params* p, const double* rho_n,
+                                const double* rho_o, const double* rho_oo, const double* z_n, const double* z_o,
+                                const double* z_oo, const double* source, const double* dZdx, const double* dZdy,
+                                const double* dZdz, const double* flux_u, double* aval, double* b, double* abou,
+                                double* rhoz, int32_t mem, void* stream) {
+    return guarded([&] {
+        if (!h || !p) throw std::invalid_argument("null handle or parameters");
+        const double* in[10] = {rho_n, rho_o, rho_oo, z_n, z_o, z_oo, source, dZdx, dZdy, dZdz};
+        for (const double* q : in)
+            if (!q) throw std::invalid_argument("null field");
+        tflows::CalcPS& C = *h->cps;
+        if (!aval || !b || !rhoz || (C.ns() > 0 && !flux_u) || (C.nbc() > 0 && !abou))
+            throw std::invalid_argument("null array");
+        tflows::UvwParams P;
+        P.convec = p->convec;
+        P.diffus = p->diffus;
+        P.cross = p->cross;
+        P.inert = p->inert;
+        P.visc = p->visc;
+        P.dt = p->dt;
+        P.urf = p->urf;
+        for (int m = 0; m < tflows::kMaxMaterials; ++m) {
+            P.blend_tem[m] = p->blend_tem[m];
+            P.urfc[m] = p->urfc[m];
+        }
+        const hipStream_t st = as_stream(stream);
+        const size_t ncell = (size_t)C.nbc() + C.nc() + 1;
+        if (mem == TFLOWS_MEM_DEVICE) {
+            const tflows::UvwFields F{rho_n, rho_o, rho_oo, z_n, z_o, z_oo, source, nullptr,
+                                      {dZdx, dZdy, dZdz, nullptr, nullptr, nullptr, nullptr}, flux_u};
+            C.zmix_assemble(P, F, aval, b, abou, rhoz, st);
+            return;
+        }
+        const size_t total = 11 * ncell + (size_t)C.ns() + C.nonz() + C.nc() + C.nbc();
+        double* d = C.staging(total);
+        const double* dv[10];
+        for (int q = 0; q < 10; ++q) {
+            TF_CHECK(hipMemcpyAsync(d + q * ncell, in[q], sizeof(double) * ncell, hipMemcpyHostToDevice, st));
+            dv[q] = d + q * ncell;
+        }
+        double* drhoz = d + 10 * ncell;
+        double* dflux = drhoz + ncell;
+        double* daval = dflux + C.ns();
+        double* db = daval + C.nonz();
+        double* dabou = db + C.nc();
+        if (C.ns() > 0) TF_CHECK(hipMemcpyAsync(dflux, flux_u, sizeof(double) * C.ns(), hipMemcpyHostToDevice, st));
+        const tflows::UvwFields F{dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], nullptr,
+                                  {dv[7], dv[8], dv[9], nullptr, nullptr, nullptr, nullptr}, dflux};
+        C.zmix_assemble(P, F, daval, db, dabou, drhoz, st);
+        TF_CHECK(hipMemcpyAsync(aval, daval, sizeof(double) * C.nonz(), hipMemcpyDeviceToHost, st));
+        TF_CHECK(hipMemcpyAsync(b, db, sizeof(double) * C.nc(), hipMemcpyDeviceToHost, st));
+        TF_CHECK(hipMemcpyAsync(rhoz, drhoz, sizeof(double) * ncell, hipMemcpyDeviceToHost, st));
+        if (C.nbc() > 0) TF_CHECK(hipMemcpyAsync(abou, dabou, sizeof(double) * C.nbc(), hipMemcpyDeviceToHost, st));
+        TF_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+int tflows_calcps_zmix_update(tflows_calcps h, const double* rhoz, const double* rho_n, double* z_n, int32_t mem,
+                              void* stream) {
+    return guarded([&] {
+        if (!h || !rhoz || !rho_n || !z_n) throw std::invalid_argument("null argument");
+        tflows::CalcPS& C = *h->cps;
+        const hipStream_t st = as_stream(stream);
+        if (mem == TFLOWS_MEM_DEVICE) {
+            C.zmix_update(rhoz, rho_n, z_n, st);
+            return;
+        }
+        const size_t ncell = (size_t)C.nbc() + C.nc() + 1;
+        double* d = C.staging(3 * ncell);
+        TF_CHECK(hipMemcpyAsync(d, rhoz, sizeof(double) * ncell, hipMemcpyHostToDevice, st));
+        TF_CHECK(hipMemcpyAsync(d + ncell, rho_n, sizeof(double) * ncell, hipMemcpyHostToDevice, st));
+        TF_CHECK(hipMemcpyAsync(d + 2 * ncell, z_n, sizeof(double) * ncell, hipMemcpyHostToDevice, st));
+        C.zmix_update(d, d + ncell, d + 2 * ncell, st);
+        TF_CHECK(hipMemcpyAsync(z_n, d + 2 * ncell, sizeof(double) * ncell, hipMemcpyDeviceToHost, st));
+        TF_CHECK(hipStreamSynchronize(st));
+    });
+}

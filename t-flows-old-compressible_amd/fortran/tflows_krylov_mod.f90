@@ -22,6 +22,16 @@ module tflows_krylov
   integer(c_int32_t), parameter :: TFLOWS_METHOD_CG = 0, TFLOWS_METHOD_BICG = 1, TFLOWS_METHOD_CGS = 2
   integer(c_int32_t), parameter :: TFLOWS_MEM_HOST = 0, TFLOWS_MEM_DEVICE = 1
 
+  ! flags and constants of one momentum / mixture-fraction assembly
+  ! (tflows_uvw_params); codes as Process/IniPar.f90:22-32, per-material
+  ! arrays for materials 1..8
+  type, bind(C) :: tflows_uvw_params
+    integer(c_int32_t) :: var, convec, diffus, cross, inert
+    real(c_double)     :: visc, dt, urf
+    integer(c_int32_t) :: blend(8), blend_tem(8)
+    real(c_double)     :: urfc(8)
+  end type tflows_uvw_params
+
   interface
     !--- drop-ins (host arrays, reference layout) -----------------------------
     subroutine tflows_cg(n, nb, nonz, a, acol, arow, adia, ab, x, r1, prec, niter, tol, inires, finres) &
@@ -135,5 +145,65 @@ module tflows_krylov
       real(c_double)            :: errmax, mean
       integer(c_int32_t)        :: rc
     end function tflows_calcps_assemble
+
+    !--- the correction after the pressure solve (CorUVW.f90:36-121) -------
+    function tflows_calcps_correct(h, dt, pp, aval, abou, asave, rho_n, rho_o, rho_oo, px, py, pz, u, v, w, &
+                                   flux, b, errmax, mem, stream) result(rc) bind(C, name="tflows_calcps_correct")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value        :: h, pp, aval, abou, asave, rho_n, rho_o, rho_oo, px, py, pz, u, v, w, flux, b, &
+                                   stream
+      real(c_double), value     :: dt
+      real(c_double)            :: errmax
+      integer(c_int32_t), value :: mem
+      integer(c_int32_t)        :: rc
+    end function tflows_calcps_correct
+
+    !--- momentum assembly (NewUVW.f90:137-499) -----------------------------
+    function tflows_calcps_uvw_setup(h, scoef, ff, material, nmaterials, mem, stream) result(rc) &
+        bind(C, name="tflows_calcps_uvw_setup")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value        :: h, scoef, ff, material, stream
+      integer(c_int32_t), value :: nmaterials, mem
+      integer(c_int32_t)        :: rc
+    end function tflows_calcps_uvw_setup
+
+    function tflows_calcps_uvw_assemble(h, p, rho_n, rho_o, rho_oo, ui_n, ui_o, ui_oo, source, pi, dUidi, &
+                                        dUidj, dUidk, dUjdi, dUkdi, dUjdj, dUkdk, flux_u, aval, b, abou, asave, &
+                                        rhoui, mem, stream) result(rc) bind(C, name="tflows_calcps_uvw_assemble")
+      import :: c_int32_t, c_ptr, tflows_uvw_params
+      type(c_ptr), value          :: h, rho_n, rho_o, rho_oo, ui_n, ui_o, ui_oo, source, pi, dUidi, dUidj, dUidk, &
+                                     dUjdi, dUkdi, dUjdj, dUkdk, flux_u, aval, b, abou, asave, rhoui, stream
+      type(tflows_uvw_params)     :: p
+      integer(c_int32_t), value   :: mem
+      integer(c_int32_t)          :: rc
+    end function tflows_calcps_uvw_assemble
+
+    function tflows_calcps_uvw_update(h, rhoui, rho_n, ui_n, mem, stream) result(rc) &
+        bind(C, name="tflows_calcps_uvw_update")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value        :: h, rhoui, rho_n, ui_n, stream
+      integer(c_int32_t), value :: mem
+      integer(c_int32_t)        :: rc
+    end function tflows_calcps_uvw_update
+
+    !--- mixture-fraction assembly (CalcZmix.f90:100-405) -------------------
+    function tflows_calcps_zmix_assemble(h, p, rho_n, rho_o, rho_oo, z_n, z_o, z_oo, source, dZdx, dZdy, dZdz, &
+                                         flux_u, aval, b, abou, rhoz, mem, stream) result(rc) &
+        bind(C, name="tflows_calcps_zmix_assemble")
+      import :: c_int32_t, c_ptr, tflows_uvw_params
+      type(c_ptr), value          :: h, rho_n, rho_o, rho_oo, z_n, z_o, z_oo, source, dZdx, dZdy, dZdz, flux_u, &
+                                     aval, b, abou, rhoz, stream
+      type(tflows_uvw_params)     :: p
+      integer(c_int32_t), value   :: mem
+      integer(c_int32_t)          :: rc
+    end function tflows_calcps_zmix_assemble
+
+    function tflows_calcps_zmix_update(h, rhoz, rho_n, z_n, mem, stream) result(rc) &
+        bind(C, name="tflows_calcps_zmix_update")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value        :: h, rhoz, rho_n, z_n, stream
+      integer(c_int32_t), value :: mem
+      integer(c_int32_t)        :: rc
+    end function tflows_calcps_zmix_update
   end interface
 end module tflows_krylov

@@ -155,6 +155,10 @@ def _bind_extensions(lib):
         lib.tflows_calcps_uvw_setup.argtypes = [vp, vp, vp, vp, C.c_int32, C.c_int32, vp]
         lib.tflows_calcps_uvw_assemble.restype = C.c_int
         lib.tflows_calcps_uvw_assemble.argtypes = [vp, C.POINTER(UvwParams)] + [vp] * 16 + [vp] * 5 + [C.c_int32, vp]
+        lib.tflows_calcps_zmix_assemble.restype = C.c_int
+        lib.tflows_calcps_zmix_assemble.argtypes = [vp, C.POINTER(UvwParams)] + [vp] * 11 + [vp] * 4 + [C.c_int32, vp]
+        lib.tflows_calcps_zmix_update.restype = C.c_int
+        lib.tflows_calcps_zmix_update.argtypes = [vp, vp, vp, vp, C.c_int32, vp]
         lib.tflows_calcps_uvw_update.restype = C.c_int
         lib.tflows_calcps_uvw_update.argtypes = [vp, vp, vp, vp, C.c_int32, vp]
         lib.tflows_calcps_correct.restype = C.c_int
@@ -628,6 +632,53 @@ class DeviceCalcPS:
             self.handle, C.byref(P), *(a.ctypes.data for a in cells), fl.ctypes.data, aval.ctypes.data,
             b.ctypes.data, abou.ctypes.data, asave.ctypes.data, rhoui.ctypes.data, 0, None))
         return dict(aval=aval, b=b, abou=abou[:c.nbc], asave=asave, rhoui=rhoui)
+
+    # -- mixture-fraction assembly (Process/CalcZmix.f90:100-387) --------
+    @staticmethod
+    def zmix_inputs(args):
+        """cuda float64 tensors of meshgen.zmix_args(...) in the order of
+        tflows_calcps_zmix_assemble: 10 cell arrays, then Flux_u."""
+        import torch
+        t64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()   # noqa: E731
+        cells = [args[k] for k in ("rho_n", "rho_o", "rho_oo", "z_n", "z_o", "z_oo", "source")]
+        return [t64(a) for a in cells + list(args["g"])] + [t64(args["flux_u"])]
+
+    def assemble_zmix(self, params, inputs, out=None, stream=None):
+        """The mixture-fraction system from cuda tensors ``inputs``
+        (zmix_inputs); returns dict of cuda tensors aval, b, abou, rhoz."""
+        import torch
+        c = self.case
+        if out is None:
+            dev = torch.device("cuda")
+            out = dict(aval=torch.empty(c.nonz, dtype=torch.float64, device=dev),
+                       b=torch.empty(c.nc, dtype=torch.float64, device=dev),
+                       abou=torch.zeros(max(c.nbc, 1), dtype=torch.float64, device=dev),
+                       rhoz=torch.empty(c.nbc + c.nc + 1, dtype=torch.float64, device=dev))
+        P = UvwParams.from_dict(params)
+        _check(self.lib.tflows_calcps_zmix_assemble(
+            self.handle, C.byref(P), *(_ptr(t) for t in inputs), _ptr(out["aval"]), _ptr(out["b"]),
+            _ptr(out["abou"]), _ptr(out["rhoz"]), 1, _stream(stream)))
+        return out
+
+    def assemble_zmix_host(self, params, args):
+        """The same from host arrays (meshgen.zmix_args), copied in and out."""
+        c = self.case
+        cells = [np.ascontiguousarray(args[k], np.float64)
+                 for k in ("rho_n", "rho_o", "rho_oo", "z_n", "z_o", "z_oo", "source")]
+        cells += [np.ascontiguousarray(a, np.float64) for a in args["g"]]
+        fl = np.ascontiguousarray(args["flux_u"], np.float64)
+        aval, b = np.empty(c.nonz), np.empty(c.nc)
+        abou, rhoz = np.zeros(max(c.nbc, 1)), np.empty(c.nbc + c.nc + 1)
+        P = UvwParams.from_dict(params)
+        _check(self.lib.tflows_calcps_zmix_assemble(
+            self.handle, C.byref(P), *(a.ctypes.data for a in cells), fl.ctypes.data, aval.ctypes.data,
+            b.ctypes.data, abou.ctypes.data, rhoz.ctypes.data, 0, None))
+        return dict(aval=aval, b=b, abou=abou[:c.nbc], rhoz=rhoz)
+
+    def zmix_update(self, rhoz, rho_n, z_n, stream=None):
+        """Zmix%n(1:NC) = rhoZ / rho%n limited to [0, 1] on cuda tensors."""
+        _check(self.lib.tflows_calcps_zmix_update(self.handle, _ptr(rhoz), _ptr(rho_n), _ptr(z_n), 1,
+                                                  _stream(stream)))
 
     def uvw_update(self, rhoui, rho_n, ui_n, stream=None):
         """Ui%n(1:NC) = rhoUi / rho%n on cuda tensors (NewUVW.f90:526)."""

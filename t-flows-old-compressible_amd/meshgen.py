@@ -1051,6 +1051,40 @@ def uvw_fields(case: CalcPSCase, seed: int = 5, nmaterials: int = 1, linear: np.
     return out
 
 
+def zmix_fields(case: CalcPSCase, seed: int = 6, nmaterials: int = 1, linear: np.ndarray | None = None,
+                zero_flux_types=("wall", "wallfl", "symmetry")) -> dict:
+    """The inputs of the mixture-fraction assembly (Process/CalcZmix.f90):
+    those of uvw_fields (Scoef, fF, material, Flux_u) and the scalar: z
+    (n-level, in [0, 1]), z_o, z_oo, its gradients zx, zy, zz and a source
+    src_z, cell arrays (-NbC:NC).  With ``linear`` (3 numbers g) z = 0.5 +
+    g . x and the gradients are the constants g."""
+    out = uvw_fields(case, seed=seed, nmaterials=nmaterials, zero_flux_types=zero_flux_types)
+    rng = np.random.default_rng(seed + 1000)
+    ntot = case.nbc + case.nc + 1
+    xc = case.xc
+    if linear is not None:
+        g = np.asarray(linear, float)
+        out["z"] = 0.5 + xc @ g
+        for k, gk in zip(("zx", "zy", "zz"), g):
+            out[k] = np.full(ntot, gk)
+    else:
+        ph = rng.uniform(0, 2 * np.pi, 3)
+        out["z"] = 0.5 + 0.4 * np.sin(xc[:, 0] + ph[0]) * np.cos(0.7 * xc[:, 1] + ph[1])
+        for k in ("zx", "zy", "zz"):
+            out[k] = 0.2 * rng.standard_normal(ntot)
+    out["z_o"] = np.clip(out["z"] + 0.01 * rng.standard_normal(ntot), 0.0, 1.0)
+    out["z_oo"] = np.clip(out["z"] + 0.01 * rng.standard_normal(ntot), 0.0, 1.0)
+    out["src_z"] = 0.05 * rng.standard_normal(ntot)
+    return out
+
+
+def zmix_args(case: CalcPSCase, zf: dict) -> dict:
+    """The cell and face arrays of the mixture-fraction assembly."""
+    return {"rho_n": case.rho_n, "rho_o": case.rho_o, "rho_oo": case.rho_oo, "z_n": zf["z"], "z_o": zf["z_o"],
+            "z_oo": zf["z_oo"], "source": zf["src_z"], "g": [zf["zx"], zf["zy"], zf["zz"]],
+            "flux_u": zf["flux_u"]}
+
+
 def uvw_params(nmaterials: int = 1, **kw) -> dict:
     """Flags and constants of one newuvw call (defaults: fully implicit
     convection, diffusion and cross diffusion, two-level inertia, central
