@@ -346,6 +346,37 @@ int tflows_calcps_zmix_assemble(tflows_calcps h, const tflows_uvw_params* p, con
 int tflows_calcps_zmix_update(tflows_calcps h, const double* rhoz, const double* rho_n, double* z_n, int32_t mem,
                               void* stream);
 
+/* T-FlowS mesh files (host side; no GPU needed).
+ * tflows_mesh_load reads NAME.cns as Process/CnsLoa.f90:38-62 does and, when
+ * geo_path is not NULL, NAME.geo as Library/GeoLoa.f90:22-49 does (Fortran
+ * sequential unformatted records), then builds the system pattern as
+ * Process/TopolM.f90 does and the centre distances D and Scoef as
+ * Process/Calc3.f90:9-30 does.
+ * tflows_mesh_int_array / tflows_mesh_real_array copy one array out by name;
+ * count must equal its length.  Integer arrays: "material", "bcmark",
+ * "copyc" (NbC + NC + 1, cell c at [NbC + c]), "sidec" (3 NS, SideC(0:2,NS)
+ * column-major), "acol" (NC + 1), "arow" (NONZERO), "adia" (NC), "sidaij"
+ * (2 NS, SidAij(1:2,NS) column-major, 0 on boundary faces).  Real arrays:
+ * "xc", "yc", "zc", "volume", "delta", "wallds" (NbC + NC + 1; volume, delta,
+ * WallDs zero outside 1..NC), "sx", "sy", "sz", "f", "xsp", "ysp", "zsp",
+ * "dx_file", "dy_file", "dz_file" (the file's Dx: periodic shifts), "dx",
+ * "dy", "dz" (Calc3's centre distances), "scoef" (NS).
+ * tflows_mesh_typebc: TypeBC(-NbC:NC) from bcmark and the boundary-type
+ * table typBou(1:nb) of the .b file (Process/BouLoa.f90:167-175, 292-294).
+ * tflows_mesh_ff: fF(NS) as Calc3.f90:53-71, with StateMat(1:Nmat) of the
+ * .b file (FLUID 7, SOLID 8; NULL: every material fluid).
+ * The arrays feed tflows_system_create (acol, arow, adia) and
+ * tflows_calcps_create / tflows_calcps_uvw_setup directly. */
+typedef struct tflows_mesh_s* tflows_mesh;
+int tflows_mesh_load(const char* cns_path, const char* geo_path, tflows_mesh* out);
+int tflows_mesh_destroy(tflows_mesh m);
+int tflows_mesh_info(tflows_mesh m, int32_t* nc, int32_t* nbc, int32_t* ns, int32_t* nmat, int32_t* nonz,
+                     int32_t* has_geo);
+int tflows_mesh_int_array(tflows_mesh m, const char* name, int32_t* dst, int64_t count);
+int tflows_mesh_real_array(tflows_mesh m, const char* name, double* dst, int64_t count);
+int tflows_mesh_typebc(tflows_mesh m, const int32_t* typbou, int32_t nb, int32_t* typebc, int64_t count);
+int tflows_mesh_ff(tflows_mesh m, const int32_t* typebc, const int32_t* statemat, double* ff, int64_t count);
+
 #ifdef __cplusplus
 }
 #endif

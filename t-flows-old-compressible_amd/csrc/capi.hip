@@ -95,6 +95,12 @@ void drop_in(int method, const int32_t* N, const int32_t* NB, const int32_t* NON
 
 }  // namespace
 
+namespace tflows {
+// errors of host-side entry points compiled elsewhere (meshio.cpp)
+int set_last_error(const std::string& msg) { return fail(msg); }
+}  // namespace tflows
+
+
 extern "C" {
 
 const char* tflows_last_error(void) { return g_last_error.c_str(); }

@@ -1,0 +1,401 @@
+// meshio.cpp -- the T-FlowS mesh files on the host, and the system pattern.
+//
+//   NAME.cns  connectivity, read as Process/CnsLoa.f90:38-62 reads it
+//             (written by Library/GeoSav.f90:16-112): NC, NbC, NS, NSsh,
+//             Nmat; material(1:NC), material(-1:-NbC:-1); SideC(0,:),
+//             SideC(1,:), SideC(2,:); bcmark(-1:-NbC:-1); CopyC(-1:-NbC:-1)
+//   NAME.geo  geometry, read as Library/GeoLoa.f90:22-49: xc, yc, zc of the
+//             cells 1..NC and of the boundary cells -1..-NbC, volume, delta,
+//             WallDs (1..NC), Sx, Sy, Sz, Dx, Dy, Dz, f, xsp, ysp, zsp (NS)
+//
+// Both are Fortran sequential unformatted files: each record is framed by
+// its byte count as a 4-byte integer before and after it; a record longer
+// than 2^31 - 1 bytes is split into subrecords whose leading count is
+// negated on all but the last.  INTEGER is 4 bytes and REAL 8 bytes, the
+// kinds of the reference build (-fdefault-real-8); a record of 4-byte reals
+// or 8-byte integers (another build's kinds) is widened or narrowed by its
+// byte count.
+//
+// From the connectivity, the pattern of the system matrix as
+// Process/TopolM.f90 builds it: per row the diagonal and one entry per
+// interior face, columns sorted ascending, Adia the diagonal's position,
+// SidAij(1,s) / SidAij(2,s) the positions of A(c1,c2) / A(c2,c1) -- where
+// two faces join the same two cells, both take the last matching position
+// of the row, as TopolM's search does (:65-74).  And from the geometry,
+// what Process/Calc3.f90:9-49 derives for the assemblies: the centre
+// distance D = x(c2) + D_file - x(c1) (D_file carries the periodic shift),
+// Scoef = |S|^2 / (S . D), and fF (f; 1 or 0 on fluid-solid faces; 1 on
+// non-BUFFER boundary faces).  TypeBC comes from bcmark and the
+// boundary-type table of the .b file as Process/BouLoa.f90:167-175, 292-294
+// sets it.
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/tflows_krylov.h"
+
+namespace {
+
+constexpr int kBuffer = 11, kFluid = 7, kSolid = 8;   // Library/allp_mod.f90:23-28
+
+class RecordFile {
+public:
+    explicit RecordFile(const char* path) : path_(path ? path : "") {
+        if (!path) throw std::invalid_argument("null file name");
+        f_ = std::fopen(path, "rb");
+        if (!f_) throw std::runtime_error(std::string("cannot open ") + path);
+    }
+    ~RecordFile() {
+        if (f_) std::fclose(f_);
+    }
+    RecordFile(const RecordFile&) = delete;
+    RecordFile& operator=(const RecordFile&) = delete;
+
+    // The next record's bytes (subrecords joined).
+    std::vector<unsigned char> next() {
+        std::vector<unsigned char> out;
+        for (;;) {
+            int32_t head = 0, tail = 0;
+            if (std::fread(&head, 4, 1, f_) != 1) fail("end of file where a record was expected");
+            const bool more = head < 0;
+            const uint64_t len = more ? (uint64_t)(-(int64_t)head) : (uint64_t)head;
+            const size_t at = out.size();
+            out.resize(at + len);
+            if (len && std::fread(out.data() + at, 1, len, f_) != len) fail("record shorter than its count");
+            if (std::fread(&tail, 4, 1, f_) != 1) fail("record without a closing count");
+            if ((uint64_t)(tail < 0 ? -(int64_t)tail : tail) != len) fail("record counts disagree");
+            if (!more) return out;
+        }
+    }
+
+    std::vector<int32_t> ints(size_t count, const char* what) {
+        const std::vector<unsigned char> r = next();
+        std::vector<int32_t> v(count);
+        if (r.size() == 4 * count) {
+            if (count) std::memcpy(v.data(), r.data(), r.size());
+        } else if (r.size() == 8 * count) {
+            for (size_t i = 0; i < count; ++i) {
+                int64_t x;
+                std::memcpy(&x, r.data() + 8 * i, 8);
+                if (x < INT32_MIN || x > INT32_MAX) fail(std::string(what) + ": integer out of range");
+                v[i] = (int32_t)x;
+            }
+        } else {
+            fail(std::string(what) + ": record of " + std::to_string(r.size()) + " bytes for " +
+                 std::to_string(count) + " integers");
+        }
+        return v;
+    }
+
+    int32_t one_int(const char* what) { return ints(1, what)[0]; }
+
+    std::vector<double> reals(size_t count, const char* what) {
+        const std::vector<unsigned char> r = next();
+        std::vector<double> v(count);
+        if (r.size() == 8 * count) {
+            if (count) std::memcpy(v.data(), r.data(), r.size());
+        } else if (r.size() == 4 * count) {
+            for (size_t i = 0; i < count; ++i) {
+                float x;
+                std::memcpy(&x, r.data() + 4 * i, 4);
+                v[i] = x;
+            }
+        } else {
+            fail(std::string(what) + ": record of " + std::to_string(r.size()) + " bytes for " +
+                 std::to_string(count) + " reals");
+        }
+        return v;
+    }
+
+private:
+    [[noreturn]] void fail(const std::string& msg) { throw std::runtime_error(path_ + ": " + msg); }
+    std::string path_;
+    std::FILE* f_ = nullptr;
+};
+
+}  // namespace
+
+struct tflows_mesh_s {
+    int32_t nc = 0, nbc = 0, ns = 0, nsh = 0, nmat = 0, nonz = 0;
+    bool has_geo = false;
+    // (-NbC:NC) arrays at [NbC + c]
+    std::vector<int32_t> material, bcmark_cells, copyc_cells;
+    std::vector<int32_t> sidec;          // SideC(0:2,NS), column-major: [3 (s-1) + k]
+    std::vector<double> xc, yc, zc;      // (-NbC:NC)
+    std::vector<double> volume, delta, wallds;   // (-NbC:NC), cells 1..NC from the file
+    std::vector<double> sx, sy, sz, dxf, dyf, dzf, f, xsp, ysp, zsp;   // (NS), as in the file
+    std::vector<double> dx, dy, dz, scoef;                             // Calc3
+    // TopolM
+    std::vector<int32_t> acol, arow, adia, sidaij;   // 1-based; SidAij(1:2,NS) column-major
+};
+
+namespace {
+
+void load_cns(tflows_mesh_s& m, const char* path) {
+    RecordFile r(path);
+    m.nc = r.one_int("NC");
+    m.nbc = r.one_int("NbC");
+    m.ns = r.one_int("NS");
+    m.nsh = r.one_int("NSsh");
+    m.nmat = r.one_int("Nmat");
+    if (m.nc <= 0 || m.nbc < 0 || m.ns < 0) throw std::runtime_error(std::string(path) + ": bad sizes");
+    const size_t nall = (size_t)m.nbc + m.nc + 1;
+    m.material.assign(nall, 0);
+    const std::vector<int32_t> mi = r.ints(m.nc, "material(1:NC)");
+    for (int c = 1; c <= m.nc; ++c) m.material[m.nbc + c] = mi[c - 1];
+    const std::vector<int32_t> mb = r.ints(m.nbc, "material(-1:-NbC)");
+    for (int k = 0; k < m.nbc; ++k) m.material[m.nbc - 1 - k] = mb[k];        // c = -1 - k
+    m.sidec.assign(3 * (size_t)m.ns, 0);
+    for (int k = 0; k < 3; ++k) {
+        const std::vector<int32_t> v = r.ints(m.ns, "SideC");
+        for (int s = 0; s < m.ns; ++s) m.sidec[3 * (size_t)s + k] = v[s];
+    }
+    m.bcmark_cells.assign(nall, 0);
+    m.copyc_cells.assign(nall, 0);
+    const std::vector<int32_t> bm = r.ints(m.nbc, "bcmark");
+    const std::vector<int32_t> cc = r.ints(m.nbc, "CopyC");
+    for (int k = 0; k < m.nbc; ++k) {
+        m.bcmark_cells[m.nbc - 1 - k] = bm[k];
+        m.copyc_cells[m.nbc - 1 - k] = cc[k];
+    }
+    for (int s = 0; s < m.ns; ++s) {
+        const int c1 = m.sidec[3 * (size_t)s + 1], c2 = m.sidec[3 * (size_t)s + 2];
+        if (c1 < 1 || c1 > m.nc || c2 < -m.nbc || c2 > m.nc || c2 == 0)
+            throw std::runtime_error(std::string(path) + ": SideC out of range at face " + std::to_string(s + 1));
+    }
+}
+
+void load_geo(tflows_mesh_s& m, const char* path) {
+    RecordFile r(path);
+    const size_t nall = (size_t)m.nbc + m.nc + 1;
+    std::vector<double>* xyz[3] = {&m.xc, &m.yc, &m.zc};
+    for (auto* a : xyz) a->assign(nall, 0.0);
+    for (int k = 0; k < 3; ++k) {
+        const std::vector<double> v = r.reals(m.nc, "xc(1:NC)");
+        for (int c = 1; c <= m.nc; ++c) (*xyz[k])[m.nbc + c] = v[c - 1];
+    }
+    for (int k = 0; k < 3; ++k) {
+        const std::vector<double> v = r.reals(m.nbc, "xc(-1:-NbC)");
+        for (int q = 0; q < m.nbc; ++q) (*xyz[k])[m.nbc - 1 - q] = v[q];
+    }
+    std::vector<double>* cell[3] = {&m.volume, &m.delta, &m.wallds};
+    for (auto* a : cell) {
+        a->assign(nall, 0.0);
+        const std::vector<double> v = r.reals(m.nc, "volume/delta/WallDs");
+        for (int c = 1; c <= m.nc; ++c) (*a)[m.nbc + c] = v[c - 1];
+    }
+    std::vector<double>* face[10] = {&m.sx, &m.sy, &m.sz, &m.dxf, &m.dyf, &m.dzf, &m.f, &m.xsp, &m.ysp, &m.zsp};
+    for (auto* a : face) *a = r.reals(m.ns, "face array");
+    m.has_geo = true;
+}
+
+// Process/TopolM.f90:15-76
+void topolm(tflows_mesh_s& m) {
+    const int nc = m.nc, ns = m.ns;
+    std::vector<int32_t> stencw(nc + 1, 1);                            // 1-based
+    for (int s = 0; s < ns; ++s) {
+        const int c1 = m.sidec[3 * (size_t)s + 1], c2 = m.sidec[3 * (size_t)s + 2];
+        if (c2 > 0) {
+            stencw[c1]++;
+            stencw[c2]++;
+        }
+    }
+    int64_t n = 0;
+    for (int c = 1; c <= nc; ++c) n += stencw[c];
+    if (n + 1 > INT32_MAX) throw std::runtime_error("NONZERO exceeds 32 bits");
+    m.nonz = (int32_t)(n + 1);
+    m.arow.assign((size_t)n + 1, 0);
+    m.acol.assign((size_t)nc + 1, 0);
+    m.adia.assign(nc, 0);
+    m.acol[0] = 1;
+    for (int c = 1; c <= nc; ++c) {
+        m.acol[c] = m.acol[c - 1] + stencw[c];
+        m.arow[m.acol[c - 1] - 1] = c;                                 // the cell itself first
+        stencw[c] = 1;
+    }
+    for (int s = 0; s < ns; ++s) {
+        const int c1 = m.sidec[3 * (size_t)s + 1], c2 = m.sidec[3 * (size_t)s + 2];
+        if (c2 > 0) {
+            m.arow[m.acol[c1 - 1] - 1 + stencw[c1]] = c2;
+            m.arow[m.acol[c2 - 1] - 1 + stencw[c2]] = c1;
+            stencw[c1]++;
+            stencw[c2]++;
+        }
+    }
+    for (int c = 1; c <= nc; ++c) {
+        int32_t* row = m.arow.data() + m.acol[c - 1] - 1;
+        std::sort(row, row + stencw[c]);
+        for (int j = m.acol[c - 1]; j < m.acol[c]; ++j)
+            if (m.arow[j - 1] == c) m.adia[c - 1] = j;
+    }
+    m.sidaij.assign(2 * (size_t)ns, 0);
+    for (int s = 0; s < ns; ++s) {
+        const int c1 = m.sidec[3 * (size_t)s + 1], c2 = m.sidec[3 * (size_t)s + 2];
+        if (c2 <= 0) continue;
+        for (int j = m.acol[c1 - 1]; j < m.acol[c1]; ++j)
+            if (m.arow[j - 1] == c2) m.sidaij[2 * (size_t)s] = j;
+        for (int j = m.acol[c2 - 1]; j < m.acol[c2]; ++j)
+            if (m.arow[j - 1] == c1) m.sidaij[2 * (size_t)s + 1] = j;
+    }
+}
+
+// Process/Calc3.f90:9-30: the centre distance and Scoef
+void calc3(tflows_mesh_s& m) {
+    const int ns = m.ns, nb = m.nbc;
+    m.dx.resize(ns);
+    m.dy.resize(ns);
+    m.dz.resize(ns);
+    m.scoef.resize(ns);
+    for (int s = 0; s < ns; ++s) {
+        const int c1 = nb + m.sidec[3 * (size_t)s + 1], c2 = nb + m.sidec[3 * (size_t)s + 2];
+        const double sc = (m.sx[s] * m.sx[s] + m.sy[s] * m.sy[s]) + m.sz[s] * m.sz[s];
+        const double xc2 = m.xc[c2] + m.dxf[s], yc2 = m.yc[c2] + m.dyf[s], zc2 = m.zc[c2] + m.dzf[s];
+        m.dx[s] = xc2 - m.xc[c1];
+        m.dy[s] = yc2 - m.yc[c1];
+        m.dz[s] = zc2 - m.zc[c1];
+        m.scoef[s] = sc / ((m.dx[s] * m.sx[s] + m.dy[s] * m.sy[s]) + m.dz[s] * m.sz[s]);
+    }
+}
+
+template <class T>
+int copy_out(const std::vector<T>& v, T* dst, int64_t count) {
+    if (!dst) throw std::invalid_argument("null destination");
+    if (count != (int64_t)v.size())
+        throw std::invalid_argument("array holds " + std::to_string(v.size()) + " entries, not " +
+                                    std::to_string(count));
+    if (count) std::memcpy(dst, v.data(), sizeof(T) * (size_t)count);
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C API (errors through tflows_last_error, defined in capi.hip)
+// ---------------------------------------------------------------------------
+namespace tflows {
+int set_last_error(const std::string& msg);
+}
+
+template <class F>
+static int mesh_guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        return tflows::set_last_error(e.what());
+    }
+}
+
+extern "C" {
+
+int tflows_mesh_load(const char* cns_path, const char* geo_path, tflows_mesh* out) {
+    return mesh_guard([&] {
+        if (!out) throw std::invalid_argument("out is NULL");
+        auto m = std::make_unique<tflows_mesh_s>();
+        load_cns(*m, cns_path);
+        topolm(*m);
+        if (geo_path) {
+            load_geo(*m, geo_path);
+            calc3(*m);
+        }
+        *out = m.release();
+    });
+}
+
+int tflows_mesh_destroy(tflows_mesh m) {
+    return mesh_guard([&] { delete m; });
+}
+
+int tflows_mesh_info(tflows_mesh m, int32_t* nc, int32_t* nbc, int32_t* ns, int32_t* nmat, int32_t* nonz,
+                     int32_t* has_geo) {
+    return mesh_guard([&] {
+        if (!m) throw std::invalid_argument("null mesh");
+        if (nc) *nc = m->nc;
+        if (nbc) *nbc = m->nbc;
+        if (ns) *ns = m->ns;
+        if (nmat) *nmat = m->nmat;
+        if (nonz) *nonz = m->nonz;
+        if (has_geo) *has_geo = m->has_geo ? 1 : 0;
+    });
+}
+
+int tflows_mesh_int_array(tflows_mesh m, const char* name, int32_t* dst, int64_t count) {
+    return mesh_guard([&] {
+        if (!m || !name) throw std::invalid_argument("null mesh or name");
+        const std::string n(name);
+        if (n == "material") copy_out(m->material, dst, count);
+        else if (n == "bcmark") copy_out(m->bcmark_cells, dst, count);
+        else if (n == "copyc") copy_out(m->copyc_cells, dst, count);
+        else if (n == "sidec") copy_out(m->sidec, dst, count);
+        else if (n == "acol") copy_out(m->acol, dst, count);
+        else if (n == "arow") copy_out(m->arow, dst, count);
+        else if (n == "adia") copy_out(m->adia, dst, count);
+        else if (n == "sidaij") copy_out(m->sidaij, dst, count);
+        else throw std::invalid_argument("unknown integer array " + n);
+    });
+}
+
+int tflows_mesh_real_array(tflows_mesh m, const char* name, double* dst, int64_t count) {
+    return mesh_guard([&] {
+        if (!m || !name) throw std::invalid_argument("null mesh or name");
+        if (!m->has_geo) throw std::invalid_argument("the mesh was loaded without its .geo file");
+        const std::string n(name);
+        const std::pair<const char*, const std::vector<double>*> table[] = {
+            {"xc", &m->xc},         {"yc", &m->yc},         {"zc", &m->zc},         {"volume", &m->volume},
+            {"delta", &m->delta},   {"wallds", &m->wallds}, {"sx", &m->sx},         {"sy", &m->sy},
+            {"sz", &m->sz},         {"dx_file", &m->dxf},   {"dy_file", &m->dyf},   {"dz_file", &m->dzf},
+            {"f", &m->f},           {"xsp", &m->xsp},       {"ysp", &m->ysp},       {"zsp", &m->zsp},
+            {"dx", &m->dx},         {"dy", &m->dy},         {"dz", &m->dz},         {"scoef", &m->scoef}};
+        for (const auto& t : table)
+            if (n == t.first) {
+                copy_out(*t.second, dst, count);
+                return;
+            }
+        throw std::invalid_argument("unknown real array " + n);
+    });
+}
+
+int tflows_mesh_typebc(tflows_mesh m, const int32_t* typbou, int32_t nb, int32_t* typebc, int64_t count) {
+    return mesh_guard([&] {
+        if (!m || !typebc) throw std::invalid_argument("null argument");
+        if (nb < 0 || (nb > 0 && !typbou)) throw std::invalid_argument("bad boundary-type table");
+        const int64_t nall = (int64_t)m->nbc + m->nc + 1;
+        if (count != nall) throw std::invalid_argument("TypeBC holds NbC + NC + 1 entries");
+        std::fill(typebc, typebc + nall, 0);
+        for (int c = -1; c >= -m->nbc; --c) {                          // BouLoa.f90:167-175
+            const int mark = m->bcmark_cells[m->nbc + c];
+            if (mark >= 1 && mark <= nb) typebc[m->nbc + c] = typbou[mark - 1];
+        }
+        for (int c = -1; c >= -m->nbc; --c)                            // :292-294
+            if (m->bcmark_cells[m->nbc + c] == kBuffer) typebc[m->nbc + c] = kBuffer;
+    });
+}
+
+int tflows_mesh_ff(tflows_mesh m, const int32_t* typebc, const int32_t* statemat, double* ff, int64_t count) {
+    return mesh_guard([&] {
+        if (!m || !typebc || !ff) throw std::invalid_argument("null argument");
+        if (!m->has_geo) throw std::invalid_argument("the mesh was loaded without its .geo file");
+        if (count != m->ns) throw std::invalid_argument("fF holds NS entries");
+        auto state = [&](int c) {
+            const int mat = m->material[m->nbc + c];
+            if (!statemat) return kFluid;
+            if (mat < 1 || mat > m->nmat) throw std::invalid_argument("material out of range");
+            return statemat[mat - 1];
+        };
+        for (int s = 0; s < m->ns; ++s) {                              // Calc3.f90:53-71
+            const int c1 = m->sidec[3 * (size_t)s + 1], c2 = m->sidec[3 * (size_t)s + 2];
+            ff[s] = m->f[s];
+            const int s1 = state(c1), s2 = state(c2);
+            if (s1 == kFluid && s2 == kSolid) ff[s] = 1.0;
+            if (s1 == kSolid && s2 == kFluid) ff[s] = 0.0;
+            if (c2 < 0 && typebc[m->nbc + c2] != kBuffer) ff[s] = 1.0;
+        }
+    });
+}
+
+}  // extern "C"

@@ -143,6 +143,20 @@ def _bind_extensions(lib):
         lib.tflows_batch_solve_mixed.restype = C.c_int
         lib.tflows_batch_solve_mixed.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, C.c_double,
                                                  C.c_int32, vp, vp, vp, _ip, vp]
+    if hasattr(lib, "tflows_mesh_load"):
+        lib.tflows_mesh_load.restype = C.c_int
+        lib.tflows_mesh_load.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(vp)]
+        lib.tflows_mesh_destroy.restype = C.c_int
+        lib.tflows_mesh_destroy.argtypes = [vp]
+        lib.tflows_mesh_info.restype = C.c_int
+        lib.tflows_mesh_info.argtypes = [vp] + [C.POINTER(C.c_int32)] * 6
+        for name in ("tflows_mesh_int_array", "tflows_mesh_real_array"):
+            getattr(lib, name).restype = C.c_int
+            getattr(lib, name).argtypes = [vp, C.c_char_p, vp, C.c_int64]
+        lib.tflows_mesh_typebc.restype = C.c_int
+        lib.tflows_mesh_typebc.argtypes = [vp, vp, C.c_int32, vp, C.c_int64]
+        lib.tflows_mesh_ff.restype = C.c_int
+        lib.tflows_mesh_ff.argtypes = [vp, vp, vp, vp, C.c_int64]
     if hasattr(lib, "tflows_calcps_create"):
         lib.tflows_calcps_create.restype = C.c_int
         lib.tflows_calcps_create.argtypes = [C.c_int32] * 4 + [vp] * 12 + [C.c_int32, vp, C.POINTER(vp)]
@@ -503,6 +517,74 @@ class UvwParams(C.Structure):
             q.blend_tem[m] = int(p["blend_tem"][m])
             q.urfc[m] = float(p["urfc"][m])
         return q
+
+
+class Mesh:
+    """A T-FlowS mesh read from NAME.cns and NAME.geo by the library's reader
+    (csrc/meshio.cpp: Process/CnsLoa.f90, Library/GeoLoa.f90, with the
+    pattern of Process/TopolM.f90 and the centre distances of
+    Process/Calc3.f90).  Host only.  Arrays are numpy copies in the
+    reference's layout: cell arrays (-NbC:NC) at [NbC + c], ``sidec`` as
+    (3, NS) (SideC(0:2,NS)), ``sidaij`` as (2, NS), 1-based indices."""
+
+    INT = ("material", "bcmark", "copyc", "sidec", "acol", "arow", "adia", "sidaij")
+    REAL = ("xc", "yc", "zc", "volume", "delta", "wallds", "sx", "sy", "sz", "f", "xsp", "ysp", "zsp",
+            "dx_file", "dy_file", "dz_file", "dx", "dy", "dz", "scoef")
+
+    def __init__(self, cns_path, geo_path=None):
+        self.lib = load_library()
+        h = C.c_void_p()
+        _check(self.lib.tflows_mesh_load(os.fsencode(cns_path), os.fsencode(geo_path) if geo_path else None,
+                                         C.byref(h)))
+        self.handle = h
+        v = [C.c_int32(0) for _ in range(6)]
+        _check(self.lib.tflows_mesh_info(h, *(C.byref(x) for x in v)))
+        self.nc, self.nbc, self.ns, self.nmat, self.nonz, geo = (x.value for x in v)
+        self.has_geo = bool(geo)
+        ncell = self.nbc + self.nc + 1
+        sizes = {"material": ncell, "bcmark": ncell, "copyc": ncell, "sidec": 3 * self.ns, "acol": self.nc + 1,
+                 "arow": self.nonz, "adia": self.nc, "sidaij": 2 * self.ns}
+        for k in self.INT:
+            a = np.empty(sizes[k], np.int32)
+            _check(self.lib.tflows_mesh_int_array(h, k.encode(), a.ctypes.data, a.size))
+            setattr(self, k, a)
+        self.sidec = self.sidec.reshape(self.ns, 3).T.copy()
+        self.sidaij = self.sidaij.reshape(self.ns, 2).T.copy()
+        if self.has_geo:
+            for k in self.REAL:
+                n = ncell if k in ("xc", "yc", "zc", "volume", "delta", "wallds") else self.ns
+                a = np.empty(n, np.float64)
+                _check(self.lib.tflows_mesh_real_array(h, k.encode(), a.ctypes.data, a.size))
+                setattr(self, k, a)
+
+    def typebc(self, typbou):
+        """TypeBC(-NbC:NC) for the boundary types typBou(1:NB) of the .b
+        file (codes of meshgen.BC_CODES)."""
+        t = np.ascontiguousarray(typbou, np.int32)
+        out = np.empty(self.nbc + self.nc + 1, np.int32)
+        _check(self.lib.tflows_mesh_typebc(self.handle, t.ctypes.data, len(t), out.ctypes.data, out.size))
+        return out
+
+    def ff(self, typebc, statemat=None):
+        """fF(NS) (Calc3.f90:53-71); statemat: StateMat(1:Nmat), FLUID 7 /
+        SOLID 8 (None: all fluid)."""
+        t = np.ascontiguousarray(typebc, np.int32)
+        st = None if statemat is None else np.ascontiguousarray(statemat, np.int32)
+        out = np.empty(self.ns, np.float64)
+        _check(self.lib.tflows_mesh_ff(self.handle, t.ctypes.data, None if st is None else st.ctypes.data,
+                                       out.ctypes.data, out.size))
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None) is not None:
+            self.lib.tflows_mesh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceCalcPS:

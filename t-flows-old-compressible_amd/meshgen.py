@@ -950,6 +950,23 @@ def calcps_case(shape, seed: int = 7, ratio: float = 1.05, perturb: float = 0.15
     bpos[-c2[bnd] - 1] = xf[bnd]
     allc = np.concatenate([bpos[::-1], np.zeros((1, 3)), cen])
 
+    nonbuf_bc = np.zeros(ntot, bool)
+    nonbuf_bc[at(c2[bnd & ~is_buf])] = True
+    volume[nonbuf_bc] = 0.0
+    asave, rho_n, rho_o, rho_oo, u, v, w, p, px, py, pz = _smooth_fields(rng, allc, nbc, volume, nonbuf_bc, dt)
+    return CalcPSCase(tuple(shape), nc, nbc, ns, topo.nonz, nmat, dt, sidec, sidaij, topo.acol, topo.arow,
+                      topo.adia, typebc, np.ascontiguousarray(S[:, 0]), np.ascontiguousarray(S[:, 1]),
+                      np.ascontiguousarray(S[:, 2]), np.ascontiguousarray(d[:, 0]), np.ascontiguousarray(d[:, 1]),
+                      np.ascontiguousarray(d[:, 2]), f, volume, asave, rho_n, rho_o, rho_oo, u, v, w, p, px, py, pz,
+                      allc)
+
+
+def _smooth_fields(rng, allc, nbc, volume, nonbuf_bc, dt):
+    """Smooth seeded cell fields (-NbC:NC) at the positions ``allc``: Asave
+    (zero on non-BUFFER boundary cells), rho n/o/oo, u, v, w, p, px, py,
+    pz; cell 0 zero."""
+    ntot = len(allc)
+
     def smooth(mean, amp, k):
         g = np.full(ntot, float(mean))
         for _ in range(3):
@@ -967,18 +984,32 @@ def calcps_case(shape, seed: int = 7, ratio: float = 1.05, perturb: float = 0.15
     p = smooth(0.0, 0.5, 1.1)
     px, py, pz = smooth(0.0, 0.4, 1.0), smooth(0.0, 0.4, 1.2), smooth(0.0, 0.4, 0.9)
     asave = volume / dt * (1.5 + 0.2 * rng.uniform(0, 1, ntot))
-    nonbuf_bc = np.zeros(ntot, bool)
-    nonbuf_bc[at(c2[bnd & ~is_buf])] = True
     asave[nonbuf_bc] = 0.0
-    volume[nonbuf_bc] = 0.0
     asave[nbc] = 0.0                              # cell 0 is not a cell
     for a in (rho_n, rho_o, rho_oo, u, v, w, p, px, py, pz):
         a[nbc] = 0.0
-    return CalcPSCase(tuple(shape), nc, nbc, ns, topo.nonz, nmat, dt, sidec, sidaij, topo.acol, topo.arow,
-                      topo.adia, typebc, np.ascontiguousarray(S[:, 0]), np.ascontiguousarray(S[:, 1]),
-                      np.ascontiguousarray(S[:, 2]), np.ascontiguousarray(d[:, 0]), np.ascontiguousarray(d[:, 1]),
-                      np.ascontiguousarray(d[:, 2]), f, volume, asave, rho_n, rho_o, rho_oo, u, v, w, p, px, py, pz,
-                      allc)
+    return asave, rho_n, rho_o, rho_oo, u, v, w, p, px, py, pz
+
+
+def case_from_mesh(mesh, typbou, seed: int = 7, nmat: int = 1, dt: float = 1e-2) -> CalcPSCase:
+    """A CalcPSCase on a mesh read from T-FlowS files (krylov.Mesh): its
+    faces, pattern (TopolM), centre distances (Calc3), volumes and cell
+    positions, TypeBC from bcmark and the boundary types ``typbou`` of the
+    .b file, and smooth seeded fields as calcps_case makes them."""
+    rng = np.random.default_rng(seed)
+    nbc = mesh.nbc
+    typebc = mesh.typebc(typbou)
+    ntot = nbc + mesh.nc + 1
+    nonbuf_bc = np.zeros(ntot, bool)
+    nonbuf_bc[:nbc] = typebc[:nbc] != BC_CODES["buffer"]
+    volume = mesh.volume.copy()
+    volume[nonbuf_bc] = 0.0
+    allc = np.stack([mesh.xc, mesh.yc, mesh.zc], axis=1)
+    fields = _smooth_fields(rng, allc, nbc, volume, nonbuf_bc, dt)
+    return CalcPSCase(("mesh", mesh.nc), mesh.nc, nbc, mesh.ns, mesh.nonz, nmat, dt,
+                      mesh.sidec.astype(np.int32), mesh.sidaij.astype(np.int32), mesh.acol, mesh.arow, mesh.adia,
+                      typebc, mesh.sx.copy(), mesh.sy.copy(), mesh.sz.copy(), mesh.dx.copy(), mesh.dy.copy(),
+                      mesh.dz.copy(), mesh.f.copy(), volume, *fields, allc)
 
 
 # --------------------------------------------------------------------------

@@ -1,0 +1,102 @@
+"""GPU tests on meshes from T-FlowS files: the fixtures of
+tests/golden/meshes (written by the reference's mesh generator) read by the
+library's reader, their pattern (TopolM) and geometry (Calc3) handed to the
+GPU assemblies and solver.
+
+On every mesh -- a single block, a refined cavity with up to 12 neighbours
+per cell, periodic channels, two materials with a solid -- the pressure,
+momentum and mixture-fraction assemblies match the oracle bit for bit (the
+mean removal of the pressure assembly to the rounding of its sum), and the
+pressure system solved with the GPU CG matches the oracle's CG.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from test_mesh_reader import CASES, STATEMAT, TYPBOU, _mesh
+
+pytestmark = pytest.mark.gpu
+
+YES = 10
+
+
+# The cavity's .b file closes it (INFLOW, INFLOW, SYMMETRY): its pressure
+# system is singular, and with the volume-weighted mean removal (CalcPS.f90:
+# 201-204) on unequal volumes not consistent, so CG stalls there as the
+# reference's would.  The solve test opens mark 3 as an OUTFLOW.
+SOLVE_TYPBOU = dict(TYPBOU, cavity_small=[1, 1, 3])
+
+
+def _setup(pkg, name, typbou=None):
+    m = _mesh(pkg, name)
+    case = pkg.meshgen.case_from_mesh(m, (typbou or TYPBOU)[name], seed=5, nmat=1, dt=1e-2)
+    return m, case
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_pressure_assembly_and_solve(pkg, oracle, cuda, name):
+    import torch
+    m, case = _setup(pkg, name, SOLVE_TYPBOU)
+    print(f"mesh {name} nc {case.nc} ns {case.ns} nbc {case.nbc}", flush=True)
+    want = oracle.calcps(case)
+    dc = pkg.krylov.DeviceCalcPS(case)
+    got = dc.assemble()
+    assert np.array_equal(got["aval"].cpu().numpy(), want["aval"])
+    assert np.array_equal(got["flux"].cpu().numpy()[:case.ns], want["flux"])
+    assert np.array_equal(got["abou"].cpu().numpy()[:case.nbc], want["abou"])
+    assert got["errmax"] == want["errmax"]
+    b = got["b"].cpu().numpy()
+    assert np.array_equal(b, want["b_pre"] - got["mean"])
+    V = case.volume[case.nbc + 1:]
+    exact = math.fsum(want["b_pre"] * V) / math.fsum(V)
+    bound = 4 * case.nc * np.finfo(float).eps * np.abs(want["b_pre"] * V).sum() / V.sum()
+    assert abs(got["mean"] - exact) <= bound + 1e-300
+    # the pressure solve on the assembled values (CalcPS.f90:214-219)
+    mat = case.matrix(want["aval"])
+    tol = 1e-8 * np.linalg.norm(want["b"])
+    ref = oracle.run("cg", mat, want["b"], prec=1, niter=2000, tol=tol)
+    ds = pkg.krylov.DeviceSystem(mat)
+    ds.set_values(got["aval"])
+    x = torch.zeros(case.nc, dtype=torch.float64, device=cuda)
+    r = got["b"].clone()
+    it, ini, fin = ds.solve("cg", 1, x, r, 2000, tol)
+    torch.cuda.synchronize()
+    assert fin < tol and ref["fin_res"] < tol and abs(it - ref["niter"]) <= 2
+    A = mat.to_scipy()
+    xs = x.cpu().numpy()
+    assert np.linalg.norm(b - A @ xs) < 1.01 * tol
+    ds.close()
+    dc.close()
+    m.close()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_momentum_and_scalar_assembly(pkg, oracle, cuda, name):
+    """NewUVW and CalcZmix on the mesh with its materials, Scoef and fF
+    (Calc3, with the .b file's solid material where there is one)."""
+    mg = pkg.meshgen
+    m, case = _setup(pkg, name)
+    tb = case.typebc
+    uf = mg.uvw_fields(case, seed=6)
+    uf.update(scoef=m.scoef, ff=m.ff(tb, STATEMAT.get(name)), material=m.material, nmaterials=m.nmat)
+    zf = mg.zmix_fields(case, seed=7)
+    zf.update({k: uf[k] for k in ("scoef", "ff", "material", "nmaterials", "flux_u")})
+    nm = m.nmat
+    dc = pkg.krylov.DeviceCalcPS(case)
+    dc.uvw_setup(uf)
+    for var in (1, 2, 3):
+        p = mg.uvw_params(nmaterials=nm, var=var, blend=[YES] * nm, blend_tem=[YES] * nm,
+                          urfc=[1.0, 0.8][:nm], urf=0.7)
+        want = oracle.newuvw(case, uf, p)
+        got = dc.assemble_uvw(p, dc.uvw_inputs(mg.uvw_args(case, uf, var)))
+        for k in ("aval", "b", "rhoui"):
+            assert np.array_equal(got[k].cpu().numpy(), want[k]), (var, k)
+        assert np.array_equal(got["abou"].cpu().numpy()[:case.nbc], want["abou"])
+    pz = mg.uvw_params(nmaterials=nm, blend_tem=[YES] * nm, urf=0.8, visc=1e-3)
+    want = oracle.calczmix(case, zf, pz)
+    got = dc.assemble_zmix(pz, dc.zmix_inputs(mg.zmix_args(case, zf)))
+    for k in ("aval", "b", "rhoz"):
+        assert np.array_equal(got[k].cpu().numpy(), want[k]), k
+    dc.close()
+    m.close()
