@@ -43,17 +43,24 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def spmv_bytes(n, nnz, word, sliced=False):
+def spmv_bytes(n, nnz, word, sliced=False, col_bytes=4):
     """Algorithmic HBM bytes of one SpMV launch, every array touched once.
     FP64 (reference CSR form): 12*nnz + 4*(N+1) + 16*N.
-    FP32 (difference form used by the mixed solver): off-diagonal col+val
-    8*(nnz-N), row sums, x and y 4*N each, and the row index: the CSR row
+    FP32 (difference form used by the mixed solver): off-diagonal values
+    4*(nnz-N) and columns col_bytes*(nnz-N) (4, or 2 for the 16-bit column
+    offsets), row sums, x and y 4*N each, and the row index: the CSR row
     pointer 4*(N+1), or for the sliced ELL layout its slice offsets
     4*(ceil(N/64)+1); the slices' padding is not counted."""
     if word == 8:
         return 12 * nnz + 4 * (n + 1) + 16 * n
     index = 4 * ((n + 63) // 64 + 1) if sliced else 4 * (n + 1)
-    return 8 * (nnz - n) + index + 12 * n
+    return (4 + col_bytes) * (nnz - n) + index + 12 * n
+
+
+def spmv32_canonical_bytes(n, nnz):
+    """The FP32 SpMV's bytes with int32 indices and the diagonal stored
+    (8*nnz + 4*(N+1) + 8*N), the count a CSR kernel moves."""
+    return 8 * nnz + 4 * (n + 1) + 8 * n
 
 
 def time_spmv(torch, ds, n, reps=50, fp32=False):
@@ -79,13 +86,13 @@ COPY_BYTES = 512 * 1024 * 1024
 
 def profile_spmv(torch, ds, n, reps=20, ds_csr=None):
     """The launches the traffic counters are read on: `reps` FP64 and FP32
-    SpMVs of the workload (FP32 in the system's layout, and in the CSR
-    layout when ds_csr is given) and `reps` copies of COPY_BYTES (a kernel
-    whose HBM traffic is known: COPY_BYTES read, COPY_BYTES written), which
-    calibrates FETCH_SIZE / WRITE_SIZE."""
+    SpMVs of the workload in the layouts of ds (and of ds_csr when given)
+    and `reps` copies of COPY_BYTES (a kernel whose HBM traffic is known:
+    COPY_BYTES read, COPY_BYTES written), which calibrates FETCH_SIZE /
+    WRITE_SIZE."""
     ops = [(torch.float64, ds.spmv), (torch.float32, ds.spmv32)]
     if ds_csr is not None:
-        ops.append((torch.float32, ds_csr.spmv32))
+        ops += [(torch.float64, ds_csr.spmv), (torch.float32, ds_csr.spmv32)]
     for dt, f in ops:
         x = torch.randn(n, dtype=dt, device="cuda")
         y = torch.empty(n, dtype=dt, device="cuda")
@@ -625,15 +632,19 @@ def main():
     m = s.matrix
     N, nnz = m.n, m.nnz
     log(f"N={N} nnz={nnz}")
-    ds = pkg.krylov.DeviceSystem(m)
     if args.profile_spmv:
-        os.environ["TFLOWS_FP32_OP"] = "csr"
-        ds_csr = pkg.krylov.DeviceSystem(m)
-        del os.environ["TFLOWS_FP32_OP"]
-        profile_spmv(torch, ds, N, ds_csr=ds_csr)
-        ds_csr.close()
-        ds.close()
+        # both operator forms in both precisions, fixed rather than chosen
+        # by timing (a profiler's per-dispatch overhead disturbs the timing)
+        systems = []
+        for form in ("sliced", "csr"):
+            os.environ["TFLOWS_FP32_OP"] = os.environ["TFLOWS_FP64_OP"] = form
+            systems.append(pkg.krylov.DeviceSystem(m))
+        del os.environ["TFLOWS_FP32_OP"], os.environ["TFLOWS_FP64_OP"]
+        profile_spmv(torch, systems[0], N, ds_csr=systems[1])
+        for d in systems:
+            d.close()
         return
+    ds = pkg.krylov.DeviceSystem(m)
     b_dev = torch.from_numpy(s.b).cuda()
     x = torch.zeros(N, dtype=torch.float64, device="cuda")
     r = torch.empty_like(b_dev)
@@ -681,7 +692,7 @@ def main():
     b64 = spmv_bytes(N, nnz, 8)
     t32 = time_spmv(torch, ds, N, fp32=True)
     layout = ds.fp32_layout()
-    b32 = spmv_bytes(N, nnz, 4, sliced=layout["sliced"])
+    b32 = spmv_bytes(N, nnz, 4, sliced=layout["sliced"], col_bytes=layout["col_bytes"])
     k32 = "k_spmv<DiffSell32>" if layout["sliced"] else "k_spmv<DiffCsr32>"
     head = "mixed" if "mixed" in results else "fp64"
     spmv_t, spmv_b = (t32, b32) if head == "mixed" else (t64, b64)
@@ -716,12 +727,17 @@ def main():
         "spmv_fp32_GBs": round(b32 / t32 / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": (measured_traffic(("fp32_sliced" if layout["sliced"] else "fp32_csr")
+                     "traffic": (measured_traffic((("fp32_sliced16" if layout["col_bytes"] == 2 else "fp32_sliced")
+                                                   if layout["sliced"] else "fp32_csr")
                                                   if head == "mixed" else "fp64") if n == 128 else None),
                      "kernel": k32 if head == "mixed" else "k_spmv<Csr<double>>",
                      "bytes_per_launch": spmv_b, "us_per_launch": round(spmv_t * 1e6, 2)},
-        "fp32_operator": {"layout": "sliced ELL (64-row slices)" if layout["sliced"] else "CSR",
-                          "stored_entries": layout["stored_entries"], "off_diagonal_entries": nnz - N},
+        "fp32_operator": {"layout": ("sliced ELL (64-row slices)" if layout["sliced"] else "CSR") +
+                                    (", 16-bit column offsets" if layout["col_bytes"] == 2 else ""),
+                          "stored_entries": layout["stored_entries"], "off_diagonal_entries": nnz - N,
+                          # the same launch priced as a CSR kernel with int32 indices and
+                          # the diagonal stored moves it: 8 nnz + 4 (N+1) + 8 N
+                          "csr_equivalent_GBs": round(spmv32_canonical_bytes(N, nnz) / t32 / 1e9, 1)},
     }
     if not args.no_cpu:
         log("timing the CPU baseline")

@@ -189,8 +189,10 @@ int32_t tflows_system_reordered(tflows_system s);
 /* Layout of the FP32 inner operator of the mixed-precision solves (the
  * difference form of A: off-diagonal a_ij and FP64 row sums rounded to
  * FP32).  *sliced = 1: sliced ELL, 64-row slices stored slot-major and
- * padded to the slice's longest row (the default); 0: CSR
- * (environment TFLOWS_FP32_OP=csr when the system was created).
+ * padded to the slice's longest row; 2: the same with each column held as
+ * a 16-bit offset from its row (chosen where every offset fits, unless
+ * TFLOWS_FP32_COL16=0); 0: CSR (environment TFLOWS_FP32_OP=csr when the
+ * system was created, or slower by the first use's timing).
  * *stored_entries: off-diagonal entries held, padding included.  Builds
  * the operator if no solve has yet.  No counterpart in the reference. */
 int tflows_system_fp32_layout(tflows_system s, int32_t* sliced, int64_t* stored_entries, void* stream);

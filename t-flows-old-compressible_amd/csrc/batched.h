@@ -114,6 +114,11 @@ class Batch {
     DevArray<SolverState> st_, st32_;
     HostMirror* hm_ = nullptr;         // K mirrors each, coherent pinned memory
     HostMirror* hm32_ = nullptr;
+    // the `done` flags of every system at the end of each replayed chunk,
+    // two slots by chunk parity (coherent pinned memory): the host's
+    // rebalancing decisions read the state at a fixed chunk boundary, so
+    // they (and every result) do not depend on how far the device has run
+    int* snap_ = nullptr;
     BatchConfig fixed_;
     std::map<Key, BatchConfig> tuned_;
     std::map<GraphKey, hipGraphExec_t> graphs_;

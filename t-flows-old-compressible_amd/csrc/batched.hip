@@ -25,6 +25,7 @@
 //    autotuner that times a few iterations of every candidate on the
 //    caller's own systems, once per (precision, method, preconditioner).
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <functional>
 #include <numeric>
@@ -248,6 +249,8 @@ Batch::Batch(int k, const int* n, const int* nonz, const int* const* acol, const
     st32_.ensure(k);
     TF_CHECK(hipHostMalloc((void**)&hm_, sizeof(HostMirror) * k, hipHostMallocMapped | hipHostMallocCoherent));
     TF_CHECK(hipHostMalloc((void**)&hm32_, sizeof(HostMirror) * k, hipHostMallocMapped | hipHostMallocCoherent));
+    TF_CHECK(hipHostMalloc((void**)&snap_, sizeof(int) * 2 * k, hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(snap_, 0, sizeof(int) * 2 * k);
     for (int i = 0; i < 2; ++i) TF_CHECK(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
     for (int i = 2; i < 4; ++i) TF_CHECK(hipEventCreate(&ev_[i]));
     TF_CHECK(hipEventCreateWithFlags(&ev_[4], hipEventDisableTiming));
@@ -262,6 +265,7 @@ Batch::~Batch() {
     if (h_maps_) (void)hipHostFree(h_maps_);
     if (hm_) (void)hipHostFree(hm_);
     if (hm32_) (void)hipHostFree(hm32_);
+    if (snap_) (void)hipHostFree(snap_);
     sys_.reset();
     if (bs_) (void)hipStreamDestroy(bs_);
 }
@@ -382,6 +386,15 @@ BatchConfig Batch::config_for(int bits, int method, int prec) const {
     return c;
 }
 
+// The done flags after a chunk of `chunk` iterations into slot
+// (chunk index & 1) of snap; the chunk index follows from the iteration
+// base, which the chunk has just advanced.
+__global__ __launch_bounds__(kBlock) void k_done_snapshot(int k, int chunk, const SolverState* __restrict__ st,
+                                                          const int* __restrict__ kbase, int* snap) {
+    const int m = *kbase / chunk - 1;
+    for (int i = threadIdx.x; i < k; i += kBlock) snap[(m & 1) * k + i] = st[i].done;
+}
+
 template <typename T, typename Op>
 hipGraphExec_t Batch::graph(const GraphKey& gk, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
                             T* partials, SolverState* st, HostMirror* hm) {
@@ -396,6 +409,8 @@ hipGraphExec_t Batch::graph(const GraphKey& gk, const Op& Aop, const Csr<T>& A, 
         for (int i = 1; i <= gk.chunk; ++i)
             enqueue_iteration<T, Op>(c, gk.method, gk.prec != 2, i, Aop, A, w, S, st, hm, L, bs_);
         launch_advance(kbase_.p, gk.chunk, bs_);
+        hipLaunchKernelGGL(k_done_snapshot, dim3(1), dim3(kBlock), 0, bs_, k_, gk.chunk, (const SolverState*)st,
+                           (const int*)kbase_.p, snap_);
     } catch (...) {
         (void)hipStreamEndCapture(bs_, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -512,21 +527,25 @@ void Batch::run(int bits, int method, int prec, const Op& Aop, const Csr<T>& A, 
     int64_t mapped = tile0_[k_];
     bool pending[2] = {false, false};
     int slot = 0;
-    long long issued = 0;
+    long long issued = 0, launched = 0;
     bool all_done = false;
     while (issued < limit) {
         TF_CHECK(hipGraphLaunch(exec, bs_));
         issued += C;
+        ++launched;
         TF_CHECK(hipEventRecord(ev_[slot], bs_));
         pending[slot] = true;
         const int other = slot ^ 1;
         if (pending[other]) {
+            // chunk launched - 2 has ended: its snapshot is the state at
+            // that boundary, whatever the device has done since
             TF_CHECK(hipEventSynchronize(ev_[other]));
             pending[other] = false;
+            const int* sn = snap_ + ((launched - 2) & 1) * k_;
             int64_t running = 0;
             all_done = true;
             for (int i = 0; i < k_; ++i) {
-                act[i] = hm[i].done == 0;
+                act[i] = sn[i] == 0;
                 if (act[i]) {
                     all_done = false;
                     running += tiles_[i];

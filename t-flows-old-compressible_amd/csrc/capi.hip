@@ -275,7 +275,7 @@ int tflows_system_fp32_layout(tflows_system s, int32_t* sliced, int64_t* stored_
         if (!sliced || !stored_entries) throw std::invalid_argument("null output pointer");
         const hipStream_t st = as_stream(stream);
         const bool sl = s->sys->op32_sliced(st);
-        *sliced = sl ? 1 : 0;
+        *sliced = sl ? (s->sys->sell32_col_bytes(st) == 2 ? 2 : 1) : 0;
         *stored_entries = sl ? s->sys->sell32_entries(st) : s->sys->nnz() - s->sys->n();
         TF_CHECK(hipStreamSynchronize(st));
     });

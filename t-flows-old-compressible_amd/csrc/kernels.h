@@ -137,6 +137,9 @@ struct DiffSell32 {
     const int* __restrict__ col;
     const float* __restrict__ val;   // off-diagonal a_ij, 0 in padding
     const float* __restrict__ rs;    // row sums (from FP64)
+    // when not null: col - row of every slot as 16 bits, read instead of
+    // col (made where every column lies within 32767 rows of its row)
+    const short* __restrict__ dc = nullptr;
 };
 constexpr int kSlice = 64;
 
@@ -153,7 +156,11 @@ struct Sell {
     const int* __restrict__ off;     // ceil(n/64)+1 slice offsets
     const int* __restrict__ col;     // -1: padding
     const T* __restrict__ val;
+    // when not null: col - row of every slot as 16 bits (kPad16: padding),
+    // read instead of col (made where every column lies within 32767 rows)
+    const short* __restrict__ dc = nullptr;
 };
+constexpr short kPad16 = -32768;
 
 template <typename Op> struct OpValue;
 template <typename T> struct OpValue<Csr<T>> { using type = T; };
@@ -537,20 +544,25 @@ __device__ __forceinline__ void sell_tile(const DiffSell32& A, int r0, int r1, G
         const int sl = row / kSlice;
         const int b = A.off[sl];
         const int w = (A.off[sl + 1] - b) / kSlice;
-        const int* __restrict__ cp = A.col + b + (row - sl * kSlice);
         const float* __restrict__ vp = A.val + b + (row - sl * kSlice);
         float own[NRHS], acc[NRHS];
         g(row, own);
 #pragma unroll
         for (int v = 0; v < NRHS; ++v) acc[v] = 0.0f;
-#pragma unroll 2
-        for (int j = 0; j < w; ++j) {
-            const int c = cp[j * kSlice];
-            const float a = vp[j * kSlice];
+        auto slot = [&](int c, float a) {
             float gv[NRHS];
             g(c, gv);
 #pragma unroll
             for (int v = 0; v < NRHS; ++v) acc[v] = acc[v] + a * (gv[v] - own[v]);
+        };
+        if (A.dc) {                                  // the same columns, 16-bit
+            const short* __restrict__ dp = A.dc + b + (row - sl * kSlice);
+#pragma unroll 2
+            for (int j = 0; j < w; ++j) slot(row + (int)dp[j * kSlice], vp[j * kSlice]);
+        } else {
+            const int* __restrict__ cp = A.col + b + (row - sl * kSlice);
+#pragma unroll 2
+            for (int j = 0; j < w; ++j) slot(cp[j * kSlice], vp[j * kSlice]);
         }
         const float rsi = A.rs[row];
 #pragma unroll
@@ -577,18 +589,27 @@ __device__ __forceinline__ void sell_rows(const Sell<T>& A, int r0, int r1, Gath
         T acc[NRHS];
 #pragma unroll
         for (int v = 0; v < NRHS; ++v) acc[v] = SUBTRACT ? init(row, v) : T(0);
-#pragma unroll 2
-        for (int j = 0; j < w; ++j) {
-            const int c = cp[j * kSlice];
-            if (c >= 0) {
-                const T a = vp[j * kSlice];
-                T gv[NRHS];
-                g(c, gv);
+        auto slot = [&](int c, T a) {
+            T gv[NRHS];
+            g(c, gv);
 #pragma unroll
-                for (int v = 0; v < NRHS; ++v) {
-                    const T prod = a * gv[v];
-                    acc[v] = SUBTRACT ? acc[v] - prod : acc[v] + prod;
-                }
+            for (int v = 0; v < NRHS; ++v) {
+                const T prod = a * gv[v];
+                acc[v] = SUBTRACT ? acc[v] - prod : acc[v] + prod;
+            }
+        };
+        if (A.dc) {                                  // the same columns, 16-bit
+            const short* __restrict__ dp = A.dc + b + (row - sl * kSlice);
+#pragma unroll 2
+            for (int j = 0; j < w; ++j) {
+                const short d = dp[j * kSlice];
+                if (d != kPad16) slot(row + (int)d, vp[j * kSlice]);
+            }
+        } else {
+#pragma unroll 2
+            for (int j = 0; j < w; ++j) {
+                const int c = cp[j * kSlice];
+                if (c >= 0) slot(c, vp[j * kSlice]);
             }
         }
         epi(row, acc);

@@ -23,6 +23,7 @@
 // plain FP32 product loses the near-null mode (rounding of a_ii against
 // sum a_ij exceeds its eigenvalue), which makes the refinement diverge.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -229,6 +230,15 @@ DiffSell32 System::sell32(hipStream_t s) {
             TF_CHECK(hipMemcpy(sell_col_.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
             TF_CHECK(hipMemcpy(sell_map_.p, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));
         }
+        // the 16-bit column form where every slot's column lies within
+        // 32767 rows of its row (TFLOWS_FP32_COL16=0 keeps 32-bit columns)
+        std::vector<short> dc;
+        if (col16_from_env("TFLOWS_FP32_COL16") && slot_offsets16(n_, off, col, true, dc)) {
+            sell_dc_.ensure(dc.size());
+            TF_CHECK(hipMemcpy(sell_dc_.p, dc.data(), sizeof(short) * dc.size(), hipMemcpyHostToDevice));
+        } else {
+            sell_dc_.release();
+        }
         sell_entries_ = (int64_t)col.size();
         sell_built_ = true;
         sell_valid_ = false;
@@ -240,7 +250,7 @@ DiffSell32 System::sell32(hipStream_t s) {
         TF_CHECK(hipGetLastError());
         sell_valid_ = true;
     }
-    return DiffSell32{n_, sell_off_.p, sell_col_.p, sell_val_.p, d.rs};
+    return DiffSell32{n_, sell_off_.p, sell_col_.p, sell_val_.p, d.rs, sell_dc_.p};
 }
 
 const float* System::diag32(hipStream_t s) {

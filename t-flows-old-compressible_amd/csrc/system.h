@@ -106,6 +106,13 @@ struct Levels {
 // row pointer rp and columns cl: slice offsets, the column of every slot
 // and the CSR entry it holds (-1 for padding).  Padding slots take the
 // row's own index when pad_own, else -1.
+// 16-bit column offsets (col - row) of a sliced layout; false when one
+// does not fit.  pad_own: padding slots hold the row itself; otherwise
+// column -1, which becomes kPad16.
+bool slot_offsets16(int n, const std::vector<int>& off, const std::vector<int>& col, bool pad_own,
+                    std::vector<short>& dc);
+// false when the environment variable `name` is "0"
+bool col16_from_env(const char* name);
 void slice_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl, bool pad_own, std::vector<int>& off,
                   std::vector<int>& col, std::vector<int>& map);
 
@@ -214,6 +221,12 @@ class System {
         else f(csr());
     }
     bool op64_sliced(hipStream_t s);
+    // bytes per column index of the sliced FP32 operator: 2 when it holds
+    // 16-bit column offsets, else 4
+    int sell32_col_bytes(hipStream_t s) {
+        sell32(s);
+        return sell_dc_.p ? 2 : 4;
+    }
     int64_t sell32_entries(hipStream_t s) {
         sell32(s);
         return sell_entries_;
@@ -311,11 +324,13 @@ class System {
     DevArray<double> r64_;
     DevArray<int> off_rowptr_, off_col_;
     DevArray<int> sell_off_, sell_col_, sell_map_;
+    DevArray<short> sell_dc_;       // col - row, 16-bit, where every slot fits
     DevArray<float> sell_val_;
     int64_t sell_entries_ = 0;
     bool sell_built_ = false, sell_valid_ = false;
     int op32_mode_ = 2;                  // 0 CSR, 1 sliced, 2 decide by timing on first use
     DevArray<int> sell64_off_, sell64_col_, sell64_map_;
+    DevArray<short> sell64_dc_;     // col - row, 16-bit, where every slot fits
     DevArray<double> sell64_val_;
     int64_t sell64_entries_ = 0;
     bool sell64_built_ = false, sell64_valid_ = false;

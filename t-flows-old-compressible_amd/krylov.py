@@ -372,10 +372,11 @@ class DeviceSystem:
 
     def fp32_layout(self, stream=None) -> dict:
         """Layout of the FP32 inner operator: {"sliced": bool,
-        "stored_entries": off-diagonal entries held, padding included}."""
+        "col_bytes": 4, or 2 for 16-bit column offsets, "stored_entries":
+        off-diagonal entries held, padding included}."""
         sl, ent = C.c_int32(0), C.c_int64(0)
         _check(self.lib.tflows_system_fp32_layout(self.handle, C.byref(sl), C.byref(ent), _stream(stream)))
-        return {"sliced": bool(sl.value), "stored_entries": int(ent.value)}
+        return {"sliced": sl.value > 0, "col_bytes": 2 if sl.value == 2 else 4, "stored_entries": int(ent.value)}
 
     def set_lanes(self, lanes: int):
         """SpMV form: lanes per row (1, 2, 4 or 8); results are identical."""

@@ -208,6 +208,31 @@ def test_batch_lane_forms(pkg, oracle, cuda, mixed_sizes, monkeypatch, form):
         assert np.array_equal(np.concatenate(xs), first)
 
 
+def test_batch_rebalance_reproducible(pkg, cuda, mixed_sizes):
+    """Re-mapping the blocks of finished systems is decided on the state at
+    a chunk boundary (a snapshot the last chunk writes), never on how far
+    the device has run when the host looks: the same batch solved twice,
+    rebalancing on the way, gives the same bits."""
+    import scipy.sparse as sp
+    mg = pkg.meshgen
+    one = mg.from_scipy(sp.csr_matrix(np.array([[4.0]])))
+    systems = list(mixed_sizes[:5]) + [mg.LinearSystem(one, np.array([2.0 + k]), np.zeros(1), 1e-8, 1, 100, {})
+                                       for k in range(12)]
+    runs = []
+    for _ in range(3):
+        B, res, xs, rs = _batch_solve(pkg, cuda, systems, "cgs", 1, 2000, grid=128, lanes=1, chunk=2,
+                                      rebalance=True)
+        runs.append((res, xs, B.config("cgs", 1)["rebalances"]))
+        B.close()
+    assert runs[0][2] >= 1
+    for res, xs, reb in runs[1:]:
+        assert reb == runs[0][2]
+        assert np.array_equal(res["niter"], runs[0][0]["niter"])
+        assert np.array_equal(res["fin_res"], runs[0][0]["fin_res"])
+        for a, b in zip(xs, runs[0][1]):
+            assert np.array_equal(a, b)
+
+
 def test_batch_rebalance_and_chunks(pkg, oracle, cuda, mixed_sizes):
     """Handing blocks of finished systems to running ones, and different
     graph chunk sizes, leave every system's result within the parity bar.

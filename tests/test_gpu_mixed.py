@@ -136,6 +136,45 @@ def test_sliced_fp32_operator_matches_csr_form(pkg, cuda, monkeypatch, case):
     assert c[1][2] < s.tol
 
 
+@pytest.mark.parametrize("case", ["pressure", "pressure_perm_wide"])
+def test_fp32_sliced_column_width(pkg, cuda, monkeypatch, case):
+    """The sliced FP32 operator holds its columns as 16-bit offsets from the
+    row where every offset fits (TFLOWS_FP32_COL16=0 keeps 32-bit columns):
+    the same columns, so the same products and mixed solves bit for bit.  A
+    randomly renumbered system of 64,000 rows stored in the caller's order
+    (TFLOWS_REORDER=0; the default locality order would bring its offsets
+    within 16 bits) keeps 32-bit columns."""
+    import torch
+    mg = pkg.meshgen
+    s = {"pressure": lambda: mg.pressure_system((23, 9, 7), seed=62),
+         "pressure_perm_wide": lambda: mg.pressure_system(40, seed=63, perm_seed=6)}[case]()
+    m = s.matrix
+    p = torch.from_numpy(np.random.default_rng(64).standard_normal(m.n).astype(np.float32)).to(cuda)
+    b = torch.from_numpy(s.b).to(cuda)
+    out = {}
+    monkeypatch.setenv("TFLOWS_FP32_OP", "sliced")
+    if case == "pressure_perm_wide":
+        monkeypatch.setenv("TFLOWS_REORDER", "0")
+    for flag in ("0", "1"):
+        monkeypatch.setenv("TFLOWS_FP32_COL16", flag)
+        ds = pkg.krylov.DeviceSystem(m)
+        y = torch.zeros(m.n, dtype=torch.float32, device=cuda)
+        ds.spmv32(p, y)
+        lay = ds.fp32_layout()
+        x = torch.zeros(m.n, dtype=torch.float64, device=cuda)
+        res = ds.solve_mixed_reliable(1, x, b, 3000, s.tol, interval=50)
+        torch.cuda.synchronize()
+        out[flag] = (lay, y.cpu().numpy(), res, x.cpu().numpy())
+        ds.close()
+    assert out["0"][0]["col_bytes"] == 4
+    assert out["1"][0]["col_bytes"] == (2 if case == "pressure" else 4)
+    assert np.array_equal(out["0"][1], out["1"][1])
+    # iterations, residual norms and x bit for bit; the replacement count is
+    # the host's (it may queue one more chunk before it sees the end)
+    assert out["0"][2][:3] == out["1"][2][:3] and np.array_equal(out["0"][3], out["1"][3])
+    assert out["1"][2][2] < s.tol
+
+
 @pytest.mark.parametrize("case,prec,interval", [
     ("pressure12", 1, 50), ("pressure12", 0, 7), ("pressure10_perm", 1, 1), ("pressure10_perm", 1, 1000),
     ("momentum10", 1, 20),

@@ -83,3 +83,39 @@ def test_sliced_batch_bitwise(pkg, cuda, monkeypatch):
         assert np.array_equal(a[0][key], b[0][key]), key
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
     assert np.all(b[0]["fin_res"] < np.array([t.tol for t in systems]))
+
+
+@pytest.mark.parametrize("case", ["pressure", "pressure_perm_wide", "momentum"])
+def test_fp64_sliced_column_width(pkg, cuda, monkeypatch, case):
+    """The sliced FP64 operator holds its columns as 16-bit offsets from the
+    row where every offset fits, padding marked apart (TFLOWS_FP64_COL16=0
+    keeps 32-bit columns): the same columns in the same order, so SpMV and
+    solves agree bit for bit.  The randomly renumbered 64,000-row system in
+    the caller's order (TFLOWS_REORDER=0) does not fit and keeps 32-bit
+    columns under both settings."""
+    import torch
+    mg = pkg.meshgen
+    s = {"pressure": lambda: mg.pressure_system((21, 8, 7), seed=81),
+         "pressure_perm_wide": lambda: mg.pressure_system(40, seed=82, perm_seed=7),
+         "momentum": lambda: mg.momentum_system(9, seed=83)}[case]()
+    m = s.matrix
+    p = torch.from_numpy(np.random.default_rng(84).standard_normal(m.n)).to(cuda)
+    monkeypatch.setenv("TFLOWS_FP64_OP", "sliced")
+    if case == "pressure_perm_wide":
+        monkeypatch.setenv("TFLOWS_REORDER", "0")
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("TFLOWS_FP64_COL16", flag)
+        ds = pkg.krylov.DeviceSystem(m)
+        q = torch.empty_like(p)
+        ds.spmv(p, q)
+        x = torch.zeros(m.n, dtype=torch.float64, device=cuda)
+        r = torch.from_numpy(s.b.copy()).to(cuda)
+        res = ds.solve("cgs" if case == "momentum" else "cg", 1, x, r, 3000, s.tol)
+        torch.cuda.synchronize()
+        out[flag] = (q.cpu().numpy(), res, x.cpu().numpy(), r.cpu().numpy())
+        ds.close()
+    a, b = out["0"], out["1"]
+    assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+    assert b[1][2] < s.tol
