@@ -28,6 +28,19 @@ namespace tflows {
 
 namespace {
 __device__ __forceinline__ bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
+// True when the solve is over or paused.  The leader of the same launch may
+// set these flags (in its stopping test) while the block's waves start, so
+// thread 0 reads them once and the whole block takes its answer: a block
+// whose waves disagreed would sum the block totals of only some waves.
+// Blocks that continue reach the leader's decision themselves, from the
+// same totals.
+__device__ __forceinline__ bool block_halted(const SolverState* st) {
+    __shared__ int h;
+    if (threadIdx.x == 0) h = st->done | st->suspend;
+    __syncthreads();
+    return h != 0;
+}
 }
 
 // Stopping test before iteration kd (CG.f90:93-108 for kd == 1, :248-257
@@ -41,22 +54,45 @@ __device__ __forceinline__ bool leader() { return blockIdx.x == 0 && threadIdx.x
 // true when iteration kd must not run.
 constexpr double kReplaceDrop = 1e-2;
 
+// The state fields a kernel's stopping test and scalars read, loaded
+// together at its start: the test then waits for one memory round trip
+// (these loads and the partial sums in flight together) instead of a chain
+// of dependent ones.  A kernel never reads here a field it writes itself
+// (err_rep is written only when the test does not read it).
+struct StopIn {
+    double tol, err_rep, rho0, rho1;
+    int cap, replaced_before, it, kcur, halt;
+    // rho of iteration k; a select, not an indexed array, so that the
+    // struct stays in registers
+    __device__ __forceinline__ double rho(int k) const { return (k & 1) ? rho1 : rho0; }
+};
+__device__ __forceinline__ StopIn load_stop_in(const SolverState* st) {
+    StopIn in;
+    in.tol = st->tol;
+    in.err_rep = st->err_rep;
+    in.rho0 = st->rho[0];
+    in.rho1 = st->rho[1];
+    in.cap = st->cap;
+    in.replaced_before = st->replaced_before;
+    in.it = st->it;
+    in.kcur = st->kcur;
+    in.halt = st->done | st->suspend;
+    return in;
+}
+
 template <typename T, bool MIXED>
-__device__ __forceinline__ bool fused_check(int kd, const T* __restrict__ partB, const double* __restrict__ partE,
-                                            SolverState* st, HostMirror* hm, T& rho_next) {
-    T tot[2];
-    site_total<T, 2>(partB, tot);
-    rho_next = tot[1];
-    const double tol = st->tol;
-    const int cap = st->cap;
+__device__ __forceinline__ bool stop_decision(int kd, T rr, const StopIn& in, const double* __restrict__ partE,
+                                              SolverState* st, HostMirror* hm) {
+    const double tol = in.tol;
+    const int cap = in.cap;
     bool authoritative = true;
     double err;
-    if (MIXED && st->replaced_before == kd) {
+    if (MIXED && in.replaced_before == kd) {
         double e64[1];
         site_total<double, 1>(partE, e64);
         err = sqrt(e64[0]);
     } else {
-        err = sqrt((double)tot[0]);
+        err = sqrt((double)rr);
         authoritative = !MIXED;
     }
     if (leader() && authoritative) {
@@ -67,7 +103,7 @@ __device__ __forceinline__ bool fused_check(int kd, const T* __restrict__ partB,
             st->ini_written = 1;
         }
     }
-    if (MIXED && !authoritative && err < kReplaceDrop * st->err_rep) {
+    if (MIXED && !authoritative && err < kReplaceDrop * in.err_rep) {
         if (leader()) {
             st->suspend = 1;
             publish(hm, 0, kd - 1);
@@ -97,6 +133,18 @@ __device__ __forceinline__ bool fused_check(int kd, const T* __restrict__ partB,
     }
     if (leader()) publish(hm, 0, kd - 1);
     return false;
+}
+
+// The stopping test on site B, whose two values are ||r||^2 and the next
+// rho = (r, z).
+template <typename T, bool MIXED>
+__device__ __forceinline__ bool fused_check(int kd, const T* __restrict__ partB, const double* __restrict__ partE,
+                                            SolverState* st, HostMirror* hm, T& rho_next) {
+    const StopIn in = load_stop_in(st);
+    T tot[2];
+    site_total<T, 2>(partB, tot);
+    rho_next = tot[1];
+    return stop_decision<T, MIXED>(kd, tot[0], in, partE, st, hm);
 }
 
 template <typename T>
@@ -131,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_cgf_spmv(Op A, const T* __restrict__
                                                      T* __restrict__ p1, T* __restrict__ q,
                                                      const T* __restrict__ partB, const double* __restrict__ partE,
                                                      T* __restrict__ partD, SolverState* st, HostMirror* hm) {
-    if (st->done || st->suspend) return;
+    if (block_halted(st)) return;
     const int kd = st->it + 1;
     T rho;
     if (fused_check<T, MIXED>(kd, partB, partE, st, hm, rho)) return;
@@ -185,7 +233,7 @@ __global__ __launch_bounds__(kBlock) void k_cgf_update(int n, T* __restrict__ x,
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_cgf_check(const T* __restrict__ partB, const double* __restrict__ partE,
                                                       SolverState* st, HostMirror* hm) {
-    if (st->done || st->suspend) return;
+    if (block_halted(st)) return;
     T rho;
     fused_check<T, MIXED>(st->it + 1, partB, partE, st, hm, rho);
 }
@@ -226,14 +274,16 @@ template <typename T, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_cgj_p(const T* __restrict__ z, T* __restrict__ p,
                                                   const T* __restrict__ partB, const double* __restrict__ partE,
                                                   int n, SolverState* st, HostMirror* hm) {
-    if (st->done || st->suspend) return;
-    const int kd = st->it + 1;
+    // issued together: the state, this thread's first group of rows and
+    // (in site_total) the partial sums of the stopping test
+    const StopIn in = load_stop_in(st);
+    __shared__ int halt;                 // thread 0's flags, for the whole block (block_halted)
+    if (threadIdx.x == 0) halt = in.halt;
+    const int kd = in.it + 1;
     const int ntiles = (n + kTileRows - 1) / kTileRows;
     int t0, t1;
     block_tiles(ntiles, t0, t1);
     const int rb = t0 * kTileRows, re = min(t1 * kTileRows, n);
-    // the first group of rows is loaded before the grid reduction of the
-    // stopping test, so its memory latency overlaps the reduction's
     int i0 = rb + (int)threadIdx.x;
     T zv[kUnroll], pv[kUnroll];
 #pragma unroll
@@ -242,14 +292,17 @@ __global__ __launch_bounds__(kBlock) void k_cgj_p(const T* __restrict__ z, T* __
         zv[u] = i < re ? z[i] : T(0);
         pv[u] = i < re ? p[i] : T(0);
     }
-    T rho;
-    if (fused_check<T, MIXED>(kd, partB, partE, st, hm, rho)) return;
+    T tot[2];
+    site_total<T, 2>(partB, tot);        // its barriers publish `halt`
+    if (halt) return;
+    if (stop_decision<T, MIXED>(kd, tot[0], in, partE, st, hm)) return;
+    const T rho = tot[1];
     if (leader()) {
         st->rho[kd & 1] = (double)rho;
         st->kcur = kd;
     }
     const bool first = kd == 1;
-    const T beta = first ? T(0) : rho / (T)st->rho[(kd - 1) & 1];   // CG.f90:183
+    const T beta = first ? T(0) : rho / (T)in.rho(kd - 1);        // CG.f90:183
     for (;;) {
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
@@ -273,16 +326,18 @@ __global__ __launch_bounds__(kBlock) void k_cgj_update(int n, T* __restrict__ x,
                                                        const T* __restrict__ q, const T* __restrict__ D,
                                                        const T* __restrict__ partD, T* __restrict__ partB,
                                                        SolverState* st) {
-    if (st->done || st->suspend) return;
-    const int kd = st->kcur;
+    // issued together: the state, this thread's first group of rows and
+    // (in site_total) the partial sums of (p,q); the flags are not written
+    // by this kernel, so every thread reads the same ones
+    const StopIn in = load_stop_in(st);
+    const int kd = in.kcur;
     const int ntiles = (n + kTileRows - 1) / kTileRows;
     int t0, t1;
     block_tiles(ntiles, t0, t1);
     const int rb = t0 * kTileRows, re = min(t1 * kTileRows, n);
     T v[2] = {T(0), T(0)};
     T xv[kUnroll], pv[kUnroll], rv[kUnroll], qv[kUnroll], dv[kUnroll];
-    // loads of the next group of rows; the first is issued before the grid
-    // reduction of (p,q), so its latency overlaps the reduction's
+    // loads of the next group of rows
     auto load = [&](int i0) {
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
@@ -298,7 +353,8 @@ __global__ __launch_bounds__(kBlock) void k_cgj_update(int n, T* __restrict__ x,
     load(rb + (int)threadIdx.x);
     T pq[1];
     site_total<T, 1>(partD, pq);
-    const T alfa = (T)st->rho[kd & 1] / pq[0];          // CG.f90:219
+    if (in.halt) return;
+    const T alfa = (T)in.rho(kd) / pq[0];               // CG.f90:219
     for (int i0 = rb + (int)threadIdx.x; i0 < re; i0 += kUnroll * kBlock) {
         if (i0 != rb + (int)threadIdx.x) load(i0);
 #pragma unroll
@@ -418,5 +474,6 @@ template void launch_cgf_init_z<double>(int, const double*, const double*, doubl
                                         const SolverState*, hipStream_t);
 template void launch_cgf_init_z<float>(int, const float*, const float*, float*, const Sites<float>&,
                                        const SolverState*, hipStream_t);
+
 
 }  // namespace tflows

@@ -103,16 +103,38 @@ __device__ __forceinline__ void site_write(T (&v)[NV], T* __restrict__ partials)
     }
 }
 
+// Each thread adds the partials of blocks b0 + t, b0 + t + kBlock, ... of
+// [b0, b1) in that order.  The loads are issued kSiteBatch at a time before
+// any of them is added: a plain loop waits for each load before it issues
+// the next, a chain of memory round trips at the start of every consumer.
+constexpr int kSiteBatch = 8;
+template <typename T, int NV>
+__device__ __forceinline__ void thread_partials(const T* __restrict__ partials, int G, int b0, int b1,
+                                                T (&tot)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) tot[k] = T(0);
+    int b = b0 + (int)threadIdx.x;
+    for (; b + (kSiteBatch - 1) * kBlock < b1; b += kSiteBatch * kBlock) {
+        T v[NV][kSiteBatch];
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+#pragma unroll
+            for (int j = 0; j < kSiteBatch; ++j) v[k][j] = partials[k * G + b + j * kBlock];
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+#pragma unroll
+            for (int j = 0; j < kSiteBatch; ++j) tot[k] += v[k][j];
+    }
+    for (; b < b1; b += kBlock) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) tot[k] += partials[k * G + b];
+    }
+}
+
 template <typename T, int NV>
 __device__ __forceinline__ void site_total(const T* __restrict__ partials, T (&tot)[NV]) {
     __shared__ T sm[kWavesPerBlock * NV];
-    const int G = gridDim.x;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        T s = T(0);
-        for (int b = threadIdx.x; b < G; b += kBlock) s += partials[k * G + b];
-        tot[k] = s;
-    }
+    thread_partials<T, NV>(partials, (int)gridDim.x, 0, (int)gridDim.x, tot);
     block_sum<T, NV>(tot, (T*)sm);
 }
 
@@ -120,13 +142,7 @@ __device__ __forceinline__ void site_total(const T* __restrict__ partials, T (&t
 template <typename T, int NV>
 __device__ __forceinline__ void site_total_range(const T* __restrict__ partials, int b0, int b1, T (&tot)[NV]) {
     __shared__ T sm[kWavesPerBlock * NV];
-    const int G = gridDim.x;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        T s = T(0);
-        for (int b = b0 + threadIdx.x; b < b1; b += kBlock) s += partials[k * G + b];
-        tot[k] = s;
-    }
+    thread_partials<T, NV>(partials, (int)gridDim.x, b0, b1, tot);
     block_sum<T, NV>(tot, (T*)sm);
 }
 
