@@ -104,26 +104,27 @@ __device__ __forceinline__ void site_write(T (&v)[NV], T* __restrict__ partials)
 }
 
 // Each thread adds the partials of blocks b0 + t, b0 + t + kBlock, ... of
-// [b0, b1) in that order.  The loads are issued kSiteBatch at a time before
-// any of them is added: a plain loop waits for each load before it issues
-// the next, a chain of memory round trips at the start of every consumer.
-constexpr int kSiteBatch = 8;
+// [b0, b1) in that order.  The loads are issued B at a time before any of
+// them is added (8 floats or 4 doubles per value, 32 bytes): a plain loop
+// waits for each load before it issues the next, a chain of memory round
+// trips at the start of every consumer.
 template <typename T, int NV>
 __device__ __forceinline__ void thread_partials(const T* __restrict__ partials, int G, int b0, int b1,
                                                 T (&tot)[NV]) {
+    constexpr int B = 32 / (int)sizeof(T);
 #pragma unroll
     for (int k = 0; k < NV; ++k) tot[k] = T(0);
     int b = b0 + (int)threadIdx.x;
-    for (; b + (kSiteBatch - 1) * kBlock < b1; b += kSiteBatch * kBlock) {
-        T v[NV][kSiteBatch];
+    for (; b + (B - 1) * kBlock < b1; b += B * kBlock) {
+        T v[NV][B];
 #pragma unroll
         for (int k = 0; k < NV; ++k)
 #pragma unroll
-            for (int j = 0; j < kSiteBatch; ++j) v[k][j] = partials[k * G + b + j * kBlock];
+            for (int j = 0; j < B; ++j) v[k][j] = partials[k * G + b + j * kBlock];
 #pragma unroll
         for (int k = 0; k < NV; ++k)
 #pragma unroll
-            for (int j = 0; j < kSiteBatch; ++j) tot[k] += v[k][j];
+            for (int j = 0; j < B; ++j) tot[k] += v[k][j];
     }
     for (; b < b1; b += kBlock) {
 #pragma unroll

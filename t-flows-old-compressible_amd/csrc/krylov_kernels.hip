@@ -59,13 +59,23 @@ constexpr int kPrefetch = 4;
 // (CG.f90:259-262).  Every block takes the same decision; the leader
 // records it.  Returns true when the solve is over.
 // ---------------------------------------------------------------------------
+// The state fields are read before the partial sums are waited for, so the
+// two sets of loads are in flight together.  `halt`, when given, is a flag
+// in shared memory that thread 0 set before the call (the system was over
+// before this launch); it is read after the reduction's barriers.
 template <typename T>
 __device__ __forceinline__ bool check_stop(const BatchCtx& ctx, int k, const T* __restrict__ partB,
-                                           SolverState* st, HostMirror* hm, T (&tot)[2]) {
-    ctx_total<T, 2>(ctx, partB, tot);
-    const double err = sqrt((double)tot[0]);
+                                           SolverState* st, HostMirror* hm, T (&tot)[2],
+                                           const int* halt = nullptr) {
     const double tol = st->tol;
     const int cap = st->cap;
+    const double div_limit = st->div_limit;
+    const double prev_min = st->err_min[(k - 1) & 1];
+    const int prev_kmin = st->k_min[(k - 1) & 1];
+    const int stall_min = st->stall_min;
+    ctx_total<T, 2>(ctx, partB, tot);
+    if (halt && *halt) return true;
+    const double err = sqrt((double)tot[0]);
     const bool lead = ctx.leader();
     if (lead) {
         st->error = err;
@@ -82,17 +92,17 @@ __device__ __forceinline__ bool check_stop(const BatchCtx& ctx, int k, const T* 
         }
         return true;
     }
-    if (st->div_limit > 0.0) {
-        const double prev = st->err_min[(k - 1) & 1];
+    if (div_limit > 0.0) {
+        const double prev = prev_min;
         const bool lower = err < prev;
-        const int kmin = lower ? k : st->k_min[(k - 1) & 1];
+        const int kmin = lower ? k : prev_kmin;
         if (lead) {
             st->err_min[k & 1] = lower ? err : prev;
             st->k_min[k & 1] = kmin;
         }
-        const bool diverged = !(err <= st->div_limit);
-        const int window = st->stall_min > kmin ? st->stall_min : kmin;
-        if (diverged || (st->stall_min > 0 && k - kmin > window)) {
+        const bool diverged = !(err <= div_limit);
+        const int window = stall_min > kmin ? stall_min : kmin;
+        if (diverged || (stall_min > 0 && k - kmin > window)) {
             if (lead) {
                 st->done = 1;
                 st->diverged = diverged ? 1 : 0;
@@ -538,13 +548,14 @@ __global__ __launch_bounds__(kBlock) void k_rho(BatchCtx ctx, int n, const T* __
 // or site C (DIC forms); recorded by the leader in st->rho[k & 1].
 template <typename T, bool JAC>
 __device__ __forceinline__ bool iteration_rho(const BatchCtx& ctx, int k, const T* partB, const T* partC,
-                                              SolverState* st, HostMirror* hm, T& rho) {
+                                              SolverState* st, HostMirror* hm, T& rho, const int* halt = nullptr) {
     T tot[2];
     if (JAC) {
-        if (check_stop<T>(ctx, k, partB, st, hm, tot)) return false;
+        if (check_stop<T>(ctx, k, partB, st, hm, tot, halt)) return false;
         rho = tot[1];
     } else {
         ctx_total<T, 2>(ctx, partC, tot);
+        if (halt && *halt) return false;
         rho = tot[0];
     }
     if (ctx.leader()) st->rho[k & 1] = (double)rho;
@@ -575,12 +586,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(BatchCtx ctx, int k_arg, int n,
                                                  const T* __restrict__ partC, SolverState* st_base,
                                                  HostMirror* hm_base) {
     TF_STATE;
-    if (st->done) return;
+    // done is read once per block (its leader may set it in this launch)
+    // and tested after the reduction, so that it, rho_{k-1}, the first
+    // kPrefetch rows of this thread and the partial sums are loaded together
+    __shared__ int halt;
+    if (threadIdx.x == 0) halt = st->done;
     const int k = ctx.k(k_arg);
+    const double rho_prev = st->rho[(k - 1) & 1];
     int rb, re;
     ctx_rows(ctx, n, rb, re);
-    // the first kPrefetch rows of this thread are loaded before the grid
-    // reduction, so their memory latency overlaps its latency
     T av[kPrefetch], dv[kPrefetch], pv[kPrefetch];
 #pragma unroll
     for (int u = 0; u < kPrefetch; ++u) {
@@ -591,9 +605,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(BatchCtx ctx, int k_arg, int n,
         pv[u] = ok ? p[i] : T(0);
     }
     T rho;
-    if (!iteration_rho<T, JAC>(ctx, k, partB, partC, st, hm, rho)) return;
+    if (!iteration_rho<T, JAC>(ctx, k, partB, partC, st, hm, rho, &halt)) return;
     const bool first = k == 1;
-    const T beta = first ? T(0) : rho / (T)st->rho[(k - 1) & 1];
+    const T beta = first ? T(0) : rho / (T)rho_prev;
 #pragma unroll
     for (int u = 0; u < kPrefetch; ++u) {
         const int i = rb + (int)threadIdx.x + u * kBlock;
@@ -645,8 +659,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(BatchCtx ctx, int k_arg, i
                                                       const T* __restrict__ partD, T* __restrict__ partB,
                                                       const SolverState* st_base) {
     const SolverState* st = st_base + ctx.sys();
-    TF_DONE_RETURN(partB, 2);
+    // done (set by earlier kernels only, so the same in every thread) and
+    // rho_k are loaded with the first rows and the partial sums of (p,q)
+    const int done = st->done;
     const int k = ctx.k(k_arg);
+    const double rho_k = st->rho[k & 1];
     int rb, re;
     ctx_rows(ctx, n, rb, re);
     T xv[kPrefetch], pv[kPrefetch], rv[kPrefetch], qv[kPrefetch], dv[kPrefetch];
@@ -662,7 +679,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(BatchCtx ctx, int k_arg, i
     }
     T pq[1];
     ctx_total<T, 1>(ctx, partD, pq);
-    const T alfa = (T)st->rho[k & 1] / pq[0];
+    if (done) {
+        if (ctx.blk_sys) {                   // a batch's consumers read every block's partials
+            T z_[2] = {};
+            site_write<T, 2>(z_, partB);
+        }
+        return;
+    }
+    const T alfa = (T)rho_k / pq[0];
     T v[2] = {T(0), T(0)};
 #pragma unroll
     for (int u = 0; u < kPrefetch; ++u) {
