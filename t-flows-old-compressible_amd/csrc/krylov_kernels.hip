@@ -659,9 +659,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(BatchCtx ctx, int k_arg, i
                                                       const T* __restrict__ partD, T* __restrict__ partB,
                                                       const SolverState* st_base) {
     const SolverState* st = st_base + ctx.sys();
-    // done (set by earlier kernels only, so the same in every thread) and
-    // rho_k are loaded with the first rows and the partial sums of (p,q)
-    const int done = st->done;
+    // done is set by earlier kernels only, so this return is the same in
+    // every thread; rho_k is loaded with the first rows and the partial sums
+    // of (p,q) (testing done after them as well costs this kernel its
+    // eighth wave per SIMD)
+    TF_DONE_RETURN(partB, 2);
     const int k = ctx.k(k_arg);
     const double rho_k = st->rho[k & 1];
     int rb, re;
@@ -679,13 +681,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(BatchCtx ctx, int k_arg, i
     }
     T pq[1];
     ctx_total<T, 1>(ctx, partD, pq);
-    if (done) {
-        if (ctx.blk_sys) {                   // a batch's consumers read every block's partials
-            T z_[2] = {};
-            site_write<T, 2>(z_, partB);
-        }
-        return;
-    }
     const T alfa = (T)rho_k / pq[0];
     T v[2] = {T(0), T(0)};
 #pragma unroll
