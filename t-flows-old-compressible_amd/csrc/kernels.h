@@ -627,5 +627,11 @@ __device__ __forceinline__ void resid_tile(const Csr<T>& A, int r0, int r1, Gath
     if constexpr (L == 1) stream_tile<T, 1, true>(A, r0, r1, g, init, epi);
     else group_tile<L, T, 1, true>(A, r0, r1, g, init, epi);
 }
+// The same on the sliced copy: the products subtracted from b in CSR
+// order, the same bits.
+template <int L, typename T, class Gather, class Init, class Epi>
+__device__ __forceinline__ void resid_tile(const Sell<T>& A, int r0, int r1, Gather& g, Init& init, Epi& epi) {
+    sell_rows<T, 1, true>(A, r0, r1, g, init, epi);
+}
 
 }  // namespace tflows

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "system.h"
@@ -140,7 +141,8 @@ struct RelEpi {
     }
 };
 
-__global__ __launch_bounds__(kBlock) void k_rel_resid(Csr<double> A, const double* __restrict__ x64,
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_rel_resid(Op A, const double* __restrict__ x64,
                                                       const double* __restrict__ b, float* __restrict__ r32,
                                                       float* __restrict__ z32, const float* __restrict__ D32,
                                                       double* __restrict__ r64, float* __restrict__ partB,
@@ -151,8 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_rel_resid(Csr<double> A, const doubl
     Gather1<double> g{x64};
     InitVec<double> init{b};
     RelEpi epi{r32, z32, D32, r64, {0.0f, 0.0f}, {0.0}};
-    for (int t = t0; t < t1; ++t)
-        stream_tile<double, 1, true>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi);
+    for (int t = t0; t < t1; ++t) resid_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi);
     site_write<float, 2>(epi.a32, partB);
     site_write<double, 1>(epi.a64, partE);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -542,11 +543,14 @@ SolveResult System::solve_mixed_reliable_impl(int prec, double* x, const double*
     TF_CHECK(hipMemcpyAsync(state32_.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
     mirror32_->done = 0;
     mirror32_->iter = 0;
-    const Csr<double> A = csr();
+    // the FP64 residual on the operator of the FP64 iterations (sliced or
+    // CSR: the same bits)
     auto replace = [&](double* rout) {
         hipLaunchKernelGGL(k_rel_merge, dim3(nb), dim3(kBlock), 0, s, n_, x, x32);
-        hipLaunchKernelGGL(k_rel_resid, dim3(grid_), dim3(kBlock), 0, s, A, (const double*)x, b, r32, z, D32, rout,
-                           S32.B, partE, state32_.p);
+        with_op64(s, [&](const auto& A) {
+            hipLaunchKernelGGL((k_rel_resid<std::decay_t<decltype(A)>>), dim3(grid_), dim3(kBlock), 0, s, A,
+                               (const double*)x, b, r32, z, D32, rout, S32.B, partE, state32_.p);
+        });
     };
     replace(nullptr);
     TF_CHECK(hipGetLastError());

@@ -119,3 +119,32 @@ def test_fp64_sliced_column_width(pkg, cuda, monkeypatch, case):
     assert np.array_equal(a[0], b[0]) and a[1] == b[1]
     assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
     assert b[1][2] < s.tol
+
+
+@pytest.mark.parametrize("case,prec,interval", [("pressure", 1, 50), ("pressure_perm", 0, 9), ("pressure", 1, 1)])
+def test_mixed_replacement_residual_bitwise(pkg, cuda, monkeypatch, case, prec, interval):
+    """The FP64 residual of every replacement of the reliable-update solve
+    runs on the FP64 operator of the system (r = b - A x: the products
+    subtracted from b in CSR order in either layout), so the mixed solve
+    gives the same iterations, norms, x and final residual whichever layout
+    the system keeps."""
+    import torch
+    mg = pkg.meshgen
+    s = {"pressure": lambda: mg.pressure_system((23, 9, 7), seed=81),
+         "pressure_perm": lambda: mg.pressure_system(11, seed=82, perm_seed=83)}[case]()
+    m = s.matrix
+    b = torch.from_numpy(s.b.copy()).to(cuda)
+
+    def run():
+        ds = pkg.krylov.DeviceSystem(m)
+        x = torch.zeros(m.n, dtype=torch.float64, device=cuda)
+        r = torch.zeros_like(b)
+        res = ds.solve_mixed_reliable(prec, x, b, 3000, s.tol, interval=interval, r_out=r)
+        torch.cuda.synchronize()
+        ds.close()
+        return res, x.cpu().numpy(), r.cpu().numpy()
+
+    a, c = _both(monkeypatch, run)
+    assert a[0][:3] == c[0][:3]
+    assert np.array_equal(a[1], c[1]) and np.array_equal(a[2], c[2])
+    assert c[0][2] < s.tol
