@@ -37,6 +37,7 @@
  */
 #include <math.h>
 #include <stddef.h>
+#include <stdlib.h>
 
 enum { BC_INFLOW = 1, BC_WALL = 2, BC_OUTFLOW = 3, BC_SYMMETRY = 4, BC_CONVECT = 5, BC_WALLFL = 6,
        BC_BUFFER = 11, BC_PRESSURE = 12 };
@@ -217,9 +218,11 @@ double oracle_coruvw(int nc, int nbc, int ns, const int *sidec, const int *sidai
  *   :137-138  rhoUi = rho^n Ui^n on (-NbC:NC)
  *   :146-149  Abou = 0, Aval = 0, b = 0
  *   :175-176  C = 0, X = 0
+ *   :167-172  PHImax / PHImin of rhoUi (CalMinMax) when a face takes a
+ *             ConvScheme scheme
  *   :205-240  convection face loop: rhoUiS = f rhoUi(c1) + (1-f) rhoUi(c2),
- *             left as it is by ConvScheme for BLEND = YES
- *             (Process/ConvScheme.f90:107-108); C(c1) -= Flux_u rhoUiS,
+ *             then ConvScheme's face value when max(BLEND(m1), BLEND(m2))
+ *             /= NO (kept as it is for YES); C(c1) -= Flux_u rhoUiS,
  *             C(c2) += ...; the upwind part X when BLEND_TEM /= NO
  *   :252-257  CONVEC = FI: b += URFC(material) (C - X)
  *   :264-266  X = 0 on the cells
@@ -236,19 +239,117 @@ double oracle_coruvw(int nc, int nbc, int ns, const int *sidec, const int *sidai
  * The flag and boundary codes are those of Process/IniPar.f90:24-32 and
  * Library/allp_mod.f90:16-24.  Per-material arrays BLEND, BLEND_TEM, URFC
  * are 1-based (material m at [m-1]); cell arrays (-NbC:NC) as elsewhere.
- * VISeff = VISc (the default branch of :287-295).  A face whose materials'
- * BLEND values have max(...) other than YES calls a ConvScheme branch this
- * restatement does not cover (a TVD scheme, or NO mixed with YES, for which
- * the reference leaves Phij undefined): returns -1.  abou is (-NbC:-1);
- * asave (NC) may be NULL; cbuf and xbuf hold NbC+NC+1 doubles.  Returns 0.
+ * VISeff = VISc (the default branch of :287-295).  flux is the module's
+ * mass flux Flux, whose sign picks ConvScheme's upwind cell.  A face whose
+ * materials' BLEND values have max(...) = NO next to YES (for which the
+ * reference leaves Phij undefined), or any other uncovered code, returns
+ * -1.  abou is (-NbC:-1); asave (NC) may be NULL; cbuf and xbuf hold
+ * NbC+NC+1 doubles.  Returns 0.
  */
 enum { FL_FI = 3, FL_CN = 2, IN_LIN = 4, IN_PAR = 5, BL_YES = 10, BL_NO = 11 };
+/* ConvScheme's face schemes, Process/IniPar.f90:42-49 */
+enum { CS_CDS = 20, CS_QUICK = 21, CS_LUDS = 22, CS_MINMOD = 23, CS_SMART = 24, CS_AVL_SMART = 25,
+       CS_SUPERBEE = 26, CS_GAMMA = 27 };
+
+/* MAX / MIN of two or three values as plain comparisons (the first
+ * argument is kept unless another is strictly beyond it). */
+static double mx2(double a, double b) { return b > a ? b : a; }
+static double mn2(double a, double b) { return b < a ? b : a; }
+static double mn3(double a, double b, double c) { return mn2(mn2(a, b), c); }
+static double mx3(double a, double b, double c) { return mx2(mx2(a, b), c); }
+
+/* A BLEND value max(BLEND(m1), BLEND(m2)) that this restatement covers:
+ * YES, or a ConvScheme scheme. */
+static int blend_covered(int bl) { return bl == BL_YES || (bl >= CS_CDS && bl <= CS_GAMMA); }
+
+/*
+ * Process/CalMinMax.f90:21-37: PHImax / PHImin of every cell (-NbC:NC) over
+ * the cell and the neighbours across its interior and BUFFER faces.
+ */
+void oracle_calminmax(int nc, int nbc, int ns, const int *sidec, const int *typebc, const double *phi,
+                      double *phimax, double *phimin)
+{
+    for (int c = 0; c < nbc + nc + 1; ++c) {                             /* :21-22 */
+        phimax[c] = phi[c];
+        phimin[c] = phi[c];
+    }
+    for (int s = 1; s <= ns; ++s) {                                     /* :24-35 */
+        const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
+        if (c2 > 0 || (c2 < 0 && typebc[nbc + c2] == BC_BUFFER)) {
+            const int i1 = nbc + c1, i2 = nbc + c2;
+            phimax[i1] = mx2(phimax[i1], phi[i2]);
+            phimin[i1] = mn2(phimin[i1], phi[i2]);
+            phimax[i2] = mx2(phimax[i2], phi[i1]);
+            phimin[i2] = mn2(phimin[i2], phi[i1]);
+        }
+    }
+}
+
+/*
+ * Process/ConvScheme.f90:55-135: the face value of Phi at face s for
+ * scheme `blenda`, given the central value phis.  The upwind cell C and the
+ * downwind cell D follow the sign of Flux(s) (the mass flux of the module,
+ * :55-65); gi, gj, gk are the gradient components and di, dj, dk the centre
+ * distances in the order the caller passes them.  YES keeps phis (:116).
+ * Indices are 0-based into (-NbC:NC) arrays (nbc added by the caller).
+ */
+static double conv_scheme(double phis, int s, int c1, int c2, int nbc, const double *phi, const double *gi,
+                          const double *gj, const double *gk, const double *di, const double *dj,
+                          const double *dk, const double *f, const double *flux, const double *phimax,
+                          const double *phimin, int blenda)
+{
+    if (blenda == BL_YES) return phis;                                  /* :116 (PhiU etc. unused) */
+    const int pos = flux[s - 1] > 0.0;
+    const double fj = pos ? 1.0 - f[s - 1] : f[s - 1];                   /* :55-65 */
+    const int Ci = nbc + (pos ? c1 : c2), Di = nbc + (pos ? c2 : c1);
+    const double sign = pos ? 1.0 : -1.0;
+    const double gd = gi[Ci] * di[s - 1] + gj[Ci] * dj[s - 1] + gk[Ci] * dk[s - 1];
+    const double PhiUstar = pos ? phi[Di] - 2.0 * gd : phi[Di] + 2.0 * gd;       /* :67-75 */
+    const double PhiU = mx2(phimin[Ci], mn2(PhiUstar, phimax[Ci]));             /* :77 */
+    double rj = (phi[Ci] - PhiU) / (phi[Di] - phi[Ci] + 1.0e-16);               /* :79 */
+    const double gD = 0.5 * fj * (1.0 + fj), gU = 0.5 * fj * (1.0 - fj);        /* :81-82 */
+    double Phij = 0.0, alfa, beta1, beta2;
+    switch (blenda) {                                                   /* :84-116 */
+    case CS_CDS: Phij = fj; break;
+    case CS_QUICK:
+        rj = (phi[Ci] - PhiU) / (phi[Di] - phi[Ci] + 1.0e-12);
+        alfa = 0.0;
+        Phij = (gD - alfa) + (gU + alfa) * rj;
+        break;
+    case CS_LUDS:
+        alfa = 0.5 * fj * (1 + fj);
+        Phij = (gD - alfa) + (gU + alfa) * rj;
+        break;
+    case CS_MINMOD: Phij = fj * mx2(0.0, mn2(rj, 1.0)); break;
+    case CS_SMART:
+        beta1 = 3.0;
+        beta2 = 1.0;
+        Phij = mx2(0.0, mn3((beta1 - 1.0) * rj, gD + gU * rj, beta2));
+        break;
+    case CS_AVL_SMART:
+        beta1 = 1.0 + fj * (2.0 + fj);
+        beta2 = fj * (2.0 - fj);
+        Phij = mx2(0.0, mn3((beta1 - 1.0) * rj, gD + gU * rj, beta2));
+        break;
+    case CS_SUPERBEE: Phij = 0.5 * mx3(0.0, mn2(2.0 * rj, 1.0), mn2(rj, 2.0)); break;
+    default: break;                      /* GAMMA: the value is set below */
+    }
+    if (blenda != CS_GAMMA) return phi[Ci] + Phij * sign * (phi[nbc + c2] - phi[nbc + c1]);   /* :118 */
+    const double Beta = 0.1;                                            /* :120-134 */
+    const double PhiUg = pos ? 1.0 - (phi[Di] - phi[Ci]) / (2.0 * gd) : 1.0 + (phi[Di] - phi[Ci]) / (2.0 * gd);
+    const double GammaC = PhiUg / Beta;
+    if (PhiUg < Beta && PhiUg > 0.0)
+        return (1.0 - GammaC * (1.0 - f[s - 1])) * phi[Ci] + GammaC * (1.0 - f[s - 1]) * phi[Di];
+    if (PhiUg < 1.0 && PhiUg >= Beta) return f[s - 1] * phi[Ci] + (1.0 - f[s - 1]) * phi[Di];
+    return phi[Ci];
+}
 
 int oracle_newuvw(int nc, int nbc, int ns, int nonz, int var, int convec, int diffus, int cross, int inert,
                   const int *sidec, const int *sidaij, const int *adia, const int *typebc, const int *material,
                   const int *blend, const int *blend_tem, const double *urfc, const double *f, const double *fF,
-                  const double *scoef, const double *flux_u, const double *si, const double *sj, const double *sk,
-                  const double *di, const double *dj, const double *dk, const double *volume, const double *rho_n,
+                  const double *scoef, const double *flux_u, const double *flux, const double *si,
+                  const double *sj, const double *sk, const double *di, const double *dj, const double *dk,
+                  const double *volume, const double *rho_n,
                   const double *rho_o, const double *rho_oo, const double *ui_n, const double *ui_o,
                   const double *ui_oo, const double *source, const double *pi, const double *dUidi,
                   const double *dUidj, const double *dUidk, const double *dUjdi, const double *dUkdi,
@@ -264,12 +365,33 @@ int oracle_newuvw(int nc, int nbc, int ns, int nonz, int var, int convec, int di
     for (int c = 0; c < nc; ++c) b[c] = 0.0;                            /* :149 */
     for (int c = 0; c < nall; ++c) { cbuf[c] = 0.0; xbuf[c] = 0.0; }   /* :175-176 */
 
-    for (int s = 1; s <= ns; ++s) {                                     /* :205-240 */
+    /* PHImax / PHImin of rhoUi when any face takes a ConvScheme scheme
+     * (:167-172; CalMinMax is called when any material's BLEND /= NO) */
+    double *phimax = NULL, *phimin = NULL;
+    for (int s = 1; s <= ns; ++s) {
         const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
-        const double rhoUiS = f[s - 1] * C(rhoui, c1) + (1.0 - f[s - 1]) * C(rhoui, c2);
         if (MAT(blend, c1) != BL_NO || MAT(blend, c2) != BL_NO) {
             const int bl = MAT(blend, c1) > MAT(blend, c2) ? MAT(blend, c1) : MAT(blend, c2);
-            if (bl != BL_YES) return -1;                                /* ConvScheme: not covered */
+            if (!blend_covered(bl)) {                                   /* NO next to YES: Phij undefined */
+                free(phimax);
+                free(phimin);
+                return -1;
+            }
+            if (bl != BL_YES && !phimax) {
+                phimax = (double *)malloc(sizeof(double) * nall);
+                phimin = (double *)malloc(sizeof(double) * nall);
+                oracle_calminmax(nc, nbc, ns, sidec, typebc, rhoui, phimax, phimin);
+            }
+        }
+    }
+
+    for (int s = 1; s <= ns; ++s) {                                     /* :205-240 */
+        const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
+        double rhoUiS = f[s - 1] * C(rhoui, c1) + (1.0 - f[s - 1]) * C(rhoui, c2);
+        if (MAT(blend, c1) != BL_NO || MAT(blend, c2) != BL_NO) {       /* :213-216 */
+            const int bl = MAT(blend, c1) > MAT(blend, c2) ? MAT(blend, c1) : MAT(blend, c2);
+            rhoUiS = conv_scheme(rhoUiS, s, c1, c2, nbc, rhoui, dUidi, dUidj, dUidk, di, dj, dk, f, flux, phimax,
+                                 phimin, bl);
         }
         const double F = flux_u[s - 1];
         if (c2 > 0) {
@@ -362,6 +484,8 @@ int oracle_newuvw(int nc, int nbc, int ns, int nonz, int var, int convec, int di
         b[c - 1] = b[c - 1] + aval[adia[c - 1] - 1] * (1.0 - urf) * C(rhoui, c) / urf;
         aval[adia[c - 1] - 1] = aval[adia[c - 1] - 1] / urf;
     }
+    free(phimax);
+    free(phimin);
     return 0;
 #undef MAT
 #undef C
@@ -375,9 +499,11 @@ int oracle_newuvw(int nc, int nbc, int ns, int nonz, int var, int convec, int di
  *   :100      rhoZ = rho^n Z^n on (-NbC:NC)
  *   :104-107  Abou = 0, Aval = 0, b = 0
  *   :134-135  C = 0, X = 0
+ *   :126-131  PHImax / PHImin of rhoZ (CalMinMax) when a face takes a
+ *             ConvScheme scheme
  *   :155-189  convection face loop: rhoZS = f rhoZ(c1) + (1-f) rhoZ(c2),
- *             left as it is by ConvScheme for BLEND_TEM = YES; C, and the
- *             upwind part X when BLEND_TEM /= NO
+ *             then ConvScheme's face value (kept as it is for YES); C,
+ *             and the upwind part X when BLEND_TEM /= NO
  *   :201-207  CONVEC = FI: b += URFC_Tem(material) (C - X)
  *   :217-219  X = 0 on the cells
  *   :224-341  diffusion face loop: Fex = VISeff grad(Z)_f . S,
@@ -390,14 +516,15 @@ int oracle_newuvw(int nc, int nbc, int ns, int nonz, int var, int convec, int di
  *   :386-387  under-relaxation with Zmix%URF
  *
  * VISeff = VIScZmix (the default of :236-242).  Per-material BLEND_TEM and
- * URFC_Tem at [m-1].  Returns -1 where a face's max(BLEND_TEM) other than
- * YES calls a ConvScheme branch not covered; abou is (-NbC:-1); cbuf, xbuf
- * hold NbC+NC+1 doubles.  Returns 0.
+ * URFC_Tem at [m-1]; flux as for oracle_newuvw.  Returns -1 where a face's
+ * max(BLEND_TEM) is NO next to YES or an uncovered code; abou is
+ * (-NbC:-1); cbuf, xbuf hold NbC+NC+1 doubles.  Returns 0.
  */
 int oracle_calczmix(int nc, int nbc, int ns, int nonz, int convec, int diffus, int cross, int inert,
                     const int *sidec, const int *sidaij, const int *adia, const int *typebc, const int *material,
                     const int *blend_tem, const double *urfc_tem, const double *f, const double *fF,
-                    const double *scoef, const double *flux_u, const double *sx, const double *sy, const double *sz,
+                    const double *scoef, const double *flux_u, const double *flux, const double *sx,
+                    const double *sy, const double *sz,
                     const double *dx, const double *dy, const double *dz, const double *volume,
                     const double *rho_n, const double *rho_o, const double *rho_oo, const double *z_n,
                     const double *z_o, const double *z_oo, const double *source, const double *dZdx,
@@ -413,13 +540,34 @@ int oracle_calczmix(int nc, int nbc, int ns, int nonz, int convec, int diffus, i
     for (int c = 0; c < nc; ++c) b[c] = 0.0;                            /* :107 */
     for (int c = 0; c < nall; ++c) { cbuf[c] = 0.0; xbuf[c] = 0.0; }   /* :134-135 */
 
+    /* PHImax / PHImin of rhoZ when any face takes a ConvScheme scheme
+     * (:126-131) */
+    double *phimax = NULL, *phimin = NULL;
+    for (int s = 1; s <= ns; ++s) {
+        const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
+        if (MAT(blend_tem, c1) != BL_NO || MAT(blend_tem, c2) != BL_NO) {
+            const int bl = MAT(blend_tem, c1) > MAT(blend_tem, c2) ? MAT(blend_tem, c1) : MAT(blend_tem, c2);
+            if (!blend_covered(bl)) {                                   /* NO next to YES: Phij undefined */
+                free(phimax);
+                free(phimin);
+                return -1;
+            }
+            if (bl != BL_YES && !phimax) {
+                phimax = (double *)malloc(sizeof(double) * nall);
+                phimin = (double *)malloc(sizeof(double) * nall);
+                oracle_calminmax(nc, nbc, ns, sidec, typebc, rhoz, phimax, phimin);
+            }
+        }
+    }
+
     for (int s = 1; s <= ns; ++s) {                                     /* :155-189 */
         const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
-        const double rhoZS = f[s - 1] * C(rhoz, c1) + (1.0 - f[s - 1]) * C(rhoz, c2);
+        double rhoZS = f[s - 1] * C(rhoz, c1) + (1.0 - f[s - 1]) * C(rhoz, c2);
         const int up = MAT(blend_tem, c1) != BL_NO || MAT(blend_tem, c2) != BL_NO;
-        if (up) {
+        if (up) {                                                       /* :163-166 */
             const int bl = MAT(blend_tem, c1) > MAT(blend_tem, c2) ? MAT(blend_tem, c1) : MAT(blend_tem, c2);
-            if (bl != BL_YES) return -1;                                /* ConvScheme: not covered */
+            rhoZS = conv_scheme(rhoZS, s, c1, c2, nbc, rhoz, dZdx, dZdy, dZdz, dx, dy, dz, f, flux, phimax, phimin,
+                                bl);
         }
         const double F = flux_u[s - 1];
         if (c2 > 0) {
@@ -503,6 +651,8 @@ int oracle_calczmix(int nc, int nbc, int ns, int nonz, int convec, int diffus, i
         b[c - 1] = b[c - 1] + aval[adia[c - 1] - 1] * (1.0 - urf) * C(rhoz, c) / urf;
         aval[adia[c - 1] - 1] = aval[adia[c - 1] - 1] / urf;
     }
+    free(phimax);
+    free(phimin);
     return 0;
 #undef MAT
 #undef C

@@ -200,11 +200,13 @@ class Oracle(_Solvers):
         of meshgen.uvw_fields and the flags of meshgen.uvw_params.  Returns
         dict aval (NONZ), b (NC), abou (NbC), asave ((-NbC:NC), cells 1..NC
         written for var = 1, zero elsewhere), rhoui ((-NbC:NC)); raises
-        ValueError for a BLEND this restatement does not cover."""
+        ValueError for a BLEND this restatement does not cover (NO next to
+        YES).  uf["flux"], the mass flux whose sign picks ConvScheme's
+        upwind cell, defaults to uf["flux_u"]."""
         fn = self.lib.oracle_newuvw
         if not getattr(fn, "_typed", False):
             fn.restype = C.c_int
-            fn.argtypes = [C.c_int] * 9 + [_i32p] * 7 + [_f64p] * 27 + [C.c_double] * 3 + [_f64p] * 7
+            fn.argtypes = [C.c_int] * 9 + [_i32p] * 7 + [_f64p] * 28 + [C.c_double] * 3 + [_f64p] * 7
             fn._typed = True
         var = int(params["var"])
         a = _uvw_args(case, uf, var)
@@ -220,7 +222,8 @@ class Oracle(_Solvers):
         rc = fn(case.nc, case.nbc, case.ns, case.nonz, var, params["convec"], params["diffus"], params["cross"],
                 params["inert"], c32(case.sidec.T), c32(case.sidaij.T), c32(case.adia), c32(case.typebc),
                 c32(uf["material"]), c32(params["blend"]), c32(params["blend_tem"]), c64(params["urfc"]),
-                c64(case.f), c64(uf["ff"]), c64(uf["scoef"]), c64(a["flux_u"]), *(c64(x) for x in a["S"]),
+                c64(case.f), c64(uf["ff"]), c64(uf["scoef"]), c64(a["flux_u"]), c64(uf.get("flux", a["flux_u"])),
+                *(c64(x) for x in a["S"]),
                 *(c64(x) for x in a["D"]), c64(case.volume), c64(a["rho_n"]), c64(a["rho_o"]), c64(a["rho_oo"]),
                 c64(a["ui_n"]), c64(a["ui_o"]), c64(a["ui_oo"]), c64(a["source"]), c64(a["pi"]),
                 *(c64(x) for x in a["g"]), float(params["visc"]), float(params["dt"]), float(params["urf"]),
@@ -235,11 +238,12 @@ def _calczmix(self, case, zf, params):
     Process/CalcZmix.f90:100-387) of a meshgen.CalcPSCase with the inputs of
     meshgen.zmix_fields; params as meshgen.uvw_params with BLEND_TEM in
     "blend_tem", URFC_Tem in "urfc", VIScZmix in "visc", Zmix%URF in "urf".
-    Returns dict aval, b, abou (NbC), rhoz ((-NbC:NC))."""
+    Returns dict aval, b, abou (NbC), rhoz ((-NbC:NC)); zf["flux"] as for
+    newuvw."""
     fn = self.lib.oracle_calczmix
     if not getattr(fn, "_typed", False):
         fn.restype = C.c_int
-        fn.argtypes = [C.c_int] * 8 + [_i32p] * 6 + [_f64p] * 22 + [C.c_double] * 3 + [_f64p] * 6
+        fn.argtypes = [C.c_int] * 8 + [_i32p] * 6 + [_f64p] * 23 + [C.c_double] * 3 + [_f64p] * 6
         fn._typed = True
     nall = case.nbc + case.nc + 1
     c32 = lambda x: np.ascontiguousarray(x, np.int32)     # noqa: E731
@@ -252,7 +256,8 @@ def _calczmix(self, case, zf, params):
     rc = fn(case.nc, case.nbc, case.ns, case.nonz, params["convec"], params["diffus"], params["cross"],
             params["inert"], c32(case.sidec.T), c32(case.sidaij.T), c32(case.adia), c32(case.typebc),
             c32(zf["material"]), c32(params["blend_tem"]), c64(params["urfc"]), c64(case.f), c64(zf["ff"]),
-            c64(zf["scoef"]), c64(zf["flux_u"]), c64(case.sx), c64(case.sy), c64(case.sz), c64(case.dx),
+            c64(zf["scoef"]), c64(zf["flux_u"]), c64(zf.get("flux", zf["flux_u"])), c64(case.sx), c64(case.sy),
+            c64(case.sz), c64(case.dx),
             c64(case.dy), c64(case.dz), c64(case.volume), c64(case.rho_n), c64(case.rho_o), c64(case.rho_oo),
             c64(zf["z"]), c64(zf["z_o"]), c64(zf["z_oo"]), c64(zf["src_z"]), c64(zf["zx"]), c64(zf["zy"]),
             c64(zf["zz"]), float(params["visc"]), float(params["dt"]), float(params["urf"]), aval, b, abou, rhoz,
@@ -274,7 +279,21 @@ def _zmix_update(self, case, rhoz, rho_n, z_n):
     return z
 
 
+def _calminmax(self, case, phi):
+    """Process/CalMinMax.f90 on phi ((-NbC:NC)): returns (PHImax, PHImin)."""
+    fn = self.lib.oracle_calminmax
+    if not getattr(fn, "_typed", False):
+        fn.restype = None
+        fn.argtypes = [C.c_int] * 3 + [_i32p] * 2 + [_f64p] * 3
+        fn._typed = True
+    mx, mn = np.zeros(len(phi)), np.zeros(len(phi))
+    fn(case.nc, case.nbc, case.ns, np.ascontiguousarray(case.sidec.T, np.int32),
+       np.ascontiguousarray(case.typebc, np.int32), np.ascontiguousarray(phi, np.float64), mx, mn)
+    return mx, mn
+
+
 Oracle.calczmix = _calczmix
+Oracle.calminmax = _calminmax
 Oracle.zmix_update = _zmix_update
 
 

@@ -19,6 +19,9 @@ reference holds no vectors for it, so the restatement is pinned by:
         tau = G + G^T - 2/3 tr(G) I on a single interior face;
       - inertia and relaxation: the diagonal-only system solves to the
         Euler / BDF2 step blended with the old value by URF;
+  * ConvScheme's face schemes (Process/ConvScheme.f90) against their
+    textbook forms on a uniform mesh, the direction taken from the mass
+    flux, CDS = central on any mesh, and CalMinMax against numpy;
   * the BLEND values it refuses.
 """
 import dataclasses
@@ -255,15 +258,194 @@ def test_inertia_relaxation_closed_form(pkg, oracle, inert):
     np.testing.assert_allclose(out["asave"][cells], V, rtol=1e-15)
 
 
+CDS, QUICK, LUDS, MINMOD, SMART, AVL_SMART, SUPERBEE, GAMMA = range(20, 28)   # Process/IniPar.f90:42-49
+SCHEMES = [CDS, QUICK, LUDS, MINMOD, SMART, AVL_SMART, SUPERBEE, GAMMA]
+
+
+def one_face_case(pkg, triple, flux_sign, mass_sign=None):
+    """A uniform 8 x 3 x 3 box (f = 1/2 on interior faces, spacing h in x)
+    with Flux_u nonzero (flux_sign) on one interior x-face only, between the
+    cells at x index 3 (c1) and 4 (c2), and a field phi of x index alone:
+    (phi_U, phi_C, phi_D) = triple on (2, 3, 4) when the mass flux (of sign
+    mass_sign, default flux_sign) runs from c1 to c2, on (5, 4, 3) when it
+    runs back; other cells take phi_C.  The x gradient of every cell is the
+    central difference of its x neighbours, so ConvScheme's extrapolated
+    upstream value phi_D -/+ 2 grad . d lands on phi_U.  Returns (case,
+    face, c1, c2, phi on (-NbC:NC), d phi/dx)."""
+    mg = pkg.meshgen
+    case = mg.calcps_case((8, 3, 3), seed=5, ratio=1.0, perturb=0.0)
+    nb = case.nbc
+    mass_sign = flux_sign if mass_sign is None else mass_sign
+    h = 2.0 * np.pi / 8
+    ix = np.clip(np.floor(case.xc[:, 0] / h).astype(int), 0, 7)
+    table = np.full(8, float(triple[1]))
+    idx = (2, 3, 4) if mass_sign > 0 else (5, 4, 3)
+    for i, v in zip(idx, triple):
+        table[i] = v
+    phi = table[ix]
+    grad = (table[np.minimum(ix + 1, 7)] - table[np.maximum(ix - 1, 0)]) / (2.0 * h)
+    c1s, c2s = case.sidec[1], case.sidec[2]
+    inner = c2s > 0
+    mid = lambda c: (ix[nb + c] , np.rint(case.xc[nb + c, 1] * 1e6), np.rint(case.xc[nb + c, 2] * 1e6))  # noqa: E731
+    cand = [s for s in np.nonzero(inner & (np.abs(case.sx) > 0))[0]
+            if ix[nb + c1s[s]] == 3 and ix[nb + c2s[s]] == 4]
+    face = int(cand[len(cand) // 2])
+    return case, face, int(c1s[face]), int(c2s[face]), phi, grad
+
+
+def _psi(scheme, r):
+    """The limiters in the textbook TVD form phi_f = phi_C + psi(r)/2
+    (phi_D - phi_C), r = (phi_C - phi_U) / (phi_D - phi_C), on a uniform
+    mesh: QUICK 3/4 + r/4, linear upwind r, minmod, superbee (Roe), SMART
+    (Gaskell & Lau) min(4r, 3/4 + r/4, 2), central 1."""
+    return {CDS: 1.0, QUICK: 0.75 + 0.25 * r, LUDS: r, MINMOD: max(0.0, min(r, 1.0)),
+            SUPERBEE: max(0.0, min(2 * r, 1.0), min(r, 2.0)), SMART: max(0.0, min(4 * r, 0.75 + 0.25 * r, 2.0)),
+            YES: 1.0}[scheme]
+
+
+def expected_face_value(scheme, triple):
+    """phi_f of a scheme for (phi_U, phi_C, phi_D) on a uniform mesh, from
+    the textbook forms: _psi; AVL-SMART equal to SMART wherever SMART is not
+    bounded by 2 (the two share the QUICK branch), upwind for r <= 0, and
+    bounded by fj (2 - fj) = 3/4 of the step above; the Gamma scheme (Jasak)
+    on the normalised phi~_C = (phi_C - phi_U) / (phi_D - phi_U): upwind
+    outside (0, 1), central on [beta, 1), the blend (1 - g/2) phi_C +
+    g/2 phi_D with g = phi~_C / beta below beta = 0.1."""
+    pu, pc, pd = triple
+    r = (pc - pu) / (pd - pc)
+    if scheme == GAMMA:
+        nc = (pc - pu) / (pd - pu)
+        if nc <= 0.0 or nc >= 1.0:
+            return pc
+        if nc >= 0.1:
+            return 0.5 * (pc + pd)
+        g = nc / 0.1
+        return (1 - g / 2) * pc + g / 2 * pd
+    if scheme == AVL_SMART:
+        half = 0.0 if r <= 0 else min(1.25 * r, 0.375 + 0.125 * r, 0.75)
+        return pc + half * (pd - pc)
+    return pc + 0.5 * _psi(scheme, r) * (pd - pc)
+
+
+# (phi_U, phi_C, phi_D): r = -0.5 (a maximum at C), 1 (linear), 2, 10, 0.05
+TRIPLES = [(-1.0, 0.0, -2.0), (1.0, 2.0, 3.0), (0.0, 2.0, 3.0), (0.0, 10.0, 11.0), (0.0, 0.05, 1.05)]
+
+
+def _uvw_face_value(pkg, oracle, scheme, triple, flux_sign, mass_sign=None, grad_scale=1.0):
+    mg = pkg.meshgen
+    case, face, c1, c2, phi, grad = one_face_case(pkg, triple, flux_sign, mass_sign)
+    grad = grad * grad_scale
+    n = case.nbc + case.nc + 1
+    case = dataclasses.replace(case, rho_n=np.ones(n), px=np.zeros(n))
+    uf = mg.uvw_fields(case, seed=9)
+    uf["u"] = phi
+    uf["ux"], uf["uy"], uf["uz"] = grad, np.zeros(n), np.zeros(n)
+    uf["u_o"], uf["u_oo"], uf["src_u"] = np.zeros(n), np.zeros(n), np.zeros(n)
+    F = 0.75 * flux_sign
+    uf["flux_u"] = np.zeros(case.ns)
+    uf["flux_u"][face] = F
+    if mass_sign is not None:
+        uf["flux"] = np.zeros(case.ns)
+        uf["flux"][face] = 0.5 * mass_sign
+    p = mg.uvw_params(var=1, visc=0.0, inert=0, urf=1.0, blend=[scheme], blend_tem=[NO], urfc=[1.0])
+    out = oracle.newuvw(case, uf, p)
+    # only this face convects: b(c1) = -F phi_f, b(c2) = +F phi_f
+    assert out["b"][c1 - 1] == pytest.approx(-out["b"][c2 - 1], rel=1e-15)
+    return out["b"][c2 - 1] / F
+
+
+@pytest.mark.parametrize("scheme", SCHEMES + [YES])
+@pytest.mark.parametrize("flux_sign", [1.0, -1.0])
+def test_conv_scheme_textbook_face_values(pkg, oracle, scheme, flux_sign):
+    """ConvScheme (Process/ConvScheme.f90:55-135) through the assembly: the
+    face value of each scheme, in both flow directions, for an extremum at
+    the upwind cell, a linear profile, and steep and shallow ones, against
+    the textbook forms of the schemes on a uniform mesh."""
+    for triple in TRIPLES:
+        got = _uvw_face_value(pkg, oracle, scheme, triple, flux_sign)
+        want = expected_face_value(scheme, triple) if scheme != YES else 0.5 * (triple[1] + triple[2])
+        assert got == pytest.approx(want, rel=1e-12, abs=1e-13), (scheme, triple)
+
+
+@pytest.mark.parametrize("scheme", [LUDS, SUPERBEE, GAMMA])
+def test_conv_scheme_direction_is_the_mass_flux(pkg, oracle, scheme):
+    """The upwind cell follows the module's mass flux Flux (ConvScheme.f90:
+    57), not Flux_u, which only weighs the face value into C: with the two of
+    opposite sign the face value is the one of the mass flux's direction."""
+    for triple in TRIPLES[2:]:
+        for fs in (1.0, -1.0):
+            got = _uvw_face_value(pkg, oracle, scheme, triple, fs, mass_sign=-fs)
+            assert got == pytest.approx(expected_face_value(scheme, triple), rel=1e-12), (scheme, triple, fs)
+
+
+@pytest.mark.parametrize("scheme", [QUICK, LUDS, SMART, SUPERBEE])
+def test_conv_scheme_upstream_value_is_bounded(pkg, oracle, scheme):
+    """The extrapolated upstream value is limited to the upwind cell's
+    PHImin .. PHImax (ConvScheme.f90:77): with the upwind cell's gradient
+    ten times too steep it overshoots far beyond its neighbours and is
+    brought back to the nearest bound -- here phi_U's own value, so the
+    face value is the textbook one again.  A ten times too flat gradient
+    stays inside the bounds and is not limited, so the value changes."""
+    for triple in ((0.0, 2.0, 3.0), (1.0, 2.0, 3.0), (0.0, 10.0, 11.0)):
+        for fs in (1.0, -1.0):
+            got = _uvw_face_value(pkg, oracle, scheme, triple, fs, grad_scale=10.0)
+            assert got == pytest.approx(expected_face_value(scheme, triple), rel=1e-12), (scheme, triple, fs)
+            flat = _uvw_face_value(pkg, oracle, scheme, triple, fs, grad_scale=0.1)
+            assert abs(flat - expected_face_value(scheme, triple)) > 1e-6
+
+
+@pytest.mark.parametrize("blend", [[CDS], [CDS, CDS], [GAMMA, YES]])
+def test_cds_keeps_the_convection_identity(pkg, oracle, blend):
+    """CDS gives the central value f phi1 + (1 - f) phi2 in both directions
+    on any mesh, so on the stretched, perturbed box (f != 1/2) the assembly
+    with CDS equals the one with YES (BLEND /= NO on every face either way,
+    so the implicit upwind part is the same).  With GAMMA it does not."""
+    mg = pkg.meshgen
+    nmat = len(blend)
+    case = mg.calcps_case((7, 6, 5), seed=41, bc=CLOSED_BC, face_seed=3)
+    case = dataclasses.replace(case, px=np.zeros_like(case.px))
+    uf = mg.uvw_fields(case, seed=42, nmaterials=nmat)
+    p = mg.uvw_params(nmaterials=nmat, visc=0.0, inert=0, urf=1.0, blend=blend, blend_tem=[YES] * nmat,
+                      urfc=[1.0] * nmat)
+    ref = oracle.newuvw(case, uf, dict(p, blend=[YES] * nmat))
+    out = oracle.newuvw(case, uf, p)
+    scale = np.abs(ref["b"]).max()
+    if blend[0] == CDS:
+        np.testing.assert_allclose(out["b"], ref["b"], rtol=0, atol=1e-13 * scale)
+        assert np.array_equal(out["aval"], ref["aval"])
+    else:
+        assert np.abs(out["b"] - ref["b"]).max() > 1e-6 * scale
+
+
+def test_calminmax_brute_force(pkg, oracle):
+    """Process/CalMinMax.f90: each cell's max / min over itself and the
+    cells across its interior and BUFFER faces, against numpy's
+    maximum.at / minimum.at over the face list."""
+    mg = pkg.meshgen
+    case = mg.calcps_case((6, 5, 4), seed=46, bc=CLOSED_BC, face_seed=4)
+    phi = np.random.default_rng(47).standard_normal(case.nbc + case.nc + 1)
+    mx, mn = oracle.calminmax(case, phi)
+    nb = case.nbc
+    c1, c2 = case.sidec[1].astype(int), case.sidec[2].astype(int)
+    use = (c2 > 0) | ((c2 < 0) & (case.typebc[nb + np.minimum(c2, 0)] == 11))
+    wmx, wmn = phi.copy(), phi.copy()
+    for a, b in ((c1, c2), (c2, c1)):
+        np.maximum.at(wmx, nb + a[use], phi[nb + b[use]])
+        np.minimum.at(wmn, nb + a[use], phi[nb + b[use]])
+    assert np.array_equal(mx, wmx) and np.array_equal(mn, wmn)
+    assert np.any(use & (c2 < 0)) and np.any(mx != phi)
+
+
 def test_blend_schemes_refused(pkg, oracle):
-    """A TVD face scheme (QUICK = 21) and YES next to NO at a face (where
-    the reference's ConvScheme leaves Phij undefined) are refused; YES on
-    both materials is not."""
+    """YES next to NO at a face (where the reference's ConvScheme leaves
+    Phij undefined) and codes that are no scheme are refused; YES on both
+    materials and every scheme are not."""
     mg = pkg.meshgen
     case = mg.calcps_case((6, 4, 3), seed=48)
     uf = mg.uvw_fields(case, seed=49, nmaterials=2)
-    with pytest.raises(ValueError):
-        oracle.newuvw(case, uf, mg.uvw_params(nmaterials=2, blend=[21, 21]))
-    with pytest.raises(ValueError):
-        oracle.newuvw(case, uf, mg.uvw_params(nmaterials=2, blend=[YES, NO]))
+    for bad in ([YES, NO], [28, 28], [19, 19]):
+        with pytest.raises(ValueError):
+            oracle.newuvw(case, uf, mg.uvw_params(nmaterials=2, blend=bad))
     oracle.newuvw(case, uf, mg.uvw_params(nmaterials=2, blend=[YES, YES]))
+    for sch in SCHEMES:
+        oracle.newuvw(case, uf, mg.uvw_params(nmaterials=2, blend=[sch, NO]))

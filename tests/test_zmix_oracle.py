@@ -7,14 +7,17 @@ on the two-cell mesh (every boundary type), the order of the two-level
 inertia product (V/dt Z_o rho_o, which differs from the momentum equation's
 V/dt rho_o U_o), the convection identity with the upwind matrix, the
 diffusion identity for a linear field, the inertia / relaxation closed form,
-the [0, 1] limiter after the solve, and the BLEND_TEM values it refuses.
+the [0, 1] limiter after the solve, ConvScheme's face schemes against their
+textbook forms (both directions, the mass flux's), and the BLEND_TEM values
+it refuses.
 """
 import dataclasses
 
 import numpy as np
 import pytest
 
-from test_newuvw_oracle import CLOSED_BC, DIFFUSIVE_BC, _apply, _hand_case, _hand_fields
+from test_newuvw_oracle import (CLOSED_BC, DIFFUSIVE_BC, GAMMA, SCHEMES, TRIPLES, _apply, _hand_case, _hand_fields,
+                                expected_face_value, one_face_case)
 
 YES, NO, LIN, PAR = 10, 11, 4, 5
 
@@ -173,12 +176,55 @@ def test_update_limits_to_unit_interval(pkg, oracle):
     assert oracle.zmix_update(case, rhoz, rho, z)[case.nbc + 1:].tolist() == [0.5, 0.5]
 
 
+def _zmix_face_value(pkg, oracle, scheme, triple, flux_sign, mass_sign=None):
+    mg = pkg.meshgen
+    case, face, c1, c2, phi, grad = one_face_case(pkg, triple, flux_sign, mass_sign)
+    n = case.nbc + case.nc + 1
+    case = dataclasses.replace(case, rho_n=np.ones(n))
+    zf = mg.zmix_fields(case, seed=10)
+    zf["z"] = phi
+    zf["zx"], zf["zy"], zf["zz"] = grad, np.zeros(n), np.zeros(n)
+    zf["z_o"], zf["z_oo"], zf["src_z"] = np.zeros(n), np.zeros(n), np.zeros(n)
+    F = 0.75 * flux_sign
+    zf["flux_u"] = np.zeros(case.ns)
+    zf["flux_u"][face] = F
+    if mass_sign is not None:
+        zf["flux"] = np.zeros(case.ns)
+        zf["flux"][face] = 0.5 * mass_sign
+    p = mg.uvw_params(visc=0.0, inert=0, urf=1.0, blend_tem=[scheme], urfc=[1.0])
+    out = oracle.calczmix(case, zf, p)
+    # b = C - X with the upwind part X of the same face (Flux_u's upwind cell)
+    up = phi[case.nbc + (c1 if F >= 0 else c2)]
+    assert out["b"][c1 - 1] == pytest.approx(-out["b"][c2 - 1], rel=1e-15)
+    return out["b"][c2 - 1] / F + up
+
+
+@pytest.mark.parametrize("scheme", SCHEMES + [YES])
+@pytest.mark.parametrize("flux_sign", [1.0, -1.0])
+def test_conv_scheme_textbook_face_values(pkg, oracle, scheme, flux_sign):
+    """ConvScheme through the scalar assembly (CalcZmix.f90:163-166, with
+    rhoZ and the gradients dZdx, dZdy, dZdz and Dx, Dy, Dz): each scheme's
+    face value against its textbook form, both directions."""
+    for triple in TRIPLES:
+        got = _zmix_face_value(pkg, oracle, scheme, triple, flux_sign)
+        want = expected_face_value(scheme, triple) if scheme != YES else 0.5 * (triple[1] + triple[2])
+        assert got == pytest.approx(want, rel=1e-12, abs=1e-13), (scheme, triple)
+
+
+def test_conv_scheme_direction_is_the_mass_flux(pkg, oracle):
+    for triple in TRIPLES[2:]:
+        for fs in (1.0, -1.0):
+            got = _zmix_face_value(pkg, oracle, GAMMA, triple, fs, mass_sign=-fs)
+            assert got == pytest.approx(expected_face_value(GAMMA, triple), rel=1e-12)
+
+
 def test_blend_tem_schemes_refused(pkg, oracle):
     mg = pkg.meshgen
     case = mg.calcps_case((5, 4, 3), seed=67)
     zf = mg.zmix_fields(case, seed=68, nmaterials=2)
-    with pytest.raises(ValueError):
-        oracle.calczmix(case, zf, mg.uvw_params(nmaterials=2, blend_tem=[22, 22]))
-    with pytest.raises(ValueError):
-        oracle.calczmix(case, zf, mg.uvw_params(nmaterials=2, blend_tem=[YES, NO]))
+    for bad in ([YES, NO], [28, 28], [12, 12]):
+        with pytest.raises(ValueError):
+            oracle.calczmix(case, zf, mg.uvw_params(nmaterials=2, blend_tem=bad))
     oracle.calczmix(case, zf, mg.uvw_params(nmaterials=2, blend_tem=[NO, NO], blend=[21, 21]))
+    for sch in SCHEMES:
+        oracle.calczmix(case, zf, mg.uvw_params(nmaterials=2, blend_tem=[sch, NO]))
