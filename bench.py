@@ -424,6 +424,16 @@ def workload_uvw(args, torch, pkg):
         return e0.elapsed_time(e1) / reps
     t_var = [event_ms(lambda v=v: dc.assemble_uvw(params[v], inputs[v], out=outs[v])) for v in range(3)]
     t_z = event_ms(lambda: dc.assemble_zmix(zparams, zinputs, out=zout))
+    # ConvScheme face schemes (CalMinMax pass + the face value in the cell
+    # kernel) on the same case: U and Zmix with SMART (24) and GAMMA (27)
+    t_tvd = {}
+    for name, code in (("SMART", 24), ("GAMMA", 27)):
+        pu = dict(params[0], blend=[code])
+        pz = dict(zparams, blend_tem=[code])
+        t_tvd[name] = {"u_ms": round(event_ms(lambda: dc.assemble_uvw(pu, inputs[0], out=outs[0])), 4),
+                       "zmix_ms": round(event_ms(lambda: dc.assemble_zmix(pz, zinputs, out=zout)), 4)}
+    dc.assemble_uvw(params[0], inputs[0], out=outs[0])
+    dc.assemble_zmix(zparams, zinputs, out=zout)
     nbytes = [uvw_bytes(case, v) for v in (1, 2, 3)]
     zb = zmix_bytes(case)
     m = case.matrix(outs[0]["aval"].cpu().numpy())
@@ -488,6 +498,7 @@ def workload_uvw(args, torch, pkg):
            "assembly_ms": [round(t, 4) for t in t_var], "assembly_bytes": nbytes,
            "assembly_GBs": [round(g, 1) for g in gbs],
            "zmix_assembly_ms": round(t_z, 4), "zmix_bytes": zb, "zmix_GBs": round(zb / (t_z * 1e-3) / 1e9, 1),
+           "conv_scheme_assembly": t_tvd,
            "roofline": {"bound": "hbm", "achieved": round(gbs[0], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(gbs[0] / HBM_PEAK_GBS, 4)},
            "step_sequential_ms": round(ms_seq, 2), "iters": its,

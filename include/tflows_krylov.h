@@ -308,8 +308,12 @@ int tflows_calcps_correct(tflows_calcps h, double dt, const double* pp, const do
  * Process/IniPar.f90:22-32; per-material arrays at [m - 1]); var selects
  * Si..Dk from the handle's Sx..Dz as Process/Processor.f90:314-340 does; the
  * cell arrays (-NbC:NC) rho_n/o/oo, Ui n/o/oo, source, Pi and the seven
- * gradients in newuvw's argument order (NewUVW.f90:1-41); Flux_u(NS).
- * Writes Aval(NONZ), b(NC), Abou(-NbC:-1), Asave (cells 1..NC of a
+ * gradients in newuvw's argument order (NewUVW.f90:1-41); Flux_u(NS); flux,
+ * the mass flux Flux(NS) whose sign picks ConvScheme's upwind cell
+ * (Process/ConvScheme.f90:55-65), or NULL for Flux_u.  BLEND may be YES, NO
+ * or a ConvScheme scheme (CDS 20 .. GAMMA 27, with PHImax/PHImin of
+ * Process/CalMinMax.f90); a face whose materials put YES next to NO is
+ * refused.  Writes Aval(NONZ), b(NC), Abou(-NbC:-1), Asave (cells 1..NC of a
  * (-NbC:NC) array, var = 1, may be NULL) and rhoUi(-NbC:NC).
  * tflows_calcps_uvw_update: Ui%n(1:NC) = rhoUi / rho%n (:526). */
 typedef struct tflows_uvw_params {
@@ -324,8 +328,9 @@ int tflows_calcps_uvw_assemble(tflows_calcps h, const tflows_uvw_params* p, cons
                                const double* rho_o, const double* rho_oo, const double* ui_n, const double* ui_o,
                                const double* ui_oo, const double* source, const double* pi, const double* dUidi,
                                const double* dUidj, const double* dUidk, const double* dUjdi, const double* dUkdi,
-                               const double* dUjdj, const double* dUkdk, const double* flux_u, double* aval,
-                               double* b, double* abou, double* asave, double* rhoui, int32_t mem, void* stream);
+                               const double* dUjdj, const double* dUkdk, const double* flux_u, const double* flux,
+                               double* aval, double* b, double* abou, double* asave, double* rhoui, int32_t mem,
+                               void* stream);
 int tflows_calcps_uvw_update(tflows_calcps h, const double* rhoui, const double* rho_n, double* ui_n, int32_t mem,
                              void* stream);
 
@@ -335,16 +340,16 @@ int tflows_calcps_uvw_update(tflows_calcps h, const double* rhoui, const double*
  * the face scheme, the upwind matrix and the upwind part), urfc = URFC_Tem,
  * visc = VIScZmix, urf = Zmix%URF, convec/diffus/cross/inert/dt as for the
  * momentum (var and blend are not read).  Cell arrays (-NbC:NC): rho_n/o/oo,
- * Z n/o/oo, the source and dZ/dx, dZ/dy, dZ/dz; Flux_u(NS).  Writes
- * Aval(NONZ), b(NC), Abou(-NbC:-1) and rhoZ(-NbC:NC), bit for bit the
- * reference's; BLEND_TEM schemes other than YES/NO are refused.
+ * Z n/o/oo, the source and dZ/dx, dZ/dy, dZ/dz; Flux_u(NS); flux as for the
+ * momentum.  Writes Aval(NONZ), b(NC), Abou(-NbC:-1) and rhoZ(-NbC:NC), bit
+ * for bit the reference's; BLEND_TEM takes the codes BLEND does.
  * tflows_calcps_zmix_update: Zmix%n(1:NC) = rhoZ / rho%n limited to [0, 1]
  * (:401-405). */
 int tflows_calcps_zmix_assemble(tflows_calcps h, const tflows_uvw_params* p, const double* rho_n,
                                 const double* rho_o, const double* rho_oo, const double* z_n, const double* z_o,
                                 const double* z_oo, const double* source, const double* dZdx, const double* dZdy,
-                                const double* dZdz, const double* flux_u, double* aval, double* b, double* abou,
-                                double* rhoz, int32_t mem, void* stream);
+                                const double* dZdz, const double* flux_u, const double* flux, double* aval,
+                                double* b, double* abou, double* rhoz, int32_t mem, void* stream);
 int tflows_calcps_zmix_update(tflows_calcps h, const double* rhoz, const double* rho_n, double* z_n, int32_t mem,
                               void* stream);
 

@@ -49,6 +49,11 @@ struct UvwFields {
     const double *ui_n, *ui_o, *ui_oo, *source, *pi;
     const double* g[7];                 // dUidi, dUidj, dUidk, dUjdi, dUkdi, dUjdj, dUkdk
     const double* flux_u;               // Flux_u(NS)
+    const double* flux = nullptr;       // the mass flux Flux(NS) whose sign picks ConvScheme's upwind
+                                        // cell; null: Flux_u
+    // PHImax / PHImin (-NbC:NC) of CalMinMax, set by the assembly itself
+    const double* phimax = nullptr;
+    const double* phimin = nullptr;
 };
 
 // Geometry the momentum assembly adds to the mesh (device).
@@ -128,6 +133,8 @@ private:
     std::vector<unsigned char> ukind_host_;
     std::vector<int> c1_host_, c2_host_;
     bool offdiag_shared_ = false;       // two faces write one off-diagonal
+    bool bcell_single_ = true;          // every boundary cell has at most one face
+    double* minmax_ = nullptr;          // PHImax, PHImin of ConvScheme ((-NbC:NC) each)
     int nmaterials_ = 0;
     unsigned pairs_[kMaxMaterials] = {};  // material pairs that meet at a face
 };

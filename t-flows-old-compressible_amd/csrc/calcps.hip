@@ -318,6 +318,9 @@ __global__ __launch_bounds__(kBlock) void k_cor_total(int parts, const double* _
 // ---------------------------------------------------------------------------
 constexpr int kFlagCN = 2, kFlagFI = 3, kInertLin = 4, kInertPar = 5, kBlendYes = 10,
               kBlendNo = 11;   // IniPar.f90:22-32
+// ConvScheme's face schemes (IniPar.f90:42-49)
+constexpr int kCds = 20, kQuick = 21, kLuds = 22, kMinmod = 23, kSmart = 24, kAvlSmart = 25, kSuperbee = 26,
+              kGamma = 27;
 
 struct UvwGeom {        // Si, Sj, Sk, Di, Dj, Dk of the component (Processor.f90:314-340)
     const double *si, *sj, *sk, *di, *dj, *dk;
@@ -328,6 +331,7 @@ struct UvwCell {
     double rho, ru;     // rho%n and rhoUi = rho%n Ui%n (:138)
     double g[7];        // the seven gradients
     int mat;            // material - 1
+    int idx;            // NbC + cell
 };
 
 template <int NG>
@@ -338,6 +342,7 @@ __device__ __forceinline__ UvwCell uvw_cell(const UvwMesh& U, const UvwFields& F
 #pragma unroll
     for (int k = 0; k < 7; ++k) v.g[k] = k < NG ? F.g[k][idx] : 0.0;
     v.mat = U.material[idx] - 1;
+    v.idx = idx;
     return v;
 }
 
@@ -352,14 +357,79 @@ struct UvwFace {
     double a12, a21, d12, d21;
 };
 
-template <bool SCALAR>
+// MAX / MIN as plain comparisons (the first argument is kept unless the
+// other is strictly beyond it), the same in the C restatement.
+__device__ __forceinline__ double cs_max(double a, double b) { return b > a ? b : a; }
+__device__ __forceinline__ double cs_min(double a, double b) { return b < a ? b : a; }
+
+// Process/ConvScheme.f90:55-135: the face value of Phi = rhoUi at face s for
+// scheme `blenda` (YES: the central value phis as it is).  The upwind cell C
+// and the downwind cell D follow the sign of the mass flux (:55-65); the
+// gradient of C is dotted with Di, Dj, Dk in the order of the call
+// (NewUVW.f90:214: dUidi, dUidj, dUidk; CalcZmix.f90:164: dZdx, dZdy, dZdz).
+__device__ __forceinline__ double conv_scheme(double phis, int s, const UvwCell& a, const UvwCell& b,
+                                              const UvwGeom& G, const CalcPSMesh& M, const UvwFields& F,
+                                              int blenda) {
+    if (blenda == kBlendYes) return phis;
+    const bool pos = (F.flux ? F.flux[s] : F.flux_u[s]) > 0.0;
+    const double fj = pos ? 1.0 - M.f[s] : M.f[s];
+    const UvwCell& Cc = pos ? a : b;
+    const UvwCell& Dc = pos ? b : a;
+    const double sign = pos ? 1.0 : -1.0;
+    const double gd = Cc.g[0] * G.di[s] + Cc.g[1] * G.dj[s] + Cc.g[2] * G.dk[s];
+    const double PhiUstar = pos ? Dc.ru - 2.0 * gd : Dc.ru + 2.0 * gd;                 // :67-75
+    const double PhiU = cs_max(F.phimin[Cc.idx], cs_min(PhiUstar, F.phimax[Cc.idx]));   // :77
+    double rj = (Cc.ru - PhiU) / (Dc.ru - Cc.ru + 1.0e-16);                             // :79
+    const double gD = 0.5 * fj * (1.0 + fj), gU = 0.5 * fj * (1.0 - fj);                // :81-82
+    double Phij = 0.0, alfa, beta1, beta2;
+    switch (blenda) {                                                                    // :84-114
+    case kCds: Phij = fj; break;
+    case kQuick:
+        rj = (Cc.ru - PhiU) / (Dc.ru - Cc.ru + 1.0e-12);
+        alfa = 0.0;
+        Phij = (gD - alfa) + (gU + alfa) * rj;
+        break;
+    case kLuds:
+        alfa = 0.5 * fj * (1 + fj);
+        Phij = (gD - alfa) + (gU + alfa) * rj;
+        break;
+    case kMinmod: Phij = fj * cs_max(0.0, cs_min(rj, 1.0)); break;
+    case kSmart:
+        beta1 = 3.0;
+        beta2 = 1.0;
+        Phij = cs_max(0.0, cs_min(cs_min((beta1 - 1.0) * rj, gD + gU * rj), beta2));
+        break;
+    case kAvlSmart:
+        beta1 = 1.0 + fj * (2.0 + fj);
+        beta2 = fj * (2.0 - fj);
+        Phij = cs_max(0.0, cs_min(cs_min((beta1 - 1.0) * rj, gD + gU * rj), beta2));
+        break;
+    case kSuperbee: Phij = 0.5 * cs_max(cs_max(0.0, cs_min(2.0 * rj, 1.0)), cs_min(rj, 2.0)); break;
+    default: break;                     // GAMMA: the value is set below
+    }
+    if (blenda != kGamma) return Cc.ru + Phij * sign * (b.ru - a.ru);                    // :118
+    const double Beta = 0.1;                                                             // :120-134
+    const double PhiUg = pos ? 1.0 - (Dc.ru - Cc.ru) / (2.0 * gd) : 1.0 + (Dc.ru - Cc.ru) / (2.0 * gd);
+    const double GammaC = PhiUg / Beta;
+    if (PhiUg < Beta && PhiUg > 0.0)
+        return (1.0 - GammaC * (1.0 - M.f[s])) * Cc.ru + GammaC * (1.0 - M.f[s]) * Dc.ru;
+    if (PhiUg < 1.0 && PhiUg >= Beta) return M.f[s] * Cc.ru + (1.0 - M.f[s]) * Dc.ru;
+    return Cc.ru;
+}
+
+// TVD: some face takes a ConvScheme scheme other than YES (without one,
+// every face keeps the central value and the kernel needs no more registers
+// than that).
+template <bool SCALAR, bool TVD>
 __device__ __forceinline__ UvwFace uvw_face(const CalcPSMesh& M, const UvwMesh& U, const UvwGeom& G,
                                             const UvwParams& P, const UvwFields& F, int s, const UvwCell& a,
                                             const UvwCell& b) {
     UvwFace q;
     const double Fu = F.flux_u[s];
     const double fs = M.f[s];
-    const double rhoUiS = fs * a.ru + (1.0 - fs) * b.ru;                             // :211
+    double rhoUiS = fs * a.ru + (1.0 - fs) * b.ru;                                   // :211
+    if (TVD && (P.blend[a.mat] != kBlendNo || P.blend[b.mat] != kBlendNo))            // :213-216
+        rhoUiS = conv_scheme(rhoUiS, s, a, b, G, M, F, max(P.blend[a.mat], P.blend[b.mat]));
     q.conv = Fu * rhoUiS;
     q.upwind = P.blend_tem[a.mat] != kBlendNo || P.blend_tem[b.mat] != kBlendNo;
     q.up = Fu * (Fu < 0 ? b.ru : a.ru);
@@ -397,7 +467,49 @@ __device__ __forceinline__ UvwCell pick(bool first, const UvwCell& x, const UvwC
 #pragma unroll
     for (int k = 0; k < 7; ++k) v.g[k] = k < NG ? (first ? x.g[k] : y.g[k]) : 0.0;
     v.mat = first ? x.mat : y.mat;
+    v.idx = first ? x.idx : y.idx;
     return v;
+}
+
+// CalMinMax (Process/CalMinMax.f90:21-37) of rhoUi = rho%n Ui%n: an interior
+// cell over itself and the cells across its interior and BUFFER faces, in
+// ascending face order (the order in which the reference's face loop
+// reaches it); a boundary cell over itself and, on a BUFFER face, the face's
+// cell c1 (a boundary cell has one face, checked on the host).
+__global__ __launch_bounds__(kBlock) void k_uvw_minmax(CalcPSMesh M, UvwMesh U, UvwFields F,
+                                                        double* __restrict__ pmax, double* __restrict__ pmin) {
+    const int nb = M.nbc;
+    const int stride = gridDim.x * kBlock;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < M.nc; i += stride) {
+        const int ic = nb + i + 1;
+        const double v = F.rho_n[ic] * F.ui_n[ic];
+        double mx = v, mn = v;
+        for (int e = M.cptr[i]; e < M.cptr[i + 1]; ++e) {
+            const int2 ie = U.inc[e];
+            const int o = ie.y;
+            if (o > 0 || U.ukind[ie.x >> 1] == kUvwBuffer) {
+                const double w = F.rho_n[nb + o] * F.ui_n[nb + o];
+                mx = cs_max(mx, w);
+                mn = cs_min(mn, w);
+            }
+        }
+        pmax[ic] = mx;
+        pmin[ic] = mn;
+    }
+    for (int s = blockIdx.x * kBlock + threadIdx.x; s < M.ns; s += stride) {
+        const int c2 = M.c2[s];
+        if (c2 >= 0) continue;
+        const int i2 = nb + c2, i1 = nb + M.c1[s];
+        const double v = F.rho_n[i2] * F.ui_n[i2];
+        double mx = v, mn = v;
+        if (U.ukind[s] == kUvwBuffer) {
+            const double w = F.rho_n[i1] * F.ui_n[i1];
+            mx = cs_max(mx, w);
+            mn = cs_min(mn, w);
+        }
+        pmax[i2] = mx;
+        pmin[i2] = mn;
+    }
 }
 
 // The off-diagonal pair of every interior face, (0 - A12) - D12 at
@@ -442,7 +554,7 @@ __global__ __launch_bounds__(kBlock) void k_uvw_faces(CalcPSMesh M, UvwMesh U, U
 // k_uvw_faces, except where several faces write one position (SHARED): then
 // each off-diagonal of row c is accumulated, in face order, on a zeroed Aval
 // by the thread of cell c.
-template <bool SHARED, bool SCALAR>
+template <bool SHARED, bool SCALAR, bool TVD>
 __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, UvwGeom G, UvwParams P,
                                                        UvwFields F, double* __restrict__ aval,
                                                        double* __restrict__ b, double* __restrict__ abou,
@@ -470,7 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, U
             const int other = ie.y;
             const UvwCell oth = uvw_cell<NG>(U, F, nb + other);
             const UvwFace f =
-                uvw_face<SCALAR>(M, U, G, P, F, s, pick<NG>(!second, own, oth), pick<NG>(!second, oth, own));
+                uvw_face<SCALAR, TVD>(M, U, G, P, F, s, pick<NG>(!second, own, oth), pick<NG>(!second, oth, own));
             if (!second) {
                 Cs = Cs - f.conv;
                 if (f.upwind) Xc = Xc - f.up;
@@ -599,6 +711,7 @@ CalcPS::CalcPS(int nc, int nbc, int ns, int nonz, const int* sidec, const int* s
     fetch(h_type.data(), typebc, sizeof(int) * h_type.size());
     std::vector<int> c1(ns), c2(ns), s1(ns, 0), s2(ns, 0), ad0(nc);
     std::vector<unsigned char> kind(ns, kNone), ukind(ns, kUvwNone);
+    std::vector<int> bfaces(nbc + 1, 0);     // faces of each boundary cell
     std::vector<char> covered(nonz, 0);
     for (int i = 0; i < nc; ++i) {
         if (h_adia[i] < 1 || h_adia[i] > nonz - 1) throw std::invalid_argument("Adia out of range");
@@ -634,6 +747,7 @@ CalcPS::CalcPS(int nc, int nbc, int ns, int nonz, const int* sidec, const int* s
                      : t == kBcOutflow ? kUvwOutflow
                      : t == kBcBuffer ? kUvwBuffer
                      : kUvwNone;
+            if (bfaces[-c2[q]]++ > 0) bcell_single_ = false;
         }
         count[c1[q]]++;
         if (kind[q] == kInterior) count[c2[q]]++;
@@ -720,6 +834,7 @@ CalcPS::~CalcPS() {
     (void)hipFree(fa12_);
     (void)hipFree(part_);
     (void)hipFree(out2_);
+    (void)hipFree(minmax_);
     for (void* p : staging_) (void)hipFree(p);
 }
 
@@ -805,19 +920,37 @@ void CalcPS::assemble_equation(const UvwParams& P, const UvwFields& F, double* a
     if (!(P.dt != 0.0) && (P.inert == kInertLin || P.inert == kInertPar))
         throw std::invalid_argument("dt must be non-zero");
     if (!(P.urf != 0.0)) throw std::invalid_argument("URF must be non-zero");
-    // ConvScheme (Process/ConvScheme.f90) leaves rhoUiS as it is only for
-    // BLEND = YES; a face whose materials give max(BLEND) another value
-    // would need a TVD face value (or, for YES next to NO, an undefined one)
+    // ConvScheme (Process/ConvScheme.f90) runs where max(BLEND) of a face's
+    // materials is not NO: YES keeps the central value, CDS .. GAMMA form
+    // the face value from PHImax / PHImin of CalMinMax.  YES next to NO (max
+    // = NO, which leaves Phij undefined in the reference) and other codes
+    // are refused.
+    bool tvd = false;
     for (int a = 0; a < nmaterials_; ++a)
         for (int c = 0; c < nmaterials_; ++c)
             if (pairs_[a] >> c & 1u) {
                 const int ba = P.blend[a], bc = P.blend[c];
-                if ((ba != kBlendNo || bc != kBlendNo) && std::max(ba, bc) != kBlendYes)
-                    throw std::invalid_argument(scalar ? "BLEND_TEM other than YES/NO (a ConvScheme face scheme) "
-                                                         "is not covered by the scalar assembly"
-                                                       : "BLEND other than YES/NO (a ConvScheme face scheme) is not "
-                                                         "covered by the momentum assembly");
+                if (ba == kBlendNo && bc == kBlendNo) continue;
+                const int bl = std::max(ba, bc);
+                if (bl != kBlendYes && (bl < kCds || bl > kGamma))
+                    throw std::invalid_argument(scalar ? "BLEND_TEM of a face is neither YES nor a ConvScheme "
+                                                         "scheme (20-27), or YES next to NO"
+                                                       : "BLEND of a face is neither YES nor a ConvScheme scheme "
+                                                         "(20-27), or YES next to NO");
+                tvd = tvd || bl != kBlendYes;
             }
+    UvwFields Fx = F;
+    if (tvd) {
+        if (!bcell_single_)
+            throw std::invalid_argument("ConvScheme schemes need every boundary cell on one face");
+        const size_t ncell = (size_t)nc_ + nbc_ + 1;
+        if (!minmax_) TF_CHECK(hipMalloc(&minmax_, sizeof(double) * 2 * ncell));
+        const int work = std::max(nc_, ns_);
+        hipLaunchKernelGGL(k_uvw_minmax, dim3(std::max(1, std::min(2048, (work + kBlock - 1) / kBlock))),
+                           dim3(kBlock), 0, s, M_, U_, F, minmax_, minmax_ + ncell);
+        Fx.phimax = minmax_;
+        Fx.phimin = minmax_ + ncell;
+    }
     const UvwGeom G = P.var == 1 ? UvwGeom{M_.sx, M_.sy, M_.sz, M_.dx, M_.dy, M_.dz}
                     : P.var == 2 ? UvwGeom{M_.sy, M_.sx, M_.sz, M_.dy, M_.dx, M_.dz}
                                  : UvwGeom{M_.sz, M_.sx, M_.sy, M_.dz, M_.dx, M_.dy};
@@ -835,23 +968,22 @@ void CalcPS::assemble_equation(const UvwParams& P, const UvwFields& F, double* a
     // at most 2048 blocks, a multiple of 8 (the XCD-aware tile walk)
     const int tiles = (nc_ + kBlock - 1) / kBlock;
     const int grid = tiles < kNumXcd ? tiles : std::min(2048, (tiles + kNumXcd - 1) / kNumXcd * kNumXcd);
-    if (offdiag_shared_) {
-        if (scalar)
-            hipLaunchKernelGGL((k_uvw_cells<true, true>), dim3(grid), dim3(kBlock), 0, s, M_, U_, G, P, F, aval, b,
-                               abou, asave, rhoui);
-        else
-            hipLaunchKernelGGL((k_uvw_cells<true, false>), dim3(grid), dim3(kBlock), 0, s, M_, U_, G, P, F, aval, b,
-                               abou, asave, rhoui);
-    } else {
-        if (diff && ns_ > 0)
-            hipLaunchKernelGGL(k_uvw_faces, dim3((ns_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, M_, U_, P, F, aval,
-                               abou);
-        if (scalar)
-            hipLaunchKernelGGL((k_uvw_cells<false, true>), dim3(grid), dim3(kBlock), 0, s, M_, U_, G, P, F, aval, b,
-                               abou, asave, rhoui);
-        else
-            hipLaunchKernelGGL((k_uvw_cells<false, false>), dim3(grid), dim3(kBlock), 0, s, M_, U_, G, P, F, aval, b,
-                               abou, asave, rhoui);
+    if (!offdiag_shared_ && diff && ns_ > 0)
+        hipLaunchKernelGGL(k_uvw_faces, dim3((ns_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, M_, U_, P, Fx, aval,
+                           abou);
+    auto cells = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, M_, U_, G, P, Fx, aval, b, abou, asave, rhoui);
+    };
+    const int form = (offdiag_shared_ ? 4 : 0) + (scalar ? 2 : 0) + (tvd ? 1 : 0);
+    switch (form) {
+    case 0: cells(k_uvw_cells<false, false, false>); break;
+    case 1: cells(k_uvw_cells<false, false, true>); break;
+    case 2: cells(k_uvw_cells<false, true, false>); break;
+    case 3: cells(k_uvw_cells<false, true, true>); break;
+    case 4: cells(k_uvw_cells<true, false, false>); break;
+    case 5: cells(k_uvw_cells<true, false, true>); break;
+    case 6: cells(k_uvw_cells<true, true, false>); break;
+    default: cells(k_uvw_cells<true, true, true>); break;
     }
     TF_CHECK(hipGetLastError());
 }

@@ -512,8 +512,9 @@ int tflows_calcps_uvw_assemble(tflows_calcps h, const tflows_uvw_params* p, cons
                                const double* rho_o, const double* rho_oo, const double* ui_n, const double* ui_o,
                                const double* ui_oo, const double* source, const double* pi, const double* dUidi,
                                const double* dUidj, const double* dUidk, const double* dUjdi, const double* dUkdi,
-                               const double* dUjdj, const double* dUkdk, const double* flux_u, double* aval,
-                               double* b, double* abou, double* asave, double* rhoui, int32_t mem, void* stream) {
+                               const double* dUjdj, const double* dUkdk, const double* flux_u, const double* flux,
+                               double* aval, double* b, double* abou, double* asave, double* rhoui, int32_t mem,
+                               void* stream) {
     return guarded([&] {
         if (!h || !p) throw std::invalid_argument("null handle or parameters");
         const double* in[15] = {rho_n, rho_o, rho_oo, ui_n,  ui_o,  ui_oo, source, pi,
@@ -541,11 +542,11 @@ int tflows_calcps_uvw_assemble(tflows_calcps h, const tflows_uvw_params* p, cons
         const size_t ncell = (size_t)C.nbc() + C.nc() + 1;
         if (mem == TFLOWS_MEM_DEVICE) {
             const tflows::UvwFields F{rho_n, rho_o, rho_oo, ui_n, ui_o, ui_oo, source, pi,
-                                      {dUidi, dUidj, dUidk, dUjdi, dUkdi, dUjdj, dUkdk}, flux_u};
+                                      {dUidi, dUidj, dUidk, dUjdi, dUkdi, dUjdj, dUkdk}, flux_u, flux};
             C.uvw_assemble(P, F, aval, b, abou, asave, rhoui, st);
             return;
         }
-        const size_t total = 17 * ncell + (size_t)C.ns() + C.nonz() + C.nc() + C.nbc();
+        const size_t total = 17 * ncell + 2 * (size_t)C.ns() + C.nonz() + C.nc() + C.nbc();
         double* d = C.staging(total);
         const double* dv[15];
         for (int q = 0; q < 15; ++q) {
@@ -558,11 +559,15 @@ int tflows_calcps_uvw_assemble(tflows_calcps h, const tflows_uvw_params* p, cons
         double* daval = dflux + C.ns();
         double* db = daval + C.nonz();
         double* dabou = db + C.nc();
+        double* dmass = dabou + C.nbc();
         if (C.ns() > 0) TF_CHECK(hipMemcpyAsync(dflux, flux_u, sizeof(double) * C.ns(), hipMemcpyHostToDevice, st));
+        if (C.ns() > 0 && flux)
+            TF_CHECK(hipMemcpyAsync(dmass, flux, sizeof(double) * C.ns(), hipMemcpyHostToDevice, st));
         // Asave keeps the caller's values outside 1..NC
         if (asave) TF_CHECK(hipMemcpyAsync(dasave, asave, sizeof(double) * ncell, hipMemcpyHostToDevice, st));
         const tflows::UvwFields F{dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7],
-                                  {dv[8], dv[9], dv[10], dv[11], dv[12], dv[13], dv[14]}, dflux};
+                                  {dv[8], dv[9], dv[10], dv[11], dv[12], dv[13], dv[14]}, dflux,
+                                  flux ? dmass : nullptr};
         C.uvw_assemble(P, F, daval, db, dabou, asave ? dasave : nullptr, drhoui, st);
         TF_CHECK(hipMemcpyAsync(aval, daval, sizeof(double) * C.nonz(), hipMemcpyDeviceToHost, st));
         TF_CHECK(hipMemcpyAsync(b, db, sizeof(double) * C.nc(), hipMemcpyDeviceToHost, st));
@@ -600,8 +605,8 @@ int tflows_calcps_uvw_update(tflows_calcps h, const double* rhoui, const double*
 int tflows_calcps_zmix_assemble(tflows_calcps h, const tflows_uvw_params* p, const double* rho_n,
                                 const double* rho_o, const double* rho_oo, const double* z_n, const double* z_o,
                                 const double* z_oo, const double* source, const double* dZdx, const double* dZdy,
-                                const double* dZdz, const double* flux_u, double* aval, double* b, double* abou,
-                                double* rhoz, int32_t mem, void* stream) {
+                                const double* dZdz, const double* flux_u, const double* flux, double* aval,
+                                double* b, double* abou, double* rhoz, int32_t mem, void* stream) {
     return guarded([&] {
         if (!h || !p) throw std::invalid_argument("null handle or parameters");
         const double* in[10] = {rho_n, rho_o, rho_oo, z_n, z_o, z_oo, source, dZdx, dZdy, dZdz};
@@ -626,11 +631,11 @@ int tflows_calcps_zmix_assemble(tflows_calcps h, const tflows_uvw_params* p, con
         const size_t ncell = (size_t)C.nbc() + C.nc() + 1;
         if (mem == TFLOWS_MEM_DEVICE) {
             const tflows::UvwFields F{rho_n, rho_o, rho_oo, z_n, z_o, z_oo, source, nullptr,
-                                      {dZdx, dZdy, dZdz, nullptr, nullptr, nullptr, nullptr}, flux_u};
+                                      {dZdx, dZdy, dZdz, nullptr, nullptr, nullptr, nullptr}, flux_u, flux};
             C.zmix_assemble(P, F, aval, b, abou, rhoz, st);
             return;
         }
-        const size_t total = 11 * ncell + (size_t)C.ns() + C.nonz() + C.nc() + C.nbc();
+        const size_t total = 11 * ncell + 2 * (size_t)C.ns() + C.nonz() + C.nc() + C.nbc();
         double* d = C.staging(total);
         const double* dv[10];
         for (int q = 0; q < 10; ++q) {
@@ -642,9 +647,13 @@ int tflows_calcps_zmix_assemble(tflows_calcps h, const tflows_uvw_params* p, con
         double* daval = dflux + C.ns();
         double* db = daval + C.nonz();
         double* dabou = db + C.nc();
+        double* dmass = dabou + C.nbc();
         if (C.ns() > 0) TF_CHECK(hipMemcpyAsync(dflux, flux_u, sizeof(double) * C.ns(), hipMemcpyHostToDevice, st));
+        if (C.ns() > 0 && flux)
+            TF_CHECK(hipMemcpyAsync(dmass, flux, sizeof(double) * C.ns(), hipMemcpyHostToDevice, st));
         const tflows::UvwFields F{dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], nullptr,
-                                  {dv[7], dv[8], dv[9], nullptr, nullptr, nullptr, nullptr}, dflux};
+                                  {dv[7], dv[8], dv[9], nullptr, nullptr, nullptr, nullptr}, dflux,
+                                  flux ? dmass : nullptr};
         C.zmix_assemble(P, F, daval, db, dabou, drhoz, st);
         TF_CHECK(hipMemcpyAsync(aval, daval, sizeof(double) * C.nonz(), hipMemcpyDeviceToHost, st));
         TF_CHECK(hipMemcpyAsync(b, db, sizeof(double) * C.nc(), hipMemcpyDeviceToHost, st));

@@ -168,11 +168,13 @@ module tflows_krylov
     end function tflows_calcps_uvw_setup
 
     function tflows_calcps_uvw_assemble(h, p, rho_n, rho_o, rho_oo, ui_n, ui_o, ui_oo, source, pi, dUidi, &
-                                        dUidj, dUidk, dUjdi, dUkdi, dUjdj, dUkdk, flux_u, aval, b, abou, asave, &
-                                        rhoui, mem, stream) result(rc) bind(C, name="tflows_calcps_uvw_assemble")
+                                        dUidj, dUidk, dUjdi, dUkdi, dUjdj, dUkdk, flux_u, flux, aval, b, abou, &
+                                        asave, rhoui, mem, stream) result(rc) &
+        bind(C, name="tflows_calcps_uvw_assemble")
       import :: c_int32_t, c_ptr, tflows_uvw_params
       type(c_ptr), value          :: h, rho_n, rho_o, rho_oo, ui_n, ui_o, ui_oo, source, pi, dUidi, dUidj, dUidk, &
-                                     dUjdi, dUkdi, dUjdj, dUkdk, flux_u, aval, b, abou, asave, rhoui, stream
+                                     dUjdi, dUkdi, dUjdj, dUkdk, flux_u, flux, aval, b, abou, asave, rhoui, &
+                                     stream
       type(tflows_uvw_params)     :: p
       integer(c_int32_t), value   :: mem
       integer(c_int32_t)          :: rc
@@ -188,11 +190,11 @@ module tflows_krylov
 
     !--- mixture-fraction assembly (CalcZmix.f90:100-405) -------------------
     function tflows_calcps_zmix_assemble(h, p, rho_n, rho_o, rho_oo, z_n, z_o, z_oo, source, dZdx, dZdy, dZdz, &
-                                         flux_u, aval, b, abou, rhoz, mem, stream) result(rc) &
+                                         flux_u, flux, aval, b, abou, rhoz, mem, stream) result(rc) &
         bind(C, name="tflows_calcps_zmix_assemble")
       import :: c_int32_t, c_ptr, tflows_uvw_params
       type(c_ptr), value          :: h, rho_n, rho_o, rho_oo, z_n, z_o, z_oo, source, dZdx, dZdy, dZdz, flux_u, &
-                                     aval, b, abou, rhoz, stream
+                                     flux, aval, b, abou, rhoz, stream
       type(tflows_uvw_params)     :: p
       integer(c_int32_t), value   :: mem
       integer(c_int32_t)          :: rc

@@ -168,9 +168,9 @@ def _bind_extensions(lib):
         lib.tflows_calcps_uvw_setup.restype = C.c_int
         lib.tflows_calcps_uvw_setup.argtypes = [vp, vp, vp, vp, C.c_int32, C.c_int32, vp]
         lib.tflows_calcps_uvw_assemble.restype = C.c_int
-        lib.tflows_calcps_uvw_assemble.argtypes = [vp, C.POINTER(UvwParams)] + [vp] * 16 + [vp] * 5 + [C.c_int32, vp]
+        lib.tflows_calcps_uvw_assemble.argtypes = [vp, C.POINTER(UvwParams)] + [vp] * 17 + [vp] * 5 + [C.c_int32, vp]
         lib.tflows_calcps_zmix_assemble.restype = C.c_int
-        lib.tflows_calcps_zmix_assemble.argtypes = [vp, C.POINTER(UvwParams)] + [vp] * 11 + [vp] * 4 + [C.c_int32, vp]
+        lib.tflows_calcps_zmix_assemble.argtypes = [vp, C.POINTER(UvwParams)] + [vp] * 12 + [vp] * 4 + [C.c_int32, vp]
         lib.tflows_calcps_zmix_update.restype = C.c_int
         lib.tflows_calcps_zmix_update.argtypes = [vp, vp, vp, vp, C.c_int32, vp]
         lib.tflows_calcps_uvw_update.restype = C.c_int
@@ -673,11 +673,13 @@ class DeviceCalcPS:
     @staticmethod
     def uvw_inputs(args):
         """cuda float64 tensors of meshgen.uvw_args(...) in the order of
-        tflows_calcps_uvw_assemble: 15 cell arrays, then Flux_u."""
+        tflows_calcps_uvw_assemble: 15 cell arrays, Flux_u, then the mass
+        flux Flux (None when args has none: Flux_u decides)."""
         import torch
         t64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()   # noqa: E731
         cells = [args[k] for k in ("rho_n", "rho_o", "rho_oo", "ui_n", "ui_o", "ui_oo", "source", "pi")]
-        return [t64(a) for a in cells + list(args["g"])] + [t64(args["flux_u"])]
+        fl = args.get("flux")
+        return [t64(a) for a in cells + list(args["g"])] + [t64(args["flux_u"]), None if fl is None else t64(fl)]
 
     def assemble_uvw(self, params, inputs, out=None, stream=None):
         """One component's system from cuda tensors ``inputs`` (uvw_inputs);
@@ -696,7 +698,8 @@ class DeviceCalcPS:
                        rhoui=torch.empty(ncell, dtype=torch.float64, device=dev))
         P = UvwParams.from_dict(params)
         _check(self.lib.tflows_calcps_uvw_assemble(
-            self.handle, C.byref(P), *(_ptr(t) for t in inputs), _ptr(out["aval"]), _ptr(out["b"]),
+            self.handle, C.byref(P), *(None if t is None else _ptr(t) for t in inputs), _ptr(out["aval"]),
+            _ptr(out["b"]),
             _ptr(out["abou"]), _ptr(out["asave"]), _ptr(out["rhoui"]), 1, _stream(stream)))
         return out
 
@@ -708,11 +711,13 @@ class DeviceCalcPS:
                  for k in ("rho_n", "rho_o", "rho_oo", "ui_n", "ui_o", "ui_oo", "source", "pi")]
         cells += [np.ascontiguousarray(a, np.float64) for a in args["g"]]
         fl = np.ascontiguousarray(args["flux_u"], np.float64)
+        mf = None if args.get("flux") is None else np.ascontiguousarray(args["flux"], np.float64)
         aval, b = np.empty(c.nonz), np.empty(c.nc)
         abou, asave, rhoui = np.zeros(max(c.nbc, 1)), np.zeros(ncell), np.empty(ncell)
         P = UvwParams.from_dict(params)
         _check(self.lib.tflows_calcps_uvw_assemble(
-            self.handle, C.byref(P), *(a.ctypes.data for a in cells), fl.ctypes.data, aval.ctypes.data,
+            self.handle, C.byref(P), *(a.ctypes.data for a in cells), fl.ctypes.data,
+            None if mf is None else mf.ctypes.data, aval.ctypes.data,
             b.ctypes.data, abou.ctypes.data, asave.ctypes.data, rhoui.ctypes.data, 0, None))
         return dict(aval=aval, b=b, abou=abou[:c.nbc], asave=asave, rhoui=rhoui)
 
@@ -720,11 +725,13 @@ class DeviceCalcPS:
     @staticmethod
     def zmix_inputs(args):
         """cuda float64 tensors of meshgen.zmix_args(...) in the order of
-        tflows_calcps_zmix_assemble: 10 cell arrays, then Flux_u."""
+        tflows_calcps_zmix_assemble: 10 cell arrays, Flux_u, then the mass
+        flux Flux (None when args has none)."""
         import torch
         t64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()   # noqa: E731
         cells = [args[k] for k in ("rho_n", "rho_o", "rho_oo", "z_n", "z_o", "z_oo", "source")]
-        return [t64(a) for a in cells + list(args["g"])] + [t64(args["flux_u"])]
+        fl = args.get("flux")
+        return [t64(a) for a in cells + list(args["g"])] + [t64(args["flux_u"]), None if fl is None else t64(fl)]
 
     def assemble_zmix(self, params, inputs, out=None, stream=None):
         """The mixture-fraction system from cuda tensors ``inputs``
@@ -739,7 +746,8 @@ class DeviceCalcPS:
                        rhoz=torch.empty(c.nbc + c.nc + 1, dtype=torch.float64, device=dev))
         P = UvwParams.from_dict(params)
         _check(self.lib.tflows_calcps_zmix_assemble(
-            self.handle, C.byref(P), *(_ptr(t) for t in inputs), _ptr(out["aval"]), _ptr(out["b"]),
+            self.handle, C.byref(P), *(None if t is None else _ptr(t) for t in inputs), _ptr(out["aval"]),
+            _ptr(out["b"]),
             _ptr(out["abou"]), _ptr(out["rhoz"]), 1, _stream(stream)))
         return out
 
@@ -750,11 +758,13 @@ class DeviceCalcPS:
                  for k in ("rho_n", "rho_o", "rho_oo", "z_n", "z_o", "z_oo", "source")]
         cells += [np.ascontiguousarray(a, np.float64) for a in args["g"]]
         fl = np.ascontiguousarray(args["flux_u"], np.float64)
+        mf = None if args.get("flux") is None else np.ascontiguousarray(args["flux"], np.float64)
         aval, b = np.empty(c.nonz), np.empty(c.nc)
         abou, rhoz = np.zeros(max(c.nbc, 1)), np.empty(c.nbc + c.nc + 1)
         P = UvwParams.from_dict(params)
         _check(self.lib.tflows_calcps_zmix_assemble(
-            self.handle, C.byref(P), *(a.ctypes.data for a in cells), fl.ctypes.data, aval.ctypes.data,
+            self.handle, C.byref(P), *(a.ctypes.data for a in cells), fl.ctypes.data,
+            None if mf is None else mf.ctypes.data, aval.ctypes.data,
             b.ctypes.data, abou.ctypes.data, rhoz.ctypes.data, 0, None))
         return dict(aval=aval, b=b, abou=abou[:c.nbc], rhoz=rhoz)
 

@@ -1113,7 +1113,7 @@ def zmix_args(case: CalcPSCase, zf: dict) -> dict:
     """The cell and face arrays of the mixture-fraction assembly."""
     return {"rho_n": case.rho_n, "rho_o": case.rho_o, "rho_oo": case.rho_oo, "z_n": zf["z"], "z_o": zf["z_o"],
             "z_oo": zf["z_oo"], "source": zf["src_z"], "g": [zf["zx"], zf["zy"], zf["zz"]],
-            "flux_u": zf["flux_u"]}
+            "flux_u": zf["flux_u"], "flux": zf.get("flux")}
 
 
 def uvw_params(nmaterials: int = 1, **kw) -> dict:
@@ -1138,4 +1138,5 @@ def uvw_args(case: CalcPSCase, uf: dict, var: int) -> dict:
     return {"rho_n": case.rho_n, "rho_o": case.rho_o, "rho_oo": case.rho_oo,
             "ui_n": uf.get(comp, getattr(case, comp)), "ui_o": uf[comp + "_o"], "ui_oo": uf[comp + "_oo"],
             "source": uf["src_" + comp], "pi": getattr(case, pname), "g": [uf[k] for k in grads],
-            "S": [S[q] for q in order], "D": [D[q] for q in order], "flux_u": uf["flux_u"]}
+            "S": [S[q] for q in order], "D": [D[q] for q in order], "flux_u": uf["flux_u"],
+            "flux": uf.get("flux")}

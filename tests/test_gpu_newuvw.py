@@ -7,7 +7,8 @@ face order, the order in which the reference's loops reach the cell, and
 computes each face's terms in the reference's operation order.  Cases: the
 two-cell mesh with every boundary type, channel and shuffled meshes, two
 materials, faces that share an off-diagonal (split faces), U, V and W, and
-the flag sets the assembly covers.  End to end, U, V and W are assembled,
+the flag sets the assembly covers, and every ConvScheme face scheme.  End
+to end, U, V and W are assembled,
 solved with the GPU BiCG and updated (:505-526), against the oracle's
 assembly solved by the oracle's BiCG.
 """
@@ -118,18 +119,65 @@ def test_shared_offdiagonals_present(pkg):
     assert len(np.unique(pos)) < len(pos)
 
 
+SCHEMES = list(range(20, 28))   # CDS .. GAMMA (Process/IniPar.f90:42-49)
+
+
+def with_mass_flux(uf):
+    """The mass flux Flux that picks ConvScheme's upwind cell: Flux_u with
+    every third face's sign flipped, so there the face scheme's upwind side
+    and the upwind part's differ."""
+    mf = np.array(uf["flux_u"], dtype=float)
+    mf[::3] *= -1.0
+    return dict(uf, flux=mf)
+
+
+@pytest.mark.parametrize("var", [1, 2, 3])
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("name", CASES)
+def test_conv_scheme_bitwise(pkg, oracle, cuda, name, scheme, var):
+    """ConvScheme's face values (with CalMinMax's bounds) on the GPU: Aval,
+    b, Abou, rhoUi bit for bit the oracle's, for every scheme, U, V and W,
+    with a mass flux whose sign differs from Flux_u's on some faces; on two
+    materials the scheme sits next to NO."""
+    case, uf = _case(pkg, name)
+    uf = with_mass_flux(uf)
+    nmat = uf["nmaterials"]
+    p = pkg.meshgen.uvw_params(nmaterials=nmat, var=var, blend=[scheme, NO][:nmat], blend_tem=[YES] * nmat,
+                               urf=0.7, urfc=[0.9, 0.8][:nmat])
+    want = oracle.newuvw(case, uf, p)
+    dc, args, got = _run(pkg, case, uf, p)
+    print(f"case {name} scheme {scheme} var {var}", flush=True)
+    assert np.array_equal(got["aval"], want["aval"])
+    assert np.array_equal(got["b"], want["b"])
+    assert np.array_equal(got["abou"][:case.nbc], want["abou"])
+    assert np.array_equal(got["rhoui"], want["rhoui"])
+    if var == 1:
+        host = dc.assemble_uvw_host(p, args)
+        for k in ("aval", "b", "rhoui"):
+            assert np.array_equal(host[k], got[k]), k
+        # without a mass flux Flux_u picks the upwind side, as in the oracle
+        uf0 = dict(uf)
+        del uf0["flux"]
+        want0 = oracle.newuvw(case, uf0, p)
+        got0 = dc.assemble_uvw(p, dc.uvw_inputs(pkg.meshgen.uvw_args(case, uf0, 1)))
+        assert np.array_equal(got0["b"].cpu().numpy(), want0["b"])
+    dc.close()
+
+
 def test_blend_schemes_refused(pkg, cuda):
-    """A TVD face scheme, or YES next to NO at a face, is refused with an
-    error before anything runs."""
+    """YES next to NO at a face (the reference leaves the face value
+    undefined there), or a code that is no scheme, is refused with an error
+    before anything runs; a ConvScheme scheme is not."""
     mg = pkg.meshgen
     case, uf = _case(pkg, "two_materials")
     dc = pkg.krylov.DeviceCalcPS(case)
     dc.uvw_setup(uf)
     inputs = dc.uvw_inputs(mg.uvw_args(case, uf, 1))
-    for blend in ([21, 21], [YES, NO]):
+    for blend in ([YES, NO], [28, 28], [19, NO]):
         with pytest.raises(pkg.krylov.TFlowsError):
             dc.assemble_uvw(mg.uvw_params(nmaterials=2, blend=blend), inputs)
     dc.assemble_uvw(mg.uvw_params(nmaterials=2, blend=[YES, YES]), inputs)
+    dc.assemble_uvw(mg.uvw_params(nmaterials=2, blend=[21, 21]), inputs)
     dc.close()
 
 

@@ -9,7 +9,7 @@ against the oracle's assembly and CG.
 import numpy as np
 import pytest
 
-from test_gpu_newuvw import CASES, _case
+from test_gpu_newuvw import CASES, SCHEMES, _case, with_mass_flux
 from test_zmix_oracle import _hand_zfields
 
 pytestmark = pytest.mark.gpu
@@ -60,15 +60,40 @@ def test_assembly_bitwise(pkg, oracle, cuda, name, flags):
     dc.close()
 
 
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("name", CASES)
+def test_conv_scheme_bitwise(pkg, oracle, cuda, name, scheme):
+    """ConvScheme through the scalar assembly (BLEND_TEM, with rhoZ and
+    dZ/dx, dZ/dy, dZ/dz): bit for bit the oracle's, every scheme, a mass
+    flux differing in sign from Flux_u on some faces."""
+    case, zf = _zcase(pkg, name)
+    zf = with_mass_flux(zf)
+    nmat = zf["nmaterials"]
+    p = pkg.meshgen.uvw_params(nmaterials=nmat, blend_tem=[scheme, NO][:nmat], urf=0.7, urfc=[0.9, 0.8][:nmat])
+    want = oracle.calczmix(case, zf, p)
+    dc = pkg.krylov.DeviceCalcPS(case)
+    dc.uvw_setup(zf)
+    args = pkg.meshgen.zmix_args(case, zf)
+    print(f"case {name} scheme {scheme}", flush=True)
+    got = {k: v.cpu().numpy() for k, v in dc.assemble_zmix(p, dc.zmix_inputs(args)).items()}
+    for k in ("aval", "b", "rhoz"):
+        assert np.array_equal(got[k], want[k]), k
+    assert np.array_equal(got["abou"][:case.nbc], want["abou"])
+    host = dc.assemble_zmix_host(p, args)
+    assert np.array_equal(host["b"], got["b"])
+    dc.close()
+
+
 def test_blend_tem_schemes_refused(pkg, cuda):
     mg = pkg.meshgen
     case, zf = _zcase(pkg, "two_materials")
     dc = pkg.krylov.DeviceCalcPS(case)
     dc.uvw_setup(zf)
     inputs = dc.zmix_inputs(mg.zmix_args(case, zf))
-    for bt in ([23, 23], [YES, NO]):
+    for bt in ([28, 28], [YES, NO]):
         with pytest.raises(pkg.krylov.TFlowsError):
             dc.assemble_zmix(mg.uvw_params(nmaterials=2, blend_tem=bt), inputs)
+    dc.assemble_zmix(mg.uvw_params(nmaterials=2, blend_tem=[23, 23]), inputs)
     # BLEND (momentum) is not read by the scalar assembly
     dc.assemble_zmix(mg.uvw_params(nmaterials=2, blend=[21, 21], blend_tem=[NO, NO]), inputs)
     dc.close()
