@@ -375,6 +375,28 @@ int tflows_calcps_zmix_update(tflows_calcps h, const double* rhoz, const double*
  * The arrays feed tflows_system_create (acol, arow, adia) and
  * tflows_calcps_create / tflows_calcps_uvw_setup directly. */
 typedef struct tflows_mesh_s* tflows_mesh;
+/* NAME.b, the boundary-condition file, read as Process/BouLoa.f90:30-160 does
+ * (comment and token rules of Library/ReadC.f90).  hot: HOT = YES (the
+ * materials then carry CONc and CAPc, the conditions a Zmix value).
+ * Materials: StateMat (FLUID 7, SOLID 8), DENc, CONc, CAPc (0 unless hot);
+ * VISc is one value, the last material's, as the reference keeps it.
+ * Boundary conditions 1..NB in file order: typBou (allp_mod codes), a FILE
+ * flag with the profile name (tflows_bfile_name kind 0), else U, V, W, P
+ * (PRESSURE only) and Zmix (hot only), 0 where absent.  Initial conditions
+ * 1..Ninit: a FILE flag and name (kind 1), else U, V, W, Zmix.  TypeBC of
+ * the mesh's boundary cells is then typBou(bcmark) (tflows_mesh_typebc).
+ * An unknown state or type, a short line or a bad number is an error.
+ * Host code, no GPU: the reading part of BouLoa, not its profile files. */
+typedef struct tflows_bfile_s* tflows_bfile;
+int tflows_bfile_load(const char* path, int32_t hot, tflows_bfile* out);
+int tflows_bfile_destroy(tflows_bfile b);
+int tflows_bfile_info(tflows_bfile b, int32_t* nmat, int32_t* nb, int32_t* ninit, double* visc);
+int tflows_bfile_materials(tflows_bfile b, int32_t* statemat, double* denc, double* conc, double* capc);
+int tflows_bfile_bounds(tflows_bfile b, int32_t* typbou, int32_t* from_file, double* u, double* v, double* w,
+                        double* p, double* zmix);
+int tflows_bfile_inits(tflows_bfile b, int32_t* from_file, double* u, double* v, double* w, double* zmix);
+int tflows_bfile_name(tflows_bfile b, int32_t kind, int32_t n, char* buf, int32_t len);
+
 int tflows_mesh_load(const char* cns_path, const char* geo_path, tflows_mesh* out);
 int tflows_mesh_destroy(tflows_mesh m);
 int tflows_mesh_info(tflows_mesh m, int32_t* nc, int32_t* nbc, int32_t* ns, int32_t* nmat, int32_t* nonz,

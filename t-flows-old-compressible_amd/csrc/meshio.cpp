@@ -32,9 +32,12 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <cctype>
+#include <cstdlib>
 #include <vector>
 
 #include "../../include/tflows_krylov.h"
@@ -275,6 +278,164 @@ int copy_out(const std::vector<T>& v, T* dst, int64_t count) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// NAME.b, the boundary-condition file (Process/BouLoa.f90:30-160): materials
+// (state, VISc, DENc, and CONc, CAPc when HOT), boundary conditions in file
+// order (type, then U V W [P for PRESSURE] [Zmix when HOT], or FILE and a
+// profile name) and initial conditions (U V W [Zmix], or FILE and a name).
+// Lines are read as Library/ReadC.f90 does: a line whose first column is one
+// of ! # $ % / & [ ] { } : ; @ < > * is a comment, blank lines are skipped,
+// tokens are separated by characters below '!'.
+// ---------------------------------------------------------------------------
+struct tflows_bfile_s {
+    int nmat = 0, nb = 0, ninit = 0;
+    double visc = 0.0;                           // VISc: the last material's (:51)
+    std::vector<int32_t> statemat, typbou, bfile, ifile;
+    std::vector<double> denc, conc, capc, bu, bv, bw, bp, bz, iu, iv, iw, iz;
+    std::vector<std::string> profile, initname;
+};
+
+namespace {
+
+struct BLines {
+    std::ifstream in;
+    std::string path;
+    int lineno = 0;
+    std::vector<std::string> next(const char* what) {        // ReadC
+        std::string line;
+        while (std::getline(in, line)) {
+            ++lineno;
+            if (!line.empty() && line.back() == '\r') line.pop_back();
+            bool blank = true;
+            for (char ch : line) blank = blank && (unsigned char)ch < '!';
+            if (blank) continue;
+            if (std::strchr("!#$%/&[]{}:;@<>*", line[0])) continue;
+            std::vector<std::string> tok;
+            size_t i = 0;
+            while (i < line.size()) {
+                while (i < line.size() && (unsigned char)line[i] < '!') ++i;
+                size_t j = i;
+                while (j < line.size() && (unsigned char)line[j] >= '!') ++j;
+                if (j > i) tok.push_back(line.substr(i, j - i));
+                i = j;
+            }
+            return tok;
+        }
+        throw std::runtime_error(path + ": end of file before " + what);
+    }
+    std::string where() const { return path + " line " + std::to_string(lineno); }
+};
+
+std::string upper(std::string t) {                          // ToUppr
+    for (char& ch : t) ch = (char)std::toupper((unsigned char)ch);
+    return t;
+}
+
+// a number as Fortran's list-directed read takes it (D exponents included)
+double fnum(const BLines& L, const std::vector<std::string>& t, size_t k, const char* what) {
+    if (k >= t.size()) throw std::runtime_error(L.where() + ": missing " + what);
+    std::string v = t[k];
+    for (char& ch : v)
+        if (ch == 'd' || ch == 'D') ch = 'e';
+    char* end = nullptr;
+    const double x = std::strtod(v.c_str(), &end);
+    if (end == v.c_str() || *end != '\0') throw std::runtime_error(L.where() + ": bad " + what + " '" + t[k] + "'");
+    return x;
+}
+
+int fint(const BLines& L, const std::vector<std::string>& t, size_t k, const char* what) {
+    const double x = fnum(L, t, k, what);
+    if (x != (double)(int)x) throw std::runtime_error(L.where() + ": " + what + " is not an integer");
+    return (int)x;
+}
+
+void load_b(tflows_bfile_s& B, const char* path, bool hot) {
+    BLines L;
+    L.path = path ? path : "";
+    L.in.open(L.path);
+    if (!L.in) throw std::runtime_error("cannot open " + L.path);
+    auto t = L.next("the number of materials");
+    B.nmat = fint(L, t, 0, "Nmat");                             // :40
+    if (B.nmat < 1 || B.nmat > 1024) throw std::runtime_error(L.where() + ": Nmat out of range");
+    for (int n = 0; n < B.nmat; ++n) {                          // :41-56
+        t = L.next("a material");
+        if (t.size() < 2) throw std::runtime_error(L.where() + ": missing material state");
+        const std::string st = upper(t[1]);
+        if (st == "FLUID") B.statemat.push_back(7);
+        else if (st == "SOLID") B.statemat.push_back(8);
+        else throw std::runtime_error(L.where() + ": unknown material state " + t[1]);
+        B.visc = fnum(L, t, 2, "VISc");
+        B.denc.push_back(fnum(L, t, 3, "DENc"));
+        B.conc.push_back(hot ? fnum(L, t, 4, "CONc") : 0.0);
+        B.capc.push_back(hot ? fnum(L, t, 5, "CAPc") : 0.0);
+    }
+    t = L.next("the number of boundary conditions");
+    B.nb = fint(L, t, 0, "NB");                                 // :61
+    if (B.nb < 0 || B.nb > 128) throw std::runtime_error(L.where() + ": NB out of range (0..128)");
+    static const std::pair<const char*, int> kTypes[] = {{"INFLOW", 1},  {"WALL", 2},     {"OUTFLOW", 3},
+                                                         {"SYMMETRY", 4}, {"WALLFLUX", 6}, {"CONVECTIVE", 5},
+                                                         {"PRESSURE", 12}};
+    for (int n = 0; n < B.nb; ++n) {                            // :62-124
+        t = L.next("a boundary condition");
+        fint(L, t, 0, "the boundary index");
+        if (t.size() < 3) throw std::runtime_error(L.where() + ": incomplete boundary condition");
+        const std::string ty = upper(t[1]);
+        int code = 0;
+        for (const auto& kv : kTypes)
+            if (ty == kv.first) code = kv.second;
+        if (!code) throw std::runtime_error(L.where() + ": unknown boundary condition type " + t[1]);
+        B.typbou.push_back(code);
+        double u = 0.0, v = 0.0, w = 0.0, pb = 0.0, z = 0.0;
+        std::string prof;
+        const bool file = upper(t[2]) == "FILE";
+        if (file) {
+            if (t.size() < 4) throw std::runtime_error(L.where() + ": FILE without a name");
+            prof = t[3];
+        } else {
+            u = fnum(L, t, 2, "U");
+            v = fnum(L, t, 3, "V");
+            w = fnum(L, t, 4, "W");
+            size_t k = 5;
+            if (code == 12) pb = fnum(L, t, k++, "P");
+            if (hot) z = fnum(L, t, k, "Zmix");
+        }
+        B.bfile.push_back(file ? 1 : 0);
+        B.profile.push_back(prof);
+        B.bu.push_back(u);
+        B.bv.push_back(v);
+        B.bw.push_back(w);
+        B.bp.push_back(pb);
+        B.bz.push_back(z);
+    }
+    t = L.next("the number of initial conditions");
+    B.ninit = fint(L, t, 0, "Ninit");                           // :130
+    if (B.ninit < 0 || B.ninit > 128) throw std::runtime_error(L.where() + ": Ninit out of range (0..128)");
+    for (int n = 0; n < B.ninit; ++n) {                         // :135-160
+        t = L.next("an initial condition");
+        if (t.size() < 2) throw std::runtime_error(L.where() + ": incomplete initial condition");
+        const bool file = upper(t[1]) == "FILE";
+        double u = 0.0, v = 0.0, w = 0.0, z = 0.0;
+        std::string nm;
+        if (file) {
+            if (t.size() < 3) throw std::runtime_error(L.where() + ": FILE without a name");
+            nm = t[2];
+        } else {
+            u = fnum(L, t, 1, "U");
+            v = fnum(L, t, 2, "V");
+            w = fnum(L, t, 3, "W");
+            if (hot) z = fnum(L, t, 4, "Zmix");
+        }
+        B.ifile.push_back(file ? 1 : 0);
+        B.initname.push_back(nm);
+        B.iu.push_back(u);
+        B.iv.push_back(v);
+        B.iw.push_back(w);
+        B.iz.push_back(z);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // C API (errors through tflows_last_error, defined in capi.hip)
 // ---------------------------------------------------------------------------
 namespace tflows {
@@ -395,6 +556,82 @@ int tflows_mesh_ff(tflows_mesh m, const int32_t* typebc, const int32_t* statemat
             if (s1 == kSolid && s2 == kFluid) ff[s] = 0.0;
             if (c2 < 0 && typebc[m->nbc + c2] != kBuffer) ff[s] = 1.0;
         }
+    });
+}
+
+int tflows_bfile_load(const char* path, int32_t hot, tflows_bfile* out) {
+    return mesh_guard([&] {
+        if (!out) throw std::invalid_argument("out is NULL");
+        auto b = std::make_unique<tflows_bfile_s>();
+        load_b(*b, path, hot != 0);
+        *out = b.release();
+    });
+}
+
+int tflows_bfile_destroy(tflows_bfile b) {
+    return mesh_guard([&] { delete b; });
+}
+
+int tflows_bfile_info(tflows_bfile b, int32_t* nmat, int32_t* nb, int32_t* ninit, double* visc) {
+    return mesh_guard([&] {
+        if (!b) throw std::invalid_argument("null handle");
+        if (nmat) *nmat = b->nmat;
+        if (nb) *nb = b->nb;
+        if (ninit) *ninit = b->ninit;
+        if (visc) *visc = b->visc;
+    });
+}
+
+int tflows_bfile_materials(tflows_bfile b, int32_t* statemat, double* denc, double* conc, double* capc) {
+    return mesh_guard([&] {
+        if (!b) throw std::invalid_argument("null handle");
+        for (int n = 0; n < b->nmat; ++n) {
+            if (statemat) statemat[n] = b->statemat[n];
+            if (denc) denc[n] = b->denc[n];
+            if (conc) conc[n] = b->conc[n];
+            if (capc) capc[n] = b->capc[n];
+        }
+    });
+}
+
+int tflows_bfile_bounds(tflows_bfile b, int32_t* typbou, int32_t* from_file, double* u, double* v, double* w,
+                        double* p, double* zmix) {
+    return mesh_guard([&] {
+        if (!b) throw std::invalid_argument("null handle");
+        for (int n = 0; n < b->nb; ++n) {
+            if (typbou) typbou[n] = b->typbou[n];
+            if (from_file) from_file[n] = b->bfile[n];
+            if (u) u[n] = b->bu[n];
+            if (v) v[n] = b->bv[n];
+            if (w) w[n] = b->bw[n];
+            if (p) p[n] = b->bp[n];
+            if (zmix) zmix[n] = b->bz[n];
+        }
+    });
+}
+
+int tflows_bfile_inits(tflows_bfile b, int32_t* from_file, double* u, double* v, double* w, double* zmix) {
+    return mesh_guard([&] {
+        if (!b) throw std::invalid_argument("null handle");
+        for (int n = 0; n < b->ninit; ++n) {
+            if (from_file) from_file[n] = b->ifile[n];
+            if (u) u[n] = b->iu[n];
+            if (v) v[n] = b->iv[n];
+            if (w) w[n] = b->iw[n];
+            if (zmix) zmix[n] = b->iz[n];
+        }
+    });
+}
+
+int tflows_bfile_name(tflows_bfile b, int32_t kind, int32_t n, char* buf, int32_t len) {
+    return mesh_guard([&] {
+        if (!b || !buf || len < 1) throw std::invalid_argument("null handle or buffer");
+        const std::vector<std::string>& names = kind == 0 ? b->profile : b->initname;
+        if (kind != 0 && kind != 1) throw std::invalid_argument("kind must be 0 (profile) or 1 (initial condition)");
+        if (n < 1 || n > (int)names.size()) throw std::invalid_argument("entry out of range");
+        const std::string& s = names[n - 1];
+        if ((int)s.size() >= len) throw std::invalid_argument("buffer too short");
+        std::memcpy(buf, s.c_str(), s.size() + 1);
     });
 }
 

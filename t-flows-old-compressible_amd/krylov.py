@@ -157,6 +157,21 @@ def _bind_extensions(lib):
         lib.tflows_mesh_typebc.argtypes = [vp, vp, C.c_int32, vp, C.c_int64]
         lib.tflows_mesh_ff.restype = C.c_int
         lib.tflows_mesh_ff.argtypes = [vp, vp, vp, vp, C.c_int64]
+    if hasattr(lib, "tflows_bfile_load"):
+        lib.tflows_bfile_load.restype = C.c_int
+        lib.tflows_bfile_load.argtypes = [C.c_char_p, C.c_int32, C.POINTER(vp)]
+        lib.tflows_bfile_destroy.restype = C.c_int
+        lib.tflows_bfile_destroy.argtypes = [vp]
+        lib.tflows_bfile_info.restype = C.c_int
+        lib.tflows_bfile_info.argtypes = [vp] + [C.POINTER(C.c_int32)] * 3 + [C.POINTER(C.c_double)]
+        lib.tflows_bfile_materials.restype = C.c_int
+        lib.tflows_bfile_materials.argtypes = [vp] * 5
+        lib.tflows_bfile_bounds.restype = C.c_int
+        lib.tflows_bfile_bounds.argtypes = [vp] * 8
+        lib.tflows_bfile_inits.restype = C.c_int
+        lib.tflows_bfile_inits.argtypes = [vp] * 6
+        lib.tflows_bfile_name.restype = C.c_int
+        lib.tflows_bfile_name.argtypes = [vp, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
     if hasattr(lib, "tflows_calcps_create"):
         lib.tflows_calcps_create.restype = C.c_int
         lib.tflows_calcps_create.argtypes = [C.c_int32] * 4 + [vp] * 12 + [C.c_int32, vp, C.POINTER(vp)]
@@ -518,6 +533,50 @@ class UvwParams(C.Structure):
             q.blend_tem[m] = int(p["blend_tem"][m])
             q.urfc[m] = float(p["urfc"][m])
         return q
+
+
+class BFile:
+    """NAME.b, the boundary-condition file, read by the library
+    (csrc/meshio.cpp, after Process/BouLoa.f90:30-160).  ``hot`` is HOT =
+    YES.  Attributes: nmat, visc (the last material's VISc), statemat (FLUID
+    7, SOLID 8), denc, conc, capc; nb, typbou (allp_mod codes, file order),
+    bfile (1 where the condition is a FILE profile), profiles (names, '' for
+    constants), bu, bv, bw, bp, bz; ninit, ifile, inits (names), iu, iv, iw,
+    iz.  Host only."""
+
+    def __init__(self, path, hot=False):
+        lib = load_library()
+        h = C.c_void_p()
+        _check(lib.tflows_bfile_load(os.fsencode(path), 1 if hot else 0, C.byref(h)))
+        try:
+            n = [C.c_int32(0) for _ in range(3)]
+            visc = C.c_double(0.0)
+            _check(lib.tflows_bfile_info(h, *(C.byref(x) for x in n), C.byref(visc)))
+            self.nmat, self.nb, self.ninit = (x.value for x in n)
+            self.visc = visc.value
+            i32 = lambda k: np.zeros(max(k, 1), np.int32)        # noqa: E731
+            f64 = lambda k: np.zeros(max(k, 1), np.float64)      # noqa: E731
+            self.statemat, self.denc, self.conc, self.capc = i32(self.nmat), f64(self.nmat), f64(self.nmat), \
+                f64(self.nmat)
+            _check(lib.tflows_bfile_materials(h, *(a.ctypes.data for a in (self.statemat, self.denc, self.conc,
+                                                                             self.capc))))
+            b = [i32(self.nb), i32(self.nb)] + [f64(self.nb) for _ in range(5)]
+            _check(lib.tflows_bfile_bounds(h, *(a.ctypes.data for a in b)))
+            self.typbou, self.bfile, self.bu, self.bv, self.bw, self.bp, self.bz = (a[:self.nb] for a in b)
+            q = [i32(self.ninit)] + [f64(self.ninit) for _ in range(4)]
+            _check(lib.tflows_bfile_inits(h, *(a.ctypes.data for a in q)))
+            self.ifile, self.iu, self.iv, self.iw, self.iz = (a[:self.ninit] for a in q)
+            for k in ("statemat", "denc", "conc", "capc"):
+                setattr(self, k, getattr(self, k)[:self.nmat])
+            buf = C.create_string_buffer(512)
+
+            def name(kind, k):
+                _check(lib.tflows_bfile_name(h, kind, k, buf, len(buf)))
+                return buf.value.decode()
+            self.profiles = [name(0, k) for k in range(1, self.nb + 1)]
+            self.inits = [name(1, k) for k in range(1, self.ninit + 1)]
+        finally:
+            lib.tflows_bfile_destroy(h)
 
 
 class Mesh:
