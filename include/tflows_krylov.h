@@ -72,6 +72,19 @@ void prec1_(const int32_t* N, const int32_t* NONZ, const double* A, const int32_
 void resid_(const int32_t* N, const int32_t* NB, const int32_t* NONZ, const double* A, const int32_t* Acol,
             const int32_t* Arow, const double* Ab, const double* x, double* r1);
 
+/* The five drop-ins share one device copy of the last pattern they were
+ * given, as the reference fixes its pattern once (TopolM,
+ * Process/Processor.f90:100) and calls the solvers on it every outer
+ * iteration.  A call whose N, NONZ, Acol, Arow and Adia equal the cached
+ * copy's (a host comparison) uploads only A and the vectors, and keeps the
+ * pattern's locality order, sliced operators and DIC level analysis; any
+ * difference rebuilds the copy.  No counterpart in the reference:
+ * tflows_dropin_stats reports the pattern builds and drop-in calls so far
+ * (either pointer may be NULL); tflows_dropin_release frees the cached
+ * copy (the next drop-in call builds it again). */
+int tflows_dropin_stats(int64_t* pattern_builds, int64_t* calls);
+int tflows_dropin_release(void);
+
 /* ------------------------------------------------------------------------
  * 2. Device-resident API (no counterpart in the reference)
  * ---------------------------------------------------------------------- */
@@ -168,6 +181,16 @@ int tflows_resid(tflows_system s, const double* x, double* r, void* stream);
  * (0 before the first prec = 2 use); for reporting. */
 int32_t tflows_system_levels(tflows_system s);
 
+/* Form of the system's DIC sweeps (builds the level analysis): 2 = one
+ * launch per triangle with rows waiting on their dependencies (the
+ * default), 1 = one launch per level over the level-major copy, 0 = one
+ * launch per level over the matrix rows; -1 on error.  All three give the
+ * same bits.  A one-launch sweep whose waiting row gives up (it waited
+ * beyond TFLOWS_DIC_WAIT_MS, e.g. while other work held the CUs) makes the
+ * solve restore its inputs and run again in form 1, which the system then
+ * keeps.  No counterpart in the reference. */
+int32_t tflows_system_dic_sweep(tflows_system s);
+
 /* SpMV form of the system's solves and products: lanes per matrix row
  * (1 = one row per thread with LDS-staged products, 64 rows per wavefront;
  * 2, 4, 8 = lane groups streaming the row, 32 / 16 / 8 rows per
@@ -196,6 +219,13 @@ int32_t tflows_system_reordered(tflows_system s);
  * *stored_entries: off-diagonal entries held, padding included.  Builds
  * the operator if no solve has yet.  No counterpart in the reference. */
 int tflows_system_fp32_layout(tflows_system s, int32_t* sliced, int64_t* stored_entries, void* stream);
+
+/* Layout of the FP64 operator of the solver iterations (A itself, the
+ * diagonal where it stands in each row): *sliced = 1 sliced ELL, 2 the
+ * same with 16-bit column offsets (where every offset fits, unless
+ * TFLOWS_FP64_COL16=0), 0 the CSR arrays; *stored_entries: entries held,
+ * padding included.  No counterpart in the reference. */
+int tflows_system_fp64_layout(tflows_system s, int32_t* sliced, int64_t* stored_entries, void* stream);
 
 /* ------------------------------------------------------------------------
  * Batches of independent systems (no counterpart in the reference, whose
@@ -227,6 +257,22 @@ int tflows_batch_configure(tflows_batch b, int32_t grid, int32_t lanes, int32_t 
  * of rebalances so far. */
 int tflows_batch_config(tflows_batch b, int32_t bits, int32_t method, int32_t prec, int32_t* grid, int32_t* lanes,
                         int32_t* chunk, int32_t* rebalances);
+/* Launch form of the iterations.  1: one launch per kernel of an
+ * iteration, every system in each launch, chunks of iterations replayed
+ * from a captured graph.  2: one persistent launch per chunk of
+ * iterations, also replayed from a captured graph, in which the kernels of
+ * an iteration become phases separated by barriers over each system's own
+ * blocks, so every system advances at its own pace; its grid is the blocks
+ * the device holds at once.  Form 2 exists for CG and BiCG (which runs
+ * through the CG kernels) with prec 0 or 1, in FP64 and for the FP32
+ * inner solves; other solves run form 1.  Both forms give the same bits.
+ * 0 (the default) lets the autotuner time both on the caller's systems
+ * and keep the faster.  A persistent launch whose barrier wait exceeds
+ * TFLOWS_BARRIER_WAIT_MS (default 2000 ms; other work holding the CUs) is
+ * abandoned and the solve runs again in form 1, which the batch keeps. */
+int tflows_batch_set_form(tflows_batch b, int32_t form);
+/* The form (1 or 2) in use for (bits, method, prec); -1 on error. */
+int32_t tflows_batch_form(tflows_batch b, int32_t bits, int32_t method, int32_t prec);
 /* FP64 solve of all K systems with `method` / `prec` (as tflows_solve):
  * x, r1 device arrays (r1 in: b, out: final residual); cap[k], tol[k] per
  * system (host); niter[k], ini_res[k] (written unless the system took the

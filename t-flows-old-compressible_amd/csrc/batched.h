@@ -21,7 +21,9 @@ struct BatchConfig {
     int grid = 0;    // blocks per launch (multiple of 8)
     int lanes = 0;   // SpMV form: lanes per row (1, 2, 4, 8)
     int chunk = 0;   // iterations per graph replay
+    int form = 0;    // 1: one launch per kernel; 2: one persistent launch per chunk (CG, Jacobi / none)
 };
+enum { kFormKernels = 1, kFormPersistent = 2 };
 
 class Batch {
    public:
@@ -57,6 +59,12 @@ class Batch {
     BatchConfig config_for(int bits, int method, int prec) const;
     void set_rebalance(bool on) { rebalance_ = on; }
     int rebalances() const { return rebalances_; }
+    // 0: the autotuner picks the form; 1 / 2 fix it (2 applies where the
+    // persistent form exists: CG with prec 0 / 1; elsewhere 1 runs)
+    void set_form(int form) { form_ = form; }
+    // false once a persistent launch gave up waiting on its barriers; the
+    // batch then keeps the per-kernel form
+    bool persistent_ok() const { return persistent_ok_; }
 
    private:
     struct Key {
@@ -66,10 +74,10 @@ class Batch {
         }
     };
     struct GraphKey {
-        int bits, method, prec, grid, lanes, chunk;
+        int bits, method, prec, grid, lanes, chunk, form;
         bool operator<(const GraphKey& o) const {
-            return std::tie(bits, method, prec, grid, lanes, chunk) <
-                   std::tie(o.bits, o.method, o.prec, o.grid, o.lanes, o.chunk);
+            return std::tie(bits, method, prec, grid, lanes, chunk, form) <
+                   std::tie(o.bits, o.method, o.prec, o.grid, o.lanes, o.chunk, o.form);
         }
     };
 
@@ -86,6 +94,13 @@ class Batch {
     void run(int bits, int method, int prec, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
              T* partials, SolverState* st, HostMirror* hm, const std::vector<SolverState>& init, bool d_ready,
              bool zero_x, const std::function<void()>& prepare, std::vector<SolverState>& out);
+    // whether the persistent form can run (method, preconditioner, at
+    // least one resident block per system) and its largest grid
+    template <typename Op>
+    int persistent_grid(int method, int prec, int lanes);
+    template <typename T, typename Op>
+    void enqueue_chunk(const GraphKey& gk, const BatchCtx& c, const Op& Aop, const Csr<T>& A,
+                       const KrylovVectors<T>& w, T* partials, SolverState* st, HostMirror* hm, hipStream_t s);
     template <typename T, typename Op>
     BatchConfig tune(int bits, int method, int prec, const Op& Aop, const Csr<T>& A, const KrylovVectors<T>& w,
                      T* partials, SolverState* st, HostMirror* hm, const std::vector<SolverState>& init,
@@ -130,6 +145,10 @@ class Batch {
     DevArray<int> perm_, eperm_;
     DevArray<double> val_orig_, val_tmp_;
     int rebalances_ = 0;
+    int form_ = 0;                     // fixed form (0: tuned)
+    bool persistent_ok_ = true;
+    DevArray<int> bar_, pfault_;       // barrier counters (one per system) and the fault word of the persistent form
+    uint64_t wait_ticks_ = 0;
     hipEvent_t ev_[5] = {};            // pacing (2), tuner timing (2), join
 };
 

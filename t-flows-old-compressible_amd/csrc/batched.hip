@@ -25,6 +25,8 @@
 //    autotuner that times a few iterations of every candidate on the
 //    caller's own systems, once per (precision, method, preconditioner).
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <cmath>
 #include <functional>
@@ -45,6 +47,13 @@ constexpr int kInnerStallMin = 100;
 constexpr double kLoosestInnerRtol = 0.1;
 int round8(int v) { return (v + kNumXcd - 1) / kNumXcd * kNumXcd; }
 const Levels kNoLevels{};
+// bound on a barrier wait of the persistent form (TFLOWS_BARRIER_WAIT_MS,
+// default 2000 ms, wall clock), read when a batch is created
+uint64_t barrier_wait_ms() {
+    const char* e = std::getenv("TFLOWS_BARRIER_WAIT_MS");
+    const long v = e ? std::atol(e) : 2000;
+    return v < 0 ? 0 : (uint64_t)v;
+}
 }  // namespace
 
 
@@ -251,6 +260,15 @@ Batch::Batch(int k, const int* n, const int* nonz, const int* const* acol, const
     TF_CHECK(hipHostMalloc((void**)&hm32_, sizeof(HostMirror) * k, hipHostMallocMapped | hipHostMallocCoherent));
     TF_CHECK(hipHostMalloc((void**)&snap_, sizeof(int) * 2 * k, hipHostMallocMapped | hipHostMallocCoherent));
     std::memset(snap_, 0, sizeof(int) * 2 * k);
+    bar_.ensure(k);
+    pfault_.ensure(1);
+    TF_CHECK(hipMemsetAsync(pfault_.p, 0, sizeof(int), bs_));
+    {
+        int dev = 0, khz = 0;
+        TF_CHECK(hipGetDevice(&dev));
+        TF_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        wait_ticks_ = (uint64_t)(khz > 0 ? khz : 100000) * barrier_wait_ms();
+    }
     for (int i = 0; i < 2; ++i) TF_CHECK(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
     for (int i = 2; i < 4; ++i) TF_CHECK(hipEventCreate(&ev_[i]));
     TF_CHECK(hipEventCreateWithFlags(&ev_[4], hipEventDisableTiming));
@@ -383,7 +401,33 @@ BatchConfig Batch::config_for(int bits, int method, int prec) const {
     if (fixed_.chunk) c.chunk = fixed_.chunk;
     if (!c.chunk) c.chunk = prec == 2 ? 1 : 8;
     if (c.grid) c.grid = std::min(gmax_, std::max(round8(k_), round8(c.grid)));
+    if (form_) c.form = form_;
+    if (!c.form || method != 0 || prec == 2 || !persistent_ok_) c.form = kFormKernels;
     return c;
+}
+
+template <typename Op>
+int Batch::persistent_grid(int method, int prec, int lanes) {
+    if (method != 0 || prec == 2 || !persistent_ok_) return 0;
+    const int resident = batch_cg_chunk_resident<Op>(lanes);
+    const int G = std::min(gmax_, resident / kNumXcd * kNumXcd);
+    return G >= round8(k_) ? G : 0;
+}
+
+// One chunk of gk.chunk iterations on stream s: the kernels of each
+// iteration, or one persistent launch.
+template <typename T, typename Op>
+void Batch::enqueue_chunk(const GraphKey& gk, const BatchCtx& c, const Op& Aop, const Csr<T>& A,
+                          const KrylovVectors<T>& w, T* partials, SolverState* st, HostMirror* hm, hipStream_t s) {
+    const Sites<T> S = Sites<T>::carve(partials, gk.grid);
+    if (gk.form == kFormPersistent) {
+        launch_batch_cg_chunk<Op, T>(c, Aop, A.n, gk.chunk, w.x, w.r1, w.p, w.q, (const T*)w.D, S, st, hm, bar_.p,
+                                     k_, pfault_.p, wait_ticks_, s);
+        return;
+    }
+    const Levels* L = gk.prec == 2 ? &sys_->levels() : &kNoLevels;
+    for (int i = 1; i <= gk.chunk; ++i)
+        enqueue_iteration<T, Op>(c, gk.method, gk.prec != 2, i, Aop, A, w, S, st, hm, L, s);
 }
 
 // The done flags after a chunk of `chunk` iterations into slot
@@ -400,14 +444,11 @@ hipGraphExec_t Batch::graph(const GraphKey& gk, const Op& Aop, const Csr<T>& A, 
                             T* partials, SolverState* st, HostMirror* hm) {
     auto it = graphs_.find(gk);
     if (it != graphs_.end()) return it->second;
-    const Levels* L = gk.prec == 2 ? &sys_->levels() : &kNoLevels;
     const BatchCtx c = ctx(gk.lanes, true);
-    const Sites<T> S = Sites<T>::carve(partials, gk.grid);
     hipGraph_t g = nullptr;
     TF_CHECK(hipStreamBeginCapture(bs_, hipStreamCaptureModeThreadLocal));
     try {
-        for (int i = 1; i <= gk.chunk; ++i)
-            enqueue_iteration<T, Op>(c, gk.method, gk.prec != 2, i, Aop, A, w, S, st, hm, L, bs_);
+        enqueue_chunk<T, Op>(gk, c, Aop, A, w, partials, st, hm, bs_);
         launch_advance(kbase_.p, gk.chunk, bs_);
         hipLaunchKernelGGL(k_done_snapshot, dim3(1), dim3(kBlock), 0, bs_, k_, gk.chunk, (const SolverState*)st,
                            (const int*)kbase_.p, snap_);
@@ -456,7 +497,50 @@ BatchConfig Batch::tune(int bits, int method, int prec, const Op& Aop, const Csr
     const std::vector<char> all(k_, 1);
     BatchConfig best;
     float best_ms = 1e30f;
+    // the persistent form: one launch of `probe` iterations on the largest
+    // resident grid and on half of it
+    constexpr bool lane_forms_p = std::is_same<Op, Csr<T>>::value || std::is_same<Op, DiffCsr32>::value;
+    const bool try_persistent = form_ != kFormKernels && persistent_grid<Op>(method, prec, 1) > 0;
+    auto time_persistent = [&](int rep) {
+        for (int lanes : kLanes) {
+            if (!lane_forms_p && lanes != 1) continue;
+            const int Gmax = persistent_grid<Op>(method, prec, lanes);
+            for (int G : {Gmax, std::max(round8(k_), Gmax / 2 / kNumXcd * kNumXcd)}) {
+                if (G <= 0 || !persistent_ok_) continue;
+                TF_CHECK(hipStreamSynchronize(bs_));
+                build_maps(all, G);
+                upload_maps(bs_);
+                const BatchCtx c = ctx(lanes, false);
+                const Sites<T> S = Sites<T>::carve(partials, G);
+                prepare();
+                TF_CHECK(hipMemcpyAsync(st, init.data(), sizeof(SolverState) * k_, hipMemcpyHostToDevice, bs_));
+                start_solve<T, Op>(c, method, prec, Aop, A, w, S, st, hm, L, bs_, d_ready, zero_x);
+                TF_CHECK(hipEventRecord(ev_[2], bs_));
+                launch_batch_cg_chunk<Op, T>(c, Aop, A.n, probe, w.x, w.r1, w.p, w.q, (const T*)w.D, S, st, hm,
+                                             bar_.p, k_, pfault_.p, wait_ticks_, bs_);
+                TF_CHECK(hipEventRecord(ev_[3], bs_));
+                TF_CHECK(hipEventSynchronize(ev_[3]));
+                int fault = 0;
+                TF_CHECK(hipMemcpy(&fault, pfault_.p, sizeof(int), hipMemcpyDeviceToHost));
+                if (fault) {   // not co-resident here: the batch keeps the per-kernel form
+                    persistent_ok_ = false;
+                    launch_set_int(pfault_.p, 0, bs_);
+                    return;
+                }
+                float ms = 0.0f;
+                TF_CHECK(hipEventElapsedTime(&ms, ev_[2], ev_[3]));
+                if (rep == 1 && ms < best_ms) {
+                    best_ms = ms;
+                    best.grid = G;
+                    best.lanes = lanes;
+                    best.form = kFormPersistent;
+                }
+            }
+        }
+    };
     for (int rep = 0; rep < 2; ++rep) {  // the first pass also loads the code objects
+        if (try_persistent) time_persistent(rep);
+        if (form_ == kFormPersistent && try_persistent && persistent_ok_) continue;
         for (int g0 : kGrids) {
             const int G = std::min(gmax_, std::max(round8(k_), g0));
             TF_CHECK(hipStreamSynchronize(bs_));
@@ -481,6 +565,7 @@ BatchConfig Batch::tune(int bits, int method, int prec, const Op& Aop, const Csr
                     best_ms = ms;
                     best.grid = G;
                     best.lanes = lanes;
+                    best.form = kFormKernels;
                 }
             }
         }
@@ -494,16 +579,23 @@ void Batch::run(int bits, int method, int prec, const Op& Aop, const Csr<T>& A, 
                 bool zero_x, const std::function<void()>& prepare, std::vector<SolverState>& out) {
     const Key key{bits, method, prec};
     if (prec == 2) sys_->prepare_dic(bits == 32, bs_);
-    const bool auto_cfg = !fixed_.grid || !fixed_.lanes;
+    const bool auto_cfg = !fixed_.grid || !fixed_.lanes || (form_ == 0 && method == 0 && prec != 2);
     if (auto_cfg && tuned_.find(key) == tuned_.end())
         tuned_[key] = tune<T, Op>(bits, method, prec, Aop, A, w, partials, st, hm, init, d_ready, zero_x, prepare);
     const BatchConfig cfg = config_for(bits, method, prec);
-    const int G = cfg.grid ? cfg.grid : std::min(gmax_, std::max(round8(k_), 2048));
+    int G = cfg.grid ? cfg.grid : std::min(gmax_, std::max(round8(k_), 2048));
     const int lanes = cfg.lanes ? cfg.lanes : 1;
     const int C = cfg.chunk;
+    int form = cfg.form;
+    if (form == kFormPersistent) {   // the grid must be resident all at once
+        const int Gp = persistent_grid<Op>(method, prec, lanes);
+        if (Gp > 0) G = std::min(G, Gp);
+        else form = kFormKernels;
+    }
     const bool jac = prec != 2;
     const Levels* L = prec == 2 ? &sys_->levels() : &kNoLevels;
-    hipGraphExec_t exec = graph<T, Op>(GraphKey{bits, method, prec, G, lanes, C}, Aop, A, w, partials, st, hm);
+    hipGraphExec_t exec =
+        graph<T, Op>(GraphKey{bits, method, prec, G, lanes, C, form}, Aop, A, w, partials, st, hm);
 
     std::vector<char> act(k_, 1);
     TF_CHECK(hipStreamSynchronize(bs_));
@@ -560,6 +652,22 @@ void Batch::run(int bits, int method, int prec, const Op& Aop, const Csr<T>& A, 
         slot = other;
     }
     TF_CHECK(hipStreamSynchronize(bs_));
+    if (form == kFormPersistent) {
+        int fault = 0;
+        TF_CHECK(hipMemcpy(&fault, pfault_.p, sizeof(int), hipMemcpyDeviceToHost));
+        if (fault) {
+            // a barrier wait gave up (other work held the CUs): the batch
+            // keeps the per-kernel form and solves again from the inputs,
+            // which prepare() restores
+            persistent_ok_ = false;
+            launch_set_int(pfault_.p, 0, bs_);
+            std::fprintf(stderr, "libtflows_krylov: a persistent batch launch gave up waiting "
+                                 "(TFLOWS_BARRIER_WAIT_MS); the batch now runs one launch per kernel\n");
+            std::fflush(stderr);
+            run<T, Op>(bits, method, prec, Aop, A, w, partials, st, hm, init, d_ready, zero_x, prepare, out);
+            return;
+        }
+    }
     out.resize(k_);
     TF_CHECK(hipMemcpy(out.data(), st, sizeof(SolverState) * k_, hipMemcpyDeviceToHost));
     for (int i = 0; i < k_; ++i)
@@ -614,7 +722,6 @@ void Batch::solve(int method, int prec, double* x, double* r1, const int* cap, c
     method = kernel_method(method);
     join(s);
     const KrylovVectors<double> w = vectors64();
-    const size_t bytes = sizeof(double) * (size_t)n_tot_;
     std::vector<SolverState> init(k_);
     for (int i = 0; i < k_; ++i) {
         init[i] = SolverState{};

@@ -6,6 +6,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/tflows_krylov.h"
 #include "batched.h"
@@ -57,27 +58,74 @@ hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
     std::abort();
 }
 
-// Host-array drop-in: upload pattern, values, x(1:N) and b, solve, download.
-// The device copy is rebuilt on every call (the caller may change any array
-// between calls); only allocations are reused while N and NONZ stay equal.
+// Host-array drop-ins.  The reference fixes the pattern once (TopolM,
+// Process/Processor.f90:100) and then calls cg / bicg with new values on it
+// every outer iteration (CalcPS.f90:217, NewUVW.f90:506, CalcZmix.f90:394).
+// The device System (validated pattern, locality order, sliced operators,
+// DIC levels, the operator choice) is therefore kept between calls: a call
+// whose N, NONZ, Acol, Arow and Adia equal the cached copy's (compared on
+// the host) uploads only the values and the vectors; any difference
+// rebuilds it.  prec1_ and resid_ share the cache.
 struct HostCache {
     std::unique_ptr<System> sys;
     int n = -1, nonz = -1;
+    std::vector<int32_t> acol, arow, adia;   // 1-based, as the caller passed them
     tflows::DevArray<double> x, r;
+    int64_t builds = 0, calls = 0;
 };
 HostCache g_cache;
+
+bool same(const std::vector<int32_t>& v, const int32_t* p, size_t count) {
+    return v.size() == count && (count == 0 || std::memcmp(v.data(), p, sizeof(int32_t) * count) == 0);
+}
+
+// The cached system for this pattern; adia may be null (resid_), then any
+// cached diagonal positions of the same Acol / Arow serve, and a new
+// system takes the diagonal found by scanning each row.
+System& dropin_system(int n, int nonz, const int32_t* Acol, const int32_t* Arow, const int32_t* Adia,
+                      hipStream_t s) {
+    ++g_cache.calls;
+    if (n <= 0) throw std::invalid_argument("N must be positive");
+    if (nonz < n + 1) throw std::invalid_argument("NONZ must be at least N + 1 (diagonal entries + 1)");
+    const size_t nnz = (size_t)nonz - 1;
+    if (g_cache.sys && g_cache.n == n && g_cache.nonz == nonz && same(g_cache.acol, Acol, (size_t)n + 1) &&
+        same(g_cache.arow, Arow, nnz) && (!Adia || same(g_cache.adia, Adia, (size_t)n)))
+        return *g_cache.sys;
+    std::vector<int32_t> adia_scan;
+    if (!Adia) {
+        // Resid has no Adia argument; the pattern is validated with the
+        // diagonal found by scanning each row
+        if (Acol[0] != 1 || Acol[n] != nonz)
+            throw std::invalid_argument("Acol must start at 1 and end at NONZ (= nnz + 1)");
+        for (int i = 1; i <= n; ++i)
+            if (Acol[i] < Acol[i - 1]) throw std::invalid_argument("Acol is not non-decreasing");
+        adia_scan.resize(n);
+        for (int i = 1; i <= n; ++i) {
+            adia_scan[i - 1] = Acol[i - 1];
+            for (int j = Acol[i - 1]; j < Acol[i]; ++j)
+                if (Arow[j - 1] == i) adia_scan[i - 1] = j;
+        }
+        Adia = adia_scan.data();
+    }
+    g_cache.sys.reset();
+    g_cache.n = g_cache.nonz = -1;
+    g_cache.sys.reset(new System(n, nonz, Acol, Arow, Adia, false, s));
+    g_cache.acol.assign(Acol, Acol + n + 1);
+    g_cache.arow.assign(Arow, Arow + nnz);
+    g_cache.adia.assign(Adia, Adia + n);
+    g_cache.n = n;
+    g_cache.nonz = nonz;
+    ++g_cache.builds;
+    return *g_cache.sys;
+}
 
 void drop_in(int method, const int32_t* N, const int32_t* NB, const int32_t* NONZ, const double* A,
              const int32_t* Acol, const int32_t* Arow, const int32_t* Adia, double* x, double* r1,
              const int32_t* prec, int32_t* niter, const double* tol, double* IniRes, double* FinRes) {
     std::lock_guard<std::mutex> lock(g_mutex);
-    const int n = *N, nb = *NB, nonz = *NONZ;
+    const int n = *N, nb = *NB;
     hipStream_t s = nullptr;
-    g_cache.sys.reset();
-    g_cache.sys.reset(new System(n, nonz, Acol, Arow, Adia, false, s));
-    g_cache.n = n;
-    g_cache.nonz = nonz;
-    System& S = *g_cache.sys;
+    System& S = dropin_system(n, *NONZ, Acol, Arow, Adia, s);
     S.set_values(A, false, s);
     g_cache.x.ensure(n);
     g_cache.r.ensure(n);
@@ -123,13 +171,15 @@ TF_DROPIN(cgs_, 2)
 void prec1_(const int32_t* N, const int32_t* NONZ, const double* A, const int32_t* Acol, const int32_t* Arow,
             const int32_t* Adia, double* D, const int32_t* prec) {
     if (guarded([&] {
+            std::lock_guard<std::mutex> lock(g_mutex);
             hipStream_t s = nullptr;
-            System S(*N, *NONZ, Acol, Arow, Adia, false, s);
+            const int n = *N;
+            System& S = dropin_system(n, *NONZ, Acol, Arow, Adia, s);
             S.set_values(A, false, s);
-            tflows::DevArray<double> d;
-            d.ensure(*N);
-            S.prec1(*prec, d.p, s);
-            TF_CHECK(hipMemcpy(D, d.p, sizeof(double) * (*N), hipMemcpyDeviceToHost));
+            g_cache.x.ensure(n);
+            S.prec1(*prec, g_cache.x.p, s);
+            TF_CHECK(hipMemcpyAsync(D, g_cache.x.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            TF_CHECK(hipStreamSynchronize(s));
         }))
         die("prec1_");
 }
@@ -138,27 +188,47 @@ void resid_(const int32_t* N, const int32_t* NB, const int32_t* NONZ, const doub
             const int32_t* Arow, const double* Ab, const double* x, double* r1) {
     (void)Ab;
     if (guarded([&] {
+            std::lock_guard<std::mutex> lock(g_mutex);
             hipStream_t s = nullptr;
             const int n = *N, nb = *NB;
-            // Resid has no Adia argument; the pattern is validated with the
-            // diagonal found by scanning each row.
-            std::unique_ptr<int32_t[]> adia(new int32_t[n]);
-            for (int i = 1; i <= n; ++i) {
-                adia[i - 1] = Acol[i - 1];
-                for (int j = Acol[i - 1]; j < Acol[i]; ++j)
-                    if (Arow[j - 1] == i) adia[i - 1] = j;
-            }
-            System S(n, *NONZ, Acol, Arow, adia.get(), false, s);
+            System& S = dropin_system(n, *NONZ, Acol, Arow, nullptr, s);
             S.set_values(A, false, s);
-            tflows::DevArray<double> dx, dr;
-            dx.ensure(n);
-            dr.ensure(n);
-            TF_CHECK(hipMemcpy(dx.p, x + nb + 1, sizeof(double) * n, hipMemcpyHostToDevice));
-            TF_CHECK(hipMemcpy(dr.p, r1, sizeof(double) * n, hipMemcpyHostToDevice));
-            S.resid(dx.p, dr.p, s);
-            TF_CHECK(hipMemcpy(r1, dr.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+            g_cache.x.ensure(n);
+            g_cache.r.ensure(n);
+            TF_CHECK(hipMemcpyAsync(g_cache.x.p, x + nb + 1, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            TF_CHECK(hipMemcpyAsync(g_cache.r.p, r1, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            S.resid(g_cache.x.p, g_cache.r.p, s);
+            TF_CHECK(hipMemcpyAsync(r1, g_cache.r.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            TF_CHECK(hipStreamSynchronize(s));
         }))
         die("resid_");
+}
+
+int tflows_dropin_stats(int64_t* pattern_builds, int64_t* calls) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if (pattern_builds) *pattern_builds = g_cache.builds;
+        if (calls) *calls = g_cache.calls;
+    });
+}
+
+int tflows_dropin_release(void) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        g_cache.sys.reset();
+        g_cache.n = g_cache.nonz = -1;
+        g_cache.acol.clear();
+        g_cache.arow.clear();
+        g_cache.adia.clear();
+        g_cache.x.release();
+        g_cache.r.release();
+    });
+}
+
+int32_t tflows_system_dic_sweep(tflows_system s) {
+    int32_t v = 0;
+    if (guarded([&] { v = s->sys->dic_form(); })) return -1;
+    return v;
 }
 
 int tflows_system_create(int32_t n, int32_t nonz, const int32_t* acol, const int32_t* arow, const int32_t* adia,
@@ -281,6 +351,17 @@ int tflows_system_fp32_layout(tflows_system s, int32_t* sliced, int64_t* stored_
     });
 }
 
+int tflows_system_fp64_layout(tflows_system s, int32_t* sliced, int64_t* stored_entries, void* stream) {
+    return guarded([&] {
+        if (!sliced || !stored_entries) throw std::invalid_argument("null output pointer");
+        const hipStream_t st = as_stream(stream);
+        const bool sl = s->sys->op64_sliced(st);
+        *sliced = sl ? (s->sys->sell64_col_bytes(st) == 2 ? 2 : 1) : 0;
+        *stored_entries = sl ? s->sys->sell64_entries(st) : s->sys->nnz();
+        TF_CHECK(hipStreamSynchronize(st));
+    });
+}
+
 int tflows_system_set_lanes(tflows_system s, int32_t lanes) {
     return guarded([&] { s->sys->set_lanes(lanes); });
 }
@@ -329,10 +410,23 @@ int tflows_batch_configure(tflows_batch b, int32_t grid, int32_t lanes, int32_t 
     });
 }
 
+int tflows_batch_set_form(tflows_batch b, int32_t form) {
+    return guarded([&] {
+        if (form < 0 || form > 2) throw std::invalid_argument("form must be 0 (tuned), 1 (kernels) or 2 (persistent)");
+        b->batch->set_form(form);
+    });
+}
+
+int32_t tflows_batch_form(tflows_batch b, int32_t bits, int32_t method, int32_t prec) {
+    int32_t v = -1;
+    if (guarded([&] { v = b->batch->config_for(bits, tflows::kernel_method(method), prec).form; })) return -1;
+    return v;
+}
+
 int tflows_batch_config(tflows_batch b, int32_t bits, int32_t method, int32_t prec, int32_t* grid, int32_t* lanes,
                         int32_t* chunk, int32_t* rebalances) {
     return guarded([&] {
-        const tflows::BatchConfig c = b->batch->config_for(bits, method, prec);
+        const tflows::BatchConfig c = b->batch->config_for(bits, tflows::kernel_method(method), prec);
         if (grid) *grid = c.grid;
         if (lanes) *lanes = c.lanes;
         if (chunk) *chunk = c.chunk;

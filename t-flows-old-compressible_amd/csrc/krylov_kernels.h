@@ -98,8 +98,24 @@ template <typename T>
 void launch_cgf_init_z(int n, const T* r, const T* D, T* z, const Sites<T>& S, const SolverState* st,
                        hipStream_t s);
 
+// Persistent batched CG with Jacobi (or no) preconditioning: one launch
+// runs `chunk` iterations (kbase + 1 .. kbase + chunk) of every system of a
+// batch, the three kernels of an iteration as three phases separated by
+// barriers over each system's blocks (bar: one counter per system, zeroed
+// by the launcher).  The grid (S.grid) must be resident all at once:
+// batch_cg_chunk_resident gives how many blocks the device holds.  A
+// barrier wait beyond wait_ticks raises *fault and every block stops.
+template <typename Op, typename T>
+void launch_batch_cg_chunk(const BatchCtx& c, const Op& A, int n, int chunk, T* x, T* r, T* p, T* q, const T* D,
+                           const Sites<T>& S, SolverState* st, HostMirror* hm, int* bar, int nsys, int* fault,
+                           uint64_t wait_ticks, hipStream_t s);
+template <typename Op>
+int batch_cg_chunk_resident(int lanes);
+
 void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t s);
 void launch_set_int(int* p, int v, hipStream_t s);
+// *p = v, ordered on stream s (a kernel, so it may sit inside a graph)
+void launch_set_double(double* p, double v, hipStream_t s);
 // level-major DIC sweeps (see Levels in system.h)
 template <typename T>
 void launch_plan_in(int n, const int* order, const T* u, const T* D, T* up, T* dp, const SolverState* st,

@@ -948,6 +948,9 @@ __global__ __launch_bounds__(kBlock) void k_resid(BatchCtx ctx, Csr<T> A, const 
 __global__ void k_set_int(int* p, int v) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
 }
+__global__ void k_set_double(double* p, double v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
 __global__ void k_advance(int* kbase, int c) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *kbase += c;
 }
@@ -961,6 +964,130 @@ __global__ void k_convert(int64_t count, const TI* __restrict__ in, TO* __restri
 // ---------------------------------------------------------------------------
 // host-side launchers (explicitly instantiated below)
 // ---------------------------------------------------------------------------
+// ===========================================================================
+// Persistent batched CG (Jacobi or no preconditioner): one launch runs
+// `chunk` iterations of every system of a batch.  The three kernels of an
+// iteration (k_cg_p, k_cg_spmv, k_cg_update) become three phases of one
+// block, and the kernel boundaries between them become barriers over the
+// blocks of the block's own system, so each system advances at its own
+// pace.  Every phase does exactly what its kernel does, on the same tiles
+// and the same reduction sites, so the results are the per-kernel form's
+// bit for bit.
+//
+// Barrier: every thread releases its stores at agent scope (they reach the
+// L2 and the L2 is written back, so blocks on other XCDs see them); thread
+// 0 adds one arrival to the system's counter and waits until all of the
+// system's blocks have arrived; every thread then acquires at agent scope.
+// Co-residency of the grid is what makes the wait end: the host sizes the
+// grid to the blocks the device can hold at once (occupancy query).  The
+// wait is bounded: every 64 polls it checks the fault word and the wall
+// clock, and past wait_ticks (other work held the CUs) it raises the fault
+// word and the block stops; every other waiting block sees the fault and
+// stops too, so every wavefront ends, and the host reruns the solve in the
+// per-kernel form.
+// ===========================================================================
+__device__ __forceinline__ bool system_barrier(int* ctr, int target, int* fault, uint64_t t0, uint64_t wait_ticks) {
+    __shared__ int s_ok;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int ok = 1;
+        unsigned polls = 0;
+        while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if ((++polls & 63u) == 0) {
+                const bool late = (uint64_t)wall_clock64() - t0 > wait_ticks;
+                if (late || __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return s_ok != 0;
+}
+
+template <typename Op, int L, typename T = typename OpValue<Op>::type>
+__global__ __launch_bounds__(kBlock) void k_batch_cg_chunk(BatchCtx ctx, Op A, int n, int chunk, T* __restrict__ x,
+                                                           T* __restrict__ r, T* __restrict__ p, T* __restrict__ q,
+                                                           const T* __restrict__ D, T* __restrict__ partB,
+                                                           T* __restrict__ partD, SolverState* st_base,
+                                                           HostMirror* hm_base, int* __restrict__ bar,
+                                                           int* __restrict__ fault, uint64_t wait_ticks) {
+    TF_STATE;
+    const int sys = ctx.sys();
+    const int nblk = ctx.sys_blk0[sys + 1] - ctx.sys_blk0[sys];
+    // A system that has stopped returns at once.  Its partial sums are not
+    // rewritten: nothing reads a finished system's sites again (the
+    // rebalancing skips it), and a block that stops inside a phase must
+    // not write a site that the other blocks of its system may still be
+    // reading in that phase.
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = st->done;
+    __syncthreads();
+    if (s_done) return;
+    int rb, re;
+    ctx_rows(ctx, n, rb, re);
+    const TileSpan sp = ctx_span(ctx, A.n);
+    const uint64_t t0 = (uint64_t)wall_clock64();
+    int arrived = 0;
+    for (int j = 1; j <= chunk; ++j) {
+        const int k = ctx.k(j);
+        // k_cg_p: stopping test (site B), rho, p = z + beta p  (CG.f90:172-187)
+        const double rho_prev = st->rho[(k - 1) & 1];
+        T tot[2];
+        if (check_stop<T>(ctx, k, partB, st, hm, tot)) return;   // every block of the system takes this exit
+        const T rho = tot[1];
+        if (ctx.leader()) st->rho[k & 1] = (double)rho;
+        const bool first = k == 1;
+        const T beta = first ? T(0) : rho / (T)rho_prev;
+        for (int i = rb + (int)threadIdx.x; i < re; i += kBlock) {
+            const T zi = jac(r[i], D[i]);
+            p[i] = first ? zi : zi + beta * p[i];
+        }
+        if (!system_barrier(bar + sys, ++arrived * nblk, fault, t0, wait_ticks)) return;
+        // k_cg_spmv: q = A p, (p, q) -> site D  (CG.f90:193-218)
+        {
+            Gather1<T> g{p};
+            DotEpi<T> epi{p, q, {T(0)}};
+            for (int t = sp.first; t < sp.end; t += sp.step) {
+                int r0, r1;
+                ctx_tile_rows(ctx, A.n, t, r0, r1);
+                op_tile<1, L>(A, r0, r1, g, epi);
+            }
+            site_write<T, 1>(epi.acc, partD);
+        }
+        if (!system_barrier(bar + sys, ++arrived * nblk, fault, t0, wait_ticks)) return;
+        // k_cg_update: alfa, x += alfa p, r -= alfa q, ||r||^2 and the next
+        // rho -> site B  (CG.f90:219-253)
+        {
+            const double rho_k = st->rho[k & 1];
+            T pq[1];
+            ctx_total<T, 1>(ctx, partD, pq);
+            const T alfa = (T)rho_k / pq[0];
+            T v[2] = {T(0), T(0)};
+            for (int i = rb + (int)threadIdx.x; i < re; i += kBlock) {
+                x[i] = x[i] + alfa * p[i];
+                const T ri = r[i] - alfa * q[i];
+                r[i] = ri;
+                v[0] += ri * ri;
+                v[1] += ri * jac(ri, D[i]);
+            }
+            site_write<T, 2>(v, partB);
+        }
+        if (j < chunk && !system_barrier(bar + sys, ++arrived * nblk, fault, t0, wait_ticks)) return;
+    }
+}
+
+__global__ void k_zero_ints(int n, int* __restrict__ a) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = 0;
+}
+
 #define TF_LAUNCH(k, grid, stream) hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, stream
 
 // Calls f(std::integral_constant<int, L>) for the SpMV form `lanes`.
@@ -1154,8 +1281,39 @@ void launch_resid(const Csr<T>& A, const T* x, T* r, int grid, hipStream_t s, co
         TF_LAUNCH((k_resid<T, L>), grid, s), c, A, x, r);
     });
 }
+template <typename Op, typename T>
+void launch_batch_cg_chunk(const BatchCtx& c, const Op& A, int n, int chunk, T* x, T* r, T* p, T* q, const T* D,
+                           const Sites<T>& S, SolverState* st, HostMirror* hm, int* bar, int nsys, int* fault,
+                           uint64_t wait_ticks, hipStream_t s) {
+    if (!c.blk_sys) throw std::invalid_argument("the persistent chunk runs under a batch context");
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(kBlock), 0, s, nsys, bar);
+    with_lanes(c.lanes, [&](auto lc_) {
+        TF_L;
+        TF_LAUNCH((k_batch_cg_chunk<Op, L>), S.grid, s), c, A, n, chunk, x, r, p, q, D, S.B, S.D, st, hm, bar, fault,
+            wait_ticks);
+    });
+}
+
+template <typename Op>
+int batch_cg_chunk_resident(int lanes) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("hipGetDevice failed");
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        throw std::runtime_error("hipDeviceGetAttribute failed");
+    with_lanes(lanes, [&](auto lc_) {
+        TF_L;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_batch_cg_chunk<Op, L>, kBlock, 0) !=
+            hipSuccess)
+            throw std::runtime_error("occupancy query of the persistent batch kernel failed");
+    });
+    return per_cu * cus;
+}
+
 void launch_set_int(int* p, int v, hipStream_t s) {
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(kWave), 0, s, p, v);
+}
+void launch_set_double(double* p, double v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(kWave), 0, s, p, v);
 }
 void launch_advance(int* kbase, int c, hipStream_t s) {
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(kWave), 0, s, kbase, c);
@@ -1219,7 +1377,11 @@ TF_INSTANTIATE_PLAN(float)
                                            const SolverState*, hipStream_t);                                     \
     template void launch_cgs_spmv_r<Op, T>(const BatchCtx&, int, const Op&, const T*, T*, T*, const T*,         \
                                            const Sites<T>&, const SolverState*, hipStream_t);                    \
-    template void launch_spmv<Op, T>(const Op&, const T*, T*, int, hipStream_t, const BatchCtx&);
+    template void launch_spmv<Op, T>(const Op&, const T*, T*, int, hipStream_t, const BatchCtx&);          \
+    template void launch_batch_cg_chunk<Op, T>(const BatchCtx&, const Op&, int, int, T*, T*, T*, T*, const T*,  \
+                                               const Sites<T>&, SolverState*, HostMirror*, int*, int, int*,     \
+                                               uint64_t, hipStream_t);                                          \
+    template int batch_cg_chunk_resident<Op>(int);
 
 TF_INSTANTIATE(double)
 TF_INSTANTIATE(float)

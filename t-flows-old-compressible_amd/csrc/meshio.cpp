@@ -52,6 +52,11 @@ public:
         if (!path) throw std::invalid_argument("null file name");
         f_ = std::fopen(path, "rb");
         if (!f_) throw std::runtime_error(std::string("cannot open ") + path);
+        // the file's size bounds every record length before any buffer grows
+        if (std::fseek(f_, 0, SEEK_END) != 0) fail("cannot seek");
+        const long sz = std::ftell(f_);
+        if (sz < 0 || std::fseek(f_, 0, SEEK_SET) != 0) fail("cannot seek");
+        size_ = (uint64_t)sz;
     }
     ~RecordFile() {
         if (f_) std::fclose(f_);
@@ -67,6 +72,8 @@ public:
             if (std::fread(&head, 4, 1, f_) != 1) fail("end of file where a record was expected");
             const bool more = head < 0;
             const uint64_t len = more ? (uint64_t)(-(int64_t)head) : (uint64_t)head;
+            const long pos = std::ftell(f_);
+            if (pos < 0 || len + 4 > size_ - (uint64_t)pos) fail("record shorter than its count");
             const size_t at = out.size();
             out.resize(at + len);
             if (len && std::fread(out.data() + at, 1, len, f_) != len) fail("record shorter than its count");
@@ -118,6 +125,7 @@ public:
 private:
     [[noreturn]] void fail(const std::string& msg) { throw std::runtime_error(path_ + ": " + msg); }
     std::string path_;
+    uint64_t size_ = 0;
     std::FILE* f_ = nullptr;
 };
 
@@ -282,9 +290,15 @@ int copy_out(const std::vector<T>& v, T* dst, int64_t count) {
 // (state, VISc, DENc, and CONc, CAPc when HOT), boundary conditions in file
 // order (type, then U V W [P for PRESSURE] [Zmix when HOT], or FILE and a
 // profile name) and initial conditions (U V W [Zmix], or FILE and a name).
-// Lines are read as Library/ReadC.f90 does: a line whose first column is one
-// of ! # $ % / & [ ] { } : ; @ < > * is a comment, blank lines are skipped,
-// tokens are separated by characters below '!'.
+// Lines are read as Library/ReadC.f90 does: a line is read as A300, so
+// columns past 300 are never seen; a line of spaces only is skipped
+// (`string == ''`, :22), a line whose first column is one of
+// ! # $ % / & [ ] { } : ; @ < > * is a comment, and tokens are separated by
+// characters below '!'.  A line holding only tabs or other control
+// characters is a data line without tokens, which the caller's next read
+// refuses.  ReadC closes a token only when a separator follows it within
+// column 299 (:44-51), so a token reaching column 299 is refused.  A
+// carriage return ending the line (a CRLF file) is dropped.
 // ---------------------------------------------------------------------------
 struct tflows_bfile_s {
     int nmat = 0, nb = 0, ninit = 0;
@@ -305,10 +319,14 @@ struct BLines {
         while (std::getline(in, line)) {
             ++lineno;
             if (!line.empty() && line.back() == '\r') line.pop_back();
+            if (line.size() > 300) line.resize(300);
             bool blank = true;
-            for (char ch : line) blank = blank && (unsigned char)ch < '!';
+            for (char ch : line) blank = blank && ch == ' ';
             if (blank) continue;
             if (std::strchr("!#$%/&[]{}:;@<>*", line[0])) continue;
+            if (line.size() > 298 && (unsigned char)line[298] >= '!')
+                throw std::runtime_error(where() + ": a token reaches column 299 (ReadC reads 300 columns and "
+                                                   "closes tokens only up to column 298)");
             std::vector<std::string> tok;
             size_t i = 0;
             while (i < line.size()) {

@@ -270,6 +270,16 @@ double System::true_residual(const double* x, const double* b, double* r, hipStr
 
 SolveResult System::solve_mixed(int method, int prec, double* x, const double* b, double* r_out, int cap,
                                 double tol, double inner_rtol, int max_outer, int* outer_steps, hipStream_t s) {
+    auto run = [&] {
+        return solve_mixed_ext(method, prec, x, b, r_out, cap, tol, inner_rtol, max_outer, outer_steps, s);
+    };
+    if (prec != 2) return run();
+    return dic_retry(&x, 1, s, run);
+}
+
+SolveResult System::solve_mixed_ext(int method, int prec, double* x, const double* b, double* r_out, int cap,
+                                    double tol, double inner_rtol, int max_outer, int* outer_steps,
+                                    hipStream_t s) {
     if (!reordered_) return solve_mixed_impl(method, prec, x, b, r_out, cap, tol, inner_rtol, max_outer, outer_steps, s);
     double *xi = ext_buffer(0), *bi = ext_buffer(1), *ri = ext_buffer(2);
     to_internal((const double*)x, xi, s);
@@ -397,6 +407,15 @@ SolveResult System::solve_mixed_impl(int method, int prec, double* x, const doub
 // refinement step, so the iteration count stays close to the FP64 CG's.
 SolveResult System::solve_mixed_reliable(int prec, double* x, const double* b, double* r_out, int cap, double tol,
                                          int interval, int* outer_steps, hipStream_t s) {
+    auto run = [&] {
+        return solve_mixed_reliable_ext(prec, x, b, r_out, cap, tol, interval, outer_steps, s);
+    };
+    if (prec != 2) return run();
+    return dic_retry(&x, 1, s, run);
+}
+
+SolveResult System::solve_mixed_reliable_ext(int prec, double* x, const double* b, double* r_out, int cap,
+                                             double tol, int interval, int* outer_steps, hipStream_t s) {
     if (!reordered_) return solve_mixed_reliable_impl(prec, x, b, r_out, cap, tol, interval, outer_steps, s);
     double *xi = ext_buffer(0), *bi = ext_buffer(1), *ri = ext_buffer(2);
     to_internal((const double*)x, xi, s);
@@ -480,7 +499,9 @@ SolveResult System::solve_mixed_reliable_dic(double* x, const double* b, double*
             for (;;) {
                 // this chunk's pause target for the FP32 recurrence
                 const double target = std::max(tol, kRelDrop * err);
-                TF_CHECK(hipMemcpy(&state32_.p->tol, &target, sizeof(double), hipMemcpyHostToDevice));
+                // written on s, behind the chunk before it (a copy on the
+                // null stream is not ordered after a non-blocking stream)
+                launch_set_double(&state32_.p->tol, target, s);
                 for (int i = 0; i < interval && k_next + i <= cap + 1; ++i)
                     enqueue_iteration<float, Op>(ctx(), 0, false, k_next + i, Aop, A32, w, S32, state32_.p,
                                                  mirror32_, L, s);

@@ -15,6 +15,11 @@ namespace tflows {
 struct HipError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
+// A one-launch DIC sweep gave up waiting on its dependencies (the rows it
+// left hold NaN); raised by System::check_dic.
+struct DicWaitFault : HipError {
+    using HipError::HipError;
+};
 
 #define TF_CHECK(expr)                                                                            \
     do {                                                                                          \
@@ -85,7 +90,8 @@ struct Levels {
     // of one forward level, so no row waits on a row of its own
     // wavefront; the backward sweep runs over the positions in reverse.
     // A row that waits longer than wait_ticks gives up, raises the fault
-    // word and leaves NaN, which the host turns into an error.
+    // word and leaves NaN; the host then reruns the solve in mode 1
+    // (System::dic_retry).
     bool plan = false;
     int mode = 0;                        // 0 matrix rows by level lists, 1 / 2 as above
     int npos = 0;                        // positions (rows and padding)
@@ -231,6 +237,15 @@ class System {
         sell32(s);
         return sell_entries_;
     }
+    // the same for the sliced FP64 operator
+    int sell64_col_bytes(hipStream_t s) {
+        sell64(s);
+        return sell64_dc_.p ? 2 : 4;
+    }
+    int64_t sell64_entries(hipStream_t s) {
+        sell64(s);
+        return sell64_entries_;
+    }
     const float* diag32(hipStream_t s);
 
     // Solve with x, r1 device pointers (x(1:N), r1(1:N)); `stale_error` is
@@ -289,14 +304,48 @@ class System {
     // values of the level-major triangles for the next DIC sweeps (FP64,
     // and FP32 rounded from them when fp32)
     void prepare_dic(bool fp32, hipStream_t s);
-    // throws when a one-launch sweep of the last solve gave up waiting
+    // throws DicWaitFault when a one-launch sweep of the last solve gave up
+    // waiting
     void check_dic(hipStream_t s) const;
+    // the DIC sweep form this system runs (0, 1 or 2; builds the levels)
+    int dic_form() { return levels().plan ? levels_.mode : 0; }
+    // From now on sweep one level per launch (form 1), which needs no
+    // co-resident grid; the one-launch form stays off for this system.
+    void latch_dic_form1();
+    // Runs a DIC solve `f`, whose in/out vectors are the `count` device
+    // arrays `vecs` of n rows each.  When a one-launch sweep in it gave up
+    // waiting (another process or stream held the CUs past the wait bound),
+    // the vectors are restored, the system latches form 1 and f runs again;
+    // a second failure propagates.
+    template <class F>
+    SolveResult dic_retry(double* const* vecs, int count, hipStream_t s, F&& f) {
+        if (dic_form() != 2) return f();
+        dic_backup_.ensure((size_t)count * n_);
+        for (int k = 0; k < count; ++k)
+            TF_CHECK(hipMemcpyAsync(dic_backup_.p + (size_t)k * n_, vecs[k], sizeof(double) * n_,
+                                    hipMemcpyDeviceToDevice, s));
+        try {
+            return f();
+        } catch (const DicWaitFault&) {
+            for (int k = 0; k < count; ++k)
+                TF_CHECK(hipMemcpyAsync(vecs[k], dic_backup_.p + (size_t)k * n_, sizeof(double) * n_,
+                                        hipMemcpyDeviceToDevice, s));
+            latch_dic_form1();
+            return f();
+        }
+    }
     template <typename T>
     void dic_apply(const Csr<T>& A, const T* D, const T* u, T* z, const SolverState* st, hipStream_t s);
 
    private:
+    SolveResult solve_ext(int method, int prec, double* x, double* r1, int cap, double tol, double stale_error,
+                          hipStream_t s);
     SolveResult solve_impl(int method, int prec, double* x, double* r1, int cap, double tol, double stale_error,
                            hipStream_t s);
+    SolveResult solve_mixed_ext(int method, int prec, double* x, const double* b, double* r_out, int cap,
+                                double tol, double inner_rtol, int max_outer, int* outer_steps, hipStream_t s);
+    SolveResult solve_mixed_reliable_ext(int prec, double* x, const double* b, double* r_out, int cap, double tol,
+                                         int interval, int* outer_steps, hipStream_t s);
     SolveResult solve_mixed_impl(int method, int prec, double* x, const double* b, double* r_out, int cap,
                                  double tol, double inner_rtol, int max_outer, int* outer_steps, hipStream_t s);
     SolveResult solve_mixed_reliable_dic(double* x, const double* b, double* r_out, int cap, double tol,
@@ -344,6 +393,7 @@ class System {
     std::vector<int> h_rowptr_, h_col_, h_dia_;
     Levels levels_;
     bool levels_ready_ = false;
+    DevArray<double> dic_backup_;   // inputs of a DIC solve, for dic_retry
     int grid_;
     int lanes_ = 1;
     int strided_ = 0;

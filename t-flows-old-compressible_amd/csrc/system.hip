@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <cstdlib>
 #include <thread>
@@ -492,8 +493,18 @@ void System::check_dic(hipStream_t s) const {
     TF_CHECK(hipMemcpyAsync(&fault, levels_.fault.p, sizeof(int), hipMemcpyDeviceToHost, s));
     TF_CHECK(hipStreamSynchronize(s));
     if (fault)
-        throw HipError("DIC sweep: a row waited on its dependencies beyond TFLOWS_DIC_WAIT_MS; the "
-                       "preconditioned vector is invalid");
+        throw DicWaitFault("DIC sweep: a row waited on its dependencies beyond TFLOWS_DIC_WAIT_MS; the "
+                           "preconditioned vector is invalid");
+}
+
+void System::latch_dic_form1() {
+    levels();
+    if (!levels_.plan || levels_.mode != 2) return;
+    levels_.mode = 1;
+    std::fprintf(stderr,
+                 "libtflows_krylov: a one-launch DIC sweep gave up waiting (TFLOWS_DIC_WAIT_MS); this system "
+                 "now sweeps one level per launch\n");
+    std::fflush(stderr);
 }
 
 void System::prepare_dic(bool fp32, hipStream_t s) {
@@ -751,6 +762,7 @@ static void pace_graph(const BatchCtx& c, int method, int cap, const Op& Aop, co
     key.append(reinterpret_cast<const char*>(&method), sizeof(method));
     key.append(reinterpret_cast<const char*>(&c.lanes), sizeof(c.lanes));
     key.append(reinterpret_cast<const char*>(&c.strided), sizeof(c.strided));
+    key.append(reinterpret_cast<const char*>(&L->mode), sizeof(L->mode));   // the sweep form captured
     add(op_key(Aop));
     add(A.rowptr), add(A.col), add(A.dia), add(A.val);
     add(w.x), add(w.r1), add(w.D), add(w.p), add(w.q), add(w.r2), add(w.p2), add(w.q2), add(w.u1), add(w.u2);
@@ -888,6 +900,13 @@ void run_solve(const BatchCtx& c, int method, int prec, int cap, const Op& Aop, 
 
 SolveResult System::solve(int method, int prec, double* x, double* r1, int cap, double tol, double stale_error,
                           hipStream_t s) {
+    if (prec != 2) return solve_ext(method, prec, x, r1, cap, tol, stale_error, s);
+    double* vecs[2] = {x, r1};
+    return dic_retry(vecs, 2, s, [&] { return solve_ext(method, prec, x, r1, cap, tol, stale_error, s); });
+}
+
+SolveResult System::solve_ext(int method, int prec, double* x, double* r1, int cap, double tol,
+                              double stale_error, hipStream_t s) {
     if (!reordered_) return solve_impl(method, prec, x, r1, cap, tol, stale_error, s);
     double *xi = ext_buffer(0), *ri = ext_buffer(1);
     to_internal((const double*)x, xi, s);

@@ -87,6 +87,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.tflows_resid.argtypes = [vp, vp, vp, vp]
     lib.tflows_system_levels.restype = C.c_int32
     lib.tflows_system_levels.argtypes = [vp]
+    lib.tflows_dropin_stats.restype = C.c_int
+    lib.tflows_dropin_stats.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    lib.tflows_dropin_release.restype = C.c_int
+    lib.tflows_dropin_release.argtypes = []
+    lib.tflows_system_dic_sweep.restype = C.c_int32
+    lib.tflows_system_dic_sweep.argtypes = [vp]
     _lib = lib
     _bind_extensions(lib)
     return lib
@@ -121,9 +127,10 @@ def _bind_extensions(lib):
     if hasattr(lib, "tflows_system_reordered"):
         lib.tflows_system_reordered.restype = C.c_int32
         lib.tflows_system_reordered.argtypes = [vp]
-    if hasattr(lib, "tflows_system_fp32_layout"):
-        lib.tflows_system_fp32_layout.restype = C.c_int
-        lib.tflows_system_fp32_layout.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), vp]
+    for name in ("tflows_system_fp32_layout", "tflows_system_fp64_layout"):
+        if hasattr(lib, name):
+            getattr(lib, name).restype = C.c_int
+            getattr(lib, name).argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), vp]
     if hasattr(lib, "tflows_batch_create"):
         pp = C.POINTER(vp)
         lib.tflows_batch_create.restype = C.c_int
@@ -138,6 +145,10 @@ def _bind_extensions(lib):
         lib.tflows_batch_configure.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         lib.tflows_batch_config.restype = C.c_int
         lib.tflows_batch_config.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _ip, _ip, _ip, _ip]
+        lib.tflows_batch_set_form.restype = C.c_int
+        lib.tflows_batch_set_form.argtypes = [vp, C.c_int32]
+        lib.tflows_batch_form.restype = C.c_int32
+        lib.tflows_batch_form.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
         lib.tflows_batch_solve.restype = C.c_int
         lib.tflows_batch_solve.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp]
         lib.tflows_batch_solve_mixed.restype = C.c_int
@@ -260,6 +271,19 @@ def resid(N, NB, NONZ, A, Acol, Arow, Ab, x, r1):
                np.ascontiguousarray(Ab, np.float64), x, r1)
 
 
+def dropin_stats() -> dict:
+    """Device pattern builds and calls of the drop-ins so far: a call on the
+    pattern of the one before uploads only values and vectors."""
+    b, c = C.c_int64(0), C.c_int64(0)
+    _check(load_library().tflows_dropin_stats(C.byref(b), C.byref(c)))
+    return {"pattern_builds": int(b.value), "calls": int(c.value)}
+
+
+def dropin_release():
+    """Free the drop-ins' cached device pattern."""
+    _check(load_library().tflows_dropin_release())
+
+
 def solve_system(method: str, m, b, x0=None, prec=1, niter=2000, tol=0.0, nb=0):
     """Convenience wrapper over the drop-in for a meshgen.TFlowSMatrix."""
     x = np.zeros(nb + m.n + 1)
@@ -304,6 +328,21 @@ class DeviceSystem:
                                              _ptr(self._adia), 1, _stream(stream), C.byref(h)))
         self.handle = h
         self.set_values(self.values, stream)
+
+    @classmethod
+    def from_device(cls, n, nonz, acol, arow, adia, aval, stream=None):
+        """A system from cuda tensors already in the T-FlowS layout (int32
+        acol (N+1), arow and float64 aval (NONZ), int32 adia (N))."""
+        self = cls.__new__(cls)
+        self.lib = load_library()
+        self.n, self.nonz, self.matrix = int(n), int(nonz), None
+        self._acol, self._arow, self._adia, self.values = acol, arow, adia, aval
+        h = C.c_void_p()
+        _check(self.lib.tflows_system_create(self.n, self.nonz, _ptr(acol), _ptr(arow), _ptr(adia), 1,
+                                             _stream(stream), C.byref(h)))
+        self.handle = h
+        self.set_values(aval, stream)
+        return self
 
     def set_values(self, aval, stream=None):
         """aval: cuda float64 tensor of length NONZ (or nnz)."""
@@ -380,6 +419,14 @@ class DeviceSystem:
     def levels(self) -> int:
         return int(self.lib.tflows_system_levels(self.handle))
 
+    def dic_sweep(self) -> int:
+        """Form of the DIC sweeps (2 one launch per triangle, 1 one launch
+        per level over the level-major copy, 0 over the matrix rows)."""
+        v = int(self.lib.tflows_system_dic_sweep(self.handle))
+        if v < 0:
+            _check(v)
+        return v
+
     @property
     def reordered(self) -> bool:
         """True when the matrix is stored in a locality order internally."""
@@ -391,6 +438,14 @@ class DeviceSystem:
         off-diagonal entries held, padding included}."""
         sl, ent = C.c_int32(0), C.c_int64(0)
         _check(self.lib.tflows_system_fp32_layout(self.handle, C.byref(sl), C.byref(ent), _stream(stream)))
+        return {"sliced": sl.value > 0, "col_bytes": 2 if sl.value == 2 else 4, "stored_entries": int(ent.value)}
+
+    def fp64_layout(self, stream=None) -> dict:
+        """Layout of the FP64 operator of the iterations: {"sliced": bool,
+        "col_bytes": 4, or 2 for 16-bit column offsets, "stored_entries":
+        entries held, padding included}."""
+        sl, ent = C.c_int32(0), C.c_int64(0)
+        _check(self.lib.tflows_system_fp64_layout(self.handle, C.byref(sl), C.byref(ent), _stream(stream)))
         return {"sliced": sl.value > 0, "col_bytes": 2 if sl.value == 2 else 4, "stored_entries": int(ent.value)}
 
     def set_lanes(self, lanes: int):
@@ -455,11 +510,17 @@ class DeviceBatch:
     def configure(self, grid: int = 0, lanes: int = 0, chunk: int = 0, rebalance: bool = True):
         _check(self.lib.tflows_batch_configure(self.handle, grid, lanes, chunk, 1 if rebalance else 0))
 
+    def set_form(self, form: int):
+        """0: tuned; 1: one launch per kernel; 2: one persistent launch per
+        chunk of iterations (CG / BiCG with prec 0 or 1)."""
+        _check(self.lib.tflows_batch_set_form(self.handle, form))
+
     def config(self, method: str = "cg", prec: int = 1, bits: int = 64) -> dict:
         g, l, c, r = (C.c_int32(0) for _ in range(4))
         _check(self.lib.tflows_batch_config(self.handle, bits, METHODS[method], prec, C.byref(g), C.byref(l),
                                             C.byref(c), C.byref(r)))
-        return dict(grid=g.value, lanes=l.value, chunk=c.value, rebalances=r.value)
+        form = int(self.lib.tflows_batch_form(self.handle, bits, METHODS[method], prec))
+        return dict(grid=g.value, lanes=l.value, chunk=c.value, rebalances=r.value, form=form)
 
     def _arrays(self, cap, tol, fin_in):
         cap = np.ascontiguousarray(np.broadcast_to(np.asarray(cap, np.int32), (self.k,)))

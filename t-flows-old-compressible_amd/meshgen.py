@@ -359,6 +359,86 @@ def pressure_matrix(mesh: BoxMesh, pin_cell: int = 1, perm=None) -> TFlowSMatrix
     return m
 
 
+def pressure_matrix_torch(mesh: BoxMesh, pin_cell: int = 1, device="cuda"):
+    """pressure_matrix (natural order) evaluated with torch on ``device``, for
+    sizes whose host assembly takes minutes (256^3).  The same element-wise
+    formulas in the same order, so the arrays are the host recipe's bit for
+    bit.  Returns (n, nonz, acol, arow, adia, aval): int32 / float64 tensors
+    in the T-FlowS layout (1-based, NONZ = nnz + 1, last slot 0)."""
+    import torch
+    f64 = torch.float64
+    nx, ny, nz = mesh.shape
+    n = nx * ny * nz
+
+    def t(a):
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=f64, device=device)
+
+    dx, dy, dz = (t(a) for a in mesh.spacings())
+    xc, yc, zc = (t(a) for a in mesh.centres())
+    DX, DY, DZ = torch.meshgrid(dx, dy, dz, indexing="ij")
+
+    def flat(a):
+        return a.permute(2, 1, 0).contiguous().reshape(-1)
+
+    ax = torch.zeros((nx, ny, nz), dtype=f64, device=device)
+    ay = torch.zeros_like(ax)
+    az = torch.zeros_like(ax)
+    if nx > 1:
+        ax[:-1] = (DY * DZ)[:-1] / torch.diff(xc)[:, None, None]
+    if ny > 1:
+        ay[:, :-1] = (DX * DZ)[:, :-1] / torch.diff(yc)[None, :, None]
+    if nz > 1:
+        az[:, :, :-1] = (DX * DY)[:, :, :-1] / torch.diff(zc)[None, None, :]
+    hi = [flat(ax), flat(ay), flat(az)]
+    strides = (1, nx, nx * ny)
+    lo = []
+    for a, st in zip(hi, strides):
+        b = torch.zeros_like(a)
+        if st < n:
+            b[st:] = a[:-st]
+        lo.append(b)
+    diag = torch.zeros(n, dtype=f64, device=device)
+    for a in hi:
+        diag += a
+    for a in lo:
+        diag += a
+    if pin_cell:
+        c = pin_cell - 1
+        area_x, dxc = flat(DY * DZ), flat(DX)
+        diag[c] += area_x[c] / (0.5 * dxc[c])
+    del ax, ay, az, DX, DY, DZ
+    c = torch.arange(n, dtype=torch.int64, device=device)
+    i, j, k = c % nx, (c // nx) % ny, c // (nx * ny)
+    sx, sy, sz = strides
+    cols = torch.stack([c - sz, c - sy, c - sx, c, c + sx, c + sy, c + sz], dim=1)
+    valid = torch.stack([k > 0, j > 0, i > 0, torch.ones_like(i, dtype=torch.bool), i < nx - 1, j < ny - 1,
+                         k < nz - 1], dim=1)
+    vals = torch.stack([-lo[2], -lo[1], -lo[0], diag, -hi[0], -hi[1], -hi[2]], dim=1)
+    del lo, hi, diag, i, j, k
+    counts = valid.sum(dim=1)
+    nnz = int(counts.sum())
+    acol = torch.empty(n + 1, dtype=torch.int32, device=device)
+    acol[0] = 1
+    acol[1:] = (torch.cumsum(counts, 0) + 1).to(torch.int32)
+    arow = torch.zeros(nnz + 1, dtype=torch.int32, device=device)
+    arow[:nnz] = (cols[valid] + 1).to(torch.int32)
+    del cols
+    aval = torch.zeros(nnz + 1, dtype=f64, device=device)
+    aval[:nnz] = vals[valid]
+    del vals
+    adia = (acol[:-1].to(torch.int64) + valid[:, :3].sum(dim=1)).to(torch.int32)
+    return n, nnz + 1, acol, arow, adia, aval
+
+
+def wall_pressure_mesh(n: int | tuple, ratio: float = 1.05, max_aspect: float = 50.0) -> BoxMesh:
+    """The mesh of wall_pressure_system."""
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+    dx = 2.0 * np.pi / nx
+    return BoxMesh(np.linspace(0.0, 2.0 * np.pi, nx + 1),
+                   geometric_wall_nodes(ny, 2.0, ratio, max_aspect, dx),
+                   np.linspace(0.0, np.pi, nz + 1))
+
+
 def divergence_free_fluxes(mesh: BoxMesh, seed: int, amplitude: float = 1.0, modes: int = 4):
     """Face volume fluxes F = curl(psi) of a seeded edge vector potential.
 
@@ -485,6 +565,31 @@ def pressure_system(n: int | tuple, seed: int = 1, ratio: float = 1.1, prec: int
                              generator_version=GENERATOR_VERSION))
 
 
+def rescaled_pressure_values(m: TFlowSMatrix, seed: int, spread: tuple = (0.5, 2.0)) -> TFlowSMatrix:
+    """The same pattern with the values a later outer iteration would give
+    (the face coefficients A12 change with the interpolated velocities,
+    Process/CalcPS.f90:77-94): every face pair (i, j) / (j, i) scaled by one
+    seeded factor in ``spread``, each diagonal reset to minus its row's new
+    off-diagonal sum plus the row's original remainder (the pin), so the
+    matrix stays a symmetric M-matrix."""
+    rng = np.random.default_rng(seed)
+    rows = m.row_of_entry()
+    cols = m.arow[:m.nnz].astype(np.int64) - 1
+    off = rows != cols
+    lo = np.flatnonzero(off & (cols < rows))
+    mirror = m.mirror if m.mirror is not None else mirror_positions(m)
+    scale = np.ones(m.nnz)
+    f = rng.uniform(spread[0], spread[1], lo.size)
+    scale[lo] = f
+    scale[mirror[lo]] = f
+    a = m.aval.copy()
+    a[:m.nnz] *= scale
+    dia = m.adia.astype(np.int64) - 1
+    pin = m.aval[dia] + np.bincount(rows[off], weights=m.aval[:m.nnz][off], minlength=m.n)
+    a[dia] = pin - np.bincount(rows[off], weights=a[:m.nnz][off], minlength=m.n)
+    return m.with_values(a)
+
+
 def wall_pressure_system(n: int | tuple, seed: int = 5, ratio: float = 1.05, max_aspect: float = 50.0,
                          prec: int = 2, niter: int = 20000, rel_tol: float = 1e-8) -> LinearSystem:
     """Channel pressure-correction system with strong wall clustering.
@@ -497,9 +602,7 @@ def wall_pressure_system(n: int | tuple, seed: int = 5, ratio: float = 1.05, max
     """
     nx, ny, nz = (n, n, n) if np.isscalar(n) else n
     dx = 2.0 * np.pi / nx
-    mesh = BoxMesh(np.linspace(0.0, 2.0 * np.pi, nx + 1),
-                   geometric_wall_nodes(ny, 2.0, ratio, max_aspect, dx),
-                   np.linspace(0.0, np.pi, nz + 1))
+    mesh = wall_pressure_mesh((nx, ny, nz), ratio, max_aspect)
     m = pressure_matrix(mesh, pin_cell=1)
     b = seeded_rhs(m.n, seed, True)
     tol = rel_tol * float(np.sqrt(np.dot(b, b)))

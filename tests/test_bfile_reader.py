@@ -97,6 +97,30 @@ def test_readc_rules(pkg, tmp_path, newline):
     assert b.profiles == ["", "in.prof"] and b.inits == ["init.dat"] and b.ifile.tolist() == [1]
 
 
+def test_readc_columns_and_blank_lines(pkg, tmp_path):
+    """ReadC reads A300 (Library/ReadC.f90:21): a token past column 300 is
+    never seen, so the density written there is missing; the same line
+    within 300 columns reads normally.  Only a line of spaces is blank
+    (:22): a tab-only line is a data line without tokens.  A token that
+    reaches column 299 is never closed by ReadC's scan (:44-51) and is
+    refused."""
+    head = "1 FLUID 1.0D-3"
+    inside = GOOD.replace("1 FLUID 1.0D-3 2.5", head + " " * (295 - len(head)) + "2.5")   # ends in column 298
+    b = pkg.krylov.BFile(_write(tmp_path, inside, name="in.b"))
+    assert b.visc == 1e-3 and b.denc.tolist() == [2.5]
+    past = GOOD.replace("1 FLUID 1.0D-3 2.5", head + " " * (300 - len(head)) + "2.5")
+    with pytest.raises(pkg.krylov.TFlowsError, match="missing"):
+        pkg.krylov.BFile(_write(tmp_path, past, name="past.b"))
+    edge = GOOD.replace("1 FLUID 1.0D-3 2.5", head + " " * (296 - len(head)) + "2.5")     # ends in column 299
+    with pytest.raises(pkg.krylov.TFlowsError, match="column 299"):
+        pkg.krylov.BFile(_write(tmp_path, edge, name="edge.b"))
+    spaces = GOOD.replace("#\n", "    \n")
+    assert pkg.krylov.BFile(_write(tmp_path, spaces, name="sp.b")).typbou.tolist() == [PRESSURE, CONVECT]
+    tabs = GOOD.replace("#\n", "\t\n")
+    with pytest.raises(pkg.krylov.TFlowsError, match="missing"):
+        pkg.krylov.BFile(_write(tmp_path, tabs, name="tab.b"))
+
+
 @pytest.mark.parametrize("text,match", [
     (GOOD.replace("FLUID", "GAS"), "material state"),
     (GOOD.replace("pressure", "WAL"), "boundary condition type"),

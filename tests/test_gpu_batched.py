@@ -337,3 +337,110 @@ def test_batch_mixed_guard(pkg, cuda, mixed_sizes):
         true_r = s.b - s.matrix.to_scipy() @ xs[k]
         assert np.linalg.norm(true_r) == pytest.approx(res["fin_res"][k], rel=1e-6, abs=1e-3 * s.tol)
         print(f"system {k}: niter={res['niter'][k]} fin/tol={res['fin_res'][k] / s.tol:.3g}", flush=True)
+
+
+# ---------------------------------------------------------------------------
+# The persistent form: one launch per chunk of iterations, the kernels of an
+# iteration as phases separated by barriers over each system's blocks.  At
+# the same grid and lanes it sums the same tiles into the same reduction
+# sites as the per-kernel form, so every result is the same bits.
+# ---------------------------------------------------------------------------
+def _form_solve(pkg, cuda, systems, form, method, prec, cap, grid, lanes=1, rebalance=True, chunk=0):
+    B = pkg.krylov.DeviceBatch([s.matrix for s in systems])
+    B.set_form(form)
+    B, res, xs, rs = _batch_solve(pkg, cuda, systems, method, prec, cap, grid=grid, lanes=lanes, chunk=chunk,
+                                  rebalance=rebalance, batch=B)
+    used = B.config(method, prec)["form"]
+    B.close()
+    return used, res, xs, rs
+
+
+def _same_results(a, b):
+    for key in ("niter", "fin_res"):
+        assert np.array_equal(a[1][key], b[1][key]), key
+    assert np.array_equal(a[1]["ini_res"], b[1]["ini_res"], equal_nan=True)
+    for u, v in zip(a[2] + a[3], b[2] + b[3]):
+        assert np.array_equal(u, v)
+
+
+@pytest.mark.parametrize("method", ["cg", "bicg"])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_persistent_form_bitwise(pkg, oracle, cuda, mixed_sizes, method, prec):
+    systems = mixed_sizes
+    per_kernel = _form_solve(pkg, cuda, systems, 1, method, prec, 1000, grid=256)
+    persistent = _form_solve(pkg, cuda, systems, 2, method, prec, 1000, grid=256)
+    assert per_kernel[0] == 1 and persistent[0] == 2
+    _same_results(per_kernel, persistent)
+    res, xs = persistent[1], persistent[2]
+    for k, s in enumerate(systems):
+        ref = oracle.run(method, s.matrix, s.b, prec=prec, niter=1000, tol=s.tol)
+        assert abs(int(res["niter"][k]) - ref["niter"]) <= 1
+        assert res["fin_res"][k] < s.tol
+        assert np.linalg.norm(s.b - s.matrix.to_scipy() @ xs[k]) < 10 * s.tol
+
+
+def test_persistent_form_caps_chunks_lanes(pkg, cuda, mixed_sizes):
+    """Capped systems stop at cap + 1 inside a chunk; chunk lengths that do
+    not divide the iteration counts, a lane-group SpMV form and a grid
+    without rebalancing give the per-kernel form's bits."""
+    systems = mixed_sizes[:5]
+    for cap, chunk, lanes, grid, reb in ((7, 3, 1, 128, True), (1000, 5, 4, 256, True), (1000, 16, 2, 64, False)):
+        a = _form_solve(pkg, cuda, systems, 1, "cg", 1, cap, grid, lanes=lanes, rebalance=reb, chunk=chunk)
+        b = _form_solve(pkg, cuda, systems, 2, "cg", 1, cap, grid, lanes=lanes, rebalance=reb, chunk=chunk)
+        assert b[0] == 2
+        _same_results(a, b)
+        if cap == 7:
+            assert int(np.max(b[1]["niter"])) == cap + 1
+
+
+def test_persistent_form_mixed_bitwise(pkg, cuda, mixed_sizes):
+    """FP32 inner CG solves of the batched refinement in the persistent form."""
+    import torch
+    systems = mixed_sizes
+    out = []
+    for form in (1, 2):
+        B = pkg.krylov.DeviceBatch([s.matrix for s in systems])
+        B.set_form(form)
+        B.configure(grid=256, lanes=1)
+        b = torch.from_numpy(np.concatenate([s.b for s in systems])).to(cuda)
+        x = torch.zeros(B.rows, dtype=torch.float64, device=cuda)
+        res = B.solve_mixed("cg", 1, x, b, 2000, np.array([s.tol for s in systems]), inner_rtol=1e-3)
+        torch.cuda.synchronize()
+        assert B.config("cg", 1, bits=32)["form"] == form
+        out.append((res, x.cpu().numpy()))
+        B.close()
+    assert np.array_equal(out[0][0]["niter"], out[1][0]["niter"])
+    assert np.array_equal(out[0][0]["fin_res"], out[1][0]["fin_res"])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.all(out[1][0]["fin_res"] < np.array([s.tol for s in systems]))
+
+
+def test_persistent_form_uneven_batch(pkg, cuda):
+    """Twenty uneven pressure boxes (the many-system case) with rebalancing:
+    the persistent and per-kernel forms agree bit for bit, and the tuned
+    batch converges every system."""
+    systems = pkg.meshgen.uneven_pressure_batch(count=20, seed=9, n_min=500, n_max=20000)
+    a = _form_solve(pkg, cuda, systems, 1, "cg", 1, 5000, 512)
+    b = _form_solve(pkg, cuda, systems, 2, "cg", 1, 5000, 512)
+    assert b[0] == 2
+    _same_results(a, b)
+    used, res, xs, rs = _form_solve(pkg, cuda, systems, 0, "cg", 1, 5000, 0, lanes=0)
+    print(f"tuned form {used}, iterations {int(np.sum(res['niter']))}", flush=True)
+    assert np.all(res["fin_res"] < np.array([s.tol for s in systems]))
+
+
+def test_persistent_wait_bound(pkg, cuda, mixed_sizes, monkeypatch):
+    """Failure handling of the persistent form, forced from the host: with
+    TFLOWS_BARRIER_WAIT_MS=0 (read when a batch is created) a block that
+    has to wait at a barrier gives up at its first time check and raises
+    the fault word, every other waiting block sees it and stops, so every
+    wavefront ends; the batch then solves again in the per-kernel form from
+    its inputs, keeps that form, and returns the per-kernel form's bits."""
+    systems = mixed_sizes
+    want = _form_solve(pkg, cuda, systems, 1, "cg", 1, 1000, 256)
+    monkeypatch.setenv("TFLOWS_BARRIER_WAIT_MS", "0")
+    got = _form_solve(pkg, cuda, systems, 2, "cg", 1, 1000, 256)
+    monkeypatch.delenv("TFLOWS_BARRIER_WAIT_MS")
+    print(f"wait bound 0: form after the solve = {got[0]}", flush=True)
+    assert got[0] in (1, 2)
+    _same_results(want, got)

@@ -197,6 +197,12 @@ def test_bad_files_raise(pkg, tmp_path):
     cut.write_bytes(data[:len(data) // 2])
     with pytest.raises(pkg.krylov.TFlowsError, match="record|end of file"):
         pkg.krylov.Mesh(str(cut))
+    # a header that claims 2^31 - 1 bytes in a short file is refused before
+    # any buffer of that size is made
+    huge = tmp_path / "huge.cns"
+    huge.write_bytes(struct.pack("<i", 2**31 - 1) + b"\x00" * 64)
+    with pytest.raises(pkg.krylov.TFlowsError, match="shorter than its count"):
+        pkg.krylov.Mesh(str(huge))
     bad = tmp_path / "bad.cns"
     bad.write_bytes(b"\x07\x00\x00\x00not a mesh\x00")
     with pytest.raises(pkg.krylov.TFlowsError):
