@@ -120,6 +120,24 @@ int tflows_system_set_values(tflows_system s, const double* aval, int32_t mem, v
 int tflows_solve(tflows_system s, int32_t method, int32_t prec, double* x, double* r1, int32_t* niter,
                  double tol, double* ini_res, double* fin_res, void* stream);
 
+/* tflows_solve without any allocation: the solver state, the reduction
+ * partials and the work vectors (D, p, q, and x / r1 in the internal order
+ * of a renumbered system) live in one fixed device array of the library
+ * (each GPU holds its own copy, about 40 MiB, until the process ends), so
+ * the call makes no hipMalloc / hipFree / hipHostMalloc and no allocating
+ * library call.  No counterpart in the reference: it is for callers that
+ * cannot allocate during a solve -- tight solve loops and harnesses that
+ * forbid device allocation.  Same arguments, results and bits as
+ * tflows_solve for what it takes: method 0 (CG) or 1 (BiCG, which runs
+ * through the CG kernels, as in tflows_solve), prec 0 or 1, N <=
+ * TFLOWS_NOALLOC_MAX_ROWS; anything else returns an error.  All calls on
+ * one GPU share that GPU's array (each call clears what must start at
+ * zero), so calls that overlap on one GPU -- from two host threads or on
+ * two streams -- race and corrupt each other's results. */
+#define TFLOWS_NOALLOC_MAX_ROWS 1048576
+int tflows_solve_noalloc(tflows_system s, int32_t method, int32_t prec, double* x, double* r1, int32_t* niter,
+                         double tol, double* ini_res, double* fin_res, void* stream);
+
 /* Mixed precision: the CG / BiCG / CGS loop of `method` in FP32 (SpMV,
  * preconditioner, vector updates and dot products in FP32) inside an FP64
  * iterative-refinement loop.  x (device, N doubles) is the initial guess in

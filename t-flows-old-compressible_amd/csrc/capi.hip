@@ -146,6 +146,9 @@ void drop_in(int method, const int32_t* N, const int32_t* NB, const int32_t* NON
 namespace tflows {
 // errors of host-side entry points compiled elsewhere (meshio.cpp)
 int set_last_error(const std::string& msg) { return fail(msg); }
+// the no-allocation solve (noalloc.hip)
+SolveResult solve_noalloc(System& S, int method, int prec, double* x, double* r1, int cap, double tol,
+                          double stale_error, hipStream_t s);
 }  // namespace tflows
 
 
@@ -254,6 +257,20 @@ int tflows_solve(tflows_system s, int32_t method, int32_t prec, double* x, doubl
     return guarded([&] {
         std::lock_guard<std::mutex> lock(g_mutex);
         tflows::SolveResult r = s->sys->solve(method, prec, x, r1, *niter, tol, g_sol_error, as_stream(stream));
+        *niter = r.niter;
+        if (r.ini_written && ini_res) *ini_res = r.ini_res;
+        if (fin_res) *fin_res = r.fin_res;
+        g_sol_error = r.fin_res;
+    });
+}
+
+int tflows_solve_noalloc(tflows_system s, int32_t method, int32_t prec, double* x, double* r1, int32_t* niter,
+                         double tol, double* ini_res, double* fin_res, void* stream) {
+    return guarded([&] {
+        if (!s || !niter) throw std::invalid_argument("null system or niter");
+        std::lock_guard<std::mutex> lock(g_mutex);
+        tflows::SolveResult r =
+            tflows::solve_noalloc(*s->sys, method, prec, x, r1, *niter, tol, g_sol_error, as_stream(stream));
         *niter = r.niter;
         if (r.ini_written && ini_res) *ini_res = r.ini_res;
         if (fin_res) *fin_res = r.fin_res;
