@@ -171,7 +171,9 @@ int tflows_solve_mixed_reliable(tflows_system s, int32_t prec, double* x, const 
 
 /* Adjoint (transposed) solve A^T lambda = g with the Krylov method and
  * preconditioner of the forward solve, for a structurally symmetric pattern
- * (the values of A^T are gathered from the mirror positions).  lambda (device,
+ * (the values of A^T are gathered from the mirror positions; when every
+ * value equals its mirror, as for the pressure-correction matrix, the solve
+ * runs on the system itself).  lambda (device,
  * N doubles): initial guess in, solution out; r1 (device): g in, final
  * residual out; other arguments as tflows_solve.  No counterpart in the
  * reference. */

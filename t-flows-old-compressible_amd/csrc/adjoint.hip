@@ -21,16 +21,41 @@ __global__ __launch_bounds__(kBlock) void k_mirror_gather(int64_t nnz, const int
     if (p < nnz) a_t[p] = a[mirror[p]];
 }
 
-// grad[p] = -lambda[row(p)] * x[col(p)] for the entries of each row; one
-// thread per row; slot NONZ (the trailing unused one) is zeroed by the host.
+// grad[p] = -lambda[row(p)] * x[col(p)] for every stored entry.  One block
+// per tile of 256 rows: the tile's row pointers go to LDS, and thread t
+// takes entries e0 + t, e0 + t + 256, ... of the tile, finding each entry's
+// row by binary search there, so the loads of col and the stores of grad
+// are contiguous across the block.  Slot NONZ (the trailing unused one) is
+// zeroed by the host.
 __global__ __launch_bounds__(kBlock) void k_outer_grad(int n, const int* __restrict__ rowptr,
                                                        const int* __restrict__ col,
                                                        const double* __restrict__ lambda,
                                                        const double* __restrict__ x, double* __restrict__ grad) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const double li = lambda[i];
-    for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) grad[p] = -(li * x[col[p]]);
+    __shared__ int s_rp[kTileRows + 1];
+    const int ntiles = (n + kTileRows - 1) / kTileRows;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int r0 = t * kTileRows, nr = min(kTileRows, n - r0);
+        __syncthreads();
+        for (int i = threadIdx.x; i <= nr; i += kBlock) s_rp[i] = rowptr[r0 + i];
+        __syncthreads();
+        const int e0 = s_rp[0], e1 = s_rp[nr];
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlock) {
+            int lo = 0, hi = nr;                // the row: s_rp[lo] <= e < s_rp[lo + 1]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_rp[mid] <= e) lo = mid;
+                else hi = mid;
+            }
+            grad[e] = -(lambda[r0 + lo] * x[col[e]]);
+        }
+    }
+}
+
+// 1 in *flag when some a[p] differs from a[mirror[p]] (A is not symmetric)
+__global__ __launch_bounds__(kBlock) void k_asymmetry(int64_t nnz, const int* __restrict__ mirror,
+                                                      const double* __restrict__ a, int* __restrict__ flag) {
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * kBlock)
+        if (!(a[p] == a[mirror[p]])) *flag = 1;
 }
 
 // Mirror positions, built on the host once per pattern: for entry p = (i, j)
@@ -57,6 +82,26 @@ const int* System::transpose_map() {
 
 System& System::transposed(hipStream_t s) {
     const int* mir = transpose_map();
+    if (!transposed_valid_) {
+        // a symmetric A (the pressure-correction matrix) is its own
+        // transpose: the adjoint solve then runs on this system
+        const int64_t nnz = nnz_;
+        sym_flag_.ensure(1);
+        TF_CHECK(hipMemsetAsync(sym_flag_.p, 0, sizeof(int), s));
+        hipLaunchKernelGGL(k_asymmetry, dim3((unsigned)std::min<int64_t>(2048, (nnz + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, s, nnz, mir, values_original(), sym_flag_.p);
+        TF_CHECK(hipGetLastError());
+        int asym = 0;
+        TF_CHECK(hipMemcpyAsync(&asym, sym_flag_.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        TF_CHECK(hipStreamSynchronize(s));
+        symmetric_ = asym == 0;
+        if (symmetric_) {
+            transposed_valid_ = true;
+            return *this;
+        }
+    } else if (symmetric_) {
+        return *this;
+    }
     if (!transposed_) {
         // same pattern (the constructor validates and uploads it again)
         std::vector<int> acol(n_ + 1), arow(nnz_), adia(n_);
@@ -84,8 +129,9 @@ void System::outer_grad(const double* lambda, const double* x, double* grad, hip
     // over the caller's pattern and numbering (lambda, x, grad all in it)
     const int* rp = reordered_ ? orowptr_.p : rowptr_.p;
     const int* cl = reordered_ ? ocol_.p : col_.p;
-    hipLaunchKernelGGL(k_outer_grad, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n_, rp, cl,
-                       lambda, x, grad);
+    const int tiles = (n_ + kTileRows - 1) / kTileRows;
+    hipLaunchKernelGGL(k_outer_grad, dim3(std::min(tiles, 8192)), dim3(kBlock), 0, s, n_, rp, cl, lambda, x,
+                       grad);
     TF_CHECK(hipMemsetAsync(grad + nnz_, 0, sizeof(double), s));   // the NONZ-th slot
     TF_CHECK(hipGetLastError());
 }

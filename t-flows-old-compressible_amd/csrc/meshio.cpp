@@ -205,53 +205,57 @@ void load_geo(tflows_mesh_s& m, const char* path) {
     m.has_geo = true;
 }
 
-// Process/TopolM.f90:15-76
+// The matrix pattern from the face list, built as a sort of (row, column)
+// keys: one key per cell for its diagonal and two per interior face,
+// (c1, c2) and (c2, c1), ordered by row, then column.  Row c then holds the
+// diagonal and one entry per face of c in ascending column order, the
+// layout of Process/TopolM.f90 (its ISORT, :72-74), with NONZERO = keys + 1
+// (:49).  A key's position is where TopolM's searches find it (:75-94): the
+// last entry of its row that holds the column, so two faces that join the
+// same two cells both receive the later of their positions and the
+// assemblies accumulate there.
 void topolm(tflows_mesh_s& m) {
     const int nc = m.nc, ns = m.ns;
-    std::vector<int32_t> stencw(nc + 1, 1);                            // 1-based
-    for (int s = 0; s < ns; ++s) {
-        const int c1 = m.sidec[3 * (size_t)s + 1], c2 = m.sidec[3 * (size_t)s + 2];
-        if (c2 > 0) {
-            stencw[c1]++;
-            stencw[c2]++;
-        }
+    std::vector<int> inner;                          // 0-based interior faces
+    for (int s = 0; s < ns; ++s)
+        if (m.sidec[3 * (size_t)s + 2] > 0) inner.push_back(s);
+    const size_t nkeys = (size_t)nc + 2 * inner.size();
+    if (nkeys + 1 > (size_t)INT32_MAX) throw std::runtime_error("NONZERO exceeds 32 bits");
+    // key = row << 32 | column (1-based cells); origin k < nc: diagonal of
+    // cell k + 1; nc + 2 f (+1): face inner[f], side 1 (side 2)
+    std::vector<uint64_t> key(nkeys);
+    for (int c = 1; c <= nc; ++c) key[c - 1] = (uint64_t)c << 32 | (uint32_t)c;
+    for (size_t f = 0; f < inner.size(); ++f) {
+        const uint64_t c1 = (uint32_t)m.sidec[3 * (size_t)inner[f] + 1];
+        const uint64_t c2 = (uint32_t)m.sidec[3 * (size_t)inner[f] + 2];
+        key[nc + 2 * f] = c1 << 32 | c2;
+        key[nc + 2 * f + 1] = c2 << 32 | c1;
     }
-    int64_t n = 0;
-    for (int c = 1; c <= nc; ++c) n += stencw[c];
-    if (n + 1 > INT32_MAX) throw std::runtime_error("NONZERO exceeds 32 bits");
-    m.nonz = (int32_t)(n + 1);
-    m.arow.assign((size_t)n + 1, 0);
+    std::vector<uint32_t> order(nkeys);
+    for (size_t k = 0; k < nkeys; ++k) order[k] = (uint32_t)k;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+    // where each origin lands, taken to the last position of its key's run
+    std::vector<int32_t> at(nkeys);
+    size_t run_end = nkeys;
+    for (size_t p = nkeys; p-- > 0;) {
+        if (p + 1 == nkeys || key[order[p]] != key[order[p + 1]]) run_end = p;
+        at[order[p]] = (int32_t)run_end + 1;          // 1-based
+    }
+    m.nonz = (int32_t)nkeys + 1;
+    m.arow.assign(nkeys + 1, 0);
     m.acol.assign((size_t)nc + 1, 0);
     m.adia.assign(nc, 0);
+    for (size_t p = 0; p < nkeys; ++p) {
+        m.arow[p] = (int32_t)(key[order[p]] & 0xffffffffu);
+        m.acol[key[order[p]] >> 32]++;                // entries of each row
+    }
     m.acol[0] = 1;
-    for (int c = 1; c <= nc; ++c) {
-        m.acol[c] = m.acol[c - 1] + stencw[c];
-        m.arow[m.acol[c - 1] - 1] = c;                                 // the cell itself first
-        stencw[c] = 1;
-    }
-    for (int s = 0; s < ns; ++s) {
-        const int c1 = m.sidec[3 * (size_t)s + 1], c2 = m.sidec[3 * (size_t)s + 2];
-        if (c2 > 0) {
-            m.arow[m.acol[c1 - 1] - 1 + stencw[c1]] = c2;
-            m.arow[m.acol[c2 - 1] - 1 + stencw[c2]] = c1;
-            stencw[c1]++;
-            stencw[c2]++;
-        }
-    }
-    for (int c = 1; c <= nc; ++c) {
-        int32_t* row = m.arow.data() + m.acol[c - 1] - 1;
-        std::sort(row, row + stencw[c]);
-        for (int j = m.acol[c - 1]; j < m.acol[c]; ++j)
-            if (m.arow[j - 1] == c) m.adia[c - 1] = j;
-    }
+    for (int c = 1; c <= nc; ++c) m.acol[c] += m.acol[c - 1];
+    for (int c = 1; c <= nc; ++c) m.adia[c - 1] = at[c - 1];
     m.sidaij.assign(2 * (size_t)ns, 0);
-    for (int s = 0; s < ns; ++s) {
-        const int c1 = m.sidec[3 * (size_t)s + 1], c2 = m.sidec[3 * (size_t)s + 2];
-        if (c2 <= 0) continue;
-        for (int j = m.acol[c1 - 1]; j < m.acol[c1]; ++j)
-            if (m.arow[j - 1] == c2) m.sidaij[2 * (size_t)s] = j;
-        for (int j = m.acol[c2 - 1]; j < m.acol[c2]; ++j)
-            if (m.arow[j - 1] == c1) m.sidaij[2 * (size_t)s + 1] = j;
+    for (size_t f = 0; f < inner.size(); ++f) {
+        m.sidaij[2 * (size_t)inner[f]] = at[nc + 2 * f];
+        m.sidaij[2 * (size_t)inner[f] + 1] = at[nc + 2 * f + 1];
     }
 }
 

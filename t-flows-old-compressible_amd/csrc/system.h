@@ -266,7 +266,8 @@ class System {
 
     // adjoint support (adjoint.hip): mirror positions of the structurally
     // symmetric pattern, a system holding A^T on the same pattern (values
-    // refreshed after set_values), and grad[p] = -lambda_i x_j
+    // refreshed after set_values; this system itself when its values equal
+    // their mirrors), and grad[p] = -lambda_i x_j
     const int* transpose_map();
     System& transposed(hipStream_t s);
     void outer_grad(const double* lambda, const double* x, double* grad, hipStream_t s);
@@ -390,6 +391,8 @@ class System {
     bool tpos_ready_ = false;
     std::unique_ptr<System> transposed_;
     bool transposed_valid_ = false;
+    bool symmetric_ = false;         // the values equal their mirrors: A is its own transpose
+    DevArray<int> sym_flag_;
     std::vector<int> h_rowptr_, h_col_, h_dia_;
     Levels levels_;
     bool levels_ready_ = false;

@@ -54,7 +54,8 @@ def test_gradients_match_fd_and_cpu_reference(pkg, cuda):
     print(out)
     assert out["ref_rel_grad_b"] < 1e-9
     assert out["ref_rel_grad_coef"] < 1e-9
+    # 1e-6 relative against central differences (h = 1e-6) over the checked
+    # entries as a vector; per entry within 1e-6 relative plus the
+    # difference's own error bound (||lambda|| rtol ||b|| + eps L) / h
     assert out["fd_rel"] < 1e-6
-    # per entry: within the FD cancellation floor eps*L/h (relative to the
-    # entry, floored at 1e-3 of the largest checked gradient)
-    assert out["fd_median_rel"] < 1e-6
+    assert out["fd_floor_ratio"] <= 1.0
