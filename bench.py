@@ -57,6 +57,44 @@ def spmv_bytes(n, nnz, word, sliced=False, col_bytes=4):
     return (4 + col_bytes) * (nnz - n) + index + 12 * n
 
 
+def format_bytes(n, nnz, word, layout):
+    """Bytes the operator in use moves per launch, every array touched once.
+    FP64 (word 8) holds every entry, FP32 (word 4, difference form) the
+    off-diagonal ones plus FP32 row sums.  Sliced layouts: values of the
+    stored slots, column indices (col_bytes each: 4, or 2 for 16-bit
+    offsets; padding slots are not counted), the slice offsets
+    4*(ceil(N/64)+1); with offsets shared by each slice, no per-entry
+    column but every stored slot's value (absent neighbours included), one
+    mask byte per row and 4 bytes per slot column of a slice.  CSR: values
+    and int32 columns of the entries, the row pointer 4*(N+1).  Plus x and
+    y (word*N each)."""
+    ent = nnz if word == 8 else nnz - n
+    vec = 2 * word * n + (4 * n if word == 4 else 0)
+    if not layout["sliced"]:
+        return (word + 4) * ent + 4 * (n + 1) + vec
+    index = 4 * ((n + 63) // 64 + 1)
+    if layout.get("shared"):
+        stored = layout["stored_entries"]
+        return word * stored + n + 4 * (stored // 64) + index + vec
+    return (word + layout["col_bytes"]) * ent + index + vec
+
+
+def layout_name(l):
+    if not l["sliced"]:
+        return "CSR"
+    if l.get("shared"):
+        return "sliced ELL, column offsets shared by each slice"
+    return "sliced ELL, " + ("16-bit column offsets" if l["col_bytes"] == 2 else "int32 columns")
+
+
+def layout_tag(l, bits):
+    """Key of a layout in the traffic summaries under profiles/."""
+    base = "fp32" if bits == 32 else "fp64"
+    if not l["sliced"]:
+        return base + "_csr"
+    return base + ("_shared" if l.get("shared") else ("_sliced16" if l["col_bytes"] == 2 else "_sliced"))
+
+
 def spmv32_canonical_bytes(n, nnz):
     """The FP32 SpMV's bytes with int32 indices and the diagonal stored
     (8*nnz + 4*(N+1) + 8*N), the count a CSR kernel moves."""
@@ -84,6 +122,83 @@ def time_spmv(torch, ds, n, reps=50, fp32=False):
 COPY_BYTES = 512 * 1024 * 1024
 
 
+def achievable_bandwidth(torch, nbytes=2 ** 31, reps=10):
+    """Achievable HBM bandwidth on this device: a device copy (2 x nbytes
+    moved) and a STREAM triad a = b + 3 c (3 x nbytes), each timed with HIP
+    events on the stream it runs on, mean of `reps` launches."""
+    n = nbytes // 8
+    a = torch.empty(n, dtype=torch.float64, device="cuda")
+    b = torch.full((n,), 1.0, dtype=torch.float64, device="cuda")
+    c = torch.full((n,), 2.0, dtype=torch.float64, device="cuda")
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = {}
+    for name, fn, moved in (("copy", lambda: a.copy_(b), 2 * nbytes),
+                            ("triad", lambda: torch.add(b, c, alpha=3.0, out=a), 3 * nbytes)):
+        for _ in range(3):
+            fn()
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        e1.synchronize()
+        t = e0.elapsed_time(e1) / reps * 1e-3
+        out[name + "_GBs"] = round(moved / t / 1e9, 1)
+    assert float(a[12345]) == 7.0
+    del a, b, c
+    torch.cuda.empty_cache()
+    return out
+
+
+def device_wall_system(torch, pkg, n):
+    """The wall-clustered pressure system of the 256^3 configuration
+    (stretching 1.05, aspect up to 50:1, cell 1 pinned), assembled on the GPU
+    by the torch form of the recipe (bit for bit the host recipe)."""
+    mg = pkg.meshgen
+    N, nonz, acol, arow, adia, aval = mg.pressure_matrix_torch(mg.wall_pressure_mesh(n), device="cuda")
+    torch.cuda.synchronize()
+    return N, nonz, (acol, arow, adia, aval)
+
+
+def spmv_hbm(torch, pkg, n=256):
+    """The SpMVs at n^3, where the matrix (1.4 GB in FP64) is far larger
+    than the 256 MB MALL, so every launch streams it from HBM: FP64 and FP32
+    operators of the solves, each priced at the bytes its format moves and
+    at the CSR bytes (12 nnz + 4 (N+1) + 16 N and 8 nnz + 4 (N+1) + 8 N)."""
+    log(f"building the {n}^3 wall-clustered pressure matrix on the GPU")
+    N, nonz, arrays = device_wall_system(torch, pkg, n)
+    ds = pkg.krylov.DeviceSystem.from_device(N, nonz, *arrays)
+    nnz = nonz - 1
+    l64 = ds.fp64_layout()
+    t64 = time_spmv(torch, ds, N, reps=20)
+    l32 = ds.fp32_layout()
+    t32 = time_spmv(torch, ds, N, reps=20, fp32=True)
+    f64 = format_bytes(N, nnz, 8, l64)
+    c64 = spmv_bytes(N, nnz, 8)
+    f32 = format_bytes(N, nnz, 4, l32)
+    c32 = spmv32_canonical_bytes(N, nnz)
+
+    def entry(kernel, layout, t, fmt, csr, tag):
+        return {"kernel": kernel, "layout": layout, "us_per_launch": round(t * 1e6, 2),
+                "traffic": measured_traffic(tag, f"spmv_traffic_wall{n}.md"),
+                "format_bytes": fmt, "GBs_format": round(fmt / t / 1e9, 1),
+                "frac_format": round(fmt / t / 1e9 / HBM_PEAK_GBS, 4),
+                "csr_bytes": csr, "GBs_csr": round(csr / t / 1e9, 1),
+                "frac_csr": round(csr / t / 1e9 / HBM_PEAK_GBS, 4)}
+
+    def name(l, base):
+        return layout_name(l)
+    out = {"workload": f"wall-clustered pressure {n}^3 (N={N}, nnz={nnz}), natural order",
+           "fp64": entry("k_spmv<Sell<double>>" if l64["sliced"] else "k_spmv<Csr<double>>", name(l64, "fp64"),
+                         t64, f64, c64, layout_tag(l64, 64)),
+           "fp32": entry("k_spmv<DiffSell32>" if l32["sliced"] else "k_spmv<DiffCsr32>", name(l32, "fp32"),
+                         t32, f32, c32, layout_tag(l32, 32))}
+    ds.close()
+    del arrays
+    torch.cuda.empty_cache()
+    return out
+
+
 def profile_spmv(torch, ds, n, reps=20, ds_csr=None):
     """The launches the traffic counters are read on: `reps` FP64 and FP32
     SpMVs of the workload in the layouts of ds (and of ds_csr when given)
@@ -106,11 +221,11 @@ def profile_spmv(torch, ds, n, reps=20, ds_csr=None):
     log(f"profile-spmv: {reps} x (spmv fp64, spmv fp32 x{len(ops) - 1}, copy {COPY_BYTES} B) done")
 
 
-def measured_traffic(kernel_tag):
+def measured_traffic(kernel_tag, summary="spmv_traffic_pressure128.md"):
     """HBM bytes per launch of the SpMV kernel from the committed counter
     summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy calibration),
     or None when no summary for this workload is present."""
-    path = os.path.join(REPO, "profiles", "spmv_traffic_pressure128.md")
+    path = os.path.join(REPO, "profiles", summary)
     if not os.path.exists(path):
         return None
     key = f"traffic_bytes_per_launch[{kernel_tag}]"
@@ -123,27 +238,51 @@ def measured_traffic(kernel_tag):
     return None
 
 
-def cpu_baseline(sysm, b, tol, full_iters, budget_s=20.0):
-    """Reference Fortran CG (or its C restatement) on one core, bounded sample."""
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(sysm, b, tol, full_iters, budget_s=20.0, runs=3):
+    """Reference Fortran CG (or its C restatement) on one pinned host core:
+    the median per-iteration time of `runs` bounded samples (about budget_s
+    in all) of the same solve, scaled to the full iteration count."""
     from oracle import oracle as O
     kind = "reference" if O.available_reference() else "port"
     solver = O.Reference() if kind == "reference" else O.Oracle()
     m = sysm
-    # calibrate on a few iterations, then run a sample of ~budget_s
-    t0 = time.perf_counter()
-    solver.run("cg", m, b, prec=1, niter=3, tol=tol)
-    per_it = (time.perf_counter() - t0) / 3.0
-    k = int(max(5, min(full_iters, budget_s / max(per_it, 1e-6))))
-    t0 = time.perf_counter()
-    res = solver.run("cg", m, b, prec=1, niter=k, tol=tol)
-    el = time.perf_counter() - t0
-    it_done = res["niter"] if res["fin_res"] < tol else k
-    per_it = el / it_done
+    allowed = os.sched_getaffinity(0)
+    core = max(allowed)
+    os.sched_setaffinity(0, {core})          # taskset -c core
+    try:
+        t0 = time.perf_counter()
+        solver.run("cg", m, b, prec=1, niter=3, tol=tol)
+        per_it = (time.perf_counter() - t0) / 3.0
+        k = int(max(5, min(full_iters, budget_s / runs / max(per_it, 1e-6))))
+        samples = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            res = solver.run("cg", m, b, prec=1, niter=k, tol=tol)
+            el = time.perf_counter() - t0
+            it_done = res["niter"] if res["fin_res"] < tol else k
+            samples.append(el / it_done)
+    finally:
+        os.sched_setaffinity(0, allowed)
+    per_it = float(np.median(samples))
     est_ms = per_it * full_iters * 1e3
+    model = cpu_model()
     return dict(value=round(est_ms, 1), unit="ms", cores=1, kind=kind,
-                sample=f"{it_done} of {full_iters} FP64 Jacobi-CG iterations on the same 128^3 system, "
-                       f"1 thread, {per_it*1e3:.1f} ms/iteration, scaled to the full solve",
-                ms_per_iteration=round(per_it * 1e3, 3))
+                sample=f"median of {runs} runs of {k} of the {full_iters} FP64 Jacobi-CG iterations on the same "
+                       f"128^3 system, 1 thread pinned to core {core} ({model}), {per_it*1e3:.2f} ms/iteration, "
+                       f"scaled to the full solve",
+                ms_per_iteration=round(per_it * 1e3, 3),
+                runs_ms_per_iteration=[round(v * 1e3, 3) for v in samples], cpu_model=model)
 
 
 # ---------------------------------------------------------------------------
@@ -204,44 +343,77 @@ def workload_momentum(args, torch, pkg):
 
 
 def workload_batch(args, torch, pkg, kind):
-    """Batched solves (one launch per kernel for all systems, graph replay)
-    against the same systems solved one by one on the GPU."""
+    """Batched solves against the same systems solved one by one on the GPU.
+    Each method is timed in the batch's two launch forms -- one launch per
+    kernel of an iteration, and one persistent launch per chunk of
+    iterations (CG and BiCG with Jacobi) -- and in the form the autotuner
+    keeps; FP64 and FP32 inner solves with FP64 refinement."""
     mg = pkg.meshgen
     if kind == "transport":
-        systems, method, cap = mg.transport_batch(n=96, count=5, seed=3), args.method or "cgs", 2000
+        systems, cap = mg.transport_batch(n=96, count=5, seed=3), 2000
+        methods = [args.method] if args.method else ["bicg", "cgs"]
     else:
-        systems, method, cap = mg.uneven_pressure_batch(count=64, seed=3), args.method or "cg", 5000
-    B = pkg.krylov.DeviceBatch([t.matrix for t in systems])
-    b = torch.from_numpy(np.concatenate([t.b for t in systems])).cuda()
+        systems, cap = mg.uneven_pressure_batch(count=64, seed=3), 5000
+        methods = [args.method] if args.method else ["cg"]
     tol = np.array([t.tol for t in systems])
-    x = torch.zeros(B.rows, dtype=torch.float64, device="cuda")
+    b = torch.from_numpy(np.concatenate([t.b for t in systems])).cuda()
+    x = torch.zeros(b.numel(), dtype=torch.float64, device="cuda")
     r = torch.empty_like(b)
-
-    def batched():
-        x.zero_(); r.copy_(b)
-        return B.solve(method, 1, x, r, cap, tol)
-    ms, res = _timed(torch, batched, args.steps + 1)   # the first call also tunes and captures
-    cfg = B.config(method, 1)
-    B.close()
     Ds = [pkg.krylov.DeviceSystem(t.matrix) for t in systems]
     bs = [torch.from_numpy(t.b).cuda() for t in systems]
+    out = {"metric": "batched solve time vs one GPU solve per system", "unit": "ms",
+           "workload": f"{len(systems)} {'transport (shared 96^3 pattern)' if kind == 'transport' else 'pressure'} "
+                       f"systems, {sum(t.matrix.n for t in systems)} rows, Jacobi, cap {cap}"}
+    for method in methods:
+        entry = {}
+        for form, name in ((1, "kernels"), (2, "persistent"), (0, "tuned")):
+            if form == 2 and method == "cgs":
+                continue
+            B = pkg.krylov.DeviceBatch([t.matrix for t in systems])
+            B.set_form(form)
 
-    def serial():
-        its = []
-        for D, t, bb in zip(Ds, systems, bs):
-            xx = torch.zeros(t.matrix.n, dtype=torch.float64, device="cuda")
-            rr = bb.clone()
-            its.append(D.solve(method, 1, xx, rr, cap, t.tol)[0])
-        return its
-    mss, its = _timed(torch, serial, args.steps)
+            def batched():
+                x.zero_(); r.copy_(b)
+                return B.solve(method, 1, x, r, cap, tol)
+            ms, res = _timed(torch, batched, args.steps + 1)   # the first call also tunes and captures
+            cfg = B.config(method, 1)
+            xm = torch.zeros_like(x)
+
+            def batched_mixed():
+                xm.zero_()
+                return B.solve_mixed(method, 1, xm, b, cap, tol, inner_rtol=args.inner_rtol)
+            msm, resm = _timed(torch, batched_mixed, args.steps + 1)
+            cfgm = B.config(method, 1, bits=32)
+            entry[name] = dict(ms=round(ms, 2), iterations_sum=int(np.sum(res["niter"])),
+                               iterations_max=int(np.max(res["niter"])), converged=int(np.sum(res["fin_res"] < tol)),
+                               config=cfg, mixed_ms=round(msm, 2), mixed_iterations_sum=int(np.sum(resm["niter"])),
+                               mixed_converged=int(np.sum(resm["fin_res"] < tol)), mixed_config=cfgm)
+            B.close()
+            log(f"batch {kind} {method} {name}: {entry[name]}")
+
+        def serial():
+            its = []
+            for D, t, bb in zip(Ds, systems, bs):
+                xx = torch.zeros(t.matrix.n, dtype=torch.float64, device="cuda")
+                its.append(D.solve(method, 1, xx, bb.clone(), cap, t.tol)[0])
+            return its
+        mss, its = _timed(torch, serial, args.steps)
+
+        def serial_mixed():
+            its = []
+            for D, t, bb in zip(Ds, systems, bs):
+                xx = torch.zeros(t.matrix.n, dtype=torch.float64, device="cuda")
+                its.append(D.solve_mixed(method, 1, xx, bb, cap, t.tol, inner_rtol=args.inner_rtol)[0])
+            return its
+        mssm, _ = _timed(torch, serial_mixed, args.steps)
+        entry["serial_ms"] = round(mss, 2)
+        entry["serial_mixed_ms"] = round(mssm, 2)
+        entry["speedup_vs_serial"] = round(mss / entry["tuned"]["ms"], 3)
+        entry["mixed_speedup_vs_serial"] = round(mssm / entry["tuned"]["mixed_ms"], 3)
+        out[method] = entry
     for D in Ds:
         D.close()
-    return {"metric": "batched solve time vs one GPU solve per system", "unit": "ms",
-            "workload": f"{len(systems)} {'transport (shared 96^3 pattern)' if kind == 'transport' else 'pressure'} "
-                        f"systems, {sum(t.matrix.n for t in systems)} rows, {method} + Jacobi, cap {cap}",
-            "batched_ms": round(ms, 2), "serial_ms": round(mss, 2), "speedup": round(mss / ms, 3),
-            "iterations_sum": int(np.sum(res["niter"])), "iterations_max": int(np.max(res["niter"])),
-            "converged": int(np.sum(res["fin_res"] < tol)), "config": cfg}
+    return out
 
 
 def dic_sweep_cost(torch, ds, s, iters=50, steps=3):
@@ -571,6 +743,62 @@ def workload_wall(args, torch, pkg):
     return out
 
 
+def workload_dropin(args, torch, pkg):
+    """The drop-in path as the reference's outer loop uses it: cg_ called
+    from host arrays (Process/CalcPS.f90:217) on one pattern, each call with
+    new values (rescaled face coefficients, as a later outer iteration makes
+    them) and a new right-hand side.  The first call builds the device
+    pattern; later calls upload only A, x and b and download x and r.
+    Reported against the device-resident solve of the same system (values
+    and vectors already in HBM) and the bytes each call moves over PCIe."""
+    n = args.n
+    mg = pkg.meshgen
+    kr = pkg.krylov
+    log(f"generating {n}^3 pressure system and value sets")
+    s = mg.pressure_system(n, seed=1, prec=1)
+    m0 = s.matrix
+    sets = [mg.rescaled_pressure_values(m0, 10 + k) for k in range(3)]
+    rhs = [mg.seeded_rhs(m0.n, 20 + k) for k in range(3)]
+    out = {"metric": "drop-in cg_ from host arrays, one pattern, new values per call", "unit": "ms",
+           "workload": f"pressure-correction {n}^3 channel (N={m0.n}, nnz={m0.nnz}), tolerance 1e-8 ||b||",
+           "pcie_bytes_per_call": 8 * m0.nonz + 8 * 2 * m0.n + 8 * 2 * m0.n}
+    for prec, calls in ((1, args.dropin_calls), (2, max(3, args.dropin_calls // 2))):
+        kr.dropin_release()
+        times, iters = [], []
+        for c in range(calls):
+            mv, b = sets[c % 3], rhs[c % 3]
+            tol = 1e-8 * float(np.linalg.norm(b))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res = kr.solve_system("cg", mv, b, prec=prec, niter=20000, tol=tol)
+            times.append((time.perf_counter() - t0) * 1e3)
+            iters.append(res["niter"])
+            assert res["fin_res"] < tol, f"drop-in call {c} (prec {prec}) did not converge"
+        st = kr.dropin_stats()
+        # the same solves device resident: values and vectors already in HBM
+        ds = kr.DeviceSystem(sets[0])
+        dev = []
+        for c in range(3):
+            mv, b = sets[c], rhs[c]
+            ds.set_values(torch.from_numpy(mv.aval).cuda())
+            bt = torch.from_numpy(b).cuda()
+            x = torch.zeros(m0.n, dtype=torch.float64, device="cuda")
+            tol = 1e-8 * float(np.linalg.norm(b))
+            ds.solve("cg", prec, x, bt.clone(), 20000, tol)      # first use: operator choice, levels
+            ms, _ = _timed(torch, lambda: (x.zero_(), ds.solve("cg", prec, x, bt.clone(), 20000, tol))[1], 1)
+            dev.append(ms)
+        ds.close()
+        steady = float(np.median(times[1:]))
+        out[f"prec{prec}"] = dict(calls=calls, first_call_ms=round(times[0], 1), steady_ms=round(steady, 1),
+                                  call_ms=[round(t, 1) for t in times], iters=iters,
+                                  device_resident_ms=round(float(np.median(dev)), 1),
+                                  overhead_ms=round(steady - float(np.median(dev)), 1),
+                                  pattern_builds=st["pattern_builds"])
+        log(f"dropin prec {prec}: {out[f'prec{prec}']}")
+    kr.dropin_release()
+    return out
+
+
 def workload_adjoint(args, torch, pkg):
     """96^3 perturbed, renumbered pressure problem: adjoint gradients vs
     central differences and an FP64 CPU gradient."""
@@ -589,16 +817,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128, help="cells per direction")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-hbm", action="store_true", help="skip the 256^3 SpMV and copy / triad figures")
     ap.add_argument("--interval", type=int, default=None,
                     help="FP32 CG iterations between FP64 residual replacements (default 75 for the pressure "
                          "metric, the best of 50 / 75 / 100 there; 50 for the wall workload)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--workload", default="pressure",
                     choices=["pressure", "momentum", "batch-transport", "batch-pressure", "wall", "dic-sweep",
-                             "adjoint", "calcps", "uvw"],
+                             "adjoint", "calcps", "uvw", "dropin"],
                     help="pressure: the headline 128^3 benchmark; the others measure the remaining "
                          "configurations and print their own JSON line")
     ap.add_argument("--method", default=None, help="Krylov method of the batch workloads")
+    ap.add_argument("--dropin-calls", type=int, default=10, help="cg_ calls of the dropin workload")
     ap.add_argument("--inner-rtol", type=float, default=1e-3, help="inner tolerance of FP32 refinement steps")
     ap.add_argument("--dic-cap", type=int, default=4000, help="iteration cap of the DIC solve (wall workload)")
     ap.add_argument("--sweep-iters", type=int, default=50,
@@ -608,8 +838,8 @@ def main():
                          "for hardware-counter runs; prints nothing on stdout")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note "
-              "(DESIGN.md), not a measured path", file=sys.stderr)
+        print("bench.py: this build runs on one GPU only; sharding over several GPUs is designed, "
+              "not built or measured", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -626,6 +856,8 @@ def main():
             out = workload_calcps(args, torch, pkg)
         elif args.workload == "uvw":
             out = workload_uvw(args, torch, pkg)
+        elif args.workload == "dropin":
+            out = workload_dropin(args, torch, pkg)
         elif args.workload in ("wall", "dic-sweep"):
             if args.n == 128:
                 args.n = 256
@@ -639,23 +871,38 @@ def main():
         return
 
     n = args.n
+    if args.profile_spmv:
+        # both operator forms in both precisions, fixed rather than chosen
+        # by timing (a profiler's per-dispatch overhead disturbs the timing):
+        # the 128^3 channel pressure system, or from 256^3 on the
+        # wall-clustered one built on the GPU
+        if n >= 256:
+            N, nonz, arrays = device_wall_system(torch, pkg, n)
+
+            def make():
+                return pkg.krylov.DeviceSystem.from_device(N, nonz, *arrays)
+        else:
+            m = pkg.meshgen.pressure_system(n, seed=1, prec=1).matrix
+            N = m.n
+
+            def make():
+                return pkg.krylov.DeviceSystem(m)
+        systems = []
+        for form in ("sliced", "csr"):
+            os.environ["TFLOWS_FP32_OP"] = os.environ["TFLOWS_FP64_OP"] = form
+            systems.append(make())
+        del os.environ["TFLOWS_FP32_OP"], os.environ["TFLOWS_FP64_OP"]
+        for d in systems:
+            log(f"layouts: fp64 {d.fp64_layout()} fp32 {d.fp32_layout()}")
+        profile_spmv(torch, systems[0], N, ds_csr=systems[1])
+        for d in systems:
+            d.close()
+        return
     log(f"generating {n}^3 pressure system")
     s = pkg.meshgen.pressure_system(n, seed=1, prec=1)
     m = s.matrix
     N, nnz = m.n, m.nnz
     log(f"N={N} nnz={nnz}")
-    if args.profile_spmv:
-        # both operator forms in both precisions, fixed rather than chosen
-        # by timing (a profiler's per-dispatch overhead disturbs the timing)
-        systems = []
-        for form in ("sliced", "csr"):
-            os.environ["TFLOWS_FP32_OP"] = os.environ["TFLOWS_FP64_OP"] = form
-            systems.append(pkg.krylov.DeviceSystem(m))
-        del os.environ["TFLOWS_FP32_OP"], os.environ["TFLOWS_FP64_OP"]
-        profile_spmv(torch, systems[0], N, ds_csr=systems[1])
-        for d in systems:
-            d.close()
-        return
     ds = pkg.krylov.DeviceSystem(m)
     b_dev = torch.from_numpy(s.b).cuda()
     x = torch.zeros(N, dtype=torch.float64, device="cuda")
@@ -702,10 +949,11 @@ def main():
 
     # SpMV bandwidth (dominant kernel), timed on the stream it runs on
     t64 = time_spmv(torch, ds, N, fp32=False)
-    b64 = spmv_bytes(N, nnz, 8)
+    l64 = ds.fp64_layout()
+    b64 = format_bytes(N, nnz, 8, l64)   # what the kernel in use moves
     t32 = time_spmv(torch, ds, N, fp32=True)
     layout = ds.fp32_layout()
-    b32 = spmv_bytes(N, nnz, 4, sliced=layout["sliced"], col_bytes=layout["col_bytes"])
+    b32 = format_bytes(N, nnz, 4, layout)
     k32 = "k_spmv<DiffSell32>" if layout["sliced"] else "k_spmv<DiffCsr32>"
     head = "mixed" if "mixed" in results else "fp64"
     spmv_t, spmv_b = (t32, b32) if head == "mixed" else (t64, b64)
@@ -737,25 +985,44 @@ def main():
         "speedup_mixed_vs_fp64": (round(results["fp64"]["ms"] / results["mixed"]["ms"], 3)
                                   if "mixed" in results else None),
         "spmv_fp64_GBs": round(b64 / t64 / 1e9, 1),
+        "spmv_fp64_layout": layout_name(l64),
+        "spmv_fp64_csr_equivalent_GBs": round(spmv_bytes(N, nnz, 8) / t64 / 1e9, 1),
         "spmv_fp32_GBs": round(b32 / t32 / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": (measured_traffic((("fp32_sliced16" if layout["col_bytes"] == 2 else "fp32_sliced")
-                                                   if layout["sliced"] else "fp32_csr")
-                                                  if head == "mixed" else "fp64") if n == 128 else None),
-                     "kernel": k32 if head == "mixed" else "k_spmv<Csr<double>>",
+                     "traffic": (measured_traffic(layout_tag(layout, 32) if head == "mixed" else layout_tag(l64, 64))
+                                 if n == 128 else None),
+                     "kernel": k32 if head == "mixed" else ("k_spmv<Sell<double>>" if l64["sliced"] else
+                                                            "k_spmv<Csr<double>>"),
                      "bytes_per_launch": spmv_b, "us_per_launch": round(spmv_t * 1e6, 2)},
-        "fp32_operator": {"layout": ("sliced ELL (64-row slices)" if layout["sliced"] else "CSR") +
-                                    (", 16-bit column offsets" if layout["col_bytes"] == 2 else ""),
+        "fp32_operator": {"layout": layout_name(layout),
                           "stored_entries": layout["stored_entries"], "off_diagonal_entries": nnz - N,
                           # the same launch priced as a CSR kernel with int32 indices and
                           # the diagonal stored moves it: 8 nnz + 4 (N+1) + 8 N
                           "csr_equivalent_GBs": round(spmv32_canonical_bytes(N, nnz) / t32 / 1e9, 1)},
     }
+    ds.close()
+    del ds, x, r, b_dev
+    torch.cuda.empty_cache()
+    if not args.no_hbm:
+        # the MALL-free figures: the same operators at 256^3 and what a copy
+        # and a STREAM triad reach on this device
+        hbm = spmv_hbm(torch, pkg, 256)
+        ach = achievable_bandwidth(torch)
+        hbm["achievable"] = ach
+        for key in ("fp64", "fp32"):
+            hbm[key]["frac_of_triad"] = round(hbm[key]["GBs_format"] / ach["triad_GBs"], 4)
+        out["spmv_hbm_256"] = hbm
+        h = hbm["fp32" if head == "mixed" else "fp64"]
+        out["roofline"]["hbm_256"] = {"kernel": h["kernel"], "achieved": h["GBs_format"], "peak": HBM_PEAK_GBS,
+                                      "frac": h["frac_format"], "frac_csr_bytes": h["frac_csr"],
+                                      "achievable_triad": ach["triad_GBs"], "frac_of_triad": h["frac_of_triad"]}
+        out["roofline"]["note"] = ("achieved/frac: the 128^3 launch, whose matrix sits largely in the 256 MB MALL; "
+                                   "hbm_256: the same kernel at 256^3, streamed from HBM")
+        log(f"hbm 256: {hbm}")
     if not args.no_cpu:
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(m, s.b, s.tol, results["fp64"]["iters"], args.cpu_budget)
-    ds.close()
     print(json.dumps(out), flush=True)
 
 

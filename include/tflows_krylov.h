@@ -234,17 +234,24 @@ int32_t tflows_system_reordered(tflows_system s);
  * FP32).  *sliced = 1: sliced ELL, 64-row slices stored slot-major and
  * padded to the slice's longest row; 2: the same with each column held as
  * a 16-bit offset from its row (chosen where every offset fits, unless
- * TFLOWS_FP32_COL16=0); 0: CSR (environment TFLOWS_FP32_OP=csr when the
- * system was created, or slower by the first use's timing).
- * *stored_entries: off-diagonal entries held, padding included.  Builds
- * the operator if no solve has yet.  No counterpart in the reference. */
+ * TFLOWS_FP32_COL16=0); 3: the same slices with the column offsets shared
+ * by each slice -- one ascending list of at most 8 offsets per slice and a
+ * slot mask per row, no column stored per entry (chosen where each row's
+ * columns ascend and the slices' offset lists stay short, as on a
+ * naturally numbered structured mesh, unless TFLOWS_SHARED_SLOTS=0); 0:
+ * CSR (environment TFLOWS_FP32_OP=csr when the system was created, or
+ * slower by the first use's timing).  *stored_entries: off-diagonal
+ * entries held, padding included.  Every layout gives the same bits.
+ * Builds the operator if no solve has yet.  No counterpart in the
+ * reference. */
 int tflows_system_fp32_layout(tflows_system s, int32_t* sliced, int64_t* stored_entries, void* stream);
 
 /* Layout of the FP64 operator of the solver iterations (A itself, the
  * diagonal where it stands in each row): *sliced = 1 sliced ELL, 2 the
  * same with 16-bit column offsets (where every offset fits, unless
- * TFLOWS_FP64_COL16=0), 0 the CSR arrays; *stored_entries: entries held,
- * padding included.  No counterpart in the reference. */
+ * TFLOWS_FP64_COL16=0), 3 the same with column offsets shared by each
+ * slice (as for the FP32 operator), 0 the CSR arrays; *stored_entries:
+ * entries held, padding included.  No counterpart in the reference. */
 int tflows_system_fp64_layout(tflows_system s, int32_t* sliced, int64_t* stored_entries, void* stream);
 
 /* ------------------------------------------------------------------------

@@ -140,8 +140,13 @@ struct DiffSell32 {
     // when not null: col - row of every slot as 16 bits, read instead of
     // col (made where every column lies within 32767 rows of its row)
     const short* __restrict__ dc = nullptr;
+    // when not null: shared slot offsets (see Sell), read instead of col
+    const int* __restrict__ soff = nullptr;
+    const unsigned char* __restrict__ mask = nullptr;
 };
 constexpr int kSlice = 64;
+// most slots of a slice in the shared-offset form (bits of the row mask)
+constexpr int kMaxSharedSlots = 8;
 
 // A matrix in sliced ELL layout: rows in slices of 64 (one wavefront), the
 // entries of a slice stored slot-major -- slot j of row 64 s + l at
@@ -159,6 +164,14 @@ struct Sell {
     // when not null: col - row of every slot as 16 bits (kPad16: padding),
     // read instead of col (made where every column lies within 32767 rows)
     const short* __restrict__ dc = nullptr;
+    // when not null: shared slot offsets.  Every row of slice s uses the
+    // slice's ascending list of column offsets, slot j holding offset
+    // soff[off[s] / 64 + j] (column = row + offset) for every row that has
+    // it, which bit j of mask[row] marks; no column index is stored per
+    // entry.  Made where each row's columns ascend without repeats, so the
+    // slots of a row run in its CSR order.
+    const int* __restrict__ soff = nullptr;
+    const unsigned char* __restrict__ mask = nullptr;
 };
 constexpr short kPad16 = -32768;
 
@@ -537,11 +550,20 @@ __device__ __forceinline__ void op_tile(const DiffCsr32& A, int r0, int r1, Gath
 }
 // Rows [r0, r1) of the sliced operator, row r0 + t on thread t (no early
 // return: the caller's block reductions follow).
+// The slice of a row.  When the tile starts on a slice boundary (every
+// tile of one system does) a wavefront's rows are one slice: the index is
+// then made wavefront-uniform, so the slice's offsets and width come from
+// scalar loads and the slot loop is uniform.
+__device__ __forceinline__ int row_slice(int r0, int row) {
+    const int sl = row / kSlice;
+    return (r0 & (kSlice - 1)) == 0 ? __builtin_amdgcn_readfirstlane(sl) : sl;
+}
+
 template <int NRHS, class Gather, class Epi>
 __device__ __forceinline__ void sell_tile(const DiffSell32& A, int r0, int r1, Gather& g, Epi& epi) {
     const int row = r0 + (int)threadIdx.x;
     if (row < r1) {
-        const int sl = row / kSlice;
+        const int sl = row_slice(r0, row);
         const int b = A.off[sl];
         const int w = (A.off[sl + 1] - b) / kSlice;
         const float* __restrict__ vp = A.val + b + (row - sl * kSlice);
@@ -555,7 +577,16 @@ __device__ __forceinline__ void sell_tile(const DiffSell32& A, int r0, int r1, G
 #pragma unroll
             for (int v = 0; v < NRHS; ++v) acc[v] = acc[v] + a * (gv[v] - own[v]);
         };
-        if (A.dc) {                                  // the same columns, 16-bit
+        if (A.soff) {                                // offsets shared by the slice
+            // a slot the row lacks takes the row's own column with a = 0 and
+            // adds a (p_i - p_i) = +-0, which leaves every sum as it is (one
+            // that starts at +0 never becomes -0); no branch, so the loads of
+            // all slots are in flight together
+            const int* __restrict__ so = A.soff + b / kSlice;
+            const unsigned m = A.mask[row];
+#pragma unroll 2
+            for (int j = 0; j < w; ++j) slot(((m >> j) & 1u) ? row + so[j] : row, vp[j * kSlice]);
+        } else if (A.dc) {                           // the same columns, 16-bit
             const short* __restrict__ dp = A.dc + b + (row - sl * kSlice);
 #pragma unroll 2
             for (int j = 0; j < w; ++j) slot(row + (int)dp[j * kSlice], vp[j * kSlice]);
@@ -581,7 +612,7 @@ template <typename T, int NRHS, bool SUBTRACT, class Gather, class Init, class E
 __device__ __forceinline__ void sell_rows(const Sell<T>& A, int r0, int r1, Gather& g, Init& init, Epi& epi) {
     const int row = r0 + (int)threadIdx.x;
     if (row < r1) {
-        const int sl = row / kSlice;
+        const int sl = row_slice(r0, row);
         const int b = A.off[sl];
         const int w = (A.off[sl + 1] - b) / kSlice;
         const int* __restrict__ cp = A.col + b + (row - sl * kSlice);
@@ -598,7 +629,13 @@ __device__ __forceinline__ void sell_rows(const Sell<T>& A, int r0, int r1, Gath
                 acc[v] = SUBTRACT ? acc[v] - prod : acc[v] + prod;
             }
         };
-        if (A.dc) {                                  // the same columns, 16-bit
+        if (A.soff) {                                // offsets shared by the slice
+            const int* __restrict__ so = A.soff + b / kSlice;
+            const unsigned m = A.mask[row];
+#pragma unroll 2
+            for (int j = 0; j < w; ++j)
+                if ((m >> j) & 1u) slot(row + so[j], vp[j * kSlice]);
+        } else if (A.dc) {                           // the same columns, 16-bit
             const short* __restrict__ dp = A.dc + b + (row - sl * kSlice);
 #pragma unroll 2
             for (int j = 0; j < w; ++j) {

@@ -221,26 +221,40 @@ DiffSell32 System::sell32(hipStream_t s) {
         if (noff > 0)
             TF_CHECK(hipMemcpyAsync(ocol.data(), d.col, sizeof(int) * noff, hipMemcpyDeviceToHost, s));
         TF_CHECK(hipStreamSynchronize(s));
-        slice_layout(n_, orp, ocol, true, off, col, map);
-        sell_off_.ensure(off.size());
-        sell_col_.ensure(col.size());
-        sell_map_.ensure(map.size());
-        sell_val_.ensure(col.size());
-        TF_CHECK(hipMemcpy(sell_off_.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
-        if (!col.empty()) {
-            TF_CHECK(hipMemcpy(sell_col_.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
-            TF_CHECK(hipMemcpy(sell_map_.p, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));
-        }
-        // the 16-bit column form where every slot's column lies within
-        // 32767 rows of its row (TFLOWS_FP32_COL16=0 keeps 32-bit columns)
-        std::vector<short> dc;
-        if (col16_from_env("TFLOWS_FP32_COL16") && slot_offsets16(n_, off, col, true, dc)) {
-            sell_dc_.ensure(dc.size());
-            TF_CHECK(hipMemcpy(sell_dc_.p, dc.data(), sizeof(short) * dc.size(), hipMemcpyHostToDevice));
-        } else {
+        std::vector<int> soff;
+        std::vector<unsigned char> mask;
+        if (shared_slots_from_env() && noff > 0 && shared_slot_layout(n_, orp, ocol, off, soff, mask, map)) {
+            // one column offset per slot of a slice, a mask per row
+            sell_soff_.ensure(soff.size());
+            sell_mask_.ensure(mask.size());
+            TF_CHECK(hipMemcpy(sell_soff_.p, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice));
+            TF_CHECK(hipMemcpy(sell_mask_.p, mask.data(), mask.size(), hipMemcpyHostToDevice));
+            sell_col_.release();
             sell_dc_.release();
+        } else {
+            sell_soff_.release();
+            sell_mask_.release();
+            slice_layout(n_, orp, ocol, true, off, col, map);
+            sell_col_.ensure(col.size());
+            if (!col.empty())
+                TF_CHECK(hipMemcpy(sell_col_.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
+            // the 16-bit column form where every slot's column lies within
+            // 32767 rows of its row (TFLOWS_FP32_COL16=0 keeps 32-bit columns)
+            std::vector<short> dc;
+            if (col16_from_env("TFLOWS_FP32_COL16") && slot_offsets16(n_, off, col, true, dc)) {
+                sell_dc_.ensure(dc.size());
+                TF_CHECK(hipMemcpy(sell_dc_.p, dc.data(), sizeof(short) * dc.size(), hipMemcpyHostToDevice));
+            } else {
+                sell_dc_.release();
+            }
         }
-        sell_entries_ = (int64_t)col.size();
+        sell_off_.ensure(off.size());
+        sell_map_.ensure(map.size());
+        sell_val_.ensure(map.size());
+        TF_CHECK(hipMemcpy(sell_off_.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+        if (!map.empty())
+            TF_CHECK(hipMemcpy(sell_map_.p, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));
+        sell_entries_ = (int64_t)map.size();
         sell_built_ = true;
         sell_valid_ = false;
     }
@@ -251,7 +265,7 @@ DiffSell32 System::sell32(hipStream_t s) {
         TF_CHECK(hipGetLastError());
         sell_valid_ = true;
     }
-    return DiffSell32{n_, sell_off_.p, sell_col_.p, sell_val_.p, d.rs, sell_dc_.p};
+    return DiffSell32{n_, sell_off_.p, sell_col_.p, sell_val_.p, d.rs, sell_dc_.p, sell_soff_.p, sell_mask_.p};
 }
 
 const float* System::diag32(hipStream_t s) {

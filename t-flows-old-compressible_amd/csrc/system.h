@@ -121,6 +121,17 @@ bool slot_offsets16(int n, const std::vector<int>& off, const std::vector<int>& 
 bool col16_from_env(const char* name);
 void slice_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl, bool pad_own, std::vector<int>& off,
                   std::vector<int>& col, std::vector<int>& map);
+// The shared-offset sliced layout (kernels.h, Sell::soff) of a CSR pattern
+// with row pointer rp and columns cl: slice value offsets, each slice's
+// ascending column offsets, the per-row slot masks and the CSR entry of
+// every slot (-1 where the row lacks the slot: value 0, never read).  Made
+// when every row's columns ascend without repeats, no slice needs more than
+// kMaxSharedSlots offsets and the slots stay within 1.15 x the entries;
+// false otherwise.  TFLOWS_SHARED_SLOTS=0 (shared_slots_from_env) never
+// makes it.
+bool shared_slot_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl, std::vector<int>& off,
+                        std::vector<int>& soff, std::vector<unsigned char>& mask, std::vector<int>& map);
+bool shared_slots_from_env();
 
 // TFLOWS_DIC_SWEEP (read when a system first builds its levels): 0 sweeps
 // the matrix rows level by level, 1 the level-major copy level by level,
@@ -231,7 +242,7 @@ class System {
     // 16-bit column offsets, else 4
     int sell32_col_bytes(hipStream_t s) {
         sell32(s);
-        return sell_dc_.p ? 2 : 4;
+        return sell_soff_.p ? 0 : (sell_dc_.p ? 2 : 4);
     }
     int64_t sell32_entries(hipStream_t s) {
         sell32(s);
@@ -240,7 +251,7 @@ class System {
     // the same for the sliced FP64 operator
     int sell64_col_bytes(hipStream_t s) {
         sell64(s);
-        return sell64_dc_.p ? 2 : 4;
+        return sell64_soff_.p ? 0 : (sell64_dc_.p ? 2 : 4);
     }
     int64_t sell64_entries(hipStream_t s) {
         sell64(s);
@@ -375,12 +386,16 @@ class System {
     DevArray<int> off_rowptr_, off_col_;
     DevArray<int> sell_off_, sell_col_, sell_map_;
     DevArray<short> sell_dc_;       // col - row, 16-bit, where every slot fits
+    DevArray<int> sell_soff_;       // shared slot offsets, where the pattern allows
+    DevArray<unsigned char> sell_mask_;
     DevArray<float> sell_val_;
     int64_t sell_entries_ = 0;
     bool sell_built_ = false, sell_valid_ = false;
     int op32_mode_ = 2;                  // 0 CSR, 1 sliced, 2 decide by timing on first use
     DevArray<int> sell64_off_, sell64_col_, sell64_map_;
     DevArray<short> sell64_dc_;     // col - row, 16-bit, where every slot fits
+    DevArray<int> sell64_soff_;     // shared slot offsets, where the pattern allows
+    DevArray<unsigned char> sell64_mask_;
     DevArray<double> sell64_val_;
     int64_t sell64_entries_ = 0;
     bool sell64_built_ = false, sell64_valid_ = false;

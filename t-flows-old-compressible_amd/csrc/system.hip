@@ -140,6 +140,50 @@ void slice_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl,
     }
 }
 
+bool shared_slots_from_env() {
+    const char* e = std::getenv("TFLOWS_SHARED_SLOTS");
+    return !(e && e[0] == '0');
+}
+
+bool shared_slot_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl, std::vector<int>& off,
+                        std::vector<int>& soff, std::vector<unsigned char>& mask, std::vector<int>& map) {
+    const int nsl = (n + kSlice - 1) / kSlice;
+    const int64_t nnz = rp[n];
+    off.assign(nsl + 1, 0);
+    soff.clear();
+    std::vector<int> set;
+    for (int sl = 0; sl < nsl; ++sl) {
+        set.clear();
+        for (int i = sl * kSlice; i < std::min(n, (sl + 1) * kSlice); ++i)
+            for (int e = rp[i]; e < rp[i + 1]; ++e) {
+                if (e > rp[i] && cl[e] <= cl[e - 1]) return false;   // the slots must run in CSR order
+                set.push_back(cl[e] - i);
+            }
+        std::sort(set.begin(), set.end());
+        set.erase(std::unique(set.begin(), set.end()), set.end());
+        if ((int)set.size() > kMaxSharedSlots) return false;
+        const int64_t next = (int64_t)off[sl] + (int64_t)set.size() * kSlice;
+        if (next > INT32_MAX) return false;
+        off[sl + 1] = (int)next;
+        soff.insert(soff.end(), set.begin(), set.end());
+    }
+    if ((double)off[nsl] > 1.15 * (double)std::max<int64_t>(nnz, 1)) return false;
+    mask.assign((size_t)nsl * kSlice, 0);
+    map.assign(off[nsl], -1);
+    for (int sl = 0; sl < nsl; ++sl) {
+        const int* so = soff.data() + off[sl] / kSlice;
+        for (int i = sl * kSlice; i < std::min(n, (sl + 1) * kSlice); ++i) {
+            int j = 0;
+            for (int e = rp[i]; e < rp[i + 1]; ++e) {
+                while (so[j] != cl[e] - i) ++j;                      // both ascend; every offset is in the set
+                mask[i] |= (unsigned char)(1u << j);
+                map[off[sl] + j * kSlice + (i - sl * kSlice)] = e;
+            }
+        }
+    }
+    return true;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_slot_values(int64_t count, const int* __restrict__ map,
                                                         const T* __restrict__ src, T* __restrict__ dst) {
@@ -153,24 +197,38 @@ Sell<double> System::sell64(hipStream_t s) {
         TF_CHECK(hipMemcpyAsync(rp.data(), rowptr_.p, sizeof(int) * (n_ + 1), hipMemcpyDeviceToHost, s));
         TF_CHECK(hipMemcpyAsync(cl.data(), col_.p, sizeof(int) * nnz_, hipMemcpyDeviceToHost, s));
         TF_CHECK(hipStreamSynchronize(s));
-        slice_layout(n_, rp, cl, false, off, col, map);
-        sell64_off_.ensure(off.size());
-        sell64_col_.ensure(col.size());
-        sell64_map_.ensure(map.size());
-        sell64_val_.ensure(col.size());
-        TF_CHECK(hipMemcpy(sell64_off_.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
-        TF_CHECK(hipMemcpy(sell64_col_.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
-        TF_CHECK(hipMemcpy(sell64_map_.p, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));
-        // 16-bit column offsets where every one fits (TFLOWS_FP64_COL16=0
-        // keeps 32-bit columns); padding slots (column -1) become kPad16
-        std::vector<short> dc;
-        if (col16_from_env("TFLOWS_FP64_COL16") && slot_offsets16(n_, off, col, false, dc)) {
-            sell64_dc_.ensure(dc.size());
-            TF_CHECK(hipMemcpy(sell64_dc_.p, dc.data(), sizeof(short) * dc.size(), hipMemcpyHostToDevice));
-        } else {
+        std::vector<int> soff;
+        std::vector<unsigned char> mask;
+        if (shared_slots_from_env() && shared_slot_layout(n_, rp, cl, off, soff, mask, map)) {
+            // one column offset per slot of a slice, a mask per row
+            sell64_soff_.ensure(soff.size());
+            sell64_mask_.ensure(mask.size());
+            TF_CHECK(hipMemcpy(sell64_soff_.p, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice));
+            TF_CHECK(hipMemcpy(sell64_mask_.p, mask.data(), mask.size(), hipMemcpyHostToDevice));
+            sell64_col_.release();
             sell64_dc_.release();
+        } else {
+            sell64_soff_.release();
+            sell64_mask_.release();
+            slice_layout(n_, rp, cl, false, off, col, map);
+            sell64_col_.ensure(col.size());
+            TF_CHECK(hipMemcpy(sell64_col_.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
+            // 16-bit column offsets where every one fits (TFLOWS_FP64_COL16=0
+            // keeps 32-bit columns); padding slots (column -1) become kPad16
+            std::vector<short> dc;
+            if (col16_from_env("TFLOWS_FP64_COL16") && slot_offsets16(n_, off, col, false, dc)) {
+                sell64_dc_.ensure(dc.size());
+                TF_CHECK(hipMemcpy(sell64_dc_.p, dc.data(), sizeof(short) * dc.size(), hipMemcpyHostToDevice));
+            } else {
+                sell64_dc_.release();
+            }
         }
-        sell64_entries_ = (int64_t)col.size();
+        sell64_off_.ensure(off.size());
+        sell64_map_.ensure(map.size());
+        sell64_val_.ensure(map.size());
+        TF_CHECK(hipMemcpy(sell64_off_.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+        TF_CHECK(hipMemcpy(sell64_map_.p, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));
+        sell64_entries_ = (int64_t)map.size();
         sell64_built_ = true;
         sell64_valid_ = false;
     }
@@ -181,7 +239,8 @@ Sell<double> System::sell64(hipStream_t s) {
         TF_CHECK(hipGetLastError());
         sell64_valid_ = true;
     }
-    return Sell<double>{n_, sell64_off_.p, sell64_col_.p, sell64_val_.p, sell64_dc_.p};
+    return Sell<double>{n_, sell64_off_.p, sell64_col_.p, sell64_val_.p, sell64_dc_.p, sell64_soff_.p,
+                        sell64_mask_.p};
 }
 
 bool col16_from_env(const char* name) {

@@ -298,6 +298,14 @@ def solve_system(method: str, m, b, x0=None, prec=1, niter=2000, tol=0.0, nb=0):
 # --------------------------------------------------------------------------
 # device-resident interface (torch tensors in HBM)
 # --------------------------------------------------------------------------
+def _layout(code: int, entries: int) -> dict:
+    """Operator layout from the library's code: sliced or CSR; bytes per
+    stored column index (4, 2 for 16-bit offsets, 0 when the slices share
+    their column offsets); entries held, padding included."""
+    return {"sliced": code > 0, "shared": code == 3, "col_bytes": {3: 0, 2: 2}.get(code, 4),
+            "stored_entries": int(entries)}
+
+
 def _ptr(t) -> int:
     return int(t.data_ptr())
 
@@ -433,20 +441,19 @@ class DeviceSystem:
         return bool(self.lib.tflows_system_reordered(self.handle))
 
     def fp32_layout(self, stream=None) -> dict:
-        """Layout of the FP32 inner operator: {"sliced": bool,
-        "col_bytes": 4, or 2 for 16-bit column offsets, "stored_entries":
-        off-diagonal entries held, padding included}."""
-        sl, ent = C.c_int32(0), C.c_int64(0)
-        _check(self.lib.tflows_system_fp32_layout(self.handle, C.byref(sl), C.byref(ent), _stream(stream)))
-        return {"sliced": sl.value > 0, "col_bytes": 2 if sl.value == 2 else 4, "stored_entries": int(ent.value)}
-
-    def fp64_layout(self, stream=None) -> dict:
-        """Layout of the FP64 operator of the iterations: {"sliced": bool,
-        "col_bytes": 4, or 2 for 16-bit column offsets, "stored_entries":
+        """Layout of the FP32 inner operator: {"sliced": bool, "shared":
+        bool (column offsets shared by each slice), "col_bytes": 4, 2 for
+        16-bit column offsets, 0 when shared, "stored_entries": off-diagonal
         entries held, padding included}."""
         sl, ent = C.c_int32(0), C.c_int64(0)
+        _check(self.lib.tflows_system_fp32_layout(self.handle, C.byref(sl), C.byref(ent), _stream(stream)))
+        return _layout(sl.value, ent.value)
+
+    def fp64_layout(self, stream=None) -> dict:
+        """Layout of the FP64 operator of the iterations, as fp32_layout."""
+        sl, ent = C.c_int32(0), C.c_int64(0)
         _check(self.lib.tflows_system_fp64_layout(self.handle, C.byref(sl), C.byref(ent), _stream(stream)))
-        return {"sliced": sl.value > 0, "col_bytes": 2 if sl.value == 2 else 4, "stored_entries": int(ent.value)}
+        return _layout(sl.value, ent.value)
 
     def set_lanes(self, lanes: int):
         """SpMV form: lanes per row (1, 2, 4 or 8); results are identical."""

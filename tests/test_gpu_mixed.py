@@ -153,6 +153,7 @@ def test_fp32_sliced_column_width(pkg, cuda, monkeypatch, case):
     b = torch.from_numpy(s.b).to(cuda)
     out = {}
     monkeypatch.setenv("TFLOWS_FP32_OP", "sliced")
+    monkeypatch.setenv("TFLOWS_SHARED_SLOTS", "0")       # per-entry columns, the layout under test
     if case == "pressure_perm_wide":
         monkeypatch.setenv("TFLOWS_REORDER", "0")
     for flag in ("0", "1"):

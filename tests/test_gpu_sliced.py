@@ -101,6 +101,7 @@ def test_fp64_sliced_column_width(pkg, cuda, monkeypatch, case):
     m = s.matrix
     p = torch.from_numpy(np.random.default_rng(84).standard_normal(m.n)).to(cuda)
     monkeypatch.setenv("TFLOWS_FP64_OP", "sliced")
+    monkeypatch.setenv("TFLOWS_SHARED_SLOTS", "0")       # per-entry columns, the layout under test
     if case == "pressure_perm_wide":
         monkeypatch.setenv("TFLOWS_REORDER", "0")
     out = {}
@@ -148,3 +149,71 @@ def test_mixed_replacement_residual_bitwise(pkg, cuda, monkeypatch, case, prec, 
     assert a[0][:3] == c[0][:3]
     assert np.array_equal(a[1], c[1]) and np.array_equal(a[2], c[2])
     assert c[0][2] < s.tol
+
+
+# layouts of the sliced operators: per-entry int32 columns, 16-bit offsets,
+# and column offsets shared by each slice (one list per slice, a mask per row)
+LAYOUTS = {"int32": {"TFLOWS_SHARED_SLOTS": "0", "TFLOWS_FP64_COL16": "0", "TFLOWS_FP32_COL16": "0"},
+           "col16": {"TFLOWS_SHARED_SLOTS": "0", "TFLOWS_FP64_COL16": "1", "TFLOWS_FP32_COL16": "1"},
+           "shared": {"TFLOWS_SHARED_SLOTS": "1", "TFLOWS_FP64_COL16": "1", "TFLOWS_FP32_COL16": "1"}}
+
+
+@pytest.mark.parametrize("case", ["pressure", "channel_aniso", "pressure_perm"])
+def test_shared_slot_layout_bitwise(pkg, oracle, cuda, monkeypatch, case):
+    """The shared-offset layout holds each slot's column as one offset per
+    slice and marks the slots each row has; a row's slots run in its CSR
+    order, so the FP64 and FP32 SpMVs, the solves and the reliable-update
+    mixed solve give the other layouts' bits (and the FP64 SpMV the
+    oracle's).  It is chosen on naturally numbered structured meshes; the
+    randomly renumbered system (locality order, columns no longer ascending
+    within a row) keeps per-entry columns."""
+    import torch
+    mg = pkg.meshgen
+    # cubes: flat boxes are stored in their reverse Cuthill-McKee order,
+    # which the shared layout cannot follow
+    s = {"pressure": lambda: mg.pressure_system(20, seed=91),
+         "channel_aniso": lambda: mg.wall_pressure_system(24),
+         "pressure_perm": lambda: mg.pressure_system(12, seed=92, perm_seed=5)}[case]()
+    m = s.matrix
+    rng = np.random.default_rng(93)
+    p64 = rng.standard_normal(m.n)
+    p = torch.from_numpy(p64).to(cuda)
+    p32 = p.float()
+    monkeypatch.setenv("TFLOWS_FP64_OP", "sliced")
+    monkeypatch.setenv("TFLOWS_FP32_OP", "sliced")
+    b = torch.from_numpy(s.b.copy()).to(cuda)
+    out = {}
+    for name, env in LAYOUTS.items():
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        ds = pkg.krylov.DeviceSystem(m)
+        lay = (ds.fp64_layout(), ds.fp32_layout())
+        q = torch.empty_like(p)
+        ds.spmv(p, q)
+        q32 = torch.empty_like(p32)
+        ds.spmv32(p32, q32)
+        runs = []
+        for method, prec in (("cg", 1), ("cgs", 1), ("cg", 2)):
+            x = torch.zeros(m.n, dtype=torch.float64, device=cuda)
+            r = b.clone()
+            runs.append((ds.solve(method, prec, x, r, 2000, s.tol), x.cpu().numpy(), r.cpu().numpy()))
+        xm = torch.zeros(m.n, dtype=torch.float64, device=cuda)
+        mres = ds.solve_mixed_reliable(1, xm, b, 3000, s.tol, interval=40)
+        torch.cuda.synchronize()
+        out[name] = (lay, q.cpu().numpy(), q32.cpu().numpy(), runs, mres, xm.cpu().numpy())
+        ds.close()
+    shared = out["shared"]
+    print(f"{case}: layouts {shared[0]}", flush=True)
+    if case == "pressure_perm":
+        assert not shared[0][0]["shared"] and not shared[0][1]["shared"]
+    else:
+        assert shared[0][0]["shared"] and shared[0][1]["shared"]
+        assert shared[0][0]["stored_entries"] <= 1.15 * m.nnz
+    assert np.array_equal(shared[1], oracle.spmv(m, p64))
+    for other in ("int32", "col16"):
+        o = out[other]
+        assert np.array_equal(o[1], shared[1]) and np.array_equal(o[2], shared[2]), other
+        for (ra, xa, rra), (rb, xb, rrb) in zip(o[3], shared[3]):
+            assert ra == rb and np.array_equal(xa, xb) and np.array_equal(rra, rrb), other
+        assert o[4][:3] == shared[4][:3] and np.array_equal(o[5], shared[5]), other
+    assert shared[3][0][0][2] < s.tol and shared[4][2] < s.tol
