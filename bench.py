@@ -352,9 +352,14 @@ def workload_batch(args, torch, pkg, kind):
     if kind == "transport":
         systems, cap = mg.transport_batch(n=96, count=5, seed=3), 2000
         methods = [args.method] if args.method else ["bicg", "cgs"]
+        with_mixed = True
     else:
+        # FP64 as specified; the batched mixed solve refines by restarts,
+        # which stall on the pinned pressure matrices (their near-null mode
+        # is below what an FP32 correction resolves)
         systems, cap = mg.uneven_pressure_batch(count=64, seed=3), 5000
         methods = [args.method] if args.method else ["cg"]
+        with_mixed = False
     tol = np.array([t.tol for t in systems])
     b = torch.from_numpy(np.concatenate([t.b for t in systems])).cuda()
     x = torch.zeros(b.numel(), dtype=torch.float64, device="cuda")
@@ -377,17 +382,19 @@ def workload_batch(args, torch, pkg, kind):
                 return B.solve(method, 1, x, r, cap, tol)
             ms, res = _timed(torch, batched, args.steps + 1)   # the first call also tunes and captures
             cfg = B.config(method, 1)
-            xm = torch.zeros_like(x)
-
-            def batched_mixed():
-                xm.zero_()
-                return B.solve_mixed(method, 1, xm, b, cap, tol, inner_rtol=args.inner_rtol)
-            msm, resm = _timed(torch, batched_mixed, args.steps + 1)
-            cfgm = B.config(method, 1, bits=32)
             entry[name] = dict(ms=round(ms, 2), iterations_sum=int(np.sum(res["niter"])),
                                iterations_max=int(np.max(res["niter"])), converged=int(np.sum(res["fin_res"] < tol)),
-                               config=cfg, mixed_ms=round(msm, 2), mixed_iterations_sum=int(np.sum(resm["niter"])),
-                               mixed_converged=int(np.sum(resm["fin_res"] < tol)), mixed_config=cfgm)
+                               config=cfg)
+            if with_mixed:
+                xm = torch.zeros_like(x)
+
+                def batched_mixed():
+                    xm.zero_()
+                    return B.solve_mixed(method, 1, xm, b, cap, tol, inner_rtol=args.inner_rtol)
+                msm, resm = _timed(torch, batched_mixed, args.steps + 1)
+                entry[name].update(mixed_ms=round(msm, 2), mixed_iterations_sum=int(np.sum(resm["niter"])),
+                                   mixed_converged=int(np.sum(resm["fin_res"] < tol)),
+                                   mixed_config=B.config(method, 1, bits=32))
             B.close()
             log(f"batch {kind} {method} {name}: {entry[name]}")
 
@@ -398,18 +405,18 @@ def workload_batch(args, torch, pkg, kind):
                 its.append(D.solve(method, 1, xx, bb.clone(), cap, t.tol)[0])
             return its
         mss, its = _timed(torch, serial, args.steps)
-
-        def serial_mixed():
-            its = []
-            for D, t, bb in zip(Ds, systems, bs):
-                xx = torch.zeros(t.matrix.n, dtype=torch.float64, device="cuda")
-                its.append(D.solve_mixed(method, 1, xx, bb, cap, t.tol, inner_rtol=args.inner_rtol)[0])
-            return its
-        mssm, _ = _timed(torch, serial_mixed, args.steps)
         entry["serial_ms"] = round(mss, 2)
-        entry["serial_mixed_ms"] = round(mssm, 2)
         entry["speedup_vs_serial"] = round(mss / entry["tuned"]["ms"], 3)
-        entry["mixed_speedup_vs_serial"] = round(mssm / entry["tuned"]["mixed_ms"], 3)
+        if with_mixed:
+            def serial_mixed():
+                its = []
+                for D, t, bb in zip(Ds, systems, bs):
+                    xx = torch.zeros(t.matrix.n, dtype=torch.float64, device="cuda")
+                    its.append(D.solve_mixed(method, 1, xx, bb, cap, t.tol, inner_rtol=args.inner_rtol)[0])
+                return its
+            mssm, _ = _timed(torch, serial_mixed, args.steps)
+            entry["serial_mixed_ms"] = round(mssm, 2)
+            entry["mixed_speedup_vs_serial"] = round(mssm / entry["tuned"]["mixed_ms"], 3)
         out[method] = entry
     for D in Ds:
         D.close()

@@ -366,9 +366,8 @@ int tflows_calcps_correct(tflows_calcps h, double dt, const double* pp, const do
 /* The momentum assembly of one velocity component, Process/NewUVW.f90:137-499
  * (the reference's `newuvw` up to its BiCG call at :505-507), on the mesh of
  * a tflows_calcps handle.  Covered: CONVEC, DIFFUS (CN or FI), CROSS = FI or
- * not; INERT = LIN or PAR; BLEND = YES or NO (central convection; the TVD
- * face schemes of Process/ConvScheme.f90 are refused with an error);
- * BLEND_TEM upwind part; every boundary type of :383-408; Asave (var = 1);
+ * not; INERT = LIN or PAR; BLEND = YES, NO or one of the face schemes of
+ * Process/ConvScheme.f90 (listed below); BLEND_TEM upwind part; every boundary type of :383-408; Asave (var = 1);
  * under-relaxation with U%URF.  VISeff = VISc (the default of :287-295).
  * Diagonal, off-diagonals, b, Abou, Asave and rhoUi are the reference's bit
  * for bit.  The caller keeps the old-level shift (:154-157) and supplies
