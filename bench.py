@@ -423,16 +423,15 @@ def workload_batch(args, torch, pkg, kind):
     return out
 
 
-def dic_sweep_cost(torch, ds, s, iters=50, steps=3):
+def dic_sweep_cost(torch, ds, b_host, n_rows, nnz, iters=50, steps=3):
     """Cost of one DIC apply: CG with Jacobi and with DIC, each capped at
     `iters` iterations (best of `steps`, after a warm-up that builds the
     levels and the level-major triangles), differ per iteration by the two
     triangular sweeps.  Algorithmic bytes of one sweep: 12 F + 4 (N+1) +
     28 N with F = (nnz - N) / 2 (triangle col + val, its row pointer, u,
     D, z and the level-major staging of u and D)."""
-    m = s.matrix
-    b = torch.from_numpy(s.b).cuda()
-    x = torch.zeros(m.n, dtype=torch.float64, device="cuda")
+    b = torch.from_numpy(b_host).cuda()
+    x = torch.zeros(n_rows, dtype=torch.float64, device="cuda")
     r = torch.empty_like(b)
     per_it = {}
     for prec in (1, 2):
@@ -443,8 +442,8 @@ def dic_sweep_cost(torch, ds, s, iters=50, steps=3):
         ms, _ = _timed(torch, f, steps)
         per_it[prec] = ms / iters
     sweep = (per_it[2] - per_it[1]) / 2
-    F = (m.nnz - m.n) // 2
-    sweep_bytes = 12 * F + 4 * (m.n + 1) + 28 * m.n
+    F = (nnz - n_rows) // 2
+    sweep_bytes = 12 * F + 4 * (n_rows + 1) + 28 * n_rows
     return dict(jacobi_ms_per_iteration=round(per_it[1], 4), dic_ms_per_iteration=round(per_it[2], 4),
                 sweep_ms=round(sweep, 4), sweep_bytes=sweep_bytes,
                 sweep_GBs=round(sweep_bytes / (sweep * 1e-3) / 1e9, 1) if sweep > 0 else None)
@@ -454,12 +453,20 @@ def workload_dic_sweep(args, torch, pkg):
     """The DIC apply alone on the wall-clustered system (capped solves, a
     short run for kernel profiles)."""
     n = args.n
-    s = pkg.meshgen.wall_pressure_system(n)
-    m = s.matrix
-    ds = pkg.krylov.DeviceSystem(m)
+    mg = pkg.meshgen
+    log(f"building the {n}^3 wall-clustered pressure matrix on the GPU")
+    N, nonz, acol, arow, adia, aval = mg.pressure_matrix_torch(mg.wall_pressure_mesh(n), device="cuda")
+    ds = pkg.krylov.DeviceSystem.from_device(N, nonz, acol, arow, adia, aval)
+    nnz = nonz - 1
+    b_host = mg.seeded_rhs(N, 5, True)
+    t0 = time.perf_counter()
+    form = ds.dic_sweep()          # builds the level analysis and the sweep plan
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    log(f"DIC form {form}, level analysis and plan {setup_ms:.0f} ms")
     out = {"metric": "DIC apply (two triangular sweeps) per CG iteration", "unit": "ms",
-           "workload": f"wall-clustered pressure {n}^3 (N={m.n}, nnz={m.nnz}), CG capped at {args.sweep_iters}",
-           "dic_levels": ds.levels(), **dic_sweep_cost(torch, ds, s, args.sweep_iters, args.steps)}
+           "workload": f"wall-clustered pressure {n}^3 (N={N}, nnz={nnz}), CG capped at {args.sweep_iters}",
+           "dic_levels": ds.levels(), "dic_form": form, "dic_plan_ms": round(setup_ms, 1),
+           **dic_sweep_cost(torch, ds, b_host, N, nnz, args.sweep_iters, args.steps)}
     ds.close()
     return out
 
@@ -721,6 +728,7 @@ def workload_wall(args, torch, pkg):
     ds.solve("cg", 2, x, r, 1, s.tol)
     torch.cuda.synchronize()
     out["dic_setup_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+    out["dic_form"] = ds.dic_sweep()
     for prec, cap in ((1, 40000), (2, args.dic_cap)):
         def f():
             x.zero_(); r.copy_(b)
@@ -743,7 +751,7 @@ def workload_wall(args, torch, pkg):
     out["mixed_cg_prec2"] = dict(ms=round(ms, 1), iters=it, converged=bool(fin < s.tol), replacements=rep)
     t = time_spmv(torch, ds, m.n, reps=20)
     out["spmv_fp64_GBs"] = round(spmv_bytes(m.n, m.nnz, 8) / t / 1e9, 1)
-    sw = dic_sweep_cost(torch, ds, s, args.sweep_iters)
+    sw = dic_sweep_cost(torch, ds, s.b, m.n, m.nnz, args.sweep_iters)
     out["dic_sweep_ms"] = sw["sweep_ms"]
     out["dic_sweep_GBs"] = sw["sweep_GBs"]
     ds.close()

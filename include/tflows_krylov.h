@@ -201,14 +201,20 @@ int tflows_resid(tflows_system s, const double* x, double* r, void* stream);
  * (0 before the first prec = 2 use); for reporting. */
 int32_t tflows_system_levels(tflows_system s);
 
-/* Form of the system's DIC sweeps (builds the level analysis): 2 = one
- * launch per triangle with rows waiting on their dependencies (the
- * default), 1 = one launch per level over the level-major copy, 0 = one
- * launch per level over the matrix rows; -1 on error.  All three give the
- * same bits.  A one-launch sweep whose waiting row gives up (it waited
- * beyond TFLOWS_DIC_WAIT_MS, e.g. while other work held the CUs) makes the
- * solve restore its inputs and run again in form 1, which the system then
- * keeps.  No counterpart in the reference. */
+/* Form of the system's DIC sweeps (builds the level analysis): 3 = one
+ * launch per triangle (and one for the pivots) in which each wavefront
+ * sweeps compact tiles of up to 8 levels with the tile's values in LDS,
+ * waiting only on values of other tiles (the default where the pattern's
+ * rows are sorted around the diagonal, as TopolM makes them, with at most 4
+ * entries per triangle); 2 = one launch per triangle with every row
+ * waiting on its dependencies (the default elsewhere); 1 = one launch per
+ * level over a level-major copy; 0 = one launch per level over the matrix
+ * rows; -1 on error.  TFLOWS_DIC_SWEEP picks a form when the levels are
+ * built.  All four give the same bits.  A one-launch sweep whose waiting
+ * row gives up (it waited beyond TFLOWS_DIC_WAIT_MS, e.g. while other work
+ * held the CUs) makes the solve restore its inputs and run again with one
+ * launch per level (form 1 after form 2, form 0 after form 3), which the
+ * system then keeps.  No counterpart in the reference. */
 int32_t tflows_system_dic_sweep(tflows_system s);
 
 /* SpMV form of the system's solves and products: lanes per matrix row

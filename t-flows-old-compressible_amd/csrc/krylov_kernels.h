@@ -137,6 +137,27 @@ void launch_plan_fwd(int a, int b, const int* lptr, const int* lcol, const T* lv
 template <typename T>
 void launch_plan_bwd(const int* pos, int count, const int* uptr, const int* ucol, const T* uval, const T* dp, T* zp,
                      const SolverState* st, hipStream_t s);
+// tiled one-launch DIC sweeps and pivots (Levels mode 3, tile_plan.h)
+struct TileSweep {
+    int n = 0, slots = 0, waves = 0;
+    const int* wstart = nullptr;  // waves + 1: each wavefront's tiles in wtiles
+    const int* wtiles = nullptr;
+    const int* tph = nullptr;     // phase starts, kTilePhases + 1 per tile
+    const int* order = nullptr;   // position -> row
+    int* fault = nullptr;
+    uint64_t wait_ticks = 0;
+};
+// wavefronts a tiled launch with `slots` entry slots may use (all resident at once)
+int tile_sweep_waves(int slots);
+template <typename T>
+void launch_tile_fwd(const TileSweep& ts, const T* u, const T* D, const int* lcol, const T* lval, T* dp, T* zf, T* zb,
+                     const SolverState* st, hipStream_t s);
+template <typename T>
+void launch_tile_bwd(const TileSweep& ts, T* z, const int* ucol, const T* uval, const T* dp, T* zf, T* zb,
+                     const SolverState* st, hipStream_t s);
+void launch_tile_pivots(const TileSweep& ts, const double* dg, const int* lcol, const double* lval, double* D,
+                        double* zs, hipStream_t s);
+// dst[e] = src[map[e]], 0 where map[e] < 0
 void launch_entry_gather(size_t count, const int* map, const double* src, double* dst, hipStream_t s);
 void launch_advance(int* kbase, int c, hipStream_t s);
 

@@ -29,11 +29,13 @@
 //   CGS    Process/CGS.f90:62-337           -> k_cgs_u, k_cgs_spmv_v, k_cgs_q, k_cgs_spmv_r
 //   DIC    Process/CG.f90:135-156           -> k_dic_fwd_level, k_dic_bwd_level
 #include <algorithm>
+#include <climits>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
 
 #include "krylov_kernels.h"
+#include "tile_plan.h"
 
 namespace tflows {
 
@@ -413,6 +415,307 @@ int flow_grid(int n) {
     }();
     return std::max(1, std::min(resident, (n + kBlock - 1) / kBlock));
 }
+// ---------------------------------------------------------------------------
+// Tiled one-launch sweeps (Levels mode 3; tiles as tile_plan.h builds them).
+// Each block is one wavefront; all blocks are resident.  Wavefront w sweeps
+// the tiles of its columns, listed in increasing index in wtiles[wstart[w]
+// .. wstart[w+1]) (backward: the same lists in reverse), so it climbs each
+// column level chunk by level chunk and its predecessor in the column is the
+// tile it has just swept.  For a tile it issues the loads of all its phases
+// at once (row, entry codes and values, u and D), reads the values it needs
+// from other tiles kTileAhead phases before the phase that uses them, and
+// runs the phases in order, one row per lane, with the tile's own values in
+// LDS; a value not yet published when read is waited for in the phase that
+// uses it, as the per-row sweeps wait.  Every wait ends: the unfinished tile
+// of smallest index (largest, backward) has all its inputs published, and
+// its wavefront has finished every tile of its list before it.  The
+// arithmetic of each row is that of k_dic_fwd_level / k_dic_bwd_level /
+// k_dic_setup_level, in entry order.
+//
+// The phases are straight-line code: a lane without a row in a phase loads
+// from a row of the tile that exists, computes a value nobody reads, and
+// stores it to a spare LDS slot and to a spare slot past the n positions of
+// the slot arrays (which hold n + kTileWave entries); only its store to the
+// caller's rows (z, D) is skipped.  Without branches around the loads the
+// compiler counts the outstanding loads exactly and waits for each value
+// alone.
+constexpr int kTileAhead = 2;
+
+// LDS stores of one phase are seen by the loads of the next: the wavefront's
+// LDS operations execute in program order; the fences keep the compiler from
+// moving them across the phase boundary.
+__device__ __forceinline__ void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The slow path of a tile's wait for another tile's value (out of line, so
+// the phases stay straight-line): re-reads the slot until it is published,
+// or, past wait_ticks or once another row gave up, raises the fault word
+// and returns a quiet NaN.
+template <typename T>
+__device__ __noinline__ typename Slot<T>::U tile_wait(const T* p, int* fault, uint64_t t0, uint64_t wait_ticks) {
+    return __builtin_bit_cast(typename Slot<T>::U, settle<T>(peek(p), p, fault, t0, wait_ticks));
+}
+
+// sum1 - sum_j a_j z_j over the E slots of one row: a code c >= 0 is the
+// value v read from another tile, c < 0 the tile's own value in LDS, a
+// padding slot (kTileNone, after the row's entries) leaves the sum as is
+template <typename T, int E>
+__device__ __forceinline__ T tile_row(T sum1, const int (&c)[E], const T (&a)[E], typename Slot<T>::U (&v)[E],
+                                      const T* zl, const T* zs, int* fault, uint64_t t0, uint64_t wait_ticks) {
+    bool empty = false;
+#pragma unroll
+    for (int j = 0; j < E; ++j) empty = empty || (c[j] >= 0 && v[j] == Slot<T>::kEmpty);
+    if (__builtin_expect(empty, 0)) {
+#pragma unroll
+        for (int j = 0; j < E; ++j)
+            if (c[j] >= 0 && v[j] == Slot<T>::kEmpty) v[j] = tile_wait<T>(zs + c[j], fault, t0, wait_ticks);
+    }
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+        const int cj = c[j];
+        const T own = zl[(cj < 0 && cj != kTileNone) ? -cj - 1 : 0];
+        const T zj = cj >= 0 ? __builtin_bit_cast(T, v[j]) : own;
+        const T s2 = sum1 - a[j] * zj;
+        sum1 = cj == kTileNone ? sum1 : s2;
+    }
+    return sum1;
+}
+
+// forward sweep (z_k = (u_k - sum_lower a z) * D_k) into zf, with dp = D in
+// position order and every zb slot emptied for the backward sweep
+template <typename T, int E>
+__global__ __launch_bounds__(kTileWave) void k_tile_fwd(int n, const int* __restrict__ wstart,
+                                                        const int* __restrict__ wtiles, const int* __restrict__ tph,
+                                                        const int* __restrict__ order, const T* __restrict__ u,
+                                                        const T* __restrict__ D, const int* __restrict__ lcol,
+                                                        const T* __restrict__ lval, T* __restrict__ dp, T* zf,
+                                                        T* __restrict__ zb, int* fault, uint64_t wait_ticks,
+                                                        const SolverState* st) {
+    if (st && st->done) return;
+    using U = typename Slot<T>::U;
+    __shared__ T zl[kTileMaxRows + kTileWave];
+    const int lane = threadIdx.x;
+    const uint64_t t0 = (uint64_t)wall_clock64();
+    const int i1 = wstart[blockIdx.x + 1];
+    for (int i = wstart[blockIdx.x]; i < i1; ++i) {
+        const int* ph = tph + (size_t)wtiles[i] * (kTilePhases + 1);
+        int bnd[kTilePhases + 1];
+#pragma unroll
+        for (int p = 0; p <= kTilePhases; ++p) bnd[p] = ph[p];
+        const int base = bnd[0];
+        bool ok[kTilePhases];
+        int k[kTilePhases], c[kTilePhases][E];
+        T a[kTilePhases][E], uu[kTilePhases], dd[kTilePhases];
+        U v[kTilePhases][E];
+#pragma unroll
+        for (int p = 0; p < kTilePhases; ++p) {
+            ok[p] = bnd[p] + lane < bnd[p + 1];
+            k[p] = ok[p] ? bnd[p] + lane : bnd[kTilePhases] - 1;
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                c[p][j] = lcol[(size_t)j * n + k[p]];
+                a[p][j] = lval[(size_t)j * n + k[p]];
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < kTilePhases; ++p) {
+            const int r = order[k[p]];
+            uu[p] = u[r];
+            dd[p] = D[r];
+        }
+#pragma unroll
+        for (int p = 0; p < kTileAhead; ++p)
+#pragma unroll
+            for (int j = 0; j < E; ++j) v[p][j] = peek(zf + (c[p][j] >= 0 ? c[p][j] : 0));
+#pragma unroll
+        for (int p = 0; p < kTilePhases; ++p) {
+            if (p + kTileAhead < kTilePhases) {
+#pragma unroll
+                for (int j = 0; j < E; ++j)
+                    v[p + kTileAhead][j] = peek(zf + (c[p + kTileAhead][j] >= 0 ? c[p + kTileAhead][j] : 0));
+            }
+            const T zk = tile_row<T, E>(uu[p], c[p], a[p], v[p], zl, zf, fault, t0, wait_ticks) * dd[p];
+            const int kp = ok[p] ? k[p] : n + lane;
+            zl[ok[p] ? kp - base : kTileMaxRows + lane] = zk;
+            dp[kp] = dd[p];
+            zb[kp] = __builtin_bit_cast(T, Slot<T>::kEmpty);
+            publish(zf + kp, zk);
+            wave_lds_order();
+        }
+    }
+}
+
+// backward sweep (z_k = (zf_k / (D_k + TINY) - sum_upper a z) * D_k) over
+// the tiles and phases in reverse, into zb and the caller's rows of z; each
+// row empties its zf slot for the next forward sweep
+template <typename T, int E>
+__global__ __launch_bounds__(kTileWave) void k_tile_bwd(int n, const int* __restrict__ wstart,
+                                                        const int* __restrict__ wtiles, const int* __restrict__ tph,
+                                                        const int* __restrict__ order, T* __restrict__ z,
+                                                        const int* __restrict__ ucol, const T* __restrict__ uval,
+                                                        const T* __restrict__ dp, T* __restrict__ zf, T* zb,
+                                                        int* fault, uint64_t wait_ticks, const SolverState* st) {
+    if (st && st->done) return;
+    using U = typename Slot<T>::U;
+    __shared__ T zl[kTileMaxRows + kTileWave];
+    const int lane = threadIdx.x;
+    const uint64_t t0 = (uint64_t)wall_clock64();
+    const int i0 = wstart[blockIdx.x];
+    for (int i = wstart[blockIdx.x + 1] - 1; i >= i0; --i) {
+        const int* ph = tph + (size_t)wtiles[i] * (kTilePhases + 1);
+        int bnd[kTilePhases + 1];
+#pragma unroll
+        for (int p = 0; p <= kTilePhases; ++p) bnd[p] = ph[p];
+        const int base = bnd[0];
+        bool ok[kTilePhases];
+        int k[kTilePhases], row[kTilePhases], c[kTilePhases][E];
+        T a[kTilePhases][E], zk[kTilePhases], dd[kTilePhases];
+        U v[kTilePhases][E];
+#pragma unroll
+        for (int p = 0; p < kTilePhases; ++p) {
+            ok[p] = bnd[p] + lane < bnd[p + 1];
+            k[p] = ok[p] ? bnd[p] + lane : base;
+            row[p] = order[k[p]];
+            zk[p] = zf[k[p]];
+            dd[p] = dp[k[p]];
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                c[p][j] = ucol[(size_t)j * n + k[p]];
+                a[p][j] = uval[(size_t)j * n + k[p]];
+            }
+        }
+#pragma unroll
+        for (int p = kTilePhases - 1; p >= kTilePhases - kTileAhead; --p)
+#pragma unroll
+            for (int j = 0; j < E; ++j) v[p][j] = peek(zb + (c[p][j] >= 0 ? c[p][j] : 0));
+#pragma unroll
+        for (int p = kTilePhases - 1; p >= 0; --p) {
+            if (p - kTileAhead >= 0) {
+#pragma unroll
+                for (int j = 0; j < E; ++j)
+                    v[p - kTileAhead][j] = peek(zb + (c[p - kTileAhead][j] >= 0 ? c[p - kTileAhead][j] : 0));
+            }
+            const int kp = ok[p] ? k[p] : n + lane;
+            zf[kp] = __builtin_bit_cast(T, Slot<T>::kEmpty);
+            const T di = dd[p];
+            const T vk = tile_row<T, E>(zk[p] / (di + T(kTiny)), c[p], a[p], v[p], zl, zb, fault, t0, wait_ticks) * di;
+            zl[ok[p] ? kp - base : kTileMaxRows + lane] = vk;
+            publish(zb + kp, vk);
+            if (ok[p]) z[row[p]] = vk;
+            wave_lds_order();
+        }
+    }
+}
+
+// DIC pivots (Prec1.f90:51-59): D_i = 1 / (a_ii - sum_lower (D_k a) a), from
+// the lower values and the diagonal in position order (lval, dg) into D and
+// the slots zs, every one of which is empty at the start
+template <int E>
+__global__ __launch_bounds__(kTileWave) void k_tile_pivots(int n, const int* __restrict__ wstart,
+                                                           const int* __restrict__ wtiles,
+                                                           const int* __restrict__ tph,
+                                                           const int* __restrict__ order,
+                                                           const double* __restrict__ dg,
+                                                           const int* __restrict__ lcol,
+                                                           const double* __restrict__ lval, double* __restrict__ D,
+                                                           double* zs, int* fault, uint64_t wait_ticks) {
+    using U = typename Slot<double>::U;
+    __shared__ double zl[kTileMaxRows + kTileWave];
+    const int lane = threadIdx.x;
+    const uint64_t t0 = (uint64_t)wall_clock64();
+    const int i1 = wstart[blockIdx.x + 1];
+    for (int i = wstart[blockIdx.x]; i < i1; ++i) {
+        const int* ph = tph + (size_t)wtiles[i] * (kTilePhases + 1);
+        int bnd[kTilePhases + 1];
+#pragma unroll
+        for (int p = 0; p <= kTilePhases; ++p) bnd[p] = ph[p];
+        const int base = bnd[0];
+        bool ok[kTilePhases];
+        int k[kTilePhases], row[kTilePhases], c[kTilePhases][E];
+        double a[kTilePhases][E], dd[kTilePhases];
+        U v[kTilePhases][E];
+#pragma unroll
+        for (int p = 0; p < kTilePhases; ++p) {
+            ok[p] = bnd[p] + lane < bnd[p + 1];
+            k[p] = ok[p] ? bnd[p] + lane : bnd[kTilePhases] - 1;
+            row[p] = order[k[p]];
+            dd[p] = dg[k[p]];
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                c[p][j] = lcol[(size_t)j * n + k[p]];
+                a[p][j] = lval[(size_t)j * n + k[p]];
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < kTileAhead; ++p)
+#pragma unroll
+            for (int j = 0; j < E; ++j) v[p][j] = peek(zs + (c[p][j] >= 0 ? c[p][j] : 0));
+#pragma unroll
+        for (int p = 0; p < kTilePhases; ++p) {
+            if (p + kTileAhead < kTilePhases) {
+#pragma unroll
+                for (int j = 0; j < E; ++j)
+                    v[p + kTileAhead][j] = peek(zs + (c[p + kTileAhead][j] >= 0 ? c[p + kTileAhead][j] : 0));
+            }
+            // sum1 - (D_k a) a: the row form with the coefficients D_k a and a
+            bool empty = false;
+#pragma unroll
+            for (int j = 0; j < E; ++j) empty = empty || (c[p][j] >= 0 && v[p][j] == Slot<double>::kEmpty);
+            if (__builtin_expect(empty, 0)) {
+#pragma unroll
+                for (int j = 0; j < E; ++j)
+                    if (c[p][j] >= 0 && v[p][j] == Slot<double>::kEmpty)
+                        v[p][j] = tile_wait<double>(zs + c[p][j], fault, t0, wait_ticks);
+            }
+            double sum1 = dd[p];
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const int cj = c[p][j];
+                const double own = zl[(cj < 0 && cj != kTileNone) ? -cj - 1 : 0];
+                const double dj = cj >= 0 ? __builtin_bit_cast(double, v[p][j]) : own;
+                const double s2 = sum1 - dj * a[p][j] * a[p][j];
+                sum1 = cj == kTileNone ? sum1 : s2;
+            }
+            const double dk = 1.0 / sum1;
+            const int kp = ok[p] ? k[p] : n + lane;
+            zl[ok[p] ? kp - base : kTileMaxRows + lane] = dk;
+            if (ok[p]) D[row[p]] = dk;
+            publish(zs + kp, dk);
+            wave_lds_order();
+        }
+    }
+}
+
+// Wavefronts of the tiled launches: a share of those that fit on the device
+// at once, for the kernel of the lowest occupancy among the tiled sweeps
+// and pivots (every tiled launch runs the one schedule built for it).
+static int tile_resident(const void* k) {
+    auto check = [](hipError_t e) {
+        if (e != hipSuccess) throw std::runtime_error(std::string("tiled sweep grid: ") + hipGetErrorString(e));
+    };
+    int dev = 0, cus = 0, per_cu = 0;
+    check(hipGetDevice(&dev));
+    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kTileWave, 0));
+    return std::max(1, cus * per_cu / kFlowShare);
+}
+template <int E>
+static int tile_waves_for() {
+    static const int r = [] {
+        const void* ks[] = {(const void*)k_tile_fwd<double, E>, (const void*)k_tile_bwd<double, E>,
+                            (const void*)k_tile_fwd<float, E>, (const void*)k_tile_bwd<float, E>,
+                            (const void*)k_tile_pivots<E>};
+        int m = INT_MAX;
+        for (const void* k : ks) m = std::min(m, tile_resident(k));
+        return m;
+    }();
+    return r;
+}
+int tile_sweep_waves(int slots) { return slots == 3 ? tile_waves_for<3>() : tile_waves_for<4>(); }
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_plan_fwd(int a, int b, const int* __restrict__ lptr,
                                                      const int* __restrict__ lcol, const T* __restrict__ lval,
@@ -443,7 +746,10 @@ __global__ __launch_bounds__(kBlock) void k_plan_bwd(const int* __restrict__ pos
 __global__ __launch_bounds__(kBlock) void k_entry_gather(int count, const int* __restrict__ map,
                                                          const double* __restrict__ src, double* __restrict__ dst) {
     const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e < count) dst[e] = src[map[e]];
+    if (e < count) {
+        const int m = map[e];
+        dst[e] = m < 0 ? 0.0 : src[m];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1146,6 +1452,38 @@ void launch_flow_bwd(int n, const int* order, T* z, const int* uptr, const int* 
             wait_ticks, st);
 }
 template <typename T>
+void launch_tile_fwd(const TileSweep& ts, const T* u, const T* D, const int* lcol, const T* lval, T* dp, T* zf, T* zb,
+                     const SolverState* st, hipStream_t s) {
+    if (ts.waves <= 0) return;
+    if (ts.slots == 3)
+        hipLaunchKernelGGL((k_tile_fwd<T, 3>), dim3(ts.waves), dim3(kTileWave), 0, s, ts.n, ts.wstart, ts.wtiles,
+                           ts.tph, ts.order, u, D, lcol, lval, dp, zf, zb, ts.fault, ts.wait_ticks, st);
+    else
+        hipLaunchKernelGGL((k_tile_fwd<T, 4>), dim3(ts.waves), dim3(kTileWave), 0, s, ts.n, ts.wstart, ts.wtiles,
+                           ts.tph, ts.order, u, D, lcol, lval, dp, zf, zb, ts.fault, ts.wait_ticks, st);
+}
+template <typename T>
+void launch_tile_bwd(const TileSweep& ts, T* z, const int* ucol, const T* uval, const T* dp, T* zf, T* zb,
+                     const SolverState* st, hipStream_t s) {
+    if (ts.waves <= 0) return;
+    if (ts.slots == 3)
+        hipLaunchKernelGGL((k_tile_bwd<T, 3>), dim3(ts.waves), dim3(kTileWave), 0, s, ts.n, ts.wstart, ts.wtiles,
+                           ts.tph, ts.order, z, ucol, uval, dp, zf, zb, ts.fault, ts.wait_ticks, st);
+    else
+        hipLaunchKernelGGL((k_tile_bwd<T, 4>), dim3(ts.waves), dim3(kTileWave), 0, s, ts.n, ts.wstart, ts.wtiles,
+                           ts.tph, ts.order, z, ucol, uval, dp, zf, zb, ts.fault, ts.wait_ticks, st);
+}
+void launch_tile_pivots(const TileSweep& ts, const double* dg, const int* lcol, const double* lval, double* D,
+                        double* zs, hipStream_t s) {
+    if (ts.waves <= 0) return;
+    if (ts.slots == 3)
+        hipLaunchKernelGGL((k_tile_pivots<3>), dim3(ts.waves), dim3(kTileWave), 0, s, ts.n, ts.wstart, ts.wtiles,
+                           ts.tph, ts.order, dg, lcol, lval, D, zs, ts.fault, ts.wait_ticks);
+    else
+        hipLaunchKernelGGL((k_tile_pivots<4>), dim3(ts.waves), dim3(kTileWave), 0, s, ts.n, ts.wstart, ts.wtiles,
+                           ts.tph, ts.order, dg, lcol, lval, D, zs, ts.fault, ts.wait_ticks);
+}
+template <typename T>
 void launch_plan_out(int n, const int* order, const T* zp, T* z, const SolverState* st, hipStream_t s) {
     if (n > 0) TF_LAUNCH((k_plan_out<T>), (n + kBlock - 1) / kBlock, s), n, order, zp, z, st);
 }
@@ -1336,7 +1674,11 @@ void launch_f64_to_f32(int64_t count, const double* in, float* out, hipStream_t 
     template void launch_flow_fwd<T>(int, const int*, const T*, const T*, const int*, const int*, const T*, T*,  \
                                      T*, T*, int*, uint64_t, const SolverState*, hipStream_t);                  \
     template void launch_flow_bwd<T>(int, const int*, T*, const int*, const int*, const T*, const T*, T*, T*,    \
-                                     int*, uint64_t, const SolverState*, hipStream_t);
+                                     int*, uint64_t, const SolverState*, hipStream_t);                         \
+    template void launch_tile_fwd<T>(const TileSweep&, const T*, const T*, const int*, const T*, T*, T*, T*,     \
+                                     const SolverState*, hipStream_t);                                         \
+    template void launch_tile_bwd<T>(const TileSweep&, T*, const int*, const T*, const T*, T*, T*,               \
+                                     const SolverState*, hipStream_t);
 TF_INSTANTIATE_PLAN(double)
 TF_INSTANTIATE_PLAN(float)
 

@@ -92,8 +92,23 @@ struct Levels {
     // A row that waits longer than wait_ticks gives up, raises the fault
     // word and leaves NaN; the host then reruns the solve in mode 1
     // (System::dic_retry).
+    //
+    // mode 3 (tiles, tile_plan.h) sweeps each triangle, and computes the
+    // pivots, in one launch of one-wavefront blocks that each walk compact
+    // tiles of up to kTilePhases levels with the tile's values in LDS; only
+    // dependencies between tiles wait on device-coherent slots.  Its
+    // positions are tile by tile, its entries in slot-major ELL form
+    // (lcol / lmap / ucol / umap hold tslots entries per position, coded as
+    // tile_plan.h describes).  A mode-3 sweep that gives up waiting sends
+    // the system to mode 0.
     bool plan = false;
-    int mode = 0;                        // 0 matrix rows by level lists, 1 / 2 as above
+    int mode = 0;                        // 0 matrix rows by level lists, 1 / 2 / 3 as above
+    int ntiles = 0, tslots = 0;          // mode 3: tiles and entry slots per position
+    int twaves = 0;                      // mode 3: wavefronts of a tiled launch
+    DevArray<int> tph;                   // mode 3: phase starts of every tile
+    DevArray<int> wstart, wtiles;        // mode 3: each wavefront's tiles (tile_schedule)
+    DevArray<int> dmap;                  // mode 3: the diagonal entry of every position
+    mutable DevArray<double> dgp;        // mode 3: the diagonal in position order (pivots)
     int npos = 0;                        // positions (rows and padding)
     std::vector<int> pos_ptr;            // host: position range of each forward level
     size_t nlow = 0, nup = 0;            // lower / upper entries
@@ -312,6 +327,8 @@ class System {
     KrylovVectors<double> vectors(double* x, double* r1);
     SolverState* state() { return state_.p; }
     HostMirror* mirror() { return mirror_; }
+    // D = the DIC pivots of the current values (Prec1.f90:51-59), in the
+    // internal numbering: one tiled launch in mode 3, else one per level
     void dic_setup(double* D, hipStream_t s);
     // values of the level-major triangles for the next DIC sweeps (FP64,
     // and FP32 rounded from them when fp32)
@@ -319,19 +336,22 @@ class System {
     // throws DicWaitFault when a one-launch sweep of the last solve gave up
     // waiting
     void check_dic(hipStream_t s) const;
-    // the DIC sweep form this system runs (0, 1 or 2; builds the levels)
+    // the DIC sweep form this system runs (0 .. 3; builds the levels)
     int dic_form() { return levels().plan ? levels_.mode : 0; }
-    // From now on sweep one level per launch (form 1), which needs no
-    // co-resident grid; the one-launch form stays off for this system.
+    // From now on sweep one level per launch, which needs no co-resident
+    // grid: form 1 after form 2, form 0 (the matrix rows) after form 3,
+    // whose positions are not level-major; the one-launch form stays off
+    // for this system.
     void latch_dic_form1();
     // Runs a DIC solve `f`, whose in/out vectors are the `count` device
     // arrays `vecs` of n rows each.  When a one-launch sweep in it gave up
     // waiting (another process or stream held the CUs past the wait bound),
-    // the vectors are restored, the system latches form 1 and f runs again;
+    // the vectors are restored, the system latches a form with one launch
+    // per level and f runs again;
     // a second failure propagates.
     template <class F>
     SolveResult dic_retry(double* const* vecs, int count, hipStream_t s, F&& f) {
-        if (dic_form() != 2) return f();
+        if (dic_form() < 2) return f();
         dic_backup_.ensure((size_t)count * n_);
         for (int k = 0; k < count; ++k)
             TF_CHECK(hipMemcpyAsync(dic_backup_.p + (size_t)k * n_, vecs[k], sizeof(double) * n_,
@@ -365,7 +385,7 @@ class System {
     SolveResult solve_mixed_reliable_impl(int prec, double* x, const double* b, double* r_out, int cap, double tol,
                                           int interval, int* outer_steps, hipStream_t s);
     void build_storage(bool allow_reorder, hipStream_t s);
-    void build_dic_plan();
+    void build_dic_plan(const std::vector<int>& flev, bool sorted);
     void permute_values(hipStream_t s);
     double* ext_buffer(int k);   // k-th N-vector of the gather / scatter buffer
 
@@ -445,6 +465,7 @@ template <typename T, typename Op>
 void enqueue_iteration(const BatchCtx& c, int method, bool jac, int k, const Op& Aop, const Csr<T>& A,
                        const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm,
                        const Levels* L, hipStream_t s);
+void dic_pivots_levels(const Levels& L, const Csr<double>& A, double* D, hipStream_t s);
 template <typename T>
 void dic_apply_levels(const Levels& L, const Csr<T>& A, const T* D, const T* u, T* z, const SolverState* st,
                       hipStream_t s);

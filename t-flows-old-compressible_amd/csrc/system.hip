@@ -2,6 +2,7 @@
 // matrix: pattern upload, preconditioner set-up, level analysis for the
 // DIC sweeps, and the host loop that keeps the stream full of iterations.
 #include "system.h"
+#include "tile_plan.h"
 
 #include <algorithm>
 #include <chrono>
@@ -9,6 +10,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <thread>
+#include <type_traits>
 
 namespace tflows {
 
@@ -439,17 +441,27 @@ const Levels& System::levels() {
         TF_CHECK(hipMemcpy(rows.p, out.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
         return out;
     };
-    levels_.nfwd = dependency_levels(n_, h_rowptr_, h_col_, h_dia_, true, lev);
+    // a pattern whose lower columns lie below each row and upper ones above
+    // (every TopolM pattern in the caller's numbering) has its levels in
+    // one pass in index order; others need the topological walk
+    int nf = sorted_pattern_levels(n_, h_rowptr_.data(), h_col_.data(), h_dia_.data(), true, lev);
+    const bool sorted = nf >= 0;
+    if (!sorted) nf = dependency_levels(n_, h_rowptr_, h_col_, h_dia_, true, lev);
+    levels_.nfwd = nf;
     levels_.fwd_rows_h = bucket(levels_.nfwd, levels_.fwd_ptr, levels_.fwd_rows);
-    levels_.nbwd = dependency_levels(n_, h_rowptr_, h_col_, h_dia_, false, lev);
+    std::vector<int> flev;
+    flev.swap(lev);
+    int nb = sorted ? sorted_pattern_levels(n_, h_rowptr_.data(), h_col_.data(), h_dia_.data(), false, lev) : -1;
+    if (nb < 0) nb = dependency_levels(n_, h_rowptr_, h_col_, h_dia_, false, lev);
+    levels_.nbwd = nb;
     levels_.bwd_rows_h = bucket(levels_.nbwd, levels_.bwd_ptr, levels_.bwd_rows);
-    build_dic_plan();
+    build_dic_plan(flev, sorted);
     levels_ready_ = true;
     return levels_;
 }
 
 // The level-major triangles (Levels::plan) from the stored pattern.
-void System::build_dic_plan() {
+void System::build_dic_plan(const std::vector<int>& flev, bool sorted) {
     Levels& L = levels_;
     L.mode = dic_sweep_mode();
     if (L.mode == 0) return;
@@ -457,6 +469,45 @@ void System::build_dic_plan() {
     TF_CHECK(hipMemcpy(rp.data(), rowptr_.p, sizeof(int) * (n_ + 1), hipMemcpyDeviceToHost));
     TF_CHECK(hipMemcpy(cl.data(), col_.p, sizeof(int) * nnz_, hipMemcpyDeviceToHost));
     TF_CHECK(hipMemcpy(dg.data(), dia_.p, sizeof(int) * n_, hipMemcpyDeviceToHost));
+    auto up = [](DevArray<int>& d, const std::vector<int>& h) {
+        d.ensure(h.size());
+        if (!h.empty()) TF_CHECK(hipMemcpy(d.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
+    };
+    auto arm_waits = [&] {
+        L.fault.ensure(1);
+        TF_CHECK(hipMemset(L.fault.p, 0, sizeof(int)));
+        int dev = 0, khz = 0;
+        TF_CHECK(hipGetDevice(&dev));
+        TF_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        L.wait_ticks = (uint64_t)(khz > 0 ? khz : 100000) * dic_wait_ms();
+    };
+    if (L.mode == 3) {
+        TilePlan tp;
+        if (sorted && build_tile_plan(n_, h_rowptr_.data(), h_col_.data(), h_dia_.data(), flev.data(),
+                                      reordered_ ? iperm_h_.data() : nullptr, rp.data(), cl.data(), tp)) {
+            std::vector<int> ws, wt;
+            L.twaves = std::min(tile_sweep_waves(tp.slots), tp.ncols);
+            tile_schedule(tp, L.twaves, ws, wt);
+            up(L.wstart, ws);
+            up(L.wtiles, wt);
+            up(L.dmap, tp.dmap);
+            up(L.order, tp.order);
+            up(L.tph, tp.tph);
+            up(L.lcol, tp.lcol);
+            up(L.lmap, tp.lmap);
+            up(L.ucol, tp.ucol);
+            up(L.umap, tp.umap);
+            L.npos = n_;
+            L.ntiles = tp.ntiles;
+            L.tslots = tp.slots;
+            L.nlow = L.nup = (size_t)tp.slots * n_;
+            arm_waits();
+            L.plan = true;
+            L.vals64 = L.vals32 = false;
+            return;
+        }
+        L.mode = 2;   // the per-row one-launch sweeps over the level-major copy
+    }
     // positions: forward level l occupies [pos_ptr[l], pos_ptr[l+1]), its
     // start a multiple of the wavefront size; the padding positions (row
     // -1) have no entries
@@ -505,10 +556,6 @@ void System::build_dic_plan() {
         }
     }
     for (int t = 0; t < n_; ++t) bpos[t] = pos[L.bwd_rows_h[t]];
-    auto up = [](DevArray<int>& d, const std::vector<int>& h) {
-        d.ensure(h.size());
-        if (!h.empty()) TF_CHECK(hipMemcpy(d.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
-    };
     up(L.order, order);
     up(L.lptr, lptr);
     up(L.lcol, lcol);
@@ -521,23 +568,16 @@ void System::build_dic_plan() {
     L.nlow = lptr[np];
     L.nup = uptr[np];
     if (L.mode == 2 && !upper_later) L.mode = 1;
-    if (L.mode == 2) {
-        L.fault.ensure(1);
-        TF_CHECK(hipMemset(L.fault.p, 0, sizeof(int)));
-        int dev = 0, khz = 0;
-        TF_CHECK(hipGetDevice(&dev));
-        TF_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-        L.wait_ticks = (uint64_t)(khz > 0 ? khz : 100000) * dic_wait_ms();
-    }
+    if (L.mode == 2) arm_waits();
     L.plan = true;
     L.vals64 = L.vals32 = false;
 }
 
 int dic_sweep_mode() {
     const char* e = std::getenv("TFLOWS_DIC_SWEEP");
-    if (!e) return 2;
+    if (!e) return 3;
     const int m = std::atoi(e);
-    return m < 0 ? 0 : (m > 2 ? 2 : m);
+    return m < 0 ? 0 : (m > 3 ? 3 : m);
 }
 
 uint64_t dic_wait_ms() {
@@ -547,7 +587,7 @@ uint64_t dic_wait_ms() {
 }
 
 void System::check_dic(hipStream_t s) const {
-    if (!levels_ready_ || levels_.mode != 2 || !levels_.plan) return;
+    if (!levels_ready_ || levels_.mode < 2 || !levels_.plan) return;
     int fault = 0;
     TF_CHECK(hipMemcpyAsync(&fault, levels_.fault.p, sizeof(int), hipMemcpyDeviceToHost, s));
     TF_CHECK(hipStreamSynchronize(s));
@@ -558,26 +598,31 @@ void System::check_dic(hipStream_t s) const {
 
 void System::latch_dic_form1() {
     levels();
-    if (!levels_.plan || levels_.mode != 2) return;
-    levels_.mode = 1;
+    if (!levels_.plan || levels_.mode < 2) return;
+    levels_.mode = levels_.mode == 3 ? 0 : 1;
     std::fprintf(stderr,
                  "libtflows_krylov: a one-launch DIC sweep gave up waiting (TFLOWS_DIC_WAIT_MS); this system "
                  "now sweeps one level per launch\n");
     std::fflush(stderr);
 }
 
-void System::prepare_dic(bool fp32, hipStream_t s) {
-    const Levels& L = levels();
+// The values of the level-major (or tiled) triangles, FP64 and, when fp32,
+// FP32 rounded from them, whenever the matrix has changed since the last
+// gather; the work vectors of the one-launch sweeps.
+void gather_dic_values(const Levels& L, const double* val, bool fp32, hipStream_t s) {
     if (!L.plan) return;
+    // the slot arrays hold kTileWave spare slots past the positions (the
+    // tiled sweeps' lanes without a row store there)
+    const size_t slots = (size_t)L.npos + kTileWave;
     if (!L.vals64) {
         L.lval.ensure(L.nlow);
         L.uval.ensure(L.nup);
         L.up.ensure(L.npos);
-        L.zp.ensure(L.npos);
-        L.zb.ensure(L.npos);
-        L.dp.ensure(L.npos);
-        launch_entry_gather(L.nlow, L.lmap.p, val_.p, L.lval.p, s);
-        launch_entry_gather(L.nup, L.umap.p, val_.p, L.uval.p, s);
+        L.zp.ensure(slots);
+        L.zb.ensure(slots);
+        L.dp.ensure(slots);
+        launch_entry_gather(L.nlow, L.lmap.p, val, L.lval.p, s);
+        launch_entry_gather(L.nup, L.umap.p, val, L.uval.p, s);
         L.vals64 = true;
         L.vals32 = false;
     }
@@ -585,32 +630,73 @@ void System::prepare_dic(bool fp32, hipStream_t s) {
         L.lval32.ensure(L.nlow);
         L.uval32.ensure(L.nup);
         L.up32.ensure(L.npos);
-        L.zp32.ensure(L.npos);
-        L.zb32.ensure(L.npos);
-        L.dp32.ensure(L.npos);
+        L.zp32.ensure(slots);
+        L.zb32.ensure(slots);
+        L.dp32.ensure(slots);
         launch_f64_to_f32(L.nlow, L.lval.p, L.lval32.p, s);
         launch_f64_to_f32(L.nup, L.uval.p, L.uval32.p, s);
         L.vals32 = true;
     }
-    if (L.mode == 2) {   // a solve starts: no fault, every forward slot empty
-        launch_set_int(L.fault.p, 0, s);
+}
+
+void System::prepare_dic(bool fp32, hipStream_t s) {
+    const Levels& L = levels();
+    if (!L.plan) return;
+    gather_dic_values(L, val_.p, fp32, s);
+    if (L.mode >= 2) {   // a solve starts: every forward slot empty
         launch_flow_reset(L.npos, L.zp.p, s);
         if (fp32) launch_flow_reset(L.npos, L.zp32.p, s);
     }
 }
 
-void System::dic_setup(double* D, hipStream_t s) {
-    const Levels& L = levels();
-    Csr<double> A = csr();
+// The DIC pivots of A into D (internal numbering): the tiled one-launch
+// form in mode 3 (over the zb slots, emptied first), else one launch per
+// forward level.
+static TileSweep tile_sweep(const Levels& L) {
+    TileSweep ts;
+    ts.n = L.npos;
+    ts.slots = L.tslots;
+    ts.waves = L.twaves;
+    ts.wstart = L.wstart.p;
+    ts.wtiles = L.wtiles.p;
+    ts.tph = L.tph.p;
+    ts.order = L.order.p;
+    ts.fault = L.fault.p;
+    ts.wait_ticks = L.wait_ticks;
+    return ts;
+}
+
+void dic_pivots_levels(const Levels& L, const Csr<double>& A, double* D, hipStream_t s) {
+    if (L.plan && L.mode == 3) {
+        gather_dic_values(L, A.val, false, s);
+        L.dgp.ensure(L.npos);
+        launch_entry_gather(L.npos, L.dmap.p, A.val, L.dgp.p, s);
+        launch_flow_reset(L.npos, L.zb.p, s);
+        launch_tile_pivots(tile_sweep(L), L.dgp.p, L.lcol.p, L.lval.p, D, L.zb.p, s);
+        return;
+    }
     for (int l = 0; l < L.nfwd; ++l)
         launch_dic_setup_level(A, L.fwd_rows.p + L.fwd_ptr[l], L.fwd_ptr[l + 1] - L.fwd_ptr[l], D, s);
 }
+
+void System::dic_setup(double* D, hipStream_t s) { dic_pivots_levels(levels(), csr(), D, s); }
 
 template <typename T>
 void dic_apply_levels(const Levels& L, const Csr<T>& A, const T* D, const T* u, T* z, const SolverState* st,
                       hipStream_t s) {
     const bool f32 = sizeof(T) == sizeof(float);
-    if (L.plan && (f32 ? L.vals32 : L.vals64)) {
+    if (L.plan && L.mode == 3 && (f32 ? L.vals32 : L.vals64)) {
+        T* zp = f32 ? (T*)L.zp32.p : (T*)L.zp.p;
+        T* zb = f32 ? (T*)L.zb32.p : (T*)L.zb.p;
+        T* dp = f32 ? (T*)L.dp32.p : (T*)L.dp.p;
+        const T* lv = f32 ? (const T*)L.lval32.p : (const T*)L.lval.p;
+        const T* uv = f32 ? (const T*)L.uval32.p : (const T*)L.uval.p;
+        const TileSweep ts = tile_sweep(L);
+        launch_tile_fwd(ts, u, D, L.lcol.p, lv, dp, zp, zb, st, s);
+        launch_tile_bwd(ts, z, L.ucol.p, uv, (const T*)dp, zp, zb, st, s);
+        return;
+    }
+    if (L.plan && L.mode != 0 && L.mode != 3 && (f32 ? L.vals32 : L.vals64)) {
         T* up = f32 ? (T*)L.up32.p : (T*)L.up.p;
         T* zp = f32 ? (T*)L.zp32.p : (T*)L.zp.p;
         T* zb = f32 ? (T*)L.zb32.p : (T*)L.zb.p;
@@ -651,7 +737,17 @@ void System::prec1(int prec, double* d_ext, hipStream_t s) {
     ws_.ensure((size_t)V_COUNT * n_);
     TF_CHECK(hipMemsetAsync(ws_.p, 0, sizeof(double) * n_, s));
     double* d = reordered_ ? ext_buffer(0) : d_ext;
-    if (prec == 2) dic_setup(d, s);
+    if (prec == 2) {
+        dic_setup(d, s);
+        if (dic_form() >= 2) {
+            try {
+                check_dic(s);
+            } catch (const DicWaitFault&) {
+                latch_dic_form1();
+                dic_setup(d, s);
+            }
+        }
+    }
     launch_setup(ctx(), csr(), prec, prec != 2, d, (const double*)ws_.p, sites(), s);
     TF_CHECK(hipGetLastError());
     if (reordered_) to_external(d, d_ext, s);
@@ -931,9 +1027,14 @@ void start_solve(const BatchCtx& c, int method, int prec, const Op& Aop, const C
                  hipStream_t s, bool d_ready, bool zero_x) {
     (void)Aop;
     const bool jac = prec != 2;
-    if (prec == 2 && !d_ready)
-        for (int l = 0; l < L->nfwd; ++l)
-            launch_dic_setup_level(A, L->fwd_rows.p + L->fwd_ptr[l], L->fwd_ptr[l + 1] - L->fwd_ptr[l], w.D, s);
+    if (prec == 2 && !d_ready) {
+        if constexpr (std::is_same_v<T, double>)
+            dic_pivots_levels(*L, A, w.D, s);
+        else
+            for (int l = 0; l < L->nfwd; ++l)
+                launch_dic_setup_level(A, L->fwd_rows.p + L->fwd_ptr[l], L->fwd_ptr[l + 1] - L->fwd_ptr[l], w.D,
+                                       s);
+    }
     launch_setup(c, A, prec, jac && !d_ready, w.D, (const T*)w.r1, S, s);
     int mode = RHO_NONE_;
     T* r2 = nullptr;

@@ -428,8 +428,9 @@ class DeviceSystem:
         return int(self.lib.tflows_system_levels(self.handle))
 
     def dic_sweep(self) -> int:
-        """Form of the DIC sweeps (2 one launch per triangle, 1 one launch
-        per level over the level-major copy, 0 over the matrix rows)."""
+        """Form of the DIC sweeps (3 one launch per triangle over compact
+        tiles, 2 one launch per triangle row by row, 1 one launch per level
+        over the level-major copy, 0 over the matrix rows)."""
         v = int(self.lib.tflows_system_dic_sweep(self.handle))
         if v < 0:
             _check(v)

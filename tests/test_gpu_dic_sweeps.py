@@ -1,14 +1,19 @@
-"""GPU tests of the three forms of the DIC sweeps.
+"""GPU tests of the four forms of the DIC sweeps.
 
 The DIC apply (Process/CG.f90:135-156) runs over the matrix rows level by
 level (TFLOWS_DIC_SWEEP=0), over a level-major copy of the triangles level
 by level (1: rows of one forward level at consecutive positions, entries in
-matrix order), or over that copy in one launch per triangle with each row
-waiting for its dependencies (2, the default).  Every z_i is the same sum of
-the same products in all three, so solves must agree bit for bit: single
-systems in natural and locality order, the three methods, FP32 inner solves,
+matrix order), over that copy in one launch per triangle with each row
+waiting for its dependencies (2), or in one launch per triangle over compact
+tiles of up to 8 levels, each swept by one wavefront with its values in LDS
+(3, the default where the pattern allows it; the pivots of Prec1.f90:51-59
+then take one launch too).  Every z_i and every pivot is the same sum of the
+same products in all four, so solves must agree bit for bit: single systems
+in natural and locality order, the three methods, FP32 inner solves,
 batches, and after a value update.  The form is read when a system first
-builds its levels.
+builds its levels; a pattern that does not allow tiles (a randomly
+renumbered one has rows with more than 4 entries per triangle) runs form 2
+when 3 is asked for.
 """
 import numpy as np
 import pytest
@@ -16,7 +21,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-FORMS = (0, 1, 2)
+FORMS = (0, 1, 2, 3)
 
 
 def _forms(monkeypatch, make):
@@ -65,7 +70,7 @@ def test_sweep_forms_solves_bitwise(pkg, cuda, monkeypatch, case, method):
             assert a[:3] == b[:3], f
             assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4]), f
     if case != "momentum" or method == "cgs":
-        assert runs[2][1][2] < s.tol
+        assert runs[3][1][2] < s.tol
     for o in objs:
         o.close()
 
@@ -97,7 +102,7 @@ def test_sweep_forms_after_value_update(pkg, cuda, monkeypatch, method):
     for f in FORMS[1:]:
         for a, b in zip(runs[0], runs[f]):
             assert a[:2] == b[:2] and np.array_equal(a[2], b[2]), f
-    assert not np.array_equal(runs[2][0][2], runs[2][1][2])
+    assert not np.array_equal(runs[3][0][2], runs[3][1][2])
     for o in objs:
         o.close()
 
@@ -124,7 +129,7 @@ def test_sweep_forms_mixed_bitwise(pkg, cuda, monkeypatch, method):
     for f in FORMS[1:]:
         assert runs[0][0] == runs[f][0], f
         assert np.array_equal(runs[0][1], runs[f][1]), f
-    assert runs[2][0][2] < s.tol
+    assert runs[3][0][2] < s.tol
     for o in objs:
         o.close()
 
@@ -160,25 +165,28 @@ def test_sweep_forms_batch_bitwise(pkg, cuda, monkeypatch, mixed):
             assert np.array_equal(runs[0][0][key], runs[f][0][key]), (f, key)
         assert np.array_equal(runs[0][1], runs[f][1]), f
         assert np.array_equal(runs[0][2], runs[f][2]), f
-    assert np.all(runs[2][0]["fin_res"] < np.array([t.tol for t in systems]))
+    assert np.all(runs[3][0]["fin_res"] < np.array([t.tol for t in systems]))
     for o in objs:
         o.close()
 
 
-def test_sweep_wait_bound(pkg, cuda, monkeypatch):
+@pytest.mark.parametrize("form, latched", [(2, 1), (3, 0)])
+def test_sweep_wait_bound(pkg, cuda, monkeypatch, form, latched):
     """Failure handling of the one-launch sweeps, forced from the host: with
     TFLOWS_DIC_WAIT_MS=0 (read when the levels are built) a row that has to
     wait gives up at its first time check, raises the fault word and leaves
     NaN; the other waiting rows see the fault and give up too, so every
     wavefront ends.  The solve then restores x and r1, latches the system to
-    one launch per level (form 1, which needs no co-resident grid) and runs
-    again, so it returns the unbounded result bit for bit -- as it does when
-    no row had to wait.  Later solves on that system stay in form 1 and keep
-    the same bits.  A system built afterwards with the default bound solves
-    normally in form 2."""
+    one launch per level (form 1 after form 2, form 0 after form 3; neither
+    needs a co-resident grid) and runs again, so it returns the unbounded
+    result bit for bit -- as it does when no row had to wait.  Later solves
+    on that system stay in the latched form and keep the same bits.  A
+    system built afterwards with the default bound solves normally in the
+    one-launch form."""
     import torch
     s = pkg.meshgen.pressure_system(48, seed=66, prec=2)
     m = s.matrix
+    monkeypatch.setenv("TFLOWS_DIC_SWEEP", str(form))
 
     def run(ds, cap):
         x = torch.zeros(m.n, dtype=torch.float64, device=cuda)
@@ -189,22 +197,35 @@ def test_sweep_wait_bound(pkg, cuda, monkeypatch):
 
     ref = pkg.krylov.DeviceSystem(m)
     want = run(ref, 20)
-    assert ref.dic_sweep() == 2
+    assert ref.dic_sweep() == form
     monkeypatch.setenv("TFLOWS_DIC_WAIT_MS", "0")
     ds = pkg.krylov.DeviceSystem(m)
     got = run(ds, 20)
-    form = ds.dic_sweep()
-    print(f"wait bound 0: sweep form after the solve = {form}", flush=True)
-    assert form in (1, 2)
+    now = ds.dic_sweep()
+    print(f"wait bound 0: sweep form after the solve = {now}", flush=True)
+    assert now in (latched, form)
     assert got[0] == want[0] and np.array_equal(got[1], want[1]) and np.array_equal(got[2], want[2])
     again_same = run(ds, 20)
-    assert ds.dic_sweep() == form
+    assert ds.dic_sweep() == now
     assert again_same[0] == want[0] and np.array_equal(again_same[1], want[1])
     ds.close()
     monkeypatch.delenv("TFLOWS_DIC_WAIT_MS")
     again = pkg.krylov.DeviceSystem(m)
     res, x, _ = run(again, 20)
-    assert again.dic_sweep() == 2
+    assert again.dic_sweep() == form
     assert res == want[0] and np.array_equal(x, want[1])
     again.close()
     ref.close()
+
+
+def test_tiles_taken_where_the_pattern_allows(pkg, cuda, monkeypatch):
+    """The default form: tiles on naturally numbered boxes (locality order
+    inside the library or not); row-by-row waits on a randomly renumbered
+    pattern, whose rows have up to 6 entries per triangle."""
+    monkeypatch.delenv("TFLOWS_DIC_SWEEP", raising=False)
+    mg = pkg.meshgen
+    for sysm, want in ((mg.pressure_system(20, seed=1, prec=2), 3),
+                       (mg.pressure_system(12, seed=62, perm_seed=7, prec=2), 2)):
+        ds = pkg.krylov.DeviceSystem(sysm.matrix)
+        assert ds.dic_sweep() == want, want
+        ds.close()

@@ -16,7 +16,7 @@ oracle; these run them at the sizes they are specified at:
     central differences and an FP64 CPU gradient;
   * 256^3 wall-clustered pressure (16.8M rows, 766 DIC levels, int32
     columns since a plane exceeds 32,767 rows): DIC-CG in FP64 and with
-    FP32 reliable updates, true residuals.
+    FP32 reliable updates (tiled sweeps), true residuals.
 
 Error bound for the nonsymmetric momentum / transport systems: with
 divergence-free upwind fluxes every row is strictly diagonally dominant
@@ -171,7 +171,7 @@ def test_wall_256_dic(pkg, cuda):
     b = torch.from_numpy(b_h).to(cuda)
     bn = float(np.linalg.norm(b_h))
     tol = 1e-8 * bn
-    assert ds.levels() == 766 and ds.dic_sweep() == 2
+    assert ds.levels() == 766 and ds.dic_sweep() == 3
 
     def true_rel(x):
         r = b.clone()
