@@ -3,7 +3,9 @@
 #include "tile_plan.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <thread>
 #include <utility>
 
 namespace tflows {
@@ -40,6 +42,24 @@ int sorted_pattern_levels(int n, const int* rowptr, const int* col, const int* d
 
 namespace {
 
+// f(lo, hi) over [0, n) in contiguous chunks on up to 16 host threads
+template <class F>
+void parallel_for(int n, F&& f) {
+    const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int nt = n < (1 << 16) ? 1 : hw;
+    if (nt == 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int chunk = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+        const int lo = t * chunk, hi = std::min(n, lo + chunk);
+        if (lo < hi) th.emplace_back([&f, lo, hi] { f(lo, hi); });
+    }
+    for (auto& x : th) x.join();
+}
+
 // median of v (v is reordered); 0 when empty
 int64_t median(std::vector<int64_t>& v) {
     if (v.empty()) return 0;
@@ -73,17 +93,17 @@ void radix_sort(std::vector<int>& idx, const std::vector<uint64_t>& key, uint64_
 bool attempt(int n, const int* rowptr, const int* dia, const int* flev, const int* iperm, const int* rp,
              const int* cl, int64_t R1, int64_t S, int64_t R2, TilePlan& out) {
     const int W = kTilePhases;
-    int maxlev = 0;
-    for (int r = 0; r < n; ++r) maxlev = std::max(maxlev, flev[r]);
     const uint64_t n2 = (uint64_t)((n - 1) / R1 + 1), n3 = (uint64_t)((S - 1) / R2 + 1);
     std::vector<uint64_t> key(n);
+    parallel_for(n, [&](int lo, int hi) {
+        for (int r = lo; r < hi; ++r) {
+            const uint64_t c1 = (uint64_t)(flev[r] / W), c2 = (uint64_t)(r / R1), c3 = (uint64_t)((r % S) / R2);
+            const uint64_t ph = (uint64_t)(flev[r] % W);
+            key[r] = ((c1 * n2 + c2) * n3 + c3) * W + ph;
+        }
+    });
     uint64_t kmax = 0;
-    for (int r = 0; r < n; ++r) {
-        const uint64_t c1 = (uint64_t)(flev[r] / W), c2 = (uint64_t)(r / R1), c3 = (uint64_t)((r % S) / R2);
-        const uint64_t ph = (uint64_t)(flev[r] % W);
-        key[r] = ((c1 * n2 + c2) * n3 + c3) * W + ph;
-        kmax = std::max(kmax, key[r]);
-    }
+    for (int r = 0; r < n; ++r) kmax = std::max(kmax, key[r]);
     std::vector<int> rows(n);
     for (int r = 0; r < n; ++r) rows[r] = r;
     radix_sort(rows, key, kmax);
@@ -165,53 +185,65 @@ bool attempt(int n, const int* rowptr, const int* dia, const int* flev, const in
     out.lmap.assign((size_t)E * n, -1);
     out.ucol.assign((size_t)E * n, kTileNone);
     out.umap.assign((size_t)E * n, -1);
-    for (int k = 0; k < n; ++k) {
-        const int r = rows[k];
-        const int i = out.order[k];
-        const int ts = tstart[ktile[k]], te = tstart[ktile[k] + 1];
-        const int nl = dia[r] - rowptr[r], nu = rowptr[r + 1] - dia[r] - 1;
-        const int j0 = rp[i];               // the internal row keeps the caller's entry order
-        out.dmap[k] = j0 + nl;
-        for (int s = 0; s < nl; ++s) {
-            const int j = j0 + s;
-            const int c = pos[cl[j]];
-            int code;
-            if (c >= ts && c < te) {
-                if (c >= kph_lo[k]) {
-                    out.why = "a lower dependency inside the tile but not in an earlier phase";
-                    return false;
+    // codes of every entry; a dependency out of order fails the attempt
+    static const char* const kWhy[] = {nullptr, "a lower dependency inside the tile but not in an earlier phase",
+                                       "a lower dependency in a later tile",
+                                       "an upper dependency inside the tile but not in a later phase",
+                                       "an upper dependency in an earlier tile"};
+    std::atomic<int> fail{0};
+    parallel_for(n, [&](int lo, int hi) {
+        for (int k = lo; k < hi && fail.load(std::memory_order_relaxed) == 0; ++k) {
+            const int r = rows[k];
+            const int i = out.order[k];
+            const int ts = tstart[ktile[k]], te = tstart[ktile[k] + 1];
+            const int nl = dia[r] - rowptr[r], nu = rowptr[r + 1] - dia[r] - 1;
+            const int j0 = rp[i];               // the internal row keeps the caller's entry order
+            out.dmap[k] = j0 + nl;
+            for (int s = 0; s < nl; ++s) {
+                const int j = j0 + s;
+                const int c = pos[cl[j]];
+                int code;
+                if (c >= ts && c < te) {
+                    if (c >= kph_lo[k]) {
+                        fail = 1;
+                        return;
+                    }
+                    code = -(c - ts) - 1;
+                } else {
+                    if (c >= ts) {
+                        fail = 2;
+                        return;
+                    }
+                    code = c;
                 }
-                code = -(c - ts) - 1;
-            } else {
-                if (c >= ts) {
-                    out.why = "a lower dependency in a later tile";
-                    return false;
-                }
-                code = c;
+                out.lcol[(size_t)s * n + k] = code;
+                out.lmap[(size_t)s * n + k] = j;
             }
-            out.lcol[(size_t)s * n + k] = code;
-            out.lmap[(size_t)s * n + k] = j;
-        }
-        for (int s = 0; s < nu; ++s) {
-            const int j = j0 + nl + 1 + s;
-            const int c = pos[cl[j]];
-            int code;
-            if (c >= ts && c < te) {
-                if (c < kph_hi[k]) {
-                    out.why = "an upper dependency inside the tile but not in a later phase";
-                    return false;
+            for (int s = 0; s < nu; ++s) {
+                const int j = j0 + nl + 1 + s;
+                const int c = pos[cl[j]];
+                int code;
+                if (c >= ts && c < te) {
+                    if (c < kph_hi[k]) {
+                        fail = 3;
+                        return;
+                    }
+                    code = -(c - ts) - 1;
+                } else {
+                    if (c < te) {
+                        fail = 4;
+                        return;
+                    }
+                    code = c;
                 }
-                code = -(c - ts) - 1;
-            } else {
-                if (c < te) {
-                    out.why = "an upper dependency in an earlier tile";
-                    return false;
-                }
-                code = c;
+                out.ucol[(size_t)s * n + k] = code;
+                out.umap[(size_t)s * n + k] = j;
             }
-            out.ucol[(size_t)s * n + k] = code;
-            out.umap[(size_t)s * n + k] = j;
         }
+    });
+    if (fail.load()) {
+        out.why = kWhy[fail.load()];
+        return false;
     }
     return true;
 }
