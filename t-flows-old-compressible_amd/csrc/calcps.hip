@@ -512,37 +512,6 @@ __global__ __launch_bounds__(kBlock) void k_uvw_minmax(CalcPSMesh M, UvwMesh U, 
     }
 }
 
-// The off-diagonal pair of every interior face, (0 - A12) - D12 at
-// SidAij(1,s) and (0 - A21) - D21 at SidAij(2,s) (:364-367), and Abou(c2) of
-// BUFFER faces (:406), one thread per face; used where no two faces share a
-// position.
-__global__ __launch_bounds__(kBlock) void k_uvw_faces(CalcPSMesh M, UvwMesh U, UvwParams P, UvwFields F,
-                                                       double* __restrict__ aval, double* __restrict__ abou) {
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= M.ns) return;
-    const unsigned char k = U.ukind[s];
-    if (k != kUvwInterior && k != kUvwBuffer) return;
-    const int nb = M.nbc;
-    const int i1 = nb + M.c1[s], i2 = nb + M.c2[s];
-    const double Fu = F.flux_u[s];
-    const double A0 = P.visc * U.scoef[s];
-    double a12 = 0.0, a21 = 0.0, d12 = 0.0, d21 = 0.0;
-    if (P.diffus == kFlagFI) {
-        d12 = 0.0 + A0 / F.rho_n[i2];
-        d21 = 0.0 + A0 / F.rho_n[i1];
-    }
-    if (P.blend[U.material[i1] - 1] != kBlendNo || P.blend[U.material[i2] - 1] != kBlendNo) {
-        a12 = 0.0 - (Fu < 0.0 ? Fu : 0.0);
-        a21 = 0.0 + (Fu > 0.0 ? Fu : 0.0);
-    }
-    if (k == kUvwInterior) {
-        aval[M.sid1[s]] = 0.0 - a12 - d12;
-        aval[M.sid2[s]] = 0.0 - a21 - d21;
-    } else {
-        abou[i2] = -a12 - d12;
-    }
-}
-
 // One thread per cell: the two face loops walked over the cell's faces in
 // ascending face order (the order in which the reference's loops reach the
 // cell), then the cell loops (:252-257, :434-499).  SCALAR is the
@@ -550,10 +519,13 @@ __global__ __launch_bounds__(kBlock) void k_uvw_faces(CalcPSMesh M, UvwMesh U, U
 // the flux VISc grad(Z) . S, the two-level inertia product in its order, no
 // pressure term and no Asave.  The cell's own values
 // are read once; per face only the other cell's and the face's.  The
-// diagonal and b stay in registers.  The off-diagonals and Abou come from
-// k_uvw_faces, except where several faces write one position (SHARED): then
-// each off-diagonal of row c is accumulated, in face order, on a zeroed Aval
-// by the thread of cell c.
+// diagonal and b stay in registers.  The thread of cell c also writes the
+// off-diagonals of row c, (0 - A12) - D12 at SidAij(1,s) and (0 - A21) - D21
+// at SidAij(2,s) (:364-367), from the same A12, A21, D12, D21 it adds to the
+// diagonal, and Abou of its BUFFER faces (:406); where several faces write
+// one position (SHARED) each off-diagonal is accumulated, in face order, on
+// a zeroed Aval.
+constexpr int kUvwStaged = 6;   // off-diagonals per row staged in LDS (a hexahedron's faces)
 template <bool SHARED, bool SCALAR, bool TVD>
 __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, UvwGeom G, UvwParams P,
                                                        UvwFields F, double* __restrict__ aval,
@@ -564,6 +536,8 @@ __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, U
     const int t0 = blockIdx.x * kBlock + threadIdx.x;
     for (int q = t0; q <= nb; q += stride) rhoui[q] = F.rho_n[q] * F.ui_n[q];       // boundary cells and 0
     const bool diff = P.diffus == kFlagCN || P.diffus == kFlagFI;
+    __shared__ double st_val[SHARED ? 1 : kUvwStaged * kBlock];
+    __shared__ int st_pos[SHARED ? 1 : kUvwStaged * kBlock];
     // 256-cell tiles; each XCD walks one contiguous eighth of them
     const TileSpan sp = block_span_strided((M.nc + kBlock - 1) / kBlock);
     for (int t = sp.first; t < sp.end; t += sp.step) {
@@ -572,9 +546,20 @@ __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, U
         const int c = i + 1, ic = nb + c;
         constexpr int NG = SCALAR ? 3 : 7;
         const UvwCell own = uvw_cell<NG>(U, F, ic);
-        rhoui[ic] = own.ru;
         const int e0 = M.cptr[i], e1 = M.cptr[i + 1];
         double Cs = 0.0, Xc = 0.0, Xd = 0.0, diag = 0.0;
+        // the row's off-diagonals wait in LDS until the face loop is done: a
+        // store among the loop's loads would make every later load wait for it
+        int staged = 0;
+        auto stage = [&](int pos, double v) {
+            if (staged < kUvwStaged) {
+                st_val[staged * kBlock + threadIdx.x] = v;
+                st_pos[staged * kBlock + threadIdx.x] = pos;
+                ++staged;
+            } else {
+                aval[pos] = v;
+            }
+        };
         for (int e = e0; e < e1; ++e) {
             const int2 ie = U.inc[e];
             const int s = ie.x >> 1;
@@ -596,19 +581,17 @@ __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, U
             const unsigned char k = U.ukind[s];
             if (second) {                                                            // :366-367
                 diag = diag + f.a12 + f.d12;
-                if (SHARED) {
-                    const int pos = M.sid2[s];
-                    aval[pos] = aval[pos] - f.a21 - f.d21;
-                }
+                const int pos = M.sid2[s];
+                if (SHARED) aval[pos] = aval[pos] - f.a21 - f.d21;
+                else stage(pos, 0.0 - f.a21 - f.d21);
             } else if (k == kUvwInterior) {                                          // :364-365, :367
                 diag = diag + f.a21 + f.d21;
-                if (SHARED) {
-                    const int pos = M.sid1[s];
-                    aval[pos] = aval[pos] - f.a12 - f.d12;
-                }
+                const int pos = M.sid1[s];
+                if (SHARED) aval[pos] = aval[pos] - f.a12 - f.d12;
+                else stage(pos, 0.0 - f.a12 - f.d12);
             } else if (k == kUvwWallLike || k == kUvwBuffer) {                       // :391, :405
                 diag = diag + f.a21 + f.d21;
-                if (SHARED && k == kUvwBuffer) abou[nb + other] = -f.a12 - f.d12;   // Abou(-NbC:-1)
+                if (k == kUvwBuffer) abou[nb + other] = -f.a12 - f.d12;             // Abou(-NbC:-1)
             } else if (k == kUvwOutflow) {
                 diag = diag + f.a21;
             }
@@ -655,6 +638,9 @@ __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, U
         if (!SCALAR) bb = bb - F.pi[ic] * V;                                         // :484
         if (!SCALAR && P.var == 1 && asave) asave[ic] = diag;                        // :497
         bb = bb + diag * (1.0 - P.urf) * own.ru / P.urf;                             // :498
+        // the cell's stores, after all its loads
+        for (int j = 0; j < staged; ++j) aval[st_pos[j * kBlock + threadIdx.x]] = st_val[j * kBlock + threadIdx.x];
+        rhoui[ic] = own.ru;
         aval[M.adia[i]] = diag / P.urf;                                              // :499
         b[i] = bb;
     }
@@ -968,9 +954,6 @@ void CalcPS::assemble_equation(const UvwParams& P, const UvwFields& F, double* a
     // at most 2048 blocks, a multiple of 8 (the XCD-aware tile walk)
     const int tiles = (nc_ + kBlock - 1) / kBlock;
     const int grid = tiles < kNumXcd ? tiles : std::min(2048, (tiles + kNumXcd - 1) / kNumXcd * kNumXcd);
-    if (!offdiag_shared_ && diff && ns_ > 0)
-        hipLaunchKernelGGL(k_uvw_faces, dim3((ns_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, M_, U_, P, Fx, aval,
-                           abou);
     auto cells = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, M_, U_, G, P, Fx, aval, b, abou, asave, rhoui);
     };
