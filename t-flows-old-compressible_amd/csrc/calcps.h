@@ -59,6 +59,7 @@ struct UvwFields {
 // Geometry the momentum assembly adds to the mesh (device).
 struct UvwMesh {
     const int2* inc = nullptr;          // per cface entry: (2 s + role, the face's other cell)
+    const int* tiles = nullptr;         // visiting order of the 256-cell tiles (nullptr: in order)
     const unsigned char* ukind = nullptr;  // what a face does in the diffusion loop
     const double *scoef = nullptr, *ff = nullptr;
     const int* material = nullptr;      // (-NbC:NC), 1-based

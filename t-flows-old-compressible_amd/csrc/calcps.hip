@@ -31,6 +31,7 @@
 // volume over the same cell-to-face lists (k_cor_cells) and its maximum,
 // all bitwise the reference's.
 #include <algorithm>
+#include <map>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
@@ -541,7 +542,7 @@ __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, U
     // 256-cell tiles; each XCD walks one contiguous eighth of them
     const TileSpan sp = block_span_strided((M.nc + kBlock - 1) / kBlock);
     for (int t = sp.first; t < sp.end; t += sp.step) {
-        const int i = t * kBlock + threadIdx.x;
+        const int i = (U.tiles ? U.tiles[t] : t) * kBlock + threadIdx.x;
         if (i >= M.nc) continue;
         const int c = i + 1, ic = nb + c;
         constexpr int NG = SCALAR ? 3 : 7;
@@ -679,6 +680,42 @@ T* upload(std::vector<void*>& owned, const T* src, size_t count, bool device_src
 
 }  // namespace
 
+// The order in which the cell kernels visit their 256-cell tiles.  On a
+// naturally numbered box (interior faces joining cells 1, nx and nx * ny
+// apart, planes a whole number of tiles) the tiles go brick by brick:
+// 8 tile rows (16 lines of 128 cells) of 32 consecutive planes, bricks in
+// plane-block, then row-block order.  The XCD-strided walk gives each XCD
+// 256 consecutive entries at a time, one brick, so the cells in flight on
+// an XCD form a compact window and their neighbours one line and one
+// plane away are read while still in its L2, instead of four whole planes
+// apart.  Any other mesh: empty (tiles in order).
+static std::vector<int> brick_tile_order(int nc, const std::vector<int>& c1, const std::vector<int>& c2,
+                                         const std::vector<unsigned char>& kind) {
+    std::vector<int> none;
+    std::map<int, int64_t> hist;
+    const size_t ns = c1.size(), step = std::max<size_t>(1, ns / 200000);
+    for (size_t q = 0; q < ns; q += step)
+        if (kind[q] == kInterior) hist[std::abs(c2[q] - c1[q])]++;
+    std::vector<std::pair<int64_t, int>> top;
+    for (auto& h : hist) top.push_back({h.second, h.first});
+    if (top.size() < 3) return none;
+    std::sort(top.rbegin(), top.rend());
+    int o[3] = {top[0].second, top[1].second, top[2].second};
+    std::sort(o, o + 3);
+    const int line = o[1], plane = o[2];
+    if (o[0] != 1 || line < 2 || plane % line != 0 || plane % kBlock != 0 || nc % plane != 0) return none;
+    const int rows = plane / kBlock, planes = nc / plane;   // tile rows per plane
+    const int BY = std::min(rows, 8), BZ = std::min(planes, 32);
+    if (rows % BY != 0) return none;
+    std::vector<int> order;
+    order.reserve((size_t)rows * planes);
+    for (int z0 = 0; z0 < planes; z0 += BZ)
+        for (int y0 = 0; y0 < rows; y0 += BY)
+            for (int z = z0; z < std::min(planes, z0 + BZ); ++z)
+                for (int y = y0; y < y0 + BY; ++y) order.push_back(z * rows + y);
+    return order;
+}
+
 CalcPS::CalcPS(int nc, int nbc, int ns, int nonz, const int* sidec, const int* sidaij, const int* adia,
                const int* typebc, const double* const geom[7], const double* volume, bool device_ptrs, hipStream_t s)
     : nc_(nc), nbc_(nbc), ns_(ns), nonz_(nonz) {
@@ -777,6 +814,10 @@ CalcPS::CalcPS(int nc, int nbc, int ns, int nonz, const int* sidec, const int* s
         U_.inc = upload(owned_, inc.data(), inc.size(), false, s);
     }
     zero_idx_ = upload(owned_, zeros.data(), zeros.size(), false, s);
+    {
+        std::vector<int> order = brick_tile_order(nc, c1, c2, kind);
+        if (!order.empty()) U_.tiles = upload(owned_, order.data(), order.size(), false, s);
+    }
     const double** g[7] = {&M_.sx, &M_.sy, &M_.sz, &M_.dx, &M_.dy, &M_.dz, &M_.f};
     for (int q = 0; q < 7; ++q) *g[q] = upload(owned_, geom[q], (size_t)ns, device_ptrs, s);
     M_.volume = upload(owned_, volume, (size_t)(nbc + nc + 1), device_ptrs, s);
