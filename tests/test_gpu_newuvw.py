@@ -224,3 +224,28 @@ def test_assemble_solve_update(pkg, oracle, cuda):
         assert np.array_equal(got[:nb + 1], args["ui_n"][:nb + 1])          # boundary values untouched
         ds.close()
     dc.close()
+
+
+@pytest.mark.parametrize("flags,var", [("implicit", 1), ("upwind_par_cn", 2), ("no_diffusion", 3)])
+def test_assembly_bitwise_brick_order(pkg, oracle, cuda, flags, var):
+    """A naturally numbered channel whose planes hold 16 tiles of 256 cells:
+    the cell kernel visits its tiles brick by brick (8 tile rows x 32 planes,
+    then the remaining 8 planes) instead of in order, and every output stays
+    the oracle's bit for bit."""
+    mg = pkg.meshgen
+    case = mg.calcps_case((64, 64, 40), seed=61)
+    uf = mg.uvw_fields(case, seed=62)
+    inner = case.sidec[2] > 0
+    off = np.abs(case.sidec[2][inner] - case.sidec[1][inner])
+    vals, counts = np.unique(off, return_counts=True)
+    line, plane = sorted(vals[np.argsort(counts)[-3:]])[1:]
+    rows = plane // 256
+    assert plane % 256 == 0 and case.nc % plane == 0 and rows > 8 and rows % 8 == 0   # the brick order applies
+    p = _params(pkg, flags, uf["nmaterials"], var)
+    want = oracle.newuvw(case, uf, p)
+    dc, args, got = _run(pkg, case, uf, p)
+    print(f"brick order: nc {case.nc} line {line} plane {plane} flags {flags} var {var}", flush=True)
+    for k in ("aval", "b", "rhoui"):
+        assert np.array_equal(got[k], want[k]), k
+    assert np.array_equal(got["abou"][:case.nbc], want["abou"])
+    dc.close()
