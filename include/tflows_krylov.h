@@ -350,6 +350,14 @@ int tflows_calcps_create(int32_t nc, int32_t nbc, int32_t ns, int32_t nonz, cons
                          const double* sy, const double* sz, const double* dx, const double* dy, const double* dz,
                          const double* f, const double* volume, int32_t mem, void* stream, tflows_calcps* out);
 int tflows_calcps_destroy(tflows_calcps h);
+/* How the momentum and mixture-fraction cell kernels visit the cells of
+ * this mesh: 2 in blocks of 64 cells x 2 lines x 2 planes (a naturally
+ * numbered box whose lines are whole multiples of 64 cells, with an even
+ * number of lines and of planes), 1 in 256-cell tiles brick by brick (a box
+ * whose planes are whole tiles), 0 in 256-cell tiles in order; -1 on
+ * error.  TFLOWS_UVW_ORDER=0 or 1 caps it when the handle is created.  All
+ * give the same bits.  No counterpart in the reference. */
+int32_t tflows_calcps_visit_order(tflows_calcps h);
 int tflows_calcps_assemble(tflows_calcps h, int32_t nmat, double dt, const double* asave, const double* rho_n,
                            const double* rho_o, const double* rho_oo, const double* u, const double* v,
                            const double* w, const double* p, const double* px, const double* py, const double* pz,

@@ -60,6 +60,7 @@ struct UvwFields {
 struct UvwMesh {
     const int2* inc = nullptr;          // per cface entry: (2 s + role, the face's other cell)
     const int* tiles = nullptr;         // visiting order of the 256-cell tiles (nullptr: in order)
+    const int* wbase = nullptr;         // first cell of each wavefront of each visited tile (else: tiles)
     const unsigned char* ukind = nullptr;  // what a face does in the diffusion loop
     const double *scoef = nullptr, *ff = nullptr;
     const int* material = nullptr;      // (-NbC:NC), 1-based
@@ -108,6 +109,9 @@ public:
     void zmix_update(const double* rhoz, const double* rho_n, double* z_n, hipStream_t s);
     bool uvw_ready() const { return U_.ff != nullptr; }
     int nmaterials() const { return nmaterials_; }
+    // how the momentum and scalar cell kernels visit their cells: 0 tiles in
+    // order, 1 tiles brick by brick, 2 blocks of 64 cells x 2 lines x 2 planes
+    int visit_order() const { return visit_order_; }
 
     int nc() const { return nc_; }
     int nbc() const { return nbc_; }
@@ -137,6 +141,7 @@ private:
     bool bcell_single_ = true;          // every boundary cell has at most one face
     double* minmax_ = nullptr;          // PHImax, PHImin of ConvScheme ((-NbC:NC) each)
     int nmaterials_ = 0;
+    int visit_order_ = 0;
     unsigned pairs_[kMaxMaterials] = {};  // material pairs that meet at a face
 };
 

@@ -32,6 +32,7 @@
 // all bitwise the reference's.
 #include <algorithm>
 #include <map>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
@@ -518,9 +519,11 @@ __global__ __launch_bounds__(kBlock) void k_uvw_minmax(CalcPSMesh M, UvwMesh U, 
 // cell), then the cell loops (:252-257, :434-499).  SCALAR is the
 // mixture-fraction equation (Process/CalcZmix.f90:100-387): three gradients,
 // the flux VISc grad(Z) . S, the two-level inertia product in its order, no
-// pressure term and no Asave.  The cell's own values
-// are read once; per face only the other cell's and the face's.  The
-// diagonal and b stay in registers.  The thread of cell c also writes the
+// pressure term and no Asave.  The cell's own values are read once and
+// kept in LDS for the block; per face the face's values are read, and the
+// other cell's from LDS when it belongs to the same block (a tile is four
+// runs of 64 cells, listed in U.wbase or consecutive), else from memory:
+// the same bits either way.  The diagonal and b stay in registers.  The thread of cell c also writes the
 // off-diagonals of row c, (0 - A12) - D12 at SidAij(1,s) and (0 - A21) - D21
 // at SidAij(2,s) (:364-367), from the same A12, A21, D12, D21 it adds to the
 // diagonal, and Abou of its BUFFER faces (:406); where several faces write
@@ -539,111 +542,150 @@ __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, U
     const bool diff = P.diffus == kFlagCN || P.diffus == kFlagFI;
     __shared__ double st_val[SHARED ? 1 : kUvwStaged * kBlock];
     __shared__ int st_pos[SHARED ? 1 : kUvwStaged * kBlock];
-    // 256-cell tiles; each XCD walks one contiguous eighth of them
+    // 256-cell tiles; each XCD walks one contiguous eighth of them (in U.wbase or
+    // U.tiles order when the mesh has one)
+    // the values of the block's own cells, for the faces whose other cell is
+    // one of them
+    constexpr int NG = SCALAR ? 3 : 7;
+    __shared__ double sh_rho[kBlock], sh_ru[kBlock], sh_g[NG][kBlock];
+    __shared__ int sh_mat[kBlock];
     const TileSpan sp = block_span_strided((M.nc + kBlock - 1) / kBlock);
     for (int t = sp.first; t < sp.end; t += sp.step) {
-        const int i = (U.tiles ? U.tiles[t] : t) * kBlock + threadIdx.x;
-        if (i >= M.nc) continue;
+        // the first cell of each wavefront of the tile; wavefront w holds 64 consecutive cells
+        int wb[kWavesPerBlock];
+#pragma unroll
+        for (int w = 0; w < kWavesPerBlock; ++w)
+            wb[w] = U.wbase ? U.wbase[t * kWavesPerBlock + w] : (U.tiles ? U.tiles[t] : t) * kBlock + w * kWave;
+        const int i = wb[threadIdx.x / kWave] + (int)(threadIdx.x & (kWave - 1));
+        const bool live = i < M.nc;
         const int c = i + 1, ic = nb + c;
-        constexpr int NG = SCALAR ? 3 : 7;
-        const UvwCell own = uvw_cell<NG>(U, F, ic);
-        const int e0 = M.cptr[i], e1 = M.cptr[i + 1];
-        double Cs = 0.0, Xc = 0.0, Xd = 0.0, diag = 0.0;
-        // the row's off-diagonals wait in LDS until the face loop is done: a
-        // store among the loop's loads would make every later load wait for it
-        int staged = 0;
-        auto stage = [&](int pos, double v) {
-            if (staged < kUvwStaged) {
-                st_val[staged * kBlock + threadIdx.x] = v;
-                st_pos[staged * kBlock + threadIdx.x] = pos;
-                ++staged;
-            } else {
-                aval[pos] = v;
-            }
-        };
-        for (int e = e0; e < e1; ++e) {
-            const int2 ie = U.inc[e];
-            const int s = ie.x >> 1;
-            const bool second = ie.x & 1;                                            // the cell is c2
-            const int other = ie.y;
-            const UvwCell oth = uvw_cell<NG>(U, F, nb + other);
-            const UvwFace f =
-                uvw_face<SCALAR, TVD>(M, U, G, P, F, s, pick<NG>(!second, own, oth), pick<NG>(!second, oth, own));
-            if (!second) {
-                Cs = Cs - f.conv;
-                if (f.upwind) Xc = Xc - f.up;
-                Xd = Xd + f.fex - f.fim;                                             // :338
-            } else {
-                Cs = Cs + f.conv;
-                if (f.upwind) Xc = Xc + f.up;
-                Xd = Xd - f.fex + f.fim;                                             // :340
-            }
-            if (!diff) continue;
-            const unsigned char k = U.ukind[s];
-            if (second) {                                                            // :366-367
-                diag = diag + f.a12 + f.d12;
-                const int pos = M.sid2[s];
-                if (SHARED) aval[pos] = aval[pos] - f.a21 - f.d21;
-                else stage(pos, 0.0 - f.a21 - f.d21);
-            } else if (k == kUvwInterior) {                                          // :364-365, :367
-                diag = diag + f.a21 + f.d21;
-                const int pos = M.sid1[s];
-                if (SHARED) aval[pos] = aval[pos] - f.a12 - f.d12;
-                else stage(pos, 0.0 - f.a12 - f.d12);
-            } else if (k == kUvwWallLike || k == kUvwBuffer) {                       // :391, :405
-                diag = diag + f.a21 + f.d21;
-                if (k == kUvwBuffer) abou[nb + other] = -f.a12 - f.d12;             // Abou(-NbC:-1)
-            } else if (k == kUvwOutflow) {
-                diag = diag + f.a21;
-            }
+        UvwCell own{};
+        if (live) {
+            own = uvw_cell<NG>(U, F, ic);
+            sh_rho[threadIdx.x] = own.rho;
+            sh_ru[threadIdx.x] = own.ru;
+#pragma unroll
+            for (int k = 0; k < NG; ++k) sh_g[k][threadIdx.x] = own.g[k];
+            sh_mat[threadIdx.x] = own.mat;
         }
-        double bb = 0.0;
-        if (P.convec == kFlagFI) bb = bb + P.urfc[own.mat] * (Cs - Xc);              // :252-257
-        if (diff) {
-            // b(c1) of WALL-like and OUTFLOW faces (A12 and D12 of :356-359),
-            // after the convection term
+        __syncthreads();
+        if (live) {
+            const int e0 = M.cptr[i], e1 = M.cptr[i + 1];
+            double Cs = 0.0, Xc = 0.0, Xd = 0.0, diag = 0.0;
+            // the row's off-diagonals wait in LDS until the face loop is done: a
+            // store among the loop's loads would make every later load wait for it
+            int staged = 0;
+            auto stage = [&](int pos, double v) {
+                if (staged < kUvwStaged) {
+                    st_val[staged * kBlock + threadIdx.x] = v;
+                    st_pos[staged * kBlock + threadIdx.x] = pos;
+                    ++staged;
+                } else {
+                    aval[pos] = v;
+                }
+            };
             for (int e = e0; e < e1; ++e) {
                 const int2 ie = U.inc[e];
-                if (ie.x & 1) continue;
                 const int s = ie.x >> 1;
-                const unsigned char k = U.ukind[s];
-                if (k != kUvwWallLike && k != kUvwOutflow) continue;
-                const int i2 = nb + ie.y;
-                const double rho2 = F.rho_n[i2], ru2 = rho2 * F.ui_n[i2];
-                const double Fu = F.flux_u[s];
-                const int m2 = U.material[i2] - 1;
-                const double a12 = (P.blend[own.mat] != kBlendNo || P.blend[m2] != kBlendNo)
-                                       ? 0.0 - (Fu < 0.0 ? Fu : 0.0) : 0.0;
-                if (k == kUvwWallLike) {
-                    const double d12 = P.diffus == kFlagFI ? 0.0 + (P.visc * U.scoef[s]) / rho2 : 0.0;
-                    bb = bb + (a12 + d12) * ru2;                                     // :392
+                const bool second = ie.x & 1;                                            // the cell is c2
+                const int other = ie.y;
+                // a cell of this block: its values from LDS (the same bits)
+                int slot = -1;
+#pragma unroll
+                for (int w = 0; w < kWavesPerBlock; ++w) {
+                    const unsigned d = (unsigned)(other - 1 - wb[w]);
+                    if (d < (unsigned)kWave) slot = w * kWave + (int)d;
+                }
+                UvwCell oth;
+                if (slot >= 0) {
+                    oth.rho = sh_rho[slot];
+                    oth.ru = sh_ru[slot];
+#pragma unroll
+                    for (int k = 0; k < 7; ++k) oth.g[k] = k < NG ? sh_g[k < NG ? k : 0][slot] : 0.0;
+                    oth.mat = sh_mat[slot];
+                    oth.idx = nb + other;
                 } else {
-                    bb = bb + a12 * ru2;                                             // :399
+                    oth = uvw_cell<NG>(U, F, nb + other);
+                }
+                const UvwFace f =
+                    uvw_face<SCALAR, TVD>(M, U, G, P, F, s, pick<NG>(!second, own, oth), pick<NG>(!second, oth, own));
+                if (!second) {
+                    Cs = Cs - f.conv;
+                    if (f.upwind) Xc = Xc - f.up;
+                    Xd = Xd + f.fex - f.fim;                                             // :338
+                } else {
+                    Cs = Cs + f.conv;
+                    if (f.upwind) Xc = Xc + f.up;
+                    Xd = Xd - f.fex + f.fim;                                             // :340
+                }
+                if (!diff) continue;
+                const unsigned char k = U.ukind[s];
+                if (second) {                                                            // :366-367
+                    diag = diag + f.a12 + f.d12;
+                    const int pos = M.sid2[s];
+                    if (SHARED) aval[pos] = aval[pos] - f.a21 - f.d21;
+                    else stage(pos, 0.0 - f.a21 - f.d21);
+                } else if (k == kUvwInterior) {                                          // :364-365, :367
+                    diag = diag + f.a21 + f.d21;
+                    const int pos = M.sid1[s];
+                    if (SHARED) aval[pos] = aval[pos] - f.a12 - f.d12;
+                    else stage(pos, 0.0 - f.a12 - f.d12);
+                } else if (k == kUvwWallLike || k == kUvwBuffer) {                       // :391, :405
+                    diag = diag + f.a21 + f.d21;
+                    if (k == kUvwBuffer) abou[nb + other] = -f.a12 - f.d12;             // Abou(-NbC:-1)
+                } else if (k == kUvwOutflow) {
+                    diag = diag + f.a21;
                 }
             }
+            double bb = 0.0;
+            if (P.convec == kFlagFI) bb = bb + P.urfc[own.mat] * (Cs - Xc);              // :252-257
+            if (diff) {
+                // b(c1) of WALL-like and OUTFLOW faces (A12 and D12 of :356-359),
+                // after the convection term
+                for (int e = e0; e < e1; ++e) {
+                    const int2 ie = U.inc[e];
+                    if (ie.x & 1) continue;
+                    const int s = ie.x >> 1;
+                    const unsigned char k = U.ukind[s];
+                    if (k != kUvwWallLike && k != kUvwOutflow) continue;
+                    const int i2 = nb + ie.y;
+                    const double rho2 = F.rho_n[i2], ru2 = rho2 * F.ui_n[i2];
+                    const double Fu = F.flux_u[s];
+                    const int m2 = U.material[i2] - 1;
+                    const double a12 = (P.blend[own.mat] != kBlendNo || P.blend[m2] != kBlendNo)
+                                           ? 0.0 - (Fu < 0.0 ? Fu : 0.0) : 0.0;
+                    if (k == kUvwWallLike) {
+                        const double d12 = P.diffus == kFlagFI ? 0.0 + (P.visc * U.scoef[s]) / rho2 : 0.0;
+                        bb = bb + (a12 + d12) * ru2;                                     // :392
+                    } else {
+                        bb = bb + a12 * ru2;                                             // :399
+                    }
+                }
+            }
+            if (P.cross == kFlagFI) bb = bb + Xd;                                        // :434-438
+            const double V = M.volume[ic];
+            if (P.inert == kInertLin) {                                                  // :447-450
+                diag = diag + V / P.dt;
+                if (SCALAR)                                                              // CalcZmix.f90:371
+                    bb = bb + V / P.dt * F.ui_o[ic] * F.rho_o[ic] + F.source[ic] * V;
+                else
+                    bb = bb + V / P.dt * F.rho_o[ic] * F.ui_o[ic] + F.source[ic] * V;
+            }
+            if (P.inert == kInertPar) {                                                  // :453-456
+                diag = diag + 1.5 * V / P.dt;
+                bb = bb + 2.0 * V / P.dt * F.rho_o[ic] * F.ui_o[ic] - 0.5 * V / P.dt * F.rho_oo[ic] * F.ui_oo[ic] +
+                     F.source[ic] * V;
+            }
+            if (!SCALAR) bb = bb - F.pi[ic] * V;                                         // :484
+            if (!SCALAR && P.var == 1 && asave) asave[ic] = diag;                        // :497
+            bb = bb + diag * (1.0 - P.urf) * own.ru / P.urf;                             // :498
+            // the cell's stores, after all its loads
+            for (int j = 0; j < staged; ++j) aval[st_pos[j * kBlock + threadIdx.x]] = st_val[j * kBlock + threadIdx.x];
+            rhoui[ic] = own.ru;
+            aval[M.adia[i]] = diag / P.urf;                                              // :499
+            b[i] = bb;
         }
-        if (P.cross == kFlagFI) bb = bb + Xd;                                        // :434-438
-        const double V = M.volume[ic];
-        if (P.inert == kInertLin) {                                                  // :447-450
-            diag = diag + V / P.dt;
-            if (SCALAR)                                                              // CalcZmix.f90:371
-                bb = bb + V / P.dt * F.ui_o[ic] * F.rho_o[ic] + F.source[ic] * V;
-            else
-                bb = bb + V / P.dt * F.rho_o[ic] * F.ui_o[ic] + F.source[ic] * V;
-        }
-        if (P.inert == kInertPar) {                                                  // :453-456
-            diag = diag + 1.5 * V / P.dt;
-            bb = bb + 2.0 * V / P.dt * F.rho_o[ic] * F.ui_o[ic] - 0.5 * V / P.dt * F.rho_oo[ic] * F.ui_oo[ic] +
-                 F.source[ic] * V;
-        }
-        if (!SCALAR) bb = bb - F.pi[ic] * V;                                         // :484
-        if (!SCALAR && P.var == 1 && asave) asave[ic] = diag;                        // :497
-        bb = bb + diag * (1.0 - P.urf) * own.ru / P.urf;                             // :498
-        // the cell's stores, after all its loads
-        for (int j = 0; j < staged; ++j) aval[st_pos[j * kBlock + threadIdx.x]] = st_val[j * kBlock + threadIdx.x];
-        rhoui[ic] = own.ru;
-        aval[M.adia[i]] = diag / P.urf;                                              // :499
-        b[i] = bb;
+        __syncthreads();   // the next tile's cells overwrite the LDS values
     }
 }
 
@@ -680,30 +722,35 @@ T* upload(std::vector<void*>& owned, const T* src, size_t count, bool device_src
 
 }  // namespace
 
-// The order in which the cell kernels visit their 256-cell tiles.  On a
-// naturally numbered box (interior faces joining cells 1, nx and nx * ny
-// apart, planes a whole number of tiles) the tiles go brick by brick:
-// 8 tile rows (16 lines of 128 cells) of 32 consecutive planes, bricks in
-// plane-block, then row-block order.  The XCD-strided walk gives each XCD
-// 256 consecutive entries at a time, one brick, so the cells in flight on
-// an XCD form a compact window and their neighbours one line and one
-// plane away are read while still in its L2, instead of four whole planes
-// apart.  Any other mesh: empty (tiles in order).
-static std::vector<int> brick_tile_order(int nc, const std::vector<int>& c1, const std::vector<int>& c2,
-                                         const std::vector<unsigned char>& kind) {
-    std::vector<int> none;
+// The line and plane offsets of a naturally numbered box: interior faces
+// joining cells 1, nx and nx * ny apart (the three most common offsets of a
+// sample of the interior faces), planes whole, NC a whole number of planes.
+static bool box_offsets(int nc, const std::vector<int>& c1, const std::vector<int>& c2,
+                        const std::vector<unsigned char>& kind, int& line, int& plane) {
     std::map<int, int64_t> hist;
     const size_t ns = c1.size(), step = std::max<size_t>(1, ns / 200000);
     for (size_t q = 0; q < ns; q += step)
         if (kind[q] == kInterior) hist[std::abs(c2[q] - c1[q])]++;
     std::vector<std::pair<int64_t, int>> top;
     for (auto& h : hist) top.push_back({h.second, h.first});
-    if (top.size() < 3) return none;
+    if (top.size() < 3) return false;
     std::sort(top.rbegin(), top.rend());
     int o[3] = {top[0].second, top[1].second, top[2].second};
     std::sort(o, o + 3);
-    const int line = o[1], plane = o[2];
-    if (o[0] != 1 || line < 2 || plane % line != 0 || plane % kBlock != 0 || nc % plane != 0) return none;
+    line = o[1];
+    plane = o[2];
+    return o[0] == 1 && line >= 2 && plane % line == 0 && nc % plane == 0;
+}
+
+// The order in which the cell kernels visit their 256-cell tiles on a box
+// whose planes are a whole number of tiles: brick by brick, 8 tile rows (16
+// lines of 128 cells) of 32 consecutive planes, bricks in plane-block, then
+// row-block order.  The XCD-strided walk gives each XCD 256 consecutive
+// entries at a time, one brick, so the cells in flight on an XCD form a
+// compact window.  Otherwise empty (tiles in order).
+static std::vector<int> brick_tile_order(int nc, int line, int plane) {
+    std::vector<int> none;
+    if (plane % kBlock != 0) return none;
     const int rows = plane / kBlock, planes = nc / plane;   // tile rows per plane
     const int BY = std::min(rows, 8), BZ = std::min(planes, 32);
     if (rows % BY != 0) return none;
@@ -714,6 +761,39 @@ static std::vector<int> brick_tile_order(int nc, const std::vector<int>& c1, con
             for (int z = z0; z < std::min(planes, z0 + BZ); ++z)
                 for (int y = y0; y < y0 + BY; ++y) order.push_back(z * rows + y);
     return order;
+}
+
+// Cells of each wavefront when a block is a small box: on a box whose lines
+// are whole wavefronts, with an even number of lines and of planes, tile t
+// is 64 cells of 2 lines of 2 planes, its wavefront w (w = 0..3) the 64
+// cells of line (w & 1) of plane (w >> 1), and wb[4 t + w] the first of
+// them.  Four of a cell's six neighbours are then in its own block (the
+// cell kernel reads those from LDS); tiles go in bricks of 16 lines by 32
+// planes (whole lines), one brick per XCD window as above.  Otherwise empty.
+static std::vector<int> wave_block_bases(int nc, int line, int plane) {
+    std::vector<int> none;
+    const int rows = plane / line, planes = nc / plane;
+    if (line % kWave != 0 || rows % 2 != 0 || planes % 2 != 0) return none;
+    const int xs = line / kWave, BY = std::min(rows, 16), BZ = std::min(planes, 32);
+    std::vector<int> wb;
+    wb.reserve((size_t)nc / kWave);
+    for (int z0 = 0; z0 < planes; z0 += BZ)
+        for (int y0 = 0; y0 < rows; y0 += BY)
+            for (int z = z0; z < std::min(planes, z0 + BZ); z += 2)
+                for (int y = y0; y < std::min(rows, y0 + BY); y += 2)
+                    for (int x = 0; x < xs; ++x)
+                        for (int w = 0; w < kWavesPerBlock; ++w)
+                            wb.push_back((z + (w >> 1)) * plane + (y + (w & 1)) * line + x * kWave);
+    // every cell exactly once
+    if (wb.size() * kWave != (size_t)nc) return none;
+    std::vector<char> seen((size_t)nc, 0);
+    for (int base : wb)
+        for (int l = 0; l < kWave; ++l) {
+            const int i = base + l;
+            if (i < 0 || i >= nc || seen[i]) return none;
+            seen[i] = 1;
+        }
+    return wb;
 }
 
 CalcPS::CalcPS(int nc, int nbc, int ns, int nonz, const int* sidec, const int* sidaij, const int* adia,
@@ -815,8 +895,23 @@ CalcPS::CalcPS(int nc, int nbc, int ns, int nonz, const int* sidec, const int* s
     }
     zero_idx_ = upload(owned_, zeros.data(), zeros.size(), false, s);
     {
-        std::vector<int> order = brick_tile_order(nc, c1, c2, kind);
-        if (!order.empty()) U_.tiles = upload(owned_, order.data(), order.size(), false, s);
+        // TFLOWS_UVW_ORDER: 0 tiles in order, 1 at most the brick order, 2 (default) the best that applies
+        const char* env = std::getenv("TFLOWS_UVW_ORDER");
+        const int want = env ? std::atoi(env) : 2;
+        int line = 0, plane = 0;
+        if (want > 0 && box_offsets(nc, c1, c2, kind, line, plane)) {
+            std::vector<int> wb = want >= 2 ? wave_block_bases(nc, line, plane) : std::vector<int>();
+            if (!wb.empty()) {
+                U_.wbase = upload(owned_, wb.data(), wb.size(), false, s);
+                visit_order_ = 2;
+            } else {
+                std::vector<int> order = brick_tile_order(nc, line, plane);
+                if (!order.empty()) {
+                    U_.tiles = upload(owned_, order.data(), order.size(), false, s);
+                    visit_order_ = 1;
+                }
+            }
+        }
     }
     const double** g[7] = {&M_.sx, &M_.sy, &M_.sz, &M_.dx, &M_.dy, &M_.dz, &M_.f};
     for (int q = 0; q < 7; ++q) *g[q] = upload(owned_, geom[q], (size_t)ns, device_ptrs, s);

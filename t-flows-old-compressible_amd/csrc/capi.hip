@@ -515,6 +515,16 @@ int tflows_calcps_destroy(tflows_calcps h) {
     return guarded([&] { delete h; });
 }
 
+int32_t tflows_calcps_visit_order(tflows_calcps h) {
+    int32_t v = 0;
+    if (guarded([&] {
+            if (!h) throw std::invalid_argument("null handle");
+            v = h->cps->visit_order();
+        }))
+        return -1;
+    return v;
+}
+
 int tflows_calcps_assemble(tflows_calcps h, int32_t nmat, double dt, const double* asave, const double* rho_n,
                            const double* rho_o, const double* rho_oo, const double* u, const double* v,
                            const double* w, const double* p, const double* px, const double* py, const double* pz,

@@ -188,6 +188,8 @@ def _bind_extensions(lib):
         lib.tflows_calcps_create.argtypes = [C.c_int32] * 4 + [vp] * 12 + [C.c_int32, vp, C.POINTER(vp)]
         lib.tflows_calcps_destroy.restype = C.c_int
         lib.tflows_calcps_destroy.argtypes = [vp]
+        lib.tflows_calcps_visit_order.restype = C.c_int32
+        lib.tflows_calcps_visit_order.argtypes = [vp]
         lib.tflows_calcps_assemble.restype = C.c_int
         lib.tflows_calcps_assemble.argtypes = ([vp, C.c_int32, C.c_double] + [vp] * 11 + [vp] * 4 +
                                                [C.POINTER(C.c_double)] * 2 + [C.c_int32, vp])
@@ -735,6 +737,15 @@ class DeviceCalcPS:
         _check(self.lib.tflows_calcps_create(case.nc, case.nbc, case.ns, case.nonz, *(_ptr(t) for t in keep), 1,
                                              _stream(stream), C.byref(h)))
         self.handle = h
+
+    def visit_order(self) -> int:
+        """How the momentum and scalar cell kernels visit the cells: 2 blocks
+        of 64 cells x 2 lines x 2 planes, 1 tiles brick by brick, 0 tiles in
+        order."""
+        v = int(self.lib.tflows_calcps_visit_order(self.handle))
+        if v < 0:
+            _check(v)
+        return v
 
     def assemble(self, fields=None, nmat=None, dt=None, out=None, stream=None):
         """Assemble from cuda float64 field tensors ((-NbC:NC) layout; the

@@ -226,12 +226,15 @@ def test_assemble_solve_update(pkg, oracle, cuda):
     dc.close()
 
 
+@pytest.mark.parametrize("cap,order", [("1", 1), ("2", 2)])
 @pytest.mark.parametrize("flags,var", [("implicit", 1), ("upwind_par_cn", 2), ("no_diffusion", 3)])
-def test_assembly_bitwise_brick_order(pkg, oracle, cuda, flags, var):
-    """A naturally numbered channel whose planes hold 16 tiles of 256 cells:
-    the cell kernel visits its tiles brick by brick (8 tile rows x 32 planes,
-    then the remaining 8 planes) instead of in order, and every output stays
-    the oracle's bit for bit."""
+def test_assembly_bitwise_visit_orders(pkg, oracle, cuda, monkeypatch, flags, var, cap, order):
+    """A naturally numbered channel of 64 x 64 x 40 cells: its planes hold 16
+    tiles of 256 cells, its lines one wavefront.  Capped at 1 the cell kernel
+    visits its tiles brick by brick (8 tile rows x 32 planes, then the
+    remaining 8 planes); by default each block is 64 cells x 2 lines x 2
+    planes, whose faces inside the block take the other cell's values from
+    LDS.  Every output stays the oracle's bit for bit."""
     mg = pkg.meshgen
     case = mg.calcps_case((64, 64, 40), seed=61)
     uf = mg.uvw_fields(case, seed=62)
@@ -241,11 +244,35 @@ def test_assembly_bitwise_brick_order(pkg, oracle, cuda, flags, var):
     line, plane = sorted(vals[np.argsort(counts)[-3:]])[1:]
     rows = plane // 256
     assert plane % 256 == 0 and case.nc % plane == 0 and rows > 8 and rows % 8 == 0   # the brick order applies
+    assert line % 64 == 0 and (plane // line) % 2 == 0 and (case.nc // plane) % 2 == 0   # so do the blocks
+    monkeypatch.setenv("TFLOWS_UVW_ORDER", cap)
     p = _params(pkg, flags, uf["nmaterials"], var)
     want = oracle.newuvw(case, uf, p)
     dc, args, got = _run(pkg, case, uf, p)
-    print(f"brick order: nc {case.nc} line {line} plane {plane} flags {flags} var {var}", flush=True)
+    print(f"visit order {dc.visit_order()} (cap {cap}): nc {case.nc} line {line} plane {plane} flags {flags} "
+          f"var {var}", flush=True)
+    assert dc.visit_order() == order
     for k in ("aval", "b", "rhoui"):
         assert np.array_equal(got[k], want[k]), k
     assert np.array_equal(got["abou"][:case.nbc], want["abou"])
     dc.close()
+
+
+@pytest.mark.parametrize("scheme,var", [(24, 1), (27, 2)])
+def test_conv_scheme_bitwise_block_order(pkg, oracle, cuda, scheme, var):
+    """ConvScheme face values with the block-shaped visiting order (64 x 64
+    x 40 channel): the upwind extrapolation reads the other cell's gradients
+    while its density and velocity come from LDS; bit for bit the oracle's."""
+    mg = pkg.meshgen
+    case = mg.calcps_case((64, 64, 40), seed=63)
+    uf = with_mass_flux(mg.uvw_fields(case, seed=64))
+    p = mg.uvw_params(nmaterials=1, var=var, blend=[scheme], blend_tem=[YES], urf=0.7, urfc=[0.9])
+    want = oracle.newuvw(case, uf, p)
+    dc, args, got = _run(pkg, case, uf, p)
+    print(f"visit order {dc.visit_order()} scheme {scheme} var {var}", flush=True)
+    assert dc.visit_order() == 2
+    for k in ("aval", "b", "rhoui"):
+        assert np.array_equal(got[k], want[k]), k
+    assert np.array_equal(got["abou"][:case.nbc], want["abou"])
+    dc.close()
+

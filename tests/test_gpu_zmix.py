@@ -139,3 +139,30 @@ def test_assemble_solve_update(pkg, oracle, cuda):
     assert np.all((got[nb + 1:] >= 0.0) & (got[nb + 1:] <= 1.0))
     ds.close()
     dc.close()
+
+
+@pytest.mark.parametrize("flags,scheme", [("implicit", None), ("upwind_par_cn", None), ("implicit", 27)])
+def test_assembly_bitwise_block_order(pkg, oracle, cuda, flags, scheme):
+    """The scalar form with the block-shaped visiting order (64 x 64 x 40
+    channel; three gradients per cell in LDS): Aval, b, Abou and rhoZ bit for
+    bit the oracle's, with and without a face scheme."""
+    mg = pkg.meshgen
+    case = mg.calcps_case((64, 64, 40), seed=73)
+    uf = with_mass_flux(mg.uvw_fields(case, seed=74))
+    zf = mg.zmix_fields(case, seed=75, nmaterials=1)
+    zf.update({k: uf[k] for k in ("scoef", "ff", "material", "flux_u", "flux")})
+    p = _zparams(pkg, flags, 1)
+    if scheme is not None:
+        p = dict(p, blend_tem=[scheme])
+    want = oracle.calczmix(case, zf, p)
+    dc = pkg.krylov.DeviceCalcPS(case)
+    dc.uvw_setup(zf)
+    args = mg.zmix_args(case, zf)
+    print(f"visit order {dc.visit_order()} flags {flags} scheme {scheme}", flush=True)
+    assert dc.visit_order() == 2
+    got = {k: v.cpu().numpy() for k, v in dc.assemble_zmix(p, dc.zmix_inputs(args)).items()}
+    for k in ("aval", "b", "rhoz"):
+        assert np.array_equal(got[k], want[k]), k
+    assert np.array_equal(got["abou"][:case.nbc], want["abou"])
+    dc.close()
+
