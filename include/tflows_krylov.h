@@ -377,6 +377,19 @@ int tflows_calcps_correct(tflows_calcps h, double dt, const double* pp, const do
                           const double* px, const double* py, const double* pz, double* u, double* v, double* w,
                           double* flux, double* b, double* errmax, int32_t mem, void* stream);
 
+/* The CFL and Peclet monitor of Process/CorUVW.f90:124-147, per material m
+ * (m = 1..Nmat of tflows_calcps_uvw_setup, which must come first: it brings
+ * Scoef and the materials): over the interior and BUFFER faces with a cell
+ * of material m on either side, cflmax[m-1] = max |dt Flux / (RhoS Scoef
+ * (Dx^2 + Dy^2 + Dz^2))| with RhoS = f rho%n(c1) + (1 - f) rho%n(c2), and
+ * pemax[m-1] = max |Flux / Scoef / (VISc + TINY)|; zero for a material
+ * with no such face.  rho_n (-NbC:NC) and the corrected Flux (NS) as mem
+ * says; cflmax and pemax are host arrays of Nmat values, the reference's
+ * bit for bit (a maximum does not depend on the order).  Call it after
+ * tflows_calcps_correct, as CorUVW does. */
+int tflows_calcps_cfl_peclet(tflows_calcps h, double dt, double visc, const double* rho_n, const double* flux,
+                             double* cflmax, double* pemax, int32_t mem, void* stream);
+
 /* The momentum assembly of one velocity component, Process/NewUVW.f90:137-499
  * (the reference's `newuvw` up to its BiCG call at :505-507), on the mesh of
  * a tflows_calcps handle.  Covered: CONVEC, DIFFUS (CN or FI), CROSS = FI or
@@ -407,6 +420,9 @@ typedef struct tflows_uvw_params {
     double visc, dt, urf;
     int32_t blend[8], blend_tem[8];
     double urfc[8];
+    int32_t visc_rho;   /* 1: VISeff = visc * RhoS, RhoS = fF rho%n(c1) + (1 - fF) rho%n(c2) (the
+                           Moin_Problem() == 4 branch, NewUVW.f90:287-295, CalcZmix.f90:232-242);
+                           0: VISeff = visc */
 } tflows_uvw_params;
 int tflows_calcps_uvw_setup(tflows_calcps h, const double* scoef, const double* fF, const int32_t* material,
                             int32_t nmaterials, int32_t mem, void* stream);

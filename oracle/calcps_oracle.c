@@ -143,8 +143,8 @@ void oracle_calcps(int nc, int nbc, int ns, int nonz, int nmat, const int *sidec
 
 /*
  * oracle_coruvw -- Process/CorUVW.f90:36-121 and :183 (the correction
- * after the pressure solve; the CFL / Peclet monitor of :126-147 only feeds
- * the printed monitor lines and is left out):
+ * after the pressure solve; the CFL / Peclet monitor of :124-147 is
+ * oracle_cfl_peclet below):
  *
  *   :36-46    U = U - Px V / Asave / rho (both algorithm branches compute
  *             the same values), V, W likewise, cells 1..NC
@@ -210,6 +210,47 @@ double oracle_coruvw(int nc, int nbc, int ns, const int *sidec, const int *sidai
 }
 
 /*
+ * oracle_cfl_peclet -- the CFL and Peclet monitor of Process/CorUVW.f90:
+ * 124-147, after the correction:
+ *
+ *   for m = 1..Nmat: CFLmax(m) = PeMax(m) = 0, then over s = 1..NS with
+ *   material(c1) = m or material(c2) = m, and c2 > 0 or TypeBC(c2) = BUFFER:
+ *     RhoS = f(s) rho^n(c1) + (1 - f(s)) rho^n(c2)   (the geometric f)
+ *     CFLs = |dt Flux(s) / (RhoS Scoef(s) (Dx^2 + Dy^2 + Dz^2))|
+ *     PeS  = |Flux(s) / Scoef(s) / (VISc + TINY)|
+ *     CFLmax(m) = max(CFLmax(m), CFLs), PeMax(m) = max(PeMax(m), PeS)
+ *   (glomax is a no-op in one domain).
+ *
+ * material is (-NbC:NC), 1-based values; cflmax, pemax receive Nmat values.
+ */
+void oracle_cfl_peclet(int nc, int nbc, int ns, int nmat, const int *sidec, const int *typebc, const int *material,
+                       const double *f, const double *scoef, const double *dx, const double *dy, const double *dz,
+                       const double *rho_n, const double *flux, double dt, double visc, double *cflmax,
+                       double *pemax)
+{
+#define C(x, c) (x)[nbc + (c)]
+    (void)nc;
+    const double TINY = 1.0e-64;                                        /* Library/allp_mod.f90:31 */
+    for (int m = 1; m <= nmat; ++m) {
+        double cmax = 0.0, pmax = 0.0;
+        for (int s = 1; s <= ns; ++s) {
+            const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
+            if (C(material, c1) != m && C(material, c2) != m) continue;
+            if (!(c2 > 0 || (c2 < 0 && C(typebc, c2) == BC_BUFFER))) continue;
+            const double RhoS = f[s - 1] * C(rho_n, c1) + (1.0 - f[s - 1]) * C(rho_n, c2);
+            const double dd = dx[s - 1] * dx[s - 1] + dy[s - 1] * dy[s - 1] + dz[s - 1] * dz[s - 1];
+            const double CFLs = fabs(dt * flux[s - 1] / (RhoS * scoef[s - 1] * dd));
+            const double PeS = fabs(flux[s - 1] / scoef[s - 1] / (visc + TINY));
+            cmax = CFLs > cmax ? CFLs : cmax;
+            pmax = PeS > pmax ? PeS : pmax;
+        }
+        cflmax[m - 1] = cmax;
+        pemax[m - 1] = pmax;
+    }
+#undef C
+}
+
+/*
  * oracle_newuvw -- the momentum assembly of one velocity component,
  * Process/NewUVW.f90:137-499 (everything before the BiCG call at :507; the
  * shift of the old levels at :154-157 and the source terms at :180-199 are
@@ -239,7 +280,8 @@ double oracle_coruvw(int nc, int nbc, int ns, const int *sidec, const int *sidai
  * The flag and boundary codes are those of Process/IniPar.f90:24-32 and
  * Library/allp_mod.f90:16-24.  Per-material arrays BLEND, BLEND_TEM, URFC
  * are 1-based (material m at [m-1]); cell arrays (-NbC:NC) as elsewhere.
- * VISeff = VISc (the default branch of :287-295).  flux is the module's
+ * VISeff (:285-295): VISc, or with visc_rho = 1 the Moin_Problem() == 4
+ * branch VISc RhoS, RhoS = fF rho^n(c1) + (1-fF) rho^n(c2).  flux is the module's
  * mass flux Flux, whose sign picks ConvScheme's upwind cell.  A face whose
  * materials' BLEND values have max(...) = NO next to YES (for which the
  * reference leaves Phij undefined), or any other uncovered code, returns
@@ -353,7 +395,8 @@ int oracle_newuvw(int nc, int nbc, int ns, int nonz, int var, int convec, int di
                   const double *rho_o, const double *rho_oo, const double *ui_n, const double *ui_o,
                   const double *ui_oo, const double *source, const double *pi, const double *dUidi,
                   const double *dUidj, const double *dUidk, const double *dUjdi, const double *dUkdi,
-                  const double *dUjdj, const double *dUkdk, double visc, double dt, double urf, double *aval,
+                  const double *dUjdj, const double *dUkdk, double visc, int visc_rho, double dt, double urf,
+                  double *aval,
                   double *b, double *abou, double *asave, double *rhoui, double *cbuf, double *xbuf)
 {
 #define C(x, c) (x)[nbc + (c)]
@@ -419,7 +462,8 @@ int oracle_newuvw(int nc, int nbc, int ns, int nonz, int var, int convec, int di
         double A12 = 0.0, A21 = 0.0, D12 = 0.0, D21 = 0.0;
         const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
         const double ff = fF[s - 1], gg = 1.0 - fF[s - 1];
-        const double VISeff = visc;
+        const double RhoS = ff * C(rho_n, c1) + gg * C(rho_n, c2);       /* NewUVW :285, CalcZmix :232 */
+        const double VISeff = visc_rho ? visc * RhoS : visc;            /* Moin_Problem() == 4 */
         const double dUidiS = ff * C(dUidi, c1) + gg * C(dUidi, c2);
         const double dUidjS = ff * C(dUidj, c1) + gg * C(dUidj, c2);
         const double dUidkS = ff * C(dUidk, c1) + gg * C(dUidk, c2);
@@ -515,7 +559,8 @@ int oracle_newuvw(int nc, int nbc, int ns, int nonz, int var, int convec, int di
  *             PAR: b += 2 V/dt rho_o Z_o - V/(2 dt) rho_oo Z_oo + S V
  *   :386-387  under-relaxation with Zmix%URF
  *
- * VISeff = VIScZmix (the default of :236-242).  Per-material BLEND_TEM and
+ * VISeff (:232-242): VIScZmix, or with visc_rho = 1 the Moin_Problem() == 4
+ * branch VIScZmix RhoS, RhoS as for oracle_newuvw.  Per-material BLEND_TEM and
  * URFC_Tem at [m-1]; flux as for oracle_newuvw.  Returns -1 where a face's
  * max(BLEND_TEM) is NO next to YES or an uncovered code; abou is
  * (-NbC:-1); cbuf, xbuf hold NbC+NC+1 doubles.  Returns 0.
@@ -528,7 +573,8 @@ int oracle_calczmix(int nc, int nbc, int ns, int nonz, int convec, int diffus, i
                     const double *dx, const double *dy, const double *dz, const double *volume,
                     const double *rho_n, const double *rho_o, const double *rho_oo, const double *z_n,
                     const double *z_o, const double *z_oo, const double *source, const double *dZdx,
-                    const double *dZdy, const double *dZdz, double visc, double dt, double urf, double *aval,
+                    const double *dZdy, const double *dZdz, double visc, int visc_rho, double dt, double urf,
+                    double *aval,
                     double *b, double *abou, double *rhoz, double *cbuf, double *xbuf)
 {
 #define C(x, c) (x)[nbc + (c)]
@@ -595,7 +641,8 @@ int oracle_calczmix(int nc, int nbc, int ns, int nonz, int convec, int diffus, i
         double A12 = 0.0, A21 = 0.0, D12 = 0.0, D21 = 0.0;
         const int c1 = sidec[3 * (s - 1) + 1], c2 = sidec[3 * (s - 1) + 2];
         const double ff = fF[s - 1], gg = 1.0 - fF[s - 1];
-        const double VISeff = visc;
+        const double RhoS = ff * C(rho_n, c1) + gg * C(rho_n, c2);       /* NewUVW :285, CalcZmix :232 */
+        const double VISeff = visc_rho ? visc * RhoS : visc;            /* Moin_Problem() == 4 */
         const double dZdxS = ff * C(dZdx, c1) + gg * C(dZdx, c2);
         const double dZdyS = ff * C(dZdy, c1) + gg * C(dZdy, c2);
         const double dZdzS = ff * C(dZdz, c1) + gg * C(dZdz, c2);

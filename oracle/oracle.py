@@ -206,7 +206,8 @@ class Oracle(_Solvers):
         fn = self.lib.oracle_newuvw
         if not getattr(fn, "_typed", False):
             fn.restype = C.c_int
-            fn.argtypes = [C.c_int] * 9 + [_i32p] * 7 + [_f64p] * 28 + [C.c_double] * 3 + [_f64p] * 7
+            fn.argtypes = [C.c_int] * 9 + [_i32p] * 7 + [_f64p] * 28 + [C.c_double, C.c_int] + [C.c_double] * 2 + \
+                [_f64p] * 7
             fn._typed = True
         var = int(params["var"])
         a = _uvw_args(case, uf, var)
@@ -226,7 +227,8 @@ class Oracle(_Solvers):
                 *(c64(x) for x in a["S"]),
                 *(c64(x) for x in a["D"]), c64(case.volume), c64(a["rho_n"]), c64(a["rho_o"]), c64(a["rho_oo"]),
                 c64(a["ui_n"]), c64(a["ui_o"]), c64(a["ui_oo"]), c64(a["source"]), c64(a["pi"]),
-                *(c64(x) for x in a["g"]), float(params["visc"]), float(params["dt"]), float(params["urf"]),
+                *(c64(x) for x in a["g"]), float(params["visc"]), int(params.get("visc_rho", 0)),
+                float(params["dt"]), float(params["urf"]),
                 aval, b, abou, asave[case.nbc + 1:], rhoui, cbuf, xbuf)
         if rc != 0:
             raise ValueError("BLEND not covered by the restatement (a ConvScheme face scheme)")
@@ -243,7 +245,8 @@ def _calczmix(self, case, zf, params):
     fn = self.lib.oracle_calczmix
     if not getattr(fn, "_typed", False):
         fn.restype = C.c_int
-        fn.argtypes = [C.c_int] * 8 + [_i32p] * 6 + [_f64p] * 23 + [C.c_double] * 3 + [_f64p] * 6
+        fn.argtypes = [C.c_int] * 8 + [_i32p] * 6 + [_f64p] * 23 + [C.c_double, C.c_int] + [C.c_double] * 2 + \
+            [_f64p] * 6
         fn._typed = True
     nall = case.nbc + case.nc + 1
     c32 = lambda x: np.ascontiguousarray(x, np.int32)     # noqa: E731
@@ -260,8 +263,8 @@ def _calczmix(self, case, zf, params):
             c64(case.sz), c64(case.dx),
             c64(case.dy), c64(case.dz), c64(case.volume), c64(case.rho_n), c64(case.rho_o), c64(case.rho_oo),
             c64(zf["z"]), c64(zf["z_o"]), c64(zf["z_oo"]), c64(zf["src_z"]), c64(zf["zx"]), c64(zf["zy"]),
-            c64(zf["zz"]), float(params["visc"]), float(params["dt"]), float(params["urf"]), aval, b, abou, rhoz,
-            cbuf, xbuf)
+            c64(zf["zz"]), float(params["visc"]), int(params.get("visc_rho", 0)), float(params["dt"]),
+            float(params["urf"]), aval, b, abou, rhoz, cbuf, xbuf)
     if rc != 0:
         raise ValueError("BLEND_TEM not covered by the restatement (a ConvScheme face scheme)")
     return dict(aval=aval, b=b, abou=abou[:case.nbc], rhoz=rhoz)
@@ -292,6 +295,25 @@ def _calminmax(self, case, phi):
     return mx, mn
 
 
+def _cfl_peclet(self, case, material, nmat, scoef, rho_n, flux, dt, visc):
+    """The CFL and Peclet monitor of CorUVW.f90:124-147 (oracle/calcps_oracle.c)
+    on a meshgen.CalcPSCase: material ((-NbC:NC), 1-based), Scoef (NS),
+    rho_n ((-NbC:NC)), flux (NS).  Returns (cflmax, pemax), Nmat values
+    each."""
+    fn = self.lib.oracle_cfl_peclet
+    if not getattr(fn, "_typed", False):
+        fn.restype = None
+        fn.argtypes = [C.c_int] * 4 + [_i32p] * 3 + [_f64p] * 7 + [C.c_double] * 2 + [_f64p] * 2
+        fn._typed = True
+    c32 = lambda x: np.ascontiguousarray(x, np.int32)     # noqa: E731
+    c64 = lambda x: np.ascontiguousarray(x, np.float64)   # noqa: E731
+    cfl, pe = np.zeros(max(nmat, 1)), np.zeros(max(nmat, 1))
+    fn(case.nc, case.nbc, case.ns, nmat, c32(case.sidec.T), c32(case.typebc), c32(material), c64(case.f),
+       c64(scoef), c64(case.dx), c64(case.dy), c64(case.dz), c64(rho_n), c64(flux), float(dt), float(visc), cfl, pe)
+    return cfl[:nmat], pe[:nmat]
+
+
+Oracle.cfl_peclet = _cfl_peclet
 Oracle.calczmix = _calczmix
 Oracle.calminmax = _calminmax
 Oracle.zmix_update = _zmix_update

@@ -41,6 +41,10 @@ struct UvwParams {
     double visc = 0.0, dt = 1.0, urf = 1.0;
     int blend[kMaxMaterials] = {}, blend_tem[kMaxMaterials] = {};
     double urfc[kMaxMaterials] = {};
+    // 1: VISeff = visc * (fF rho%n(c1) + (1 - fF) rho%n(c2)) at each face, the
+    // Moin_Problem() == 4 branch of NewUVW.f90:287-295 and CalcZmix.f90:232-242;
+    // 0: VISeff = visc
+    int visc_rho = 0;
 };
 
 // The per-call momentum fields, device pointers; cell arrays (-NbC:NC).
@@ -107,6 +111,12 @@ public:
                        hipStream_t s);
     // Zmix%n(1:NC) = rhoZ / rho%n limited to [0, 1] (:401-405).
     void zmix_update(const double* rhoz, const double* rho_n, double* z_n, hipStream_t s);
+    // The CFL and Peclet numbers of CorUVW.f90:124-147, per material, after
+    // uvw_setup (which brings Scoef and the materials): rho_n (-NbC:NC) and
+    // the corrected Flux (NS), device pointers; cfl and pe receive
+    // nmaterials() values each on the host.
+    void cfl_peclet(double dt, double visc, const double* rho_n, const double* flux, double* cfl, double* pe,
+                    hipStream_t s);
     bool uvw_ready() const { return U_.ff != nullptr; }
     int nmaterials() const { return nmaterials_; }
     // how the momentum and scalar cell kernels visit their cells: 0 tiles in
@@ -140,6 +150,8 @@ private:
     bool offdiag_shared_ = false;       // two faces write one off-diagonal
     bool bcell_single_ = true;          // every boundary cell has at most one face
     double* minmax_ = nullptr;          // PHImax, PHImin of ConvScheme ((-NbC:NC) each)
+    double* cflpart_ = nullptr;         // block maxima of the CFL and Peclet numbers
+    double* cflout_ = nullptr;          // their totals, 2 per material
     int nmaterials_ = 0;
     int visit_order_ = 0;
     unsigned pairs_[kMaxMaterials] = {};  // material pairs that meet at a face

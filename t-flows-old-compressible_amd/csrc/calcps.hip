@@ -441,7 +441,7 @@ __device__ __forceinline__ UvwFace uvw_face(const CalcPSMesh& M, const UvwMesh& 
 #pragma unroll
     for (int k = 0; k < NG; ++k) gS[k] = ff * a.g[k] + gg * b.g[k];                  // :300-308
     const double Si = G.si[s], Sj = G.sj[s], Sk = G.sk[s];
-    const double visc = P.visc;                                                      // VISeff (:294)
+    const double visc = P.visc_rho ? P.visc * (ff * a.rho + gg * b.rho) : P.visc;  // VISeff (:285-295)
     if (SCALAR)                                                                      // CalcZmix.f90:250
         q.fex = visc * ((gS[0] * Si + gS[1] * Sj) + gS[2] * Sk);
     else                                                                             // :312-315
@@ -511,6 +511,80 @@ __global__ __launch_bounds__(kBlock) void k_uvw_minmax(CalcPSMesh M, UvwMesh U, 
         }
         pmax[i2] = mx;
         pmin[i2] = mn;
+    }
+}
+
+// The CFL and Peclet monitor of CorUVW.f90:124-147: for every material m,
+// the largest |dt Flux / (RhoS Scoef |D|^2)| and |Flux / Scoef / (VISc +
+// TINY)| over the interior and BUFFER faces with a cell of material m on
+// either side, RhoS = f rho%n(c1) + (1 - f) rho%n(c2) (the geometric f, as
+// the reference writes it).  A maximum is exact, so the block partials and
+// their total give the reference's values whatever the order.  Partials:
+// part[(2 m) G + b] the CFL, part[(2 m + 1) G + b] the Peclet number.
+__global__ __launch_bounds__(kBlock) void k_cfl_peclet(CalcPSMesh M, UvwMesh U, int nmat, double dt, double visc,
+                                                        const double* __restrict__ rho_n,
+                                                        const double* __restrict__ flux, double* __restrict__ part) {
+    double cfl[kMaxMaterials], pe[kMaxMaterials];
+#pragma unroll
+    for (int m = 0; m < kMaxMaterials; ++m) cfl[m] = pe[m] = 0.0;
+    const int nb = M.nbc;
+    for (int s = blockIdx.x * kBlock + threadIdx.x; s < M.ns; s += gridDim.x * kBlock) {
+        const int c2 = M.c2[s];
+        if (!(c2 > 0 || U.ukind[s] == kUvwBuffer)) continue;
+        const int i1 = nb + M.c1[s], i2 = nb + c2;
+        const double fs = M.f[s];
+        const double rhos = fs * rho_n[i1] + (1.0 - fs) * rho_n[i2];
+        const double dx = M.dx[s], dy = M.dy[s], dz = M.dz[s];
+        const double sc = U.scoef[s], fl = flux[s];
+        const double cfls = fabs(dt * fl / (rhos * sc * (dx * dx + dy * dy + dz * dz)));
+        const double pes = fabs(fl / sc / (visc + kTiny));
+        const int m1 = U.material[i1] - 1, m2 = U.material[i2] - 1;
+#pragma unroll
+        for (int m = 0; m < kMaxMaterials; ++m) {
+            if (m < nmat && (m == m1 || m == m2)) {
+                cfl[m] = cs_max(cfl[m], cfls);
+                pe[m] = cs_max(pe[m], pes);
+            }
+        }
+    }
+    __shared__ double sm[2 * kMaxMaterials][kWavesPerBlock];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+    for (int m = 0; m < kMaxMaterials; ++m) {
+        double a = cfl[m], b = pe[m];
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            a = cs_max(a, __shfl_xor(a, off, kWave));
+            b = cs_max(b, __shfl_xor(b, off, kWave));
+        }
+        if (lane == 0) {
+            sm[2 * m][w] = a;
+            sm[2 * m + 1][w] = b;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < 2 * nmat) {
+        double v = sm[threadIdx.x][0];
+#pragma unroll
+        for (int k = 1; k < kWavesPerBlock; ++k) v = cs_max(v, sm[threadIdx.x][k]);
+        part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+// out[j] = the largest of part[j G .. j G + G - 1], j < nvals (one block)
+__global__ __launch_bounds__(kBlock) void k_cfl_total(int nvals, int parts, const double* __restrict__ part,
+                                                       double* __restrict__ out) {
+    __shared__ double sm[kBlock];
+    for (int j = 0; j < nvals; ++j) {
+        double t = 0.0;
+        for (int q = threadIdx.x; q < parts; q += kBlock) t = cs_max(t, part[(size_t)j * parts + q]);
+        sm[threadIdx.x] = t;
+        __syncthreads();
+        for (int h = kBlock / 2; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) sm[threadIdx.x] = cs_max(sm[threadIdx.x], sm[threadIdx.x + h]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[j] = sm[0];
+        __syncthreads();
     }
 }
 
@@ -655,7 +729,9 @@ __global__ __launch_bounds__(kBlock) void k_uvw_cells(CalcPSMesh M, UvwMesh U, U
                     const double a12 = (P.blend[own.mat] != kBlendNo || P.blend[m2] != kBlendNo)
                                            ? 0.0 - (Fu < 0.0 ? Fu : 0.0) : 0.0;
                     if (k == kUvwWallLike) {
-                        const double d12 = P.diffus == kFlagFI ? 0.0 + (P.visc * U.scoef[s]) / rho2 : 0.0;
+                        const double ff = U.ff[s];                                       // VISeff (:285-295)
+                        const double visc = P.visc_rho ? P.visc * (ff * own.rho + (1.0 - ff) * rho2) : P.visc;
+                        const double d12 = P.diffus == kFlagFI ? 0.0 + (visc * U.scoef[s]) / rho2 : 0.0;
                         bb = bb + (a12 + d12) * ru2;                                     // :392
                     } else {
                         bb = bb + a12 * ru2;                                             // :399
@@ -957,6 +1033,8 @@ CalcPS::~CalcPS() {
     (void)hipFree(part_);
     (void)hipFree(out2_);
     (void)hipFree(minmax_);
+    (void)hipFree(cflpart_);
+    (void)hipFree(cflout_);
     for (void* p : staging_) (void)hipFree(p);
 }
 
@@ -1128,6 +1206,27 @@ void CalcPS::uvw_update(const double* rhoui, const double* rho_n, double* ui_n, 
     hipLaunchKernelGGL(k_uvw_update, dim3((nc_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nc_, nbc_, rhoui, rho_n,
                        ui_n);
     TF_CHECK(hipGetLastError());
+}
+
+void CalcPS::cfl_peclet(double dt, double visc, const double* rho_n, const double* flux, double* cfl, double* pe,
+                        hipStream_t s) {
+    if (!uvw_ready()) throw std::invalid_argument("the CFL and Peclet numbers need tflows_calcps_uvw_setup first");
+    const int nv = 2 * nmaterials_;
+    if (!cflpart_) {
+        TF_CHECK(hipMalloc(&cflpart_, sizeof(double) * (size_t)2 * kMaxMaterials * grid_));
+        TF_CHECK(hipMalloc(&cflout_, sizeof(double) * 2 * kMaxMaterials));
+    }
+    hipLaunchKernelGGL(k_cfl_peclet, dim3(grid_), dim3(kBlock), 0, s, M_, U_, nmaterials_, dt, visc, rho_n, flux,
+                       cflpart_);
+    hipLaunchKernelGGL(k_cfl_total, dim3(1), dim3(kBlock), 0, s, nv, grid_, (const double*)cflpart_, cflout_);
+    TF_CHECK(hipGetLastError());
+    double h[2 * kMaxMaterials];
+    TF_CHECK(hipMemcpyAsync(h, cflout_, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+    TF_CHECK(hipStreamSynchronize(s));
+    for (int m = 0; m < nmaterials_; ++m) {
+        cfl[m] = h[2 * m];
+        pe[m] = h[2 * m + 1];
+    }
 }
 
 }  // namespace tflows

@@ -620,6 +620,26 @@ int tflows_calcps_correct(tflows_calcps h, double dt, const double* pp, const do
     });
 }
 
+int tflows_calcps_cfl_peclet(tflows_calcps h, double dt, double visc, const double* rho_n, const double* flux,
+                             double* cflmax, double* pemax, int32_t mem, void* stream) {
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("null handle");
+        tflows::CalcPS& C = *h->cps;
+        if (!rho_n || (C.ns() > 0 && !flux) || !cflmax || !pemax) throw std::invalid_argument("null array");
+        const hipStream_t st = as_stream(stream);
+        if (mem == TFLOWS_MEM_DEVICE) {
+            C.cfl_peclet(dt, visc, rho_n, flux, cflmax, pemax, st);
+        } else {
+            const size_t ncell = (size_t)C.nbc() + C.nc() + 1;
+            double* d = C.staging(ncell + C.ns());
+            TF_CHECK(hipMemcpyAsync(d, rho_n, sizeof(double) * ncell, hipMemcpyHostToDevice, st));
+            if (C.ns() > 0)
+                TF_CHECK(hipMemcpyAsync(d + ncell, flux, sizeof(double) * C.ns(), hipMemcpyHostToDevice, st));
+            C.cfl_peclet(dt, visc, d, d + ncell, cflmax, pemax, st);
+        }
+    });
+}
+
 // ---------------------------------------------------------------------------
 // momentum assembly
 // ---------------------------------------------------------------------------
@@ -654,6 +674,7 @@ int tflows_calcps_uvw_assemble(tflows_calcps h, const tflows_uvw_params* p, cons
         P.cross = p->cross;
         P.inert = p->inert;
         P.visc = p->visc;
+        P.visc_rho = p->visc_rho;
         P.dt = p->dt;
         P.urf = p->urf;
         for (int m = 0; m < tflows::kMaxMaterials; ++m) {
@@ -744,6 +765,7 @@ int tflows_calcps_zmix_assemble(tflows_calcps h, const tflows_uvw_params* p, con
         P.cross = p->cross;
         P.inert = p->inert;
         P.visc = p->visc;
+        P.visc_rho = p->visc_rho;
         P.dt = p->dt;
         P.urf = p->urf;
         for (int m = 0; m < tflows::kMaxMaterials; ++m) {

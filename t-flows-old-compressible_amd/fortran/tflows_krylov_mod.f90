@@ -30,6 +30,7 @@ module tflows_krylov
     real(c_double)     :: visc, dt, urf
     integer(c_int32_t) :: blend(8), blend_tem(8)
     real(c_double)     :: urfc(8)
+    integer(c_int32_t) :: visc_rho = 0   ! 1: VISeff = visc * RhoS (Moin_Problem() == 4); 0: visc
   end type tflows_uvw_params
 
   interface
@@ -157,6 +158,17 @@ module tflows_krylov
       integer(c_int32_t), value :: mem
       integer(c_int32_t)        :: rc
     end function tflows_calcps_correct
+
+    ! CFLmax(1:Nmat), PeMax(1:Nmat) of CorUVW.f90:124-147 (after uvw_setup)
+    function tflows_calcps_cfl_peclet(h, dt, visc, rho_n, flux, cflmax, pemax, mem, stream) result(rc) &
+        bind(C, name="tflows_calcps_cfl_peclet")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value        :: h, rho_n, flux, stream
+      real(c_double), value     :: dt, visc
+      real(c_double)            :: cflmax(*), pemax(*)
+      integer(c_int32_t), value :: mem
+      integer(c_int32_t)        :: rc
+    end function tflows_calcps_cfl_peclet
 
     !--- momentum assembly (NewUVW.f90:137-499) -----------------------------
     function tflows_calcps_uvw_setup(h, scoef, ff, material, nmaterials, mem, stream) result(rc) &

@@ -203,6 +203,8 @@ def _bind_extensions(lib):
         lib.tflows_calcps_zmix_update.argtypes = [vp, vp, vp, vp, C.c_int32, vp]
         lib.tflows_calcps_uvw_update.restype = C.c_int
         lib.tflows_calcps_uvw_update.argtypes = [vp, vp, vp, vp, C.c_int32, vp]
+        lib.tflows_calcps_cfl_peclet.restype = C.c_int
+        lib.tflows_calcps_cfl_peclet.argtypes = [vp, C.c_double, C.c_double, vp, vp, vp, vp, C.c_int32, vp]
         lib.tflows_calcps_correct.restype = C.c_int
         lib.tflows_calcps_correct.argtypes = ([vp, C.c_double] + [vp] * 10 + [vp] * 5 + [C.POINTER(C.c_double)] +
                                               [C.c_int32, vp])
@@ -588,7 +590,8 @@ class UvwParams(C.Structure):
     """tflows_uvw_params: the flags and constants of one newuvw call."""
     _fields_ = [("var", C.c_int32), ("convec", C.c_int32), ("diffus", C.c_int32), ("cross", C.c_int32),
                 ("inert", C.c_int32), ("visc", C.c_double), ("dt", C.c_double), ("urf", C.c_double),
-                ("blend", C.c_int32 * 8), ("blend_tem", C.c_int32 * 8), ("urfc", C.c_double * 8)]
+                ("blend", C.c_int32 * 8), ("blend_tem", C.c_int32 * 8), ("urfc", C.c_double * 8),
+                ("visc_rho", C.c_int32)]
 
     @classmethod
     def from_dict(cls, p):
@@ -597,6 +600,7 @@ class UvwParams(C.Structure):
             setattr(q, k, int(p[k]))
         for k in ("visc", "dt", "urf"):
             setattr(q, k, float(p[k]))
+        q.visc_rho = int(p.get("visc_rho", 0))
         if len(p["blend"]) > 8:
             raise ValueError("at most 8 materials")
         for m in range(len(p["blend"])):
@@ -808,6 +812,7 @@ class DeviceCalcPS:
         torch.cuda.synchronize()
         _check(self.lib.tflows_calcps_uvw_setup(self.handle, _ptr(keep[0]), _ptr(keep[1]), _ptr(keep[2]),
                                                 int(uf["nmaterials"]), 1, None))
+        self.nmaterials = int(uf["nmaterials"])
 
     @staticmethod
     def uvw_inputs(args):
@@ -906,6 +911,16 @@ class DeviceCalcPS:
             None if mf is None else mf.ctypes.data, aval.ctypes.data,
             b.ctypes.data, abou.ctypes.data, rhoz.ctypes.data, 0, None))
         return dict(aval=aval, b=b, abou=abou[:c.nbc], rhoz=rhoz)
+
+    def cfl_peclet(self, rho_n, flux, dt, visc, stream=None):
+        """The CFL and Peclet numbers per material (CorUVW.f90:124-147) from
+        cuda float64 rho_n ((-NbC:NC)) and flux (NS), after uvw_setup.
+        Returns two numpy arrays of Nmat values."""
+        n = getattr(self, "nmaterials", 0)
+        cfl, pe = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        _check(self.lib.tflows_calcps_cfl_peclet(self.handle, float(dt), float(visc), _ptr(rho_n), _ptr(flux),
+                                                 cfl.ctypes.data, pe.ctypes.data, 1, _stream(stream)))
+        return cfl[:n], pe[:n]
 
     def zmix_update(self, rhoz, rho_n, z_n, stream=None):
         """Zmix%n(1:NC) = rhoZ / rho%n limited to [0, 1] on cuda tensors."""

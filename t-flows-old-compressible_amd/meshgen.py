@@ -1225,7 +1225,7 @@ def uvw_params(nmaterials: int = 1, **kw) -> dict:
     convection without the upwind part, URF 0.8)."""
     p = {"var": 1, "convec": FLAGS["FI"], "diffus": FLAGS["FI"], "cross": FLAGS["FI"], "inert": FLAGS["LIN"],
          "visc": 1e-2, "dt": 1e-2, "urf": 0.8, "blend": [FLAGS["NO"]] * nmaterials,
-         "blend_tem": [FLAGS["NO"]] * nmaterials, "urfc": [1.0] * nmaterials}
+         "blend_tem": [FLAGS["NO"]] * nmaterials, "urfc": [1.0] * nmaterials, "visc_rho": 0}
     p.update(kw)
     return p
 

@@ -16,6 +16,8 @@ needs gfortran's `irand`), so the restatement is pinned by:
   a volume-weighted mean of zero, face-order independence, and Nmat = 2
   doubling the face-loop terms.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -208,3 +210,43 @@ def test_coruvw_closed_box_continuity(pkg, oracle):
     want = asm["b_pre"] - A @ pp[case.nbc + 1:]
     scale = np.abs(asm["b_pre"]).max() + abs(A).max() * np.abs(pp).max() * 8
     np.testing.assert_allclose(out["b"] * V, want, rtol=0, atol=1e-13 * scale)
+
+
+def _monitor_case(pkg):
+    """The two-cell mesh set up for the CFL and Peclet monitor: material 1 on
+    cell 1 and its BUFFER cell -5, material 2 on cell 2; f 0.25 at the
+    interior face (RhoS = 0.25 * 1 + 0.75 * 3 = 2.5; 1.5 with the sides
+    swapped) and 0.5 at the BUFFER face (RhoS = 0.5 * 1 + 0.5 * 3 = 2);
+    Scoef 4 and 2 there, |D|^2 1 and 4; the four other boundary faces carry
+    a flux of 1000 that the monitor must not see."""
+    case = _two_cell_case(pkg)
+    at = lambda c: c + case.nbc                                         # noqa: E731
+    rho = case.rho_n.copy()
+    rho[at(1)], rho[at(2)], rho[at(-5)] = 1.0, 3.0, 3.0
+    f = case.f.copy()
+    f[0], f[5] = 0.25, 0.5
+    dx = case.dx.copy()
+    dx[0], dx[5] = 1.0, -2.0
+    material = np.ones(case.nbc + case.nc + 1, np.int32)
+    material[at(2)], material[at(-2)], material[at(-3)] = 2, 2, 2
+    scoef = np.array([4.0, 1.0, 1.0, 1.0, 1.0, 2.0])
+    flux = np.array([8.0, 1000.0, -1000.0, 1000.0, 1000.0, -48.0])
+    return dataclasses.replace(case, rho_n=rho, f=f, dx=dx), material, scoef, flux
+
+
+def test_cfl_peclet_hand_values(pkg, oracle):
+    """CorUVW.f90:124-147 with dt 0.5, VISc 0.25: interior face CFL |0.5 * 8 /
+    (2.5 * 4 * 1)| = 0.4, Pe |8 / 4 / 0.25| = 8; BUFFER face CFL |0.5 * -48 /
+    (2 * 2 * 4)| = 1.5, Pe |-48 / 2 / 0.25| = 96.  Material 1 sees both faces
+    (cell 1 is c1 of each), material 2 only the interior one (its c2).  With
+    VISc = 0 the Peclet number divides by TINY = 1e-64 and stays finite."""
+    case, material, scoef, flux = _monitor_case(pkg)
+    cfl, pe = oracle.cfl_peclet(case, material, 2, scoef, case.rho_n, flux, 0.5, 0.25)
+    assert cfl.tolist() == [1.5, 0.4]
+    assert pe.tolist() == [96.0, 8.0]
+    cfl0, pe0 = oracle.cfl_peclet(case, material, 2, scoef, case.rho_n, flux, 0.5, 0.0)
+    assert cfl0.tolist() == [1.5, 0.4]
+    assert pe0.tolist() == [24.0 / 1e-64, 2.0 / 1e-64]
+    # a material with no face of its own reads 0
+    cfl3, pe3 = oracle.cfl_peclet(case, material, 3, scoef, case.rho_n, flux, 0.5, 0.25)
+    assert cfl3.tolist() == [1.5, 0.4, 0.0] and pe3.tolist() == [96.0, 8.0, 0.0]

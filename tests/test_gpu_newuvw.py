@@ -70,12 +70,13 @@ def _params(pkg, name, nmat, var):
                               urfc=[0.9, 0.7][:nmat]),
         "upwind_part_only": dict(blend_tem=[YES] * nmat, convec=1, cross=1, urf=1.0),
         "no_diffusion": dict(diffus=1, blend=[YES] * nmat, inert=0, visc=0.05),
+        "density_weighted_visc": dict(visc_rho=1, blend=[YES] * nmat, blend_tem=[YES] * nmat, visc=0.03),
     }
     return mg.uvw_params(nmaterials=nmat, var=var, **sets[name])
 
 
 CASES = ["two_cell", "channel", "all_types_shuffled", "two_materials", "split_faces"]
-FLAGS = ["implicit", "upwind_par_cn", "upwind_part_only", "no_diffusion"]
+FLAGS = ["implicit", "upwind_par_cn", "upwind_part_only", "no_diffusion", "density_weighted_visc"]
 
 
 def _run(pkg, case, uf, p):
@@ -276,3 +277,49 @@ def test_conv_scheme_bitwise_block_order(pkg, oracle, cuda, scheme, var):
     assert np.array_equal(got["abou"][:case.nbc], want["abou"])
     dc.close()
 
+
+
+@pytest.mark.parametrize("name", CASES + ["channel_64x64x40"])
+def test_cfl_peclet_monitor(pkg, oracle, cuda, name):
+    """The CFL and Peclet monitor of CorUVW.f90:124-147 on the GPU, per
+    material, bit for bit the oracle's (a maximum is exact in any order), on
+    every mesh of the momentum tests, with fF different from the geometric f
+    the monitor uses; from device and from host arrays."""
+    import ctypes
+    import torch
+    if name == "channel_64x64x40":
+        case = pkg.meshgen.calcps_case((64, 64, 40), seed=81)
+        uf = pkg.meshgen.uvw_fields(case, seed=82)
+    else:
+        case, uf = _case(pkg, name)
+    nmat = uf["nmaterials"]
+    flux = np.asarray(uf["flux_u"], float) * np.linspace(-1.5, 2.0, case.ns)
+    dc = pkg.krylov.DeviceCalcPS(case)
+    dc.uvw_setup(uf)
+    for dt, visc in ((0.01, 0.02), (0.5, 0.0)):
+        want = oracle.cfl_peclet(case, uf["material"], nmat, uf["scoef"], case.rho_n, flux, dt, visc)
+        rho = torch.from_numpy(np.ascontiguousarray(case.rho_n, np.float64)).cuda()
+        fl = torch.from_numpy(flux).cuda()
+        got = dc.cfl_peclet(rho, fl, dt, visc)
+        print(f"case {name} dt {dt} visc {visc} cfl {got[0]} pe {got[1]}", flush=True)
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+        cfl, pe = np.zeros(nmat), np.zeros(nmat)
+        rc = dc.lib.tflows_calcps_cfl_peclet(dc.handle, dt, visc, np.ascontiguousarray(case.rho_n).ctypes.data,
+                                             flux.ctypes.data, cfl.ctypes.data, pe.ctypes.data, 0, None)
+        assert rc == 0
+        assert np.array_equal(cfl, want[0]) and np.array_equal(pe, want[1])
+    assert np.all(want[0] >= 0) and np.any(want[0] > 0)
+    dc.close()
+
+
+def test_cfl_peclet_needs_setup(pkg, cuda):
+    """Without tflows_calcps_uvw_setup there are no Scoef and materials: the
+    call is refused with an error, not computed on missing arrays."""
+    import torch
+    case, uf = _case(pkg, "channel")
+    dc = pkg.krylov.DeviceCalcPS(case)
+    rho = torch.from_numpy(np.ascontiguousarray(case.rho_n, np.float64)).cuda()
+    fl = torch.from_numpy(np.asarray(uf["flux_u"], float)).cuda()
+    with pytest.raises(pkg.krylov.TFlowsError, match="uvw_setup"):
+        dc.cfl_peclet(rho, fl, 0.1, 0.01)
+    dc.close()

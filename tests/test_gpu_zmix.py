@@ -35,11 +35,13 @@ def _zparams(pkg, name, nmat):
         "upwind_par_cn": dict(blend_tem=[YES] * nmat, inert=PAR, diffus=CN, urf=0.6, urfc=[0.9, 0.7][:nmat]),
         "upwind_part_only": dict(blend_tem=[YES] * nmat, convec=1, cross=1, urf=1.0),
         "no_diffusion": dict(diffus=1, blend_tem=[YES] * nmat, inert=0, visc=0.05),
+        "density_weighted_visc": dict(visc_rho=1, blend_tem=[YES] * nmat, visc=0.03),
     }
     return pkg.meshgen.uvw_params(nmaterials=nmat, **sets[name])
 
 
-@pytest.mark.parametrize("flags", ["implicit", "upwind_par_cn", "upwind_part_only", "no_diffusion"])
+@pytest.mark.parametrize("flags", ["implicit", "upwind_par_cn", "upwind_part_only", "no_diffusion",
+                                   "density_weighted_visc"])
 @pytest.mark.parametrize("name", CASES)
 def test_assembly_bitwise(pkg, oracle, cuda, name, flags):
     case, zf = _zcase(pkg, name)

@@ -449,3 +449,45 @@ def test_blend_schemes_refused(pkg, oracle):
     oracle.newuvw(case, uf, mg.uvw_params(nmaterials=2, blend=[YES, YES]))
     for sch in SCHEMES:
         oracle.newuvw(case, uf, mg.uvw_params(nmaterials=2, blend=[sch, NO]))
+
+
+def _rt_case(pkg):
+    """The hand mesh with densities and fF such that every face has RhoS =
+    fF rho(c1) + (1 - fF) rho(c2) = 2.5 exactly: rho = 1 and 3 on the cells,
+    4 on the boundary cells of cell 1 (-1 OUTFLOW, -4 SYMMETRY, -5 BUFFER), 2
+    on those of cell 2 (-2 INFLOW, -3 PRESSURE); fF 0.25 at the interior face
+    and 0.5 at the others.  With the sides swapped the interior face would
+    give 1.5, with the geometric f instead of fF something else again."""
+    case = _hand_case(pkg)
+    at = lambda c: c + case.nbc                                         # noqa: E731
+    rho = case.rho_n.copy()
+    rho[at(1)], rho[at(2)] = 1.0, 3.0
+    for c, r in ((-1, 4.0), (-2, 2.0), (-3, 2.0), (-4, 4.0), (-5, 4.0)):
+        rho[at(c)] = r
+    case = dataclasses.replace(case, rho_n=rho, dt=0.5)
+    grads = {"ux": 1.0, "uy": -2.0, "uz": 0.5, "vx": 3.0, "vy": 0.25, "wx": -1.0, "wz": 2.0}
+    uf = _hand_fields(case, [3.0, 1.0, -2.0, 0.5, 0.0, -1.0], grads)
+    uf["ff"] = np.array([0.25, 0.5, 0.5, 0.5, 0.5, 0.5])
+    assert not np.array_equal(uf["ff"], case.f)
+    return case, uf
+
+
+@pytest.mark.parametrize("var", [1, 2, 3])
+def test_density_weighted_viscosity(pkg, oracle, var):
+    """The Moin_Problem() == 4 branch (NewUVW.f90:285-295): VISeff = VISc
+    RhoS.  Where every face has RhoS = 2.5, the assembly with visc_rho = 1
+    and VISc = 0.125 is bit for bit the plain one with VISc = 0.3125 (both
+    multiply to the same double): the branch scales both Fex and A0, with fF
+    and the sides in the reference's roles.  Without the branch the
+    diffusion terms differ."""
+    mg = pkg.meshgen
+    case, uf = _rt_case(pkg)
+    base = dict(var=var, visc=0.125, dt=0.5, urf=0.5, blend=[YES], blend_tem=[YES])
+    rt = oracle.newuvw(case, uf, mg.uvw_params(visc_rho=1, **base))
+    scaled = oracle.newuvw(case, uf, mg.uvw_params(**dict(base, visc=0.3125)))
+    plain = oracle.newuvw(case, uf, mg.uvw_params(**base))
+    for k in ("aval", "b", "abou", "asave", "rhoui"):
+        assert np.array_equal(rt[k], scaled[k]), k
+    assert not np.array_equal(rt["aval"], plain["aval"])
+    assert not np.array_equal(rt["b"], plain["b"])
+

@@ -228,3 +228,26 @@ def test_blend_tem_schemes_refused(pkg, oracle):
     oracle.calczmix(case, zf, mg.uvw_params(nmaterials=2, blend_tem=[NO, NO], blend=[21, 21]))
     for sch in SCHEMES:
         oracle.calczmix(case, zf, mg.uvw_params(nmaterials=2, blend_tem=[sch, NO]))
+
+
+def test_density_weighted_viscosity(pkg, oracle):
+    """The Moin_Problem() == 4 branch of CalcZmix.f90:232-242 (VISeff =
+    VIScZmix RhoS) on the mesh of the momentum test where every face has
+    RhoS = 2.5: visc_rho = 1 with 0.125 is bit for bit the plain assembly
+    with 0.3125, and differs from the plain one with 0.125."""
+    from test_newuvw_oracle import _rt_case
+    mg = pkg.meshgen
+    case, uf = _rt_case(pkg)
+    zf = _hand_zfields(case, uf["flux_u"])
+    zf["ff"] = uf["ff"]
+    for k, g in zip(("zx", "zy", "zz"), (0.5, -1.0, 0.25)):
+        zf[k] = np.full_like(zf["z"], g)
+    base = dict(visc=0.125, dt=0.5, urf=0.5, blend_tem=[YES])
+    rt = oracle.calczmix(case, zf, mg.uvw_params(visc_rho=1, **base))
+    scaled = oracle.calczmix(case, zf, mg.uvw_params(**dict(base, visc=0.3125)))
+    plain = oracle.calczmix(case, zf, mg.uvw_params(**base))
+    for k in ("aval", "b", "abou", "rhoz"):
+        assert np.array_equal(rt[k], scaled[k]), k
+    assert not np.array_equal(rt["aval"], plain["aval"])
+    assert not np.array_equal(rt["b"], plain["b"])
+
