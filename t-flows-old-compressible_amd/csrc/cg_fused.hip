@@ -104,10 +104,7 @@ __device__ __forceinline__ bool stop_decision(int kd, T rr, const StopIn& in, co
         }
     }
     if (MIXED && !authoritative && err < kReplaceDrop * in.err_rep) {
-        if (leader()) {
-            st->suspend = 1;
-            publish(hm, 0, kd - 1);
-        }
+        if (leader()) st->suspend = 1;
         return true;
     }
     if (err < tol) {
@@ -118,7 +115,6 @@ __device__ __forceinline__ bool stop_decision(int kd, T rr, const StopIn& in, co
                 publish(hm, 1, kd - 1);
             } else {
                 st->suspend = 1;
-                publish(hm, 0, kd - 1);
             }
         }
         return true;
@@ -131,7 +127,6 @@ __device__ __forceinline__ bool stop_decision(int kd, T rr, const StopIn& in, co
         }
         return true;
     }
-    if (leader()) publish(hm, 0, kd - 1);
     return false;
 }
 

@@ -49,9 +49,12 @@ struct SolverState {
     int diverged;
 };
 
-// Host-visible mirror of the progress, written by the device with
+// Host-visible mirror of the end of a solve, written by the device with
 // system-scope stores into coherent pinned memory so the host can pace its
-// launches without events.
+// launches without events.  Only the stopping test that ends a solve writes
+// it (done = 1 and the last iteration): a system-scope release makes the
+// writing wave write back its XCD's L2 and wait for it, which is not worth
+// paying every iteration for a flag that stays 0.
 struct HostMirror {
     volatile int done;
     volatile int iter;

@@ -122,7 +122,6 @@ __device__ __forceinline__ bool check_stop(const BatchCtx& ctx, int k, const T* 
         }
         return true;
     }
-    if (lead) publish(hm, 0, k - 1);
     return false;
 }
 
