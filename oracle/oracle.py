@@ -5,8 +5,7 @@ bench.py's CPU-baseline leg, never by the product library.
 
 * :class:`Oracle` -- the C restatement ``liboracle.so`` (krylov_oracle.c),
   which follows Process/CG.f90, BiCG.f90, CGS.f90, Prec1.f90, Resid.f90
-  operation by operation; it runs wherever gcc built it, the GPU box
-  included.
+  operation by operation; it runs wherever gcc built it, on any host.
 * :class:`Reference` -- the reference's own Fortran solvers compiled from
   their sources into ``_ref/libtflows_ref.so`` by ``oracle/Makefile``.
 
