@@ -323,3 +323,23 @@ def test_cfl_peclet_needs_setup(pkg, cuda):
     with pytest.raises(pkg.krylov.TFlowsError, match="uvw_setup"):
         dc.cfl_peclet(rho, fl, 0.1, 0.01)
     dc.close()
+
+
+@pytest.mark.parametrize("shape,order", [((64, 64, 40), 2), ((64, 64, 41), 1), ((32, 64, 16), 1),
+                                         ((128, 2, 4), 2), ((12, 10, 8), 0)])
+def test_visit_order_selection(pkg, oracle, cuda, shape, order):
+    """Which visiting order a mesh takes: blocks of 64 x 2 x 2 cells where
+    lines are whole wavefronts and lines and planes come in pairs; else tiles
+    brick by brick where planes are whole tiles; else tiles in order.  The
+    outputs are the oracle's bit for bit in each case."""
+    mg = pkg.meshgen
+    case = mg.calcps_case(shape, seed=91)
+    uf = mg.uvw_fields(case, seed=92)
+    p = _params(pkg, "implicit", uf["nmaterials"], 2)
+    dc, args, got = _run(pkg, case, uf, p)
+    print(f"shape {shape} visit order {dc.visit_order()}", flush=True)
+    assert dc.visit_order() == order
+    want = oracle.newuvw(case, uf, p)
+    for k in ("aval", "b", "rhoui"):
+        assert np.array_equal(got[k], want[k]), k
+    dc.close()
