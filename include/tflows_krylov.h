@@ -306,6 +306,17 @@ int tflows_batch_config(tflows_batch b, int32_t bits, int32_t method, int32_t pr
 int tflows_batch_set_form(tflows_batch b, int32_t form);
 /* The form (1 or 2) in use for (bits, method, prec); -1 on error. */
 int32_t tflows_batch_form(tflows_batch b, int32_t bits, int32_t method, int32_t prec);
+/* How the systems share the iterations (form 1, prec 0 or 1).  1: every
+ * system from the first iteration.  2: consecutive systems packed into
+ * groups whose operators and vectors fit 192 MB (3/4 of the 256 MB MALL),
+ * each group on the whole grid, one group after another; a system's
+ * iterations count from its group's start.  0 (default): the autotuner times
+ * both on the caller's systems and keeps the faster.  Bits may differ
+ * between the two, as between grids.  No counterpart in the reference. */
+int tflows_batch_set_groups(tflows_batch b, int32_t mode);
+/* The number of groups the solves of (bits, method, prec) run in (1: all
+ * at once); -1 on error. */
+int32_t tflows_batch_groups(tflows_batch b, int32_t bits, int32_t method, int32_t prec);
 /* FP64 solve of all K systems with `method` / `prec` (as tflows_solve):
  * x, r1 device arrays (r1 in: b, out: final residual); cap[k], tol[k] per
  * system (host); niter[k], ini_res[k] (written unless the system took the

@@ -22,6 +22,7 @@ struct BatchConfig {
     int lanes = 0;   // SpMV form: lanes per row (1, 2, 4, 8)
     int chunk = 0;   // iterations per graph replay
     int form = 0;    // 1: one launch per kernel; 2: one persistent launch per chunk (CG, Jacobi / none)
+    int groups = 0;  // 1: every system from the start; 2: in groups that fit the MALL, one after another
 };
 enum { kFormKernels = 1, kFormPersistent = 2 };
 
@@ -62,6 +63,12 @@ class Batch {
     // 0: the autotuner picks the form; 1 / 2 fix it (2 applies where the
     // persistent form exists: CG with prec 0 / 1; elsewhere 1 runs)
     void set_form(int form) { form_ = form; }
+    // 0: the autotuner decides; 1: every system from the first iteration;
+    // 2: systems in groups whose working sets fit the 256 MB MALL, one group
+    // after another (form 1, Jacobi or no preconditioner)
+    void set_groups(int mode) { groups_ = mode; }
+    // the number of groups the solves of (bits, method, prec) run in
+    int groups_in_use(int bits, int method, int prec) const;
     // false once a persistent launch gave up waiting on its barriers; the
     // batch then keeps the per-kernel form
     bool persistent_ok() const { return persistent_ok_; }
@@ -111,6 +118,9 @@ class Batch {
     template <typename T>
     void rebalance(const std::vector<char>& active, int grid, T* partials, SolverState* st);
     void tile_norms(const double* v, std::vector<double>& out);
+    // consecutive systems packed while their working sets (the operator and
+    // the method's vectors) stay within mall_budget_ bytes
+    std::vector<std::vector<int>> mall_groups(int bits, int method) const;
     // caller order <-> stored order of a batch vector (on the batch stream)
     void gather_in(const double* ext, double* in);
     void scatter_out(const double* in, double* ext);
@@ -120,8 +130,9 @@ class Batch {
     std::vector<int> roff_, eoff_, tiles_, tile0_;
     std::unique_ptr<System> sys_;      // the block-diagonal matrix
     hipStream_t bs_ = nullptr;         // stream of every batch launch (capturable)
-    DevArray<int> tile_r0_, tile_sys_, maps_, kbase_;
+    DevArray<int> tile_r0_, tile_sys_, maps_, kbase_, koff_;
     int* h_maps_ = nullptr;            // pinned staging of the block maps
+    int* h_koff_ = nullptr;            // pinned staging of the per-system iteration offsets
     int map_grid_ = 0;
     int gmax_ = 0;                     // largest grid (size of the map arrays)
     DevArray<double> ws_, partials_, tnorm_, scale_;
@@ -146,6 +157,11 @@ class Batch {
     DevArray<double> val_orig_, val_tmp_;
     int rebalances_ = 0;
     int form_ = 0;                     // fixed form (0: tuned)
+    int groups_ = 0;                   // fixed grouping (0: tuned)
+    // working-set budget of a group: the MALL (Infinity Cache) holds 256 MB,
+    // a group's operator and vectors are kept within 3/4 of it so that they
+    // stay resident from one iteration to the next while the group runs
+    double mall_budget_ = 192.0 * 1024 * 1024;
     bool persistent_ok_ = true;
     DevArray<int> bar_, pfault_;       // barrier counters (one per system) and the fault word of the persistent form
     uint64_t wait_ticks_ = 0;

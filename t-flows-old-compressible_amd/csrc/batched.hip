@@ -246,7 +246,14 @@ Batch::Batch(int k, const int* n, const int* nonz, const int* const* acol, const
     const size_t map_ints = 3 * (size_t)gmax_ + k + 1;
     maps_.ensure(map_ints);
     TF_CHECK(hipHostMalloc((void**)&h_maps_, sizeof(int) * map_ints, hipHostMallocDefault));
+    TF_CHECK(hipHostMalloc((void**)&h_koff_, sizeof(int) * k, hipHostMallocDefault));
+    // TFLOWS_BATCH_GROUP_MB: the working-set budget of a group (default 192)
+    if (const char* e = std::getenv("TFLOWS_BATCH_GROUP_MB")) {
+        const double mb = std::atof(e);
+        if (mb > 0.0) mall_budget_ = mb * 1024.0 * 1024.0;
+    }
     kbase_.ensure(1);
+    koff_.ensure(k);
     const size_t nn = (size_t)n_tot_;
     ws_.ensure((V_COUNT + 3) * nn);
     ws32_.ensure((V_COUNT + 2) * nn);
@@ -281,6 +288,7 @@ Batch::~Batch() {
     for (auto& e : ev_)
         if (e) (void)hipEventDestroy(e);
     if (h_maps_) (void)hipHostFree(h_maps_);
+    if (h_koff_) (void)hipHostFree(h_koff_);
     if (hm_) (void)hipHostFree(hm_);
     if (hm32_) (void)hipHostFree(hm32_);
     if (snap_) (void)hipHostFree(snap_);
@@ -388,6 +396,7 @@ BatchCtx Batch::ctx(int lanes, bool with_kbase) const {
     c.tile_r0 = tile_r0_.p;
     c.sys_blk0 = maps_.p + 3 * gmax_;
     c.kbase = with_kbase ? kbase_.p : nullptr;
+    c.koff = with_kbase ? koff_.p : nullptr;
     c.lanes = lanes;
     return c;
 }
@@ -403,7 +412,32 @@ BatchConfig Batch::config_for(int bits, int method, int prec) const {
     if (c.grid) c.grid = std::min(gmax_, std::max(round8(k_), round8(c.grid)));
     if (form_) c.form = form_;
     if (!c.form || method != 0 || prec == 2 || !persistent_ok_) c.form = kFormKernels;
+    if (groups_) c.groups = groups_;
+    if (c.groups != 2 || prec == 2 || c.form != kFormKernels) c.groups = 1;
     return c;
+}
+
+int Batch::groups_in_use(int bits, int method, int prec) const {
+    return config_for(bits, method, prec).groups == 2 ? (int)mall_groups(bits, method).size() : 1;
+}
+
+std::vector<std::vector<int>> Batch::mall_groups(int bits, int method) const {
+    const double vb = bits == 64 ? 8.0 : 4.0;
+    // vectors a solve touches per row (x, r, p, q, D and the method's extras)
+    const double vecs = method == 0 ? 5.0 : (method == 1 ? 8.0 : 10.0);
+    std::vector<std::vector<int>> g;
+    double used = 0.0;
+    for (int i = 0; i < k_; ++i) {
+        const double rows = roff_[i + 1] - roff_[i], ents = eoff_[i + 1] - eoff_[i];
+        const double ws = rows * vecs * vb + ents * (vb + 4.0);
+        if (g.empty() || used + ws > mall_budget_) {
+            g.emplace_back();
+            used = 0.0;
+        }
+        g.back().push_back(i);
+        used += ws;
+    }
+    return g;
 }
 
 template <typename Op>
@@ -570,6 +604,35 @@ BatchConfig Batch::tune(int bits, int method, int prec, const Op& Aop, const Csr
             }
         }
     }
+    // the same configuration with the systems in MALL-sized groups, each
+    // group on the whole grid in turn: kept if the groups' probes together
+    // take less time than the probe of every system at once
+    best.groups = 1;
+    const std::vector<std::vector<int>> groups = mall_groups(bits, method);
+    if (groups_ == 0 && prec != 2 && best.form == kFormKernels && best.grid && groups.size() > 1) {
+        const int G = best.grid;
+        float sum = 0.0f;
+        for (const auto& g : groups) {
+            std::vector<char> m(k_, 0);
+            for (int i : g) m[i] = 1;
+            TF_CHECK(hipStreamSynchronize(bs_));
+            build_maps(m, G);
+            upload_maps(bs_);
+            const BatchCtx c = ctx(best.lanes, false);
+            const Sites<T> S = Sites<T>::carve(partials, G);
+            prepare();
+            TF_CHECK(hipMemcpyAsync(st, init.data(), sizeof(SolverState) * k_, hipMemcpyHostToDevice, bs_));
+            start_solve<T, Op>(c, method, prec, Aop, A, w, S, st, hm, L, bs_, d_ready, zero_x);
+            TF_CHECK(hipEventRecord(ev_[2], bs_));
+            for (int i = 1; i <= probe; ++i) enqueue_iteration<T, Op>(c, method, jac, i, Aop, A, w, S, st, hm, L, bs_);
+            TF_CHECK(hipEventRecord(ev_[3], bs_));
+            TF_CHECK(hipEventSynchronize(ev_[3]));
+            float ms = 0.0f;
+            TF_CHECK(hipEventElapsedTime(&ms, ev_[2], ev_[3]));
+            sum += ms;
+        }
+        if (sum < best_ms) best.groups = 2;
+    }
     return best;
 }
 
@@ -613,10 +676,42 @@ void Batch::run(int bits, int method, int prec, const Op& Aop, const Csr<T>& A, 
     TF_CHECK(hipGetLastError());
     (void)jac;
 
-    int maxcap = 0;
-    for (const auto& h : init) maxcap = std::max(maxcap, h.cap);
-    const long long limit = (long long)maxcap + 1 + 2LL * C;
+    // Groups (cfg.groups == 2): the systems run group after group, each
+    // group on the whole grid; a group starts with the chunk after the one
+    // whose snapshot showed the previous group done, and its systems count
+    // their iterations from there (koff).
+    const std::vector<std::vector<int>> groups =
+        cfg.groups == 2 ? mall_groups(bits, method) : std::vector<std::vector<int>>{};
+    const bool grouped = groups.size() > 1;
+    std::vector<int> gid(k_, 0);
+    for (size_t g = 0; g < groups.size(); ++g)
+        for (int i : groups[g]) gid[i] = (int)g;
+    for (int i = 0; i < k_; ++i) h_koff_[i] = 0;
+    TF_CHECK(hipMemcpyAsync(koff_.p, h_koff_, sizeof(int) * k_, hipMemcpyHostToDevice, bs_));
+    int cur = 0;
+    auto live_in = [&](int g, const int* done) {
+        std::vector<char> m(k_, 0);
+        for (int i : groups[g]) m[i] = done ? done[i] == 0 : 1;
+        return m;
+    };
     int64_t mapped = tile0_[k_];
+    long long limit = 0;
+    if (grouped) {
+        for (const auto& g : groups) {
+            int gcap = 0;
+            for (int i : g) gcap = std::max(gcap, init[i].cap);
+            limit += (long long)gcap + 1 + 2LL * C;
+        }
+        limit += 2LL * C;
+        act = live_in(0, nullptr);
+        rebalance<T>(act, G, partials, st);
+        mapped = 0;
+        for (int i : groups[0]) mapped += tiles_[i];
+    } else {
+        int maxcap = 0;
+        for (const auto& h : init) maxcap = std::max(maxcap, h.cap);
+        limit = (long long)maxcap + 1 + 2LL * C;
+    }
     bool pending[2] = {false, false};
     int slot = 0;
     long long issued = 0, launched = 0;
@@ -637,14 +732,26 @@ void Batch::run(int bits, int method, int prec, const Op& Aop, const Csr<T>& A, 
             int64_t running = 0;
             all_done = true;
             for (int i = 0; i < k_; ++i) {
-                act[i] = sn[i] == 0;
-                if (act[i]) {
-                    all_done = false;
-                    running += tiles_[i];
-                }
+                if (sn[i] == 0) all_done = false;
+                act[i] = sn[i] == 0 && (!grouped || gid[i] == cur);
+                if (act[i]) running += tiles_[i];
             }
             if (all_done) break;
-            if (rebalance_ && running * 10 <= mapped * 7) {
+            if (grouped && running == 0) {
+                // the group is done: the next group with a running system
+                // takes the grid from the next chunk on
+                do ++cur;
+                while (cur < (int)groups.size() &&
+                       std::none_of(groups[cur].begin(), groups[cur].end(), [&](int i) { return sn[i] == 0; }));
+                if (cur >= (int)groups.size()) break;   // not reached: all_done covers it
+                for (int i : groups[cur]) h_koff_[i] = (int)(launched * C);
+                TF_CHECK(hipMemcpyAsync(koff_.p, h_koff_, sizeof(int) * k_, hipMemcpyHostToDevice, bs_));
+                act = live_in(cur, sn);
+                for (int i : groups[cur])
+                    if (act[i]) running += tiles_[i];
+                rebalance<T>(act, G, partials, st);
+                mapped = running;
+            } else if (rebalance_ && running * 10 <= mapped * 7) {
                 rebalance<T>(act, G, partials, st);
                 mapped = running;
             }

@@ -436,6 +436,20 @@ int tflows_batch_set_form(tflows_batch b, int32_t form) {
     });
 }
 
+int tflows_batch_set_groups(tflows_batch b, int32_t mode) {
+    return guarded([&] {
+        if (mode < 0 || mode > 2)
+            throw std::invalid_argument("groups must be 0 (tuned), 1 (all systems at once) or 2 (MALL-sized groups)");
+        b->batch->set_groups(mode);
+    });
+}
+
+int32_t tflows_batch_groups(tflows_batch b, int32_t bits, int32_t method, int32_t prec) {
+    int32_t v = -1;
+    if (guarded([&] { v = b->batch->groups_in_use(bits, tflows::kernel_method(method), prec); })) return -1;
+    return v;
+}
+
 int32_t tflows_batch_form(tflows_batch b, int32_t bits, int32_t method, int32_t prec) {
     int32_t v = -1;
     if (guarded([&] { v = b->batch->config_for(bits, tflows::kernel_method(method), prec).form; })) return -1;

@@ -78,7 +78,9 @@ __device__ __forceinline__ void publish(HostMirror* hm, int done, int iter) {
 //   sys_blk0[k..k+1] blocks of system k (its reduction range; the first is
 //                    its leader)
 // kbase (optional) offsets the launch's iteration index by a device value,
-// so one captured graph of a chunk of iterations can be replayed.
+// so one captured graph of a chunk of iterations can be replayed; koff
+// (optional, per system) is subtracted from it, so a system that starts
+// iterating after others (a later group of a batch) counts from 1.
 // `lanes` selects the SpMV form of the launch: 1 = one row per thread with
 // the tile's products staged in LDS (64 rows per wavefront), 2/4/8 = that
 // many lanes per row streaming col/val directly (32/16/8 rows per
@@ -90,6 +92,7 @@ struct BatchCtx {
     const int* tile_r0 = nullptr;
     const int* sys_blk0 = nullptr;
     const int* kbase = nullptr;
+    const int* koff = nullptr;
     int lanes = 1;
     // tile order of the SpMV kernels of one system: 0 = each block a
     // contiguous range, 1 = strided inside each XCD's chunk
@@ -99,7 +102,10 @@ struct BatchCtx {
     __device__ __forceinline__ bool leader() const {
         return threadIdx.x == 0 && (blk_sys ? (int)blockIdx.x == sys_blk0[sys()] : blockIdx.x == 0);
     }
-    __device__ __forceinline__ int k(int k_arg) const { return kbase ? *kbase + k_arg : k_arg; }
+    __device__ __forceinline__ int k(int k_arg) const {
+        const int k = kbase ? *kbase + k_arg : k_arg;
+        return koff ? k - koff[sys()] : k;
+    }
 };
 
 // Matrix view with 0-based indices.

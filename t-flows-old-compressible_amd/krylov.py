@@ -149,6 +149,10 @@ def _bind_extensions(lib):
         lib.tflows_batch_set_form.argtypes = [vp, C.c_int32]
         lib.tflows_batch_form.restype = C.c_int32
         lib.tflows_batch_form.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
+        lib.tflows_batch_set_groups.restype = C.c_int
+        lib.tflows_batch_set_groups.argtypes = [vp, C.c_int32]
+        lib.tflows_batch_groups.restype = C.c_int32
+        lib.tflows_batch_groups.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
         lib.tflows_batch_solve.restype = C.c_int
         lib.tflows_batch_solve.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp]
         lib.tflows_batch_solve_mixed.restype = C.c_int
@@ -527,12 +531,18 @@ class DeviceBatch:
         chunk of iterations (CG / BiCG with prec 0 or 1)."""
         _check(self.lib.tflows_batch_set_form(self.handle, form))
 
+    def set_groups(self, mode: int):
+        """0: tuned; 1: every system from the first iteration; 2: systems in
+        groups that fit the MALL, one group after another."""
+        _check(self.lib.tflows_batch_set_groups(self.handle, mode))
+
     def config(self, method: str = "cg", prec: int = 1, bits: int = 64) -> dict:
         g, l, c, r = (C.c_int32(0) for _ in range(4))
         _check(self.lib.tflows_batch_config(self.handle, bits, METHODS[method], prec, C.byref(g), C.byref(l),
                                             C.byref(c), C.byref(r)))
         form = int(self.lib.tflows_batch_form(self.handle, bits, METHODS[method], prec))
-        return dict(grid=g.value, lanes=l.value, chunk=c.value, rebalances=r.value, form=form)
+        groups = int(self.lib.tflows_batch_groups(self.handle, bits, METHODS[method], prec))
+        return dict(grid=g.value, lanes=l.value, chunk=c.value, rebalances=r.value, form=form, groups=groups)
 
     def _arrays(self, cap, tol, fin_in):
         cap = np.ascontiguousarray(np.broadcast_to(np.asarray(cap, np.int32), (self.k,)))

@@ -444,3 +444,81 @@ def test_persistent_wait_bound(pkg, cuda, mixed_sizes, monkeypatch):
     print(f"wait bound 0: form after the solve = {got[0]}", flush=True)
     assert got[0] in (1, 2)
     _same_results(want, got)
+
+
+def _grouped(pkg, monkeypatch, systems, budget_mb="0.01"):
+    """A batch whose systems run in MALL-sized groups, with the group budget
+    shrunk so that small test systems form several groups."""
+    monkeypatch.setenv("TFLOWS_BATCH_GROUP_MB", budget_mb)
+    B = pkg.krylov.DeviceBatch([s.matrix for s in systems])
+    B.set_form(1)
+    B.set_groups(2)
+    return B
+
+
+@pytest.mark.parametrize("method", ["cg", "bicg", "cgs"])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_groups_match_oracle(pkg, oracle, cuda, mixed_sizes, monkeypatch, method, prec):
+    """Systems solved group after group (each group on the whole grid, its
+    iterations counted from the group's start) follow the reference loop
+    each: niter within one, the same stopping state, x within the parity
+    bar."""
+    systems = mixed_sizes
+    cap = 1000
+    B = _grouped(pkg, monkeypatch, systems)
+    B, res, xs, rs = _batch_solve(pkg, cuda, systems, method, prec, cap, batch=B)
+    groups = B.config(method, prec)["groups"]
+    print(f"{method} prec {prec}: {groups} groups, niter {list(res['niter'])}", flush=True)
+    assert groups >= 3
+    for k, s in enumerate(systems):
+        ref = oracle.run(method, s.matrix, s.b, prec=prec, niter=cap, tol=s.tol)
+        assert abs(int(res["niter"][k]) - ref["niter"]) <= 1
+        assert res["fin_res"][k] < s.tol
+        assert res["ini_res"][k] == pytest.approx(ref["ini_res"], rel=1e-13)
+        true_r = s.b - s.matrix.to_scipy() @ xs[k]
+        assert np.linalg.norm(true_r) < 10 * s.tol
+        if s.matrix.n > 1:
+            A = s.matrix.to_scipy().toarray()
+            inv = 1.0 / np.linalg.svd(A, compute_uv=False)[-1]
+            bound = inv * (np.linalg.norm(rs[k]) + np.linalg.norm(ref["r"]))
+            assert (_rel(xs[k], ref["x"]) <= 1e-10 or
+                    np.linalg.norm(xs[k] - ref["x"]) <= 1.01 * bound)
+    B.close()
+
+
+def test_batch_groups_caps_early_exit_and_reproducible(pkg, oracle, cuda, mixed_sizes, monkeypatch):
+    """Groups with per-system caps (a capped system stops at cap + 1 with
+    the oracle's iterate) and a zero right-hand side in a later group (the
+    ||b|| < tol exit; its group is skipped); the same batch solved twice
+    gives the same bits."""
+    import torch
+    systems = mixed_sizes[:5]
+    runs = []
+    for _ in range(2):
+        B = _grouped(pkg, monkeypatch, systems)
+        b = [s.b.copy() for s in systems]
+        b[3][:] = 0.0
+        x = torch.zeros(B.rows, dtype=torch.float64, device=cuda)
+        r = torch.from_numpy(np.concatenate(b)).to(cuda)
+        caps = np.array([3, 1000, 1000, 50, 7], np.int32)
+        tol = np.array([s.tol for s in systems])
+        res = B.solve("cgs", 1, x, r, caps, tol, fin_in=np.array([7.0, 7.0, 7.0, 0.125, 7.0]))
+        torch.cuda.synchronize()
+        assert B.config("cgs", 1)["groups"] >= 3
+        xs = [v.cpu().numpy() for v in B.split(x)]
+        runs.append((res, xs))
+        B.close()
+    res, xs = runs[0]
+    assert res["niter"][0] == 4 and res["niter"][4] == 8          # cap + 1
+    assert res["niter"][3] == 0 and np.isnan(res["ini_res"][3]) and res["fin_res"][3] == 0.125
+    for k in (0, 4):
+        ref = oracle.run("cgs", systems[k].matrix, systems[k].b, prec=1, niter=int(caps[k]), tol=systems[k].tol)
+        assert ref["niter"] == caps[k] + 1
+        assert _rel(xs[k], ref["x"]) <= 1e-9
+    for k in (1, 2):
+        ref = oracle.run("cgs", systems[k].matrix, systems[k].b, prec=1, niter=1000, tol=systems[k].tol)
+        assert abs(int(res["niter"][k]) - ref["niter"]) <= 1
+    res2, xs2 = runs[1]
+    assert np.array_equal(res["niter"], res2["niter"]) and np.array_equal(res["fin_res"], res2["fin_res"])
+    for a, b2 in zip(xs, xs2):
+        assert np.array_equal(a, b2)
