@@ -93,10 +93,24 @@ def test_momentum_and_scalar_assembly(pkg, oracle, cuda, name):
         for k in ("aval", "b", "rhoui"):
             assert np.array_equal(got[k].cpu().numpy(), want[k]), (var, k)
         assert np.array_equal(got["abou"].cpu().numpy()[:case.nbc], want["abou"])
+    # the density-weighted viscosity branch (Moin_Problem() == 4) on the read mesh
+    p = mg.uvw_params(nmaterials=nm, var=2, blend=[YES] * nm, blend_tem=[YES] * nm, visc_rho=1, visc=0.02)
+    want = oracle.newuvw(case, uf, p)
+    got = dc.assemble_uvw(p, dc.uvw_inputs(mg.uvw_args(case, uf, 2)))
+    for k in ("aval", "b", "rhoui"):
+        assert np.array_equal(got[k].cpu().numpy(), want[k]), ("visc_rho", k)
     pz = mg.uvw_params(nmaterials=nm, blend_tem=[YES] * nm, urf=0.8, visc=1e-3)
     want = oracle.calczmix(case, zf, pz)
     got = dc.assemble_zmix(pz, dc.zmix_inputs(mg.zmix_args(case, zf)))
     for k in ("aval", "b", "rhoz"):
         assert np.array_equal(got[k].cpu().numpy(), want[k]), k
+    # the CFL and Peclet monitor of the correction, per material of the mesh
+    import torch
+    flux = np.asarray(uf["flux_u"], float)
+    for dt, visc in ((0.05, 1e-3), (1.0, 0.0)):
+        cfl, pe = dc.cfl_peclet(torch.from_numpy(np.ascontiguousarray(case.rho_n, np.float64)).cuda(),
+                                torch.from_numpy(flux).cuda(), dt, visc)
+        rc, rp = oracle.cfl_peclet(case, uf["material"], nm, uf["scoef"], case.rho_n, flux, dt, visc)
+        assert np.array_equal(cfl, rc) and np.array_equal(pe, rp), (dt, visc)
     dc.close()
     m.close()

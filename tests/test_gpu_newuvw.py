@@ -259,18 +259,19 @@ def test_assembly_bitwise_visit_orders(pkg, oracle, cuda, monkeypatch, flags, va
     dc.close()
 
 
-@pytest.mark.parametrize("scheme,var", [(24, 1), (27, 2)])
-def test_conv_scheme_bitwise_block_order(pkg, oracle, cuda, scheme, var):
+@pytest.mark.parametrize("scheme,var,visc_rho", [(24, 1, 0), (27, 2, 0), (21, 3, 1)])
+def test_conv_scheme_bitwise_block_order(pkg, oracle, cuda, scheme, var, visc_rho):
     """ConvScheme face values with the block-shaped visiting order (64 x 64
     x 40 channel): the upwind extrapolation reads the other cell's gradients
     while its density and velocity come from LDS; bit for bit the oracle's."""
     mg = pkg.meshgen
     case = mg.calcps_case((64, 64, 40), seed=63)
     uf = with_mass_flux(mg.uvw_fields(case, seed=64))
-    p = mg.uvw_params(nmaterials=1, var=var, blend=[scheme], blend_tem=[YES], urf=0.7, urfc=[0.9])
+    p = mg.uvw_params(nmaterials=1, var=var, blend=[scheme], blend_tem=[YES], urf=0.7, urfc=[0.9],
+                      visc_rho=visc_rho)
     want = oracle.newuvw(case, uf, p)
     dc, args, got = _run(pkg, case, uf, p)
-    print(f"visit order {dc.visit_order()} scheme {scheme} var {var}", flush=True)
+    print(f"visit order {dc.visit_order()} scheme {scheme} var {var} visc_rho {visc_rho}", flush=True)
     assert dc.visit_order() == 2
     for k in ("aval", "b", "rhoui"):
         assert np.array_equal(got[k], want[k]), k
