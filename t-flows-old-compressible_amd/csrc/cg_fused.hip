@@ -60,8 +60,8 @@ constexpr double kReplaceDrop = 1e-2;
 // of dependent ones.  A kernel never reads here a field it writes itself
 // (err_rep is written only when the test does not read it).
 struct StopIn {
-    double tol, err_rep, rho0, rho1;
-    int cap, replaced_before, it, kcur, halt;
+    double tol, err_rep, rho0, rho1, alfa;
+    int cap, replaced_before, it, kcur, halt, xupd;
     // rho of iteration k; a select, not an indexed array, so that the
     // struct stays in registers
     __device__ __forceinline__ double rho(int k) const { return (k & 1) ? rho1 : rho0; }
@@ -77,6 +77,8 @@ __device__ __forceinline__ StopIn load_stop_in(const SolverState* st) {
     in.it = st->it;
     in.kcur = st->kcur;
     in.halt = st->done | st->suspend;
+    in.alfa = st->alfa;
+    in.xupd = st->xupd;
     return in;
 }
 
@@ -265,8 +267,12 @@ __global__ __launch_bounds__(kBlock) void k_cgf_init_z(int n, const T* __restric
 // ---------------------------------------------------------------------------
 constexpr int kUnroll = 4;
 
-template <typename T, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_cgj_p(const T* __restrict__ z, T* __restrict__ p,
+// DEFER: the p kernel also applies x += alfa p of the iteration before
+// (CG.f90:226-227), which its update kernel left with alfa in the state,
+// unless a merge of the mixed-precision path has applied it already; the
+// update kernel then reads neither x nor p.
+template <typename T, bool MIXED, bool DEFER>
+__global__ __launch_bounds__(kBlock) void k_cgj_p(const T* __restrict__ z, T* __restrict__ p, T* __restrict__ x,
                                                   const T* __restrict__ partB, const double* __restrict__ partE,
                                                   int n, SolverState* st, HostMirror* hm) {
     // issued together: the state, this thread's first group of rows and
@@ -280,12 +286,15 @@ __global__ __launch_bounds__(kBlock) void k_cgj_p(const T* __restrict__ z, T* __
     block_tiles(ntiles, t0, t1);
     const int rb = t0 * kTileRows, re = min(t1 * kTileRows, n);
     int i0 = rb + (int)threadIdx.x;
-    T zv[kUnroll], pv[kUnroll];
+    T zv[kUnroll], pv[kUnroll], xv[kUnroll];
+    // x is loaded before the state says whether its update is pending (it
+    // is, except right after a merge)
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
         const int i = i0 + u * kBlock;
         zv[u] = i < re ? z[i] : T(0);
         pv[u] = i < re ? p[i] : T(0);
+        xv[u] = (DEFER && i < re) ? x[i] : T(0);
     }
     T tot[2];
     site_total<T, 2>(partB, tot);        // its barriers publish `halt`
@@ -297,12 +306,17 @@ __global__ __launch_bounds__(kBlock) void k_cgj_p(const T* __restrict__ z, T* __
         st->kcur = kd;
     }
     const bool first = kd == 1;
+    const bool carry = DEFER && in.it > in.xupd;
+    const T alfa_prev = (T)in.alfa;
     const T beta = first ? T(0) : rho / (T)in.rho(kd - 1);        // CG.f90:183
     for (;;) {
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             const int i = i0 + u * kBlock;
-            if (i < re) p[i] = first ? zv[u] : zv[u] + beta * pv[u];   // CG.f90:180, :185
+            if (i < re) {
+                if (carry) x[i] = xv[u] + alfa_prev * pv[u];            // CG.f90:227
+                p[i] = first ? zv[u] : zv[u] + beta * pv[u];            // CG.f90:180, :185
+            }
         }
         i0 += kUnroll * kBlock;
         if (i0 >= re) break;
@@ -311,11 +325,12 @@ __global__ __launch_bounds__(kBlock) void k_cgj_p(const T* __restrict__ z, T* __
             const int i = i0 + u * kBlock;
             zv[u] = i < re ? z[i] : T(0);
             pv[u] = i < re ? p[i] : T(0);
+            xv[u] = (carry && i < re) ? x[i] : T(0);
         }
     }
 }
 
-template <typename T>
+template <typename T, bool DEFER>
 __global__ __launch_bounds__(kBlock) void k_cgj_update(int n, T* __restrict__ x, T* __restrict__ r,
                                                        T* __restrict__ z, const T* __restrict__ p,
                                                        const T* __restrict__ q, const T* __restrict__ D,
@@ -338,8 +353,8 @@ __global__ __launch_bounds__(kBlock) void k_cgj_update(int n, T* __restrict__ x,
         for (int u = 0; u < kUnroll; ++u) {
             const int i = i0 + u * kBlock;
             const bool ok = i < re;
-            xv[u] = ok ? x[i] : T(0);
-            pv[u] = ok ? p[i] : T(0);
+            xv[u] = (!DEFER && ok) ? x[i] : T(0);
+            pv[u] = (!DEFER && ok) ? p[i] : T(0);
             rv[u] = ok ? r[i] : T(0);
             qv[u] = ok ? q[i] : T(0);
             dv[u] = ok ? D[i] : T(1);
@@ -350,13 +365,17 @@ __global__ __launch_bounds__(kBlock) void k_cgj_update(int n, T* __restrict__ x,
     site_total<T, 1>(partD, pq);
     if (in.halt) return;
     const T alfa = (T)in.rho(kd) / pq[0];               // CG.f90:219
+    if (leader()) {
+        if (DEFER) st->alfa = (double)alfa;
+        st->it = kd;
+    }
     for (int i0 = rb + (int)threadIdx.x; i0 < re; i0 += kUnroll * kBlock) {
         if (i0 != rb + (int)threadIdx.x) load(i0);
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             const int i = i0 + u * kBlock;
             if (i < re) {
-                x[i] = xv[u] + alfa * pv[u];                // CG.f90:227
+                if (!DEFER) x[i] = xv[u] + alfa * pv[u];    // CG.f90:227
                 const T ri = rv[u] - alfa * qv[u];          // CG.f90:228
                 r[i] = ri;
                 const T zi = ri / dv[u];                    // next z, CG.f90:126
@@ -367,7 +386,6 @@ __global__ __launch_bounds__(kBlock) void k_cgj_update(int n, T* __restrict__ x,
         }
     }
     site_write<T, 2>(v, partB);
-    if (leader()) st->it = kd;
 }
 
 template <typename Op, typename T = typename OpValue<Op>::type>
@@ -419,31 +437,37 @@ void launch_cgf_init_z(int n, const T* r, const T* D, T* z, const Sites<T>& S, c
 
 template <typename Op, typename T>
 void launch_cgj(bool mixed, bool strided, const Op& A, int n, T* x, T* r, T* z, T* p, T* q, const T* D,
-                const Sites<T>& S, const double* partE, SolverState* st, HostMirror* hm, hipStream_t s) {
-    if (mixed) TF_FL((k_cgj_p<T, true>), S.grid, s), (const T*)z, p, S.B, partE, n, st, hm);
-    else TF_FL((k_cgj_p<T, false>), S.grid, s), (const T*)z, p, S.B, partE, n, st, hm);
+                const Sites<T>& S, const double* partE, SolverState* st, HostMirror* hm, hipStream_t s, bool carry) {
+    if (carry) {
+        if (mixed) TF_FL((k_cgj_p<T, true, true>), S.grid, s), (const T*)z, p, x, S.B, partE, n, st, hm);
+        else TF_FL((k_cgj_p<T, false, true>), S.grid, s), (const T*)z, p, x, S.B, partE, n, st, hm);
+    } else {
+        if (mixed) TF_FL((k_cgj_p<T, true, false>), S.grid, s), (const T*)z, p, x, S.B, partE, n, st, hm);
+        else TF_FL((k_cgj_p<T, false, false>), S.grid, s), (const T*)z, p, x, S.B, partE, n, st, hm);
+    }
     TF_FL((k_cgj_spmv<Op>), S.grid, s), A, (const T*)p, q, S.D, st, strided ? 1 : 0);
-    TF_FL((k_cgj_update<T>), S.grid, s), n, x, r, z, (const T*)p, (const T*)q, D, S.D, S.B, st);
+    if (carry) TF_FL((k_cgj_update<T, true>), S.grid, s), n, x, r, z, (const T*)p, (const T*)q, D, S.D, S.B, st);
+    else TF_FL((k_cgj_update<T, false>), S.grid, s), n, x, r, z, (const T*)p, (const T*)q, D, S.D, S.B, st);
 }
 template void launch_cgj<Csr<double>, double>(bool, bool, const Csr<double>&, int, double*, double*, double*, double*,
                                               double*, const double*, const Sites<double>&, const double*,
-                                              SolverState*, HostMirror*, hipStream_t);
+                                              SolverState*, HostMirror*, hipStream_t, bool);
 template void launch_cgj<Csr<float>, float>(bool, bool, const Csr<float>&, int, float*, float*, float*, float*, float*,
                                             const float*, const Sites<float>&, const double*, SolverState*,
-                                            HostMirror*, hipStream_t);
+                                            HostMirror*, hipStream_t, bool);
 template void launch_cgj<DiffCsr32, float>(bool, bool, const DiffCsr32&, int, float*, float*, float*, float*, float*,
                                            const float*, const Sites<float>&, const double*, SolverState*,
-                                           HostMirror*, hipStream_t);
+                                           HostMirror*, hipStream_t, bool);
 
 template void launch_cgj<Sell<double>, double>(bool, bool, const Sell<double>&, int, double*, double*, double*,
                                                double*, double*, const double*, const Sites<double>&, const double*,
-                                               SolverState*, HostMirror*, hipStream_t);
+                                               SolverState*, HostMirror*, hipStream_t, bool);
 template void launch_cgf_spmv<Sell<double>, double>(bool, const Sell<double>&, const double*, double*, double*,
                                                     double*, const Sites<double>&, const double*, SolverState*,
                                                     HostMirror*, hipStream_t);
 template void launch_cgj<DiffSell32, float>(bool, bool, const DiffSell32&, int, float*, float*, float*, float*,
                                             float*, const float*, const Sites<float>&, const double*, SolverState*,
-                                            HostMirror*, hipStream_t);
+                                            HostMirror*, hipStream_t, bool);
 template void launch_cgf_spmv<DiffSell32, float>(bool, const DiffSell32&, const float*, float*, float*, float*,
                                                  const Sites<float>&, const double*, SolverState*, HostMirror*,
                                                  hipStream_t);

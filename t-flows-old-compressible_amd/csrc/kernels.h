@@ -47,6 +47,15 @@ struct SolverState {
     int k_min[2];
     int stall_min;
     int diverged;
+    // x += alfa p carried into the next iteration's p kernel (one system,
+    // Jacobi CG): the update kernel of iteration `it` leaves its alfa here
+    // and does not touch x; the p kernel of iteration it + 1 applies it, and
+    // when the solve stops, the finishing pass (or the FP64 merge of the
+    // mixed-precision path) does.  `xupd` = the last iteration whose update
+    // a merge has applied, so the p kernel that resumes after a residual
+    // replacement does not apply it again.
+    double alfa;
+    int xupd;
 };
 
 // Host-visible mirror of the end of a solve, written by the device with

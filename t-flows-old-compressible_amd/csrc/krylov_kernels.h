@@ -53,6 +53,17 @@ void launch_cg_spmv(const BatchCtx& c, const Op& A, const T* p, T* q, const Site
 template <typename T>
 void launch_cg_update(const BatchCtx& c, bool jac, int k, int n, T* x, T* r, const T* p, const T* q, const T* D,
                       const Sites<T>& S, const SolverState* st, hipStream_t s);
+// Jacobi CG of one system with x += alfa p carried from iteration k's update
+// kernel into iteration k + 1's p kernel (which reads p anyway); after the
+// solve stops, launch_cg_xfinish applies the update left pending.
+template <typename T>
+void launch_cg_p_carry(const BatchCtx& c, int k, int n, const T* r, const T* D, T* p, T* x, const Sites<T>& S,
+                       SolverState* st, HostMirror* hm, hipStream_t s);
+template <typename T>
+void launch_cg_update_carry(const BatchCtx& c, int k, int n, T* r, const T* q, const T* D, const Sites<T>& S,
+                            SolverState* st, hipStream_t s);
+template <typename T>
+void launch_cg_xfinish(int n, T* x, const T* p, const SolverState* st, hipStream_t s);
 template <typename T>
 void launch_bicg_p(const BatchCtx& c, bool jac, int k, int n, const T* r1, const T* r2, const T* D, const T* z1,
                    const T* z2, T* p1, T* p2, const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s);
@@ -90,10 +101,13 @@ void launch_cgf_update(int n, T* x, T* r, T* z, const T* p0, const T* p1, const 
 template <typename T>
 void launch_cgf_check(bool mixed, const Sites<T>& S, const double* partE, SolverState* st, HostMirror* hm,
                       hipStream_t s);
-// three-kernel Jacobi CG with device-side iteration counters (cg_fused.hip)
+// three-kernel Jacobi CG with device-side iteration counters (cg_fused.hip);
+// with `carry` each x += alfa p goes to the next iteration's p kernel, and
+// the solve's end applies the one left pending (launch_cg_xfinish, or the
+// FP64 merge of the mixed-precision path)
 template <typename Op, typename T>
 void launch_cgj(bool mixed, bool strided, const Op& A, int n, T* x, T* r, T* z, T* p, T* q, const T* D, const Sites<T>& S,
-                const double* partE, SolverState* st, HostMirror* hm, hipStream_t s);
+                const double* partE, SolverState* st, HostMirror* hm, hipStream_t s, bool carry);
 template <typename T>
 void launch_cgf_init_z(int n, const T* r, const T* D, T* z, const Sites<T>& S, const SolverState* st,
                        hipStream_t s);

@@ -883,13 +883,15 @@ __device__ __forceinline__ bool iteration_rho(const BatchCtx& ctx, int k, const 
 // CG  (Process/CG.f90:113-259)
 // ===========================================================================
 // [stopping test,] rho, p = z + beta*p  (z = r/D, or the DIC output z)
-// CG.f90:172-187
-template <typename T, bool JAC>
+// CG.f90:172-187.  DEFER (one system): before p is overwritten, the
+// x += alfa p of iteration k - 1 (CG.f90:226-227), whose update kernel left
+// alfa in the state; the pass then moves p once instead of twice.
+template <typename T, bool JAC, bool DEFER>
 __global__ __launch_bounds__(kBlock) void k_cg_p(BatchCtx ctx, int k_arg, int n, const T* __restrict__ r,
                                                  const T* __restrict__ D, const T* __restrict__ z,
-                                                 T* __restrict__ p, const T* __restrict__ partB,
-                                                 const T* __restrict__ partC, SolverState* st_base,
-                                                 HostMirror* hm_base) {
+                                                 T* __restrict__ p, T* __restrict__ x,
+                                                 const T* __restrict__ partB, const T* __restrict__ partC,
+                                                 SolverState* st_base, HostMirror* hm_base) {
     TF_STATE;
     // done is read once per block (its leader may set it in this launch)
     // and tested after the reduction, so that it, rho_{k-1}, the first
@@ -898,9 +900,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(BatchCtx ctx, int k_arg, int n,
     if (threadIdx.x == 0) halt = st->done;
     const int k = ctx.k(k_arg);
     const double rho_prev = st->rho[(k - 1) & 1];
+    const bool carry = DEFER && k > 1;
+    const T alfa_prev = DEFER ? (T)st->alfa : T(0);
     int rb, re;
     ctx_rows(ctx, n, rb, re);
-    T av[kPrefetch], dv[kPrefetch], pv[kPrefetch];
+    T av[kPrefetch], dv[kPrefetch], pv[kPrefetch], xv[kPrefetch];
 #pragma unroll
     for (int u = 0; u < kPrefetch; ++u) {
         const int i = rb + (int)threadIdx.x + u * kBlock;
@@ -908,6 +912,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(BatchCtx ctx, int k_arg, int n,
         av[u] = ok ? (JAC ? r[i] : z[i]) : T(0);
         dv[u] = (JAC && ok) ? D[i] : T(1);
         pv[u] = ok ? p[i] : T(0);
+        xv[u] = (carry && ok) ? x[i] : T(0);
     }
     T rho;
     if (!iteration_rho<T, JAC>(ctx, k, partB, partC, st, hm, rho, &halt)) return;
@@ -917,13 +922,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(BatchCtx ctx, int k_arg, int n,
     for (int u = 0; u < kPrefetch; ++u) {
         const int i = rb + (int)threadIdx.x + u * kBlock;
         if (i < re) {
+            if (carry) x[i] = xv[u] + alfa_prev * pv[u];
             const T zi = JAC ? jac(av[u], dv[u]) : av[u];
             p[i] = first ? zi : zi + beta * pv[u];
         }
     }
     for (int i = rb + (int)threadIdx.x + kPrefetch * kBlock; i < re; i += kBlock) {
         const T zi = JAC ? jac(r[i], D[i]) : z[i];
-        p[i] = first ? zi : zi + beta * p[i];
+        const T pi = p[i];
+        if (carry) x[i] = x[i] + alfa_prev * pi;
+        p[i] = first ? zi : zi + beta * pi;
     }
 }
 
@@ -957,13 +965,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(BatchCtx ctx, Op A, const T*
 
 // alfa = rho/(p,q) (:219); x += alfa p; r -= alfa q (CG.f90:226-229), with
 // ||r||^2 and, for the Jacobi forms, the next rho = (r, r/D) -> site B.
-template <typename T, bool JAC>
+// DEFER (one system): x is left to the next p kernel; alfa and k go to the
+// state for it.
+template <typename T, bool JAC, bool DEFER>
 __global__ __launch_bounds__(kBlock) void k_cg_update(BatchCtx ctx, int k_arg, int n, T* __restrict__ x,
                                                       T* __restrict__ r, const T* __restrict__ p,
                                                       const T* __restrict__ q, const T* __restrict__ D,
                                                       const T* __restrict__ partD, T* __restrict__ partB,
-                                                      const SolverState* st_base) {
-    const SolverState* st = st_base + ctx.sys();
+                                                      SolverState* st_base) {
+    SolverState* st = st_base + ctx.sys();
     // done is set by earlier kernels only, so this return is the same in
     // every thread; rho_k is loaded with the first rows and the partial sums
     // of (p,q) (testing done after them as well costs this kernel its
@@ -978,8 +988,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(BatchCtx ctx, int k_arg, i
     for (int u = 0; u < kPrefetch; ++u) {
         const int i = rb + (int)threadIdx.x + u * kBlock;
         const bool ok = i < re;
-        xv[u] = ok ? x[i] : T(0);
-        pv[u] = ok ? p[i] : T(0);
+        xv[u] = (!DEFER && ok) ? x[i] : T(0);
+        pv[u] = (!DEFER && ok) ? p[i] : T(0);
         rv[u] = ok ? r[i] : T(0);
         qv[u] = ok ? q[i] : T(0);
         dv[u] = (JAC && ok) ? D[i] : T(1);
@@ -987,12 +997,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(BatchCtx ctx, int k_arg, i
     T pq[1];
     ctx_total<T, 1>(ctx, partD, pq);
     const T alfa = (T)rho_k / pq[0];
+    if (DEFER && ctx.leader()) {
+        st->alfa = (double)alfa;
+        st->it = k;
+    }
     T v[2] = {T(0), T(0)};
 #pragma unroll
     for (int u = 0; u < kPrefetch; ++u) {
         const int i = rb + (int)threadIdx.x + u * kBlock;
         if (i < re) {
-            x[i] = xv[u] + alfa * pv[u];
+            if (!DEFER) x[i] = xv[u] + alfa * pv[u];
             const T ri = rv[u] - alfa * qv[u];
             r[i] = ri;
             v[0] += ri * ri;
@@ -1000,13 +1014,25 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(BatchCtx ctx, int k_arg, i
         }
     }
     for (int i = rb + (int)threadIdx.x + kPrefetch * kBlock; i < re; i += kBlock) {
-        x[i] = x[i] + alfa * p[i];
+        if (!DEFER) x[i] = x[i] + alfa * p[i];
         const T ri = r[i] - alfa * q[i];
         r[i] = ri;
         v[0] += ri * ri;
         if (JAC) v[1] += ri * jac(ri, D[i]);
     }
     site_write<T, 2>(v, partB);
+}
+
+// The update x += alfa p (CG.f90:226-227) that the last iteration left to a
+// p kernel which did not run its rows (the solve stopped first); nothing
+// when a merge has applied it already or no iteration ran.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_xfinish(int n, T* __restrict__ x, const T* __restrict__ p,
+                                                       const SolverState* st) {
+    if (st->it <= st->xupd) return;
+    const T alfa = (T)st->alfa;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] = x[i] + alfa * p[i];
 }
 
 // ===========================================================================
@@ -1537,8 +1563,14 @@ void launch_rho(const BatchCtx& c, int n, const T* a, const T* b, const Sites<T>
 template <typename T>
 void launch_cg_p(const BatchCtx& c, bool jac, int k, int n, const T* r, const T* D, const T* z, T* p,
                  const Sites<T>& S, SolverState* st, HostMirror* hm, hipStream_t s) {
-    if (jac) TF_LAUNCH((k_cg_p<T, true>), S.grid, s), c, k, n, r, D, z, p, S.B, S.C, st, hm);
-    else TF_LAUNCH((k_cg_p<T, false>), S.grid, s), c, k, n, r, D, z, p, S.B, S.C, st, hm);
+    if (jac) TF_LAUNCH((k_cg_p<T, true, false>), S.grid, s), c, k, n, r, D, z, p, (T*)nullptr, S.B, S.C, st, hm);
+    else TF_LAUNCH((k_cg_p<T, false, false>), S.grid, s), c, k, n, r, D, z, p, (T*)nullptr, S.B, S.C, st, hm);
+}
+template <typename T>
+void launch_cg_p_carry(const BatchCtx& c, int k, int n, const T* r, const T* D, T* p, T* x, const Sites<T>& S,
+                       SolverState* st, HostMirror* hm, hipStream_t s) {
+    if (c.blk_sys) throw std::logic_error("the carried x update is for one system");
+    TF_LAUNCH((k_cg_p<T, true, true>), S.grid, s), c, k, n, r, D, (const T*)nullptr, p, x, S.B, S.C, st, hm);
 }
 template <typename Op, typename T>
 void launch_cg_spmv(const BatchCtx& c, const Op& A, const T* p, T* q, const Sites<T>& S, const SolverState* st,
@@ -1551,8 +1583,19 @@ void launch_cg_spmv(const BatchCtx& c, const Op& A, const T* p, T* q, const Site
 template <typename T>
 void launch_cg_update(const BatchCtx& c, bool jac, int k, int n, T* x, T* r, const T* p, const T* q, const T* D,
                       const Sites<T>& S, const SolverState* st, hipStream_t s) {
-    if (jac) TF_LAUNCH((k_cg_update<T, true>), S.grid, s), c, k, n, x, r, p, q, D, S.D, S.B, st);
-    else TF_LAUNCH((k_cg_update<T, false>), S.grid, s), c, k, n, x, r, p, q, D, S.D, S.B, st);
+    if (jac) TF_LAUNCH((k_cg_update<T, true, false>), S.grid, s), c, k, n, x, r, p, q, D, S.D, S.B, (SolverState*)st);
+    else TF_LAUNCH((k_cg_update<T, false, false>), S.grid, s), c, k, n, x, r, p, q, D, S.D, S.B, (SolverState*)st);
+}
+template <typename T>
+void launch_cg_update_carry(const BatchCtx& c, int k, int n, T* r, const T* q, const T* D, const Sites<T>& S,
+                            SolverState* st, hipStream_t s) {
+    if (c.blk_sys) throw std::logic_error("the carried x update is for one system");
+    TF_LAUNCH((k_cg_update<T, true, true>), S.grid, s), c, k, n, (T*)nullptr, r, (const T*)nullptr, q, D, S.D,
+        S.B, st);
+}
+template <typename T>
+void launch_cg_xfinish(int n, T* x, const T* p, const SolverState* st, hipStream_t s) {
+    TF_LAUNCH((k_cg_xfinish<T>), (n + kBlock - 1) / kBlock, s), n, x, p, st);
 }
 template <typename T>
 void launch_bicg_p(const BatchCtx& c, bool jac, int k, int n, const T* r1, const T* r2, const T* D, const T* z1,
@@ -1699,6 +1742,11 @@ TF_INSTANTIATE_PLAN(float)
                                  const Sites<T>&, SolverState*, HostMirror*, hipStream_t);                       \
     template void launch_cg_update<T>(const BatchCtx&, bool, int, int, T*, T*, const T*, const T*, const T*,    \
                                       const Sites<T>&, const SolverState*, hipStream_t);                         \
+    template void launch_cg_p_carry<T>(const BatchCtx&, int, int, const T*, const T*, T*, T*, const Sites<T>&,  \
+                                       SolverState*, HostMirror*, hipStream_t);                                  \
+    template void launch_cg_update_carry<T>(const BatchCtx&, int, int, T*, const T*, const T*, const Sites<T>&, \
+                                            SolverState*, hipStream_t);                                          \
+    template void launch_cg_xfinish<T>(int, T*, const T*, const SolverState*, hipStream_t);                      \
     template void launch_bicg_p<T>(const BatchCtx&, bool, int, int, const T*, const T*, const T*, const T*,     \
                                    const T*, T*, T*, const Sites<T>&, SolverState*, HostMirror*, hipStream_t);   \
     template void launch_bicg_update<T>(const BatchCtx&, bool, int, int, T*, T*, T*, const T*, const T*,        \

@@ -117,6 +117,24 @@ __global__ __launch_bounds__(kBlock) void k_rel_merge(int n, double* __restrict_
     }
 }
 
+// The same merge when the FP32 iterations carry x += alfa p into the next p
+// kernel: the update of iteration `it`, if no p kernel has applied it yet,
+// is applied here first (in FP32, as the iteration would have), and
+// k_rel_resid then marks it applied.
+__global__ __launch_bounds__(kBlock) void k_rel_merge_carry(int n, double* __restrict__ x64, float* __restrict__ x32,
+                                                            const float* __restrict__ p,
+                                                            const SolverState* __restrict__ st) {
+    const bool carry = st->it > st->xupd;
+    const float alfa = (float)st->alfa;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        float xs = x32[i];
+        if (carry) xs = xs + alfa * p[i];
+        x64[i] = x64[i] + (double)xs;
+        x32[i] = 0.0f;
+    }
+}
+
 // FP64 residual replacement: r = b - A x64 (Resid form, FP64), then
 // r32 = fp32(r), z32 = r32/D32, with (r32,r32), (r32,z32) -> site B (FP32)
 // and (r,r) -> site E (FP64).  The leader clears `suspend` and marks the
@@ -159,6 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_rel_resid(Op A, const double* __rest
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->suspend = 0;
         st->replaced_before = st->it + 1;
+        st->xupd = st->it;              // the merge before this pass applied x's pending update
     }
 }
 
@@ -580,8 +599,13 @@ SolveResult System::solve_mixed_reliable_impl(int prec, double* x, const double*
     mirror32_->iter = 0;
     // the FP64 residual on the operator of the FP64 iterations (sliced or
     // CSR: the same bits)
+    const bool carry = cg_form() != 2 && cg_carry();
     auto replace = [&](double* rout) {
-        hipLaunchKernelGGL(k_rel_merge, dim3(nb), dim3(kBlock), 0, s, n_, x, x32);
+        if (carry)
+            hipLaunchKernelGGL(k_rel_merge_carry, dim3(nb), dim3(kBlock), 0, s, n_, x, x32, (const float*)p0,
+                               (const SolverState*)state32_.p);
+        else
+            hipLaunchKernelGGL(k_rel_merge, dim3(nb), dim3(kBlock), 0, s, n_, x, x32);
         with_op64(s, [&](const auto& A) {
             hipLaunchKernelGGL((k_rel_resid<std::decay_t<decltype(A)>>), dim3(grid_), dim3(kBlock), 0, s, A,
                                (const double*)x, b, r32, z, D32, rout, S32.B, partE, state32_.p);
@@ -606,7 +630,7 @@ SolveResult System::solve_mixed_reliable_impl(int prec, double* x, const double*
                                   (const float*)D32, S32, state32_.p, s);
             } else {
                 launch_cgj(true, ctx().strided != 0, Aop, n_, x32, r32, z, p0, q, (const float*)D32, S32,
-                           (const double*)partE, state32_.p, mirror32_, s);
+                           (const double*)partE, state32_.p, mirror32_, s, carry);
             }
         }
     };

@@ -150,6 +150,8 @@ SolveResult solve_noalloc(System& S, int method, int prec, double* x, double* r1
         chunk = std::min(64, chunk * 2);
     }
     if (!done) launch_check(c, std::max(cap, 0) + 1, sites, st, nullptr, s);
+    // the x update of the last iteration, which no p kernel applied
+    if (carries_x(c, 0, true)) launch_cg_xfinish(A.n, w.x, (const double*)w.p, st, s);
     TF_CHECK(hipGetLastError());
     SolverState h;
     TF_CHECK(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));

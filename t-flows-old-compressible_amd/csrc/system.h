@@ -449,6 +449,10 @@ class System {
 int max_blocks_from_env();
 bool fused_cg_enabled();
 int cg_form();
+// whether one system's Jacobi CG iterations carry x += alfa p into the next
+// p kernel (then the solve ends with launch_cg_xfinish)
+bool cg_carry();
+bool carries_x(const BatchCtx& c, int method, bool jac);
 template <typename T, typename Op>
 void start_solve(const BatchCtx& c, int method, int prec, const Op& Aop, const Csr<T>& A,
                  const KrylovVectors<T>& w, const Sites<T>& S, SolverState* st, HostMirror* hm, const Levels* L,

@@ -28,6 +28,18 @@ int cg_form() {
     return f;
 }
 bool fused_cg_enabled() { return cg_form() != 0; }
+// Jacobi CG of one system in forms 0 and 3 (and the mixed-precision inner
+// CG): x += alfa p moved from the update kernel into the next p kernel,
+// which reads p anyway (1, default); TFLOWS_CG_CARRY=0 keeps it in the
+// update kernel (read as iterations are issued: set it between solves).
+// Both give the same bits.
+bool cg_carry() {
+    const char* e = std::getenv("TFLOWS_CG_CARRY");
+    return !e || std::atoi(e) != 0;
+}
+bool carries_x(const BatchCtx& c, int method, bool jac) {
+    return c.blk_sys == nullptr && c.kbase == nullptr && method == 0 && jac && cg_form() != 2 && cg_carry();
+}
 
 // Lanes per row of the SpMV kernels (1, 2, 4, 8); TFLOWS_LANES sets the
 // default of new systems.
@@ -823,9 +835,16 @@ void enqueue_iteration(const BatchCtx& c, int method, bool jac, int k, const Op&
                                   (const T*)w.D, S, st, s);
                 break;
             }
+            const bool carry = carries_x(c, method, jac);
             if (single && jac && cg_form() == 3) {
                 launch_cgj(false, c.strided != 0, Aop, n, w.x, w.r1, w.u1, w.p, w.q, (const T*)w.D, S, (const double*)nullptr, st,
-                           hm, s);
+                           hm, s, carry);
+                break;
+            }
+            if (carry) {
+                launch_cg_p_carry(c, k, n, (const T*)w.r1, (const T*)w.D, w.p, w.x, S, st, hm, s);
+                launch_cg_spmv(c, Aop, (const T*)w.p, w.q, S, st, s);
+                launch_cg_update_carry(c, k, n, w.r1, (const T*)w.q, (const T*)w.D, S, st, s);
                 break;
             }
             if (!jac) {  // stopping test, z = M^-1 r into q, rho = (r, z)
@@ -1009,6 +1028,8 @@ void pace_iterations(const BatchCtx& c, int method, bool jac, int cap, const Op&
         else
             launch_check(c, std::max(cap, 0) + 1, S, st, hm, s);
     }
+    // the x update of the last iteration, which no p kernel applied
+    if (carries_x(c, method, jac)) launch_cg_xfinish(A.n, w.x, (const T*)w.p, st, s);
     TF_CHECK(hipGetLastError());
     TF_CHECK(hipStreamSynchronize(s));
     (void)hipEventDestroy(ev[0]);

@@ -222,3 +222,41 @@ def test_mixed_reliable_matches_fp64_oracle(pkg, oracle, cuda, case, prec, inter
     assert rel <= bound
     assert it <= 1.5 * ref["niter"] + interval + 5
     ds.close()
+
+
+@pytest.mark.parametrize("case", ["pressure12", "wall20", "momentum10"])
+@pytest.mark.parametrize("interval", [1, 7, 50])
+def test_mixed_reliable_carry_bitwise(pkg, cuda, case, interval):
+    """The FP32 iterations carrying x += alfa p into the next p kernel, with
+    the FP64 merge applying the update a pause or a replacement leaves
+    pending, give the bits of the update kernel applying it
+    (TFLOWS_CG_CARRY=0): converged solves and the capped momentum solve."""
+    import os
+    import torch
+    mg = pkg.meshgen
+    s = {"pressure12": lambda: mg.pressure_system(12, seed=51),
+         "momentum10": lambda: mg.momentum_system(10, seed=54),
+         "wall20": lambda: mg.wall_pressure_system(20, seed=55)}[case]()
+    m = s.matrix
+    ds = pkg.krylov.DeviceSystem(m)
+    cap = 400 if case == "momentum10" else 3000
+    out = {}
+    for carry in ("0", "1"):
+        os.environ["TFLOWS_CG_CARRY"] = carry
+        try:
+            b = torch.from_numpy(s.b.copy()).to(cuda)
+            x = torch.zeros(m.n, dtype=torch.float64, device=cuda)
+            r = torch.zeros_like(b)
+            res = ds.solve_mixed_reliable(1, x, b, cap, s.tol, interval=interval, r_out=r)
+        finally:
+            os.environ.pop("TFLOWS_CG_CARRY", None)
+        out[carry] = (res, x.cpu().numpy(), r.cpu().numpy())
+    a, c = out["0"], out["1"]
+    print(f"case {case} interval {interval}: {a[0]} / {c[0]}", flush=True)
+    # niter, IniRes, FinRes; the replacement count is the number of chunks
+    # the host had issued when it saw the stop, which depends on timing
+    assert a[0][:3] == c[0][:3]
+    assert np.array_equal(a[1], c[1]) and np.array_equal(a[2], c[2])
+    if case != "momentum10":
+        assert c[0][2] < s.tol
+    ds.close()

@@ -258,3 +258,42 @@ def test_bicg_runs_as_cg_bitwise(pkg, oracle, cases, cuda, case, prec):
     assert a[0] == ref["niter"] == 9
     assert _rel(a[3], ref["x"]) <= 1e-11
     ds.close()
+
+
+@pytest.mark.parametrize("case", ["pressure12", "pressure9_perm", "momentum10"])
+@pytest.mark.parametrize("method", ["cg", "bicg"])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_cg_carry_bitwise(pkg, oracle, cases, cuda, case, method, prec):
+    """x += alfa p applied by the next iteration's p kernel (and, for the
+    last iteration, by the pass that ends the solve) gives the same bits as
+    the update kernel applying it (TFLOWS_CG_CARRY=0): converged, capped and
+    zero-iteration solves, on a renumbered system too; the converged x is
+    the oracle's."""
+    import os
+    import torch
+    s = cases[case]
+    m = s.matrix
+    ds = pkg.krylov.DeviceSystem(m)
+    for cap, rhs in ((500, s.b), (5, s.b), (50, np.zeros_like(s.b))):
+        out = {}
+        for carry in ("0", "1"):
+            os.environ["TFLOWS_CG_CARRY"] = carry
+            try:
+                x = torch.zeros(m.n, dtype=torch.float64, device=cuda)
+                r = torch.from_numpy(rhs.copy()).to(cuda)
+                it, ini, fin = ds.solve(method, prec, x, r, cap, s.tol)
+            finally:
+                os.environ.pop("TFLOWS_CG_CARRY", None)
+            out[carry] = (it, ini, fin, x.cpu().numpy(), r.cpu().numpy())
+        a, b = out["0"], out["1"]
+        print(f"case {case} {method} prec {prec} cap {cap}: niter {a[0]} / {b[0]}", flush=True)
+        assert a[:3] == b[:3], (cap, a[:3], b[:3])
+        assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4]), cap
+        if cap == 500 and case != "momentum10":
+            ref = oracle.run("cg", m, s.b, prec=prec, niter=cap, tol=s.tol)
+            assert abs(b[0] - ref["niter"]) <= 1 and _rel(b[3], ref["x"]) <= 1e-10
+        if cap == 5:
+            assert b[0] == 6 and np.any(b[3] != 0)
+        if cap == 50 and not np.any(rhs):
+            assert b[0] == 0 and not np.any(b[3])
+    ds.close()
