@@ -121,9 +121,10 @@ int tflows_solve(tflows_system s, int32_t method, int32_t prec, double* x, doubl
                  double tol, double* ini_res, double* fin_res, void* stream);
 
 /* tflows_solve without any allocation: the solver state, the reduction
- * partials and the work vectors (D, p, q, and x / r1 in the internal order
- * of a renumbered system) live in one fixed device array of the library
- * (each GPU holds its own copy, about 40 MiB, until the process ends), so
+ * partials and the work vectors (D, p, q, z, and x / r1 in the internal
+ * order of a renumbered system) live in one fixed device array of the
+ * library (each GPU holds its own copy, about 48 MiB, until the process
+ * ends), so
  * the call makes no hipMalloc / hipFree / hipHostMalloc and no allocating
  * library call.  No counterpart in the reference: it is for callers that
  * cannot allocate during a solve -- tight solve loops and harnesses that

@@ -22,6 +22,10 @@
 // The iteration index lives on the device: k_cgf_update stores the number
 // of completed iterations in `it`, k_cgf_spmv stores the one in progress in
 // `kcur`, and each kernel only reads the field the other writes.
+#include <cstdint>
+#include <cstdlib>
+#include <initializer_list>
+
 #include "krylov_kernels.h"
 
 namespace tflows {
@@ -262,19 +266,52 @@ __global__ __launch_bounds__(kBlock) void k_cgf_init_z(int n, const T* __restric
 // ---------------------------------------------------------------------------
 // The same iteration in three kernels: the p update is its own streaming
 // pass (z and p only), and the SpMV gathers p alone.  Each element-wise loop
-// is unrolled so that a thread issues the loads of all its rows before it
-// uses any of them.
+// keeps kUnroll rows per thread in flight, issuing their loads before it
+// uses any of them.  A thread takes its rows in packs of V consecutive rows
+// (V = 1, or 16 bytes of T where every vector is 16-byte aligned: one
+// dwordx4 access per pack); the U = kUnroll / V packs of a group lie
+// kBlock * V rows apart, so V = 1 is the plain row-per-lane layout.
 // ---------------------------------------------------------------------------
 constexpr int kUnroll = 4;
+
+template <typename T, int V>
+struct alignas(sizeof(T) * V) Pack {
+    T v[V];
+};
+
+template <typename T, int V>
+__device__ __forceinline__ Pack<T, V> load_pack(const T* __restrict__ a, int i, int re, T fill) {
+    Pack<T, V> r;
+    if (i + V <= re) {
+        r = *reinterpret_cast<const Pack<T, V>*>(a + i);
+    } else {
+#pragma unroll
+        for (int e = 0; e < V; ++e) r.v[e] = i + e < re ? a[i + e] : fill;
+    }
+    return r;
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void store_pack(T* __restrict__ a, int i, int re, const Pack<T, V>& v) {
+    if (i + V <= re) {
+        *reinterpret_cast<Pack<T, V>*>(a + i) = v;
+    } else {
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+            if (i + e < re) a[i + e] = v.v[e];
+    }
+}
 
 // DEFER: the p kernel also applies x += alfa p of the iteration before
 // (CG.f90:226-227), which its update kernel left with alfa in the state,
 // unless a merge of the mixed-precision path has applied it already; the
 // update kernel then reads neither x nor p.
-template <typename T, bool MIXED, bool DEFER>
+template <typename T, bool MIXED, bool DEFER, int V>
 __global__ __launch_bounds__(kBlock) void k_cgj_p(const T* __restrict__ z, T* __restrict__ p, T* __restrict__ x,
                                                   const T* __restrict__ partB, const double* __restrict__ partE,
                                                   int n, SolverState* st, HostMirror* hm) {
+    constexpr int U = kUnroll / V;
+    using P = Pack<T, V>;
     // issued together: the state, this thread's first group of rows and
     // (in site_total) the partial sums of the stopping test
     const StopIn in = load_stop_in(st);
@@ -285,16 +322,16 @@ __global__ __launch_bounds__(kBlock) void k_cgj_p(const T* __restrict__ z, T* __
     int t0, t1;
     block_tiles(ntiles, t0, t1);
     const int rb = t0 * kTileRows, re = min(t1 * kTileRows, n);
-    int i0 = rb + (int)threadIdx.x;
-    T zv[kUnroll], pv[kUnroll], xv[kUnroll];
+    int i0 = rb + (int)threadIdx.x * V;
+    P zv[U], pv[U], xv[U];
     // x is loaded before the state says whether its update is pending (it
     // is, except right after a merge)
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-        const int i = i0 + u * kBlock;
-        zv[u] = i < re ? z[i] : T(0);
-        pv[u] = i < re ? p[i] : T(0);
-        xv[u] = (DEFER && i < re) ? x[i] : T(0);
+    for (int u = 0; u < U; ++u) {
+        const int i = i0 + u * kBlock * V;
+        zv[u] = load_pack<T, V>(z, i, re, T(0));
+        pv[u] = load_pack<T, V>(p, i, re, T(0));
+        if (DEFER) xv[u] = load_pack<T, V>(x, i, re, T(0));
     }
     T tot[2];
     site_total<T, 2>(partB, tot);        // its barriers publish `halt`
@@ -311,31 +348,37 @@ __global__ __launch_bounds__(kBlock) void k_cgj_p(const T* __restrict__ z, T* __
     const T beta = first ? T(0) : rho / (T)in.rho(kd - 1);        // CG.f90:183
     for (;;) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            const int i = i0 + u * kBlock;
-            if (i < re) {
-                if (carry) x[i] = xv[u] + alfa_prev * pv[u];            // CG.f90:227
-                p[i] = first ? zv[u] : zv[u] + beta * pv[u];            // CG.f90:180, :185
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * kBlock * V;
+            P xo, po;
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                if (carry) xo.v[e] = xv[u].v[e] + alfa_prev * pv[u].v[e];            // CG.f90:227
+                po.v[e] = first ? zv[u].v[e] : zv[u].v[e] + beta * pv[u].v[e];       // CG.f90:180, :185
             }
+            if (carry) store_pack<T, V>(x, i, re, xo);
+            store_pack<T, V>(p, i, re, po);
         }
         i0 += kUnroll * kBlock;
         if (i0 >= re) break;
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            const int i = i0 + u * kBlock;
-            zv[u] = i < re ? z[i] : T(0);
-            pv[u] = i < re ? p[i] : T(0);
-            xv[u] = (carry && i < re) ? x[i] : T(0);
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * kBlock * V;
+            zv[u] = load_pack<T, V>(z, i, re, T(0));
+            pv[u] = load_pack<T, V>(p, i, re, T(0));
+            if (carry) xv[u] = load_pack<T, V>(x, i, re, T(0));
         }
     }
 }
 
-template <typename T, bool DEFER>
+template <typename T, bool DEFER, int V>
 __global__ __launch_bounds__(kBlock) void k_cgj_update(int n, T* __restrict__ x, T* __restrict__ r,
                                                        T* __restrict__ z, const T* __restrict__ p,
                                                        const T* __restrict__ q, const T* __restrict__ D,
                                                        const T* __restrict__ partD, T* __restrict__ partB,
                                                        SolverState* st) {
+    constexpr int U = kUnroll / V;
+    using P = Pack<T, V>;
     // issued together: the state, this thread's first group of rows and
     // (in site_total) the partial sums of (p,q); the flags are not written
     // by this kernel, so every thread reads the same ones
@@ -346,21 +389,23 @@ __global__ __launch_bounds__(kBlock) void k_cgj_update(int n, T* __restrict__ x,
     block_tiles(ntiles, t0, t1);
     const int rb = t0 * kTileRows, re = min(t1 * kTileRows, n);
     T v[2] = {T(0), T(0)};
-    T xv[kUnroll], pv[kUnroll], rv[kUnroll], qv[kUnroll], dv[kUnroll];
+    P xv[U], pv[U], rv[U], qv[U], dv[U];
     // loads of the next group of rows
     auto load = [&](int i0) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            const int i = i0 + u * kBlock;
-            const bool ok = i < re;
-            xv[u] = (!DEFER && ok) ? x[i] : T(0);
-            pv[u] = (!DEFER && ok) ? p[i] : T(0);
-            rv[u] = ok ? r[i] : T(0);
-            qv[u] = ok ? q[i] : T(0);
-            dv[u] = ok ? D[i] : T(1);
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * kBlock * V;
+            if (!DEFER) {
+                xv[u] = load_pack<T, V>(x, i, re, T(0));
+                pv[u] = load_pack<T, V>(p, i, re, T(0));
+            }
+            rv[u] = load_pack<T, V>(r, i, re, T(0));
+            qv[u] = load_pack<T, V>(q, i, re, T(0));
+            dv[u] = load_pack<T, V>(D, i, re, T(1));
         }
     };
-    load(rb + (int)threadIdx.x);
+    const int ib = rb + (int)threadIdx.x * V;
+    load(ib);
     T pq[1];
     site_total<T, 1>(partD, pq);
     if (in.halt) return;
@@ -369,20 +414,27 @@ __global__ __launch_bounds__(kBlock) void k_cgj_update(int n, T* __restrict__ x,
         if (DEFER) st->alfa = (double)alfa;
         st->it = kd;
     }
-    for (int i0 = rb + (int)threadIdx.x; i0 < re; i0 += kUnroll * kBlock) {
-        if (i0 != rb + (int)threadIdx.x) load(i0);
+    for (int i0 = ib; i0 < re; i0 += kUnroll * kBlock) {
+        if (i0 != ib) load(i0);
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            const int i = i0 + u * kBlock;
-            if (i < re) {
-                if (!DEFER) x[i] = xv[u] + alfa * pv[u];    // CG.f90:227
-                const T ri = rv[u] - alfa * qv[u];          // CG.f90:228
-                r[i] = ri;
-                const T zi = ri / dv[u];                    // next z, CG.f90:126
-                z[i] = zi;
-                v[0] += ri * ri;                            // ||r||^2, CG.f90:250
-                v[1] += ri * zi;                            // next rho, CG.f90:174
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * kBlock * V;
+            P xo, ro, zo;
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                if (!DEFER) xo.v[e] = xv[u].v[e] + alfa * pv[u].v[e];     // CG.f90:227
+                const T ri = rv[u].v[e] - alfa * qv[u].v[e];              // CG.f90:228
+                ro.v[e] = ri;
+                const T zi = ri / dv[u].v[e];                             // next z, CG.f90:126
+                zo.v[e] = zi;
+                if (i + e < re) {
+                    v[0] += ri * ri;                                      // ||r||^2, CG.f90:250
+                    v[1] += ri * zi;                                      // next rho, CG.f90:174
+                }
             }
+            if (!DEFER) store_pack<T, V>(x, i, re, xo);
+            store_pack<T, V>(r, i, re, ro);
+            store_pack<T, V>(z, i, re, zo);
         }
     }
     site_write<T, 2>(v, partB);
@@ -435,19 +487,46 @@ void launch_cgf_init_z(int n, const T* r, const T* D, T* z, const Sites<T>& S, c
     TF_FL((k_cgf_init_z<T>), S.grid, s), n, r, D, z, S.B, st);
 }
 
+// 16-byte packs where every vector of the element-wise kernels allows them
+// (TFLOWS_CG_PACK=0: one row per lane everywhere)
+static bool packs16(std::initializer_list<const void*> ptrs) {
+    const char* e = std::getenv("TFLOWS_CG_PACK");
+    if (e && std::atoi(e) == 0) return false;
+    for (const void* q : ptrs)
+        if (reinterpret_cast<uintptr_t>(q) % 16 != 0) return false;
+    return true;
+}
+
+template <typename T, int V>
+static void launch_cgj_elementwise(bool mixed, bool carry, bool p_kernel, int n, T* x, T* r, T* z, T* p,
+                                   const T* q, const T* D, const Sites<T>& S, const double* partE, SolverState* st,
+                                   HostMirror* hm, hipStream_t s) {
+    if (p_kernel) {
+        if (carry) {
+            if (mixed) TF_FL((k_cgj_p<T, true, true, V>), S.grid, s), (const T*)z, p, x, S.B, partE, n, st, hm);
+            else TF_FL((k_cgj_p<T, false, true, V>), S.grid, s), (const T*)z, p, x, S.B, partE, n, st, hm);
+        } else {
+            if (mixed) TF_FL((k_cgj_p<T, true, false, V>), S.grid, s), (const T*)z, p, x, S.B, partE, n, st, hm);
+            else TF_FL((k_cgj_p<T, false, false, V>), S.grid, s), (const T*)z, p, x, S.B, partE, n, st, hm);
+        }
+    } else {
+        if (carry) TF_FL((k_cgj_update<T, true, V>), S.grid, s), n, x, r, z, (const T*)p, q, D, S.D, S.B, st);
+        else TF_FL((k_cgj_update<T, false, V>), S.grid, s), n, x, r, z, (const T*)p, q, D, S.D, S.B, st);
+    }
+}
+
 template <typename Op, typename T>
 void launch_cgj(bool mixed, bool strided, const Op& A, int n, T* x, T* r, T* z, T* p, T* q, const T* D,
                 const Sites<T>& S, const double* partE, SolverState* st, HostMirror* hm, hipStream_t s, bool carry) {
-    if (carry) {
-        if (mixed) TF_FL((k_cgj_p<T, true, true>), S.grid, s), (const T*)z, p, x, S.B, partE, n, st, hm);
-        else TF_FL((k_cgj_p<T, false, true>), S.grid, s), (const T*)z, p, x, S.B, partE, n, st, hm);
-    } else {
-        if (mixed) TF_FL((k_cgj_p<T, true, false>), S.grid, s), (const T*)z, p, x, S.B, partE, n, st, hm);
-        else TF_FL((k_cgj_p<T, false, false>), S.grid, s), (const T*)z, p, x, S.B, partE, n, st, hm);
-    }
+    constexpr int V = 16 / sizeof(T);
+    const bool wide = packs16({x, r, z, p, q, D});
+    auto elementwise = [&](bool p_kernel) {
+        if (wide) launch_cgj_elementwise<T, V>(mixed, carry, p_kernel, n, x, r, z, p, q, D, S, partE, st, hm, s);
+        else launch_cgj_elementwise<T, 1>(mixed, carry, p_kernel, n, x, r, z, p, q, D, S, partE, st, hm, s);
+    };
+    elementwise(true);
     TF_FL((k_cgj_spmv<Op>), S.grid, s), A, (const T*)p, q, S.D, st, strided ? 1 : 0);
-    if (carry) TF_FL((k_cgj_update<T, true>), S.grid, s), n, x, r, z, (const T*)p, (const T*)q, D, S.D, S.B, st);
-    else TF_FL((k_cgj_update<T, false>), S.grid, s), n, x, r, z, (const T*)p, (const T*)q, D, S.D, S.B, st);
+    elementwise(false);
 }
 template void launch_cgj<Csr<double>, double>(bool, bool, const Csr<double>&, int, double*, double*, double*, double*,
                                               double*, const double*, const Sites<double>&, const double*,

@@ -33,7 +33,7 @@ static_assert(kNoAllocMaxRows == TFLOWS_NOALLOC_MAX_ROWS, "the header states the
 constexpr int kNoAllocMaxGrid = 4096;
 constexpr size_t kStateDoubles = 64;                              // 512 B, one SolverState
 constexpr size_t kPartialDoubles = 8 * (size_t)kNoAllocMaxGrid;   // Sites<double>::elems(grid)
-constexpr size_t kVectorDoubles = 5 * (size_t)kNoAllocMaxRows;    // D, p, q, x and r1 in internal order
+constexpr size_t kVectorDoubles = 6 * (size_t)kNoAllocMaxRows;    // D, p, q, z, x and r1 in internal order
 constexpr size_t kScratchDoubles = kStateDoubles + kPartialDoubles + kVectorDoubles;
 static_assert(sizeof(SolverState) <= kStateDoubles * sizeof(double), "SolverState outgrew its scratch slot");
 
@@ -89,8 +89,8 @@ SolveResult solve_noalloc(System& S, int method, int prec, double* x, double* r1
         throw std::invalid_argument("the no-allocation solve runs CG (0) and BiCG (1) only");
     if (prec != 0 && prec != 1)
         throw std::invalid_argument("the no-allocation solve takes prec 0 (none) or 1 (diagonal) only");
-    if (cg_form() != 0)
-        throw std::invalid_argument("TFLOWS_CG_FORM selects CG kernels whose workspace the no-allocation "
+    if (cg_form() == 2)
+        throw std::invalid_argument("TFLOWS_CG_FORM=2 selects CG kernels whose workspace the no-allocation "
                                     "solve does not hold; unset it");
     const int n = S.n();
     if (n > kNoAllocMaxRows)
@@ -110,15 +110,16 @@ SolveResult solve_noalloc(System& S, int method, int prec, double* x, double* r1
     w.D = vec;
     w.p = vec + nn;
     w.q = vec + 2 * nn;
+    w.u1 = vec + 3 * nn;            // z = r / D of the device-indexed CG kernels
     w.x = x;
     w.r1 = r1;
     const bool reord = S.reordered();
     if (reord) {   // the kernels run in the system's internal numbering
-        w.x = vec + 3 * nn;
-        w.r1 = vec + 4 * nn;
+        w.x = vec + 4 * nn;
+        w.r1 = vec + 5 * nn;
     }
-    const unsigned zb = (unsigned)std::min<size_t>(2048, (3 * nn + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_zero, dim3(zb), dim3(kBlock), 0, s, 3 * nn, vec);
+    const unsigned zb = (unsigned)std::min<size_t>(2048, (4 * nn + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_zero, dim3(zb), dim3(kBlock), 0, s, 4 * nn, vec);
     hipLaunchKernelGGL(k_state_init, dim3(1), dim3(kWave), 0, s, st, tol, cap, stale_error);
     TF_CHECK(hipGetLastError());
     if (reord) {

@@ -16,16 +16,15 @@ namespace tflows {
 
 // Upper bound on the blocks of the row-streaming kernels (default 2048 =
 // 8 per CU); TFLOWS_MAX_BLOCKS overrides it for tuning.
-// Jacobi CG kernel form of the FP64 solve: 0 (default, fastest measured) =
-// host-indexed p update, SpMV, update; 3 = the same three kernels with
-// device iteration counters (the form the mixed-precision path uses);
-// 2 = p formed inside the SpMV gathers.  TFLOWS_CG_FORM selects.
+// Jacobi CG kernel form of one system's solve: 3 (default, fastest
+// measured) = p update, SpMV, update with device iteration counters (the
+// form the mixed-precision path uses); 0 = the same three kernels indexed
+// by the host (the batch kernels); 2 = p formed inside the SpMV gathers.
+// TFLOWS_CG_FORM selects (read as iterations are issued: set it between
+// solves).
 int cg_form() {
-    static const int f = [] {
-        const char* e = std::getenv("TFLOWS_CG_FORM");
-        return e ? std::atoi(e) : 0;
-    }();
-    return f;
+    const char* e = std::getenv("TFLOWS_CG_FORM");
+    return e ? std::atoi(e) : 3;
 }
 bool fused_cg_enabled() { return cg_form() != 0; }
 // Jacobi CG of one system in forms 0 and 3 (and the mixed-precision inner

@@ -235,28 +235,35 @@ def test_bicg_runs_as_cg_bitwise(pkg, oracle, cases, cuda, case, prec):
     """BiCG's shadow equals its primal bit for bit (the reference multiplies
     it by A and starts it at r), so the library runs BiCG through the CG
     kernels; the two-vector kernels (TFLOWS_BICG_SHADOW=1) must give the
-    same bits, and both the oracle's CG result."""
+    same bits as the CG kernels that sum in the same layout (host-indexed,
+    TFLOWS_CG_FORM=0), and the default CG form the same result to rounding;
+    all of them the oracle's CG result."""
     import os
     import torch
     s = cases[case]
     m = s.matrix
     ds = pkg.krylov.DeviceSystem(m)
     out = {}
-    for shadow in ("0", "1"):
+    for shadow, form in (("0", "0"), ("1", "0"), ("0", None)):
         os.environ["TFLOWS_BICG_SHADOW"] = shadow
+        if form is not None:
+            os.environ["TFLOWS_CG_FORM"] = form
         try:
             x = torch.zeros(m.n, dtype=torch.float64, device=cuda)
             r = torch.from_numpy(s.b.copy()).to(cuda)
             it, ini, fin = ds.solve("bicg", prec, x, r, 8, s.tol)
         finally:
             os.environ.pop("TFLOWS_BICG_SHADOW", None)
-        out[shadow] = (it, ini, fin, x.cpu().numpy(), r.cpu().numpy())
-    a, b = out["0"], out["1"]
+            os.environ.pop("TFLOWS_CG_FORM", None)
+        out[(shadow, form)] = (it, ini, fin, x.cpu().numpy(), r.cpu().numpy())
+    a, b, d = out[("0", "0")], out[("1", "0")], out[("0", None)]
     assert a[0] == b[0] and a[1] == b[1] and a[2] == b[2]
     assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
+    assert d[0] == a[0] and d[1] == a[1] and d[2] == pytest.approx(a[2], rel=1e-12)
+    assert _rel(d[3], a[3]) <= 1e-13
     ref = oracle.run("cg", m, s.b, prec=prec, niter=8, tol=s.tol)
     assert a[0] == ref["niter"] == 9
-    assert _rel(a[3], ref["x"]) <= 1e-11
+    assert _rel(a[3], ref["x"]) <= 1e-11 and _rel(d[3], ref["x"]) <= 1e-11
     ds.close()
 
 
