@@ -65,7 +65,10 @@ def format_bytes(n, nnz, word, layout):
     offsets; padding slots are not counted), the slice offsets
     4*(ceil(N/64)+1); with offsets shared by each slice, no per-entry
     column but every stored slot's value (absent neighbours included), one
-    mask byte per row and 4 bytes per slot column of a slice.  CSR: values
+    mask byte per row and 4 bytes per slot column of a slice; in the
+    symmetric form only the diagonal and upper slots are stored (each lower
+    value is the stored upper one, counted once; its 12 bytes of mirror
+    bases per slot column of a slice are not counted).  CSR: values
     and int32 columns of the entries, the row pointer 4*(N+1).  Plus x and
     y (word*N each)."""
     ent = nnz if word == 8 else nnz - n
@@ -82,6 +85,8 @@ def format_bytes(n, nnz, word, layout):
 def layout_name(l):
     if not l["sliced"]:
         return "CSR"
+    if l.get("symmetric"):
+        return "sliced ELL, column offsets shared by each slice, lower triangle read from its mirror"
     if l.get("shared"):
         return "sliced ELL, column offsets shared by each slice"
     return "sliced ELL, " + ("16-bit column offsets" if l["col_bytes"] == 2 else "int32 columns")
@@ -92,6 +97,8 @@ def layout_tag(l, bits):
     base = "fp32" if bits == 32 else "fp64"
     if not l["sliced"]:
         return base + "_csr"
+    if l.get("symmetric"):
+        return base + "_sym"
     return base + ("_shared" if l.get("shared") else ("_sliced16" if l["col_bytes"] == 2 else "_sliced"))
 
 

@@ -245,7 +245,11 @@ int32_t tflows_system_reordered(tflows_system s);
  * by each slice -- one ascending list of at most 8 offsets per slice and a
  * slot mask per row, no column stored per entry (chosen where each row's
  * columns ascend and the slices' offset lists stay short, as on a
- * naturally numbered structured mesh, unless TFLOWS_SHARED_SLOTS=0); 0:
+ * naturally numbered structured mesh, unless TFLOWS_SHARED_SLOTS=0); 4:
+ * layout 3 storing only the slots of non-negative offsets, each lower
+ * entry read from its stored mirror (the FP32 operator with
+ * TFLOWS_SYM_SLOTS32=1, the FP64 one unless TFLOWS_SYM_SLOTS=0, where the
+ * pattern allows it and every value so far has equalled its mirror); 0:
  * CSR (environment TFLOWS_FP32_OP=csr when the system was created, or
  * slower by the first use's timing).  *stored_entries: off-diagonal
  * entries held, padding included.  Every layout gives the same bits.
@@ -257,7 +261,8 @@ int tflows_system_fp32_layout(tflows_system s, int32_t* sliced, int64_t* stored_
  * diagonal where it stands in each row): *sliced = 1 sliced ELL, 2 the
  * same with 16-bit column offsets (where every offset fits, unless
  * TFLOWS_FP64_COL16=0), 3 the same with column offsets shared by each
- * slice (as for the FP32 operator), 0 the CSR arrays; *stored_entries:
+ * slice, 4 layout 3 with the lower entries read from their mirrors (both
+ * as for the FP32 operator), 0 the CSR arrays; *stored_entries:
  * entries held, padding included.  No counterpart in the reference. */
 int tflows_system_fp64_layout(tflows_system s, int32_t* sliced, int64_t* stored_entries, void* stream);
 

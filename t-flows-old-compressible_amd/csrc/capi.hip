@@ -363,7 +363,7 @@ int tflows_system_fp32_layout(tflows_system s, int32_t* sliced, int64_t* stored_
         const hipStream_t st = as_stream(stream);
         const bool sl = s->sys->op32_sliced(st);
         const int cb = sl ? s->sys->sell32_col_bytes(st) : 4;
-        *sliced = sl ? (cb == 0 ? 3 : (cb == 2 ? 2 : 1)) : 0;
+        *sliced = sl ? (cb == 0 ? (s->sys->sell32_symmetric(st) ? 4 : 3) : (cb == 2 ? 2 : 1)) : 0;
         *stored_entries = sl ? s->sys->sell32_entries(st) : s->sys->nnz() - s->sys->n();
         TF_CHECK(hipStreamSynchronize(st));
     });
@@ -375,7 +375,7 @@ int tflows_system_fp64_layout(tflows_system s, int32_t* sliced, int64_t* stored_
         const hipStream_t st = as_stream(stream);
         const bool sl = s->sys->op64_sliced(st);
         const int cb = sl ? s->sys->sell64_col_bytes(st) : 4;
-        *sliced = sl ? (cb == 0 ? 3 : (cb == 2 ? 2 : 1)) : 0;
+        *sliced = sl ? (cb == 0 ? (s->sys->sell64_symmetric(st) ? 4 : 3) : (cb == 2 ? 2 : 1)) : 0;
         *stored_entries = sl ? s->sys->sell64_entries(st) : s->sys->nnz();
         TF_CHECK(hipStreamSynchronize(st));
     });

@@ -161,6 +161,9 @@ struct DiffSell32 {
     // when not null: shared slot offsets (see Sell), read instead of col
     const int* __restrict__ soff = nullptr;
     const unsigned char* __restrict__ mask = nullptr;
+    // when not null: the symmetric shared-offset form (see SymSlots)
+    const int* __restrict__ sidx = nullptr;
+    const int2* __restrict__ lbase = nullptr;
 };
 constexpr int kSlice = 64;
 // most slots of a slice in the shared-offset form (bits of the row mask)
@@ -190,8 +193,29 @@ struct Sell {
     // slots of a row run in its CSR order.
     const int* __restrict__ soff = nullptr;
     const unsigned char* __restrict__ mask = nullptr;
+    // When not null: the symmetric form of the shared offsets, for a matrix
+    // whose values equal their mirrors (a_ij = a_ji bit for bit).  Only the
+    // slots of non-negative offsets are stored (slice s: off[s] + 64 (j -
+    // nlow) + l, nlow = its number of negative offsets); the offset list of
+    // slice s starts at soff[sidx[s]] and has sidx[s + 1] - sidx[s] slots.
+    // Lower slot j of slice s (offset d < 0) reads a_{i,i+d} as the stored
+    // a_{i+d,i} of row i' = i + d: at lbase[sidx[s] + j].x + i' when i' lies
+    // in slice (64 s + d) >> 6, else at .y + i' (the slice after).  The
+    // slots still run in ascending offset order, so every row's sum is the
+    // same as in the full form.
+    const int* __restrict__ sidx = nullptr;
+    const int2* __restrict__ lbase = nullptr;
 };
 constexpr short kPad16 = -32768;
+
+// Where lower slot j (offset d < 0, entry present) of row `row` in slice sl
+// finds its mirror value in a symmetric shared-offset layout.
+__device__ __forceinline__ int sym_mirror_pos(const int2* __restrict__ lbase, int sb, int j, int sl, int row, int d) {
+    const int2 lb = lbase[sb + j];
+    const int ip = row + d;
+    const int sa = (sl * 64 + d) >> 6;          // arithmetic shift: floor division by 64
+    return ((ip >> 6) == sa ? lb.x : lb.y) + ip;
+}
 
 template <typename Op> struct OpValue;
 template <typename T> struct OpValue<Csr<T>> { using type = T; };
@@ -595,7 +619,30 @@ __device__ __forceinline__ void sell_tile(const DiffSell32& A, int r0, int r1, G
 #pragma unroll
             for (int v = 0; v < NRHS; ++v) acc[v] = acc[v] + a * (gv[v] - own[v]);
         };
-        if (A.soff) {                                // offsets shared by the slice
+        if (A.lbase) {                               // symmetric: upper slots stored, lower ones mirrored
+            const int sb = A.sidx[sl];
+            const int ws = A.sidx[sl + 1] - sb;
+            const int nl = ws - w;
+            const int* __restrict__ so = A.soff + sb;
+            const unsigned m = A.mask[row];
+            // as in the full form, a missing slot adds a (p_i - p_i) with a = 0;
+            // a lower slot's load stays in bounds (index 0) when the row lacks
+            // it.  The lower slots come first in the ascending offsets: two
+            // straight-line loops keep each slot's loads independent
+#pragma unroll 2
+            for (int j = 0; j < nl; ++j) {
+                const bool has = ((m >> j) & 1u) != 0;
+                const int d = so[j];
+                const int pos = sym_mirror_pos(A.lbase, sb, j, sl, row, d);
+                const float av = A.val[has ? pos : 0];
+                slot(has ? row + d : row, has ? av : 0.0f);
+            }
+#pragma unroll 2
+            for (int j = nl; j < ws; ++j) {
+                const bool has = ((m >> j) & 1u) != 0;
+                slot(has ? row + so[j] : row, vp[(j - nl) * kSlice]);
+            }
+        } else if (A.soff) {                         // offsets shared by the slice
             // a slot the row lacks takes the row's own column with a = 0 and
             // adds a (p_i - p_i) = +-0, which leaves every sum as it is (one
             // that starts at +0 never becomes -0); no branch, so the loads of
@@ -647,7 +694,20 @@ __device__ __forceinline__ void sell_rows(const Sell<T>& A, int r0, int r1, Gath
                 acc[v] = SUBTRACT ? acc[v] - prod : acc[v] + prod;
             }
         };
-        if (A.soff) {                                // offsets shared by the slice
+        if (A.lbase) {                               // symmetric: upper slots stored, lower ones mirrored
+            const int sb = A.sidx[sl];
+            const int ws = A.sidx[sl + 1] - sb;
+            const int nl = ws - w;
+            const int* __restrict__ so = A.soff + sb;
+            const unsigned m = A.mask[row];
+            // the lower slots (mirrored) come first in the ascending offsets
+#pragma unroll 2
+            for (int j = 0; j < nl; ++j)
+                if ((m >> j) & 1u) slot(row + so[j], A.val[sym_mirror_pos(A.lbase, sb, j, sl, row, so[j])]);
+#pragma unroll 2
+            for (int j = nl; j < ws; ++j)
+                if ((m >> j) & 1u) slot(row + so[j], vp[(j - nl) * kSlice]);
+        } else if (A.soff) {                         // offsets shared by the slice
             const int* __restrict__ so = A.soff + b / kSlice;
             const unsigned m = A.mask[row];
 #pragma unroll 2

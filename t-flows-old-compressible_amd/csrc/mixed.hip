@@ -242,6 +242,10 @@ DiffSell32 System::sell32(hipStream_t s) {
         TF_CHECK(hipStreamSynchronize(s));
         std::vector<int> soff;
         std::vector<unsigned char> mask;
+        sell_sidx_.release();
+        sell_lbase_.release();
+        sell_pairs_.release();
+        sell_npairs_ = 0;
         if (shared_slots_from_env() && noff > 0 && shared_slot_layout(n_, orp, ocol, off, soff, mask, map)) {
             // one column offset per slot of a slice, a mask per row
             sell_soff_.ensure(soff.size());
@@ -250,6 +254,25 @@ DiffSell32 System::sell32(hipStream_t s) {
             TF_CHECK(hipMemcpy(sell_mask_.p, mask.data(), mask.size(), hipMemcpyHostToDevice));
             sell_col_.release();
             sell_dc_.release();
+            // the symmetric form where the pattern allows it; the values are
+            // checked at every change (below)
+            SymSlots sym;
+            if (sym_slots_from_env(true) && !sym32_refused_ &&
+                symmetric_slot_layout(n_, orp, ocol, off, soff, mask, map, sym)) {
+                off.swap(sym.off);
+                map.swap(sym.map);
+                sell_sidx_.ensure(sym.sidx.size());
+                sell_lbase_.ensure(sym.lbase.size());
+                sell_pairs_.ensure(sym.pairs.size());
+                TF_CHECK(hipMemcpy(sell_sidx_.p, sym.sidx.data(), sizeof(int) * sym.sidx.size(),
+                                   hipMemcpyHostToDevice));
+                TF_CHECK(hipMemcpy(sell_lbase_.p, sym.lbase.data(), sizeof(int2) * sym.lbase.size(),
+                                   hipMemcpyHostToDevice));
+                if (!sym.pairs.empty())
+                    TF_CHECK(hipMemcpy(sell_pairs_.p, sym.pairs.data(), sizeof(int) * sym.pairs.size(),
+                                       hipMemcpyHostToDevice));
+                sell_npairs_ = (int64_t)sym.pairs.size() / 2;
+            }
         } else {
             sell_soff_.release();
             sell_mask_.release();
@@ -278,13 +301,20 @@ DiffSell32 System::sell32(hipStream_t s) {
         sell_valid_ = false;
     }
     if (!sell_valid_) {
+        if (sell_lbase_.p && !pairs_symmetric(sell_pairs_, sell_npairs_, d.val, symflag_, s)) {
+            // values that differ from their mirrors: the full form from now on
+            sym32_refused_ = true;
+            sell_built_ = false;
+            return sell32(s);
+        }
         if (sell_entries_ > 0)
             hipLaunchKernelGGL(k_sell_values, dim3((unsigned)((sell_entries_ + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                                s, sell_entries_, (const int*)sell_map_.p, d.val, sell_val_.p);
         TF_CHECK(hipGetLastError());
         sell_valid_ = true;
     }
-    return DiffSell32{n_, sell_off_.p, sell_col_.p, sell_val_.p, d.rs, sell_dc_.p, sell_soff_.p, sell_mask_.p};
+    return DiffSell32{n_,           sell_off_.p,  sell_col_.p,  sell_val_.p,   d.rs,
+                      sell_dc_.p,   sell_soff_.p, sell_mask_.p, sell_sidx_.p, sell_lbase_.p};
 }
 
 const float* System::diag32(hipStream_t s) {

@@ -148,6 +148,26 @@ bool shared_slot_layout(int n, const std::vector<int>& rp, const std::vector<int
                         std::vector<int>& soff, std::vector<unsigned char>& mask, std::vector<int>& map);
 bool shared_slots_from_env();
 
+// The symmetric form of a shared-offset layout (kernels.h, Sell::lbase):
+// only the slots of non-negative offsets keep values (off, map as for the
+// full form); sidx = each slice's first offset in soff; lbase = per slice
+// and lower slot, where the mirror values lie; pairs = (lower entry, its
+// mirror entry) for every lower entry, flattened, so that the device can
+// check a set of values is symmetric.  Every lower entry's mirror position
+// is checked here against the pattern: false when one fails (or nothing is
+// stored).  sym_slots_from_env says whether the FP64 (TFLOWS_SYM_SLOTS,
+// default on) or FP32 (TFLOWS_SYM_SLOTS32, default off) operator tries it.
+struct SymSlots {
+    std::vector<int> off, sidx, map, pairs;
+    std::vector<int2> lbase;
+};
+bool symmetric_slot_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl, const std::vector<int>& off,
+                           const std::vector<int>& soff, const std::vector<unsigned char>& mask,
+                           const std::vector<int>& map, SymSlots& out);
+bool sym_slots_from_env(bool fp32);
+template <typename T>
+bool pairs_symmetric(const DevArray<int>& pairs, int64_t npairs, const T* a, DevArray<int>& flag, hipStream_t s);
+
 // TFLOWS_DIC_SWEEP (read when a system first builds its levels): 0 sweeps
 // the matrix rows level by level, 1 the level-major copy level by level,
 // 2 (default) the level-major copy one launch per triangle.  All three
@@ -271,6 +291,15 @@ class System {
     int64_t sell64_entries(hipStream_t s) {
         sell64(s);
         return sell64_entries_;
+    }
+    // whether the sliced operators are in the symmetric form
+    bool sell64_symmetric(hipStream_t s) {
+        sell64(s);
+        return sell64_lbase_.p != nullptr;
+    }
+    bool sell32_symmetric(hipStream_t s) {
+        sell32(s);
+        return sell_lbase_.p != nullptr;
     }
     const float* diag32(hipStream_t s);
 
@@ -419,6 +448,13 @@ class System {
     DevArray<double> sell64_val_;
     int64_t sell64_entries_ = 0;
     bool sell64_built_ = false, sell64_valid_ = false;
+    // symmetric forms of the two sliced operators (where the pattern and the
+    // values allow): slice offset starts, mirror bases, entry pairs
+    DevArray<int> sell64_sidx_, sell64_pairs_, sell_sidx_, sell_pairs_;
+    DevArray<int2> sell64_lbase_, sell_lbase_;
+    int64_t sell64_npairs_ = 0, sell_npairs_ = 0;
+    bool sym64_refused_ = false, sym32_refused_ = false;   // values seen that are not symmetric
+    DevArray<int> symflag_;
     int op64_mode_ = 2;
     DevArray<float> off_val_, rowsum32_, diag32_;
     bool diff32_valid_ = false;

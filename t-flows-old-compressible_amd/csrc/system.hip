@@ -197,6 +197,124 @@ bool shared_slot_layout(int n, const std::vector<int>& rp, const std::vector<int
     return true;
 }
 
+// The symmetric shared-offset form: FP64 operator unless TFLOWS_SYM_SLOTS=0;
+// FP32 operator when TFLOWS_SYM_SLOTS32=1 (its matrix at 128^3 sits in the
+// MALL, where halving its values saves no time and the mirror reads cost
+// some; at 256^3 the two forms measured the same).
+bool sym_slots_from_env(bool fp32) {
+    const char* e = std::getenv(fp32 ? "TFLOWS_SYM_SLOTS32" : "TFLOWS_SYM_SLOTS");
+    return fp32 ? (e && e[0] == '1') : !(e && e[0] == '0');
+}
+
+static int floor_div64(int64_t v) { return (int)(v >= 0 ? v / 64 : -((-v + 63) / 64)); }
+
+bool symmetric_slot_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl, const std::vector<int>& off,
+                           const std::vector<int>& soff, const std::vector<unsigned char>& mask,
+                           const std::vector<int>& map, SymSlots& out) {
+    const int nsl = (int)off.size() - 1;
+    if (nsl < 1) return false;
+    std::vector<int> nl(nsl), w(nsl);
+    out.sidx.assign(nsl + 1, 0);
+    out.off.assign(nsl + 1, 0);
+    for (int sl = 0; sl < nsl; ++sl) {
+        w[sl] = (off[sl + 1] - off[sl]) / kSlice;
+        out.sidx[sl] = off[sl] / kSlice;
+        const int* so = soff.data() + out.sidx[sl];
+        int c = 0;
+        while (c < w[sl] && so[c] < 0) ++c;   // the offsets ascend: the negative ones first
+        nl[sl] = c;
+        const int64_t next = (int64_t)out.off[sl] + (int64_t)(w[sl] - c) * kSlice;
+        if (next > INT32_MAX) return false;
+        out.off[sl + 1] = (int)next;
+    }
+    out.sidx[nsl] = off[nsl] / kSlice;
+    if (out.off[nsl] == 0) return false;
+    out.map.assign(out.off[nsl], -1);
+    for (int sl = 0; sl < nsl; ++sl)
+        for (int j = nl[sl]; j < w[sl]; ++j)
+            for (int l = 0; l < kSlice; ++l)
+                out.map[out.off[sl] + kSlice * (j - nl[sl]) + l] = map[off[sl] + kSlice * j + l];
+    // the slot of column offset v in slice t, -1 when the slice has none
+    auto find = [&](int t, int v) {
+        const int* so = soff.data() + out.sidx[t];
+        for (int j = 0; j < w[t]; ++j)
+            if (so[j] == v) return j;
+        return -1;
+    };
+    constexpr int kNone = INT32_MIN / 2;
+    out.lbase.assign(out.sidx[nsl], int2{0, 0});
+    for (int sl = 0; sl < nsl; ++sl) {
+        const int* so = soff.data() + out.sidx[sl];
+        for (int j = 0; j < nl[sl]; ++j) {
+            const int d = so[j];
+            int base[2];
+            for (int t = 0; t < 2; ++t) {
+                const int sp = floor_div64((int64_t)kSlice * sl + (t ? kSlice - 1 : 0) + d);
+                base[t] = kNone;
+                if (sp >= 0 && sp < nsl) {
+                    const int jp = find(sp, -d);
+                    if (jp >= nl[sp]) base[t] = out.off[sp] + kSlice * (jp - nl[sp]) - kSlice * sp;
+                }
+            }
+            out.lbase[out.sidx[sl] + j] = int2{base[0], base[1]};
+        }
+    }
+    // every lower entry, read where the kernels will read it: the stored
+    // slot there must be its mirror (j, i); the pairs let the device check
+    // that the values agree
+    out.pairs.clear();
+    for (int sl = 0; sl < nsl; ++sl) {
+        const int* so = soff.data() + out.sidx[sl];
+        for (int l = 0; l < kSlice; ++l) {
+            const int i = sl * kSlice + l;
+            if (i >= n) break;
+            for (int j = 0; j < nl[sl]; ++j) {
+                if (!((mask[i] >> j) & 1u)) continue;
+                const int d = so[j];
+                const int e = map[off[sl] + kSlice * j + l];
+                const int ip = i + d;
+                if (e < 0 || ip < 0 || ip >= n) return false;
+                const int sa = floor_div64((int64_t)kSlice * sl + d);
+                const int2 lb = out.lbase[out.sidx[sl] + j];
+                const int b = (ip / kSlice == sa) ? lb.x : lb.y;
+                if (b == kNone) return false;
+                const int64_t pos = (int64_t)b + ip;
+                if (pos < 0 || pos >= out.off[nsl]) return false;
+                const int em = out.map[pos];
+                if (em < 0 || em < rp[ip] || em >= rp[ip + 1] || cl[em] != i) return false;
+                out.pairs.push_back(e);
+                out.pairs.push_back(em);
+            }
+        }
+    }
+    return true;
+}
+
+// 1 in *flag when some pair of entries (a lower one and its mirror) differ
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pair_asymmetry(int64_t npairs, const int* __restrict__ pairs,
+                                                           const T* __restrict__ a, int* __restrict__ flag) {
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < npairs; q += (int64_t)gridDim.x * kBlock)
+        if (!(a[pairs[2 * q]] == a[pairs[2 * q + 1]])) *flag = 1;
+}
+
+// true when the values a (CSR order) equal their mirrors at every pair
+template <typename T>
+bool pairs_symmetric(const DevArray<int>& pairs, int64_t npairs, const T* a, DevArray<int>& flag, hipStream_t s) {
+    if (npairs == 0) return true;
+    flag.ensure(1);
+    TF_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+    hipLaunchKernelGGL((k_pair_asymmetry<T>), dim3((unsigned)std::min<int64_t>(2048, (npairs + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, npairs, (const int*)pairs.p, a, flag.p);
+    TF_CHECK(hipGetLastError());
+    int asym = 0;
+    TF_CHECK(hipMemcpyAsync(&asym, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    TF_CHECK(hipStreamSynchronize(s));
+    return asym == 0;
+}
+template bool pairs_symmetric<double>(const DevArray<int>&, int64_t, const double*, DevArray<int>&, hipStream_t);
+template bool pairs_symmetric<float>(const DevArray<int>&, int64_t, const float*, DevArray<int>&, hipStream_t);
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_slot_values(int64_t count, const int* __restrict__ map,
                                                         const T* __restrict__ src, T* __restrict__ dst) {
@@ -212,6 +330,10 @@ Sell<double> System::sell64(hipStream_t s) {
         TF_CHECK(hipStreamSynchronize(s));
         std::vector<int> soff;
         std::vector<unsigned char> mask;
+        sell64_sidx_.release();
+        sell64_lbase_.release();
+        sell64_pairs_.release();
+        sell64_npairs_ = 0;
         if (shared_slots_from_env() && shared_slot_layout(n_, rp, cl, off, soff, mask, map)) {
             // one column offset per slot of a slice, a mask per row
             sell64_soff_.ensure(soff.size());
@@ -220,6 +342,25 @@ Sell<double> System::sell64(hipStream_t s) {
             TF_CHECK(hipMemcpy(sell64_mask_.p, mask.data(), mask.size(), hipMemcpyHostToDevice));
             sell64_col_.release();
             sell64_dc_.release();
+            // the symmetric form where the pattern allows it; the values are
+            // checked at every change (below)
+            SymSlots sym;
+            if (sym_slots_from_env(false) && !sym64_refused_ &&
+                symmetric_slot_layout(n_, rp, cl, off, soff, mask, map, sym)) {
+                off.swap(sym.off);
+                map.swap(sym.map);
+                sell64_sidx_.ensure(sym.sidx.size());
+                sell64_lbase_.ensure(sym.lbase.size());
+                sell64_pairs_.ensure(sym.pairs.size());
+                TF_CHECK(hipMemcpy(sell64_sidx_.p, sym.sidx.data(), sizeof(int) * sym.sidx.size(),
+                                   hipMemcpyHostToDevice));
+                TF_CHECK(hipMemcpy(sell64_lbase_.p, sym.lbase.data(), sizeof(int2) * sym.lbase.size(),
+                                   hipMemcpyHostToDevice));
+                if (!sym.pairs.empty())
+                    TF_CHECK(hipMemcpy(sell64_pairs_.p, sym.pairs.data(), sizeof(int) * sym.pairs.size(),
+                                       hipMemcpyHostToDevice));
+                sell64_npairs_ = (int64_t)sym.pairs.size() / 2;
+            }
         } else {
             sell64_soff_.release();
             sell64_mask_.release();
@@ -246,14 +387,20 @@ Sell<double> System::sell64(hipStream_t s) {
         sell64_valid_ = false;
     }
     if (!sell64_valid_) {
+        if (sell64_lbase_.p && !pairs_symmetric(sell64_pairs_, sell64_npairs_, (const double*)val_.p, symflag_, s)) {
+            // values that differ from their mirrors: the full form from now on
+            sym64_refused_ = true;
+            sell64_built_ = false;
+            return sell64(s);
+        }
         hipLaunchKernelGGL((k_slot_values<double>), dim3((unsigned)((sell64_entries_ + kBlock - 1) / kBlock)),
                            dim3(kBlock), 0, s, sell64_entries_, (const int*)sell64_map_.p, (const double*)val_.p,
                            sell64_val_.p);
         TF_CHECK(hipGetLastError());
         sell64_valid_ = true;
     }
-    return Sell<double>{n_, sell64_off_.p, sell64_col_.p, sell64_val_.p, sell64_dc_.p, sell64_soff_.p,
-                        sell64_mask_.p};
+    return Sell<double>{n_,           sell64_off_.p,  sell64_col_.p,  sell64_val_.p, sell64_dc_.p,
+                        sell64_soff_.p, sell64_mask_.p, sell64_sidx_.p, sell64_lbase_.p};
 }
 
 bool col16_from_env(const char* name) {

@@ -309,9 +309,11 @@ def solve_system(method: str, m, b, x0=None, prec=1, niter=2000, tol=0.0, nb=0):
 def _layout(code: int, entries: int) -> dict:
     """Operator layout from the library's code: sliced or CSR; bytes per
     stored column index (4, 2 for 16-bit offsets, 0 when the slices share
-    their column offsets); entries held, padding included."""
-    return {"sliced": code > 0, "shared": code == 3, "col_bytes": {3: 0, 2: 2}.get(code, 4),
-            "stored_entries": int(entries)}
+    their column offsets); whether only the non-negative offsets' slots are
+    stored, lower entries read from their mirrors; entries held, padding
+    included."""
+    return {"sliced": code > 0, "shared": code in (3, 4), "symmetric": code == 4,
+            "col_bytes": {3: 0, 4: 0, 2: 2}.get(code, 4), "stored_entries": int(entries)}
 
 
 def _ptr(t) -> int:

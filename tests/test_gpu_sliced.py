@@ -153,9 +153,14 @@ def test_mixed_replacement_residual_bitwise(pkg, cuda, monkeypatch, case, prec, 
 
 # layouts of the sliced operators: per-entry int32 columns, 16-bit offsets,
 # and column offsets shared by each slice (one list per slice, a mask per row)
+# and, for symmetric values, only the diagonal and upper slots stored with
+# the lower ones read from their mirrors (TFLOWS_SYM_SLOTS)
 LAYOUTS = {"int32": {"TFLOWS_SHARED_SLOTS": "0", "TFLOWS_FP64_COL16": "0", "TFLOWS_FP32_COL16": "0"},
            "col16": {"TFLOWS_SHARED_SLOTS": "0", "TFLOWS_FP64_COL16": "1", "TFLOWS_FP32_COL16": "1"},
-           "shared": {"TFLOWS_SHARED_SLOTS": "1", "TFLOWS_FP64_COL16": "1", "TFLOWS_FP32_COL16": "1"}}
+           "shared_full": {"TFLOWS_SHARED_SLOTS": "1", "TFLOWS_SYM_SLOTS": "0", "TFLOWS_SYM_SLOTS32": "0",
+                           "TFLOWS_FP64_COL16": "1", "TFLOWS_FP32_COL16": "1"},
+           "shared": {"TFLOWS_SHARED_SLOTS": "1", "TFLOWS_SYM_SLOTS": "1", "TFLOWS_SYM_SLOTS32": "1",
+                      "TFLOWS_FP64_COL16": "1", "TFLOWS_FP32_COL16": "1"}}
 
 
 @pytest.mark.parametrize("case", ["pressure", "channel_aniso", "pressure_perm"])
@@ -166,7 +171,9 @@ def test_shared_slot_layout_bitwise(pkg, oracle, cuda, monkeypatch, case):
     mixed solve give the other layouts' bits (and the FP64 SpMV the
     oracle's).  It is chosen on naturally numbered structured meshes; the
     randomly renumbered system (locality order, columns no longer ascending
-    within a row) keeps per-entry columns."""
+    within a row) keeps per-entry columns.  With symmetric values the shared
+    layout stores only the diagonal and upper slots and reads each lower
+    value from its mirror, in the same slot order: the same bits again."""
     import torch
     mg = pkg.meshgen
     # cubes: flat boxes are stored in their reverse Cuthill-McKee order,
@@ -208,12 +215,70 @@ def test_shared_slot_layout_bitwise(pkg, oracle, cuda, monkeypatch, case):
         assert not shared[0][0]["shared"] and not shared[0][1]["shared"]
     else:
         assert shared[0][0]["shared"] and shared[0][1]["shared"]
-        assert shared[0][0]["stored_entries"] <= 1.15 * m.nnz
+        assert shared[0][0]["symmetric"] and shared[0][1]["symmetric"]
+        assert not out["shared_full"][0][0]["symmetric"] and not out["shared_full"][0][1]["symmetric"]
+        # symmetric: about (nnz + N) / 2 FP64 slots and (nnz - N) / 2 FP32 ones
+        full = out["shared_full"][0]
+        assert full[0]["stored_entries"] <= 1.15 * m.nnz
+        assert shared[0][0]["stored_entries"] <= 0.6 * full[0]["stored_entries"]
+        assert shared[0][1]["stored_entries"] <= 0.55 * full[1]["stored_entries"]
     assert np.array_equal(shared[1], oracle.spmv(m, p64))
-    for other in ("int32", "col16"):
+    for other in ("int32", "col16", "shared_full"):
         o = out[other]
         assert np.array_equal(o[1], shared[1]) and np.array_equal(o[2], shared[2]), other
         for (ra, xa, rra), (rb, xb, rrb) in zip(o[3], shared[3]):
             assert ra == rb and np.array_equal(xa, xb) and np.array_equal(rra, rrb), other
         assert o[4][:3] == shared[4][:3] and np.array_equal(o[5], shared[5]), other
     assert shared[3][0][0][2] < s.tol and shared[4][2] < s.tol
+
+
+def test_symmetric_slots_fall_back(pkg, oracle, cuda, monkeypatch):
+    """A symmetric pattern whose values are not symmetric (the momentum
+    matrix; a pressure matrix given such values later) keeps the full
+    shared-offset form: the device compares every lower entry with its
+    mirror whenever the values change.  The results are the full form's
+    bits, and the SpMVs the oracle's."""
+    import torch
+    mg = pkg.meshgen
+    monkeypatch.setenv("TFLOWS_FP64_OP", "sliced")
+    monkeypatch.setenv("TFLOWS_FP32_OP", "sliced")
+    mom = mg.momentum_system(14, seed=94).matrix
+    pre = mg.pressure_system(14, seed=95)
+    rng = np.random.default_rng(96)
+    asym = pre.matrix.aval.copy()
+    off = np.flatnonzero(pre.matrix.row_of_entry() != pre.matrix.arow[:pre.matrix.nnz] - 1)
+    asym[off] *= 1.0 + 0.01 * rng.random(off.size)
+    p64 = rng.standard_normal(pre.matrix.n)
+
+    def run(m, vals, sym):
+        monkeypatch.setenv("TFLOWS_SYM_SLOTS", sym)
+        monkeypatch.setenv("TFLOWS_SYM_SLOTS32", sym)
+        ds = pkg.krylov.DeviceSystem(m)
+        lay0 = ds.fp64_layout()
+        if vals is not None:
+            ds.set_values(torch.from_numpy(vals).to(cuda))
+        p = torch.from_numpy(p64).to(cuda)
+        q, q32 = torch.empty_like(p), torch.empty_like(p.float())
+        ds.spmv(p, q)
+        ds.spmv32(p.float(), q32)
+        b = torch.from_numpy(pre.b.copy()).to(cuda)
+        x = torch.zeros(m.n, dtype=torch.float64, device=cuda)
+        r = b.clone()
+        res = ds.solve("cgs", 1, x, r, 300, pre.tol)
+        out = (lay0, ds.fp64_layout(), ds.fp32_layout(), q.cpu().numpy(), q32.cpu().numpy(), res,
+               x.cpu().numpy(), r.cpu().numpy())
+        ds.close()
+        return out
+
+    for m, vals in ((mom, None), (pre.matrix, asym)):
+        a, b = run(m, vals, "1"), run(m, vals, "0")
+        print(f"n {m.n}: layouts {a[0]} -> {a[1]}, {a[2]} / {b[1]}, {b[2]}", flush=True)
+        assert a[1] == b[1] and a[2] == b[2]          # the full form in both
+        assert not a[1]["symmetric"] and not a[2]["symmetric"]
+        for u, v in zip(a[3:], b[3:]):
+            assert (u == v) if not isinstance(u, np.ndarray) else np.array_equal(u, v)
+        mv = m if vals is None else m.with_values(vals)
+        assert np.array_equal(a[3], oracle.spmv(mv, p64))
+    # the pressure system had started out symmetric
+    sym0 = run(pre.matrix, None, "1")[0]
+    assert sym0["symmetric"] and sym0["stored_entries"] < a[1]["stored_entries"]
