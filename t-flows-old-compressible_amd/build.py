@@ -25,7 +25,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 ARCH = "gfx950"
 SOURCES = ["krylov_kernels.hip", "cg_fused.hip", "system.hip", "reorder.hip", "capi.hip", "mixed.hip", "batched.hip",
-           "adjoint.hip", "calcps.hip", "meshio.cpp", "noalloc.hip", "tile_plan.cpp"]
+           "adjoint.hip", "calcps.hip", "meshio.cpp", "noalloc.hip", "tile_plan.cpp", "slot_layout.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

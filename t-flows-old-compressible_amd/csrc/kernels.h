@@ -165,7 +165,7 @@ struct DiffSell32 {
     const int* __restrict__ sidx = nullptr;
     const int2* __restrict__ lbase = nullptr;
 };
-constexpr int kSlice = 64;
+constexpr int kSlice = 64;   // kSliceRows of slot_layout.h
 // most slots of a slice in the shared-offset form (bits of the row mask)
 constexpr int kMaxSharedSlots = 8;
 

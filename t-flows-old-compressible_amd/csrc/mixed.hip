@@ -266,7 +266,7 @@ DiffSell32 System::sell32(hipStream_t s) {
                 sell_pairs_.ensure(sym.pairs.size());
                 TF_CHECK(hipMemcpy(sell_sidx_.p, sym.sidx.data(), sizeof(int) * sym.sidx.size(),
                                    hipMemcpyHostToDevice));
-                TF_CHECK(hipMemcpy(sell_lbase_.p, sym.lbase.data(), sizeof(int2) * sym.lbase.size(),
+                TF_CHECK(hipMemcpy(sell_lbase_.p, sym.lbase.data(), sizeof(SlotPair) * sym.lbase.size(),
                                    hipMemcpyHostToDevice));
                 if (!sym.pairs.empty())
                     TF_CHECK(hipMemcpy(sell_pairs_.p, sym.pairs.data(), sizeof(int) * sym.pairs.size(),
@@ -314,7 +314,7 @@ DiffSell32 System::sell32(hipStream_t s) {
         sell_valid_ = true;
     }
     return DiffSell32{n_,           sell_off_.p,  sell_col_.p,  sell_val_.p,   d.rs,
-                      sell_dc_.p,   sell_soff_.p, sell_mask_.p, sell_sidx_.p, sell_lbase_.p};
+                      sell_dc_.p,   sell_soff_.p, sell_mask_.p, sell_sidx_.p, reinterpret_cast<const int2*>(sell_lbase_.p)};
 }
 
 const float* System::diag32(hipStream_t s) {

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "krylov_kernels.h"
+#include "slot_layout.h"
 
 namespace tflows {
 
@@ -136,34 +137,11 @@ bool slot_offsets16(int n, const std::vector<int>& off, const std::vector<int>& 
 bool col16_from_env(const char* name);
 void slice_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl, bool pad_own, std::vector<int>& off,
                   std::vector<int>& col, std::vector<int>& map);
-// The shared-offset sliced layout (kernels.h, Sell::soff) of a CSR pattern
-// with row pointer rp and columns cl: slice value offsets, each slice's
-// ascending column offsets, the per-row slot masks and the CSR entry of
-// every slot (-1 where the row lacks the slot: value 0, never read).  Made
-// when every row's columns ascend without repeats, no slice needs more than
-// kMaxSharedSlots offsets and the slots stay within 1.15 x the entries;
-// false otherwise.  TFLOWS_SHARED_SLOTS=0 (shared_slots_from_env) never
-// makes it.
-bool shared_slot_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl, std::vector<int>& off,
-                        std::vector<int>& soff, std::vector<unsigned char>& mask, std::vector<int>& map);
 bool shared_slots_from_env();
 
-// The symmetric form of a shared-offset layout (kernels.h, Sell::lbase):
-// only the slots of non-negative offsets keep values (off, map as for the
-// full form); sidx = each slice's first offset in soff; lbase = per slice
-// and lower slot, where the mirror values lie; pairs = (lower entry, its
-// mirror entry) for every lower entry, flattened, so that the device can
-// check a set of values is symmetric.  Every lower entry's mirror position
-// is checked here against the pattern: false when one fails (or nothing is
-// stored).  sym_slots_from_env says whether the FP64 (TFLOWS_SYM_SLOTS,
-// default on) or FP32 (TFLOWS_SYM_SLOTS32, default off) operator tries it.
-struct SymSlots {
-    std::vector<int> off, sidx, map, pairs;
-    std::vector<int2> lbase;
-};
-bool symmetric_slot_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl, const std::vector<int>& off,
-                           const std::vector<int>& soff, const std::vector<unsigned char>& mask,
-                           const std::vector<int>& map, SymSlots& out);
+// Whether the FP64 (TFLOWS_SYM_SLOTS, default on) or the FP32
+// (TFLOWS_SYM_SLOTS32, default off) operator tries the symmetric form of
+// the shared-offset layout (slot_layout.h).
 bool sym_slots_from_env(bool fp32);
 template <typename T>
 bool pairs_symmetric(const DevArray<int>& pairs, int64_t npairs, const T* a, DevArray<int>& flag, hipStream_t s);
@@ -451,7 +429,7 @@ class System {
     // symmetric forms of the two sliced operators (where the pattern and the
     // values allow): slice offset starts, mirror bases, entry pairs
     DevArray<int> sell64_sidx_, sell64_pairs_, sell_sidx_, sell_pairs_;
-    DevArray<int2> sell64_lbase_, sell_lbase_;
+    DevArray<SlotPair> sell64_lbase_, sell_lbase_;   // read by the kernels as int2
     int64_t sell64_npairs_ = 0, sell_npairs_ = 0;
     bool sym64_refused_ = false, sym32_refused_ = false;   // values seen that are not symmetric
     DevArray<int> symflag_;

@@ -158,44 +158,9 @@ bool shared_slots_from_env() {
     return !(e && e[0] == '0');
 }
 
-bool shared_slot_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl, std::vector<int>& off,
-                        std::vector<int>& soff, std::vector<unsigned char>& mask, std::vector<int>& map) {
-    const int nsl = (n + kSlice - 1) / kSlice;
-    const int64_t nnz = rp[n];
-    off.assign(nsl + 1, 0);
-    soff.clear();
-    std::vector<int> set;
-    for (int sl = 0; sl < nsl; ++sl) {
-        set.clear();
-        for (int i = sl * kSlice; i < std::min(n, (sl + 1) * kSlice); ++i)
-            for (int e = rp[i]; e < rp[i + 1]; ++e) {
-                if (e > rp[i] && cl[e] <= cl[e - 1]) return false;   // the slots must run in CSR order
-                set.push_back(cl[e] - i);
-            }
-        std::sort(set.begin(), set.end());
-        set.erase(std::unique(set.begin(), set.end()), set.end());
-        if ((int)set.size() > kMaxSharedSlots) return false;
-        const int64_t next = (int64_t)off[sl] + (int64_t)set.size() * kSlice;
-        if (next > INT32_MAX) return false;
-        off[sl + 1] = (int)next;
-        soff.insert(soff.end(), set.begin(), set.end());
-    }
-    if ((double)off[nsl] > 1.15 * (double)std::max<int64_t>(nnz, 1)) return false;
-    mask.assign((size_t)nsl * kSlice, 0);
-    map.assign(off[nsl], -1);
-    for (int sl = 0; sl < nsl; ++sl) {
-        const int* so = soff.data() + off[sl] / kSlice;
-        for (int i = sl * kSlice; i < std::min(n, (sl + 1) * kSlice); ++i) {
-            int j = 0;
-            for (int e = rp[i]; e < rp[i + 1]; ++e) {
-                while (so[j] != cl[e] - i) ++j;                      // both ascend; every offset is in the set
-                mask[i] |= (unsigned char)(1u << j);
-                map[off[sl] + j * kSlice + (i - sl * kSlice)] = e;
-            }
-        }
-    }
-    return true;
-}
+static_assert(kSlice == kSliceRows && kMaxSharedSlots == kMaxSliceSlots,
+              "the host layouts and the kernels agree on the slice shape");
+static_assert(sizeof(SlotPair) == sizeof(int2) && alignof(int2) <= 8, "mirror bases read as int2");
 
 // The symmetric shared-offset form: FP64 operator unless TFLOWS_SYM_SLOTS=0;
 // FP32 operator when TFLOWS_SYM_SLOTS32=1 (its matrix at 128^3 sits in the
@@ -204,90 +169,6 @@ bool shared_slot_layout(int n, const std::vector<int>& rp, const std::vector<int
 bool sym_slots_from_env(bool fp32) {
     const char* e = std::getenv(fp32 ? "TFLOWS_SYM_SLOTS32" : "TFLOWS_SYM_SLOTS");
     return fp32 ? (e && e[0] == '1') : !(e && e[0] == '0');
-}
-
-static int floor_div64(int64_t v) { return (int)(v >= 0 ? v / 64 : -((-v + 63) / 64)); }
-
-bool symmetric_slot_layout(int n, const std::vector<int>& rp, const std::vector<int>& cl, const std::vector<int>& off,
-                           const std::vector<int>& soff, const std::vector<unsigned char>& mask,
-                           const std::vector<int>& map, SymSlots& out) {
-    const int nsl = (int)off.size() - 1;
-    if (nsl < 1) return false;
-    std::vector<int> nl(nsl), w(nsl);
-    out.sidx.assign(nsl + 1, 0);
-    out.off.assign(nsl + 1, 0);
-    for (int sl = 0; sl < nsl; ++sl) {
-        w[sl] = (off[sl + 1] - off[sl]) / kSlice;
-        out.sidx[sl] = off[sl] / kSlice;
-        const int* so = soff.data() + out.sidx[sl];
-        int c = 0;
-        while (c < w[sl] && so[c] < 0) ++c;   // the offsets ascend: the negative ones first
-        nl[sl] = c;
-        const int64_t next = (int64_t)out.off[sl] + (int64_t)(w[sl] - c) * kSlice;
-        if (next > INT32_MAX) return false;
-        out.off[sl + 1] = (int)next;
-    }
-    out.sidx[nsl] = off[nsl] / kSlice;
-    if (out.off[nsl] == 0) return false;
-    out.map.assign(out.off[nsl], -1);
-    for (int sl = 0; sl < nsl; ++sl)
-        for (int j = nl[sl]; j < w[sl]; ++j)
-            for (int l = 0; l < kSlice; ++l)
-                out.map[out.off[sl] + kSlice * (j - nl[sl]) + l] = map[off[sl] + kSlice * j + l];
-    // the slot of column offset v in slice t, -1 when the slice has none
-    auto find = [&](int t, int v) {
-        const int* so = soff.data() + out.sidx[t];
-        for (int j = 0; j < w[t]; ++j)
-            if (so[j] == v) return j;
-        return -1;
-    };
-    constexpr int kNone = INT32_MIN / 2;
-    out.lbase.assign(out.sidx[nsl], int2{0, 0});
-    for (int sl = 0; sl < nsl; ++sl) {
-        const int* so = soff.data() + out.sidx[sl];
-        for (int j = 0; j < nl[sl]; ++j) {
-            const int d = so[j];
-            int base[2];
-            for (int t = 0; t < 2; ++t) {
-                const int sp = floor_div64((int64_t)kSlice * sl + (t ? kSlice - 1 : 0) + d);
-                base[t] = kNone;
-                if (sp >= 0 && sp < nsl) {
-                    const int jp = find(sp, -d);
-                    if (jp >= nl[sp]) base[t] = out.off[sp] + kSlice * (jp - nl[sp]) - kSlice * sp;
-                }
-            }
-            out.lbase[out.sidx[sl] + j] = int2{base[0], base[1]};
-        }
-    }
-    // every lower entry, read where the kernels will read it: the stored
-    // slot there must be its mirror (j, i); the pairs let the device check
-    // that the values agree
-    out.pairs.clear();
-    for (int sl = 0; sl < nsl; ++sl) {
-        const int* so = soff.data() + out.sidx[sl];
-        for (int l = 0; l < kSlice; ++l) {
-            const int i = sl * kSlice + l;
-            if (i >= n) break;
-            for (int j = 0; j < nl[sl]; ++j) {
-                if (!((mask[i] >> j) & 1u)) continue;
-                const int d = so[j];
-                const int e = map[off[sl] + kSlice * j + l];
-                const int ip = i + d;
-                if (e < 0 || ip < 0 || ip >= n) return false;
-                const int sa = floor_div64((int64_t)kSlice * sl + d);
-                const int2 lb = out.lbase[out.sidx[sl] + j];
-                const int b = (ip / kSlice == sa) ? lb.x : lb.y;
-                if (b == kNone) return false;
-                const int64_t pos = (int64_t)b + ip;
-                if (pos < 0 || pos >= out.off[nsl]) return false;
-                const int em = out.map[pos];
-                if (em < 0 || em < rp[ip] || em >= rp[ip + 1] || cl[em] != i) return false;
-                out.pairs.push_back(e);
-                out.pairs.push_back(em);
-            }
-        }
-    }
-    return true;
 }
 
 // 1 in *flag when some pair of entries (a lower one and its mirror) differ
@@ -354,7 +235,7 @@ Sell<double> System::sell64(hipStream_t s) {
                 sell64_pairs_.ensure(sym.pairs.size());
                 TF_CHECK(hipMemcpy(sell64_sidx_.p, sym.sidx.data(), sizeof(int) * sym.sidx.size(),
                                    hipMemcpyHostToDevice));
-                TF_CHECK(hipMemcpy(sell64_lbase_.p, sym.lbase.data(), sizeof(int2) * sym.lbase.size(),
+                TF_CHECK(hipMemcpy(sell64_lbase_.p, sym.lbase.data(), sizeof(SlotPair) * sym.lbase.size(),
                                    hipMemcpyHostToDevice));
                 if (!sym.pairs.empty())
                     TF_CHECK(hipMemcpy(sell64_pairs_.p, sym.pairs.data(), sizeof(int) * sym.pairs.size(),
@@ -400,7 +281,7 @@ Sell<double> System::sell64(hipStream_t s) {
         sell64_valid_ = true;
     }
     return Sell<double>{n_,           sell64_off_.p,  sell64_col_.p,  sell64_val_.p, sell64_dc_.p,
-                        sell64_soff_.p, sell64_mask_.p, sell64_sidx_.p, sell64_lbase_.p};
+                        sell64_soff_.p, sell64_mask_.p, sell64_sidx_.p, reinterpret_cast<const int2*>(sell64_lbase_.p)};
 }
 
 bool col16_from_env(const char* name) {
