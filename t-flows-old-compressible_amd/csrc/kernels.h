@@ -164,6 +164,11 @@ struct DiffSell32 {
     // when not null: the symmetric shared-offset form (see SymSlots)
     const int* __restrict__ sidx = nullptr;
     const int2* __restrict__ lbase = nullptr;
+    // 1: the shared-offset loop runs over all kMaxSharedSlots slots without
+    // branches (every slot's loads in flight at once; a slot past the
+    // slice's width reads slot 0 and, like an absent one, adds a (p_i - p_i)
+    // with a = 0); 0: two slots at a time.  The same sums either way.
+    int unroll8 = 1;
 };
 constexpr int kSlice = 64;   // kSliceRows of slot_layout.h
 // most slots of a slice in the shared-offset form (bits of the row mask)
@@ -649,8 +654,21 @@ __device__ __forceinline__ void sell_tile(const DiffSell32& A, int r0, int r1, G
             // all slots are in flight together
             const int* __restrict__ so = A.soff + b / kSlice;
             const unsigned m = A.mask[row];
+            if (A.unroll8) {
+                if (w > 0) {
+#pragma unroll
+                    for (int j = 0; j < kMaxSharedSlots; ++j) {
+                        const bool in = j < w;
+                        const int jj = in ? j : 0;
+                        const float a = vp[jj * kSlice];
+                        const bool has = in && ((m >> j) & 1u);
+                        slot(has ? row + so[jj] : row, in ? a : 0.0f);
+                    }
+                }
+            } else {
 #pragma unroll 2
-            for (int j = 0; j < w; ++j) slot(((m >> j) & 1u) ? row + so[j] : row, vp[j * kSlice]);
+                for (int j = 0; j < w; ++j) slot(((m >> j) & 1u) ? row + so[j] : row, vp[j * kSlice]);
+            }
         } else if (A.dc) {                           // the same columns, 16-bit
             const short* __restrict__ dp = A.dc + b + (row - sl * kSlice);
 #pragma unroll 2
@@ -701,6 +719,8 @@ __device__ __forceinline__ void sell_rows(const Sell<T>& A, int r0, int r1, Gath
             const int* __restrict__ so = A.soff + sb;
             const unsigned m = A.mask[row];
             // the lower slots (mirrored) come first in the ascending offsets
+            // (two slots at a time: a branch-free loop over all slots, the
+            // FP32 operator's form, held more registers and ran slower here)
 #pragma unroll 2
             for (int j = 0; j < nl; ++j)
                 if ((m >> j) & 1u) slot(row + so[j], A.val[sym_mirror_pos(A.lbase, sb, j, sl, row, so[j])]);

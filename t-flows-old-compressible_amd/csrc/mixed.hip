@@ -313,8 +313,10 @@ DiffSell32 System::sell32(hipStream_t s) {
         TF_CHECK(hipGetLastError());
         sell_valid_ = true;
     }
-    return DiffSell32{n_,           sell_off_.p,  sell_col_.p,  sell_val_.p,   d.rs,
-                      sell_dc_.p,   sell_soff_.p, sell_mask_.p, sell_sidx_.p, reinterpret_cast<const int2*>(sell_lbase_.p)};
+    DiffSell32 op{n_,           sell_off_.p,  sell_col_.p,  sell_val_.p,   d.rs,
+                  sell_dc_.p,   sell_soff_.p, sell_mask_.p, sell_sidx_.p, reinterpret_cast<const int2*>(sell_lbase_.p)};
+    op.unroll8 = spmv_unroll_from_env() ? 1 : 0;
+    return op;
 }
 
 const float* System::diag32(hipStream_t s) {

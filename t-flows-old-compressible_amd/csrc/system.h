@@ -143,6 +143,7 @@ bool shared_slots_from_env();
 // (TFLOWS_SYM_SLOTS32, default off) operator tries the symmetric form of
 // the shared-offset layout (slot_layout.h).
 bool sym_slots_from_env(bool fp32);
+bool spmv_unroll_from_env();
 template <typename T>
 bool pairs_symmetric(const DevArray<int>& pairs, int64_t npairs, const T* a, DevArray<int>& flag, hipStream_t s);
 

@@ -162,6 +162,13 @@ static_assert(kSlice == kSliceRows && kMaxSharedSlots == kMaxSliceSlots,
               "the host layouts and the kernels agree on the slice shape");
 static_assert(sizeof(SlotPair) == sizeof(int2) && alignof(int2) <= 8, "mirror bases read as int2");
 
+// The FP32 shared-offset SpMV loop over all slots at once (TFLOWS_SPMV_UNROLL,
+// default 1) or two at a time (0); read when the operator is handed out.
+bool spmv_unroll_from_env() {
+    const char* e = std::getenv("TFLOWS_SPMV_UNROLL");
+    return !(e && e[0] == '0');
+}
+
 // The symmetric shared-offset form: FP64 operator unless TFLOWS_SYM_SLOTS=0;
 // FP32 operator when TFLOWS_SYM_SLOTS32=1 (its matrix at 128^3 sits in the
 // MALL, where halving its values saves no time and the mirror reads cost

@@ -155,12 +155,18 @@ def test_mixed_replacement_residual_bitwise(pkg, cuda, monkeypatch, case, prec, 
 # and column offsets shared by each slice (one list per slice, a mask per row)
 # and, for symmetric values, only the diagonal and upper slots stored with
 # the lower ones read from their mirrors (TFLOWS_SYM_SLOTS)
+# (the shared-offset loops over every slot at once, or two slots at a time:
+# TFLOWS_SPMV_UNROLL)
 LAYOUTS = {"int32": {"TFLOWS_SHARED_SLOTS": "0", "TFLOWS_FP64_COL16": "0", "TFLOWS_FP32_COL16": "0"},
            "col16": {"TFLOWS_SHARED_SLOTS": "0", "TFLOWS_FP64_COL16": "1", "TFLOWS_FP32_COL16": "1"},
            "shared_full": {"TFLOWS_SHARED_SLOTS": "1", "TFLOWS_SYM_SLOTS": "0", "TFLOWS_SYM_SLOTS32": "0",
-                           "TFLOWS_FP64_COL16": "1", "TFLOWS_FP32_COL16": "1"},
+                           "TFLOWS_FP64_COL16": "1", "TFLOWS_FP32_COL16": "1", "TFLOWS_SPMV_UNROLL": "1"},
+           "shared_full_unroll2": {"TFLOWS_SHARED_SLOTS": "1", "TFLOWS_SYM_SLOTS": "0", "TFLOWS_SYM_SLOTS32": "0",
+                                   "TFLOWS_FP64_COL16": "1", "TFLOWS_FP32_COL16": "1", "TFLOWS_SPMV_UNROLL": "0"},
+           "shared_unroll2": {"TFLOWS_SHARED_SLOTS": "1", "TFLOWS_SYM_SLOTS": "1", "TFLOWS_SYM_SLOTS32": "1",
+                              "TFLOWS_FP64_COL16": "1", "TFLOWS_FP32_COL16": "1", "TFLOWS_SPMV_UNROLL": "0"},
            "shared": {"TFLOWS_SHARED_SLOTS": "1", "TFLOWS_SYM_SLOTS": "1", "TFLOWS_SYM_SLOTS32": "1",
-                      "TFLOWS_FP64_COL16": "1", "TFLOWS_FP32_COL16": "1"}}
+                      "TFLOWS_FP64_COL16": "1", "TFLOWS_FP32_COL16": "1", "TFLOWS_SPMV_UNROLL": "1"}}
 
 
 @pytest.mark.parametrize("case", ["pressure", "channel_aniso", "pressure_perm"])
@@ -223,7 +229,7 @@ def test_shared_slot_layout_bitwise(pkg, oracle, cuda, monkeypatch, case):
         assert shared[0][0]["stored_entries"] <= 0.6 * full[0]["stored_entries"]
         assert shared[0][1]["stored_entries"] <= 0.55 * full[1]["stored_entries"]
     assert np.array_equal(shared[1], oracle.spmv(m, p64))
-    for other in ("int32", "col16", "shared_full"):
+    for other in ("int32", "col16", "shared_full", "shared_full_unroll2", "shared_unroll2"):
         o = out[other]
         assert np.array_equal(o[1], shared[1]) and np.array_equal(o[2], shared[2]), other
         for (ra, xa, rra), (rb, xb, rrb) in zip(o[3], shared[3]):
