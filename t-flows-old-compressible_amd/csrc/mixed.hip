@@ -165,13 +165,15 @@ __global__ __launch_bounds__(kBlock) void k_rel_resid(Op A, const double* __rest
                                                       float* __restrict__ z32, const float* __restrict__ D32,
                                                       double* __restrict__ r64, float* __restrict__ partB,
                                                       double* __restrict__ partE, SolverState* st) {
-    const int ntiles = (A.n + kTileRows - 1) / kTileRows;
-    int t0, t1;
-    block_tiles(ntiles, t0, t1);
+    // the SpMV's XCD-strided tile walk: rows a plane apart (the symmetric
+    // operator's mirror values among them) are read by the same XCD while
+    // still in its L2
+    const TileSpan sp = block_span_strided((A.n + kTileRows - 1) / kTileRows);
     Gather1<double> g{x64};
     InitVec<double> init{b};
     RelEpi epi{r32, z32, D32, r64, {0.0f, 0.0f}, {0.0}};
-    for (int t = t0; t < t1; ++t) resid_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi);
+    for (int t = sp.first; t < sp.end; t += sp.step)
+        resid_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, init, epi);
     site_write<float, 2>(epi.a32, partB);
     site_write<double, 1>(epi.a64, partE);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
