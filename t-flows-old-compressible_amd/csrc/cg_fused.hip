@@ -25,6 +25,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <initializer_list>
+#include <type_traits>
 
 #include "krylov_kernels.h"
 
@@ -457,8 +458,12 @@ __global__ __launch_bounds__(kBlock) void k_cgj_spmv(Op A, const T* __restrict__
             acc[0] += p[row] * y[0];                     // (p, q)  CG.f90:216
         }
     } epi{p, q, {T(0)}};
-    for (int t = sp.first; t < sp.end; t += sp.step)
-        op_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+    for (int t = sp.first; t < sp.end; t += sp.step) {
+        if constexpr (std::is_same_v<Op, DiffSell32>)
+            sell_tile<1, true>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+        else
+            op_tile<1>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+    }
     site_write<T, 1>(epi.acc, partD);
 }
 

@@ -167,7 +167,8 @@ struct DiffSell32 {
     // 1: the shared-offset loop runs over all kMaxSharedSlots slots without
     // branches (every slot's loads in flight at once; a slot past the
     // slice's width reads slot 0 and, like an absent one, adds a (p_i - p_i)
-    // with a = 0); 0: two slots at a time.  The same sums either way.
+    // with a = 0), in the kernels compiled with it (sell_tile's WIDE); 0: two
+    // slots at a time.  The same sums either way.
     int unroll8 = 1;
 };
 constexpr int kSlice = 64;   // kSliceRows of slot_layout.h
@@ -606,7 +607,10 @@ __device__ __forceinline__ int row_slice(int r0, int row) {
     return (r0 & (kSlice - 1)) == 0 ? __builtin_amdgcn_readfirstlane(sl) : sl;
 }
 
-template <int NRHS, class Gather, class Epi>
+// WIDE: the kernel also holds the all-slot loop (chosen at run time by
+// A.unroll8).  Only the single-system SpMV kernels hold it: in the batch
+// kernel the extra registers cost more than the loop gains.
+template <int NRHS, bool WIDE = false, class Gather, class Epi>
 __device__ __forceinline__ void sell_tile(const DiffSell32& A, int r0, int r1, Gather& g, Epi& epi) {
     const int row = r0 + (int)threadIdx.x;
     if (row < r1) {
@@ -654,7 +658,7 @@ __device__ __forceinline__ void sell_tile(const DiffSell32& A, int r0, int r1, G
             // all slots are in flight together
             const int* __restrict__ so = A.soff + b / kSlice;
             const unsigned m = A.mask[row];
-            if (A.unroll8) {
+            if (WIDE && A.unroll8) {
                 if (w > 0) {
 #pragma unroll
                     for (int j = 0; j < kMaxSharedSlots; ++j) {

@@ -1258,8 +1258,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv(BatchCtx ctx, Op A, const T* __
     const TileSpan sp = ctx_span(ctx, A.n);
     Gather1<T> g{x};
     StoreEpi<T> epi{y};
-    for (int t = sp.first; t < sp.end; t += sp.step)
-        op_tile<1, L>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+    for (int t = sp.first; t < sp.end; t += sp.step) {
+        if constexpr (std::is_same_v<Op, DiffSell32>)
+            sell_tile<1, true>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+        else
+            op_tile<1, L>(A, t * kTileRows, min((t + 1) * kTileRows, A.n), g, epi);
+    }
 }
 
 template <typename T, int L>
